@@ -1,0 +1,151 @@
+"""Throughput of the PIC cycle (push + deposit + FDTD + shift) on one MI355X.
+
+Workload (default): the two-species Kelvin-Helmholtz shear flow, 256^3 cells,
+16 macro-particles per cell per species (536,870,912 particles), TSC shape,
+Boris pusher, Yee solver, Esirkepov deposit, FP32, initial state generated on
+the device. One step is one full PIC cycle. The timed region runs K steps
+with every input resident in HBM; HIP events on the simulation's stream time
+each fused push/deposit launch to give the dominant kernel's HBM roofline.
+
+Metric: particle-steps/s = mobile particles * steps / wall time.
+Algorithmic bytes: 55 B per particle-step (read x,p,w,cellIdx,mask = 31 B,
+write x,p = 24 B) and 96 B per cell-step for the whole step; the fused
+push/deposit kernel alone moves 55 B per particle plus 36 B per cell
+(E, B read; J written).
+
+The CPU baseline is the FP32 oracle (C++/OpenMP, the same per-particle
+arithmetic) timed on this host on a bounded sample of the same physics (same
+species, shapes, ppc and cell/time step on a smaller periodic box).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PARTICLE = 55.0
+BYTES_PER_CELL_STEP = 96.0
+BYTES_PER_CELL_PUSH = 36.0
+
+
+def cpu_baseline(setup_fn, grid, steps=3, warmup=1):
+    import numpy as np  # noqa: F401
+
+    from oracle import oracle as O
+    from picongpu_amd import setups
+
+    st = setup_fn(grid=grid)
+    threads = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    ora = O.Oracle(st, 32, 0, jexp=40)
+    for sp in range(len(st.species)):
+        cells, pos, mom, w = setups.host_particles(st, sp)
+        ora.set_frames(sp, setups.bin_frames(st, sp, cells, pos, mom, w))
+    ora.step(warmup)
+    t0 = time.perf_counter()
+    ora.step(steps)
+    dt = time.perf_counter() - t0
+    n = st.mobile_particles()
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{st.name} {grid[0]}x{grid[1]}x{grid[2]} cells, {n} particles, {steps} steps "
+                      f"after {warmup} warm-up, FP32 oracle with OpenMP, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="kelvin_helmholtz")
+    ap.add_argument("--grid", default="", help="override grid, e.g. 128,128,128")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-grid", default="64,64,32")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by "
+              f"spatial domain decomposition, which this build does not run)", file=sys.stderr)
+        sys.exit(2)
+
+    from picongpu_amd import abi, setups
+
+    setup_fn = setups.CONFIGS[args.config]
+    kw = {}
+    if args.grid:
+        kw["grid"] = tuple(int(v) for v in args.grid.split(","))
+    st = setup_fn(**kw)
+    n_particles = st.mobile_particles()
+    ncells = st.ncells
+
+    sim = abi.Simulation(st.config())
+    setups.init_device(sim, st)
+    sim.sync()
+    for _ in range(args.warmup):
+        sim.step(1)
+    sim.sync()
+
+    sim.timing(True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    push_ms, push_launches = sim.kernel_timer(abi.TIMER_PUSH)
+    shift_ms, _ = sim.kernel_timer(abi.TIMER_SHIFT)
+    field_ms, _ = sim.kernel_timer(abi.TIMER_FIELD)
+    sim.timing(False)
+    counts = [sim.particle_count(i) for i in range(len(st.species))]
+    sim.close()
+
+    ms_per_step = wall * 1e3 / args.steps
+    value = n_particles * args.steps / wall
+    push_bytes = (n_particles * BYTES_PER_PARTICLE + ncells * BYTES_PER_CELL_PUSH) * args.steps
+    achieved = push_bytes / (push_ms * 1e-3) / 1e9
+    step_bytes = (n_particles * BYTES_PER_PARTICLE + ncells * BYTES_PER_CELL_STEP) * args.steps
+    step_gbs = step_bytes / wall / 1e9
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        try:
+            baseline = cpu_baseline(setup_fn, tuple(int(v) for v in args.cpu_grid.split(",")))
+        except Exception as exc:  # the baseline is reported, never required
+            baseline = {"error": str(exc)}
+
+    out = {
+        "metric": "particle-steps/s (push+deposit+FDTD) on 1 GPU",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-seeded initial state)",
+        "config": {"workload": st.name, "grid": list(st.grid), "species": [s.name for s in st.species],
+                   "ppc": [s.ppc for s in st.species], "shape": "TSC" if st.species[0].shape == 2 else
+                   ("CIC" if st.species[0].shape == 1 else "PQS"), "pusher": "Boris" if st.species[0].pusher == 1
+                   else "Vay", "solver": "Yee" if st.solver == 0 else "Lehe", "particles": n_particles,
+                   "cells": ncells},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "kPushDeposit (fused gather+push+move+Esirkepov)",
+                     "avg_launch_ms": push_ms / max(push_launches, 1), "launches": push_launches},
+        "step_roofline": {"achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS,
+                          "push_ms": push_ms / args.steps, "shift_ms": shift_ms / args.steps,
+                          "field_ms": field_ms / args.steps},
+        "particles_after": counts,
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,219 @@
+/* picongpu_amd.h -- C ABI of the MI355X-native PIC-cycle hot path.
+ *
+ * One context (pic_ctx) is one periodic 3-D simulation box on one HIP device.
+ * Every call is serialised on the context's HIP stream; functions return 0 on
+ * success and a negative PIC_E_* code on failure, with a message available from
+ * pic_last_error() (thread-local). No C++ exception crosses this boundary.
+ * Memory passed in is owned by the caller and only read or written during the
+ * call; device buffers are owned by the context.
+ *
+ * Units: every float handed across the ABI is in PIConGPU's internal unit
+ * system (include/picongpu/unitless/simulation.unitless:31-500): time unit dt,
+ * speed unit c, length unit c*dt, mass unit BASE_MASS_SI*W_typ, charge unit
+ * -BASE_CHARGE_SI*W_typ with W_typ = BASE_DENSITY_SI*cellVolume/TYPICAL_PARTICLES_PER_CELL.
+ * Particle positions are cell-relative in [0,1), momenta are macro-particle
+ * momenta, exactly as the reference stores them
+ * (include/picongpu/param/speciesDefinition.param:49).
+ *
+ * Storage layout (the reference's supercell frame semantics,
+ * include/pmacc/particles/memory/dataTypes/SuperCell.hpp:31-119): the box is
+ * tiled by supercells of 8x8x4 cells; every supercell owns `frames_per_sc`
+ * frames of 256 slots; its N particles occupy slots [0,N) so all frames are
+ * full except the last, which holds (N-1)%256+1. localCellIdx is the
+ * x-fastest linear cell index inside the supercell; multiMask is 0 (empty),
+ * 1 (valid) or 2..27 (leaving towards exchange direction multiMask-1, with
+ * RIGHT=1, LEFT=2, BOTTOM=3, TOP=6, BACK=9, FRONT=18 summed over axes,
+ * include/picongpu/particles/MoveParticle.hpp:139-157).
+ * Fields are three SoA planes (x, y, z component) of nx*ny*nz floats,
+ * x fastest, Yee-staggered as include/picongpu/fields/YeeCell.hpp:69-141.
+ */
+#ifndef PICONGPU_AMD_H
+#define PICONGPU_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC_ABI_VERSION 1
+#define PIC_MAX_SPECIES 4
+
+/* error codes */
+#define PIC_OK 0
+#define PIC_E_INVALID -1     /* bad argument or configuration */
+#define PIC_E_HIP -2         /* HIP runtime error */
+#define PIC_E_CAPACITY -3    /* a supercell ran out of frames */
+#define PIC_E_STATE -4       /* call not valid in the current state */
+
+/* particle shapes: include/picongpu/particles/shapes/{CIC,TSC,PQS}.hpp */
+#define PIC_SHAPE_CIC 1
+#define PIC_SHAPE_TSC 2
+#define PIC_SHAPE_PQS 3
+
+/* pushers: particlePusherBoris.hpp:41-90, particlePusherVay.hpp:42-111;
+ * NONE = immobile species (not pushed, no current) */
+#define PIC_PUSHER_NONE 0
+#define PIC_PUSHER_BORIS 1
+#define PIC_PUSHER_VAY 2
+
+/* field solvers: fields/MaxwellSolver/Yee/Yee.def:36-47, Lehe/Lehe.def:57-73 */
+#define PIC_SOLVER_YEE 0
+#define PIC_SOLVER_LEHE 1
+
+/* fields */
+#define PIC_FIELD_E 0
+#define PIC_FIELD_B 1
+#define PIC_FIELD_J 2
+
+/* per-kernel timers (pic_kernel_timer) */
+#define PIC_TIMER_PUSH 0      /* fused gather+push+move+Esirkepov launches */
+#define PIC_TIMER_SHIFT 1
+#define PIC_TIMER_FIELD 2
+#define PIC_TIMER_COUNT 3
+
+/* Species description: the flags of a PIConGPU species
+ * (param/speciesDefinition.param:49-101): massRatio<>, chargeRatio<>,
+ * shape<>, particlePusher<> and the frame capacity of its supercells. */
+typedef struct pic_species_desc {
+    double mass_ratio;      /* relative to BASE_MASS_SI (electron mass) */
+    double charge_ratio;    /* relative to BASE_CHARGE_SI (electron charge) */
+    int32_t shape;          /* PIC_SHAPE_* */
+    int32_t pusher;         /* PIC_PUSHER_* */
+    int32_t frames_per_sc;  /* frames of 256 slots preallocated per supercell */
+    int32_t reserved;
+} pic_species_desc;
+
+/* Simulation description: the .param surface of the hot path
+ * (param/simulation.param DELTA_T_SI/CELL_*_SI, param/density.param
+ * BASE_DENSITY_SI, param/particle.param TYPICAL_PARTICLES_PER_CELL,
+ * param/fieldSolver.param Solver). */
+typedef struct pic_config {
+    int32_t grid[3];          /* global cells, multiples of 8,8,4 */
+    double cell_si[3];        /* CELL_WIDTH_SI, CELL_HEIGHT_SI, CELL_DEPTH_SI [m] */
+    double dt_si;             /* DELTA_T_SI [s] */
+    double base_density_si;   /* BASE_DENSITY_SI [1/m^3] */
+    int32_t typical_ppc;      /* TYPICAL_PARTICLES_PER_CELL */
+    int32_t solver;           /* PIC_SOLVER_* */
+    int32_t lehe_dir;         /* Cherenkov-free axis of the Lehe solver (0,1,2) */
+    int32_t n_species;
+    pic_species_desc species[PIC_MAX_SPECIES];
+    int32_t j_scale_exp;      /* fixed-point exponent of the current tile, 0 = automatic */
+    int32_t device;           /* HIP device ordinal */
+} pic_config;
+
+/* FP32 constants in PIC units derived from a pic_config, computed as
+ * unitless/simulation.unitless:31-110,428-479 and the solver/pusher files do. */
+typedef struct pic_constants {
+    float cell[3];            /* sim.pic.getCellSize() */
+    float dt;                 /* sim.pic.getDt() (== 1) */
+    float c;                  /* sim.pic.getSpeedOfLight() (== 1) */
+    float eps0;               /* sim.pic.getEps0() */
+    float mue0;               /* sim.pic.getMue0() */
+    float mue0_eps0;          /* sim.pic.getMue0Eps0() */
+    float add_current_coeff;  /* -(1/eps0)*dt, FDTD.hpp:84 */
+    float inv_cell_volume_dt; /* 1/float(V*dt), Esirkepov.hpp:190-192 */
+    float lehe_alpha;         /* Lehe/Derivative.hpp:90-111 (FP64 -> FP32) */
+    float lehe_delta;
+    float species_mass[PIC_MAX_SPECIES];   /* frame::getMass() */
+    float species_charge[PIC_MAX_SPECIES]; /* frame::getCharge() */
+    double unit_length, unit_time, unit_mass, unit_charge; /* SI per PIC unit */
+    double typical_weighting; /* W_typ */
+} pic_constants;
+
+/* Device-side initial state in the manner of KernelFillGridWithParticles +
+ * startPosition::Random + Drift + Temperature (particles/ParticlesInit.kernel:66-302,
+ * startPosition/RandomImpl.hpp:46-56, manipulators/unary/Drift.hpp:53-76,
+ * Temperature.hpp:60-84): ppc particles per cell at uniform random in-cell
+ * positions, weighting density*V/ppc, momentum mass*(u_drift + u_th*N(0,1)).
+ * The drift may flip sign inside a y band (Kelvin-Helmholtz shear). */
+typedef struct pic_init_desc {
+    uint64_t seed;
+    int32_t ppc;              /* macro-particles per cell */
+    int32_t drift_band;       /* 0: uniform drift; 1: drift sign +1 for y in [ny/4,3ny/4), -1 outside */
+    double density_si;        /* physical density [1/m^3] */
+    double u_drift[3];        /* gamma*beta of the drift */
+    double u_th[3];           /* thermal spread of gamma*beta per component */
+} pic_init_desc;
+
+typedef struct pic_ctx pic_ctx;
+
+/* --- lifecycle ---------------------------------------------------------- */
+const char* pic_last_error(void);
+int32_t pic_abi_version(void);
+/* unitless/simulation.unitless: constants of a configuration (host only, no device) */
+int pic_constants_from_config(const pic_config* cfg, pic_constants* out);
+/* Simulation::pluginLoad + Simulation::init (simulation/control/Simulation.hpp:176-436):
+ * allocates fields (zeroed) and frame pools on the device */
+int pic_create(const pic_config* cfg, pic_ctx** out);
+int pic_destroy(pic_ctx* ctx);
+int pic_get_constants(const pic_ctx* ctx, pic_constants* out);
+int pic_sync(pic_ctx* ctx);
+
+/* --- state in and out --------------------------------------------------- */
+/* Load n particles given by global cell coordinates (3n int32, x,y,z), in-cell
+ * positions (3n), momenta (3n) and weightings (n). They are binned into their
+ * supercells in input order (replaces particle init / restart LoadSpecies). */
+int pic_load_particles(pic_ctx* ctx, int32_t species, int64_t n, const int32_t* cell,
+                       const float* pos, const float* mom, const float* weighting);
+/* Read all particles of a species in frame-canonical order (supercells in
+ * x-fastest order, slots ascending). `capacity` is the length of w; n_out gets
+ * the particle count. Any output pointer may be NULL. */
+int pic_read_particles(pic_ctx* ctx, int32_t species, int64_t capacity, int64_t* n_out,
+                       int32_t* cell, float* pos, float* mom, float* weighting);
+int64_t pic_particle_count(pic_ctx* ctx, int32_t species);
+int32_t pic_num_supercells(const pic_ctx* ctx);
+int32_t pic_slots_per_supercell(const pic_ctx* ctx, int32_t species);
+/* Raw frame pool: per-supercell counts plus the slot arrays (nsc*slots each).
+ * pos/mom are 3 planes of nsc*slots floats. */
+int pic_read_frames(pic_ctx* ctx, int32_t species, uint32_t* counts, float* pos, float* mom,
+                    float* weighting, uint16_t* local_cell, uint8_t* multi_mask);
+int pic_write_frames(pic_ctx* ctx, int32_t species, const uint32_t* counts, const float* pos,
+                     const float* mom, const float* weighting, const uint16_t* local_cell,
+                     const uint8_t* multi_mask);
+int pic_init_species(pic_ctx* ctx, int32_t species, const pic_init_desc* desc);
+int pic_load_field(pic_ctx* ctx, int32_t field, const float* data);
+int pic_read_field(pic_ctx* ctx, int32_t field, float* data);
+/* B += amp*U(-1,1) per component, seeded (initial noise of the shear-flow setup) */
+int pic_add_field_noise(pic_ctx* ctx, int32_t field, uint64_t seed, float amp);
+
+/* --- stages of one PIC step (Simulation::runOneStep, Simulation.hpp:522-542) --- */
+/* CurrentReset (simulation/stage/CurrentReset.hpp:39-53): J = 0 */
+int pic_current_reset(pic_ctx* ctx);
+/* ParticlePush (stage/ParticlePush.x.cpp:141-148 -> KernelMoveAndMarkParticles,
+ * Particles.kernel:169-316): field gather, push, moveParticle, mark leavers.
+ * species < 0 pushes every mobile species. With fuse_deposit != 0 the same
+ * launch also runs Esirkepov<Shape> (CurrentDeposition, stage/CurrentDeposition.x.cpp:49-117)
+ * on each particle's move and adds the current to J. */
+int pic_push(pic_ctx* ctx, int32_t species, int32_t fuse_deposit);
+/* shiftBetweenSupercells + periodic boundary (Particles.tpp:370-377,
+ * pmacc/particles/ParticlesBase.kernel:61-630, boundary/Apply.hpp:74-116) */
+int pic_shift(pic_ctx* ctx, int32_t species);
+/* CurrentDeposition from the current particle state (unfused form) */
+int pic_deposit(pic_ctx* ctx, int32_t species);
+/* fields::Solver::update_beforeCurrent (FDTDBase.hpp:97-121): B half step, E full step */
+int pic_fdtd_before_current(pic_ctx* ctx);
+/* CurrentInterpolationAndAdditionToEMF with currentInterpolation::None
+ * (CurrentInterpolationAndAdditionToEMF.hpp:99-148, FDTD.hpp:78-86): E += -(dt/eps0) J */
+int pic_add_current(pic_ctx* ctx);
+/* fields::Solver::update_afterCurrent (FDTDBase.hpp:151-183): B half step */
+int pic_fdtd_after_current(pic_ctx* ctx);
+/* nsteps full PIC steps in the reference stage order, push and deposit fused */
+int pic_step(pic_ctx* ctx, int32_t nsteps);
+
+/* --- diagnostics ------------------------------------------------------- */
+/* sum over cells of E^2 and B^2 in FP64 (EnergyFields.x.cpp:183-247 without units) */
+int pic_field_energy(pic_ctx* ctx, double out[2]);
+/* bit flags of device-side errors since the last call (1 = frame capacity exceeded) */
+int pic_error_flags(pic_ctx* ctx, uint32_t* flags);
+/* per-kernel timing with HIP events on the context stream */
+int pic_timing_enable(pic_ctx* ctx, int32_t enable);
+int pic_kernel_timer(pic_ctx* ctx, int32_t timer, double* total_ms, int64_t* launches);
+int32_t pic_j_scale_exp(const pic_ctx* ctx);
+int32_t pic_num_push_phases(const pic_ctx* ctx, int32_t fused);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PICONGPU_AMD_H */

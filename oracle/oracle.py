@@ -1,0 +1,249 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the picongpu_amd package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+class OracleSpecies(C.Structure):
+    _fields_ = [("mass_ratio", C.c_double), ("charge_ratio", C.c_double), ("shape", C.c_int32),
+                ("pusher", C.c_int32), ("frames_per_sc", C.c_int32), ("reserved", C.c_int32)]
+
+
+class OracleConfig(C.Structure):
+    _fields_ = [("grid", C.c_int32 * 3), ("cell_si", C.c_double * 3), ("dt_si", C.c_double),
+                ("base_density_si", C.c_double), ("typical_ppc", C.c_int32), ("solver", C.c_int32),
+                ("lehe_dir", C.c_int32), ("n_species", C.c_int32), ("species", OracleSpecies * 4),
+                ("j_scale_exp", C.c_int32), ("device", C.c_int32)]
+
+
+class OracleConstants(C.Structure):
+    _fields_ = [("cell", C.c_float * 3), ("dt", C.c_float), ("c", C.c_float), ("eps0", C.c_float),
+                ("mue0", C.c_float), ("mue0_eps0", C.c_float), ("add_current_coeff", C.c_float),
+                ("inv_cell_volume_dt", C.c_float), ("lehe_alpha", C.c_float), ("lehe_delta", C.c_float),
+                ("species_mass", C.c_float * 4), ("species_charge", C.c_float * 4), ("unit_length", C.c_double),
+                ("unit_time", C.c_double), ("unit_mass", C.c_double), ("unit_charge", C.c_double),
+                ("typical_weighting", C.c_double)]
+
+
+_lib = None
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_U32 = C.POINTER(C.c_uint32)
+_U16 = C.POINTER(C.c_uint16)
+_U8 = C.POINTER(C.c_uint8)
+_I32 = C.POINTER(C.c_int32)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        sig = {
+            "oracle_create": (_P, [C.POINTER(OracleConfig), C.c_int32, C.c_int32]),
+            "oracle_destroy": (None, [_P]),
+            "oracle_get_constants": (C.c_int32, [C.POINTER(OracleConfig), C.POINTER(OracleConstants)]),
+            "oracle_set_jexp": (None, [_P, C.c_int32]),
+            "oracle_set_threads": (C.c_int32, [C.c_int32]),
+            "oracle_set_frames": (C.c_int32, [_P, C.c_int32, _U32, _F, _F, _F, _U16, _U8]),
+            "oracle_get_frames": (C.c_int32, [_P, C.c_int32, _U32, _D, _D, _D, _U16, _U8]),
+            "oracle_set_field": (C.c_int32, [_P, C.c_int32, _F]),
+            "oracle_get_field": (C.c_int32, [_P, C.c_int32, _D]),
+            "oracle_current_reset": (None, [_P]),
+            "oracle_push": (None, [_P, C.c_int32, C.c_int32]),
+            "oracle_deposit": (None, [_P, C.c_int32]),
+            "oracle_shift": (C.c_int32, [_P, C.c_int32]),
+            "oracle_fdtd_before_current": (None, [_P]),
+            "oracle_add_current": (None, [_P]),
+            "oracle_fdtd_after_current": (None, [_P]),
+            "oracle_step": (C.c_int32, [_P, C.c_int32]),
+            "oracle_field_energy": (None, [_P, _D]),
+            "oracle_charge_density": (None, [_P, _D]),
+            "oracle_div_e": (None, [_P, _D]),
+            "oracle_esirkepov_single": (None, [C.POINTER(OracleConfig), C.c_int32, _I32, _F, _F, C.c_float, _D]),
+            "oracle_move_particle": (C.c_int32, [_F, _U16, _U8]),
+            "oracle_shape": (None, [C.c_int32, C.c_int32, C.c_float, C.c_int32, _F]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def config_from(setup) -> OracleConfig:
+    """OracleConfig from a picongpu_amd.setups.Setup (plain attribute reads)."""
+    c = OracleConfig()
+    for d in range(3):
+        c.grid[d] = int(setup.grid[d])
+        c.cell_si[d] = float(setup.cell_si[d])
+    c.dt_si = float(setup.dt_si)
+    c.base_density_si = float(setup.density_si)
+    c.typical_ppc = int(setup.typical_ppc)
+    c.solver = int(setup.solver)
+    c.lehe_dir = int(setup.lehe_dir)
+    c.n_species = len(setup.species)
+    for i, s in enumerate(setup.species):
+        c.species[i].mass_ratio = s.mass_ratio
+        c.species[i].charge_ratio = s.charge_ratio
+        c.species[i].shape = s.shape
+        c.species[i].pusher = s.pusher
+        c.species[i].frames_per_sc = s.frames()
+    return c
+
+
+def constants(setup) -> OracleConstants:
+    k = OracleConstants()
+    cfg = config_from(setup)
+    lib().oracle_get_constants(C.byref(cfg), C.byref(k))
+    return k
+
+
+class Oracle:
+    """order 0: device summation order (fixed-point tiles, fused deposit);
+    order 1: the reference CPU's sequential order. precision 32 or 64."""
+
+    def __init__(self, setup, precision=32, order=0, jexp=40):
+        self.setup = setup
+        self.cfg = config_from(setup)
+        self.h = lib().oracle_create(C.byref(self.cfg), precision, order)
+        self.grid = tuple(setup.grid)
+        self.ncells = int(np.prod(self.grid))
+        self.nsc = (self.grid[0] // 8) * (self.grid[1] // 8) * (self.grid[2] // 4)
+        lib().oracle_set_jexp(self.h, jexp)
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cap(self, sp):
+        return self.setup.species[sp].frames() * 256
+
+    def set_jexp(self, e):
+        lib().oracle_set_jexp(self.h, int(e))
+
+    def set_frames(self, sp, f):
+        lib().oracle_set_frames(self.h, sp, _p(f["counts"], C.c_uint32), _p(np.ascontiguousarray(f["pos"], np.float32), C.c_float),
+                                _p(np.ascontiguousarray(f["mom"], np.float32), C.c_float),
+                                _p(np.ascontiguousarray(f["w"], np.float32), C.c_float),
+                                _p(f["cell"], C.c_uint16), _p(f["mask"], C.c_uint8))
+
+    def get_frames(self, sp):
+        slots = self.nsc * self.cap(sp)
+        f = {"counts": np.zeros(self.nsc, np.uint32), "pos": np.zeros((3, slots)), "mom": np.zeros((3, slots)),
+             "w": np.zeros(slots), "cell": np.zeros(slots, np.uint16), "mask": np.zeros(slots, np.uint8)}
+        lib().oracle_get_frames(self.h, sp, _p(f["counts"], C.c_uint32), _p(f["pos"], C.c_double),
+                                _p(f["mom"], C.c_double), _p(f["w"], C.c_double), _p(f["cell"], C.c_uint16),
+                                _p(f["mask"], C.c_uint8))
+        return f
+
+    def set_field(self, which, data):
+        data = np.ascontiguousarray(data, np.float32).reshape(3, -1)
+        lib().oracle_set_field(self.h, which, _p(data, C.c_float))
+
+    def get_field(self, which):
+        out = np.zeros((3, self.ncells))
+        lib().oracle_get_field(self.h, which, _p(out, C.c_double))
+        return out
+
+    def current_reset(self):
+        lib().oracle_current_reset(self.h)
+
+    def push(self, species=-1, fused=True):
+        lib().oracle_push(self.h, species, 1 if fused else 0)
+
+    def deposit(self, species=-1):
+        lib().oracle_deposit(self.h, species)
+
+    def shift(self, species=-1):
+        r = lib().oracle_shift(self.h, species)
+        if r != 0:
+            raise RuntimeError("oracle: supercell capacity exceeded")
+
+    def update_before_current(self):
+        lib().oracle_fdtd_before_current(self.h)
+
+    def add_current(self):
+        lib().oracle_add_current(self.h)
+
+    def update_after_current(self):
+        lib().oracle_fdtd_after_current(self.h)
+
+    def step(self, n=1):
+        r = lib().oracle_step(self.h, int(n))
+        if r != 0:
+            raise RuntimeError("oracle: supercell capacity exceeded")
+
+    def field_energy(self):
+        out = np.zeros(2)
+        lib().oracle_field_energy(self.h, _p(out, C.c_double))
+        return out
+
+    def charge_density(self):
+        out = np.zeros(self.ncells)
+        lib().oracle_charge_density(self.h, _p(out, C.c_double))
+        return out
+
+    def div_e(self):
+        out = np.zeros(self.ncells)
+        lib().oracle_div_e(self.h, _p(out, C.c_double))
+        return out
+
+
+def set_threads(n: int) -> int:
+    return lib().oracle_set_threads(int(n))
+
+
+def esirkepov_single(setup, shape, cell, pos, vel, charge):
+    cfg = config_from(setup)
+    n = int(np.prod(setup.grid))
+    J = np.zeros((3, n))
+    cell = np.ascontiguousarray(cell, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    lib().oracle_esirkepov_single(C.byref(cfg), shape, _p(cell, C.c_int32), _p(pos, C.c_float), _p(vel, C.c_float),
+                                  float(charge), _p(J, C.c_double))
+    return J
+
+
+def move_particle(pos, local_cell=0, mask=1):
+    p = np.ascontiguousarray(pos, np.float32).copy()
+    lc = C.c_uint16(local_cell)
+    m = C.c_uint8(mask)
+    r = lib().oracle_move_particle(_p(p, C.c_float), C.byref(lc), C.byref(m))
+    return p, lc.value, m.value, r
+
+
+def shape(shape_id, kind, x, flag=0):
+    out = np.zeros(5, np.float32)
+    lib().oracle_shape(shape_id, kind, float(x), int(flag), _p(out, C.c_float))
+    return out

@@ -1,0 +1,1326 @@
+// CPU oracle of the PIC-cycle hot path.
+//
+// TEST INFRASTRUCTURE ONLY: this file is a scalar C++ restatement of the
+// reference's algorithms, used by tests/ to check the HIP library, by
+// __graft_entry__.smoke() as the checker and by bench.py as the timed CPU
+// baseline. The product never links, loads or calls it.
+//
+// Every function follows the reference file:line it names, in the same
+// floating-point operation order. The state type T is float (the FP32 oracle,
+// bit-comparable with the device) or double (the FP64 twin used to state the
+// error of FP32 results). Built with -ffp-contract=off, no -ffast-math.
+//
+// Two current-deposition orders are available:
+//  order 0 ("device"): current of each particle added as int64 fixed point
+//          into a per-supercell tile during the push (deposit fused with the
+//          push in the frame of the particle's new cell), tiles flushed into
+//          J in the checkerboard phase order of the device. Bit-identical
+//          to the HIP library by construction of the same sums.
+//  order 1 ("sequential"): the order of the reference's CPU (omp2b) path:
+//          push and shift first, then per supercell a float cache filled
+//          particle by particle in reverse frame order and flushed with a
+//          stride-(skip+1) checkerboard (Deposit.hpp:63-84, ForEach.hpp:314-324).
+// Supercell shifts follow the device's canonical order (see pic_shift) so
+// that the frame layouts can be compared slot by slot.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int SCX = 8, SCY = 8, SCZ = 4, SCV = 256;
+
+// param/physicalConstants.param:31-48
+constexpr double kC = 2.99792458e8;
+constexpr double kMue0 = 1.25663706127e-6;
+constexpr double kMe = 9.1093837139e-31;
+constexpr double kQe = -1.602176634e-19;
+constexpr double kPi = 3.141592653589793238462643383279502884197169399;
+
+}  // namespace
+
+extern "C" {
+
+typedef struct oracle_species {
+    double mass_ratio;
+    double charge_ratio;
+    int32_t shape;   // 1 CIC, 2 TSC, 3 PQS
+    int32_t pusher;  // 0 none, 1 Boris, 2 Vay
+    int32_t frames_per_sc;
+    int32_t reserved;
+} oracle_species;
+
+typedef struct oracle_config {
+    int32_t grid[3];
+    double cell_si[3];
+    double dt_si;
+    double base_density_si;
+    int32_t typical_ppc;
+    int32_t solver;  // 0 Yee, 1 Lehe
+    int32_t lehe_dir;
+    int32_t n_species;
+    oracle_species species[4];
+    int32_t j_scale_exp;
+    int32_t device;
+} oracle_config;
+
+typedef struct oracle_constants {
+    float cell[3];
+    float dt;
+    float c;
+    float eps0;
+    float mue0;
+    float mue0_eps0;
+    float add_current_coeff;
+    float inv_cell_volume_dt;
+    float lehe_alpha;
+    float lehe_delta;
+    float species_mass[4];
+    float species_charge[4];
+    double unit_length, unit_time, unit_mass, unit_charge;
+    double typical_weighting;
+} oracle_constants;
+
+}  // extern "C"
+
+namespace {
+
+// Constants in PIC units, computed in type T as unitless/simulation.unitless
+// does for T = float_X; for T = double the same formulas without the casts.
+template <typename T>
+struct Consts {
+    T cell[3];
+    T dt, c, eps0, mue0, mue0_eps0, coeff, inv_vdt;
+    T lehe_alpha, lehe_delta, lehe_beta1, lehe_beta2;
+    T mass[4], charge[4];
+    double unit_length, unit_time, unit_mass, unit_charge, wtyp;
+};
+
+template <typename T>
+Consts<T> makeConsts(const oracle_config& cfg) {
+    Consts<T> k{};
+    double const unitTime = cfg.dt_si;                  // simulation.unitless:435-438
+    double const unitSpeed = kC;                        // :440-443
+    double const unitLength = unitTime * unitSpeed;     // :430-433
+    double const vol = cfg.cell_si[0] * cfg.cell_si[1] * cfg.cell_si[2];
+    double const wTyp = (cfg.base_density_si * vol) / static_cast<double>(cfg.typical_ppc);  // :474-478
+    double const unitMass = kMe * wTyp;                 // :445-448
+    double const unitCharge = -1.0 * kQe * wTyp;        // :450-453
+    k.unit_length = unitLength;
+    k.unit_time = unitTime;
+    k.unit_mass = unitMass;
+    k.unit_charge = unitCharge;
+    k.wtyp = wTyp;
+    for (int d = 0; d < 3; ++d) k.cell[d] = static_cast<T>(cfg.cell_si[d] / unitLength);  // :36-40
+    k.dt = static_cast<T>(cfg.dt_si / unitTime);                                         // :42-46
+    k.c = static_cast<T>(kC / unitSpeed);                                                // :48-52
+    k.mue0 = static_cast<T>(kMue0 / unitLength / unitMass * unitCharge * unitCharge);    // :84-88
+    k.eps0 = static_cast<T>(1. / static_cast<double>(k.mue0) / static_cast<double>(k.c) / static_cast<double>(k.c));
+    k.mue0_eps0 = static_cast<T>(1. / static_cast<double>(k.c) / static_cast<double>(k.c));
+    k.coeff = -(T(1.0) / k.eps0) * k.dt;                 // FDTD.hpp:84
+    T const vdt = k.cell[0] * k.cell[1] * k.cell[2] * k.dt;
+    k.inv_vdt = T(1.0) / vdt;                            // Esirkepov.hpp:190-192
+    T const baseMass = static_cast<T>(kMe / unitMass);
+    T const baseCharge = static_cast<T>(kQe / unitCharge);
+    for (int s = 0; s < cfg.n_species; ++s) {
+        // speciesDefinition.unitless:45-66: base * ratio in float_X
+        k.mass[s] = baseMass * static_cast<T>(cfg.species[s].mass_ratio);
+        k.charge[s] = baseCharge * static_cast<T>(cfg.species[s].charge_ratio);
+    }
+    if (cfg.solver == 1) {  // Lehe/Derivative.hpp:90-111 (FP64 set-up), :124-130 (FP32 betas)
+        int const d0 = cfg.lehe_dir, d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
+        double const stepRatio = static_cast<double>(k.cell[d0] / (k.c * k.dt));
+        double const co = stepRatio * std::sin(kPi / 2.0 * static_cast<double>(k.c) * static_cast<double>(k.dt) /
+                                               static_cast<double>(k.cell[d0]));
+        k.lehe_delta = static_cast<T>(0.25 * (1.0 - co * co));
+        double const sr1 = static_cast<double>(k.cell[d0] / k.cell[d1]);
+        double const sr2 = static_cast<double>(k.cell[d0] / k.cell[d2]);
+        double const b1 = 0.125 * sr1 * sr1;
+        double const b2 = 0.125 * sr2 * sr2;
+        k.lehe_alpha = static_cast<T>(1.0 - 2.0 * b1 - 2.0 * b2 - 3.0 * static_cast<double>(k.lehe_delta));
+        T const r1 = k.cell[d0] / k.cell[d1];
+        T const r2 = k.cell[d0] / k.cell[d2];
+        k.lehe_beta1 = T(0.125) * r1 * r1;
+        k.lehe_beta2 = T(0.125) * r2 * r2;
+    }
+    return k;
+}
+
+template <typename T>
+struct V3 {
+    T x, y, z;
+    T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+// pmacc/math/vector/Vector.tpp:99-110
+template <typename T>
+T l2norm2(const V3<T>& v) {
+    T t = v.x * v.x;
+    t += v.y * v.y;
+    t += v.z * v.z;
+    return t;
+}
+
+// pmacc/math/vector/Vector.tpp:71-76
+template <typename T>
+V3<T> cross(const V3<T>& a, const V3<T>& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// ---- shapes: particles/shapes/CIC.hpp, TSC.hpp, PQS.hpp -------------------
+template <typename T>
+struct ShapeFns {
+    int supp;
+    int begin() const { return supp == 2 ? 0 : -1; }  // ChargeAssignment::begin
+    // detail::X::shapeArray (on support) -> supp values
+    void onSupport(T x, T* s) const {
+        if (supp == 2) {  // CIC.hpp:56-66
+            s[0] = T(1.0) - x;
+            s[1] = x;
+        } else if (supp == 3) {  // TSC.hpp:75-86
+            s[0] = tscFF2(std::fabs(T(-1.0) - x));
+            s[1] = tscFF1(std::fabs(x));
+            s[2] = T(1.0) - (s[0] + s[1]);
+        } else {  // PQS.hpp:78-89
+            s[0] = pqsFF2(std::fabs(T(-1.0) - x));
+            s[1] = pqsFF1(x);
+            s[3] = pqsFF2(T(2.0) - x);
+            s[2] = T(1.0) - (s[0] + s[1] + s[3]);
+        }
+    }
+    static T tscFF1(T x) {  // TSC.hpp:47-54
+        T const sq = x * x;
+        return T(0.75) - sq;
+    }
+    static T tscFF2(T x) {  // TSC.hpp:56-64
+        T const tmp = T(3.0) / T(2.0) - x;
+        T const sq = tmp * tmp;
+        return T(0.5) * sq;
+    }
+    static T pqsFF1(T x) {  // PQS.hpp:47-55
+        T const sq = x * x;
+        T const tr = sq * x;
+        return T(1.0) / T(6.0) * (T(4.0) - T(6.0) * sq + T(3.0) * tr);
+    }
+    static T pqsFF2(T x) {  // PQS.hpp:57-65
+        T const tmp = T(2.0) - x;
+        T const tr = tmp * tmp * tmp;
+        return T(1.0) / T(6.0) * tr;
+    }
+    // ChargeAssignment::operator() (CIC.hpp:89-107, TSC.hpp:109-132, PQS.hpp:110-133)
+    T assign(T x) const {
+        T const a = std::fabs(x);
+        T r = T(0.0);
+        if (supp == 2) {
+            if (a < T(1.0)) r = T(1.0) - a;
+        } else if (supp == 3) {
+            T const r1 = tscFF1(a), r2 = tscFF2(a);
+            if (a < T(0.5))
+                r = r1;
+            else if (a < T(1.5))
+                r = r2;
+        } else {
+            T const r1 = pqsFF1(a), r2 = pqsFF2(a);
+            if (a < T(1.0))
+                r = r1;
+            else if (a < T(2.0))
+                r = r2;
+        }
+        return r;
+    }
+    // ChargeAssignment::shapeArray(pos, isOutOfRange) -> supp+1 values
+    void cached(T pos, bool out, T* s) const {
+        T const x = out ? pos - T(1.0) : pos;
+        T v[5];
+        onSupport(x, v);
+        v[supp] = T(0.0);
+        for (int i = supp; i >= 1; --i) s[i] = out ? v[i - 1] : v[i];
+        s[0] = out ? T(0.0) : v[0];
+    }
+};
+
+// ---- state ------------------------------------------------------------------
+template <typename T>
+struct Species {
+    oracle_species desc;
+    int supp = 2;
+    int cap = 0;
+    T mass = 0, charge = 0;
+    std::vector<T> pos[3], mom[3], w;
+    std::vector<uint16_t> cell;
+    std::vector<uint8_t> mask;
+    std::vector<uint32_t> count;
+};
+
+template <typename T>
+struct State {
+    oracle_config cfg;
+    Consts<T> k;
+    int nx, ny, nz, nscx, nscy, nscz, nsc;
+    size_t ncells;
+    std::vector<T> E[3], B[3], J[3];
+    std::vector<Species<T>> sp;
+    int jexp = 40;
+    int order = 0;
+    bool capacityError = false;
+
+    size_t idx(int x, int y, int z) const {
+        x = ((x % nx) + nx) % nx;
+        y = ((y % ny) + ny) % ny;
+        z = ((z % nz) + nz) % nz;
+        return static_cast<size_t>(x) + static_cast<size_t>(nx) * (static_cast<size_t>(y) + static_cast<size_t>(ny) * z);
+    }
+};
+
+int supportOf(int shape) { return shape == 1 ? 2 : (shape == 2 ? 3 : 4); }
+
+// ---- field gather: FieldToParticleInterpolation.hpp:100-124,
+// ShiftCoordinateSystem.hpp:53-79, AssignedTrilinearInterpolation.hpp:53-86
+template <typename T>
+T gather1(const State<T>& st, const std::vector<T>& F, int supp, int gx, int gy, int gz, const T* pos, const T* fpos) {
+    ShapeFns<T> sh{supp};
+    bool const isEven = supp % 2 == 0;
+    int const begin = -supp / 2 + (supp + 1) % 2;  // FieldToParticleInterpolation.hpp:60
+    T p[3];
+    int is[3];
+    for (int d = 0; d < 3; ++d) {
+        T const v_pos = pos[d] - fpos[d] - T(0.5);
+        if (isEven)
+            is[d] = v_pos >= T(-0.5) ? 0 : -1;
+        else
+            is[d] = v_pos >= T(0.0) ? 1 : 0;
+        p[d] = v_pos - T(is[d]) + T(0.5);
+    }
+    T sx[4], sy[4], sz[4];
+    sh.onSupport(p[0], sx);
+    sh.onSupport(p[1], sy);
+    sh.onSupport(p[2], sz);
+    T rz = T(0.0);
+    for (int z = 0; z < supp; ++z) {
+        T ry = T(0.0);
+        for (int y = 0; y < supp; ++y) {
+            T rx = T(0.0);
+            for (int x = 0; x < supp; ++x)
+                rx += F[st.idx(gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)] * sx[x];
+            ry += rx * sy[y];
+        }
+        rz += ry * sz[z];
+    }
+    return rz;
+}
+
+// ---- pushers ------------------------------------------------------------------
+// Gamma.hpp:28-38
+template <typename T>
+T gammaOf(const V3<T>& mom, T mass, T c) {
+    T const fMom2 = l2norm2(mom);
+    T const c2 = c * c;
+    T const m2c2r = T(1.0) / (mass * mass * c2);
+    return std::sqrt(T(1.0) + fMom2 * m2c2r);
+}
+
+// Velocity.hpp:28-39 with the CPU rsqrt = 1/sqrt (alpaka math/Traits.hpp:729-734)
+template <typename T>
+V3<T> velocityOf(const V3<T>& mom, T mass, T rc2) {
+    T const m0_2 = mass * mass;
+    T const fMom2 = l2norm2(mom);
+    T const t = T(1.0) / std::sqrt(m0_2 + fMom2 * rc2);
+    return {t * mom.x, t * mom.y, t * mom.z};
+}
+
+// particlePusherBoris.hpp:41-90
+template <typename T>
+void boris(const V3<T>& e, const V3<T>& b, V3<T>& mom, V3<T>& vel, T mass, T charge, const Consts<T>& k) {
+    T const QoM = charge / mass;
+    T const deltaT = k.dt;
+    V3<T> const mm{mom.x + T(0.5) * charge * e.x * deltaT, mom.y + T(0.5) * charge * e.y * deltaT,
+                   mom.z + T(0.5) * charge * e.z * deltaT};
+    T const gamma_reci = T(1.0) / gammaOf(mm, mass, k.c);
+    V3<T> const t{T(0.5) * QoM * b.x * gamma_reci * deltaT, T(0.5) * QoM * b.y * gamma_reci * deltaT,
+                  T(0.5) * QoM * b.z * gamma_reci * deltaT};
+    T const f = T(1.0) / (T(1.0) + l2norm2(t));
+    V3<T> const s{T(2.0) * t.x * f, T(2.0) * t.y * f, T(2.0) * t.z * f};
+    V3<T> const c1 = cross(mm, t);
+    V3<T> const mp{mm.x + c1.x, mm.y + c1.y, mm.z + c1.z};
+    V3<T> const c2 = cross(mp, s);
+    V3<T> const mpl{mm.x + c2.x, mm.y + c2.y, mm.z + c2.z};
+    mom = {mpl.x + T(0.5) * charge * e.x * deltaT, mpl.y + T(0.5) * charge * e.y * deltaT,
+           mpl.z + T(0.5) * charge * e.z * deltaT};
+    vel = velocityOf(mom, mass, k.mue0_eps0);
+}
+
+// particlePusherVay.hpp:42-111, sqrt_Vay = precision64Bit (param/pusher.param:62)
+template <typename T>
+void vay(const V3<T>& e, const V3<T>& b, V3<T>& mom, V3<T>& vel, T mass, T charge, const Consts<T>& k) {
+    T const deltaT = k.dt;
+    T const factor = static_cast<T>(0.5 * static_cast<double>(charge) * static_cast<double>(deltaT));
+    V3<T> const vmh = velocityOf(mom, mass, k.mue0_eps0);
+    V3<T> const cvb = cross(vmh, b);
+    V3<T> const m0{mom.x + factor * (e.x + cvb.x), mom.y + factor * (e.y + cvb.y), mom.z + factor * (e.z + cvb.z)};
+    V3<T> const mp{m0.x + factor * e.x, m0.y + factor * e.y, m0.z + factor * e.z};
+    T const gp = gammaOf(mp, mass, k.c);
+    T const fm = factor / mass;
+    double const tx = static_cast<double>(fm * b.x), ty = static_cast<double>(fm * b.y),
+                 tz = static_cast<double>(fm * b.z);
+    double dot = static_cast<double>(mp.x) * tx;
+    dot += static_cast<double>(mp.y) * ty;
+    dot += static_cast<double>(mp.z) * tz;
+    double const u_star = dot / static_cast<double>(k.c * mass);
+    double tau2 = tx * tx;
+    tau2 += ty * ty;
+    tau2 += tz * tz;
+    double const sigma = static_cast<double>(gp * gp) - tau2;
+    double const gph = std::sqrt(0.5 * (sigma + std::sqrt(sigma * sigma + 4.0 * (tau2 + u_star * u_star))));
+    double const rg = 1.0 / gph;
+    V3<T> const t{static_cast<T>(tx * rg), static_cast<T>(ty * rg), static_cast<T>(tz * rg)};
+    T const s = T(1.0) / (T(1.0) + l2norm2(t));
+    T d = mp.x * t.x;
+    d += mp.y * t.y;
+    d += mp.z * t.z;
+    V3<T> const cpt = cross(mp, t);
+    mom = {s * (mp.x + d * t.x + cpt.x), s * (mp.y + d * t.y + cpt.y), s * (mp.z + d * t.z + cpt.z)};
+    vel = velocityOf(mom, mass, k.mue0_eps0);
+}
+
+// ---- moveParticle: particles/MoveParticle.hpp:47-160 -------------------------
+template <typename T>
+int moveParticle(T* newPos, uint16_t& localCellIdx, uint8_t& multiMask, int* cellStep) {
+    volatile T shift = T(0.5);
+    int const sc[3] = {SCX, SCY, SCZ};
+    int dir[3];
+    for (int i = 0; i < 3; ++i) {
+        T pos = newPos[i] - shift;
+        T moveDir = T(0.0);
+        if (pos < T(-0.5)) moveDir = T(-1.0);
+        if (pos >= T(0.5)) moveDir = T(1.0);
+        pos -= moveDir;
+        newPos[i] = pos + shift;
+        dir[i] = static_cast<int>(moveDir);
+        cellStep[i] = dir[i];
+    }
+    int newMultimask = 1;
+    if (dir[0] != 0 || dir[1] != 0 || dir[2] != 0) {
+        int const idx = localCellIdx;
+        int lc[3] = {idx % SCX, (idx / SCX) % SCY, idx / (SCX * SCY)};
+        for (int i = 0; i < 3; ++i) lc[i] += dir[i];
+        for (int i = 0; i < 3; ++i)
+            dir[i] = static_cast<uint32_t>(lc[i]) >= static_cast<uint32_t>(sc[i]) ? dir[i] : 0;
+        for (int i = 0; i < 3; ++i) lc[i] -= dir[i] * sc[i];
+        localCellIdx = static_cast<uint16_t>(lc[0] + SCX * (lc[1] + SCY * lc[2]));
+        int exchangeType = 1;
+        for (int i = 0; i < 3; ++i) {
+            newMultimask += (dir[i] == -1 ? 2 : dir[i]) * exchangeType;
+            exchangeType *= 3;
+        }
+        if (newMultimask >= 2) multiMask = static_cast<uint8_t>(newMultimask);
+    }
+    return newMultimask;
+}
+
+// ---- Esirkepov: fields/currentDeposition/Esirkepov/Esirkepov.hpp:61-242 ----
+// `add(comp, x, y, z, value)` receives each accumulated_J at cell (x,y,z)
+// relative to the particle's cell.
+template <typename T, typename Add>
+void cptCurrent1D(int supp, const int* st, const T* p0, const T* p1, T csd, int comp, const int* perm, Add add) {
+    if (p0[2] == p1[2]) return;  // :157
+    ShapeFns<T> sh{supp};
+    int const begin = sh.begin();
+    int const end = begin + supp;
+    T s0j[5], s1j[5], s0k[5], s1k[5];
+    sh.cached(p0[1], !(st[1] & 2), s0j);  // ShapeMiddleLoop = Cached (Strategy.def:52-61)
+    sh.cached(p1[1], !(st[1] & 4), s1j);
+    sh.cached(p0[2], !(st[2] & 2), s0k);  // ShapeInnerLoop = Cached
+    sh.cached(p1[2], !(st[2] & 4), s1k);
+    int const leaveI = st[0] & 1, leaveJ = st[1] & 1, leaveK = st[2] & 1;
+    for (int i = begin; i < end + 1; ++i) {
+        if (!(i < end + leaveI)) continue;
+        T const s0i = sh.assign(T(i) - p0[0]);  // ShapeOuterLoop = Jit
+        T const dsi = sh.assign(T(i) - p1[0]) - s0i;
+        for (int j = begin; j < end + 1; ++j) {
+            if (!(j < end + leaveJ)) continue;
+            T const sj = s0j[j - begin];
+            T const dsj = s1j[j - begin] - sj;
+            T const tmp = -csd * (s0i * sj + T(0.5) * (dsi * sj + s0i * dsj) + (T(1.0) / T(3.0)) * dsj * dsi);
+            T acc = T(0.0);
+            for (int kk = begin; kk < end; ++kk) {
+                if (!(kk < end + leaveK - 1)) continue;
+                T const W = (s1k[kk - begin] - s0k[kk - begin]) * tmp;
+                acc += W;
+                int const ijk[3] = {i, j, kk};
+                // permuted access (PermutatedFieldValueAccess.hpp:87-108)
+                int rel[3];
+                for (int a = 0; a < 3; ++a) rel[perm[a]] = ijk[a];
+                add(comp, rel[0], rel[1], rel[2], acc);
+            }
+        }
+    }
+}
+
+template <typename T, typename Add>
+void esirkepov(int supp, const T* pos, const V3<T>& vel, T charge, const Consts<T>& k, Add add) {
+    bool const isEven = supp % 2 == 0;
+    T const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
+    T p0[3], p1[3];
+    int gs[3], st[3];
+    for (int d = 0; d < 3; ++d) {
+        p0[d] = pos[d] - dp[d];
+        p1[d] = pos[d];
+        int iS, iE;  // relayPoint.hpp:47-64
+        if (isEven) {
+            iS = static_cast<int>(std::floor(p0[d]));
+            iE = static_cast<int>(std::floor(p1[d]));
+        } else {
+            iS = static_cast<int>(std::floor(p0[d] + T(0.5)));
+            iE = static_cast<int>(std::floor(p1[d] + T(0.5)));
+        }
+        gs[d] = iS < iE ? iS : iE;
+        st[d] = 0;
+        st[d] |= gs[d] == iS ? 2 : 0;  // START_PARTICLE_IN_ASSIGNMENT_CELL
+        st[d] |= gs[d] == iE ? 4 : 0;  // END_PARTICLE_IN_ASSIGNMENT_CELL
+        st[d] |= iS != iE ? 1 : 0;     // LEAVE_CELL
+        p0[d] -= T(gs[d]);
+        p1[d] -= T(gs[d]);
+    }
+    auto shifted = [&](int comp, int x, int y, int z, T v) { add(comp, x + gs[0], y + gs[1], z + gs[2], v); };
+    {  // J_x: rotateOrigin<1,2,0>, permutation (1,2,0)
+        int const perm[3] = {1, 2, 0};
+        T const q0[3] = {p0[1], p0[2], p0[0]}, q1[3] = {p1[1], p1[2], p1[0]};
+        int const s3[3] = {st[1], st[2], st[0]};
+        T const csd = charge * k.inv_vdt * k.cell[0];
+        cptCurrent1D(supp, s3, q0, q1, csd, 0, perm, shifted);
+    }
+    {  // J_y: rotateOrigin<2,0,1>
+        int const perm[3] = {2, 0, 1};
+        T const q0[3] = {p0[2], p0[0], p0[1]}, q1[3] = {p1[2], p1[0], p1[1]};
+        int const s3[3] = {st[2], st[0], st[1]};
+        T const csd = charge * k.inv_vdt * k.cell[1];
+        cptCurrent1D(supp, s3, q0, q1, csd, 1, perm, shifted);
+    }
+    {  // J_z
+        int const perm[3] = {0, 1, 2};
+        T const csd = charge * k.inv_vdt * k.cell[2];
+        cptCurrent1D(supp, st, p0, p1, csd, 2, perm, shifted);
+    }
+}
+
+int depLower(int supp) { return supp / 2 + 1 - (supp + 1) % 2; }  // Esirkepov.hpp:42
+int depUpper(int supp) { return (supp + 1) / 2 + 1; }             // Esirkepov.hpp:43
+
+// Checkerboard colouring of a periodic ring (same rule as the device plan).
+int ringColours(int n, int s, std::vector<int>& colour) {
+    colour.assign(n, 0);
+    int div = s;
+    while (n % div != 0) ++div;
+    int const r = n % s;
+    if (div <= s + r) {
+        for (int i = 0; i < n; ++i) colour[i] = i % div;
+        return div;
+    }
+    for (int i = 0; i < n; ++i) colour[i] = i < n - r ? i % s : s + (i - (n - r));
+    return s + r;
+}
+
+// Supercells grouped by phase, phases ascending, x fastest inside a phase.
+template <typename T>
+std::vector<std::vector<int>> phases(const State<T>& st, const int* stride) {
+    std::vector<int> col[3];
+    int const n[3] = {st.nscx, st.nscy, st.nscz};
+    int nc[3];
+    for (int d = 0; d < 3; ++d) nc[d] = ringColours(n[d], stride[d], col[d]);
+    std::vector<std::vector<int>> ph(nc[0] * nc[1] * nc[2]);
+    for (int z = 0; z < n[2]; ++z)
+        for (int y = 0; y < n[1]; ++y)
+            for (int x = 0; x < n[0]; ++x)
+                ph[col[0][x] + nc[0] * (col[1][y] + nc[1] * col[2][z])].push_back(x + n[0] * (y + n[1] * z));
+    return ph;
+}
+
+template <typename T>
+void scCoords(const State<T>& st, int sc, int& sx, int& sy, int& sz) {
+    sx = sc % st.nscx;
+    sy = (sc / st.nscx) % st.nscy;
+    sz = sc / (st.nscx * st.nscy);
+}
+
+// One particle: gather, push, move (PushParticlePerFrame, Particles.kernel:275-316).
+// Returns the velocity and writes the cell step of the move.
+template <typename T>
+V3<T> pushOne(State<T>& st, Species<T>& S, size_t g, int ox, int oy, int oz, int* step) {
+    int const lc = S.cell[g];
+    int const cx = ox + lc % SCX, cy = oy + (lc / SCX) % SCY, cz = oz + lc / (SCX * SCY);
+    T pos[3] = {S.pos[0][g], S.pos[1][g], S.pos[2][g]};
+    V3<T> mom{S.mom[0][g], S.mom[1][g], S.mom[2][g]};
+    T const w = S.w[g];
+    T const mass = S.mass * w;
+    T const charge = S.charge * w;
+    // YeeCell.hpp:69-123 staggering
+    T const fE[3][3] = {{T(0.5), T(0), T(0)}, {T(0), T(0.5), T(0)}, {T(0), T(0), T(0.5)}};
+    T const fB[3][3] = {{T(0), T(0.5), T(0.5)}, {T(0.5), T(0), T(0.5)}, {T(0.5), T(0.5), T(0)}};
+    V3<T> const e{gather1(st, st.E[0], S.supp, cx, cy, cz, pos, fE[0]), gather1(st, st.E[1], S.supp, cx, cy, cz, pos, fE[1]),
+                  gather1(st, st.E[2], S.supp, cx, cy, cz, pos, fE[2])};
+    V3<T> const b{gather1(st, st.B[0], S.supp, cx, cy, cz, pos, fB[0]), gather1(st, st.B[1], S.supp, cx, cy, cz, pos, fB[1]),
+                  gather1(st, st.B[2], S.supp, cx, cy, cz, pos, fB[2])};
+    V3<T> vel;
+    if (S.desc.pusher == 2)
+        vay(e, b, mom, vel, mass, charge, st.k);
+    else
+        boris(e, b, mom, vel, mass, charge, st.k);
+    for (int d = 0; d < 3; ++d) pos[d] += (vel[d] * st.k.dt) / st.k.cell[d];
+    moveParticle(pos, S.cell[g], S.mask[g], step);
+    for (int d = 0; d < 3; ++d) S.pos[d][g] = pos[d];
+    S.mom[0][g] = mom.x;
+    S.mom[1][g] = mom.y;
+    S.mom[2][g] = mom.z;
+    return vel;
+}
+
+// fixed-point tile of one supercell (order 0)
+struct Tile {
+    int lo, up, tx, ty, tz;
+    std::vector<int64_t> acc;
+    void reset(int l, int u) {
+        lo = l;
+        up = u;
+        tx = SCX + l + u;
+        ty = SCY + l + u;
+        tz = SCZ + l + u;
+        acc.assign(static_cast<size_t>(3) * tx * ty * tz, 0);
+    }
+};
+
+template <typename T>
+void flushTile(State<T>& st, const Tile& t, int ox, int oy, int oz) {
+    double const inv = std::ldexp(1.0, -st.jexp);
+    size_t const tv = static_cast<size_t>(t.tx) * t.ty * t.tz;
+    for (int c = 0; c < 3; ++c)
+        for (int z = 0; z < t.tz; ++z)
+            for (int y = 0; y < t.ty; ++y)
+                for (int x = 0; x < t.tx; ++x) {
+                    int64_t const a = t.acc[c * tv + (static_cast<size_t>(z) * t.ty + y) * t.tx + x];
+                    if (a == 0) continue;
+                    size_t const gi = st.idx(ox - t.lo + x, oy - t.lo + y, oz - t.lo + z);
+                    st.J[c][gi] += static_cast<T>(static_cast<double>(a) * inv);
+                }
+}
+
+template <typename T>
+std::vector<std::vector<int>> groupsOf(State<T>& st, int species, bool byPusher) {
+    std::vector<std::vector<int>> groups;
+    std::vector<std::pair<int, int>> keys;
+    for (int s = 0; s < static_cast<int>(st.sp.size()); ++s) {
+        if (species >= 0 && s != species) continue;
+        if (st.sp[s].desc.pusher == 0) continue;
+        std::pair<int, int> key{st.sp[s].supp, byPusher ? st.sp[s].desc.pusher : 0};
+        size_t gi = 0;
+        for (; gi < keys.size(); ++gi)
+            if (keys[gi] == key) break;
+        if (gi == keys.size()) {
+            keys.push_back(key);
+            groups.emplace_back();
+        }
+        groups[gi].push_back(s);
+    }
+    return groups;
+}
+
+// ParticlePush with (fused) or without the deposit.
+template <typename T>
+void push(State<T>& st, int species, bool fused) {
+    bool const fixedPoint = fused && std::is_same<T, float>::value;
+    auto groups = groupsOf(st, species, true);
+    for (const auto& grp : groups) {
+        int const supp = st.sp[grp[0]].supp;
+        if (!fused) {
+#pragma omp parallel for schedule(dynamic, 4)
+            for (int sc = 0; sc < st.nsc; ++sc) {
+                int sx, sy, sz;
+                scCoords(st, sc, sx, sy, sz);
+                for (int s : grp) {
+                    Species<T>& S = st.sp[s];
+                    size_t const base = static_cast<size_t>(sc) * S.cap;
+                    int step[3];
+                    for (uint32_t i = 0; i < S.count[sc]; ++i)
+                        pushOne(st, S, base + i, sx * SCX, sy * SCY, sz * SCZ, step);
+                }
+            }
+            continue;
+        }
+        int const lo = depLower(supp) + 1, up = depUpper(supp) + 1;
+        int const scs[3] = {SCX, SCY, SCZ};
+        int stride[3];
+        for (int d = 0; d < 3; ++d) stride[d] = (lo + up + scs[d] - 1) / scs[d] + 1;
+        auto ph = phases(st, stride);
+        for (const auto& members : ph) {
+#pragma omp parallel for schedule(dynamic, 1)
+            for (size_t m = 0; m < members.size(); ++m) {
+                int const sc = members[m];
+                int sx, sy, sz;
+                scCoords(st, sc, sx, sy, sz);
+                int const ox = sx * SCX, oy = sy * SCY, oz = sz * SCZ;
+                Tile tile;
+                tile.reset(lo, up);
+                size_t const tv = static_cast<size_t>(tile.tx) * tile.ty * tile.tz;
+                T const jscale = static_cast<T>(std::ldexp(1.0, st.jexp));
+                bool any = false;
+                for (int s : grp) {
+                    Species<T>& S = st.sp[s];
+                    size_t const base = static_cast<size_t>(sc) * S.cap;
+                    for (uint32_t i = 0; i < S.count[sc]; ++i) {
+                        any = true;
+                        size_t const g = base + i;
+                        int const lc0 = S.cell[g];
+                        int step[3];
+                        V3<T> const vel = pushOne(st, S, g, ox, oy, oz, step);
+                        // new cell relative to the supercell origin, in tile coordinates
+                        int const nx = lc0 % SCX + step[0] + lo;
+                        int const ny = (lc0 / SCX) % SCY + step[1] + lo;
+                        int const nz = lc0 / (SCX * SCY) + step[2] + lo;
+                        T const pos[3] = {S.pos[0][g], S.pos[1][g], S.pos[2][g]};
+                        T const charge = S.charge * S.w[g];
+                        if (fixedPoint) {
+                            esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
+                                int64_t const q = static_cast<int64_t>(std::nearbyint(v * jscale));
+                                tile.acc[c * tv + (static_cast<size_t>(nz + z) * tile.ty + (ny + y)) * tile.tx +
+                                         (nx + x)] += q;
+                            });
+                        } else {
+                            // FP64 twin: plain sums straight into J (each supercell of a phase is disjoint)
+                            esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
+                                st.J[c][st.idx(ox - lo + nx + x, oy - lo + ny + y, oz - lo + nz + z)] += v;
+                            });
+                        }
+                    }
+                }
+                if (fixedPoint && any) flushTile(st, tile, ox, oy, oz);
+            }
+        }
+    }
+}
+
+// CurrentDeposition from the present particle state (KernelComputeCurrent,
+// FieldJ.kernel:51-107, ComputePerFrame :109-142).
+// order 0: fixed-point tiles in the device's checkerboard; otherwise the
+// reference's CPU order: float cache per supercell, particles in reverse frame
+// order (ForEach.hpp:314-324), strided flush (Deposit.hpp:63-84).
+template <typename T>
+void deposit(State<T>& st, int species) {
+    bool const fixedPoint = st.order == 0 && std::is_same<T, float>::value;
+    auto groups = groupsOf(st, species, false);
+    for (const auto& grp : groups) {
+        int const supp = st.sp[grp[0]].supp;
+        int const lo = depLower(supp), up = depUpper(supp);
+        int const scs[3] = {SCX, SCY, SCZ};
+        int stride[3];
+        if (fixedPoint) {
+            for (int d = 0; d < 3; ++d) stride[d] = (lo + up + scs[d] - 1) / scs[d] + 1;
+        } else {
+            int const skip = (lo + up + SCZ - 1) / SCZ;  // min supercell edge is 4
+            for (int d = 0; d < 3; ++d) stride[d] = skip + 1;
+        }
+        auto ph = phases(st, stride);
+        for (const auto& members : ph) {
+#pragma omp parallel for schedule(dynamic, 1)
+            for (size_t m = 0; m < members.size(); ++m) {
+                int const sc = members[m];
+                int sx, sy, sz;
+                scCoords(st, sc, sx, sy, sz);
+                int const ox = sx * SCX, oy = sy * SCY, oz = sz * SCZ;
+                int const tx = SCX + lo + up, ty = SCY + lo + up, tz = SCZ + lo + up;
+                size_t const tv = static_cast<size_t>(tx) * ty * tz;
+                std::vector<int64_t> acc;
+                std::vector<T> cache;
+                if (fixedPoint)
+                    acc.assign(3 * tv, 0);
+                else
+                    cache.assign(3 * tv, T(0));
+                T const jscale = static_cast<T>(std::ldexp(1.0, st.jexp));
+                bool any = false;
+                for (int s : grp) {
+                    Species<T>& S = st.sp[s];
+                    size_t const base = static_cast<size_t>(sc) * S.cap;
+                    uint32_t const n = S.count[sc];
+                    if (n == 0) continue;
+                    any = true;
+                    // reverse frame order, ascending slots inside a frame
+                    uint32_t const nFrames = (n + SCV - 1) / SCV;
+                    for (int f = static_cast<int>(nFrames) - 1; f >= 0; --f) {
+                        for (uint32_t l = 0; l < SCV; ++l) {
+                            uint32_t const i = static_cast<uint32_t>(f) * SCV + l;
+                            if (i >= n) break;
+                            size_t const g = base + i;
+                            int const lc = S.cell[g];
+                            int const cx = lc % SCX + lo, cy = (lc / SCX) % SCY + lo, cz = lc / (SCX * SCY) + lo;
+                            T const pos[3] = {S.pos[0][g], S.pos[1][g], S.pos[2][g]};
+                            T const w = S.w[g];
+                            V3<T> const mom{S.mom[0][g], S.mom[1][g], S.mom[2][g]};
+                            V3<T> const vel = velocityOf(mom, S.mass * w, st.k.mue0_eps0);
+                            T const charge = S.charge * w;
+                            esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
+                                size_t const ti = c * tv + (static_cast<size_t>(cz + z) * ty + (cy + y)) * tx + (cx + x);
+                                if (fixedPoint)
+                                    acc[ti] += static_cast<int64_t>(std::nearbyint(v * jscale));
+                                else
+                                    cache[ti] += v;
+                            });
+                        }
+                    }
+                }
+                if (!any) continue;
+                if (fixedPoint) {
+                    Tile t;
+                    t.lo = lo;
+                    t.up = up;
+                    t.tx = tx;
+                    t.ty = ty;
+                    t.tz = tz;
+                    t.acc.swap(acc);
+                    flushTile(st, t, ox, oy, oz);
+                } else {
+                    for (int c = 0; c < 3; ++c)
+                        for (int z = 0; z < tz; ++z)
+                            for (int y = 0; y < ty; ++y)
+                                for (int x = 0; x < tx; ++x)
+                                    st.J[c][st.idx(ox - lo + x, oy - lo + y, oz - lo + z)] +=
+                                        cache[c * tv + (static_cast<size_t>(z) * ty + y) * tx + x];
+                }
+            }
+        }
+    }
+}
+
+// Canonical supercell shift (same order as the device):
+// collect leavers per exchange code in slot order; each supercell appends the
+// leavers its neighbours send to it for codes 1..26; gaps below the new count
+// are filled, k-th ascending gap by the k-th ascending valid slot above it.
+template <typename T>
+void shift(State<T>& st, int species) {
+    for (int s = 0; s < static_cast<int>(st.sp.size()); ++s) {
+        if (species >= 0 && s != species) continue;
+        Species<T>& S = st.sp[s];
+        if (S.desc.pusher == 0) continue;
+        std::vector<std::vector<std::vector<int>>> out(st.nsc, std::vector<std::vector<int>>(27));
+        std::vector<uint32_t> nLeave(st.nsc, 0);
+#pragma omp parallel for schedule(static)
+        for (int sc = 0; sc < st.nsc; ++sc) {
+            size_t const base = static_cast<size_t>(sc) * S.cap;
+            for (uint32_t i = 0; i < S.count[sc]; ++i) {
+                int const m = S.mask[base + i];
+                if (m >= 2) {
+                    out[sc][m - 1].push_back(static_cast<int>(i));
+                    ++nLeave[sc];
+                }
+            }
+        }
+        std::vector<uint32_t> occ(st.nsc);
+        bool overflow = false;
+#pragma omp parallel for schedule(static) reduction(|| : overflow)
+        for (int d = 0; d < st.nsc; ++d) {
+            int dx, dy, dz;
+            scCoords(st, d, dx, dy, dz);
+            size_t const dbase = static_cast<size_t>(d) * S.cap;
+            uint32_t run = S.count[d];
+            for (int code = 1; code < 27; ++code) {
+                int const ex = code % 3, ey = (code / 3) % 3, ez = code / 9;
+                int const tx = ex == 2 ? -1 : ex, ty = ey == 2 ? -1 : ey, tz = ez == 2 ? -1 : ez;
+                int const sx = ((dx - tx) % st.nscx + st.nscx) % st.nscx;
+                int const sy = ((dy - ty) % st.nscy + st.nscy) % st.nscy;
+                int const sz = ((dz - tz) % st.nscz + st.nscz) % st.nscz;
+                int const src = sx + st.nscx * (sy + st.nscy * sz);
+                size_t const sbase = static_cast<size_t>(src) * S.cap;
+                for (int slot : out[src][code]) {
+                    if (run >= static_cast<uint32_t>(S.cap)) {
+                        overflow = true;
+                        ++run;
+                        continue;
+                    }
+                    size_t const a = dbase + run, b = sbase + static_cast<size_t>(slot);
+                    for (int c = 0; c < 3; ++c) {
+                        S.pos[c][a] = S.pos[c][b];
+                        S.mom[c][a] = S.mom[c][b];
+                    }
+                    S.w[a] = S.w[b];
+                    S.cell[a] = S.cell[b];
+                    S.mask[a] = 1;
+                    ++run;
+                }
+            }
+            occ[d] = std::min<uint32_t>(run, static_cast<uint32_t>(S.cap));
+        }
+        if (overflow) st.capacityError = true;
+#pragma omp parallel for schedule(static)
+        for (int d = 0; d < st.nsc; ++d) {
+            size_t const base = static_cast<size_t>(d) * S.cap;
+            uint32_t const tot = occ[d];
+            uint32_t const L = nLeave[d];
+            if (L == 0) {
+                S.count[d] = tot;
+                continue;
+            }
+            uint32_t const nn = tot > L ? tot - L : 0;
+            std::vector<uint32_t> gaps, srcs;
+            for (uint32_t i = 0; i < nn; ++i)
+                if (S.mask[base + i] >= 2) gaps.push_back(i);
+            for (uint32_t i = nn; i < tot; ++i)
+                if (S.mask[base + i] == 1) srcs.push_back(i);
+            size_t const M = std::min(gaps.size(), srcs.size());
+            for (size_t q = 0; q < M; ++q) {
+                size_t const a = base + gaps[q], b = base + srcs[q];
+                for (int c = 0; c < 3; ++c) {
+                    S.pos[c][a] = S.pos[c][b];
+                    S.mom[c][a] = S.mom[c][b];
+                }
+                S.w[a] = S.w[b];
+                S.cell[a] = S.cell[b];
+                S.mask[a] = 1;
+            }
+            for (uint32_t i = nn; i < tot; ++i) S.mask[base + i] = 0;
+            S.count[d] = nn;
+        }
+    }
+}
+
+// ---- FDTD: FDTDBase.kernel:50-124, Curl.hpp:89, Forward/BackwardDerivative.hpp,
+// Lehe/Derivative.hpp:65-228, AddCurrentDensity + None.hpp:56-65
+template <typename T>
+T fwd(const State<T>& st, const std::vector<T>& f, int d, int x, int y, int z) {
+    int const ux = d == 0, uy = d == 1, uz = d == 2;
+    return (f[st.idx(x + ux, y + uy, z + uz)] - f[st.idx(x, y, z)]) / st.k.cell[d];
+}
+
+template <typename T>
+T derivE(const State<T>& st, const std::vector<T>& f, int d, int x, int y, int z) {
+    if (st.cfg.solver != 1) return fwd(st, f, d, x, y, z);
+    int const L = st.cfg.lehe_dir;
+    if (d == L) {
+        int const d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+        int const u1[3] = {d1 == 0, d1 == 1, d1 == 2};
+        int const u2[3] = {d2 == 0, d2 == 1, d2 == 2};
+        int const u0[3] = {d == 0, d == 1, d == 2};
+        T r = st.k.lehe_alpha * fwd(st, f, d, x, y, z);
+        r = r + st.k.lehe_beta1 * fwd(st, f, d, x + u1[0], y + u1[1], z + u1[2]);
+        r = r + st.k.lehe_beta1 * fwd(st, f, d, x - u1[0], y - u1[1], z - u1[2]);
+        r = r + st.k.lehe_beta2 * fwd(st, f, d, x + u2[0], y + u2[1], z + u2[2]);
+        r = r + st.k.lehe_beta2 * fwd(st, f, d, x - u2[0], y - u2[1], z - u2[2]);
+        r = r + st.k.lehe_delta *
+                    (f[st.idx(x + 2 * u0[0], y + 2 * u0[1], z + 2 * u0[2])] - f[st.idx(x - u0[0], y - u0[1], z - u0[2])]) /
+                    st.k.cell[d];
+        return r;
+    }
+    int const ul[3] = {L == 0, L == 1, L == 2};
+    T const beta = T(0.125);
+    T const alpha = T(1.0) - T(2.0) * beta;
+    T r = alpha * fwd(st, f, d, x, y, z);
+    r = r + beta * fwd(st, f, d, x + ul[0], y + ul[1], z + ul[2]);
+    r = r + beta * fwd(st, f, d, x - ul[0], y - ul[1], z - ul[2]);
+    return r;
+}
+
+template <typename T>
+T bwd(const State<T>& st, const std::vector<T>& f, int d, int x, int y, int z) {
+    int const ux = d == 0, uy = d == 1, uz = d == 2;
+    return (f[st.idx(x, y, z)] - f[st.idx(x - ux, y - uy, z - uz)]) / st.k.cell[d];
+}
+
+template <typename T>
+void bHalf(State<T>& st) {
+    std::vector<T> nb[3];
+    for (int c = 0; c < 3; ++c) nb[c] = st.B[c];
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < st.nz; ++z)
+        for (int y = 0; y < st.ny; ++y)
+            for (int x = 0; x < st.nx; ++x) {
+                size_t const i = st.idx(x, y, z);
+                T const cx = derivE(st, st.E[2], 1, x, y, z) - derivE(st, st.E[1], 2, x, y, z);
+                T const cy = derivE(st, st.E[0], 2, x, y, z) - derivE(st, st.E[2], 0, x, y, z);
+                T const cz = derivE(st, st.E[1], 0, x, y, z) - derivE(st, st.E[0], 1, x, y, z);
+                nb[0][i] = st.B[0][i] - cx * T(0.5) * st.k.dt;
+                nb[1][i] = st.B[1][i] - cy * T(0.5) * st.k.dt;
+                nb[2][i] = st.B[2][i] - cz * T(0.5) * st.k.dt;
+            }
+    for (int c = 0; c < 3; ++c) st.B[c].swap(nb[c]);
+}
+
+template <typename T>
+void eFull(State<T>& st) {
+    T const c2 = st.k.c * st.k.c;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < st.nz; ++z)
+        for (int y = 0; y < st.ny; ++y)
+            for (int x = 0; x < st.nx; ++x) {
+                size_t const i = st.idx(x, y, z);
+                T const cx = bwd(st, st.B[2], 1, x, y, z) - bwd(st, st.B[1], 2, x, y, z);
+                T const cy = bwd(st, st.B[0], 2, x, y, z) - bwd(st, st.B[2], 0, x, y, z);
+                T const cz = bwd(st, st.B[1], 0, x, y, z) - bwd(st, st.B[0], 1, x, y, z);
+                st.E[0][i] = st.E[0][i] + cx * c2 * st.k.dt;
+                st.E[1][i] = st.E[1][i] + cy * c2 * st.k.dt;
+                st.E[2][i] = st.E[2][i] + cz * c2 * st.k.dt;
+            }
+}
+
+template <typename T>
+void addCurrent(State<T>& st) {
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < st.ncells; ++i)
+        for (int c = 0; c < 3; ++c) st.E[c][i] = st.E[c][i] + st.k.coeff * st.J[c][i];
+}
+
+template <typename T>
+void step(State<T>& st, int n) {
+    for (int it = 0; it < n; ++it) {
+        for (int c = 0; c < 3; ++c) std::fill(st.J[c].begin(), st.J[c].end(), T(0));  // CurrentReset
+        bool const fused = st.order == 0 && std::is_same<T, float>::value;
+        push(st, -1, fused);  // ParticlePush
+        shift(st, -1);
+        bHalf(st);  // update_beforeCurrent
+        eFull(st);
+        if (!fused) deposit(st, -1);  // CurrentDeposition
+        addCurrent(st);  // CurrentInterpolationAndAdditionToEMF
+        bHalf(st);  // update_afterCurrent
+    }
+}
+
+// charge density at the cell origins from the B-spline assignment of every
+// species (immobile ones included), FP64 accumulation
+template <typename T>
+void chargeDensity(const State<T>& st, double* rho) {
+    std::fill(rho, rho + st.ncells, 0.0);
+    for (const Species<T>& S : st.sp) {
+        ShapeFns<T> sh{S.supp};
+        int const begin = sh.begin();
+        for (int sc = 0; sc < st.nsc; ++sc) {
+            int sx, sy, sz;
+            scCoords(st, sc, sx, sy, sz);
+            size_t const base = static_cast<size_t>(sc) * S.cap;
+            for (uint32_t i = 0; i < S.count[sc]; ++i) {
+                size_t const g = base + i;
+                int const lc = S.cell[g];
+                int const c[3] = {sx * SCX + lc % SCX, sy * SCY + (lc / SCX) % SCY, sz * SCZ + lc / (SCX * SCY)};
+                double const q = static_cast<double>(S.charge) * static_cast<double>(S.w[g]) *
+                                 static_cast<double>(st.k.inv_vdt) * static_cast<double>(st.k.dt);
+                double wgt[3][5];
+                for (int d = 0; d < 3; ++d)
+                    for (int o = 0; o <= S.supp; ++o)
+                        wgt[d][o] = static_cast<double>(sh.assign(T(begin + o) - S.pos[d][g]));
+                for (int oz = 0; oz <= S.supp; ++oz)
+                    for (int oy = 0; oy <= S.supp; ++oy)
+                        for (int ox = 0; ox <= S.supp; ++ox) {
+                            double const v = q * wgt[0][ox] * wgt[1][oy] * wgt[2][oz];
+                            if (v != 0.0) rho[st.idx(c[0] + begin + ox, c[1] + begin + oy, c[2] + begin + oz)] += v;
+                        }
+            }
+        }
+    }
+}
+
+template <typename T>
+State<T>* create(const oracle_config& cfg, int order) {
+    State<T>* st = new State<T>();
+    st->cfg = cfg;
+    st->k = makeConsts<T>(cfg);
+    st->nx = cfg.grid[0];
+    st->ny = cfg.grid[1];
+    st->nz = cfg.grid[2];
+    st->nscx = st->nx / SCX;
+    st->nscy = st->ny / SCY;
+    st->nscz = st->nz / SCZ;
+    st->nsc = st->nscx * st->nscy * st->nscz;
+    st->ncells = static_cast<size_t>(st->nx) * st->ny * st->nz;
+    st->order = order;
+    for (int c = 0; c < 3; ++c) {
+        st->E[c].assign(st->ncells, T(0));
+        st->B[c].assign(st->ncells, T(0));
+        st->J[c].assign(st->ncells, T(0));
+    }
+    for (int s = 0; s < cfg.n_species; ++s) {
+        Species<T> S;
+        S.desc = cfg.species[s];
+        S.supp = supportOf(S.desc.shape);
+        S.cap = S.desc.frames_per_sc * SCV;
+        S.mass = st->k.mass[s];
+        S.charge = st->k.charge[s];
+        size_t const slots = static_cast<size_t>(st->nsc) * S.cap;
+        for (int c = 0; c < 3; ++c) {
+            S.pos[c].assign(slots, T(0));
+            S.mom[c].assign(slots, T(0));
+        }
+        S.w.assign(slots, T(0));
+        S.cell.assign(slots, 0);
+        S.mask.assign(slots, 0);
+        S.count.assign(st->nsc, 0);
+        st->sp.push_back(std::move(S));
+    }
+    return st;
+}
+
+struct Handle {
+    int precision;
+    void* st;
+};
+
+template <typename R, typename F32, typename F64>
+R dispatch(Handle* h, F32 f32, F64 f64) {
+    if (h->precision == 64) return f64(static_cast<State<double>*>(h->st));
+    return f32(static_cast<State<float>*>(h->st));
+}
+
+}  // namespace
+
+extern "C" {
+
+void* oracle_create(const oracle_config* cfg, int32_t precision, int32_t order) {
+    Handle* h = new Handle();
+    h->precision = precision == 64 ? 64 : 32;
+    if (h->precision == 64)
+        h->st = create<double>(*cfg, order);
+    else
+        h->st = create<float>(*cfg, order);
+    return h;
+}
+
+void oracle_destroy(void* p) {
+    Handle* h = static_cast<Handle*>(p);
+    if (!h) return;
+    if (h->precision == 64)
+        delete static_cast<State<double>*>(h->st);
+    else
+        delete static_cast<State<float>*>(h->st);
+    delete h;
+}
+
+int32_t oracle_get_constants(const oracle_config* cfg, oracle_constants* out) {
+    Consts<float> k = makeConsts<float>(*cfg);
+    std::memset(out, 0, sizeof(*out));
+    for (int d = 0; d < 3; ++d) out->cell[d] = k.cell[d];
+    out->dt = k.dt;
+    out->c = k.c;
+    out->eps0 = k.eps0;
+    out->mue0 = k.mue0;
+    out->mue0_eps0 = k.mue0_eps0;
+    out->add_current_coeff = k.coeff;
+    out->inv_cell_volume_dt = k.inv_vdt;
+    out->lehe_alpha = k.lehe_alpha;
+    out->lehe_delta = k.lehe_delta;
+    for (int s = 0; s < cfg->n_species && s < 4; ++s) {
+        out->species_mass[s] = k.mass[s];
+        out->species_charge[s] = k.charge[s];
+    }
+    out->unit_length = k.unit_length;
+    out->unit_time = k.unit_time;
+    out->unit_mass = k.unit_mass;
+    out->unit_charge = k.unit_charge;
+    out->typical_weighting = k.wtyp;
+    return 0;
+}
+
+void oracle_set_jexp(void* p, int32_t e) {
+    dispatch<void>(static_cast<Handle*>(p), [&](State<float>* s) { s->jexp = e; },
+                   [&](State<double>* s) { s->jexp = e; });
+}
+
+int32_t oracle_set_threads(int32_t n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+// frames in the device layout: counts[nsc], pos/mom 3 planes of nsc*cap
+int32_t oracle_set_frames(void* p, int32_t sp, const uint32_t* counts, const float* pos, const float* mom,
+                          const float* w, const uint16_t* cell, const uint8_t* mask) {
+    auto impl = [&](auto* st) -> int32_t {
+        auto& S = st->sp[sp];
+        size_t const slots = static_cast<size_t>(st->nsc) * S.cap;
+        for (int i = 0; i < st->nsc; ++i) S.count[i] = counts[i];
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < slots; ++i) {
+                S.pos[c][i] = pos[c * slots + i];
+                S.mom[c][i] = mom[c * slots + i];
+            }
+        for (size_t i = 0; i < slots; ++i) {
+            S.w[i] = w[i];
+            S.cell[i] = cell[i];
+            S.mask[i] = mask[i];
+        }
+        return 0;
+    };
+    return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
+}
+
+// read frames back; float outputs (FP64 twin values rounded) or double outputs
+int32_t oracle_get_frames(void* p, int32_t sp, uint32_t* counts, double* pos, double* mom, double* w, uint16_t* cell,
+                          uint8_t* mask) {
+    auto impl = [&](auto* st) -> int32_t {
+        auto& S = st->sp[sp];
+        size_t const slots = static_cast<size_t>(st->nsc) * S.cap;
+        for (int i = 0; i < st->nsc; ++i) counts[i] = S.count[i];
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < slots; ++i) {
+                pos[c * slots + i] = static_cast<double>(S.pos[c][i]);
+                mom[c * slots + i] = static_cast<double>(S.mom[c][i]);
+            }
+        for (size_t i = 0; i < slots; ++i) {
+            w[i] = static_cast<double>(S.w[i]);
+            cell[i] = S.cell[i];
+            mask[i] = S.mask[i];
+        }
+        return 0;
+    };
+    return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
+}
+
+int32_t oracle_set_field(void* p, int32_t which, const float* data) {
+    auto impl = [&](auto* st) -> int32_t {
+        auto* f = which == 0 ? st->E : (which == 1 ? st->B : st->J);
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < st->ncells; ++i) f[c][i] = data[c * st->ncells + i];
+        return 0;
+    };
+    return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
+}
+
+int32_t oracle_get_field(void* p, int32_t which, double* data) {
+    auto impl = [&](auto* st) -> int32_t {
+        auto* f = which == 0 ? st->E : (which == 1 ? st->B : st->J);
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < st->ncells; ++i) data[c * st->ncells + i] = static_cast<double>(f[c][i]);
+        return 0;
+    };
+    return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
+}
+
+void oracle_current_reset(void* p) {
+    auto impl = [&](auto* st) {
+        for (int c = 0; c < 3; ++c) std::fill(st->J[c].begin(), st->J[c].end(), 0);
+    };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+void oracle_push(void* p, int32_t species, int32_t fused) {
+    auto impl = [&](auto* st) { push(*st, species, fused != 0); };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+void oracle_deposit(void* p, int32_t species) {
+    auto impl = [&](auto* st) { deposit(*st, species); };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+int32_t oracle_shift(void* p, int32_t species) {
+    auto impl = [&](auto* st) -> int32_t {
+        shift(*st, species);
+        return st->capacityError ? -3 : 0;
+    };
+    return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
+}
+
+void oracle_fdtd_before_current(void* p) {
+    auto impl = [&](auto* st) {
+        bHalf(*st);
+        eFull(*st);
+    };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+void oracle_add_current(void* p) {
+    auto impl = [&](auto* st) { addCurrent(*st); };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+void oracle_fdtd_after_current(void* p) {
+    auto impl = [&](auto* st) { bHalf(*st); };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+int32_t oracle_step(void* p, int32_t n) {
+    auto impl = [&](auto* st) -> int32_t {
+        step(*st, n);
+        return st->capacityError ? -3 : 0;
+    };
+    return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
+}
+
+void oracle_field_energy(void* p, double* out) {
+    auto impl = [&](auto* st) {
+        double e = 0.0, b = 0.0;
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < st->ncells; ++i) {
+                double const ev = st->E[c][i], bv = st->B[c][i];
+                e += ev * ev;
+                b += bv * bv;
+            }
+        out[0] = e;
+        out[1] = b;
+    };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+void oracle_charge_density(void* p, double* rho) {
+    auto impl = [&](auto* st) { chargeDensity(*st, rho); };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+// backward-difference divergence of E at the cell origins (ChargeConservation.tpp:118-136), FP64
+void oracle_div_e(void* p, double* div) {
+    auto impl = [&](auto* st) {
+        for (int z = 0; z < st->nz; ++z)
+            for (int y = 0; y < st->ny; ++y)
+                for (int x = 0; x < st->nx; ++x) {
+                    double r = 0.0;
+                    for (int d = 0; d < 3; ++d) {
+                        int const ux = d == 0, uy = d == 1, uz = d == 2;
+                        r += (static_cast<double>(st->E[d][st->idx(x, y, z)]) -
+                              static_cast<double>(st->E[d][st->idx(x - ux, y - uy, z - uz)])) /
+                             static_cast<double>(st->k.cell[d]);
+                    }
+                    div[st->idx(x, y, z)] = r;
+                }
+    };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
+// single-particle Esirkepov deposit into a caller array J (3 planes, nx*ny*nz)
+// for a particle at global cell `cell`, in-cell `pos`, velocity `vel`, charge q
+void oracle_esirkepov_single(const oracle_config* cfg, int32_t shape, const int32_t* cell, const float* pos,
+                             const float* vel, float charge, double* J) {
+    State<float> tmp;
+    tmp.cfg = *cfg;
+    tmp.k = makeConsts<float>(*cfg);
+    tmp.nx = cfg->grid[0];
+    tmp.ny = cfg->grid[1];
+    tmp.nz = cfg->grid[2];
+    size_t const n = static_cast<size_t>(tmp.nx) * tmp.ny * tmp.nz;
+    float const p[3] = {pos[0], pos[1], pos[2]};
+    V3<float> const v{vel[0], vel[1], vel[2]};
+    esirkepov(supportOf(shape), p, v, charge, tmp.k, [&](int c, int x, int y, int z, float val) {
+        J[c * n + tmp.idx(cell[0] + x, cell[1] + y, cell[2] + z)] += static_cast<double>(val);
+    });
+}
+
+// moveParticle on one particle (for the reference unit test cases)
+int32_t oracle_move_particle(float* pos, uint16_t* localCellIdx, uint8_t* multiMask) {
+    int step[3];
+    return moveParticle(pos, *localCellIdx, *multiMask, step);
+}
+
+// shape values: kind 0 = ChargeAssignment(x), 1 = on-support array at x
+// (supp values), 2 = cached array (supp+1 values, out-of-range flag in `flag`)
+void oracle_shape(int32_t shape, int32_t kind, float x, int32_t flag, float* out) {
+    ShapeFns<float> sh{supportOf(shape)};
+    if (kind == 0)
+        out[0] = sh.assign(x);
+    else if (kind == 1)
+        sh.onSupport(x, out);
+    else
+        sh.cached(x, flag != 0, out);
+}
+
+}  // extern "C"

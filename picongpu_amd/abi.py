@@ -1,0 +1,368 @@
+"""ctypes binding of the C ABI (include/picongpu_amd.h).
+
+`Simulation` owns one pic_ctx and exposes the stages of one PIC step with the
+names of the reference's stage functors (simulation/stage/*.hpp,
+simulation/control/Simulation.hpp:522-542). The library must be built
+(picongpu_amd.build); there is no fallback path: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+
+MAX_SPECIES = 4
+
+SHAPE_CIC, SHAPE_TSC, SHAPE_PQS = 1, 2, 3
+PUSHER_NONE, PUSHER_BORIS, PUSHER_VAY = 0, 1, 2
+SOLVER_YEE, SOLVER_LEHE = 0, 1
+FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
+TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2
+
+ERR = {0: "ok", -1: "invalid", -2: "hip", -3: "capacity", -4: "state"}
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [
+        ("mass_ratio", C.c_double),
+        ("charge_ratio", C.c_double),
+        ("shape", C.c_int32),
+        ("pusher", C.c_int32),
+        ("frames_per_sc", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("grid", C.c_int32 * 3),
+        ("cell_si", C.c_double * 3),
+        ("dt_si", C.c_double),
+        ("base_density_si", C.c_double),
+        ("typical_ppc", C.c_int32),
+        ("solver", C.c_int32),
+        ("lehe_dir", C.c_int32),
+        ("n_species", C.c_int32),
+        ("species", SpeciesDesc * MAX_SPECIES),
+        ("j_scale_exp", C.c_int32),
+        ("device", C.c_int32),
+    ]
+
+
+class Constants(C.Structure):
+    _fields_ = [
+        ("cell", C.c_float * 3),
+        ("dt", C.c_float),
+        ("c", C.c_float),
+        ("eps0", C.c_float),
+        ("mue0", C.c_float),
+        ("mue0_eps0", C.c_float),
+        ("add_current_coeff", C.c_float),
+        ("inv_cell_volume_dt", C.c_float),
+        ("lehe_alpha", C.c_float),
+        ("lehe_delta", C.c_float),
+        ("species_mass", C.c_float * MAX_SPECIES),
+        ("species_charge", C.c_float * MAX_SPECIES),
+        ("unit_length", C.c_double),
+        ("unit_time", C.c_double),
+        ("unit_mass", C.c_double),
+        ("unit_charge", C.c_double),
+        ("typical_weighting", C.c_double),
+    ]
+
+
+class InitDesc(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("ppc", C.c_int32),
+        ("drift_band", C.c_int32),
+        ("density_si", C.c_double),
+        ("u_drift", C.c_double * 3),
+        ("u_th", C.c_double * 3),
+    ]
+
+
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_I32 = C.POINTER(C.c_int32)
+_U32 = C.POINTER(C.c_uint32)
+_U16 = C.POINTER(C.c_uint16)
+_U8 = C.POINTER(C.c_uint8)
+_D = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); the exact set of symbols include/picongpu_amd.h declares
+SIGNATURES = {
+    "pic_last_error": (C.c_char_p, []),
+    "pic_abi_version": (C.c_int32, []),
+    "pic_constants_from_config": (C.c_int, [C.POINTER(Config), C.POINTER(Constants)]),
+    "pic_create": (C.c_int, [C.POINTER(Config), C.POINTER(_P)]),
+    "pic_destroy": (C.c_int, [_P]),
+    "pic_get_constants": (C.c_int, [_P, C.POINTER(Constants)]),
+    "pic_sync": (C.c_int, [_P]),
+    "pic_load_particles": (C.c_int, [_P, C.c_int32, C.c_int64, _I32, _F, _F, _F]),
+    "pic_read_particles": (C.c_int, [_P, C.c_int32, C.c_int64, C.POINTER(C.c_int64), _I32, _F, _F, _F]),
+    "pic_particle_count": (C.c_int64, [_P, C.c_int32]),
+    "pic_num_supercells": (C.c_int32, [_P]),
+    "pic_slots_per_supercell": (C.c_int32, [_P, C.c_int32]),
+    "pic_read_frames": (C.c_int, [_P, C.c_int32, _U32, _F, _F, _F, _U16, _U8]),
+    "pic_write_frames": (C.c_int, [_P, C.c_int32, _U32, _F, _F, _F, _U16, _U8]),
+    "pic_init_species": (C.c_int, [_P, C.c_int32, C.POINTER(InitDesc)]),
+    "pic_load_field": (C.c_int, [_P, C.c_int32, _F]),
+    "pic_read_field": (C.c_int, [_P, C.c_int32, _F]),
+    "pic_add_field_noise": (C.c_int, [_P, C.c_int32, C.c_uint64, C.c_float]),
+    "pic_current_reset": (C.c_int, [_P]),
+    "pic_push": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "pic_shift": (C.c_int, [_P, C.c_int32]),
+    "pic_deposit": (C.c_int, [_P, C.c_int32]),
+    "pic_fdtd_before_current": (C.c_int, [_P]),
+    "pic_add_current": (C.c_int, [_P]),
+    "pic_fdtd_after_current": (C.c_int, [_P]),
+    "pic_step": (C.c_int, [_P, C.c_int32]),
+    "pic_field_energy": (C.c_int, [_P, _D]),
+    "pic_error_flags": (C.c_int, [_P, _U32]),
+    "pic_timing_enable": (C.c_int, [_P, C.c_int32]),
+    "pic_kernel_timer": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64)]),
+    "pic_j_scale_exp": (C.c_int32, [_P]),
+    "pic_num_push_phases": (C.c_int32, [_P, C.c_int32]),
+}
+
+_lib = None
+
+
+def library_path() -> str:
+    return _build.LIB_PATH
+
+
+def load_library() -> C.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"picongpu_amd HIP library missing at {path}; run picongpu_amd.build.build_library()")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PicError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed ({ERR.get(code, code)}): {msg}")
+        self.code = code
+
+
+def check(code: int, where: str) -> None:
+    if code != 0:
+        raise PicError(code, where, load_library().pic_last_error().decode())
+
+
+def ptr(a: np.ndarray, ctype):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YEE, lehe_dir=1, j_scale_exp=0,
+                device=0) -> Config:
+    """species: list of dicts with mass_ratio, charge_ratio, shape, pusher, frames_per_sc."""
+    cfg = Config()
+    for d in range(3):
+        cfg.grid[d] = int(grid[d])
+        cfg.cell_si[d] = float(cell_si[d])
+    cfg.dt_si = float(dt_si)
+    cfg.base_density_si = float(density_si)
+    cfg.typical_ppc = int(ppc)
+    cfg.solver = int(solver)
+    cfg.lehe_dir = int(lehe_dir)
+    cfg.n_species = len(species)
+    for i, s in enumerate(species):
+        cfg.species[i].mass_ratio = float(s["mass_ratio"])
+        cfg.species[i].charge_ratio = float(s["charge_ratio"])
+        cfg.species[i].shape = int(s["shape"])
+        cfg.species[i].pusher = int(s["pusher"])
+        cfg.species[i].frames_per_sc = int(s["frames_per_sc"])
+    cfg.j_scale_exp = int(j_scale_exp)
+    cfg.device = int(device)
+    return cfg
+
+
+def constants_from_config(cfg: Config) -> Constants:
+    k = Constants()
+    check(load_library().pic_constants_from_config(C.byref(cfg), C.byref(k)), "pic_constants_from_config")
+    return k
+
+
+class Simulation:
+    """One periodic simulation box on the GPU (a pic_ctx)."""
+
+    def __init__(self, cfg: Config):
+        self.lib = load_library()
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(self.lib.pic_create(C.byref(cfg), C.byref(h)), "pic_create")
+        self.h = h
+        self.grid = tuple(cfg.grid)
+        self.ncells = int(np.prod(self.grid))
+        self.nsc = self.lib.pic_num_supercells(h)
+
+    def close(self):
+        if self.h:
+            self.lib.pic_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------
+    def constants(self) -> Constants:
+        k = Constants()
+        check(self.lib.pic_get_constants(self.h, C.byref(k)), "pic_get_constants")
+        return k
+
+    def slots(self, sp: int) -> int:
+        return self.lib.pic_slots_per_supercell(self.h, sp)
+
+    def load_particles(self, sp, cell, pos, mom, w):
+        cell = np.ascontiguousarray(cell, dtype=np.int32)
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        mom = np.ascontiguousarray(mom, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        n = w.shape[0]
+        check(self.lib.pic_load_particles(self.h, sp, n, ptr(cell, C.c_int32), ptr(pos, C.c_float),
+                                          ptr(mom, C.c_float), ptr(w, C.c_float)), "pic_load_particles")
+
+    def read_particles(self, sp):
+        n = C.c_int64()
+        check(self.lib.pic_read_particles(self.h, sp, 0, C.byref(n), None, None, None, None), "pic_read_particles")
+        n = n.value
+        cell = np.zeros((n, 3), np.int32)
+        pos = np.zeros((n, 3), np.float32)
+        mom = np.zeros((n, 3), np.float32)
+        w = np.zeros(n, np.float32)
+        m = C.c_int64()
+        check(self.lib.pic_read_particles(self.h, sp, n, C.byref(m), ptr(cell, C.c_int32), ptr(pos, C.c_float),
+                                          ptr(mom, C.c_float), ptr(w, C.c_float)), "pic_read_particles")
+        return cell, pos, mom, w
+
+    def particle_count(self, sp) -> int:
+        n = self.lib.pic_particle_count(self.h, sp)
+        if n < 0:
+            raise PicError(-2, "pic_particle_count", self.lib.pic_last_error().decode())
+        return int(n)
+
+    def read_frames(self, sp):
+        cap = self.slots(sp)
+        slots = self.nsc * cap
+        f = {
+            "counts": np.zeros(self.nsc, np.uint32),
+            "pos": np.zeros((3, slots), np.float32),
+            "mom": np.zeros((3, slots), np.float32),
+            "w": np.zeros(slots, np.float32),
+            "cell": np.zeros(slots, np.uint16),
+            "mask": np.zeros(slots, np.uint8),
+        }
+        check(self.lib.pic_read_frames(self.h, sp, ptr(f["counts"], C.c_uint32), ptr(f["pos"], C.c_float),
+                                       ptr(f["mom"], C.c_float), ptr(f["w"], C.c_float), ptr(f["cell"], C.c_uint16),
+                                       ptr(f["mask"], C.c_uint8)), "pic_read_frames")
+        return f
+
+    def write_frames(self, sp, f):
+        check(self.lib.pic_write_frames(self.h, sp, ptr(f["counts"], C.c_uint32), ptr(f["pos"], C.c_float),
+                                        ptr(f["mom"], C.c_float), ptr(f["w"], C.c_float), ptr(f["cell"], C.c_uint16),
+                                        ptr(f["mask"], C.c_uint8)), "pic_write_frames")
+
+    def init_species(self, sp, seed, ppc, density_si, u_drift=(0, 0, 0), u_th=(0, 0, 0), drift_band=0):
+        d = InitDesc()
+        d.seed = int(seed)
+        d.ppc = int(ppc)
+        d.drift_band = int(drift_band)
+        d.density_si = float(density_si)
+        for i in range(3):
+            d.u_drift[i] = float(u_drift[i])
+            d.u_th[i] = float(u_th[i])
+        check(self.lib.pic_init_species(self.h, sp, C.byref(d)), "pic_init_species")
+
+    def load_field(self, which, data):
+        data = np.ascontiguousarray(data, dtype=np.float32).reshape(3, -1)
+        assert data.shape[1] == self.ncells
+        check(self.lib.pic_load_field(self.h, which, ptr(data, C.c_float)), "pic_load_field")
+
+    def read_field(self, which):
+        out = np.zeros((3, self.ncells), np.float32)
+        check(self.lib.pic_read_field(self.h, which, ptr(out, C.c_float)), "pic_read_field")
+        return out
+
+    def add_field_noise(self, which, seed, amp):
+        check(self.lib.pic_add_field_noise(self.h, which, int(seed), float(amp)), "pic_add_field_noise")
+
+    # -- stages (Simulation::runOneStep order) -------------------------------
+    def current_reset(self):
+        check(self.lib.pic_current_reset(self.h), "pic_current_reset")
+
+    def push(self, species=-1, fuse_deposit=True):
+        check(self.lib.pic_push(self.h, species, 1 if fuse_deposit else 0), "pic_push")
+
+    def shift(self, species=-1):
+        check(self.lib.pic_shift(self.h, species), "pic_shift")
+
+    def deposit(self, species=-1):
+        check(self.lib.pic_deposit(self.h, species), "pic_deposit")
+
+    def update_before_current(self):
+        check(self.lib.pic_fdtd_before_current(self.h), "pic_fdtd_before_current")
+
+    def add_current(self):
+        check(self.lib.pic_add_current(self.h), "pic_add_current")
+
+    def update_after_current(self):
+        check(self.lib.pic_fdtd_after_current(self.h), "pic_fdtd_after_current")
+
+    def step(self, n=1):
+        check(self.lib.pic_step(self.h, int(n)), "pic_step")
+
+    def sync(self):
+        check(self.lib.pic_sync(self.h), "pic_sync")
+
+    # -- diagnostics ---------------------------------------------------------
+    def field_energy(self):
+        out = np.zeros(2, np.float64)
+        check(self.lib.pic_field_energy(self.h, ptr(out, C.c_double)), "pic_field_energy")
+        return out
+
+    def error_flags(self) -> int:
+        f = C.c_uint32()
+        check(self.lib.pic_error_flags(self.h, C.byref(f)), "pic_error_flags")
+        return f.value
+
+    def timing(self, enable: bool):
+        check(self.lib.pic_timing_enable(self.h, 1 if enable else 0), "pic_timing_enable")
+
+    def kernel_timer(self, timer: int):
+        ms = C.c_double()
+        n = C.c_int64()
+        check(self.lib.pic_kernel_timer(self.h, timer, C.byref(ms), C.byref(n)), "pic_kernel_timer")
+        return ms.value, n.value
+
+    def j_scale_exp(self) -> int:
+        return self.lib.pic_j_scale_exp(self.h)
+
+    def num_push_phases(self, fused=True) -> int:
+        return self.lib.pic_num_push_phases(self.h, 1 if fused else 0)

@@ -1,0 +1,203 @@
+// FDTD field update on the periodic box.
+//
+// UpdateBHalfFunctor (B -= curl E * 0.5 * dt) and UpdateEFunctor
+// (E += curl B * c^2 * dt) of fields/MaxwellSolver/FDTD/FDTDBase.kernel:50-124,
+// with the Yee curls (Curl<Forward> for E, Curl<Backward> for B;
+// fields/differentiation/Curl.hpp:89, ForwardDerivative.hpp:44-65,
+// BackwardDerivative.hpp:44-65) or the Lehe curl of E
+// (fields/MaxwellSolver/Lehe/Derivative.hpp:65-228), and AddCurrentDensity
+// with currentInterpolation::None (E += coeff * J, None.hpp:56-65).
+// Fields are SoA planes, x fastest; neighbours wrap periodically.
+#include "pic_kernels.hpp"
+
+namespace pic {
+
+struct Idx {
+    int nx, ny, nz;
+    int x, y, z;
+    __device__ __forceinline__ size_t at(int dx, int dy, int dz) const {
+        int xx = x + dx, yy = y + dy, zz = z + dz;
+        xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+        yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+        zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+        return (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz);
+    }
+};
+
+__device__ __forceinline__ void unit(int d, int& ux, int& uy, int& uz) {
+    ux = d == 0;
+    uy = d == 1;
+    uz = d == 2;
+}
+
+// forward difference of component f along axis d at offset (ox,oy,oz)
+__device__ __forceinline__ float fwd(const float* f, const Idx& I, int d, int ox, int oy, int oz, float step) {
+    int ux, uy, uz;
+    unit(d, ux, uy, uz);
+    return (f[I.at(ox + ux, oy + uy, oz + uz)] - f[I.at(ox, oy, oz)]) / step;
+}
+
+// Lehe derivative along axis d of component f (Lehe/Derivative.hpp:132-145 for
+// d == Cherenkov-free axis, :212-224 otherwise)
+__device__ __forceinline__ float leheDeriv(const float* f, const Idx& I, int d, const FieldArgs& a) {
+    int const L = a.lehe_dir;
+    float const step = a.cell[d];
+    if (d == L) {
+        int const d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+        int u1x, u1y, u1z, u2x, u2y, u2z, u0x, u0y, u0z;
+        unit(d1, u1x, u1y, u1z);
+        unit(d2, u2x, u2y, u2z);
+        unit(d, u0x, u0y, u0z);
+        float r = a.lehe_alpha * fwd(f, I, d, 0, 0, 0, step);
+        r = r + a.lehe_beta1 * fwd(f, I, d, u1x, u1y, u1z, step);
+        r = r + a.lehe_beta1 * fwd(f, I, d, -u1x, -u1y, -u1z, step);
+        r = r + a.lehe_beta2 * fwd(f, I, d, u2x, u2y, u2z, step);
+        r = r + a.lehe_beta2 * fwd(f, I, d, -u2x, -u2y, -u2z, step);
+        r = r + a.lehe_delta * (f[I.at(2 * u0x, 2 * u0y, 2 * u0z)] - f[I.at(-u0x, -u0y, -u0z)]) / step;
+        return r;
+    }
+    int ulx, uly, ulz;
+    unit(L, ulx, uly, ulz);
+    float const beta = 0.125f;
+    float const alpha = 1.0f - 2.0f * beta;
+    float r = alpha * fwd(f, I, d, 0, 0, 0, step);
+    r = r + beta * fwd(f, I, d, ulx, uly, ulz, step);
+    r = r + beta * fwd(f, I, d, -ulx, -uly, -ulz, step);
+    return r;
+}
+
+template <int SOLVER>
+__device__ __forceinline__ float derivE(const float* f, const Idx& I, int d, const FieldArgs& a) {
+    if (SOLVER == PIC_SOLVER_LEHE) return leheDeriv(f, I, d, a);
+    return fwd(f, I, d, 0, 0, 0, a.cell[d]);
+}
+
+template <int SOLVER>
+__global__ __launch_bounds__(256) void kBHalf(FieldArgs a) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Idx I{a.g.nx, a.g.ny, a.g.nz, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    // curl = (dFdy.z - dFdz.y, dFdz.x - dFdx.z, dFdx.y - dFdy.x)
+    float const cx = derivE<SOLVER>(a.E[2], I, 1, a) - derivE<SOLVER>(a.E[1], I, 2, a);
+    float const cy = derivE<SOLVER>(a.E[0], I, 2, a) - derivE<SOLVER>(a.E[2], I, 0, a);
+    float const cz = derivE<SOLVER>(a.E[1], I, 0, a) - derivE<SOLVER>(a.E[0], I, 1, a);
+    a.B[0][i] = a.B[0][i] - cx * 0.5f * a.dt;
+    a.B[1][i] = a.B[1][i] - cy * 0.5f * a.dt;
+    a.B[2][i] = a.B[2][i] - cz * 0.5f * a.dt;
+}
+
+__device__ __forceinline__ float bwd(const float* f, const Idx& I, int d, float step) {
+    int ux, uy, uz;
+    unit(d, ux, uy, uz);
+    return (f[I.at(0, 0, 0)] - f[I.at(-ux, -uy, -uz)]) / step;
+}
+
+__global__ __launch_bounds__(256) void kEFull(FieldArgs a) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Idx I{a.g.nx, a.g.ny, a.g.nz, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    float const cx = bwd(a.B[2], I, 1, a.cell[1]) - bwd(a.B[1], I, 2, a.cell[2]);
+    float const cy = bwd(a.B[0], I, 2, a.cell[2]) - bwd(a.B[2], I, 0, a.cell[0]);
+    float const cz = bwd(a.B[1], I, 0, a.cell[0]) - bwd(a.B[0], I, 1, a.cell[1]);
+    a.E[0][i] = a.E[0][i] + cx * a.c2 * a.dt;
+    a.E[1][i] = a.E[1][i] + cy * a.c2 * a.dt;
+    a.E[2][i] = a.E[2][i] + cz * a.c2 * a.dt;
+}
+
+__global__ __launch_bounds__(256) void kAddCurrent(FieldArgs a) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a.E[0][i] = a.E[0][i] + a.coeff * a.J[0][i];
+    a.E[1][i] = a.E[1][i] + a.coeff * a.J[1][i];
+    a.E[2][i] = a.E[2][i] + a.coeff * a.J[2][i];
+}
+
+__global__ __launch_bounds__(256) void kFill(float* p, size_t n, float v) {
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// per-block partial sums of E^2 and B^2 in FP64
+__global__ __launch_bounds__(256) void kEnergy(FieldArgs a, double* partials) {
+    __shared__ double sE[256], sB[256];
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    double e = 0.0, b = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        for (int c = 0; c < 3; ++c) {
+            double const ev = a.E[c][i], bv = a.B[c][i];
+            e += ev * ev;
+            b += bv * bv;
+        }
+    }
+    sE[threadIdx.x] = e;
+    sB[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sE[threadIdx.x] += sE[threadIdx.x + s];
+            sB[threadIdx.x] += sB[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sE[0];
+        partials[2 * blockIdx.x + 1] = sB[0];
+    }
+}
+
+__global__ void kCountSum(const uint32_t* c, int n, unsigned long long* out) {
+    __shared__ unsigned long long s[256];
+    unsigned long long acc = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += c[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+static inline int blocksFor(size_t n) { return (int)((n + 255) / 256); }
+
+hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    if (solver == PIC_SOLVER_LEHE)
+        hipLaunchKernelGGL(kBHalf<PIC_SOLVER_LEHE>, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(kBHalf<PIC_SOLVER_YEE>, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchEFull(const FieldArgs& a, hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    hipLaunchKernelGGL(kEFull, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchAddCurrent(const FieldArgs& a, hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    hipLaunchKernelGGL(kAddCurrent, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchFill(float* p, size_t n, float v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kFill, dim3(blocksFor(n)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(kEnergy, dim3(nblocks), dim3(256), 0, s, a, partials);
+    return hipGetLastError();
+}
+
+hipError_t launchCountSum(const uint32_t* counts, int n, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(kCountSum, dim3(1), dim3(256), 0, s, counts, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace pic

@@ -1,0 +1,178 @@
+// Fused field gather + particle push + moveParticle + Esirkepov deposit.
+//
+// One workgroup per supercell (the reference's KernelMoveAndMarkParticles,
+// particles/Particles.kernel:169-273, and KernelComputeCurrent,
+// fields/FieldJ.kernel:51-107). The E/B tile of the supercell plus the
+// interpolation margins is staged in LDS; particles of all species in the
+// launch are streamed as coalesced SoA slot reads; each particle's current
+// is accumulated into an int64 fixed-point J tile in LDS (order independent)
+// that is flushed once to HBM. Launches run over a checkerboard of
+// supercells whose J tiles do not overlap, so the flush is a plain
+// read-add-write in a fixed phase order.
+#include "pic_device.hpp"
+#include "pic_kernels.hpp"
+
+namespace pic {
+
+__device__ __forceinline__ int wrapIdx(int v, int n) {
+    v = v < 0 ? v + n : v;
+    return v >= n ? v - n : v;
+}
+
+template <int SUPP, int PUSHER, int MODE>
+__global__ __launch_bounds__(256) void kPushDeposit(PushArgs a) {
+    constexpr int GLO = SUPP / 2;          // FieldToParticleInterpolation.hpp:49
+    constexpr int GUP = (SUPP + 1) / 2;    // FieldToParticleInterpolation.hpp:50
+    constexpr int GX = SCX + GLO + GUP, GY = SCY + GLO + GUP, GZ = SCZ + GLO + GUP;
+    constexpr int GV = GX * GY * GZ;
+    // fused: the deposit runs in the frame of the particle's new cell, which
+    // may lie one cell outside the supercell
+    constexpr int DLO = Esirkepov<SUPP>::lowerMargin + (MODE == 1 ? 1 : 0);
+    constexpr int DUP = Esirkepov<SUPP>::upperMargin + (MODE == 1 ? 1 : 0);
+    constexpr int JX = SCX + DLO + DUP, JY = SCY + DLO + DUP, JZ = SCZ + DLO + DUP;
+    constexpr int JV = JX * JY * JZ;
+    constexpr int EB_ELEMS = MODE == 2 ? 1 : 6 * GV;
+    constexpr int J_ELEMS = MODE == 0 ? 1 : 3 * JV;
+
+    __shared__ float sEB[EB_ELEMS];
+    __shared__ unsigned long long sJ[J_ELEMS];
+
+    int const sc = a.scList[blockIdx.x];
+    int const sx = sc % a.g.nscx;
+    int const sy = (sc / a.g.nscx) % a.g.nscy;
+    int const sz = sc / (a.g.nscx * a.g.nscy);
+    int const ox = sx * SCX, oy = sy * SCY, oz = sz * SCZ;
+    int const tid = threadIdx.x;
+
+    int total = 0;
+    for (int s = 0; s < a.nsp; ++s) total += (int)a.sp[s].count[sc];
+    if (total == 0) return;  // uniform across the workgroup
+
+    if (MODE != 2) {
+        for (int t = tid; t < 6 * GV; t += blockDim.x) {
+            int const c = t / GV;
+            int const r = t - c * GV;
+            int const tx = r % GX;
+            int const ty = (r / GX) % GY;
+            int const tz = r / (GX * GY);
+            int const gx = wrapIdx(ox - GLO + tx, a.g.nx);
+            int const gy = wrapIdx(oy - GLO + ty, a.g.ny);
+            int const gz = wrapIdx(oz - GLO + tz, a.g.nz);
+            size_t const gi = (size_t)gx + (size_t)a.g.nx * ((size_t)gy + (size_t)a.g.ny * gz);
+            const float* src = c < 3 ? a.E[c] : a.B[c - 3];
+            sEB[t] = src[gi];
+        }
+    }
+    if (MODE != 0) {
+        for (int t = tid; t < 3 * JV; t += blockDim.x) sJ[t] = 0ull;
+    }
+    __syncthreads();
+
+    PushConsts pk;
+    pk.cell[0] = a.cell[0];
+    pk.cell[1] = a.cell[1];
+    pk.cell[2] = a.cell[2];
+    pk.dt = a.dt;
+    pk.c2 = a.c2;
+    pk.rc2 = a.rc2;
+    pk.c = a.c;
+    DepositConsts dk;
+    dk.cell[0] = a.cell[0];
+    dk.cell[1] = a.cell[1];
+    dk.cell[2] = a.cell[2];
+    dk.dt = a.dt;
+    dk.inv_vdt = a.inv_vdt;
+    dk.jscale = a.jscale;
+
+    for (int s = 0; s < a.nsp; ++s) {
+        const SpeciesDev& S = a.sp[s];
+        int const n = (int)S.count[sc];
+        size_t const base = (size_t)sc * (size_t)S.cap;
+        for (int i = tid; i < n; i += blockDim.x) {
+            size_t const gidx = base + i;
+            Vec3 const pos{S.pos[0][gidx], S.pos[1][gidx], S.pos[2][gidx]};
+            Vec3 mom{S.mom[0][gidx], S.mom[1][gidx], S.mom[2][gidx]};
+            float const w = S.w[gidx];
+            int const lidx = S.cell[gidx];
+            int const lx = lidx % SCX, ly = (lidx / SCX) % SCY, lz = lidx / (SCX * SCY);
+            float const mass = S.mass * w;     // attribute::getMass (GetMass.hpp:40-44)
+            float const charge = S.charge * w; // attribute::getCharge (GetCharge.hpp:45-66)
+            if (MODE != 2) {
+                Vec3 const e = gatherE<SUPP, GX, GY, GV>(sEB, lx + GLO, ly + GLO, lz + GLO, pos);
+                Vec3 const b = gatherB<SUPP, GX, GY, GV>(sEB + 3 * GV, lx + GLO, ly + GLO, lz + GLO, pos);
+                Vec3 vel;
+                if (PUSHER == PIC_PUSHER_VAY)
+                    pushVay(e, b, mom, vel, mass, charge, pk);
+                else
+                    pushBoris(e, b, mom, vel, mass, charge, pk);
+                float p[3] = {pos.x, pos.y, pos.z};
+                p[0] += (vel.x * pk.dt) / pk.cell[0];
+                p[1] += (vel.y * pk.dt) / pk.cell[1];
+                p[2] += (vel.z * pk.dt) / pk.cell[2];
+                int dir[3], newIdx, newMask;
+                moveParticle(p, lidx, dir, newIdx, newMask);
+                S.pos[0][gidx] = p[0];
+                S.pos[1][gidx] = p[1];
+                S.pos[2][gidx] = p[2];
+                S.mom[0][gidx] = mom.x;
+                S.mom[1][gidx] = mom.y;
+                S.mom[2][gidx] = mom.z;
+                if ((dir[0] | dir[1] | dir[2]) != 0) S.cell[gidx] = (uint16_t)newIdx;
+                if (newMask >= 2) S.mask[gidx] = (uint8_t)newMask;
+                if (MODE == 1) {
+                    Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + dir[0] + DLO, ly + dir[1] + DLO,
+                                                                  lz + dir[2] + DLO, Vec3{p[0], p[1], p[2]}, vel,
+                                                                  charge, dk);
+                }
+            } else {
+                Vec3 const vel = velocityOf(mom, mass, a.rc2);
+                Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge, dk);
+            }
+        }
+    }
+    __syncthreads();
+
+    if (MODE != 0) {
+        for (int t = tid; t < 3 * JV; t += blockDim.x) {
+            long long const acc = (long long)sJ[t];
+            if (acc == 0) continue;
+            int const c = t / JV;
+            int const r = t - c * JV;
+            int const tx = r % JX;
+            int const ty = (r / JX) % JY;
+            int const tz = r / (JX * JY);
+            int const gx = wrapIdx(ox - DLO + tx, a.g.nx);
+            int const gy = wrapIdx(oy - DLO + ty, a.g.ny);
+            int const gz = wrapIdx(oz - DLO + tz, a.g.nz);
+            size_t const gi = (size_t)gx + (size_t)a.g.nx * ((size_t)gy + (size_t)a.g.ny * gz);
+            float const v = (float)((double)acc * a.jinv);
+            a.J[c][gi] += v;
+        }
+    }
+}
+
+template <int SUPP, int PUSHER, int MODE>
+static hipError_t launchT(const PushArgs& a, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((kPushDeposit<SUPP, PUSHER, MODE>), dim3(nblocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int SUPP>
+static hipError_t launchS(int pusher, int mode, const PushArgs& a, int nblocks, hipStream_t s) {
+    if (mode == 2) return launchT<SUPP, PIC_PUSHER_BORIS, 2>(a, nblocks, s);
+    if (pusher == PIC_PUSHER_VAY)
+        return mode == 1 ? launchT<SUPP, PIC_PUSHER_VAY, 1>(a, nblocks, s) : launchT<SUPP, PIC_PUSHER_VAY, 0>(a, nblocks, s);
+    return mode == 1 ? launchT<SUPP, PIC_PUSHER_BORIS, 1>(a, nblocks, s) : launchT<SUPP, PIC_PUSHER_BORIS, 0>(a, nblocks, s);
+}
+
+hipError_t launchPush(int supp, int pusher, int mode, const PushArgs& a, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    switch (supp) {
+        case 2: return launchS<2>(pusher, mode, a, nblocks, s);
+        case 3: return launchS<3>(pusher, mode, a, nblocks, s);
+        case 4: return launchS<4>(pusher, mode, a, nblocks, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace pic

@@ -1,0 +1,899 @@
+// Host runtime of the C ABI declared in include/picongpu_amd.h.
+//
+// Owns the device buffers of one simulation, derives the FP32 constants in
+// PIC units the way unitless/simulation.unitless does, plans the supercell
+// checkerboard phases of the deposit, and launches the kernels of each stage
+// on the context's HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/picongpu_amd.h"
+#include "pic_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PIC_HIP(expr)                                                                                \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) return fail(PIC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// param/physicalConstants.param:31-48 (2022 CODATA values)
+constexpr double kSpeedOfLightSI = 2.99792458e8;
+constexpr double kMue0SI = 1.25663706127e-6;
+constexpr double kElectronMassSI = 9.1093837139e-31;
+constexpr double kElectronChargeSI = -1.602176634e-19;
+constexpr double kPi = 3.141592653589793238462643383279502884197169399;
+
+int supportOf(int shape) { return shape == PIC_SHAPE_CIC ? 2 : (shape == PIC_SHAPE_TSC ? 3 : 4); }
+
+// Esirkepov current margins (Esirkepov.hpp:42-43)
+int depLower(int supp) { return supp / 2 + 1 - (supp + 1) % 2; }
+int depUpper(int supp) { return (supp + 1) / 2 + 1; }
+
+int validateConfig(const pic_config& c) {
+    const int sc[3] = {pic::kSCX, pic::kSCY, pic::kSCZ};
+    for (int d = 0; d < 3; ++d) {
+        if (c.grid[d] <= 0 || c.grid[d] % sc[d] != 0)
+            return fail(PIC_E_INVALID, "grid must be a positive multiple of the supercell 8x8x4 per axis");
+        if (!(c.cell_si[d] > 0.0)) return fail(PIC_E_INVALID, "cell_si must be positive");
+    }
+    if (!(c.dt_si > 0.0)) return fail(PIC_E_INVALID, "dt_si must be positive");
+    if (!(c.base_density_si > 0.0)) return fail(PIC_E_INVALID, "base_density_si must be positive");
+    if (c.typical_ppc <= 0) return fail(PIC_E_INVALID, "typical_ppc must be positive");
+    if (c.n_species < 0 || c.n_species > PIC_MAX_SPECIES) return fail(PIC_E_INVALID, "n_species out of range");
+    if (c.solver != PIC_SOLVER_YEE && c.solver != PIC_SOLVER_LEHE) return fail(PIC_E_INVALID, "unknown solver");
+    if (c.solver == PIC_SOLVER_LEHE && (c.lehe_dir < 0 || c.lehe_dir > 2))
+        return fail(PIC_E_INVALID, "lehe_dir must be 0, 1 or 2");
+    for (int s = 0; s < c.n_species; ++s) {
+        const pic_species_desc& d = c.species[s];
+        if (d.shape < PIC_SHAPE_CIC || d.shape > PIC_SHAPE_PQS) return fail(PIC_E_INVALID, "unknown shape");
+        if (d.pusher < PIC_PUSHER_NONE || d.pusher > PIC_PUSHER_VAY) return fail(PIC_E_INVALID, "unknown pusher");
+        if (d.frames_per_sc <= 0 || d.frames_per_sc > 255)
+            return fail(PIC_E_INVALID, "frames_per_sc must be in [1,255]");
+        if (!(d.mass_ratio > 0.0)) return fail(PIC_E_INVALID, "mass_ratio must be positive");
+    }
+    return PIC_OK;
+}
+
+void computeConstants(const pic_config& c, pic_constants& k) {
+    std::memset(&k, 0, sizeof(k));
+    // unitless/simulation.unitless:428-479
+    double const unitTime = c.dt_si;
+    double const unitSpeed = kSpeedOfLightSI;
+    double const unitLength = unitTime * unitSpeed;
+    double const cellVolume = c.cell_si[0] * c.cell_si[1] * c.cell_si[2];
+    double const wTyp = (c.base_density_si * cellVolume) / static_cast<double>(c.typical_ppc);
+    double const unitMass = kElectronMassSI * wTyp;
+    double const unitCharge = -1.0 * kElectronChargeSI * wTyp;
+    k.unit_length = unitLength;
+    k.unit_time = unitTime;
+    k.unit_mass = unitMass;
+    k.unit_charge = unitCharge;
+    k.typical_weighting = wTyp;
+    for (int d = 0; d < 3; ++d) k.cell[d] = static_cast<float>(c.cell_si[d] / unitLength);
+    k.dt = static_cast<float>(c.dt_si / unitTime);
+    k.c = static_cast<float>(kSpeedOfLightSI / unitSpeed);
+    k.mue0 = static_cast<float>(kMue0SI / unitLength / unitMass * unitCharge * unitCharge);
+    k.eps0 = static_cast<float>(1. / static_cast<double>(k.mue0) / static_cast<double>(k.c) / static_cast<double>(k.c));
+    k.mue0_eps0 = static_cast<float>(1. / static_cast<double>(k.c) / static_cast<double>(k.c));
+    // FDTD.hpp:84
+    k.add_current_coeff = -(1.0f / k.eps0) * k.dt;
+    // Esirkepov.hpp:190-192: 1/float(cellSize.productOfComponents()*dt)
+    float const vdt = k.cell[0] * k.cell[1] * k.cell[2] * k.dt;
+    k.inv_cell_volume_dt = 1.0f / vdt;
+    float const baseMass = static_cast<float>(kElectronMassSI / unitMass);
+    float const baseCharge = static_cast<float>(kElectronChargeSI / unitCharge);
+    for (int s = 0; s < c.n_species; ++s) {
+        k.species_mass[s] = baseMass * static_cast<float>(c.species[s].mass_ratio);
+        k.species_charge[s] = baseCharge * static_cast<float>(c.species[s].charge_ratio);
+    }
+    // Lehe/Derivative.hpp:90-111
+    if (c.solver == PIC_SOLVER_LEHE) {
+        int const d0 = c.lehe_dir, d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
+        double const stepRatio = static_cast<double>(k.cell[d0] / (k.c * k.dt));
+        double const coeff = stepRatio * std::sin(kPi / 2.0 * static_cast<double>(k.c) * static_cast<double>(k.dt) /
+                                                  static_cast<double>(k.cell[d0]));
+        k.lehe_delta = static_cast<float>(0.25 * (1.0 - coeff * coeff));
+        double const stepRatio1 = static_cast<double>(k.cell[d0] / k.cell[d1]);
+        double const stepRatio2 = static_cast<double>(k.cell[d0] / k.cell[d2]);
+        double const betaDir1 = 0.125 * stepRatio1 * stepRatio1;
+        double const betaDir2 = 0.125 * stepRatio2 * stepRatio2;
+        k.lehe_alpha =
+            static_cast<float>(1.0 - 2.0 * betaDir1 - 2.0 * betaDir2 - 3.0 * static_cast<double>(k.lehe_delta));
+    }
+}
+
+// CFL checks: Yee/Yee.hpp:43-65, Lehe/Lehe.hpp:47-69
+int checkCfl(const pic_config& c, const pic_constants& k) {
+    if (c.solver == PIC_SOLVER_YEE) {
+        float inv = 0.0f;
+        for (int d = 0; d < 3; ++d) inv += 1.0f / (k.cell[d] * k.cell[d]);
+        if (k.c * k.c * k.dt * k.dt * inv > 1.0f)
+            return fail(PIC_E_INVALID, "Courant-Friedrichs-Lewy condition failure (Yee)");
+    } else {
+        if (k.c * k.dt > k.cell[c.lehe_dir])
+            return fail(PIC_E_INVALID, "Courant-Friedrichs-Lewy condition failure (Lehe)");
+    }
+    return PIC_OK;
+}
+
+// Colours of a periodic ring of n supercells such that equal colours are at
+// least `s` apart: either the smallest divisor of n not below s, or
+// i % s for the first n - n%s supercells plus one colour each for the rest.
+int ringColours(int n, int s, std::vector<int>& colour) {
+    colour.assign(n, 0);
+    int div = s;
+    while (n % div != 0) ++div;
+    int const r = n % s;
+    if (div <= s + r) {
+        for (int i = 0; i < n; ++i) colour[i] = i % div;
+        return div;
+    }
+    for (int i = 0; i < n; ++i) colour[i] = i < n - r ? i % s : s + (i - (n - r));
+    return s + r;
+}
+
+struct PhasePlan {
+    bool valid = false;
+    int ncol[3] = {0, 0, 0};
+    std::vector<int> offset;  // per phase into list
+    std::vector<int> count;   // per phase
+    int32_t* dList = nullptr;
+};
+
+struct SpeciesMem {
+    pic_species_desc desc{};
+    int cap = 0;
+    float mass = 0.0f;
+    float charge = 0.0f;
+    double wmax = 0.0;
+    pic::SpeciesDev dev{};
+};
+
+}  // namespace
+
+struct pic_ctx {
+    pic_config cfg{};
+    pic_constants k{};
+    pic::GridDev g{};
+    int nsc = 0;
+    size_t ncells = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float* E[3] = {nullptr, nullptr, nullptr};
+    float* B[3] = {nullptr, nullptr, nullptr};
+    float* J[3] = {nullptr, nullptr, nullptr};
+    std::vector<SpeciesMem> sp;
+    uint32_t* errFlags = nullptr;
+    double* energyPartials = nullptr;
+    int energyBlocks = 0;
+    unsigned long long* countOut = nullptr;
+    int jexp = 40;
+    // [supp-2][fused]
+    PhasePlan plans[3][2];
+    int32_t* dAll = nullptr;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[PIC_TIMER_COUNT];
+    std::vector<hipEvent_t> eventPool;
+    double totalMs[PIC_TIMER_COUNT] = {0, 0, 0};
+    int64_t launches[PIC_TIMER_COUNT] = {0, 0, 0};
+    std::vector<void*> allocations;
+};
+
+namespace {
+
+template <typename T>
+int devAlloc(pic_ctx* c, T** p, size_t n) {
+    void* q = nullptr;
+    size_t const bytes = std::max<size_t>(n * sizeof(T), 16);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(PIC_E_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(q, 0, bytes, c->stream);
+    if (e != hipSuccess) return fail(PIC_E_HIP, std::string("hipMemsetAsync failed: ") + hipGetErrorString(e));
+    c->allocations.push_back(q);
+    *p = static_cast<T*>(q);
+    return PIC_OK;
+}
+
+hipEvent_t takeEvent(pic_ctx* c) {
+    if (!c->eventPool.empty()) {
+        hipEvent_t e = c->eventPool.back();
+        c->eventPool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct TimerScope {
+    pic_ctx* c;
+    int t;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimerScope(pic_ctx* ctx, int timer) : c(ctx), t(timer) {
+        if (c->timing) {
+            a = takeEvent(c);
+            b = takeEvent(c);
+            if (a) hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimerScope() {
+        if (c->timing && a && b) {
+            hipEventRecord(b, c->stream);
+            c->pending[t].push_back({a, b});
+        }
+    }
+};
+
+int resolveTimers(pic_ctx* c) {
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    for (int t = 0; t < PIC_TIMER_COUNT; ++t) {
+        for (auto& pr : c->pending[t]) {
+            float ms = 0.0f;
+            PIC_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+            c->totalMs[t] += ms;
+            c->launches[t] += 1;
+            c->eventPool.push_back(pr.first);
+            c->eventPool.push_back(pr.second);
+        }
+        c->pending[t].clear();
+    }
+    return PIC_OK;
+}
+
+int buildPlan(pic_ctx* c, int supp, bool fused, PhasePlan& plan) {
+    int const lo = depLower(supp) + (fused ? 1 : 0);
+    int const up = depUpper(supp) + (fused ? 1 : 0);
+    const int scs[3] = {pic::kSCX, pic::kSCY, pic::kSCZ};
+    const int nsc[3] = {c->g.nscx, c->g.nscy, c->g.nscz};
+    std::vector<int> col[3];
+    for (int d = 0; d < 3; ++d) {
+        int const stride = (lo + up + scs[d] - 1) / scs[d] + 1;
+        if (nsc[d] < stride)
+            return fail(PIC_E_INVALID, "grid has fewer supercells along an axis than the current stencil needs");
+        plan.ncol[d] = ringColours(nsc[d], stride, col[d]);
+    }
+    int const nph = plan.ncol[0] * plan.ncol[1] * plan.ncol[2];
+    std::vector<std::vector<int>> members(nph);
+    for (int z = 0; z < nsc[2]; ++z)
+        for (int y = 0; y < nsc[1]; ++y)
+            for (int x = 0; x < nsc[0]; ++x) {
+                int const ph = col[0][x] + plan.ncol[0] * (col[1][y] + plan.ncol[1] * col[2][z]);
+                members[ph].push_back(x + nsc[0] * (y + nsc[1] * z));
+            }
+    // launch order: workgroup b runs on XCD b % 8; give each XCD a contiguous
+    // run of spatially ordered supercells so neighbouring tiles share its L2
+    std::vector<int> list;
+    plan.offset.clear();
+    plan.count.clear();
+    for (int ph = 0; ph < nph; ++ph) {
+        const std::vector<int>& m = members[ph];
+        int const n = static_cast<int>(m.size());
+        std::vector<int> order(n);
+        int pos = 0;
+        for (int x = 0; x < 8; ++x) {
+            for (int b = x; b < n; b += 8) order[b] = m[pos++];
+        }
+        plan.offset.push_back(static_cast<int>(list.size()));
+        plan.count.push_back(n);
+        list.insert(list.end(), order.begin(), order.end());
+    }
+    if (devAlloc(c, &plan.dList, list.size()) != PIC_OK) return PIC_E_HIP;
+    PIC_HIP(hipMemcpyAsync(plan.dList, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    plan.valid = true;
+    return PIC_OK;
+}
+
+int getPlan(pic_ctx* c, int supp, bool fused, PhasePlan** out) {
+    PhasePlan& p = c->plans[supp - 2][fused ? 1 : 0];
+    if (!p.valid) {
+        int const r = buildPlan(c, supp, fused, p);
+        if (r != PIC_OK) return r;
+    }
+    *out = &p;
+    return PIC_OK;
+}
+
+void updateJexp(pic_ctx* c) {
+    if (c->cfg.j_scale_exp != 0) {
+        c->jexp = c->cfg.j_scale_exp;
+        return;
+    }
+    // largest magnitude one particle can add to a J cell: |q| * 1/(V dt) * max cell edge
+    double bound = 0.0;
+    float const maxCell = std::max(c->k.cell[0], std::max(c->k.cell[1], c->k.cell[2]));
+    for (const SpeciesMem& s : c->sp) {
+        if (s.desc.pusher == PIC_PUSHER_NONE) continue;
+        double const b = std::fabs((double)s.charge) * s.wmax * (double)c->k.inv_cell_volume_dt * (double)maxCell;
+        bound = std::max(bound, b);
+    }
+    if (!(bound > 0.0)) {
+        c->jexp = 40;
+        return;
+    }
+    int const e = 40 - static_cast<int>(std::ceil(std::log2(bound)));
+    c->jexp = std::max(-60, std::min(100, e));
+}
+
+pic::PushArgs makePushArgs(pic_ctx* c) {
+    pic::PushArgs a{};
+    a.g = c->g;
+    for (int d = 0; d < 3; ++d) {
+        a.E[d] = c->E[d];
+        a.B[d] = c->B[d];
+        a.J[d] = c->J[d];
+        a.cell[d] = c->k.cell[d];
+    }
+    a.dt = c->k.dt;
+    a.c = c->k.c;
+    a.c2 = c->k.c * c->k.c;
+    a.rc2 = c->k.mue0_eps0;
+    a.inv_vdt = c->k.inv_cell_volume_dt;
+    a.jscale = std::ldexp(1.0f, c->jexp);
+    a.jinv = std::ldexp(1.0, -c->jexp);
+    a.errFlags = c->errFlags;
+    return a;
+}
+
+pic::FieldArgs makeFieldArgs(pic_ctx* c) {
+    pic::FieldArgs a{};
+    a.g = c->g;
+    for (int d = 0; d < 3; ++d) {
+        a.E[d] = c->E[d];
+        a.B[d] = c->B[d];
+        a.J[d] = c->J[d];
+        a.cell[d] = c->k.cell[d];
+    }
+    a.dt = c->k.dt;
+    a.c2 = c->k.c * c->k.c;
+    a.coeff = c->k.add_current_coeff;
+    a.lehe_alpha = c->k.lehe_alpha;
+    a.lehe_delta = c->k.lehe_delta;
+    a.lehe_dir = c->cfg.lehe_dir;
+    if (c->cfg.solver == PIC_SOLVER_LEHE) {
+        int const d0 = c->cfg.lehe_dir, d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
+        float const r1 = c->k.cell[d0] / c->k.cell[d1];
+        float const r2 = c->k.cell[d0] / c->k.cell[d2];
+        a.lehe_beta1 = 0.125f * r1 * r1;
+        a.lehe_beta2 = 0.125f * r2 * r2;
+    }
+    return a;
+}
+
+int checkSpecies(pic_ctx* c, int32_t s) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (s < 0 || s >= static_cast<int>(c->sp.size())) return fail(PIC_E_INVALID, "species index out of range");
+    return PIC_OK;
+}
+
+// Mobile species grouped by (support, pusher), groups ordered by their first species.
+std::vector<std::vector<int>> pushGroups(pic_ctx* c, int32_t species, bool byPusher) {
+    std::vector<std::vector<int>> groups;
+    std::vector<std::pair<int, int>> keys;
+    for (int s = 0; s < static_cast<int>(c->sp.size()); ++s) {
+        if (species >= 0 && s != species) continue;
+        const SpeciesMem& m = c->sp[s];
+        if (m.desc.pusher == PIC_PUSHER_NONE) continue;
+        std::pair<int, int> key{supportOf(m.desc.shape), byPusher ? m.desc.pusher : 0};
+        size_t gi = 0;
+        for (; gi < keys.size(); ++gi)
+            if (keys[gi] == key) break;
+        if (gi == keys.size()) {
+            keys.push_back(key);
+            groups.emplace_back();
+        }
+        groups[gi].push_back(s);
+    }
+    return groups;
+}
+
+int runPush(pic_ctx* c, int32_t species, int mode) {
+    bool const byPusher = mode != 2;
+    auto groups = pushGroups(c, species, byPusher);
+    for (const auto& grp : groups) {
+        int const supp = supportOf(c->sp[grp[0]].desc.shape);
+        int const pusher = c->sp[grp[0]].desc.pusher;
+        pic::PushArgs a = makePushArgs(c);
+        a.nsp = static_cast<int>(grp.size());
+        for (size_t i = 0; i < grp.size(); ++i) a.sp[i] = c->sp[grp[i]].dev;
+        if (mode == 0) {
+            a.scList = c->dAll;
+            TimerScope ts(c, PIC_TIMER_PUSH);
+            PIC_HIP(pic::launchPush(supp, pusher, 0, a, c->nsc, c->stream));
+            continue;
+        }
+        PhasePlan* plan = nullptr;
+        int const r = getPlan(c, supp, mode == 1, &plan);
+        if (r != PIC_OK) return r;
+        for (size_t ph = 0; ph < plan->count.size(); ++ph) {
+            a.scList = plan->dList + plan->offset[ph];
+            TimerScope ts(c, PIC_TIMER_PUSH);
+            PIC_HIP(pic::launchPush(supp, pusher, mode, a, plan->count[ph], c->stream));
+        }
+    }
+    return PIC_OK;
+}
+
+int checkErrors(pic_ctx* c) {
+    uint32_t flags = 0;
+    PIC_HIP(hipMemcpyAsync(&flags, c->errFlags, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    if (flags & 1u) return fail(PIC_E_CAPACITY, "a supercell exceeded its frame capacity during the shift");
+    return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pic_last_error(void) { return g_err.c_str(); }
+
+int32_t pic_abi_version(void) { return PIC_ABI_VERSION; }
+
+int pic_constants_from_config(const pic_config* cfg, pic_constants* out) {
+    if (!cfg || !out) return fail(PIC_E_INVALID, "null argument");
+    int const r = validateConfig(*cfg);
+    if (r != PIC_OK) return r;
+    computeConstants(*cfg, *out);
+    return PIC_OK;
+}
+
+int pic_create(const pic_config* cfg, pic_ctx** out) {
+    if (!cfg || !out) return fail(PIC_E_INVALID, "null argument");
+    *out = nullptr;
+    int r = validateConfig(*cfg);
+    if (r != PIC_OK) return r;
+    pic_ctx* c = new pic_ctx();
+    c->cfg = *cfg;
+    computeConstants(*cfg, c->k);
+    r = checkCfl(*cfg, c->k);
+    if (r != PIC_OK) {
+        delete c;
+        return r;
+    }
+    c->g.nx = cfg->grid[0];
+    c->g.ny = cfg->grid[1];
+    c->g.nz = cfg->grid[2];
+    c->g.nscx = cfg->grid[0] / pic::kSCX;
+    c->g.nscy = cfg->grid[1] / pic::kSCY;
+    c->g.nscz = cfg->grid[2] / pic::kSCZ;
+    c->nsc = c->g.nscx * c->g.nscy * c->g.nscz;
+    c->ncells = static_cast<size_t>(cfg->grid[0]) * cfg->grid[1] * cfg->grid[2];
+    c->device = cfg->device;
+    auto bail = [&](int code) {
+        pic_destroy(c);
+        return code;
+    };
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return bail(fail(PIC_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(fail(PIC_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+    for (int d = 0; d < 3; ++d) {
+        if (devAlloc(c, &c->E[d], c->ncells) || devAlloc(c, &c->B[d], c->ncells) || devAlloc(c, &c->J[d], c->ncells))
+            return bail(PIC_E_HIP);
+    }
+    if (devAlloc(c, &c->errFlags, 1)) return bail(PIC_E_HIP);
+    c->energyBlocks = 1024;
+    if (devAlloc(c, &c->energyPartials, 2 * c->energyBlocks)) return bail(PIC_E_HIP);
+    if (devAlloc(c, &c->countOut, 1)) return bail(PIC_E_HIP);
+    std::vector<int32_t> all(c->nsc);
+    for (int i = 0; i < c->nsc; ++i) all[i] = i;
+    if (devAlloc(c, &c->dAll, all.size())) return bail(PIC_E_HIP);
+    e = hipMemcpyAsync(c->dAll, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(e)));
+    for (int s = 0; s < cfg->n_species; ++s) {
+        SpeciesMem m;
+        m.desc = cfg->species[s];
+        m.cap = cfg->species[s].frames_per_sc * pic::kSCV;
+        m.mass = c->k.species_mass[s];
+        m.charge = c->k.species_charge[s];
+        size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+        pic::SpeciesDev& d = m.dev;
+        for (int a = 0; a < 3; ++a) {
+            if (devAlloc(c, &d.pos[a], slots) || devAlloc(c, &d.mom[a], slots)) return bail(PIC_E_HIP);
+        }
+        if (devAlloc(c, &d.w, slots) || devAlloc(c, &d.cell, slots) || devAlloc(c, &d.mask, slots) ||
+            devAlloc(c, &d.count, c->nsc) || devAlloc(c, &d.occ, c->nsc) || devAlloc(c, &d.outIdx, slots) ||
+            devAlloc(c, &d.scratch, slots) || devAlloc(c, &d.outOff, static_cast<size_t>(c->nsc) * pic::kOffStride))
+            return bail(PIC_E_HIP);
+        d.cap = m.cap;
+        d.mass = m.mass;
+        d.charge = m.charge;
+        c->sp.push_back(m);
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(e)));
+    updateJexp(c);
+    *out = c;
+    return PIC_OK;
+}
+
+int pic_destroy(pic_ctx* c) {
+    if (!c) return PIC_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int t = 0; t < PIC_TIMER_COUNT; ++t)
+        for (auto& pr : c->pending[t]) {
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+    for (hipEvent_t e : c->eventPool) hipEventDestroy(e);
+    for (void* p : c->allocations) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return PIC_OK;
+}
+
+int pic_get_constants(const pic_ctx* c, pic_constants* out) {
+    if (!c || !out) return fail(PIC_E_INVALID, "null argument");
+    *out = c->k;
+    return PIC_OK;
+}
+
+int pic_sync(pic_ctx* c) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int32_t pic_num_supercells(const pic_ctx* c) { return c ? c->nsc : -1; }
+
+int32_t pic_slots_per_supercell(const pic_ctx* c, int32_t s) {
+    if (!c || s < 0 || s >= static_cast<int>(c->sp.size())) return -1;
+    return c->sp[s].cap;
+}
+
+int32_t pic_j_scale_exp(const pic_ctx* c) { return c ? c->jexp : 0; }
+
+int32_t pic_num_push_phases(const pic_ctx* c, int32_t fused) {
+    if (!c) return -1;
+    pic_ctx* m = const_cast<pic_ctx*>(c);
+    int maxPh = 0;
+    for (const SpeciesMem& s : m->sp) {
+        if (s.desc.pusher == PIC_PUSHER_NONE) continue;
+        PhasePlan* p = nullptr;
+        if (getPlan(m, supportOf(s.desc.shape), fused != 0, &p) != PIC_OK) return -1;
+        maxPh = std::max(maxPh, static_cast<int>(p->count.size()));
+    }
+    return maxPh;
+}
+
+int pic_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* cell, const float* pos, const float* mom,
+                       const float* w) {
+    int r = checkSpecies(c, s);
+    if (r != PIC_OK) return r;
+    if (n < 0 || (n > 0 && (!cell || !pos || !mom || !w))) return fail(PIC_E_INVALID, "null particle arrays");
+    SpeciesMem& m = c->sp[s];
+    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    std::vector<uint32_t> counts(c->nsc, 0);
+    std::vector<float> hp[3], hm[3];
+    for (int a = 0; a < 3; ++a) {
+        hp[a].assign(slots, 0.0f);
+        hm[a].assign(slots, 0.0f);
+    }
+    std::vector<float> hw(slots, 0.0f);
+    std::vector<uint16_t> hc(slots, 0);
+    std::vector<uint8_t> hk(slots, 0);
+    double wmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        int const x = cell[3 * i], y = cell[3 * i + 1], z = cell[3 * i + 2];
+        if (x < 0 || y < 0 || z < 0 || x >= c->g.nx || y >= c->g.ny || z >= c->g.nz)
+            return fail(PIC_E_INVALID, "particle cell outside the grid");
+        for (int a = 0; a < 3; ++a) {
+            float const p = pos[3 * i + a];
+            if (!(p >= 0.0f && p < 1.0f)) return fail(PIC_E_INVALID, "in-cell position outside [0,1)");
+        }
+        int const sc = x / pic::kSCX + c->g.nscx * (y / pic::kSCY + c->g.nscy * (z / pic::kSCZ));
+        int const lc = x % pic::kSCX + pic::kSCX * (y % pic::kSCY + pic::kSCY * (z % pic::kSCZ));
+        if (counts[sc] >= static_cast<uint32_t>(m.cap))
+            return fail(PIC_E_CAPACITY, "supercell frame capacity exceeded while loading particles");
+        size_t const g = static_cast<size_t>(sc) * m.cap + counts[sc]++;
+        for (int a = 0; a < 3; ++a) {
+            hp[a][g] = pos[3 * i + a];
+            hm[a][g] = mom[3 * i + a];
+        }
+        hw[g] = w[i];
+        hc[g] = static_cast<uint16_t>(lc);
+        hk[g] = 1;
+        wmax = std::max(wmax, std::fabs(static_cast<double>(w[i])));
+    }
+    r = pic_write_frames(c, s, counts.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (r != PIC_OK) return r;
+    pic::SpeciesDev& d = m.dev;
+    for (int a = 0; a < 3; ++a) {
+        PIC_HIP(hipMemcpyAsync(d.pos[a], hp[a].data(), slots * 4, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(hipMemcpyAsync(d.mom[a], hm[a].data(), slots * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    PIC_HIP(hipMemcpyAsync(d.w, hw.data(), slots * 4, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(d.cell, hc.data(), slots * 2, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(d.mask, hk.data(), slots, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    m.wmax = wmax;
+    updateJexp(c);
+    return PIC_OK;
+}
+
+int pic_write_frames(pic_ctx* c, int32_t s, const uint32_t* counts, const float* pos, const float* mom, const float* w,
+                     const uint16_t* lc, const uint8_t* mask) {
+    int r = checkSpecies(c, s);
+    if (r != PIC_OK) return r;
+    SpeciesMem& m = c->sp[s];
+    pic::SpeciesDev& d = m.dev;
+    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    if (counts) {
+        for (int i = 0; i < c->nsc; ++i)
+            if (counts[i] > static_cast<uint32_t>(m.cap)) return fail(PIC_E_CAPACITY, "count above frame capacity");
+        PIC_HIP(hipMemcpyAsync(d.count, counts, c->nsc * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (pos) PIC_HIP(hipMemcpyAsync(d.pos[a], pos + a * slots, slots * 4, hipMemcpyHostToDevice, c->stream));
+        if (mom) PIC_HIP(hipMemcpyAsync(d.mom[a], mom + a * slots, slots * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (w) {
+        PIC_HIP(hipMemcpyAsync(d.w, w, slots * 4, hipMemcpyHostToDevice, c->stream));
+        double wmax = 0.0;
+        for (size_t i = 0; i < slots; ++i) wmax = std::max(wmax, std::fabs(static_cast<double>(w[i])));
+        m.wmax = wmax;
+        updateJexp(c);
+    }
+    if (lc) PIC_HIP(hipMemcpyAsync(d.cell, lc, slots * 2, hipMemcpyHostToDevice, c->stream));
+    if (mask) PIC_HIP(hipMemcpyAsync(d.mask, mask, slots, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_read_frames(pic_ctx* c, int32_t s, uint32_t* counts, float* pos, float* mom, float* w, uint16_t* lc,
+                    uint8_t* mask) {
+    int r = checkSpecies(c, s);
+    if (r != PIC_OK) return r;
+    SpeciesMem& m = c->sp[s];
+    pic::SpeciesDev& d = m.dev;
+    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    if (counts) PIC_HIP(hipMemcpyAsync(counts, d.count, c->nsc * 4, hipMemcpyDeviceToHost, c->stream));
+    for (int a = 0; a < 3; ++a) {
+        if (pos) PIC_HIP(hipMemcpyAsync(pos + a * slots, d.pos[a], slots * 4, hipMemcpyDeviceToHost, c->stream));
+        if (mom) PIC_HIP(hipMemcpyAsync(mom + a * slots, d.mom[a], slots * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (w) PIC_HIP(hipMemcpyAsync(w, d.w, slots * 4, hipMemcpyDeviceToHost, c->stream));
+    if (lc) PIC_HIP(hipMemcpyAsync(lc, d.cell, slots * 2, hipMemcpyDeviceToHost, c->stream));
+    if (mask) PIC_HIP(hipMemcpyAsync(mask, d.mask, slots, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int64_t pic_particle_count(pic_ctx* c, int32_t s) {
+    if (checkSpecies(c, s) != PIC_OK) return -1;
+    unsigned long long total = 0;
+    if (pic::launchCountSum(c->sp[s].dev.count, c->nsc, c->countOut, c->stream) != hipSuccess) return -1;
+    if (hipMemcpyAsync(&total, c->countOut, sizeof(total), hipMemcpyDeviceToHost, c->stream) != hipSuccess) return -1;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+    return static_cast<int64_t>(total);
+}
+
+int pic_read_particles(pic_ctx* c, int32_t s, int64_t capacity, int64_t* n_out, int32_t* cell, float* pos,
+                       float* mom, float* w) {
+    int r = checkSpecies(c, s);
+    if (r != PIC_OK) return r;
+    SpeciesMem& m = c->sp[s];
+    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    std::vector<uint32_t> counts(c->nsc);
+    std::vector<float> hp(3 * slots), hm(3 * slots), hw(slots);
+    std::vector<uint16_t> hc(slots);
+    r = pic_read_frames(c, s, counts.data(), hp.data(), hm.data(), hw.data(), hc.data(), nullptr);
+    if (r != PIC_OK) return r;
+    int64_t total = 0;
+    for (uint32_t v : counts) total += v;
+    if (n_out) *n_out = total;
+    if (total > capacity && (cell || pos || mom || w)) return fail(PIC_E_INVALID, "output capacity too small");
+    int64_t o = 0;
+    for (int sc = 0; sc < c->nsc; ++sc) {
+        int const sx = sc % c->g.nscx, sy = (sc / c->g.nscx) % c->g.nscy, sz = sc / (c->g.nscx * c->g.nscy);
+        for (uint32_t i = 0; i < counts[sc]; ++i, ++o) {
+            size_t const g = static_cast<size_t>(sc) * m.cap + i;
+            int const lc = hc[g];
+            if (cell) {
+                cell[3 * o] = sx * pic::kSCX + lc % pic::kSCX;
+                cell[3 * o + 1] = sy * pic::kSCY + (lc / pic::kSCX) % pic::kSCY;
+                cell[3 * o + 2] = sz * pic::kSCZ + lc / (pic::kSCX * pic::kSCY);
+            }
+            for (int a = 0; a < 3; ++a) {
+                if (pos) pos[3 * o + a] = hp[a * slots + g];
+                if (mom) mom[3 * o + a] = hm[a * slots + g];
+            }
+            if (w) w[o] = hw[g];
+        }
+    }
+    return PIC_OK;
+}
+
+int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
+    int r = checkSpecies(c, s);
+    if (r != PIC_OK) return r;
+    if (!desc || desc->ppc <= 0) return fail(PIC_E_INVALID, "invalid init description");
+    SpeciesMem& m = c->sp[s];
+    if (desc->ppc * pic::kSCV > m.cap) return fail(PIC_E_CAPACITY, "ppc*256 exceeds the supercell frame capacity");
+    double const vol = c->cfg.cell_si[0] * c->cfg.cell_si[1] * c->cfg.cell_si[2];
+    pic::InitArgs a{};
+    a.g = c->g;
+    a.sp = m.dev;
+    a.seed = desc->seed;
+    a.species = s;
+    a.ppc = desc->ppc;
+    a.drift_band = desc->drift_band;
+    a.weighting = static_cast<float>(desc->density_si * vol / static_cast<double>(desc->ppc));
+    a.mass = m.mass * a.weighting;
+    for (int d = 0; d < 3; ++d) {
+        a.u_drift[d] = desc->u_drift[d];
+        a.u_th[d] = desc->u_th[d];
+    }
+    PIC_HIP(pic::launchInit(a, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    m.wmax = std::fabs(static_cast<double>(a.weighting));
+    updateJexp(c);
+    return PIC_OK;
+}
+
+int pic_load_field(pic_ctx* c, int32_t field, const float* data) {
+    if (!c || !data) return fail(PIC_E_INVALID, "null argument");
+    float** f = field == PIC_FIELD_E ? c->E : (field == PIC_FIELD_B ? c->B : (field == PIC_FIELD_J ? c->J : nullptr));
+    if (!f) return fail(PIC_E_INVALID, "unknown field");
+    for (int a = 0; a < 3; ++a)
+        PIC_HIP(hipMemcpyAsync(f[a], data + a * c->ncells, c->ncells * 4, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_read_field(pic_ctx* c, int32_t field, float* data) {
+    if (!c || !data) return fail(PIC_E_INVALID, "null argument");
+    float** f = field == PIC_FIELD_E ? c->E : (field == PIC_FIELD_B ? c->B : (field == PIC_FIELD_J ? c->J : nullptr));
+    if (!f) return fail(PIC_E_INVALID, "unknown field");
+    for (int a = 0; a < 3; ++a)
+        PIC_HIP(hipMemcpyAsync(data + a * c->ncells, f[a], c->ncells * 4, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_add_field_noise(pic_ctx* c, int32_t field, uint64_t seed, float amp) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    float** f = field == PIC_FIELD_E ? c->E : (field == PIC_FIELD_B ? c->B : (field == PIC_FIELD_J ? c->J : nullptr));
+    if (!f) return fail(PIC_E_INVALID, "unknown field");
+    for (int a = 0; a < 3; ++a) PIC_HIP(pic::launchNoise(f[a], c->ncells, seed, a, amp, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_current_reset(pic_ctx* c) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    TimerScope ts(c, PIC_TIMER_FIELD);
+    for (int a = 0; a < 3; ++a) PIC_HIP(pic::launchFill(c->J[a], c->ncells, 0.0f, c->stream));
+    return PIC_OK;
+}
+
+int pic_push(pic_ctx* c, int32_t species, int32_t fuse) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (species >= static_cast<int>(c->sp.size())) return fail(PIC_E_INVALID, "species index out of range");
+    if (species >= 0 && c->sp[species].desc.pusher == PIC_PUSHER_NONE)
+        return fail(PIC_E_INVALID, "species is immobile");
+    return runPush(c, species, fuse ? 1 : 0);
+}
+
+int pic_deposit(pic_ctx* c, int32_t species) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (species >= static_cast<int>(c->sp.size())) return fail(PIC_E_INVALID, "species index out of range");
+    return runPush(c, species, 2);
+}
+
+int pic_shift(pic_ctx* c, int32_t species) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (species >= static_cast<int>(c->sp.size())) return fail(PIC_E_INVALID, "species index out of range");
+    for (int s = 0; s < static_cast<int>(c->sp.size()); ++s) {
+        if (species >= 0 && s != species) continue;
+        if (c->sp[s].desc.pusher == PIC_PUSHER_NONE) continue;
+        TimerScope ts(c, PIC_TIMER_SHIFT);
+        PIC_HIP(pic::launchShift(c->g, c->sp[s].dev, c->errFlags, c->stream));
+    }
+    return PIC_OK;
+}
+
+int pic_fdtd_before_current(pic_ctx* c) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    pic::FieldArgs a = makeFieldArgs(c);
+    TimerScope ts(c, PIC_TIMER_FIELD);
+    PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
+    PIC_HIP(pic::launchEFull(a, c->stream));
+    return PIC_OK;
+}
+
+int pic_add_current(pic_ctx* c) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    pic::FieldArgs a = makeFieldArgs(c);
+    TimerScope ts(c, PIC_TIMER_FIELD);
+    PIC_HIP(pic::launchAddCurrent(a, c->stream));
+    return PIC_OK;
+}
+
+int pic_fdtd_after_current(pic_ctx* c) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    pic::FieldArgs a = makeFieldArgs(c);
+    TimerScope ts(c, PIC_TIMER_FIELD);
+    PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
+    return PIC_OK;
+}
+
+int pic_step(pic_ctx* c, int32_t nsteps) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
+    for (int32_t i = 0; i < nsteps; ++i) {
+        int r;
+        if ((r = pic_current_reset(c)) != PIC_OK) return r;
+        if ((r = pic_push(c, -1, 1)) != PIC_OK) return r;
+        if ((r = pic_shift(c, -1)) != PIC_OK) return r;
+        if ((r = pic_fdtd_before_current(c)) != PIC_OK) return r;
+        if ((r = pic_add_current(c)) != PIC_OK) return r;
+        if ((r = pic_fdtd_after_current(c)) != PIC_OK) return r;
+    }
+    return checkErrors(c);
+}
+
+int pic_field_energy(pic_ctx* c, double out[2]) {
+    if (!c || !out) return fail(PIC_E_INVALID, "null argument");
+    pic::FieldArgs a = makeFieldArgs(c);
+    PIC_HIP(pic::launchEnergy(a, c->energyPartials, c->energyBlocks, c->stream));
+    std::vector<double> h(2 * c->energyBlocks);
+    PIC_HIP(hipMemcpyAsync(h.data(), c->energyPartials, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    double e = 0.0, b = 0.0;
+    for (int i = 0; i < c->energyBlocks; ++i) {
+        e += h[2 * i];
+        b += h[2 * i + 1];
+    }
+    out[0] = e;
+    out[1] = b;
+    return PIC_OK;
+}
+
+int pic_error_flags(pic_ctx* c, uint32_t* flags) {
+    if (!c || !flags) return fail(PIC_E_INVALID, "null argument");
+    PIC_HIP(hipMemcpyAsync(flags, c->errFlags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemsetAsync(c->errFlags, 0, sizeof(uint32_t), c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_timing_enable(pic_ctx* c, int32_t enable) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    int const r = resolveTimers(c);
+    if (r != PIC_OK) return r;
+    c->timing = enable != 0;
+    for (int t = 0; t < PIC_TIMER_COUNT; ++t) {
+        c->totalMs[t] = 0.0;
+        c->launches[t] = 0;
+    }
+    return PIC_OK;
+}
+
+int pic_kernel_timer(pic_ctx* c, int32_t timer, double* total_ms, int64_t* launches) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (timer < 0 || timer >= PIC_TIMER_COUNT) return fail(PIC_E_INVALID, "unknown timer");
+    int const r = resolveTimers(c);
+    if (r != PIC_OK) return r;
+    if (total_ms) *total_ms = c->totalMs[timer];
+    if (launches) *launches = c->launches[timer];
+    return PIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,466 @@
+// Device-side physics of the PIC cycle for gfx950.
+//
+// Every function here reproduces the floating-point operation order of the
+// reference functor it replaces (cited per function), so that results are
+// bit-identical to the reference's CPU path when compiled with
+// -ffp-contract=off and IEEE division/sqrt (the hipcc defaults we keep).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pic {
+
+// Supercell shape (param/memory.param:51 SuperCellSize = 8,8,4) and frame size.
+constexpr int SCX = 8;
+constexpr int SCY = 8;
+constexpr int SCZ = 4;
+constexpr int SCV = SCX * SCY * SCZ;
+constexpr int FRAME = 256;
+
+struct Vec3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ float comp(const Vec3& v, int d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
+
+// l2norm2: x*x, then += y*y, += z*z (pmacc/math/vector/Vector.tpp:99-110)
+__device__ __forceinline__ float l2norm2(Vec3 v) {
+    float t = v.x * v.x;
+    t += v.y * v.y;
+    t += v.z * v.z;
+    return t;
+}
+
+// cross product (pmacc/math/vector/Vector.tpp:71-76)
+__device__ __forceinline__ Vec3 cross(Vec3 a, Vec3 b) {
+    return Vec3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// ---------------------------------------------------------------------------
+// Shapes: on-support assignment arrays used by the field gather
+// (particles/shapes/{CIC,TSC,PQS}.hpp, struct detail::*::shapeArray and
+// ChargeAssignmentOnSupport::shapeArray).
+// ---------------------------------------------------------------------------
+template <int SUPP>
+struct Shape;
+
+template <>
+struct Shape<2> {  // CIC.hpp:38-161
+    static constexpr int begin = 0;      // ChargeAssignment::begin
+    static constexpr int end = 2;        // ChargeAssignment::end
+    // x in [0,1): {1-x, x}
+    __device__ __forceinline__ static void onSupport(float x, float* s) {
+        s[0] = 1.0f - x;
+        s[1] = x;
+    }
+    // ChargeAssignment::operator() (CIC.hpp:89-107)
+    __device__ __forceinline__ static float assign(float x) {
+        float const abs_x = fabsf(x);
+        bool const below_1 = abs_x < 1.0f;
+        float const onSupport = 1.0f - abs_x;
+        float result = 0.0f;
+        if (below_1) result = onSupport;
+        return result;
+    }
+};
+
+template <>
+struct Shape<3> {  // TSC.hpp:39-201
+    static constexpr int begin = -1;
+    static constexpr int end = 2;
+    __device__ __forceinline__ static float ff1(float x) {
+        float const square_x = x * x;
+        return 0.75f - square_x;
+    }
+    __device__ __forceinline__ static float ff2(float x) {
+        float const tmp = 1.5f - x;  // 3.0_X / 2.0_X
+        float const square_tmp = tmp * tmp;
+        return 0.5f * square_tmp;
+    }
+    // x in [-0.5,0.5): grid points -1,0,1
+    __device__ __forceinline__ static void onSupport(float x, float* s) {
+        s[0] = ff2(fabsf(-1.0f - x));
+        s[1] = ff1(fabsf(x));
+        s[2] = 1.0f - (s[0] + s[1]);
+    }
+    __device__ __forceinline__ static float assign(float x) {
+        float const abs_x = fabsf(x);
+        bool const below_05 = abs_x < 0.5f;
+        bool const below_1_5 = abs_x < 1.5f;
+        float const rad1 = ff1(abs_x);
+        float const rad2 = ff2(abs_x);
+        float result = 0.0f;
+        if (below_05)
+            result = rad1;
+        else if (below_1_5)
+            result = rad2;
+        return result;
+    }
+};
+
+template <>
+struct Shape<4> {  // PQS.hpp:38-205
+    static constexpr int begin = -1;
+    static constexpr int end = 3;
+    __device__ __forceinline__ static float ff1(float x) {
+        float const square_x = x * x;
+        float const triple_x = square_x * x;
+        return (1.0f / 6.0f) * (4.0f - 6.0f * square_x + 3.0f * triple_x);
+    }
+    __device__ __forceinline__ static float ff2(float x) {
+        float const tmp = 2.0f - x;
+        float const triple_tmp = tmp * tmp * tmp;
+        return (1.0f / 6.0f) * triple_tmp;
+    }
+    // x in [0,1): grid points -1,0,1,2
+    __device__ __forceinline__ static void onSupport(float x, float* s) {
+        s[0] = ff2(fabsf(-1.0f - x));
+        s[1] = ff1(x);
+        s[3] = ff2(2.0f - x);
+        s[2] = 1.0f - (s[0] + s[1] + s[3]);
+    }
+    __device__ __forceinline__ static float assign(float x) {
+        float const abs_x = fabsf(x);
+        bool const below_1 = abs_x < 1.0f;
+        bool const below_2 = abs_x < 2.0f;
+        float const rad1 = ff1(abs_x);
+        float const rad2 = ff2(abs_x);
+        float result = 0.0f;
+        if (below_1)
+            result = rad1;
+        else if (below_2)
+            result = rad2;
+        return result;
+    }
+};
+
+// ChargeAssignment::shapeArray(pos, isOutOfRange): SUPP+1 values for grid
+// points begin..end (CIC.hpp:115-128, TSC.hpp:141-155, PQS.hpp:143-157).
+template <int SUPP>
+__device__ __forceinline__ void cachedShape(float pos, bool outOfRange, float* s) {
+    float const x = outOfRange ? pos - 1.0f : pos;
+    float v[SUPP + 1];
+    Shape<SUPP>::onSupport(x, v);
+    v[SUPP] = 0.0f;
+#pragma unroll
+    for (int i = SUPP; i >= 1; --i) s[i] = outOfRange ? v[i - 1] : v[i];
+    s[0] = outOfRange ? 0.0f : v[0];
+}
+
+// ---------------------------------------------------------------------------
+// Field gather: FieldToParticleInterpolation<Shape, AssignedTrilinearInterpolation>
+// (algorithms/FieldToParticleInterpolation.hpp:43-131,
+// ShiftCoordinateSystem.hpp:36-80, AssignedTrilinearInterpolation.hpp:53-86).
+// `tile` is one field component staged as [z][y][x] with row pitch TX and
+// plane pitch TX*TY; (cx,cy,cz) is the particle's cell in tile coordinates.
+// ---------------------------------------------------------------------------
+template <int SUPP, int TX, int TY>
+__device__ __forceinline__ float gather1(const float* __restrict__ tile, int cx, int cy, int cz, Vec3 pos,
+                                         float fx, float fy, float fz) {
+    constexpr bool isEven = (SUPP % 2) == 0;
+    constexpr int begin = -SUPP / 2 + (SUPP + 1) % 2;
+    float p[3] = {pos.x, pos.y, pos.z};
+    float const f[3] = {fx, fy, fz};
+    int ishift[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float const v_pos = p[d] - f[d] - 0.5f;
+        int s;
+        if (isEven)
+            s = v_pos >= -0.5f ? 0 : -1;
+        else
+            s = v_pos >= 0.0f ? 1 : 0;
+        ishift[d] = s;
+        p[d] = v_pos - (float)s + 0.5f;
+    }
+    float sx[SUPP], sy[SUPP], sz[SUPP];
+    Shape<SUPP>::onSupport(p[0], sx);
+    Shape<SUPP>::onSupport(p[1], sy);
+    Shape<SUPP>::onSupport(p[2], sz);
+    int const bx = cx + ishift[0] + begin;
+    int const by = cy + ishift[1] + begin;
+    int const bz = cz + ishift[2] + begin;
+    float result_z = 0.0f;
+#pragma unroll
+    for (int z = 0; z < SUPP; ++z) {
+        float result_y = 0.0f;
+#pragma unroll
+        for (int y = 0; y < SUPP; ++y) {
+            float result_x = 0.0f;
+            const float* row = tile + ((bz + z) * TY + (by + y)) * TX + bx;
+#pragma unroll
+            for (int x = 0; x < SUPP; ++x) result_x += row[x] * sx[x];
+            result_y += result_x * sy[y];
+        }
+        result_z += result_y * sz[z];
+    }
+    return result_z;
+}
+
+// E components sit at (1/2,0,0),(0,1/2,0),(0,0,1/2); B at (0,1/2,1/2),
+// (1/2,0,1/2),(1/2,1/2,0) (fields/YeeCell.hpp:69-123).
+template <int SUPP, int TX, int TY, int TV>
+__device__ __forceinline__ Vec3 gatherE(const float* __restrict__ t, int cx, int cy, int cz, Vec3 pos) {
+    return Vec3{gather1<SUPP, TX, TY>(t, cx, cy, cz, pos, 0.5f, 0.0f, 0.0f),
+                gather1<SUPP, TX, TY>(t + TV, cx, cy, cz, pos, 0.0f, 0.5f, 0.0f),
+                gather1<SUPP, TX, TY>(t + 2 * TV, cx, cy, cz, pos, 0.0f, 0.0f, 0.5f)};
+}
+template <int SUPP, int TX, int TY, int TV>
+__device__ __forceinline__ Vec3 gatherB(const float* __restrict__ t, int cx, int cy, int cz, Vec3 pos) {
+    return Vec3{gather1<SUPP, TX, TY>(t, cx, cy, cz, pos, 0.0f, 0.5f, 0.5f),
+                gather1<SUPP, TX, TY>(t + TV, cx, cy, cz, pos, 0.5f, 0.0f, 0.5f),
+                gather1<SUPP, TX, TY>(t + 2 * TV, cx, cy, cz, pos, 0.5f, 0.5f, 0.0f)};
+}
+
+// ---------------------------------------------------------------------------
+// Gamma and Velocity (algorithms/Gamma.hpp:28-38, Velocity.hpp:28-39).
+// Velocity uses 1/sqrt like the reference CPU build
+// (thirdParty/alpaka/include/alpaka/math/Traits.hpp:729-734), not rsqrtf.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float gammaOf(Vec3 mom, float mass, float c2) {
+    float const fMom2 = l2norm2(mom);
+    float const m2_c2_reci = 1.0f / (mass * mass * c2);
+    return sqrtf(1.0f + fMom2 * m2_c2_reci);
+}
+
+__device__ __forceinline__ Vec3 velocityOf(Vec3 mom, float mass, float rc2) {
+    float const m0_2 = mass * mass;
+    float const fMom2 = l2norm2(mom);
+    float const t = 1.0f / sqrtf(m0_2 + fMom2 * rc2);
+    return Vec3{t * mom.x, t * mom.y, t * mom.z};
+}
+
+struct PushConsts {
+    float cell[3];
+    float dt;
+    float c2;         // speedOfLight^2
+    float rc2;        // mue0*eps0
+    float c;          // speedOfLight
+};
+
+// Boris (particles/pusher/particlePusherBoris.hpp:41-90). Returns new momentum
+// and velocity; pos is advanced in place.
+__device__ __forceinline__ void pushBoris(Vec3 e, Vec3 b, Vec3& mom, Vec3& vel, float mass, float charge,
+                                          const PushConsts& k) {
+    float const QoM = charge / mass;
+    float const deltaT = k.dt;
+    float const hq = 0.5f * charge;
+    Vec3 const mom_minus{mom.x + hq * e.x * deltaT, mom.y + hq * e.y * deltaT, mom.z + hq * e.z * deltaT};
+    float const gamma_reci = 1.0f / gammaOf(mom_minus, mass, k.c2);
+    float const hqm = 0.5f * QoM;
+    Vec3 const t{hqm * b.x * gamma_reci * deltaT, hqm * b.y * gamma_reci * deltaT, hqm * b.z * gamma_reci * deltaT};
+    float const sfac = 1.0f / (1.0f + l2norm2(t));
+    Vec3 const s{2.0f * t.x * sfac, 2.0f * t.y * sfac, 2.0f * t.z * sfac};
+    Vec3 const cp = cross(mom_minus, t);
+    Vec3 const mom_prime{mom_minus.x + cp.x, mom_minus.y + cp.y, mom_minus.z + cp.z};
+    Vec3 const cs = cross(mom_prime, s);
+    Vec3 const mom_plus{mom_minus.x + cs.x, mom_minus.y + cs.y, mom_minus.z + cs.z};
+    mom = Vec3{mom_plus.x + hq * e.x * deltaT, mom_plus.y + hq * e.y * deltaT, mom_plus.z + hq * e.z * deltaT};
+    vel = velocityOf(mom, mass, k.rc2);
+}
+
+// Vay (particles/pusher/particlePusherVay.hpp:42-111); the gamma root section
+// in FP64 as param/pusher.param:62 selects (sqrt_Vay = precision64Bit).
+__device__ __forceinline__ void pushVay(Vec3 e, Vec3 b, Vec3& mom, Vec3& vel, float mass, float charge,
+                                        const PushConsts& k) {
+    float const deltaT = k.dt;
+    float const factor = (float)(0.5 * (double)charge * (double)deltaT);
+    Vec3 const v_mh = velocityOf(mom, mass, k.rc2);
+    Vec3 const cvb = cross(v_mh, b);
+    Vec3 const mom0{mom.x + factor * (e.x + cvb.x), mom.y + factor * (e.y + cvb.y), mom.z + factor * (e.z + cvb.z)};
+    Vec3 const mp{mom0.x + factor * e.x, mom0.y + factor * e.y, mom0.z + factor * e.z};
+    float const gamma_prime = gammaOf(mp, mass, k.c2);
+    float const fm = factor / mass;
+    double const tx = (double)(fm * b.x), ty = (double)(fm * b.y), tz = (double)(fm * b.z);
+    double dotp = (double)mp.x * tx;
+    dotp += (double)mp.y * ty;
+    dotp += (double)mp.z * tz;
+    double const u_star = dotp / (double)(k.c * mass);
+    double tau2 = tx * tx;
+    tau2 += ty * ty;
+    tau2 += tz * tz;
+    double const sigma = (double)(gamma_prime * gamma_prime) - tau2;
+    double const gamma_ph = sqrt(0.5 * (sigma + sqrt(sigma * sigma + 4.0 * (tau2 + u_star * u_star))));
+    double const rg = 1.0 / gamma_ph;
+    Vec3 const t{(float)(tx * rg), (float)(ty * rg), (float)(tz * rg)};
+    float const s = 1.0f / (1.0f + l2norm2(t));
+    float dpt = mp.x * t.x;
+    dpt += mp.y * t.y;
+    dpt += mp.z * t.z;
+    Vec3 const cpt = cross(mp, t);
+    mom = Vec3{s * (mp.x + dpt * t.x + cpt.x), s * (mp.y + dpt * t.y + cpt.y), s * (mp.z + dpt * t.z + cpt.z)};
+    vel = velocityOf(mom, mass, k.rc2);
+}
+
+// ---------------------------------------------------------------------------
+// moveParticle (particles/MoveParticle.hpp:47-160). pos in [-1,2) per axis on
+// entry, [0,1) on exit; dir receives the cell step per axis; returns the new
+// local cell index (wrapped into the destination supercell) and multiMask.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void moveParticle(float* pos, int localIdx, int* dir, int& newIdx, int& newMask) {
+    // The reference marks `shift` volatile on HIP to keep (p-0.5)+0.5 from
+    // being folded; the asm barrier below has the same effect.
+    float shift = 0.5f;
+    asm volatile("" : "+v"(shift));
+    const int sc[3] = {SCX, SCY, SCZ};
+    bool moved = false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float p = pos[i] - shift;
+        float moveDir = 0.0f;
+        if (p < -0.5f) moveDir = -1.0f;
+        if (p >= 0.5f) moveDir = 1.0f;
+        p -= moveDir;
+        pos[i] = p + shift;
+        dir[i] = (int)moveDir;
+        moved |= dir[i] != 0;
+    }
+    newIdx = localIdx;
+    newMask = 1;
+    if (moved) {
+        int lc[3] = {localIdx % SCX, (localIdx / SCX) % SCY, localIdx / (SCX * SCY)};
+        int ex[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            lc[i] += dir[i];
+            ex[i] = (unsigned)lc[i] >= (unsigned)sc[i] ? dir[i] : 0;
+            lc[i] -= ex[i] * sc[i];
+        }
+        newIdx = lc[0] + SCX * (lc[1] + SCY * lc[2]);
+        int exchangeType = 1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            newMask += (ex[i] == -1 ? 2 : ex[i]) * exchangeType;
+            exchangeType *= 3;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Esirkepov<Shape> 3-D current deposition
+// (fields/currentDeposition/Esirkepov/Esirkepov.hpp:61-242, relayPoint.hpp:47-64,
+// bitPacking.hpp:37-90, TrajectoryAssignmentShapeFunction.hpp, Line.hpp,
+// PermutatedFieldValueAccess.hpp). Shape strategy (Strategy.def detail::ShapeStrategy):
+// outer loop uses the exact ("Jit") shape, middle/inner loops the cached arrays.
+// The per-cell value added is the FP32 prefix sum `accumulated_J` of the
+// reference; it is accumulated here as an int64 fixed-point number so the sum
+// over particles does not depend on their order.
+// ---------------------------------------------------------------------------
+struct DepositConsts {
+    float cell[3];
+    float dt;
+    float inv_vdt;  // 1 / float(V*dt)
+    float jscale;   // 2^e
+};
+
+template <int SUPP>
+struct Esirkepov {
+    static constexpr int begin = Shape<SUPP>::begin;
+    static constexpr int end = begin + SUPP;
+    static constexpr int lowerMargin = SUPP / 2 + 1 - (SUPP + 1) % 2;  // Esirkepov.hpp:42
+    static constexpr int upperMargin = (SUPP + 1) / 2 + 1;             // Esirkepov.hpp:43
+
+    // One rotated 1-D pass (cptCurrent1D). p0/p1 are the rotated start/end
+    // positions, st the rotated status bits; addr(i,j,k) maps the rotated
+    // loop indices to the J tile element of this component.
+    template <typename Addr>
+    __device__ __forceinline__ static void cpt1D(unsigned long long* __restrict__ jt, const float* p0, const float* p1,
+                                                 const int* st, float csd, float jscale, Addr addr) {
+        if (p0[2] == p1[2]) return;
+        float s0j[SUPP + 1], s1j[SUPP + 1], s0k[SUPP + 1], s1k[SUPP + 1];
+        cachedShape<SUPP>(p0[1], !(st[1] & 2), s0j);
+        cachedShape<SUPP>(p1[1], !(st[1] & 4), s1j);
+        cachedShape<SUPP>(p0[2], !(st[2] & 2), s0k);
+        cachedShape<SUPP>(p1[2], !(st[2] & 4), s1k);
+        int const leaveCellI = st[0] & 1;
+        int const leaveCellJ = st[1] & 1;
+        int const leaveCellK = st[2] & 1;
+#pragma unroll
+        for (int i = begin; i < end + 1; ++i) {
+            if (i < end + leaveCellI) {
+                float const s0i = Shape<SUPP>::assign((float)i - p0[0]);
+                float const dsi = Shape<SUPP>::assign((float)i - p1[0]) - s0i;
+#pragma unroll
+                for (int j = begin; j < end + 1; ++j) {
+                    if (j < end + leaveCellJ) {
+                        float const sj0 = s0j[j - begin];
+                        float const dsj = s1j[j - begin] - sj0;
+                        float const tmp =
+                            -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
+                        float accumulated_J = 0.0f;
+#pragma unroll
+                        for (int k = begin; k < end; ++k) {
+                            if (k < end + leaveCellK - 1) {
+                                float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
+                                accumulated_J += W;
+                                long long const q = (long long)rintf(accumulated_J * jscale);
+                                if (q != 0)
+                                    __hip_atomic_fetch_add(jt + addr(i, j, k), (unsigned long long)q,
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // Deposit one particle. (cx,cy,cz) is the particle's cell in J-tile
+    // coordinates, pos its in-cell position after the move, vel its velocity.
+    // Tile layout: component-major planes of TV elements, [z][y][x] with pitches TX, TX*TY.
+    template <int TX, int TY, int TV>
+    __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
+                                                   Vec3 vel, float charge, const DepositConsts& k) {
+        constexpr bool isEven = (SUPP % 2) == 0;
+        float const np[3] = {pos.x, pos.y, pos.z};
+        float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
+        float p0[3], p1[3];
+        int gs[3], st[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            p0[d] = np[d] - dp[d];
+            p1[d] = np[d];
+            int iS, iE;
+            if (isEven) {
+                iS = (int)floorf(p0[d]);
+                iE = (int)floorf(p1[d]);
+            } else {
+                iS = (int)floorf(p0[d] + 0.5f);
+                iE = (int)floorf(p1[d] + 0.5f);
+            }
+            gs[d] = iS < iE ? iS : iE;
+            st[d] = (gs[d] == iS ? 2 : 0) | (gs[d] == iE ? 4 : 0) | (iS != iE ? 1 : 0);
+            p0[d] -= (float)gs[d];
+            p1[d] -= (float)gs[d];
+        }
+        int const bx = cx + gs[0], by = cy + gs[1], bz = cz + gs[2];
+        // J_x: rotation (1,2,0): i->y, j->z, k->x
+        {
+            float const q0[3] = {p0[1], p0[2], p0[0]};
+            float const q1[3] = {p1[1], p1[2], p1[0]};
+            int const s3[3] = {st[1], st[2], st[0]};
+            float const csd = charge * k.inv_vdt * k.cell[0];
+            cpt1D(jt, q0, q1, s3, csd, k.jscale,
+                  [&](int i, int j, int kk) { return ((bz + j) * TY + (by + i)) * TX + (bx + kk); });
+        }
+        // J_y: rotation (2,0,1): i->z, j->x, k->y
+        {
+            float const q0[3] = {p0[2], p0[0], p0[1]};
+            float const q1[3] = {p1[2], p1[0], p1[1]};
+            int const s3[3] = {st[2], st[0], st[1]};
+            float const csd = charge * k.inv_vdt * k.cell[1];
+            cpt1D(jt + TV, q0, q1, s3, csd, k.jscale,
+                  [&](int i, int j, int kk) { return ((bz + i) * TY + (by + kk)) * TX + (bx + j); });
+        }
+        // J_z: identity: i->x, j->y, k->z
+        {
+            int const s3[3] = {st[0], st[1], st[2]};
+            float const csd = charge * k.inv_vdt * k.cell[2];
+            cpt1D(jt + 2 * TV, p0, p1, s3, csd, k.jscale,
+                  [&](int i, int j, int kk) { return ((bz + kk) * TY + (by + j)) * TX + (bx + i); });
+        }
+    }
+};
+
+}  // namespace pic

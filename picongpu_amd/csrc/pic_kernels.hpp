@@ -1,0 +1,90 @@
+// Kernel argument structures and launcher declarations shared by the host
+// runtime (pic_api.cpp) and the HIP kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/picongpu_amd.h"
+
+namespace pic {
+
+constexpr int kSCX = 8, kSCY = 8, kSCZ = 4, kSCV = 256;
+constexpr int kNumDirs = 27;      // exchange codes 0..26 (0 = stays)
+constexpr int kOffStride = 28;    // per-supercell offset table entries
+
+struct SpeciesDev {
+    float* pos[3];
+    float* mom[3];
+    float* w;
+    uint16_t* cell;
+    uint8_t* mask;
+    uint32_t* count;     // particles per supercell
+    uint32_t* occ;       // occupied slots per supercell during a shift
+    int32_t* outIdx;     // per supercell: leaver slots grouped by exchange code
+    int32_t* scratch;    // per supercell: gap/source lists of the compaction
+    uint32_t* outOff;    // per supercell: 28 offsets into outIdx (code 1..26, [27]=total)
+    int32_t cap;         // slots per supercell
+    float mass;          // frame::getMass()
+    float charge;        // frame::getCharge()
+};
+
+struct GridDev {
+    int nx, ny, nz;
+    int nscx, nscy, nscz;
+};
+
+struct PushArgs {
+    GridDev g;
+    const float* E[3];
+    const float* B[3];
+    float* J[3];
+    float cell[3];
+    float dt, c, c2, rc2;
+    float inv_vdt;
+    float jscale;     // 2^e
+    double jinv;      // 2^-e
+    int nsp;
+    SpeciesDev sp[PIC_MAX_SPECIES];
+    const int32_t* scList;  // supercells of this launch, in launch order
+    uint32_t* errFlags;
+};
+
+struct FieldArgs {
+    GridDev g;
+    float* E[3];
+    float* B[3];
+    float* J[3];
+    float cell[3];
+    float dt, c2, coeff;
+    float lehe_alpha, lehe_delta, lehe_beta1, lehe_beta2;
+    int lehe_dir;
+};
+
+// launchers (return hipError_t of the launch)
+// mode 0: push only, 1: push + fused deposit, 2: deposit only
+hipError_t launchPush(int supp, int pusher, int mode, const PushArgs& a, int nblocks, hipStream_t s);
+hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, hipStream_t s);
+hipError_t launchFill(float* p, size_t n, float v, hipStream_t s);
+hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s);
+hipError_t launchEFull(const FieldArgs& a, hipStream_t s);
+hipError_t launchAddCurrent(const FieldArgs& a, hipStream_t s);
+hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, hipStream_t s);
+hipError_t launchCountSum(const uint32_t* counts, int n, unsigned long long* out, hipStream_t s);
+
+struct InitArgs {
+    GridDev g;
+    SpeciesDev sp;
+    uint64_t seed;
+    int32_t species;
+    int32_t ppc;
+    int32_t drift_band;
+    float weighting;
+    float mass;          // frame mass * weighting
+    double u_drift[3];
+    double u_th[3];
+};
+hipError_t launchInit(const InitArgs& a, hipStream_t s);
+hipError_t launchNoise(float* f, size_t n, uint64_t seed, int comp, float amp, hipStream_t s);
+
+}  // namespace pic

@@ -1,0 +1,198 @@
+"""Simulation setups in the manner of a PIConGPU setup's .param files.
+
+Each setup fixes what param/simulation.param (DELTA_T_SI, CELL_*_SI),
+param/density.param (BASE_DENSITY_SI), param/particle.param
+(TYPICAL_PARTICLES_PER_CELL), param/species.param (UsedParticlePusher,
+UsedParticleShape), param/speciesDefinition.param (massRatio, chargeRatio) and
+param/fieldSolver.param (Solver) fix in the reference, plus the initial state.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from . import abi
+
+SPEED_OF_LIGHT_SI = 2.99792458e8
+MASS_RATIO_PROTON = 1836.152672  # param/speciesDefinition.param:75 (MassRatioIons)
+
+
+@dataclass
+class SpeciesSetup:
+    name: str
+    mass_ratio: float = 1.0
+    charge_ratio: float = 1.0
+    shape: int = abi.SHAPE_CIC
+    pusher: int = abi.PUSHER_BORIS
+    ppc: int = 8
+    density_si: float = 1e25
+    u_drift: tuple = (0.0, 0.0, 0.0)
+    u_th: tuple = (0.0, 0.0, 0.0)
+    drift_band: int = 0
+    seed: int = 0
+    frames_per_sc: int = 0  # 0: derived from ppc
+
+    def frames(self) -> int:
+        if self.frames_per_sc:
+            return self.frames_per_sc
+        return int(math.ceil(self.ppc * 1.5)) + 2
+
+
+@dataclass
+class Setup:
+    name: str
+    grid: tuple
+    cell_si: tuple
+    dt_si: float
+    density_si: float
+    typical_ppc: int
+    species: list
+    solver: int = abi.SOLVER_YEE
+    lehe_dir: int = 1
+    steps: int = 100
+    b_noise: float = 0.0
+    b_noise_seed: int = 1
+
+    def config(self, device: int = 0, j_scale_exp: int = 0) -> abi.Config:
+        sp = [dict(mass_ratio=s.mass_ratio, charge_ratio=s.charge_ratio, shape=s.shape, pusher=s.pusher,
+                   frames_per_sc=s.frames()) for s in self.species]
+        return abi.make_config(self.grid, self.cell_si, self.dt_si, self.density_si, self.typical_ppc, sp,
+                               solver=self.solver, lehe_dir=self.lehe_dir, j_scale_exp=j_scale_exp, device=device)
+
+    @property
+    def ncells(self) -> int:
+        return int(np.prod(self.grid))
+
+    def mobile_particles(self) -> int:
+        return sum(self.ncells * s.ppc for s in self.species if s.pusher != abi.PUSHER_NONE)
+
+
+def yee_dt(cell_si, courant=0.99) -> float:
+    inv = sum(1.0 / (c * c) for c in cell_si)
+    return courant / (SPEED_OF_LIGHT_SI * math.sqrt(inv))
+
+
+def gamma_beta(beta: float) -> float:
+    return beta / math.sqrt(1.0 - beta * beta)
+
+
+def thermal(grid=(64, 64, 64), ppc=8, shape=abi.SHAPE_CIC, pusher=abi.PUSHER_BORIS, steps=100, seed=0) -> Setup:
+    """Plain thermal plasma: electrons over an immobile neutralising
+    background, 1e25 m^-3, u_th = 0.01 c, dx = 1e-7 m, dt = 0.99 Yee limit."""
+    cell = (1e-7, 1e-7, 1e-7)
+    e = SpeciesSetup("e", shape=shape, pusher=pusher, ppc=ppc, density_si=1e25, u_th=(0.01, 0.01, 0.01), seed=seed)
+    return Setup("thermal", tuple(grid), cell, yee_dt(cell), 1e25, ppc, [e], steps=steps)
+
+
+def kelvin_helmholtz(grid=(256, 256, 256), ppc=16, shape=abi.SHAPE_TSC, steps=500, seed=0) -> Setup:
+    """Two-species shear flow: electrons and protons at 1e25 m^-3, the middle
+    half of y drifting at +0.2 c in x and the outer halves at -0.2 c, u_th =
+    0.005 c, B seeded with 1e-6 noise. Cell and time step of the reference's
+    KHI test (share/picongpu/tests/KHI_growthRate/include/picongpu/param/simulation.param:52,59)."""
+    cell = (8.037861e-8, 8.037861e-8, 8.037861e-8)
+    dt = 1.5394e-16
+    ud = (gamma_beta(0.2), 0.0, 0.0)
+    uth = (0.005, 0.005, 0.005)
+    e = SpeciesSetup("e", shape=shape, ppc=ppc, density_si=1e25, u_drift=ud, u_th=uth, drift_band=1, seed=seed)
+    p = SpeciesSetup("p", mass_ratio=MASS_RATIO_PROTON, charge_ratio=-1.0, shape=shape, ppc=ppc, density_si=1e25,
+                     u_drift=ud, u_th=uth, drift_band=1, seed=seed + 1)
+    return Setup("kelvin_helmholtz", tuple(grid), cell, dt, 1e25, ppc, [e, p], steps=steps, b_noise=1e-6)
+
+
+def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0) -> Setup:
+    """Laser-plasma geometry: PQS shape, Vay pusher, Lehe solver Cherenkov-free
+    along y, dx = dz = 0.1 um, dy = 0.05 um, cold electrons with immobile
+    protons. The density ramp and laser initial field are not part of this
+    setup yet; the electrons are uniform at 3e24 m^-3."""
+    cell = (1e-7, 5e-8, 1e-7)
+    dt = 0.995 * cell[1] / SPEED_OF_LIGHT_SI
+    e = SpeciesSetup("e", shape=abi.SHAPE_PQS, pusher=abi.PUSHER_VAY, ppc=ppc, density_si=3e24, seed=seed)
+    p = SpeciesSetup("p", mass_ratio=MASS_RATIO_PROTON, charge_ratio=-1.0, shape=abi.SHAPE_PQS,
+                     pusher=abi.PUSHER_NONE, ppc=ppc, density_si=3e24, seed=seed + 1)
+    return Setup("laser_plasma", tuple(grid), cell, dt, 3e24, ppc, [e, p], solver=abi.SOLVER_LEHE, lehe_dir=1,
+                 steps=steps)
+
+
+CONFIGS = {
+    "thermal": thermal,
+    "kelvin_helmholtz": kelvin_helmholtz,
+    "laser_plasma": laser_plasma,
+}
+
+
+# -- initial state --------------------------------------------------------
+def init_device(sim: "abi.Simulation", setup: Setup) -> None:
+    """Initial particles (and B noise) generated on the device."""
+    for i, s in enumerate(setup.species):
+        sim.init_species(i, s.seed, s.ppc, s.density_si, s.u_drift, s.u_th, s.drift_band)
+    if setup.b_noise:
+        sim.add_field_noise(abi.FIELD_B, setup.b_noise_seed, setup.b_noise)
+
+
+def host_particles(setup: Setup, sp: int, seed: int = None):
+    """Flat host-side particle arrays for a species (numpy generator, for small
+    tests): global cell coords, in-cell positions, momenta, weightings."""
+    s = setup.species[sp]
+    k = abi_free_constants(setup)
+    rng = np.random.default_rng(s.seed if seed is None else seed)
+    nx, ny, nz = setup.grid
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    cells = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], axis=1).astype(np.int32)
+    cells = np.repeat(cells, s.ppc, axis=0)
+    n = cells.shape[0]
+    pos = rng.random((n, 3), dtype=np.float32)
+    vol = setup.cell_si[0] * setup.cell_si[1] * setup.cell_si[2]
+    w = np.float32(s.density_si * vol / s.ppc)
+    mass = np.float32(k["base_mass"] * np.float32(s.mass_ratio)) * w
+    u = np.array(s.u_drift, np.float64)[None, :] * np.ones((n, 1))
+    if s.drift_band:
+        band = (cells[:, 1] >= ny // 4) & (cells[:, 1] < 3 * ny // 4)
+        u = u * np.where(band, 1.0, -1.0)[:, None]
+    u = u + rng.standard_normal((n, 3)) * np.array(s.u_th)[None, :]
+    mom = (np.float64(mass) * u).astype(np.float32)
+    weights = np.full(n, w, np.float32)
+    return cells, pos, mom, weights
+
+
+def abi_free_constants(setup: Setup) -> dict:
+    """Base mass/charge in PIC units without loading the library."""
+    vol = setup.cell_si[0] * setup.cell_si[1] * setup.cell_si[2]
+    wtyp = setup.density_si * vol / setup.typical_ppc
+    me = 9.1093837139e-31
+    qe = -1.602176634e-19
+    return {"base_mass": np.float32(me / (me * wtyp)), "base_charge": np.float32(qe / (-1.0 * qe * wtyp)),
+            "wtyp": wtyp}
+
+
+def bin_frames(setup: Setup, sp: int, cells, pos, mom, w, frames_per_sc: int = None):
+    """Bin flat particles into the supercell frame layout, stable in input
+    order (the layout pic_load_particles produces)."""
+    nx, ny, nz = setup.grid
+    nscx, nscy, nscz = nx // 8, ny // 8, nz // 4
+    nsc = nscx * nscy * nscz
+    cap = (frames_per_sc or setup.species[sp].frames()) * 256
+    sc = cells[:, 0] // 8 + nscx * (cells[:, 1] // 8 + nscy * (cells[:, 2] // 4))
+    lc = cells[:, 0] % 8 + 8 * (cells[:, 1] % 8 + 8 * (cells[:, 2] % 4))
+    order = np.argsort(sc, kind="stable")
+    counts = np.bincount(sc, minlength=nsc).astype(np.uint32)
+    if counts.max(initial=0) > cap:
+        raise ValueError("supercell capacity exceeded")
+    starts = np.zeros(nsc, np.int64)
+    starts[1:] = np.cumsum(counts.astype(np.int64))[:-1]
+    rank = np.arange(len(order), dtype=np.int64) - starts[sc[order]]
+    slot = sc[order].astype(np.int64) * cap + rank
+    slots = nsc * cap
+    f = {"counts": counts, "pos": np.zeros((3, slots), np.float32), "mom": np.zeros((3, slots), np.float32),
+         "w": np.zeros(slots, np.float32), "cell": np.zeros(slots, np.uint16), "mask": np.zeros(slots, np.uint8)}
+    f["pos"][:, slot] = pos[order].T
+    f["mom"][:, slot] = mom[order].T
+    f["w"][slot] = w[order]
+    f["cell"][slot] = lc[order]
+    f["mask"][slot] = 1
+    return f
+
+
+def small(setup: Setup, grid) -> Setup:
+    return replace(setup, grid=tuple(grid))

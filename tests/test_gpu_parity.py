@@ -1,0 +1,135 @@
+"""Parity of the HIP library with the CPU oracle on identical seeded inputs.
+
+Bar: bit-exact for particle state (positions, momenta, cell indices,
+multiMask, supercell counts) and for E, B, J, against the oracle's device
+summation order. Every case prints what it runs before the GPU work.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from picongpu_amd import abi, setups
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def small_thermal(shape=abi.SHAPE_CIC, pusher=abi.PUSHER_BORIS, grid=(16, 16, 12), ppc=4, u=0.05):
+    st = setups.thermal(grid=grid, ppc=ppc, shape=shape, pusher=pusher)
+    st.species[0].u_th = (u, u, u)
+    return st
+
+
+def small_two_species(shape=abi.SHAPE_TSC, grid=(16, 16, 12), ppc=3):
+    st = setups.kelvin_helmholtz(grid=grid, ppc=ppc, shape=shape)
+    for s in st.species:
+        s.u_th = (0.02, 0.02, 0.02)
+    return st
+
+
+CASES = [
+    ("cic-boris", lambda: small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS)),
+    ("tsc-boris", lambda: small_thermal(abi.SHAPE_TSC, abi.PUSHER_BORIS)),
+    ("pqs-boris", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_BORIS)),
+    ("cic-vay", lambda: small_thermal(abi.SHAPE_CIC, abi.PUSHER_VAY)),
+    ("pqs-vay", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_VAY)),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+def test_push_fused_bit_exact(name, make):
+    st = make()
+    print(f"case push_fused {name} grid={st.grid}", flush=True)
+    sim, ora, _ = H.make_pair(st)
+    sim.current_reset()
+    sim.push(-1, True)
+    ora.current_reset()
+    ora.push(-1, True)
+    H.assert_frames_equal(sim.read_frames(0), ora.get_frames(0), f"{name} push")
+    H.assert_field_equal(sim.read_field(abi.FIELD_J), ora.get_field(abi.FIELD_J), f"{name} J")
+    sim.close()
+
+
+@pytest.mark.parametrize("name,make", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_deposit_unfused_bit_exact(name, make):
+    st = make()
+    print(f"case deposit {name} grid={st.grid}", flush=True)
+    sim, ora, _ = H.make_pair(st)
+    sim.current_reset()
+    sim.deposit(-1)
+    ora.current_reset()
+    ora.deposit(-1)
+    H.assert_field_equal(sim.read_field(abi.FIELD_J), ora.get_field(abi.FIELD_J), f"{name} J")
+    sim.close()
+
+
+def test_shift_bit_exact():
+    st = small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS, u=0.3)
+    print(f"case shift grid={st.grid}", flush=True)
+    sim, ora, _ = H.make_pair(st)
+    for it in range(3):
+        sim.push(-1, False)
+        sim.shift(-1)
+        ora.push(-1, False)
+        ora.shift(-1)
+        H.assert_frames_equal(sim.read_frames(0), ora.get_frames(0), f"shift step {it}")
+    assert sim.error_flags() == 0
+    sim.close()
+
+
+@pytest.mark.parametrize("solver", [abi.SOLVER_YEE, abi.SOLVER_LEHE], ids=["yee", "lehe"])
+def test_fields_bit_exact(solver):
+    st = small_thermal()
+    st.solver = solver
+    st.lehe_dir = 1
+    if solver == abi.SOLVER_LEHE:
+        st.dt_si = 0.99 * st.cell_si[1] / setups.SPEED_OF_LIGHT_SI / 1.8
+    print(f"case fields solver={solver}", flush=True)
+    sim, ora, _ = H.make_pair(st, e_amp=1.0, b_amp=1.0)
+    J = H.random_fields(st, 5, 0.1, 0.1)[0]
+    sim.load_field(abi.FIELD_J, J)
+    ora.set_field(abi.FIELD_J, J)
+    sim.update_before_current()
+    ora.update_before_current()
+    sim.add_current()
+    ora.add_current()
+    sim.update_after_current()
+    ora.update_after_current()
+    for f in (abi.FIELD_E, abi.FIELD_B):
+        H.assert_field_equal(sim.read_field(f), ora.get_field(f), f"field {f}")
+    sim.close()
+
+
+STEP_CASES = [
+    ("thermal-cic-boris", lambda: small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS)),
+    ("thermal-pqs-vay", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_VAY)),
+    ("kh-tsc-2species", lambda: small_two_species(abi.SHAPE_TSC)),
+]
+
+
+@pytest.mark.parametrize("name,make", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_steps_bit_exact(name, make):
+    st = make()
+    nsteps = 10
+    print(f"case steps {name} grid={st.grid} steps={nsteps}", flush=True)
+    sim, ora, _ = H.make_pair(st, e_amp=1e-3, b_amp=1e-3)
+    sim.step(nsteps)
+    ora.step(nsteps)
+    for sp in range(len(st.species)):
+        H.assert_frames_equal(sim.read_frames(sp), ora.get_frames(sp), f"{name} species {sp}")
+    for f in (abi.FIELD_E, abi.FIELD_B, abi.FIELD_J):
+        H.assert_field_equal(sim.read_field(f), ora.get_field(f), f"{name} field {f}")
+    sim.close()
+
+
+def test_load_read_roundtrip():
+    st = small_two_species()
+    print("case load/read roundtrip", flush=True)
+    sim = abi.Simulation(st.config())
+    cells, pos, mom, w = setups.host_particles(st, 0)
+    sim.load_particles(0, cells, pos, mom, w)
+    f = setups.bin_frames(st, 0, cells, pos, mom, w)
+    H.assert_frames_equal(sim.read_frames(0), {k: v for k, v in f.items()}, "load")
+    c2, p2, m2, w2 = sim.read_particles(0)
+    assert c2.shape[0] == cells.shape[0]
+    sim.close()
