@@ -580,6 +580,13 @@ V3<T> pushOne(State<T>& st, Species<T>& S, size_t g, int ox, int oy, int oz, int
     return vel;
 }
 
+// current value -> int64 fixed point: round-to-nearest-even of v * 2^e,
+// computed exactly in FP64
+template <typename T>
+int64_t toFixed(T v, double jscale) {
+    return static_cast<int64_t>(std::nearbyint(static_cast<double>(v) * jscale));
+}
+
 // fixed-point tile of one supercell (order 0)
 struct Tile {
     int lo, up, tx, ty, tz;
@@ -666,7 +673,7 @@ void push(State<T>& st, int species, bool fused) {
                 Tile tile;
                 tile.reset(lo, up);
                 size_t const tv = static_cast<size_t>(tile.tx) * tile.ty * tile.tz;
-                T const jscale = static_cast<T>(std::ldexp(1.0, st.jexp));
+                double const jscale = std::ldexp(1.0, st.jexp);
                 bool any = false;
                 for (int s : grp) {
                     Species<T>& S = st.sp[s];
@@ -685,7 +692,7 @@ void push(State<T>& st, int species, bool fused) {
                         T const charge = S.charge * S.w[g];
                         if (fixedPoint) {
                             esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
-                                int64_t const q = static_cast<int64_t>(std::nearbyint(v * jscale));
+                                int64_t const q = toFixed(v, jscale);
                                 tile.acc[c * tv + (static_cast<size_t>(nz + z) * tile.ty + (ny + y)) * tile.tx +
                                          (nx + x)] += q;
                             });
@@ -739,7 +746,7 @@ void deposit(State<T>& st, int species) {
                     acc.assign(3 * tv, 0);
                 else
                     cache.assign(3 * tv, T(0));
-                T const jscale = static_cast<T>(std::ldexp(1.0, st.jexp));
+                double const jscale = std::ldexp(1.0, st.jexp);
                 bool any = false;
                 for (int s : grp) {
                     Species<T>& S = st.sp[s];
@@ -764,7 +771,7 @@ void deposit(State<T>& st, int species) {
                             esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
                                 size_t const ti = c * tv + (static_cast<size_t>(cz + z) * ty + (cy + y)) * tx + (cx + x);
                                 if (fixedPoint)
-                                    acc[ti] += static_cast<int64_t>(std::nearbyint(v * jscale));
+                                    acc[ti] += toFixed(v, jscale);
                                 else
                                     cache[ti] += v;
                             });

@@ -19,8 +19,10 @@ __device__ __forceinline__ int wrapIdx(int v, int n) {
     return v >= n ? v - n : v;
 }
 
+constexpr int kPushThreads = 512;
+
 template <int SUPP, int PUSHER, int MODE>
-__global__ __launch_bounds__(256) void kPushDeposit(PushArgs a) {
+__global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
     constexpr int GLO = SUPP / 2;          // FieldToParticleInterpolation.hpp:49
     constexpr int GUP = (SUPP + 1) / 2;    // FieldToParticleInterpolation.hpp:50
     constexpr int GX = SCX + GLO + GUP, GY = SCY + GLO + GUP, GZ = SCZ + GLO + GUP;
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) void kPushDeposit(PushArgs a) {
 
 template <int SUPP, int PUSHER, int MODE>
 static hipError_t launchT(const PushArgs& a, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((kPushDeposit<SUPP, PUSHER, MODE>), dim3(nblocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((kPushDeposit<SUPP, PUSHER, MODE>), dim3(nblocks), dim3(kPushThreads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -344,7 +344,7 @@ pic::PushArgs makePushArgs(pic_ctx* c) {
     a.c2 = c->k.c * c->k.c;
     a.rc2 = c->k.mue0_eps0;
     a.inv_vdt = c->k.inv_cell_volume_dt;
-    a.jscale = std::ldexp(1.0f, c->jexp);
+    a.jscale = std::ldexp(1.0, c->jexp);
     a.jinv = std::ldexp(1.0, -c->jexp);
     a.errFlags = c->errFlags;
     return a;

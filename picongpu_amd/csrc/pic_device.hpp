@@ -351,8 +351,18 @@ struct DepositConsts {
     float cell[3];
     float dt;
     float inv_vdt;  // 1 / float(V*dt)
-    float jscale;   // 2^e
+    double jscale;  // 2^e
 };
+
+// Round-to-nearest-even integer of acc*2^e as two's complement int64, for
+// |acc*2^e| < 2^51: the sum with 1.5*2^52 puts the integer in the low mantissa
+// bits (one rounding, ties to even), and removing the constant only touches
+// the high word because its low word is zero.
+__device__ __forceinline__ unsigned long long toFixed(float acc, double jscale) {
+    double const d = __fma_rn((double)acc, jscale, 6755399441055744.0);
+    unsigned long long const b = (unsigned long long)__double_as_longlong(d);
+    return b - 0x4338000000000000ull;
+}
 
 template <int SUPP>
 struct Esirkepov {
@@ -362,11 +372,12 @@ struct Esirkepov {
     static constexpr int upperMargin = (SUPP + 1) / 2 + 1;             // Esirkepov.hpp:43
 
     // One rotated 1-D pass (cptCurrent1D). p0/p1 are the rotated start/end
-    // positions, st the rotated status bits; addr(i,j,k) maps the rotated
-    // loop indices to the J tile element of this component.
-    template <typename Addr>
+    // positions, st the rotated status bits. `jt` points at the tile element
+    // of rotated offset (begin,begin,begin); SI/SJ/SK are the tile strides of
+    // the rotated i/j/k axes, so every address is jt + a compile-time offset.
+    template <int SI, int SJ, int SK>
     __device__ __forceinline__ static void cpt1D(unsigned long long* __restrict__ jt, const float* p0, const float* p1,
-                                                 const int* st, float csd, float jscale, Addr addr) {
+                                                 const int* st, float csd, double jscale) {
         if (p0[2] == p1[2]) return;
         float s0j[SUPP + 1], s1j[SUPP + 1], s0k[SUPP + 1], s1k[SUPP + 1];
         cachedShape<SUPP>(p0[1], !(st[1] & 2), s0j);
@@ -394,10 +405,9 @@ struct Esirkepov {
                             if (k < end + leaveCellK - 1) {
                                 float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
                                 accumulated_J += W;
-                                long long const q = (long long)rintf(accumulated_J * jscale);
-                                if (q != 0)
-                                    __hip_atomic_fetch_add(jt + addr(i, j, k), (unsigned long long)q,
-                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
+                                                       toFixed(accumulated_J, jscale), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         }
                     }
@@ -413,6 +423,7 @@ struct Esirkepov {
     __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
                                                    Vec3 vel, float charge, const DepositConsts& k) {
         constexpr bool isEven = (SUPP % 2) == 0;
+        constexpr int PX = 1, PY = TX, PZ = TX * TY;
         float const np[3] = {pos.x, pos.y, pos.z};
         float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
         float p0[3], p1[3];
@@ -434,15 +445,16 @@ struct Esirkepov {
             p0[d] -= (float)gs[d];
             p1[d] -= (float)gs[d];
         }
-        int const bx = cx + gs[0], by = cy + gs[1], bz = cz + gs[2];
+        // tile element of cell (gridShift + begin) in every axis
+        unsigned long long* const corner =
+            jt + ((cz + gs[2] + begin) * TY + (cy + gs[1] + begin)) * TX + (cx + gs[0] + begin);
         // J_x: rotation (1,2,0): i->y, j->z, k->x
         {
             float const q0[3] = {p0[1], p0[2], p0[0]};
             float const q1[3] = {p1[1], p1[2], p1[0]};
             int const s3[3] = {st[1], st[2], st[0]};
             float const csd = charge * k.inv_vdt * k.cell[0];
-            cpt1D(jt, q0, q1, s3, csd, k.jscale,
-                  [&](int i, int j, int kk) { return ((bz + j) * TY + (by + i)) * TX + (bx + kk); });
+            cpt1D<PY, PZ, PX>(corner, q0, q1, s3, csd, k.jscale);
         }
         // J_y: rotation (2,0,1): i->z, j->x, k->y
         {
@@ -450,15 +462,13 @@ struct Esirkepov {
             float const q1[3] = {p1[2], p1[0], p1[1]};
             int const s3[3] = {st[2], st[0], st[1]};
             float const csd = charge * k.inv_vdt * k.cell[1];
-            cpt1D(jt + TV, q0, q1, s3, csd, k.jscale,
-                  [&](int i, int j, int kk) { return ((bz + i) * TY + (by + kk)) * TX + (bx + j); });
+            cpt1D<PZ, PX, PY>(corner + TV, q0, q1, s3, csd, k.jscale);
         }
         // J_z: identity: i->x, j->y, k->z
         {
             int const s3[3] = {st[0], st[1], st[2]};
             float const csd = charge * k.inv_vdt * k.cell[2];
-            cpt1D(jt + 2 * TV, p0, p1, s3, csd, k.jscale,
-                  [&](int i, int j, int kk) { return ((bz + kk) * TY + (by + j)) * TX + (bx + i); });
+            cpt1D<PX, PY, PZ>(corner + 2 * TV, p0, p1, s3, csd, k.jscale);
         }
     }
 };

@@ -42,7 +42,7 @@ struct PushArgs {
     float cell[3];
     float dt, c, c2, rc2;
     float inv_vdt;
-    float jscale;     // 2^e
+    double jscale;    // 2^e
     double jinv;      // 2^-e
     int nsp;
     SpeciesDev sp[PIC_MAX_SPECIES];
