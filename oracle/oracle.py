@@ -64,6 +64,7 @@ def lib() -> C.CDLL:
             "oracle_set_threads": (C.c_int32, [C.c_int32]),
             "oracle_set_frames": (C.c_int32, [_P, C.c_int32, _U32, _F, _F, _F, _U16, _U8]),
             "oracle_get_frames": (C.c_int32, [_P, C.c_int32, _U32, _D, _D, _D, _U16, _U8]),
+            "oracle_get_ids": (C.c_int32, [_P, C.c_int32, _U32]),
             "oracle_set_field": (C.c_int32, [_P, C.c_int32, _F]),
             "oracle_get_field": (C.c_int32, [_P, C.c_int32, _D]),
             "oracle_current_reset": (None, [_P]),
@@ -165,6 +166,11 @@ class Oracle:
                                 _p(f["mom"], C.c_double), _p(f["w"], C.c_double), _p(f["cell"], C.c_uint16),
                                 _p(f["mask"], C.c_uint8))
         return f
+
+    def get_ids(self, sp):
+        ids = np.zeros(self.nsc * self.cap(sp), np.uint32)
+        lib().oracle_get_ids(self.h, sp, _p(ids, C.c_uint32))
+        return ids
 
     def set_field(self, which, data):
         data = np.ascontiguousarray(data, np.float32).reshape(3, -1)

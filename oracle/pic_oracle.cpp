@@ -256,6 +256,7 @@ struct Species {
     std::vector<uint16_t> cell;
     std::vector<uint8_t> mask;
     std::vector<uint32_t> count;
+    std::vector<uint32_t> id;  // particle identity, follows the particle through shifts
 };
 
 template <typename T>
@@ -515,6 +516,10 @@ int depUpper(int supp) { return (supp + 1) / 2 + 1; }             // Esirkepov.h
 // Checkerboard colouring of a periodic ring (same rule as the device plan).
 int ringColours(int n, int s, std::vector<int>& colour) {
     colour.assign(n, 0);
+    if (n <= s) {  // ring shorter than the stencil: every supercell its own colour
+        for (int i = 0; i < n; ++i) colour[i] = i;
+        return n;
+    }
     int div = s;
     while (n % div != 0) ++div;
     int const r = n % s;
@@ -853,6 +858,7 @@ void shift(State<T>& st, int species) {
                     }
                     S.w[a] = S.w[b];
                     S.cell[a] = S.cell[b];
+                    S.id[a] = S.id[b];
                     S.mask[a] = 1;
                     ++run;
                 }
@@ -884,6 +890,7 @@ void shift(State<T>& st, int species) {
                 }
                 S.w[a] = S.w[b];
                 S.cell[a] = S.cell[b];
+                S.id[a] = S.id[b];
                 S.mask[a] = 1;
             }
             for (uint32_t i = nn; i < tot; ++i) S.mask[base + i] = 0;
@@ -1059,6 +1066,7 @@ State<T>* create(const oracle_config& cfg, int order) {
         S.w.assign(slots, T(0));
         S.cell.assign(slots, 0);
         S.mask.assign(slots, 0);
+        S.id.assign(slots, 0);
         S.count.assign(st->nsc, 0);
         st->sp.push_back(std::move(S));
     }
@@ -1152,11 +1160,15 @@ int32_t oracle_set_frames(void* p, int32_t sp, const uint32_t* counts, const flo
                 S.pos[c][i] = pos[c * slots + i];
                 S.mom[c][i] = mom[c * slots + i];
             }
+        uint32_t next = 0;
         for (size_t i = 0; i < slots; ++i) {
             S.w[i] = w[i];
             S.cell[i] = cell[i];
             S.mask[i] = mask[i];
+            S.id[i] = 0xFFFFFFFFu;
         }
+        for (int sc = 0; sc < st->nsc; ++sc)
+            for (uint32_t i = 0; i < S.count[sc]; ++i) S.id[static_cast<size_t>(sc) * S.cap + i] = next++;
         return 0;
     };
     return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
@@ -1179,6 +1191,16 @@ int32_t oracle_get_frames(void* p, int32_t sp, uint32_t* counts, double* pos, do
             cell[i] = S.cell[i];
             mask[i] = S.mask[i];
         }
+        return 0;
+    };
+    return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
+}
+
+// particle identities of the frame pool (assigned in slot order by oracle_set_frames)
+int32_t oracle_get_ids(void* p, int32_t sp, uint32_t* ids) {
+    auto impl = [&](auto* st) -> int32_t {
+        auto& S = st->sp[sp];
+        std::copy(S.id.begin(), S.id.end(), ids);
         return 0;
     };
     return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
