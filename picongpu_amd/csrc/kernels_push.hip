@@ -11,6 +11,7 @@
 // read-add-write in a fixed phase order.
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
+#include "pic_shift_common.hpp"
 
 namespace pic {
 
@@ -19,7 +20,8 @@ __device__ __forceinline__ int wrapIdx(int v, int n) {
     return v >= n ? v - n : v;
 }
 
-constexpr int kPushThreads = 512;
+constexpr int kPushThreads = kPushThreadsPerBlock;
+constexpr int kPushWaves = kPushThreads / 64;
 
 template <int SUPP, int PUSHER, int MODE>
 __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
@@ -36,8 +38,19 @@ __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
     constexpr int EB_ELEMS = MODE == 2 ? 1 : 6 * GV;
     constexpr int J_ELEMS = MODE == 0 ? 1 : 3 * JV;
 
-    __shared__ float sEB[EB_ELEMS];
-    __shared__ unsigned long long sJ[J_ELEMS];
+    // One LDS buffer: the E/B tile, then the J tile. After the particle loop
+    // the E/B tile is dead and its space holds the epilogue's scratch.
+    constexpr int EB_BYTES = ((EB_ELEMS * 4 + 15) / 16) * 16;
+    constexpr int EPI_BYTES = (kPushMaxIters * kPushWaves + 1) * 4 + kPushWaves * kNumDirs * 4 + (int)sizeof(GroupScratch);
+    // 16-byte multiple: the int64 J tile that follows needs 8-byte aligned LDS addresses
+    constexpr int LO_BYTES = (((EB_BYTES > EPI_BYTES ? EB_BYTES : EPI_BYTES) + 15) / 16) * 16;
+    static_assert(LO_BYTES % 16 == 0, "J tile must be 16-byte aligned");
+    __shared__ __align__(16) unsigned char smem[LO_BYTES + J_ELEMS * 8];
+    float* const sEB = reinterpret_cast<float*>(smem);
+    unsigned long long* const sJ = reinterpret_cast<unsigned long long*>(smem + LO_BYTES);
+    uint32_t* const sPre = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* const sWaveCnt = sPre + kPushMaxIters * kPushWaves + 1;
+    GroupScratch& sGroup = *reinterpret_cast<GroupScratch*>(sWaveCnt + kPushWaves * kNumDirs);
 
     int const sc = a.scList[blockIdx.x];
     int const sx = sc % a.g.nscx;
@@ -48,7 +61,13 @@ __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
 
     int total = 0;
     for (int s = 0; s < a.nsp; ++s) total += (int)a.sp[s].count[sc];
-    if (total == 0) return;  // uniform across the workgroup
+    if (total == 0) {  // uniform across the workgroup
+        if (MODE != 2 && tid < kOffStride) {
+            for (int s = 0; s < a.nsp; ++s)
+                if (a.sp[s].lists) a.sp[s].outOff[(size_t)sc * kOffStride + tid] = 0u;
+        }
+        return;
+    }
 
     if (MODE != 2) {
         for (int t = tid; t < 6 * GV; t += blockDim.x) {
@@ -90,6 +109,7 @@ __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
         const SpeciesDev& S = a.sp[s];
         int const n = (int)S.count[sc];
         size_t const base = (size_t)sc * (size_t)S.cap;
+        uint32_t leaveBits = 0u;  // bit `it`: the particle of iteration `it` left the supercell
         for (int i = tid; i < n; i += blockDim.x) {
             size_t const gidx = base + i;
             Vec3 const pos{S.pos[0][gidx], S.pos[1][gidx], S.pos[2][gidx]};
@@ -121,6 +141,7 @@ __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
                 S.mom[2][gidx] = mom.z;
                 if ((dir[0] | dir[1] | dir[2]) != 0) S.cell[gidx] = (uint16_t)newIdx;
                 if (newMask >= 2) S.mask[gidx] = (uint8_t)newMask;
+                leaveBits |= (newMask >= 2 ? 1u : 0u) << (i / kPushThreads);
                 if (MODE == 1) {
                     Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + dir[0] + DLO, ly + dir[1] + DLO,
                                                                   lz + dir[2] + DLO, Vec3{p[0], p[1], p[2]}, vel,
@@ -131,6 +152,7 @@ __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
                 Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge, dk);
             }
         }
+        if (MODE != 2 && S.lists) S.leaveBits[(size_t)sc * kPushThreads + tid] = (uint16_t)leaveBits;
     }
     __syncthreads();
 
@@ -149,6 +171,47 @@ __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
             size_t const gi = (size_t)gx + (size_t)a.g.nx * ((size_t)gy + (size_t)a.g.ny * gz);
             float const v = (float)((double)acc * a.jinv);
             a.J[c][gi] += v;
+        }
+    }
+
+    // Epilogue: slot-ordered leaver list from the ballots, then grouped by
+    // exchange code (the input of the supercell shift).
+    if (MODE != 2) {
+        for (int s = 0; s < a.nsp; ++s) {
+            const SpeciesDev& S = a.sp[s];
+            if (!S.lists) continue;
+            int const n = (int)S.count[sc];
+            int const iters = (n + kPushThreads - 1) / kPushThreads;
+            int const wave = tid >> 6;
+            uint32_t const bits = S.leaveBits[(size_t)sc * kPushThreads + tid];
+            __syncthreads();
+            // leavers per (iteration, wave) entry, entries in slot order
+            for (int it = 0; it < iters; ++it) {
+                unsigned long long const m = __ballot((bits >> it) & 1u);
+                if ((tid & 63) == 0) sPre[it * kPushWaves + wave] = (uint32_t)__popcll(m);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t acc = 0;
+                for (int e = 0; e < iters * kPushWaves; ++e) {
+                    uint32_t const c = sPre[e];
+                    sPre[e] = acc;
+                    acc += c;
+                }
+                sPre[kPushMaxIters * kPushWaves] = acc;
+            }
+            __syncthreads();
+            int const L = (int)sPre[kPushMaxIters * kPushWaves];
+            size_t const base = (size_t)sc * (size_t)S.cap;
+            for (int it = 0; it < iters; ++it) {
+                bool const left = (bits >> it) & 1u;
+                unsigned long long const m = __ballot(left);
+                if (left)
+                    S.scratch[base + sPre[it * kPushWaves + wave] + (uint32_t)__popcll(m & laneMaskLt())] =
+                        it * kPushThreads + tid;
+            }
+            __syncthreads();
+            groupLeavers<kPushWaves>(S, sc, L, sGroup, sWaveCnt);
         }
     }
 }

@@ -160,6 +160,7 @@ struct SpeciesMem {
     float mass = 0.0f;
     float charge = 0.0f;
     double wmax = 0.0;
+    bool listsValid = false;  // leaver lists of the last push are in outIdx/scratch/outOff
     pic::SpeciesDev dev{};
 };
 
@@ -228,12 +229,12 @@ struct TimerScope {
         if (c->timing) {
             a = takeEvent(c);
             b = takeEvent(c);
-            if (a) hipEventRecord(a, c->stream);
+            if (a) (void)hipEventRecord(a, c->stream);
         }
     }
     ~TimerScope() {
         if (c->timing && a && b) {
-            hipEventRecord(b, c->stream);
+            (void)hipEventRecord(b, c->stream);
             c->pending[t].push_back({a, b});
         }
     }
@@ -411,6 +412,8 @@ int runPush(pic_ctx* c, int32_t species, int mode) {
         pic::PushArgs a = makePushArgs(c);
         a.nsp = static_cast<int>(grp.size());
         for (size_t i = 0; i < grp.size(); ++i) a.sp[i] = c->sp[grp[i]].dev;
+        if (mode != 2)
+            for (int s : grp) c->sp[s].listsValid = c->sp[s].dev.lists != 0;
         if (mode == 0) {
             a.scList = c->dAll;
             TimerScope ts(c, PIC_TIMER_PUSH);
@@ -509,9 +512,11 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
         }
         if (devAlloc(c, &d.w, slots) || devAlloc(c, &d.cell, slots) || devAlloc(c, &d.mask, slots) ||
             devAlloc(c, &d.count, c->nsc) || devAlloc(c, &d.occ, c->nsc) || devAlloc(c, &d.outIdx, slots) ||
-            devAlloc(c, &d.scratch, slots) || devAlloc(c, &d.outOff, static_cast<size_t>(c->nsc) * pic::kOffStride))
+            devAlloc(c, &d.scratch, slots) || devAlloc(c, &d.outOff, static_cast<size_t>(c->nsc) * pic::kOffStride) ||
+            devAlloc(c, &d.leaveBits, static_cast<size_t>(c->nsc) * pic::kPushThreadsPerBlock))
             return bail(PIC_E_HIP);
         d.cap = m.cap;
+        d.lists = m.cap <= pic::kListMaxSlots ? 1 : 0;
         d.mass = m.mass;
         d.charge = m.charge;
         c->sp.push_back(m);
@@ -525,15 +530,15 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
 
 int pic_destroy(pic_ctx* c) {
     if (!c) return PIC_OK;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int t = 0; t < PIC_TIMER_COUNT; ++t)
         for (auto& pr : c->pending[t]) {
-            hipEventDestroy(pr.first);
-            hipEventDestroy(pr.second);
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
         }
-    for (hipEvent_t e : c->eventPool) hipEventDestroy(e);
-    for (void* p : c->allocations) hipFree(p);
-    if (c->stream) hipStreamDestroy(c->stream);
+    for (hipEvent_t e : c->eventPool) (void)hipEventDestroy(e);
+    for (void* p : c->allocations) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PIC_OK;
 }
@@ -634,6 +639,7 @@ int pic_write_frames(pic_ctx* c, int32_t s, const uint32_t* counts, const float*
     SpeciesMem& m = c->sp[s];
     pic::SpeciesDev& d = m.dev;
     size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    m.listsValid = false;
     if (counts) {
         for (int i = 0; i < c->nsc; ++i)
             if (counts[i] > static_cast<uint32_t>(m.cap)) return fail(PIC_E_CAPACITY, "count above frame capacity");
@@ -740,6 +746,7 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
         a.u_drift[d] = desc->u_drift[d];
         a.u_th[d] = desc->u_th[d];
     }
+    m.listsValid = false;
     PIC_HIP(pic::launchInit(a, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     m.wmax = std::fabs(static_cast<double>(a.weighting));
@@ -804,7 +811,9 @@ int pic_shift(pic_ctx* c, int32_t species) {
         if (species >= 0 && s != species) continue;
         if (c->sp[s].desc.pusher == PIC_PUSHER_NONE) continue;
         TimerScope ts(c, PIC_TIMER_SHIFT);
+        if (!c->sp[s].listsValid) PIC_HIP(pic::launchCollect(c->g, c->sp[s].dev, c->stream));
         PIC_HIP(pic::launchShift(c->g, c->sp[s].dev, c->errFlags, c->stream));
+        c->sp[s].listsValid = false;
     }
     return PIC_OK;
 }

@@ -24,7 +24,9 @@ struct SpeciesDev {
     int32_t* outIdx;     // per supercell: leaver slots grouped by exchange code
     int32_t* scratch;    // per supercell: gap/source lists of the compaction
     uint32_t* outOff;    // per supercell: 28 offsets into outIdx (code 1..26, [27]=total)
+    uint16_t* leaveBits; // per supercell and push thread: leaver flags per loop iteration
     int32_t cap;         // slots per supercell
+    int32_t lists;       // 1: the push kernel builds the leaver lists of this species
     float mass;          // frame::getMass()
     float charge;        // frame::getCharge()
 };
@@ -65,6 +67,11 @@ struct FieldArgs {
 // mode 0: push only, 1: push + fused deposit, 2: deposit only
 hipError_t launchPush(int supp, int pusher, int mode, const PushArgs& a, int nblocks, hipStream_t s);
 hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, hipStream_t s);
+hipError_t launchCollect(const GridDev& g, const SpeciesDev& sp, hipStream_t s);
+// the push kernel builds leaver lists for species with at most this many slots per supercell
+constexpr int kPushThreadsPerBlock = 512;
+constexpr int kPushMaxIters = 16;
+constexpr int kListMaxSlots = kPushThreadsPerBlock * kPushMaxIters;
 hipError_t launchFill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s);
 hipError_t launchEFull(const FieldArgs& a, hipStream_t s);
