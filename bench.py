@@ -34,7 +34,7 @@ BYTES_PER_CELL_STEP = 96.0
 BYTES_PER_CELL_PUSH = 36.0
 
 
-def cpu_baseline(setup_fn, grid, steps=3, warmup=1):
+def cpu_baseline(setup_fn, grid, steps=4, warmup=1):
     import numpy as np  # noqa: F401
 
     from oracle import oracle as O
@@ -56,6 +56,22 @@ def cpu_baseline(setup_fn, grid, steps=3, warmup=1):
                       f"after {warmup} warm-up, FP32 oracle with OpenMP, {dt:.2f} s"}
 
 
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device copy of nbytes (a copy kernel of known size) so
+    rocprofv3's FETCH_SIZE/WRITE_SIZE can be scaled to bytes."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,7 +80,9 @@ def main():
     ap.add_argument("--config", default="kelvin_helmholtz")
     ap.add_argument("--grid", default="", help="override grid, e.g. 128,128,128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-grid", default="64,64,32")
+    ap.add_argument("--cpu-grid", default="128,128,64")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device copy first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -74,6 +92,8 @@ def main():
 
     from picongpu_amd import abi, setups
 
+    if args.calibrate_copy:
+        copy_calibration()
     setup_fn = setups.CONFIGS[args.config]
     kw = {}
     if args.grid:

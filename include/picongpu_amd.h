@@ -200,10 +200,19 @@ int pic_add_current(pic_ctx* ctx);
 int pic_fdtd_after_current(pic_ctx* ctx);
 /* nsteps full PIC steps in the reference stage order, push and deposit fused */
 int pic_step(pic_ctx* ctx, int32_t nsteps);
+/* the same steps replayed from one captured hipGraph of a step (captured on
+ * first use, recaptured when the current's fixed-point scale changes);
+ * not available while kernel timing is enabled */
+int pic_step_graph(pic_ctx* ctx, int32_t nsteps);
 
 /* --- diagnostics ------------------------------------------------------- */
 /* sum over cells of E^2 and B^2 in FP64 (EnergyFields.x.cpp:183-247 without units) */
 int pic_field_energy(pic_ctx* ctx, double out[2]);
+/* Gauss-law residual eps0*div(E) - rho at the cell origins, FP64, nx*ny*nz
+ * values x fastest; rho from every species' charge assignment shape
+ * (ChargeConservation.tpp:181-260). Charge conservation keeps it constant
+ * in time up to round-off. */
+int pic_gauss_residual(pic_ctx* ctx, double* out);
 /* bit flags of device-side errors since the last call (1 = frame capacity exceeded) */
 int pic_error_flags(pic_ctx* ctx, uint32_t* flags);
 /* per-kernel timing with HIP events on the context stream */
@@ -211,6 +220,39 @@ int pic_timing_enable(pic_ctx* ctx, int32_t enable);
 int pic_kernel_timer(pic_ctx* ctx, int32_t timer, double* total_ms, int64_t* launches);
 int32_t pic_j_scale_exp(const pic_ctx* ctx);
 int32_t pic_num_push_phases(const pic_ctx* ctx, int32_t fused);
+
+/* --- differentiable form and its adjoint ------------------------------- */
+/* The reference has no adjoint; these entry points expose the same step
+ * (Simulation.hpp:522-542 stage order: ParticlePush with Boris and CIC
+ * gather, Esirkepov<CIC> deposit, Yee update_beforeCurrent, addCurrent,
+ * update_afterCurrent) for one species with particles kept in identity
+ * order (global cell index and in-cell position, no supercell shift), and its
+ * hand-written FP64 reverse: transposed curl stencils, the Esirkepov transpose
+ * as a gather of the current's adjoint, reverse Boris and reverse gather.
+ * The forward arithmetic per particle is the FP32 arithmetic of pic_push.
+ * The configuration must use the Yee solver; the species must be CIC/Boris.
+ * Field and gradient buffers of pic_adjoint_forward/backward are DEVICE
+ * pointers on the context's device, 3 component planes of nx*ny*nz values
+ * (x fastest); the calls synchronise the context stream before returning. */
+
+/* initial particles of the differentiable form (host pointers): cell = global
+ * cell index, pos = in-cell position in [0,1), 3 planes of n values each */
+int pic_adjoint_load_particles(pic_ctx* ctx, int32_t species, int64_t n, const int32_t* cell, const float* pos,
+                               const float* mom, const float* w);
+/* particles after the last pic_adjoint_forward (host pointers, n as loaded) */
+int pic_adjoint_read_particles(pic_ctx* ctx, int32_t* cell, float* pos, float* mom);
+/* nsteps steps from the loaded particles and the initial fields E0, B0
+ * (float, or null to start from the context's E and B), storing the state
+ * every checkpoint_every steps; the final fields are left in the context and
+ * copied to E_out, B_out when they are not null */
+int pic_adjoint_forward(pic_ctx* ctx, const float* E0, const float* B0, int32_t nsteps, int32_t checkpoint_every,
+                        float* E_out, float* B_out);
+/* reverse of the last pic_adjoint_forward: given dF/dE and dF/dB of the
+ * final fields (double), writes dF/dE0 and dF/dB0 (double, 3*ncells each)
+ * and, when not null, dF/dX0 (cells) and dF/dp0 of the initial particles
+ * (double, 3 planes of n); recomputes each checkpoint segment forward */
+int pic_adjoint_backward(pic_ctx* ctx, const double* dE, const double* dB, double* dE0, double* dB0, double* dX0,
+                         double* dP0);
 
 #ifdef __cplusplus
 }

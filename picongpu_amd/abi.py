@@ -121,12 +121,18 @@ SIGNATURES = {
     "pic_add_current": (C.c_int, [_P]),
     "pic_fdtd_after_current": (C.c_int, [_P]),
     "pic_step": (C.c_int, [_P, C.c_int32]),
+    "pic_step_graph": (C.c_int, [_P, C.c_int32]),
     "pic_field_energy": (C.c_int, [_P, _D]),
+    "pic_gauss_residual": (C.c_int, [_P, _D]),
     "pic_error_flags": (C.c_int, [_P, _U32]),
     "pic_timing_enable": (C.c_int, [_P, C.c_int32]),
     "pic_kernel_timer": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64)]),
     "pic_j_scale_exp": (C.c_int32, [_P]),
     "pic_num_push_phases": (C.c_int32, [_P, C.c_int32]),
+    "pic_adjoint_load_particles": (C.c_int, [_P, C.c_int32, C.c_int64, _I32, _F, _F, _F]),
+    "pic_adjoint_read_particles": (C.c_int, [_P, _I32, _F, _F]),
+    "pic_adjoint_forward": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P]),
+    "pic_adjoint_backward": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -338,6 +344,9 @@ class Simulation:
     def step(self, n=1):
         check(self.lib.pic_step(self.h, int(n)), "pic_step")
 
+    def step_graph(self, n=1):
+        check(self.lib.pic_step_graph(self.h, int(n)), "pic_step_graph")
+
     def sync(self):
         check(self.lib.pic_sync(self.h), "pic_sync")
 
@@ -345,6 +354,11 @@ class Simulation:
     def field_energy(self):
         out = np.zeros(2, np.float64)
         check(self.lib.pic_field_energy(self.h, ptr(out, C.c_double)), "pic_field_energy")
+        return out
+
+    def gauss_residual(self):
+        out = np.zeros(self.ncells, np.float64)
+        check(self.lib.pic_gauss_residual(self.h, ptr(out, C.c_double)), "pic_gauss_residual")
         return out
 
     def error_flags(self) -> int:
@@ -366,3 +380,34 @@ class Simulation:
 
     def num_push_phases(self, fused=True) -> int:
         return self.lib.pic_num_push_phases(self.h, 1 if fused else 0)
+
+    # --- differentiable form (device buffers are passed as integer addresses) ---
+    def adjoint_load_particles(self, sp, cell, pos, mom, w):
+        """cell: (3, n) int32 global cell index; pos, mom: (3, n) float32; w: (n,) float32"""
+        cell = np.ascontiguousarray(cell, np.int32)
+        pos = np.ascontiguousarray(pos, np.float32)
+        mom = np.ascontiguousarray(mom, np.float32)
+        w = np.ascontiguousarray(w, np.float32)
+        n = w.shape[0]
+        assert cell.shape == (3, n) and pos.shape == (3, n) and mom.shape == (3, n)
+        check(self.lib.pic_adjoint_load_particles(self.h, sp, n, ptr(cell, C.c_int32), ptr(pos, C.c_float),
+                                                  ptr(mom, C.c_float), ptr(w, C.c_float)),
+              "pic_adjoint_load_particles")
+        self.adj_n = n
+
+    def adjoint_read_particles(self):
+        n = self.adj_n
+        cell = np.zeros((3, n), np.int32)
+        pos = np.zeros((3, n), np.float32)
+        mom = np.zeros((3, n), np.float32)
+        check(self.lib.pic_adjoint_read_particles(self.h, ptr(cell, C.c_int32), ptr(pos, C.c_float),
+                                                  ptr(mom, C.c_float)), "pic_adjoint_read_particles")
+        return cell, pos, mom
+
+    def adjoint_forward(self, E0_addr, B0_addr, nsteps, every, E_out_addr=None, B_out_addr=None):
+        check(self.lib.pic_adjoint_forward(self.h, E0_addr, B0_addr, int(nsteps), int(every), E_out_addr,
+                                           B_out_addr), "pic_adjoint_forward")
+
+    def adjoint_backward(self, dE_addr, dB_addr, dE0_addr, dB0_addr, dX0_addr=None, dP0_addr=None):
+        check(self.lib.pic_adjoint_backward(self.h, dE_addr, dB_addr, dE0_addr, dB0_addr, dX0_addr, dP0_addr),
+              "pic_adjoint_backward")

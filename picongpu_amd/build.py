@@ -19,7 +19,9 @@ BUILD_DIR = os.path.join(PKG_DIR, "_build")
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libpicongpu_amd.so")
 
-SOURCES = ["kernels_push.hip", "kernels_shift.hip", "kernels_field.hip", "kernels_init.hip", "pic_api.cpp"]
+SOURCES = ["kernels_push.hip", "kernels_shift.hip", "kernels_field.hip", "kernels_init.hip", "kernels_diag.hip",
+           "kernels_adjoint.hip",
+           "pic_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

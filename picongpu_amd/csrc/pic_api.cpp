@@ -164,6 +164,37 @@ struct SpeciesMem {
     pic::SpeciesDev dev{};
 };
 
+// one stored state of the differentiable form
+struct DiffState {
+    int32_t* cell = nullptr;  // 3 planes of n
+    float* pos = nullptr;     // 3 planes of n
+    float* mom = nullptr;     // 3 planes of n
+    float* E = nullptr;       // 3 planes of ncells
+    float* B = nullptr;       // 3 planes of ncells
+};
+
+struct AdjointMem {
+    int species = -1;
+    int64_t n = 0;
+    float mass = 0.0f;    // frame mass
+    float charge = 0.0f;  // frame charge
+    int jexp = 40;
+    float* w = nullptr;
+    DiffState init;   // particle part: loaded initial particles
+    DiffState work;   // particle part: particles of the running forward
+    std::vector<DiffState> checkpoints;  // state at steps 0, K, 2K, ...
+    std::vector<DiffState> tape;         // states of the segment being reversed
+    unsigned long long* Jfix = nullptr;  // 3 planes of ncells
+    double* lE = nullptr;  // 3 planes of ncells each
+    double* lB = nullptr;
+    double* lJ = nullptr;
+    double* lX = nullptr;  // 3 planes of n each
+    double* lP = nullptr;
+    int nsteps = 0;
+    int every = 0;
+    bool forwardValid = false;
+};
+
 }  // namespace
 
 struct pic_ctx {
@@ -182,6 +213,7 @@ struct pic_ctx {
     double* energyPartials = nullptr;
     int energyBlocks = 0;
     unsigned long long* countOut = nullptr;
+    double* diagBuf = nullptr;  // ncells doubles, allocated on first use
     int jexp = 40;
     // [supp-2][fused]
     PhasePlan plans[3][2];
@@ -193,6 +225,12 @@ struct pic_ctx {
     double totalMs[PIC_TIMER_COUNT] = {0, 0, 0};
     int64_t launches[PIC_TIMER_COUNT] = {0, 0, 0};
     std::vector<void*> allocations;
+    // captured step for pic_step_graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    int graphJexp = 0;
+    // differentiable form
+    AdjointMem adj;
 };
 
 namespace {
@@ -537,6 +575,8 @@ int pic_destroy(pic_ctx* c) {
             (void)hipEventDestroy(pr.second);
         }
     for (hipEvent_t e : c->eventPool) (void)hipEventDestroy(e);
+    if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     for (void* p : c->allocations) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -858,6 +898,47 @@ int pic_step(pic_ctx* c, int32_t nsteps) {
     return checkErrors(c);
 }
 
+int pic_step_graph(pic_ctx* c, int32_t nsteps) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
+    if (c->timing) return fail(PIC_E_STATE, "pic_step_graph cannot run with kernel timing enabled");
+    if (c->graphExec && c->graphJexp != c->jexp) {
+        (void)hipGraphExecDestroy(c->graphExec);
+        (void)hipGraphDestroy(c->graph);
+        c->graphExec = nullptr;
+        c->graph = nullptr;
+    }
+    if (!c->graphExec) {
+        // every launch plan must exist before capture (building one copies and synchronises)
+        for (const SpeciesMem& m : c->sp) {
+            if (m.desc.pusher == PIC_PUSHER_NONE) continue;
+            PhasePlan* p = nullptr;
+            int const r = getPlan(c, supportOf(m.desc.shape), true, &p);
+            if (r != PIC_OK) return r;
+        }
+        PIC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int r = pic_current_reset(c);
+        if (r == PIC_OK) r = pic_push(c, -1, 1);
+        if (r == PIC_OK) r = pic_shift(c, -1);
+        if (r == PIC_OK) r = pic_fdtd_before_current(c);
+        if (r == PIC_OK) r = pic_add_current(c);
+        if (r == PIC_OK) r = pic_fdtd_after_current(c);
+        hipGraph_t g = nullptr;
+        hipError_t const e = hipStreamEndCapture(c->stream, &g);
+        if (r != PIC_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        if (e != hipSuccess) return fail(PIC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        c->graph = g;
+        PIC_HIP(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
+        c->graphJexp = c->jexp;
+    }
+    for (int32_t i = 0; i < nsteps; ++i) PIC_HIP(hipGraphLaunch(c->graphExec, c->stream));
+    for (SpeciesMem& m : c->sp) m.listsValid = false;
+    return checkErrors(c);
+}
+
 int pic_field_energy(pic_ctx* c, double out[2]) {
     if (!c || !out) return fail(PIC_E_INVALID, "null argument");
     pic::FieldArgs a = makeFieldArgs(c);
@@ -872,6 +953,24 @@ int pic_field_energy(pic_ctx* c, double out[2]) {
     }
     out[0] = e;
     out[1] = b;
+    return PIC_OK;
+}
+
+int pic_gauss_residual(pic_ctx* c, double* out) {
+    if (!c || !out) return fail(PIC_E_INVALID, "null argument");
+    if (!c->diagBuf && devAlloc(c, &c->diagBuf, c->ncells) != PIC_OK) return PIC_E_HIP;
+    std::vector<pic::SpeciesDev> sp;
+    std::vector<int> supp;
+    for (const SpeciesMem& m : c->sp) {
+        sp.push_back(m.dev);
+        supp.push_back(supportOf(m.desc.shape));
+    }
+    pic::FieldArgs a = makeFieldArgs(c);
+    double const qScale = static_cast<double>(c->k.inv_cell_volume_dt) * static_cast<double>(c->k.dt);
+    PIC_HIP(pic::launchGaussResidual(a, sp.data(), supp.data(), static_cast<int>(sp.size()), qScale,
+                                     static_cast<double>(c->k.eps0), c->diagBuf, c->stream));
+    PIC_HIP(hipMemcpyAsync(out, c->diagBuf, c->ncells * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
 
@@ -902,6 +1001,295 @@ int pic_kernel_timer(pic_ctx* c, int32_t timer, double* total_ms, int64_t* launc
     if (r != PIC_OK) return r;
     if (total_ms) *total_ms = c->totalMs[timer];
     if (launches) *launches = c->launches[timer];
+    return PIC_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// differentiable form and adjoint
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+int allocParticles(pic_ctx* c, DiffState& st, int64_t n) {
+    if (devAlloc(c, &st.cell, 3 * (size_t)n) != PIC_OK) return PIC_E_HIP;
+    if (devAlloc(c, &st.pos, 3 * (size_t)n) != PIC_OK) return PIC_E_HIP;
+    if (devAlloc(c, &st.mom, 3 * (size_t)n) != PIC_OK) return PIC_E_HIP;
+    return PIC_OK;
+}
+
+int allocState(pic_ctx* c, DiffState& st, int64_t n) {
+    if (allocParticles(c, st, n) != PIC_OK) return PIC_E_HIP;
+    if (devAlloc(c, &st.E, 3 * c->ncells) != PIC_OK) return PIC_E_HIP;
+    if (devAlloc(c, &st.B, 3 * c->ncells) != PIC_OK) return PIC_E_HIP;
+    return PIC_OK;
+}
+
+int copyParticles(pic_ctx* c, DiffState& dst, const DiffState& src, int64_t n) {
+    size_t const b = 3 * (size_t)n * 4;
+    PIC_HIP(hipMemcpyAsync(dst.cell, src.cell, b, hipMemcpyDeviceToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(dst.pos, src.pos, b, hipMemcpyDeviceToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(dst.mom, src.mom, b, hipMemcpyDeviceToDevice, c->stream));
+    return PIC_OK;
+}
+
+int copyFields(pic_ctx* c, float* dst, const float* src) {
+    PIC_HIP(hipMemcpyAsync(dst, src, 3 * c->ncells * 4, hipMemcpyDeviceToDevice, c->stream));
+    return PIC_OK;
+}
+
+// ctx E/B are three separate planes; states keep them in one buffer
+int gatherPlanes(pic_ctx* c, float* dst, float* const planes[3]) {
+    for (int a = 0; a < 3; ++a)
+        PIC_HIP(hipMemcpyAsync(dst + a * c->ncells, planes[a], c->ncells * 4, hipMemcpyDeviceToDevice, c->stream));
+    return PIC_OK;
+}
+
+int scatterPlanes(pic_ctx* c, float* const planes[3], const float* src) {
+    for (int a = 0; a < 3; ++a)
+        PIC_HIP(hipMemcpyAsync(planes[a], src + a * c->ncells, c->ncells * 4, hipMemcpyDeviceToDevice, c->stream));
+    return PIC_OK;
+}
+
+// one step of the differentiable form on particles P and fields E[3], B[3]
+int diffStep(pic_ctx* c, const pic::DiffParticles& P, float* const E[3], float* const B[3]) {
+    AdjointMem& m = c->adj;
+    pic::DiffArgs a{};
+    a.g = c->g;
+    for (int d = 0; d < 3; ++d) {
+        a.E[d] = E[d];
+        a.B[d] = B[d];
+        a.Jfix[d] = m.Jfix + d * c->ncells;
+        a.cell[d] = c->k.cell[d];
+    }
+    a.dt = c->k.dt;
+    a.c = c->k.c;
+    a.c2 = c->k.c * c->k.c;
+    a.rc2 = c->k.mue0_eps0;
+    a.inv_vdt = c->k.inv_cell_volume_dt;
+    a.jscale = std::ldexp(1.0, m.jexp);
+    a.mass = m.mass;
+    a.charge = m.charge;
+    PIC_HIP(pic::launchDiffPush(a, P, c->stream));
+    double const jinv = std::ldexp(1.0, -m.jexp);
+    for (int d = 0; d < 3; ++d)
+        PIC_HIP(pic::launchFixToFloat(m.Jfix + d * c->ncells, c->J[d], c->ncells, jinv, c->stream));
+    pic::FieldArgs f = makeFieldArgs(c);
+    for (int d = 0; d < 3; ++d) {
+        f.E[d] = E[d];
+        f.B[d] = B[d];
+    }
+    PIC_HIP(pic::launchBHalf(PIC_SOLVER_YEE, f, c->stream));
+    PIC_HIP(pic::launchEFull(f, c->stream));
+    PIC_HIP(pic::launchAddCurrent(f, c->stream));
+    PIC_HIP(pic::launchBHalf(PIC_SOLVER_YEE, f, c->stream));
+    return PIC_OK;
+}
+
+pic::DiffParticles particlesOf(const AdjointMem& m, const DiffState& st) {
+    pic::DiffParticles P{};
+    P.cell = st.cell;
+    P.pos = st.pos;
+    P.mom = st.mom;
+    P.w = m.w;
+    P.n = m.n;
+    return P;
+}
+
+int stepState(pic_ctx* c, const DiffState& st) {
+    float* const E[3] = {st.E, st.E + c->ncells, st.E + 2 * c->ncells};
+    float* const B[3] = {st.B, st.B + c->ncells, st.B + 2 * c->ncells};
+    return diffStep(c, particlesOf(c->adj, st), E, B);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pic_adjoint_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* cell, const float* pos,
+                               const float* mom, const float* w) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    int r = checkSpecies(c, s);
+    if (r != PIC_OK) return r;
+    if (n <= 0 || !cell || !pos || !mom || !w) return fail(PIC_E_INVALID, "particle arrays and n > 0 are required");
+    if (c->cfg.solver != PIC_SOLVER_YEE) return fail(PIC_E_INVALID, "the differentiable form uses the Yee solver");
+    const SpeciesMem& sm = c->sp[s];
+    if (sm.desc.shape != PIC_SHAPE_CIC || sm.desc.pusher != PIC_PUSHER_BORIS)
+        return fail(PIC_E_INVALID, "the differentiable form supports the CIC shape with the Boris pusher");
+    int const ng[3] = {c->g.nx, c->g.ny, c->g.nz};
+    double wmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            int32_t const ci = cell[a * n + i];
+            float const p = pos[a * n + i];
+            if (ci < 0 || ci >= ng[a] || !(p >= 0.0f && p < 1.0f))
+                return fail(PIC_E_INVALID, "particle cell or in-cell position out of range");
+        }
+        if (!(w[i] > 0.0f)) return fail(PIC_E_INVALID, "weightings must be positive");
+        wmax = std::max(wmax, (double)w[i]);
+    }
+    AdjointMem& m = c->adj;
+    if (m.n != n) {
+        if (m.n != 0) return fail(PIC_E_STATE, "the differentiable particle count is fixed once loaded");
+        m.n = n;
+        if (devAlloc(c, &m.w, (size_t)n) != PIC_OK) return PIC_E_HIP;
+        if (allocParticles(c, m.init, n) != PIC_OK || allocParticles(c, m.work, n) != PIC_OK) return PIC_E_HIP;
+        if (devAlloc(c, &m.Jfix, 3 * c->ncells) != PIC_OK) return PIC_E_HIP;
+        if (devAlloc(c, &m.lE, 3 * c->ncells) != PIC_OK || devAlloc(c, &m.lB, 3 * c->ncells) != PIC_OK ||
+            devAlloc(c, &m.lJ, 3 * c->ncells) != PIC_OK)
+            return PIC_E_HIP;
+        if (devAlloc(c, &m.lX, 3 * (size_t)n) != PIC_OK || devAlloc(c, &m.lP, 3 * (size_t)n) != PIC_OK)
+            return PIC_E_HIP;
+    }
+    m.species = s;
+    m.mass = sm.mass;
+    m.charge = sm.charge;
+    if (c->cfg.j_scale_exp != 0) {
+        m.jexp = c->cfg.j_scale_exp;
+    } else {
+        float const maxCell = std::max(c->k.cell[0], std::max(c->k.cell[1], c->k.cell[2]));
+        double const bound = std::fabs((double)sm.charge) * wmax * (double)c->k.inv_cell_volume_dt * (double)maxCell;
+        m.jexp = bound > 0.0 ? std::max(-60, std::min(100, 40 - (int)std::ceil(std::log2(bound)))) : 40;
+    }
+    size_t const b = 3 * (size_t)n * 4;
+    PIC_HIP(hipMemcpyAsync(m.init.cell, cell, b, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(m.init.pos, pos, b, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(m.init.mom, mom, b, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(m.w, w, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    r = copyParticles(c, m.work, m.init, n);
+    if (r != PIC_OK) return r;
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    m.forwardValid = false;
+    return PIC_OK;
+}
+
+int pic_adjoint_read_particles(pic_ctx* c, int32_t* cell, float* pos, float* mom) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    AdjointMem& m = c->adj;
+    if (m.n == 0) return fail(PIC_E_STATE, "no differentiable particles loaded");
+    size_t const b = 3 * (size_t)m.n * 4;
+    if (cell) PIC_HIP(hipMemcpyAsync(cell, m.work.cell, b, hipMemcpyDeviceToHost, c->stream));
+    if (pos) PIC_HIP(hipMemcpyAsync(pos, m.work.pos, b, hipMemcpyDeviceToHost, c->stream));
+    if (mom) PIC_HIP(hipMemcpyAsync(mom, m.work.mom, b, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_adjoint_forward(pic_ctx* c, const float* E0, const float* B0, int32_t nsteps, int32_t every, float* E_out,
+                        float* B_out) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    AdjointMem& m = c->adj;
+    if (m.n == 0) return fail(PIC_E_STATE, "no differentiable particles loaded");
+    if (nsteps < 0 || every < 1) return fail(PIC_E_INVALID, "nsteps >= 0 and checkpoint_every >= 1 are required");
+    int r;
+    if (E0 && (r = scatterPlanes(c, c->E, E0)) != PIC_OK) return r;
+    if (B0 && (r = scatterPlanes(c, c->B, B0)) != PIC_OK) return r;
+    if ((r = copyParticles(c, m.work, m.init, m.n)) != PIC_OK) return r;
+    size_t const ncp = (size_t)(nsteps + every - 1) / (size_t)every;
+    while (m.checkpoints.size() < ncp) {
+        DiffState st;
+        if (allocState(c, st, m.n) != PIC_OK) return PIC_E_HIP;
+        m.checkpoints.push_back(st);
+    }
+    pic::DiffParticles const P = particlesOf(m, m.work);
+    for (int32_t i = 0; i < nsteps; ++i) {
+        if (i % every == 0) {
+            DiffState& cp = m.checkpoints[(size_t)(i / every)];
+            if ((r = copyParticles(c, cp, m.work, m.n)) != PIC_OK) return r;
+            if ((r = gatherPlanes(c, cp.E, c->E)) != PIC_OK) return r;
+            if ((r = gatherPlanes(c, cp.B, c->B)) != PIC_OK) return r;
+        }
+        if ((r = diffStep(c, P, c->E, c->B)) != PIC_OK) return r;
+    }
+    if (E_out && (r = gatherPlanes(c, E_out, c->E)) != PIC_OK) return r;
+    if (B_out && (r = gatherPlanes(c, B_out, c->B)) != PIC_OK) return r;
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    m.nsteps = nsteps;
+    m.every = every;
+    m.forwardValid = true;
+    return PIC_OK;
+}
+
+int pic_adjoint_backward(pic_ctx* c, const double* dE, const double* dB, double* dE0, double* dB0, double* dX0,
+                         double* dP0) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    AdjointMem& m = c->adj;
+    if (!m.forwardValid) return fail(PIC_E_STATE, "pic_adjoint_backward needs a preceding pic_adjoint_forward");
+    if (!dE || !dB || !dE0 || !dB0) return fail(PIC_E_INVALID, "dE, dB, dE0 and dB0 are required");
+    size_t const nf = 3 * c->ncells;
+    size_t const np = 3 * (size_t)m.n;
+    PIC_HIP(hipMemcpyAsync(m.lE, dE, nf * 8, hipMemcpyDeviceToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(m.lB, dB, nf * 8, hipMemcpyDeviceToDevice, c->stream));
+    PIC_HIP(pic::launchFillD(m.lX, np, 0.0, c->stream));
+    PIC_HIP(pic::launchFillD(m.lP, np, 0.0, c->stream));
+    int const K = m.every;
+    while (m.tape.size() < (size_t)std::min(K, std::max(m.nsteps, 1))) {
+        DiffState st;
+        if (allocState(c, st, m.n) != PIC_OK) return PIC_E_HIP;
+        m.tape.push_back(st);
+    }
+    pic::AdjFieldArgs fa{};
+    fa.g = c->g;
+    for (int d = 0; d < 3; ++d) {
+        fa.lE[d] = m.lE + d * c->ncells;
+        fa.lB[d] = m.lB + d * c->ncells;
+        fa.cell[d] = (double)c->k.cell[d];
+    }
+    double* lJ[3] = {m.lJ, m.lJ + c->ncells, m.lJ + 2 * c->ncells};
+    pic::AdjPartArgs pa{};
+    pa.g = c->g;
+    for (int d = 0; d < 3; ++d) {
+        pa.lJ[d] = lJ[d];
+        pa.lE[d] = fa.lE[d];
+        pa.lB[d] = fa.lB[d];
+        pa.cell[d] = (double)c->k.cell[d];
+    }
+    pa.dt = (double)c->k.dt;
+    pa.rc2 = (double)c->k.mue0_eps0;
+    pa.mass = (double)m.mass;
+    pa.charge = (double)m.charge;
+    pa.inv_vdt = (double)c->k.inv_cell_volume_dt;
+    double const dt = (double)c->k.dt;
+    double const c2 = (double)(c->k.c * c->k.c);
+    double const coeff = (double)c->k.add_current_coeff;
+    int r;
+    int const nseg = (m.nsteps + K - 1) / K;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        int const len = std::min(K, m.nsteps - seg * K);
+        const DiffState& cp = m.checkpoints[(size_t)seg];
+        if ((r = copyParticles(c, m.tape[0], cp, m.n)) != PIC_OK) return r;
+        if ((r = copyFields(c, m.tape[0].E, cp.E)) != PIC_OK) return r;
+        if ((r = copyFields(c, m.tape[0].B, cp.B)) != PIC_OK) return r;
+        for (int t = 1; t < len; ++t) {
+            if ((r = copyParticles(c, m.tape[(size_t)t], m.tape[(size_t)t - 1], m.n)) != PIC_OK) return r;
+            if ((r = copyFields(c, m.tape[(size_t)t].E, m.tape[(size_t)t - 1].E)) != PIC_OK) return r;
+            if ((r = copyFields(c, m.tape[(size_t)t].B, m.tape[(size_t)t - 1].B)) != PIC_OK) return r;
+            if ((r = stepState(c, m.tape[(size_t)t])) != PIC_OK) return r;
+        }
+        for (int t = len - 1; t >= 0; --t) {
+            const DiffState& st = m.tape[(size_t)t];
+            PIC_HIP(pic::launchAdjFields(fa, dt, c2, coeff, lJ, c->stream));
+            for (int d = 0; d < 3; ++d) {
+                pa.E[d] = st.E + d * c->ncells;
+                pa.B[d] = st.B + d * c->ncells;
+            }
+            pic::AdjState as{};
+            as.cell = st.cell;
+            as.pos = st.pos;
+            as.mom = st.mom;
+            as.w = m.w;
+            as.lX = m.lX;
+            as.lP = m.lP;
+            as.n = m.n;
+            PIC_HIP(pic::launchAdjParticles(pa, as, c->stream));
+        }
+    }
+    PIC_HIP(hipMemcpyAsync(dE0, m.lE, nf * 8, hipMemcpyDeviceToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(dB0, m.lB, nf * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (dX0) PIC_HIP(hipMemcpyAsync(dX0, m.lX, np * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (dP0) PIC_HIP(hipMemcpyAsync(dP0, m.lP, np * 8, hipMemcpyDeviceToDevice, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
 

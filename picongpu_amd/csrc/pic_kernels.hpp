@@ -78,6 +78,8 @@ hipError_t launchEFull(const FieldArgs& a, hipStream_t s);
 hipError_t launchAddCurrent(const FieldArgs& a, hipStream_t s);
 hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, hipStream_t s);
 hipError_t launchCountSum(const uint32_t* counts, int n, unsigned long long* out, hipStream_t s);
+hipError_t launchGaussResidual(const FieldArgs& f, const SpeciesDev* sp, const int* supp, int nsp, double qScale,
+                               double eps0, double* buf, hipStream_t s);
 
 struct InitArgs {
     GridDev g;
@@ -93,5 +95,67 @@ struct InitArgs {
 };
 hipError_t launchInit(const InitArgs& a, hipStream_t s);
 hipError_t launchNoise(float* f, size_t n, uint64_t seed, int comp, float amp, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// differentiable form (one CIC/Boris species in identity order) and its adjoint
+// ---------------------------------------------------------------------------
+struct DiffParticles {
+    int32_t* cell;  // global cell index, 3 planes of n
+    float* pos;     // in-cell position, 3 planes of n
+    float* mom;     // 3 planes of n
+    float* w;       // n
+    int64_t n;
+};
+
+struct DiffArgs {
+    GridDev g;
+    const float* E[3];
+    const float* B[3];
+    unsigned long long* Jfix[3];  // int64 fixed-point current, zero on entry
+    float cell[3];
+    float dt, c2, rc2, c;
+    float inv_vdt;
+    double jscale;
+    float mass;    // frame mass
+    float charge;  // frame charge
+};
+
+struct AdjFieldArgs {
+    GridDev g;
+    double* lE[3];
+    double* lB[3];
+    double cell[3];
+};
+
+struct AdjPartArgs {
+    GridDev g;
+    const float* E[3];  // fields at the start of the step
+    const float* B[3];
+    const double* lJ[3];
+    double* lE[3];  // accumulated with atomics
+    double* lB[3];
+    double cell[3];
+    double dt;
+    double rc2;
+    double mass;    // frame mass
+    double charge;  // frame charge
+    double inv_vdt;
+};
+
+struct AdjState {
+    const int32_t* cell;  // particle state at the start of the step
+    const float* pos;
+    const float* mom;
+    const float* w;
+    double* lX;  // adjoint of the global position (cells), 3 planes, in/out
+    double* lP;  // adjoint of the momentum, 3 planes, in/out
+    int64_t n;
+};
+
+hipError_t launchDiffPush(const DiffArgs& a, const DiffParticles& P, hipStream_t s);
+hipError_t launchFixToFloat(unsigned long long* fix, float* J, size_t n, double jinv, hipStream_t s);
+hipError_t launchAdjFields(const AdjFieldArgs& a, double dt, double c2, double coeff, double* lJ[3], hipStream_t s);
+hipError_t launchAdjParticles(const AdjPartArgs& a, const AdjState& st, hipStream_t s);
+hipError_t launchFillD(double* p, size_t n, double v, hipStream_t s);
 
 }  // namespace pic

@@ -123,6 +123,15 @@ CONFIGS = {
 
 
 # -- initial state --------------------------------------------------------
+def differentiable(grid=(64, 64, 64), ppc=8, steps=200, seed=0) -> Setup:
+    """Cold electron plasma for the differentiable form: dx = dy = dz = 1e-7 m,
+    dt at 0.99 of the Yee Courant limit, 1e25 m^-3, CIC, Boris, Yee, with a
+    uniform immobile neutralising background (no ion particles)."""
+    cell = (1e-7, 1e-7, 1e-7)
+    e = SpeciesSetup("e", shape=abi.SHAPE_CIC, pusher=abi.PUSHER_BORIS, ppc=ppc, density_si=1e25, seed=seed)
+    return Setup("differentiable", tuple(grid), cell, yee_dt(cell), 1e25, ppc, [e], steps=steps)
+
+
 def init_device(sim: "abi.Simulation", setup: Setup) -> None:
     """Initial particles (and B noise) generated on the device."""
     for i, s in enumerate(setup.species):

@@ -1,0 +1,146 @@
+"""The HIP differentiable form and its hand-written adjoint against the FP64
+tape (oracle/adjoint_tape.py, pinned in test_adjoint_tape.py) and FP64
+central finite differences.
+
+Bars: forward fields and particles after the run within 1e-5 (max-norm
+relative) of the FP64 tape; gradients with respect to the initial fields
+and particles within 1e-4 of the tape's autograd gradients; gradient with
+respect to the pulse parameters within 1e-3 of FP64 central differences
+(h = 1e-4) of the tape's forward; gradients independent of the checkpoint
+spacing to 1e-12.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint_tape as T
+from picongpu_amd import adjoint, setups
+
+pytestmark = pytest.mark.gpu
+
+F64 = torch.float64
+GRID = (16, 16, 16)
+LO, HI = (4, 4, 4), (12, 12, 12)
+
+
+def rel_max(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def make(nsteps, every, ppc=2, grid=GRID):
+    st = setups.differentiable(grid=grid, ppc=ppc, steps=nsteps)
+    cell, pos, mom, w = adjoint.cold_plasma(st)
+    pic = adjoint.DifferentiablePIC(st, cell, pos, mom, w, nsteps, every)
+    k = pic.sim.constants()
+    g = T.grid_from_constants(st.grid, k, w[0])
+    X = torch.tensor((cell.astype(np.float64) + pos.astype(np.float64)).T.copy())
+    p = torch.tensor(mom.astype(np.float64).T.copy())
+    return st, pic, g, X, p
+
+
+def pulse_amp(st):
+    # a0 = 1 in PIC units, scaled so the probe energy changes by the plasma
+    # response within a few steps without particles crossing many cells
+    return 0.3 * adjoint.a0_field_unit(st)
+
+
+def test_forward_matches_fp64_tape():
+    nsteps = 12
+    print(f"case adjoint forward grid={GRID} steps={nsteps}", flush=True)
+    st, pic, g, X, p = make(nsteps, 5)
+    theta = torch.tensor([1.0, 0.5, 0.05], dtype=F64)
+    E0, B0 = adjoint.gaussian_pulse(st, theta.cuda(), amp=pulse_amp(st))
+    E, B = pic.forward_raw(E0, B0)
+    cell, pos, mom = pic.sim.adjoint_read_particles()
+    e0 = torch.tensor(E0.float().cpu().numpy().astype(np.float64))
+    b0 = torch.tensor(B0.float().cpu().numpy().astype(np.float64))
+    with torch.no_grad():
+        Xt, pt, Et, Bt = X.clone(), p.clone(), e0, b0
+        for _ in range(nsteps):
+            Xt, pt, Et, Bt = T.step(g, Xt, pt, Et, Bt)
+    eE = rel_max(E.cpu().numpy(), Et.numpy())
+    eB = rel_max(B.cpu().numpy(), Bt.numpy())
+    Xg = cell.astype(np.float64) + pos.astype(np.float64)
+    L = np.array(st.grid, np.float64)[:, None]
+    dX = float(np.abs((Xg - Xt.numpy().T + L / 2) % L - L / 2).max())
+    eP = rel_max(mom, pt.numpy().T)
+    print("forward vs FP64 tape: E", eE, "B", eB, "X (cells)", dX, "p", eP)
+    assert eE < 1e-5 and eB < 1e-5 and dX < 1e-5 and eP < 1e-5
+    pic.close()
+
+
+def test_adjoint_matches_fp64_tape():
+    nsteps, every = 12, 5
+    print(f"case adjoint gradients grid={GRID} steps={nsteps} every={every}", flush=True)
+    st, pic, g, X, p = make(nsteps, every)
+    theta = torch.tensor([1.0, 0.5, 0.05], dtype=F64)
+    E0, B0 = adjoint.gaussian_pulse(st, theta.cuda(), amp=pulse_amp(st))
+    E, B = pic.forward_raw(E0, B0)
+    Ed = E.double().requires_grad_(True)
+    Bd = B.double().requires_grad_(True)
+    adjoint.probe_energy(st, Ed, Bd, LO, HI).backward()
+    dE0, dB0, dX0, dP0 = pic.backward_raw(Ed.grad, Bd.grad, particles=True)
+    e0 = torch.tensor(E0.float().cpu().numpy().astype(np.float64))
+    b0 = torch.tensor(B0.float().cpu().numpy().astype(np.float64))
+    f, tE, tB, tX, tP = T.gradient_all(g, X, p, e0, b0, nsteps, LO, HI)
+    err = {"dE0": rel_max(dE0.cpu().numpy(), tE.numpy()), "dB0": rel_max(dB0.cpu().numpy(), tB.numpy()),
+           "dX0": rel_max(dX0.cpu().numpy(), tX.numpy().T), "dP0": rel_max(dP0.cpu().numpy(), tP.numpy().T)}
+    print("adjoint vs FP64 tape:", err, "|dX0|max", float(tX.abs().max()), "|dP0|max", float(tP.abs().max()))
+    for nm, e in err.items():
+        assert e < 1e-4, (nm, e)
+    pic.close()
+
+
+def test_parameter_gradient_matches_central_differences():
+    """dF/dtheta through the torch.autograd.Function (pulse built in torch on
+    the device, HIP forward and adjoint) against FP64 central differences of
+    the tape's forward, h = 1e-4."""
+    nsteps, every = 12, 5
+    print(f"case parameter gradient grid={GRID} steps={nsteps}", flush=True)
+    st, pic, g, X, p = make(nsteps, every)
+    amp = pulse_amp(st)
+    theta = torch.tensor([1.0, 0.5, 0.05], dtype=F64, device="cuda", requires_grad=True)
+    E0, B0 = adjoint.gaussian_pulse(st, theta, amp=amp)
+    E, B = pic.fields(E0, B0)
+    F = adjoint.probe_energy(st, E, B, LO, HI)
+    F.backward()
+    grad = theta.grad.cpu().numpy()
+
+    def objective(th):
+        e0, b0 = T.laser_fields(st.grid, g.cell, th, amp=amp)
+        return float(T.run(g, X.clone(), p.clone(), e0, b0, nsteps, LO, HI))
+
+    h = 1e-4
+    fd = np.zeros(3)
+    with torch.no_grad():
+        base = torch.tensor([1.0, 0.5, 0.05], dtype=F64)
+        for i in range(3):
+            tp, tm = base.clone(), base.clone()
+            tp[i] += h
+            tm[i] -= h
+            fd[i] = (objective(tp) - objective(tm)) / (2 * h)
+    err = np.abs(grad - fd) / np.abs(fd)
+    print("dF/dtheta HIP adjoint:", grad, "FP64 central differences:", fd, "rel err:", err, "F:", float(F))
+    assert np.all(err < 1e-3)
+    pic.close()
+
+
+def test_checkpoint_spacing_does_not_change_gradient():
+    nsteps = 9
+    print(f"case checkpoint spacing steps={nsteps}", flush=True)
+    out = []
+    for every in (1, 4, nsteps):
+        st, pic, g, X, p = make(nsteps, every, ppc=1)
+        theta = torch.tensor([1.0, 0.5, 0.05], dtype=F64)
+        E0, B0 = adjoint.gaussian_pulse(st, theta.cuda(), amp=pulse_amp(st))
+        E, B = pic.forward_raw(E0, B0)
+        Ed = E.double().requires_grad_(True)
+        Bd = B.double().requires_grad_(True)
+        adjoint.probe_energy(st, Ed, Bd, LO, HI).backward()
+        dE0, dB0 = pic.backward_raw(Ed.grad, Bd.grad)
+        out.append((dE0.cpu().numpy(), dB0.cpu().numpy()))
+        pic.close()
+    for a, b in out[1:]:
+        assert rel_max(a, out[0][0]) < 1e-12 and rel_max(b, out[0][1]) < 1e-12
