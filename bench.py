@@ -56,6 +56,50 @@ def cpu_baseline(setup_fn, grid, steps=4, warmup=1):
                       f"after {warmup} warm-up, FP32 oracle with OpenMP, {dt:.2f} s"}
 
 
+def adjoint_timing(steps=200, every=20, grid=(64, 64, 64), ppc=8):
+    """Differentiable form (cold plasma, CIC, Boris, Yee, Gaussian pulse as the
+    initial field): wall time of the checkpointed forward and of the adjoint
+    (segment recompute + reverse), objective = field energy in a 16^3 probe."""
+    import torch
+
+    from picongpu_amd import adjoint, setups
+
+    st = setups.differentiable(grid=grid, ppc=ppc, steps=steps)
+    cell, pos, mom, w = adjoint.cold_plasma(st)
+    pic = adjoint.DifferentiablePIC(st, cell, pos, mom, w, steps, every)
+    theta = torch.tensor([1.0, 0.0, 0.0], dtype=torch.float64, device="cuda")
+    E0, B0 = adjoint.gaussian_pulse(st, theta, lam_cells=8.0, w0_cells=12.0, tau_cells=12.0,
+                                    amp=adjoint.a0_field_unit(st))
+    lo, hi = (40, 24, 24), (56, 40, 40)
+
+    def grads(E, B):
+        Ed = E.double().requires_grad_(True)
+        Bd = B.double().requires_grad_(True)
+        adjoint.probe_energy(st, Ed, Bd, lo, hi).backward()
+        return Ed.grad, Bd.grad
+
+    E, B = pic.forward_raw(E0, B0)  # warm-up (allocates checkpoints)
+    gE, gB = grads(E, B)
+    pic.backward_raw(gE, gB)  # warm-up (allocates the segment tape)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E, B = pic.forward_raw(E0, B0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    gE, gB = grads(E, B)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pic.backward_raw(gE, gB)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    pic.close()
+    n = len(w)
+    return {"workload": f"differentiable {grid[0]}x{grid[1]}x{grid[2]} cold plasma {ppc} ppc, CIC, Boris, Yee, "
+                        f"{steps} steps, checkpoints every {every}", "particles": n,
+            "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3, "ratio": t_bwd / t_fwd,
+            "forward_particle_steps_per_s": n * steps / t_fwd}
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of nbytes (a copy kernel of known size) so
     rocprofv3's FETCH_SIZE/WRITE_SIZE can be scaled to bytes."""
@@ -81,6 +125,7 @@ def main():
     ap.add_argument("--grid", default="", help="override grid, e.g. 128,128,128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-grid", default="128,128,64")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -129,6 +174,10 @@ def main():
     step_bytes = (n_particles * BYTES_PER_PARTICLE + ncells * BYTES_PER_CELL_STEP) * args.steps
     step_gbs = step_bytes / wall / 1e9
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_timing()
+
     baseline = None
     if not args.no_cpu_baseline:
         try:
@@ -137,7 +186,7 @@ def main():
             baseline = {"error": str(exc)}
 
     out = {
-        "metric": "particle-steps/s (push+deposit+FDTD) on 1 GPU",
+        "metric": "particle-steps/s (push+deposit+FDTD) on 1 GPU; adjoint/forward time ratio",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -162,6 +211,7 @@ def main():
                           "push_ms": push_ms / args.steps, "shift_ms": shift_ms / args.steps,
                           "field_ms": field_ms / args.steps},
         "particles_after": counts,
+        "adjoint": adj,
         "cpu_baseline": baseline,
     }
     print(json.dumps(out))

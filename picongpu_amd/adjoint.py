@@ -139,16 +139,29 @@ def gaussian_pulse(setup: setups.Setup, theta: torch.Tensor, lam_cells=8.0, w0_c
     return E, B
 
 
-def cold_plasma(setup: setups.Setup, sp: int = 0, seed: int = None):
+def cold_plasma(setup: setups.Setup, sp: int = 0, seed: int = None, tile=(8, 4, 1)):
     """Uniformly placed cold particles: ppc per cell at seeded uniform in-cell
-    positions, zero momentum. Returns (cell (3,n) int32, pos (3,n) float32,
-    mom (3,n) float32, w (n,) float32)."""
+    positions, zero momentum. Ordered tile-major (tiles of 8x4x1 cells), then
+    by slot (the first particle of every cell of the tile, then the second,
+    ...), then by cell within the tile (x fastest), so the particles a thread
+    block handles sit in a compact box of cells and their current and field
+    adjoint updates accumulate in the block's LDS window. Returns
+    (cell (3,n) int32, pos (3,n) float32, mom (3,n) float32, w (n,) float32)."""
     s = setup.species[sp]
     nx, ny, nz = setup.grid
+    tx, ty, tz = tile
+    if nx % tx or ny % ty or nz % tz:
+        raise ValueError("grid must be a multiple of the tile")
     rng = np.random.default_rng(seed if seed is not None else s.seed)
-    n = nx * ny * nz * s.ppc
-    flat = np.repeat(np.arange(nx * ny * nz, dtype=np.int64), s.ppc)
-    cell = np.stack([flat % nx, (flat // nx) % ny, flat // (nx * ny)]).astype(np.int32)
+    ntx, nty, ntz = nx // tx, ny // ty, nz // tz
+    t = np.arange(ntx * nty * ntz)
+    loc = np.arange(tx * ty * tz)
+    shape = (t.size, s.ppc, loc.size)
+    gx = np.broadcast_to(((t % ntx) * tx)[:, None, None] + (loc % tx)[None, None, :], shape)
+    gy = np.broadcast_to((((t // ntx) % nty) * ty)[:, None, None] + ((loc // tx) % ty)[None, None, :], shape)
+    gz = np.broadcast_to(((t // (ntx * nty)) * tz)[:, None, None] + (loc // (tx * ty))[None, None, :], shape)
+    cell = np.stack([gx.ravel(), gy.ravel(), gz.ravel()]).astype(np.int32)
+    n = cell.shape[1]
     pos = rng.random((3, n), dtype=np.float32)
     mom = np.zeros((3, n), np.float32)
     vol = setup.cell_si[0] * setup.cell_si[1] * setup.cell_si[2]

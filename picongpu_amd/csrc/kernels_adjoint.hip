@@ -29,6 +29,100 @@ __device__ __forceinline__ size_t gidx(const GridDev& g, int x, int y, int z) {
     return (size_t)wrapI(x, g.nx) + (size_t)g.nx * ((size_t)wrapI(y, g.ny) + (size_t)g.ny * (size_t)wrapI(z, g.nz));
 }
 
+// Signed periodic difference in [-n/2, n/2).
+__device__ __forceinline__ int wrapDiff(int d, int n) {
+    d %= n;
+    if (d >= (n + 1) / 2) d -= n;
+    if (d < -(n / 2)) d += n;
+    return d;
+}
+
+// Window of the grid around one block's particles: accumulation goes to LDS
+// when the window holds every node the block's particles touch, and the
+// window is added to global memory once at the end of the block.
+struct BlockWindow {
+    int ref[3];  // reference cell (the block's first particle)
+    int lo[3];   // window origin relative to ref
+    int ext[3];  // window extent in cells
+    int vol;
+    bool lds;
+};
+
+// Every thread of the block calls this (it synchronises). c: the particle's
+// cell, active: whether the thread has a particle; nodes c-LO .. c+HI per
+// axis must fit. rel receives c relative to the reference cell.
+template <int LO, int HI>
+__device__ __forceinline__ BlockWindow blockWindow(const GridDev& g, bool active, const int* c, int* rel, int* sBox,
+                                                   int capCells) {
+    int const n[3] = {g.nx, g.ny, g.nz};
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            sBox[d] = c[d];
+            sBox[3 + d] = 0x7fffffff;
+            sBox[6 + d] = -0x7fffffff;
+        }
+    }
+    __syncthreads();
+    BlockWindow w;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        w.ref[d] = sBox[d];
+        rel[d] = active ? wrapDiff(c[d] - w.ref[d], n[d]) : 0;
+    }
+    if (active) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&sBox[3 + d], rel[d]);
+            atomicMax(&sBox[6 + d], rel[d]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        w.lo[d] = sBox[3 + d] - LO;
+        w.ext[d] = sBox[6 + d] + HI - w.lo[d] + 1;
+    }
+    // a box larger than the LDS capacity is trimmed, one cell at a time from
+    // alternating ends of its longest axis; particles whose nodes then leave
+    // the window go to global memory (windowHolds)
+    while (w.ext[0] * w.ext[1] * w.ext[2] > capCells) {
+        if (w.ext[0] >= w.ext[1] && w.ext[0] >= w.ext[2]) {
+            w.ext[0] -= 1;
+            w.lo[0] += w.ext[0] & 1;
+        } else if (w.ext[1] >= w.ext[2]) {
+            w.ext[1] -= 1;
+            w.lo[1] += w.ext[1] & 1;
+        } else {
+            w.ext[2] -= 1;
+            w.lo[2] += w.ext[2] & 1;
+        }
+    }
+    w.vol = w.ext[0] * w.ext[1] * w.ext[2];
+    w.lds = true;
+    __syncthreads();  // sBox is reused by the next call
+    return w;
+}
+
+// whether the nodes rel-LO .. rel+HI of a particle lie in the window
+template <int LO, int HI>
+__device__ __forceinline__ bool windowHolds(const BlockWindow& w, const int* rel) {
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) in = in && rel[d] - LO >= w.lo[d] && rel[d] + HI < w.lo[d] + w.ext[d];
+    return in;
+}
+
+// global linear index of window element e (x fastest within the window)
+__device__ __forceinline__ uint32_t windowToGlobal(const GridDev& g, const BlockWindow& w, int e) {
+    int const lx = e % w.ext[0];
+    int const ly = (e / w.ext[0]) % w.ext[1];
+    int const lz = e / (w.ext[0] * w.ext[1]);
+    return (uint32_t)wrapI(w.ref[0] + w.lo[0] + lx, g.nx) +
+           (uint32_t)g.nx * ((uint32_t)wrapI(w.ref[1] + w.lo[1] + ly, g.ny) +
+                             (uint32_t)g.ny * (uint32_t)wrapI(w.ref[2] + w.lo[2] + lz, g.nz));
+}
+
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
@@ -67,11 +161,70 @@ __device__ __forceinline__ float gatherGlobalCIC(const float* __restrict__ F, co
 
 
 
-// Esirkepov CIC deposit of one particle into global int64 accumulators.
-// Arithmetic identical to Esirkepov<2>::deposit; addresses wrap periodically.
-__device__ void depositGlobalCIC(const DiffArgs& a, int cx, int cy, int cz, Vec3 pos, Vec3 vel, float charge) {
-    constexpr int SUPP = 2;
-    constexpr int begin = 0, end = 2;
+// Wrapped offsets of the nodes b, b+1, b+2 along one axis, times the axis stride.
+__device__ __forceinline__ void axisOffsets(int b, int n, uint32_t stride, uint32_t* o) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) o[m] = (uint32_t)wrapI(b + m, n) * stride;
+}
+
+// One rotated pass of Esirkepov<CIC> (cptCurrent1D) into global int64
+// accumulators: the float arithmetic of Esirkepov<2>::cpt1D; the rotated
+// axes i, j, k are the real axes AI, AJ, AK, so every node offset is a
+// compile-time pick from the per-axis offset tables.
+template <int AI, int AJ, int AK, bool LDS>
+__device__ __forceinline__ void cptGlobal(unsigned long long* __restrict__ J, const uint32_t (&off)[3][3],
+                                          const float* p0, const float* p1, const int* st, float csd, double jscale) {
+    constexpr int SUPP = 2, begin = 0, end = 2;
+    float const q0[3] = {p0[AI], p0[AJ], p0[AK]};
+    float const q1[3] = {p1[AI], p1[AJ], p1[AK]};
+    int const s3[3] = {st[AI], st[AJ], st[AK]};
+    if (q0[2] == q1[2]) return;
+    float s0j[SUPP + 1], s1j[SUPP + 1], s0k[SUPP + 1], s1k[SUPP + 1];
+    cachedShape<SUPP>(q0[1], !(s3[1] & 2), s0j);
+    cachedShape<SUPP>(q1[1], !(s3[1] & 4), s1j);
+    cachedShape<SUPP>(q0[2], !(s3[2] & 2), s0k);
+    cachedShape<SUPP>(q1[2], !(s3[2] & 4), s1k);
+    int const leaveI = s3[0] & 1, leaveJ = s3[1] & 1, leaveK = s3[2] & 1;
+#pragma unroll
+    for (int i = begin; i < end + 1; ++i) {
+        if (i < end + leaveI) {
+            float const s0i = Shape<SUPP>::assign((float)i - q0[0]);
+            float const dsi = Shape<SUPP>::assign((float)i - q1[0]) - s0i;
+#pragma unroll
+            for (int j = begin; j < end + 1; ++j) {
+                if (j < end + leaveJ) {
+                    float const sj0 = s0j[j - begin];
+                    float const dsj = s1j[j - begin] - sj0;
+                    float const tmp = -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int k = begin; k < end; ++k) {
+                        if (k < end + leaveK - 1) {
+                            float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
+                            acc += W;
+                            unsigned long long* const at = J + (off[AI][i] + off[AJ][j] + off[AK][k]);
+                            if (LDS)
+                                __hip_atomic_fetch_add(at, toFixed(acc, jscale), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            else
+                                __hip_atomic_fetch_add(at, toFixed(acc, jscale), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Esirkepov<CIC> deposit of one particle (Esirkepov.hpp:61-242, the float
+// arithmetic of Esirkepov<2>::deposit) into int64 accumulators: the block's
+// LDS window (LDS = true; rel is the cell relative to the window reference) or
+// global memory. The particle sits in global cell c at in-cell position pos
+// after its move.
+template <bool LDS>
+__device__ __forceinline__ void depositCIC(const DiffArgs& a, unsigned long long* sJ, const BlockWindow& w,
+                                           const int* c, const int* rel, Vec3 pos, Vec3 vel, float charge) {
     float const np[3] = {pos.x, pos.y, pos.z};
     float const dp[3] = {vel.x * a.dt / a.cell[0], vel.y * a.dt / a.cell[1], vel.z * a.dt / a.cell[2]};
     float p0[3], p1[3];
@@ -87,94 +240,117 @@ __device__ void depositGlobalCIC(const DiffArgs& a, int cx, int cy, int cz, Vec3
         p0[d] -= (float)gs[d];
         p1[d] -= (float)gs[d];
     }
-    int const bx = cx + gs[0], by = cy + gs[1], bz = cz + gs[2];
-    // rotations as Esirkepov.hpp:106-124; perm maps rotated (i,j,k) to (x,y,z)
-    for (int comp = 0; comp < 3; ++comp) {
-        int ax[3];  // rotated axis -> real axis
-        if (comp == 0) {
-            ax[0] = 1; ax[1] = 2; ax[2] = 0;
-        } else if (comp == 1) {
-            ax[0] = 2; ax[1] = 0; ax[2] = 1;
-        } else {
-            ax[0] = 0; ax[1] = 1; ax[2] = 2;
+    uint32_t off[3][3];
+    unsigned long long* J[3];
+    if (LDS) {
+        uint32_t const ls[3] = {1u, (uint32_t)w.ext[0], (uint32_t)(w.ext[0] * w.ext[1])};
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) off[d][m] = (uint32_t)(rel[d] + gs[d] + m - w.lo[d]) * ls[d];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[d] = sJ + d * w.vol;
+    } else {
+        axisOffsets(c[0] + gs[0], a.g.nx, 1u, off[0]);
+        axisOffsets(c[1] + gs[1], a.g.ny, (uint32_t)a.g.nx, off[1]);
+        axisOffsets(c[2] + gs[2], a.g.nz, (uint32_t)a.g.nx * (uint32_t)a.g.ny, off[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[d] = a.Jfix[d];
+    }
+    // rotations of Esirkepov.hpp:106-124: J_x (i,j,k) = (y,z,x), J_y (z,x,y), J_z (x,y,z)
+    cptGlobal<1, 2, 0, LDS>(J[0], off, p0, p1, st, charge * a.inv_vdt * a.cell[0], a.jscale);
+    cptGlobal<2, 0, 1, LDS>(J[1], off, p0, p1, st, charge * a.inv_vdt * a.cell[1], a.jscale);
+    cptGlobal<0, 1, 2, LDS>(J[2], off, p0, p1, st, charge * a.inv_vdt * a.cell[2], a.jscale);
+}
+
+// int64 LDS window of the forward deposit: 3 components of this many cells
+constexpr int kDiffLdsCells = 1024;
+
+__global__ __launch_bounds__(256) void kDiffPush(DiffArgs a, DiffParticles P) {
+    __shared__ unsigned long long sJ[3 * kDiffLdsCells];
+    __shared__ int sBox[9];
+    int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool const active = i < P.n;
+    int64_t const n = P.n;
+    int nc[3] = {0, 0, 0};
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    Vec3 vel{0.0f, 0.0f, 0.0f};
+    float charge = 0.0f;
+    if (active) {
+        int const cx = P.cell[i], cy = P.cell[n + i], cz = P.cell[2 * n + i];
+        Vec3 const pos{P.pos[i], P.pos[n + i], P.pos[2 * n + i]};
+        Vec3 mom{P.mom[i], P.mom[n + i], P.mom[2 * n + i]};
+        float const w = P.w[i];
+        float const mass = a.mass * w;
+        charge = a.charge * w;
+        Vec3 const e{gatherGlobalCIC(a.E[0], a.g, cx, cy, cz, pos, 0.5f, 0.0f, 0.0f),
+                     gatherGlobalCIC(a.E[1], a.g, cx, cy, cz, pos, 0.0f, 0.5f, 0.0f),
+                     gatherGlobalCIC(a.E[2], a.g, cx, cy, cz, pos, 0.0f, 0.0f, 0.5f)};
+        Vec3 const b{gatherGlobalCIC(a.B[0], a.g, cx, cy, cz, pos, 0.0f, 0.5f, 0.5f),
+                     gatherGlobalCIC(a.B[1], a.g, cx, cy, cz, pos, 0.5f, 0.0f, 0.5f),
+                     gatherGlobalCIC(a.B[2], a.g, cx, cy, cz, pos, 0.5f, 0.5f, 0.0f)};
+        PushConsts pk;
+        for (int d = 0; d < 3; ++d) pk.cell[d] = a.cell[d];
+        pk.dt = a.dt;
+        pk.c2 = a.c2;
+        pk.rc2 = a.rc2;
+        pk.c = a.c;
+        pushBoris(e, b, mom, vel, mass, charge, pk);
+        p[0] = pos.x;
+        p[1] = pos.y;
+        p[2] = pos.z;
+        p[0] += (vel.x * a.dt) / a.cell[0];
+        p[1] += (vel.y * a.dt) / a.cell[1];
+        p[2] += (vel.z * a.dt) / a.cell[2];
+        int dir[3], idxUnused, maskUnused;
+        moveParticle(p, 0, dir, idxUnused, maskUnused);
+        nc[0] = wrapI(cx + dir[0], a.g.nx);
+        nc[1] = wrapI(cy + dir[1], a.g.ny);
+        nc[2] = wrapI(cz + dir[2], a.g.nz);
+        P.cell[i] = nc[0];
+        P.cell[n + i] = nc[1];
+        P.cell[2 * n + i] = nc[2];
+        P.pos[i] = p[0];
+        P.pos[n + i] = p[1];
+        P.pos[2 * n + i] = p[2];
+        P.mom[i] = mom.x;
+        P.mom[n + i] = mom.y;
+        P.mom[2 * n + i] = mom.z;
+    }
+    // deposit nodes of a particle in cell c: c-1 .. c+2 per axis
+    int rel[3];
+    BlockWindow const win = blockWindow<1, 2>(a.g, active, nc, rel, sBox, kDiffLdsCells);
+    if (win.lds) {
+        for (int e = threadIdx.x; e < 3 * win.vol; e += blockDim.x) sJ[e] = 0ull;
+        __syncthreads();
+        if (active) {
+            // nodes c-1..c+2 lie in the window when the cell is held and the
+            // start and end positions lie in (-1,2) and [0,1); any other
+            // particle (outside a trimmed window, a position rounded onto the
+            // cell edge, non-finite data) deposits to global memory with
+            // wrapped indices instead
+            bool fits = windowHolds<1, 2>(win, rel);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                float const vd = d == 0 ? vel.x : (d == 1 ? vel.y : vel.z);
+                float const p0 = p[d] - vd * a.dt / a.cell[d];
+                fits = fits && p[d] >= 0.0f && p[d] < 1.0f && p0 > -1.0f && p0 < 2.0f;
+            }
+            if (fits)
+                depositCIC<true>(a, sJ, win, nc, rel, Vec3{p[0], p[1], p[2]}, vel, charge);
+            else
+                depositCIC<false>(a, sJ, win, nc, rel, Vec3{p[0], p[1], p[2]}, vel, charge);
         }
-        float const q0[3] = {p0[ax[0]], p0[ax[1]], p0[ax[2]]};
-        float const q1[3] = {p1[ax[0]], p1[ax[1]], p1[ax[2]]};
-        int const s3[3] = {st[ax[0]], st[ax[1]], st[ax[2]]};
-        if (q0[2] == q1[2]) continue;
-        float const csd = charge * a.inv_vdt * a.cell[comp];
-        float s0j[SUPP + 1], s1j[SUPP + 1], s0k[SUPP + 1], s1k[SUPP + 1];
-        cachedShape<SUPP>(q0[1], !(s3[1] & 2), s0j);
-        cachedShape<SUPP>(q1[1], !(s3[1] & 4), s1j);
-        cachedShape<SUPP>(q0[2], !(s3[2] & 2), s0k);
-        cachedShape<SUPP>(q1[2], !(s3[2] & 4), s1k);
-        int const leaveI = s3[0] & 1, leaveJ = s3[1] & 1, leaveK = s3[2] & 1;
-        for (int i = begin; i < end + 1; ++i) {
-            if (!(i < end + leaveI)) continue;
-            float const s0i = Shape<SUPP>::assign((float)i - q0[0]);
-            float const dsi = Shape<SUPP>::assign((float)i - q1[0]) - s0i;
-            for (int j = begin; j < end + 1; ++j) {
-                if (!(j < end + leaveJ)) continue;
-                float const sj0 = s0j[j - begin];
-                float const dsj = s1j[j - begin] - sj0;
-                float const tmp = -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
-                float acc = 0.0f;
-                for (int k = begin; k < end; ++k) {
-                    if (!(k < end + leaveK - 1)) continue;
-                    float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
-                    acc += W;
-                    int rel[3];
-                    rel[ax[0]] = i;
-                    rel[ax[1]] = j;
-                    rel[ax[2]] = k;
-                    atomicAdd(a.Jfix[comp] + gidx(a.g, bx + rel[0], by + rel[1], bz + rel[2]), toFixed(acc, a.jscale));
-                }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 3 * win.vol; e += blockDim.x) {
+            unsigned long long const v = sJ[e];
+            if (v != 0ull) {
+                int const comp = e / win.vol;
+                __hip_atomic_fetch_add(a.Jfix[comp] + windowToGlobal(a.g, win, e - comp * win.vol), v,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
     }
-}
-
-__global__ __launch_bounds__(256) void kDiffPush(DiffArgs a, DiffParticles P) {
-    int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    int64_t const n = P.n;
-    int const cx = P.cell[i], cy = P.cell[n + i], cz = P.cell[2 * n + i];
-    Vec3 const pos{P.pos[i], P.pos[n + i], P.pos[2 * n + i]};
-    Vec3 mom{P.mom[i], P.mom[n + i], P.mom[2 * n + i]};
-    float const w = P.w[i];
-    float const mass = a.mass * w;
-    float const charge = a.charge * w;
-    Vec3 const e{gatherGlobalCIC(a.E[0], a.g, cx, cy, cz, pos, 0.5f, 0.0f, 0.0f),
-                 gatherGlobalCIC(a.E[1], a.g, cx, cy, cz, pos, 0.0f, 0.5f, 0.0f),
-                 gatherGlobalCIC(a.E[2], a.g, cx, cy, cz, pos, 0.0f, 0.0f, 0.5f)};
-    Vec3 const b{gatherGlobalCIC(a.B[0], a.g, cx, cy, cz, pos, 0.0f, 0.5f, 0.5f),
-                 gatherGlobalCIC(a.B[1], a.g, cx, cy, cz, pos, 0.5f, 0.0f, 0.5f),
-                 gatherGlobalCIC(a.B[2], a.g, cx, cy, cz, pos, 0.5f, 0.5f, 0.0f)};
-    PushConsts pk;
-    for (int d = 0; d < 3; ++d) pk.cell[d] = a.cell[d];
-    pk.dt = a.dt;
-    pk.c2 = a.c2;
-    pk.rc2 = a.rc2;
-    pk.c = a.c;
-    Vec3 vel;
-    pushBoris(e, b, mom, vel, mass, charge, pk);
-    float p[3] = {pos.x, pos.y, pos.z};
-    p[0] += (vel.x * a.dt) / a.cell[0];
-    p[1] += (vel.y * a.dt) / a.cell[1];
-    p[2] += (vel.z * a.dt) / a.cell[2];
-    int dir[3], idxUnused, maskUnused;
-    moveParticle(p, 0, dir, idxUnused, maskUnused);
-    int const nx = wrapI(cx + dir[0], a.g.nx), ny = wrapI(cy + dir[1], a.g.ny), nz = wrapI(cz + dir[2], a.g.nz);
-    P.cell[i] = nx;
-    P.cell[n + i] = ny;
-    P.cell[2 * n + i] = nz;
-    P.pos[i] = p[0];
-    P.pos[n + i] = p[1];
-    P.pos[2 * n + i] = p[2];
-    P.mom[i] = mom.x;
-    P.mom[n + i] = mom.y;
-    P.mom[2 * n + i] = mom.z;
-    depositGlobalCIC(a, nx, ny, nz, Vec3{p[0], p[1], p[2]}, vel, charge);
 }
 
 __global__ __launch_bounds__(256) void kFixToFloat(unsigned long long* fix, float* J, size_t n, double jinv) {
@@ -220,32 +396,222 @@ __global__ __launch_bounds__(256) void kScaleCopy(const double* src, double* dst
 
 struct D3 {
     double x, y, z;
-    __device__ double& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
-    __device__ double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
 };
 __device__ __forceinline__ D3 crossD(D3 a, D3 b) {
     return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 __device__ __forceinline__ double dotD(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 axpy(double s, D3 x, D3 y) { return D3{s * x.x + y.x, s * x.y + y.y, s * x.z + y.z}; }
+__device__ __forceinline__ D3 add3(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 scale3(double s, D3 a) { return D3{s * a.x, s * a.y, s * a.z}; }
 
-// CIC weight of node i for in-cell offset p (B-spline 1-|i-p|) and its derivative d/dp
-__device__ __forceinline__ void cicW(double i, double p, double& w, double& dw) {
-    double const u = i - p;
-    double const au = fabs(u);
-    if (au < 1.0) {
-        w = 1.0 - au;
-        dw = u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0);  // d(1-|i-p|)/dp = sign(i-p)
-    } else {
-        w = 0.0;
-        dw = 0.0;
+// Values the compiler must treat as new: recomputing from them is not merged
+// with earlier identical work, so intermediates are not kept live across phases.
+__device__ __forceinline__ double opaque(double x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+__device__ __forceinline__ int opaqueI(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// Phase boundary: no instruction is scheduled across it, so each phase's
+// loads and temporaries stay within the phase.
+__device__ __forceinline__ void phaseBarrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// CIC assignment 1-|k-p| on |k-p| < 1 and its derivative with respect to p
+__device__ __forceinline__ double cicD(double k, double p) {
+    double const u = k - p;
+    return fabs(u) < 1.0 ? 1.0 - fabs(u) : 0.0;
+}
+__device__ __forceinline__ double cicDp(double k, double p) {
+    double const u = k - p;
+    return fabs(u) < 1.0 ? (u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0)) : 0.0;
+}
+
+// Yee offsets of the E and B components (YeeCell.hpp:69-123)
+template <int FIELD, int COMP>  // FIELD 0: E, 1: B
+struct YeeOff {
+    static constexpr double f(int d) {
+        return FIELD == 0 ? (d == COMP ? 0.5 : 0.0) : (d == COMP ? 0.0 : 0.5);
+    }
+};
+
+// Node offsets and CIC weights of the interpolation of one Yee component at
+// in-cell position r of cell c (ShiftCoordinateSystem for an even support).
+template <int FIELD, int COMP>
+__device__ __forceinline__ void interpNodes(const GridDev& g, const int* c, const double* r, uint32_t (&o)[3][2],
+                                            double (&w)[3][2]) {
+    int const n[3] = {g.nx, g.ny, g.nz};
+    uint32_t const stride[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double const v = r[d] - YeeOff<FIELD, COMP>::f(d) - 0.5;
+        int const s = v >= -0.5 ? 0 : -1;
+        double const p = v - (double)s + 0.5;
+        w[d][0] = 1.0 - p;
+        w[d][1] = p;
+        o[d][0] = (uint32_t)wrapI(c[d] + s, n[d]) * stride[d];
+        o[d][1] = (uint32_t)wrapI(c[d] + s + 1, n[d]) * stride[d];
     }
 }
 
-__global__ __launch_bounds__(256) void kAdjParticles(AdjPartArgs a, AdjState S) {
-    int64_t const idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= S.n) return;
+template <int FIELD, int COMP>
+__device__ __forceinline__ double gatherD(const float* __restrict__ F, const GridDev& g, const int* c, const double* r) {
+    uint32_t o[3][2];
+    double w[3][2];
+    interpNodes<FIELD, COMP>(g, c, r, o, w);
+    double v = 0.0;
+#pragma unroll
+    for (int zz = 0; zz < 2; ++zz)
+#pragma unroll
+        for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+            for (int xx = 0; xx < 2; ++xx)
+                v += (double)F[o[0][xx] + o[1][yy] + o[2][zz]] * (w[0][xx] * w[1][yy] * w[2][zz]);
+    return v;
+}
+
+// Reverse of the interpolation of one component: adds lam * weight to the
+// field adjoint at the 8 nodes (the block's LDS window plane L when LDS, with
+// rel the cell relative to the window reference; the global plane otherwise)
+// and lam * d(value)/dX to gX.
+template <int FIELD, int COMP, bool LDS>
+__device__ __forceinline__ void gatherAdj(const float* __restrict__ F, double* __restrict__ L, const GridDev& g,
+                                          const BlockWindow& win, const int* c, const int* rel, const double* r,
+                                          double lam, double* gX) {
+    uint32_t o[3][2];
+    double w[3][2];
+    interpNodes<FIELD, COMP>(g, c, r, o, w);
+    uint32_t lo[3][2];
+    if (LDS) {
+        uint32_t const ls[3] = {1u, (uint32_t)win.ext[0], (uint32_t)(win.ext[0] * win.ext[1])};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int const sh = (r[d] - YeeOff<FIELD, COMP>::f(d) - 0.5) >= -0.5 ? 0 : -1;
+            lo[d][0] = (uint32_t)(rel[d] + sh - win.lo[d]) * ls[d];
+            lo[d][1] = lo[d][0] + ls[d];
+        }
+    }
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int zz = 0; zz < 2; ++zz)
+#pragma unroll
+        for (int yy = 0; yy < 2; ++yy)
+#pragma unroll
+            for (int xx = 0; xx < 2; ++xx) {
+                uint32_t const at = o[0][xx] + o[1][yy] + o[2][zz];
+                double const f = (double)F[at];
+                double const sx = xx ? 1.0 : -1.0, sy = yy ? 1.0 : -1.0, sz = zz ? 1.0 : -1.0;
+                gx += f * sx * w[1][yy] * w[2][zz];
+                gy += f * w[0][xx] * sy * w[2][zz];
+                gz += f * w[0][xx] * w[1][yy] * sz;
+                double const v = lam * (w[0][xx] * w[1][yy] * w[2][zz]);
+                if (LDS)
+                    __hip_atomic_fetch_add(L + (lo[0][xx] + lo[1][yy] + lo[2][zz]), v, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                else
+                    unsafeAtomicAdd(L + at, v);
+            }
+    gX[0] += lam * gx;
+    gX[1] += lam * gy;
+    gX[2] += lam * gz;
+}
+
+// Reverse of the Esirkepov<CIC> deposit of one particle moving from X0 to X1
+// (positions in cells relative to the first node of the stencil): the current
+// adjoint lambda_J is gathered through the derivatives of the CIC weights of
+// both end points. With S0, S1 the weights at the nodes and DS = S1 - S0, the
+// current of component c at node (i, j, k) of its rotated axes is
+// -csd W(i,j) C(k), W = S0a S0b + (DSa S0b + S0a DSb)/2 + DSa DSb/3 and C(k)
+// the prefix sum of DS along c (Esirkepov.hpp:140-240). Accumulates
+// dF/dX0 into l0 and dF/dX1 into l1 per axis.
+template <int COMP>
+__device__ __forceinline__ void esirkepovAdjComp(const double* __restrict__ lJ, const GridDev& g, const int* bc,
+                                                 const double* X0, const double* X1, double csd, double* l0,
+                                                 double* l1) {
+    constexpr int A = (COMP + 1) % 3;  // rotated i axis
+    constexpr int B = (COMP + 2) % 3;  // rotated j axis
+    int const n[3] = {g.nx, g.ny, g.nz};
+    uint32_t const stride[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+    uint32_t const k0 = (uint32_t)wrapI(bc[COMP], n[COMP]) * stride[COMP];
+    uint32_t const k1 = (uint32_t)wrapI(bc[COMP] + 1, n[COMP]) * stride[COMP];
+    uint32_t ob[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ob[j] = (uint32_t)wrapI(bc[B] + j, n[B]) * stride[B];
+    double const C0 = cicD(0.0, X1[COMP]) - cicD(0.0, X0[COMP]);
+    double const C1 = C0 + (cicD(1.0, X1[COMP]) - cicD(1.0, X0[COMP]));
+    double dC0 = 0.0, dC1 = 0.0;
+    double l0a = 0.0, l1a = 0.0, l0b = 0.0, l1b = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+        uint32_t const oa = (uint32_t)wrapI(bc[A] + i, n[A]) * stride[A];
+        // the six current adjoints of this row, loaded together
+        double m[3][2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            m[j][0] = lJ[oa + ob[j] + k0];
+            m[j][1] = lJ[oa + ob[j] + k1];
+        }
+        double const s0a = cicD((double)i, X0[A]);
+        double const dsa = cicD((double)i, X1[A]) - s0a;
+        double const d0a = cicDp((double)i, X0[A]);
+        double const d1a = cicDp((double)i, X1[A]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double const s0b = cicD((double)j, X0[B]);
+            double const dsb = cicD((double)j, X1[B]) - s0b;
+            double const W = s0a * s0b + 0.5 * (dsa * s0b + s0a * dsb) + (1.0 / 3.0) * dsa * dsb;
+            double const m0 = -csd * m[j][0];
+            double const m1 = -csd * m[j][1];
+            double const G = m0 * C0 + m1 * C1;
+            dC0 += m0 * W;
+            dC1 += m1 * W;
+            // dW/dS0a = S0b + DSb/2 - dW/dDSa, dW/dS1a = dW/dDSa = S0b/2 + DSb/3 (and a <-> b)
+            double const wa = 0.5 * s0b + (1.0 / 3.0) * dsb;
+            double const wb = 0.5 * s0a + (1.0 / 3.0) * dsa;
+            l0a += G * (s0b + 0.5 * dsb - wa) * d0a;
+            l1a += G * wa * d1a;
+            l0b += G * (s0a + 0.5 * dsa - wb) * cicDp((double)j, X0[B]);
+            l1b += G * wb * cicDp((double)j, X1[B]);
+        }
+    }
+    l0[A] += l0a;
+    l1[A] += l1a;
+    l0[B] += l0b;
+    l1[B] += l1b;
+    // C0 = DS_0, C1 = DS_0 + DS_1 along the current's own axis
+    l1[COMP] += (dC0 + dC1) * cicDp(0.0, X1[COMP]) + dC1 * cicDp(1.0, X1[COMP]);
+    l0[COMP] -= (dC0 + dC1) * cicDp(0.0, X0[COMP]) + dC1 * cicDp(1.0, X0[COMP]);
+}
+
+struct BorisFwd {
+    D3 pm, t, s, pp, p1;
+    double g, f, beta, hq;
+};
+
+// Boris push in FP64 (particlePusherBoris.hpp:41-90)
+__device__ __forceinline__ BorisFwd borisFwd(D3 p0, D3 E, D3 B, double m, double q, double dt, double rc2) {
+    BorisFwd b;
+    b.hq = 0.5 * q * dt;
+    b.pm = axpy(b.hq, E, p0);
+    b.g = sqrt(1.0 + dotD(b.pm, b.pm) * rc2 / (m * m));
+    b.beta = 0.5 * (q / m) * dt;
+    b.t = scale3(b.beta / b.g, B);
+    b.f = 1.0 / (1.0 + dotD(b.t, b.t));
+    b.s = scale3(2.0 * b.f, b.t);
+    b.pp = add3(b.pm, crossD(b.pm, b.t));
+    b.p1 = axpy(b.hq, E, add3(b.pm, crossD(b.pp, b.s)));
+    return b;
+}
+
+__device__ __forceinline__ void adjointParticle(const AdjPartArgs& a, const AdjState& S, int64_t idx, const int* c,
+                                                const BlockWindow& win, const int* rel, double* sL, bool lds) {
     int64_t const n = S.n;
-    int const c[3] = {S.cell[idx], S.cell[n + idx], S.cell[2 * n + idx]};
     double const r[3] = {(double)S.pos[idx], (double)S.pos[n + idx], (double)S.pos[2 * n + idx]};
     D3 const p0{(double)S.mom[idx], (double)S.mom[n + idx], (double)S.mom[2 * n + idx]};
     double const w = (double)S.w[idx];
@@ -253,192 +619,149 @@ __global__ __launch_bounds__(256) void kAdjParticles(AdjPartArgs a, AdjState S) 
     double const q = a.charge * w;
     double const dt = a.dt;
 
-    // ---- forward recompute in FP64: gather (weights and their derivatives) ----
-    const double fE[3][3] = {{0.5, 0, 0}, {0, 0.5, 0}, {0, 0, 0.5}};
-    const double fB[3][3] = {{0, 0.5, 0.5}, {0.5, 0, 0.5}, {0.5, 0.5, 0}};
-    int baseE[3][3], baseB[3][3];
-    double wE[3][3][2], dwE[3][3][2], wB[3][3][2], dwB[3][3][2];
-    D3 Ep{0, 0, 0}, Bp{0, 0, 0};
-    D3 dEp[3], dBp[3];  // dEp[comp][axis]
-    for (int comp = 0; comp < 3; ++comp) {
-        for (int d = 0; d < 3; ++d) {
-            double const v = r[d] - fE[comp][d] - 0.5;
-            int const s = v >= -0.5 ? 0 : -1;
-            double const pp = v - s + 0.5;
-            baseE[comp][d] = c[d] + s;
-            wE[comp][d][0] = 1.0 - pp;
-            wE[comp][d][1] = pp;
-            dwE[comp][d][0] = -1.0;
-            dwE[comp][d][1] = 1.0;
-            double const vb = r[d] - fB[comp][d] - 0.5;
-            int const sb = vb >= -0.5 ? 0 : -1;
-            double const pb = vb - sb + 0.5;
-            baseB[comp][d] = c[d] + sb;
-            wB[comp][d][0] = 1.0 - pb;
-            wB[comp][d][1] = pb;
-            dwB[comp][d][0] = -1.0;
-            dwB[comp][d][1] = 1.0;
-        }
-        double ve = 0, vb = 0;
-        D3 ge{0, 0, 0}, gb{0, 0, 0};
-        for (int zz = 0; zz < 2; ++zz)
-            for (int yy = 0; yy < 2; ++yy)
-                for (int xx = 0; xx < 2; ++xx) {
-                    double const fe = a.E[comp][gidx(a.g, baseE[comp][0] + xx, baseE[comp][1] + yy, baseE[comp][2] + zz)];
-                    double const fb = a.B[comp][gidx(a.g, baseB[comp][0] + xx, baseB[comp][1] + yy, baseB[comp][2] + zz)];
-                    double const we = wE[comp][0][xx] * wE[comp][1][yy] * wE[comp][2][zz];
-                    double const wb = wB[comp][0][xx] * wB[comp][1][yy] * wB[comp][2][zz];
-                    ve += fe * we;
-                    vb += fb * wb;
-                    ge.x += fe * dwE[comp][0][xx] * wE[comp][1][yy] * wE[comp][2][zz];
-                    ge.y += fe * wE[comp][0][xx] * dwE[comp][1][yy] * wE[comp][2][zz];
-                    ge.z += fe * wE[comp][0][xx] * wE[comp][1][yy] * dwE[comp][2][zz];
-                    gb.x += fb * dwB[comp][0][xx] * wB[comp][1][yy] * wB[comp][2][zz];
-                    gb.y += fb * wB[comp][0][xx] * dwB[comp][1][yy] * wB[comp][2][zz];
-                    gb.z += fb * wB[comp][0][xx] * wB[comp][1][yy] * dwB[comp][2][zz];
-                }
-        Ep[comp] = ve;
-        Bp[comp] = vb;
-        dEp[comp] = ge;
-        dBp[comp] = gb;
+    // ---- forward recompute: gather, Boris, velocity ----
+    D3 const E{gatherD<0, 0>(a.E[0], a.g, c, r), gatherD<0, 1>(a.E[1], a.g, c, r), gatherD<0, 2>(a.E[2], a.g, c, r)};
+    phaseBarrier();
+    D3 const B{gatherD<1, 0>(a.B[0], a.g, c, r), gatherD<1, 1>(a.B[1], a.g, c, r), gatherD<1, 2>(a.B[2], a.g, c, r)};
+    phaseBarrier();
+    D3 p1;
+    {
+        BorisFwd const b = borisFwd(p0, E, B, m, q, dt, a.rc2);
+        p1 = b.p1;
     }
-    // ---- Boris forward (particlePusherBoris.hpp:41-90) ----
-    double const hq = 0.5 * q * dt;
-    D3 const pm{p0.x + hq * Ep.x, p0.y + hq * Ep.y, p0.z + hq * Ep.z};
-    double const g = sqrt(1.0 + dotD(pm, pm) * a.rc2 / (m * m));
-    double const beta = 0.5 * (q / m) * dt;
-    D3 const t{beta * Bp.x / g, beta * Bp.y / g, beta * Bp.z / g};
-    double const f = 1.0 / (1.0 + dotD(t, t));
-    D3 const s{2.0 * f * t.x, 2.0 * f * t.y, 2.0 * f * t.z};
-    D3 const cpt = crossD(pm, t);
-    D3 const pp{pm.x + cpt.x, pm.y + cpt.y, pm.z + cpt.z};
-    D3 const cps = crossD(pp, s);
-    D3 const p1{pm.x + cps.x + hq * Ep.x, pm.y + cps.y + hq * Ep.y, pm.z + cps.z + hq * Ep.z};
     double const G = sqrt(m * m + dotD(p1, p1) * a.rc2);
-    D3 const v{p1.x / G, p1.y / G, p1.z / G};
-    // end position in cell units relative to the particle's cell: r + v dt / cell
-    double X1[3], X0[3];
-    for (int d = 0; d < 3; ++d) {
-        X1[d] = r[d] + v[d] * dt / a.cell[d];
-        X0[d] = X1[d] - v[d] * dt / a.cell[d];
+    double const v[3] = {p1.x / G, p1.y / G, p1.z / G};
+
+    // ---- reverse Esirkepov ----
+    double lX1[3], lv[3];
+    {
+        double X0[3], X1[3];
+        int bc[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            X1[d] = r[d] + v[d] * dt / a.cell[d];
+            X0[d] = X1[d] - v[d] * dt / a.cell[d];
+            int const gs = (int)fmin(floor(X0[d]), floor(X1[d]));
+            X0[d] -= (double)gs;
+            X1[d] -= (double)gs;
+            bc[d] = c[d] + gs;
+        }
+        double l0[3] = {0.0, 0.0, 0.0}, l1[3] = {0.0, 0.0, 0.0};
+        phaseBarrier();
+        esirkepovAdjComp<0>(a.lJ[0], a.g, bc, X0, X1, q * a.inv_vdt * a.cell[0], l0, l1);
+        phaseBarrier();
+        esirkepovAdjComp<1>(a.lJ[1], a.g, bc, X0, X1, q * a.inv_vdt * a.cell[1], l0, l1);
+        phaseBarrier();
+        esirkepovAdjComp<2>(a.lJ[2], a.g, bc, X0, X1, q * a.inv_vdt * a.cell[2], l0, l1);
+        phaseBarrier();
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            // X1 = X + v dt/cell, X0 = X1 - v dt/cell
+            lX1[d] = S.lX[d * n + idx] + l0[d] + l1[d];
+            lv[d] = (lX1[d] - l0[d]) * dt / a.cell[d];
+        }
     }
 
-    // ---- reverse Esirkepov as a gather of lambda_J ----
-    // shape values at nodes gs+0..2 per axis, for start (0) and end (1)
-    int gs[3];
-    double s0[3][3], s1[3][3], ds0[3][3], ds1[3][3];
-    for (int d = 0; d < 3; ++d) {
-        gs[d] = (int)fmin(floor(X0[d]), floor(X1[d]));
-        for (int k = 0; k < 3; ++k) {
-            cicW((double)k, X0[d] - gs[d], s0[d][k], ds0[d][k]);
-            cicW((double)k, X1[d] - gs[d], s1[d][k], ds1[d][k]);
-        }
-    }
-    double g0[3][3] = {{0}}, g1[3][3] = {{0}};  // dLambda/d s0, s1 per axis and node
-    for (int comp = 0; comp < 3; ++comp) {
-        int const ax = comp == 0 ? 1 : (comp == 1 ? 2 : 0);  // rotated i axis
-        int const bx = comp == 0 ? 2 : (comp == 1 ? 0 : 1);  // rotated j axis
-        double const csd = q * a.inv_vdt * a.cell[comp];
-        double C[3], lam[3][3][3];
-        double run = 0.0;
-        for (int k = 0; k < 3; ++k) {
-            run += s1[comp][k] - s0[comp][k];
-            C[k] = run;
-        }
-        double dC[3] = {0, 0, 0};
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                double const s0a = s0[ax][i], dsa = s1[ax][i] - s0[ax][i];
-                double const s0b = s0[bx][j], dsb = s1[bx][j] - s0[bx][j];
-                double const W = s0a * s0b + 0.5 * (dsa * s0b + s0a * dsb) + (1.0 / 3.0) * dsa * dsb;
-                double Gij = 0.0;
-                for (int k = 0; k < 2; ++k) {  // the current at the last node of the axis is zero
-                    int rel[3];
-                    rel[ax] = i;
-                    rel[bx] = j;
-                    rel[comp] = k;
-                    double const l = a.lJ[comp][gidx(a.g, c[0] + gs[0] + rel[0], c[1] + gs[1] + rel[1],
-                                                     c[2] + gs[2] + rel[2])];
-                    lam[i][j][k] = l;
-                    Gij += -csd * l * C[k];
-                    dC[k] += -csd * l * W;
-                }
-                double const dWs0a = s0b + 0.5 * dsb, dWdsa = 0.5 * s0b + (1.0 / 3.0) * dsb;
-                double const dWs0b = s0a + 0.5 * dsa, dWdsb = 0.5 * s0a + (1.0 / 3.0) * dsa;
-                g0[ax][i] += Gij * (dWs0a - dWdsa);
-                g1[ax][i] += Gij * dWdsa;
-                g0[bx][j] += Gij * (dWs0b - dWdsb);
-                g1[bx][j] += Gij * dWdsb;
-            }
-        // C[k] = sum_{k'<=k} (s1 - s0): reverse prefix sum
-        double tail = 0.0;
-        for (int k = 2; k >= 0; --k) {
-            tail += dC[k];
-            g1[comp][k] += tail;
-            g0[comp][k] -= tail;
-        }
-    }
-    // weights depend on X0 = X1 - v dt/cell and X1 (offsets relative to gs)
-    double lX1[3], lv[3];
-    for (int d = 0; d < 3; ++d) {
-        double l0 = 0.0, l1 = 0.0;
-        for (int k = 0; k < 3; ++k) {
-            l0 += g0[d][k] * ds0[d][k];
-            l1 += g1[d][k] * ds1[d][k];
-        }
-        lX1[d] = S.lX[d * n + idx] + l0 + l1;
-        lv[d] = -l0 * dt / a.cell[d];
-    }
-    // X1 = X + v dt/cell
-    for (int d = 0; d < 3; ++d) lv[d] += lX1[d] * dt / a.cell[d];
-    // v = p1 / G, G = sqrt(m^2 + |p1|^2 rc2)
+    phaseBarrier();
+    // ---- reverse velocity: v = p1 / sqrt(m^2 + |p1|^2 rc2) ----
     D3 const LV{lv[0], lv[1], lv[2]};
-    double const pv = dotD(p1, LV);
-    D3 l1{S.lP[idx] + LV.x / G - p1.x * pv * a.rc2 / (G * G * G),
-          S.lP[n + idx] + LV.y / G - p1.y * pv * a.rc2 / (G * G * G),
-          S.lP[2 * n + idx] + LV.z / G - p1.z * pv * a.rc2 / (G * G * G)};
+    double const kv = dotD(p1, LV) * a.rc2 / (G * G * G);
+    D3 const l1{S.lP[idx] + LV.x / G - p1.x * kv, S.lP[n + idx] + LV.y / G - p1.y * kv,
+                S.lP[2 * n + idx] + LV.z / G - p1.z * kv};
+
     // ---- reverse Boris ----
-    D3 lEp{hq * l1.x, hq * l1.y, hq * l1.z};
-    D3 lpm = l1;                 // pl = pm + pp x s
-    D3 const lpp = crossD(s, l1);   // d(pp x s)/dpp ^T l = s x l
-    D3 const ls = crossD(l1, pp);   // d(pp x s)/ds ^T l = l x pp
-    // pp = pm + pm x t
-    D3 const tl = crossD(t, lpp);
-    lpm = D3{lpm.x + lpp.x + tl.x, lpm.y + lpp.y + tl.y, lpm.z + lpp.z + tl.z};
-    D3 const lt1 = crossD(lpp, pm);
-    // s = 2 f t, f = 1/(1+|t|^2)
-    double const tls = dotD(t, ls);
-    D3 const lt{lt1.x + 2.0 * f * ls.x - 4.0 * f * f * tls * t.x, lt1.y + 2.0 * f * ls.y - 4.0 * f * f * tls * t.y,
-                lt1.z + 2.0 * f * ls.z - 4.0 * f * f * tls * t.z};
-    // t = beta B / g
-    D3 const lBp{beta * lt.x / g, beta * lt.y / g, beta * lt.z / g};
-    double const lg = -beta * dotD(Bp, lt) / (g * g);
-    // g = sqrt(1 + |pm|^2 rc2/m^2)
-    double const kk = lg * a.rc2 / (m * m * g);
-    lpm = D3{lpm.x + kk * pm.x, lpm.y + kk * pm.y, lpm.z + kk * pm.z};
-    // pm = p0 + hq E
-    lEp = D3{lEp.x + hq * lpm.x, lEp.y + hq * lpm.y, lEp.z + hq * lpm.z};
-    D3 const lp0 = lpm;
-    // ---- reverse gather ----
-    double lX0[3] = {lX1[0], lX1[1], lX1[2]};
-    for (int comp = 0; comp < 3; ++comp) {
-        lX0[0] += lEp[comp] * dEp[comp].x + lBp[comp] * dBp[comp].x;
-        lX0[1] += lEp[comp] * dEp[comp].y + lBp[comp] * dBp[comp].y;
-        lX0[2] += lEp[comp] * dEp[comp].z + lBp[comp] * dBp[comp].z;
-        for (int zz = 0; zz < 2; ++zz)
-            for (int yy = 0; yy < 2; ++yy)
-                for (int xx = 0; xx < 2; ++xx) {
-                    double const we = wE[comp][0][xx] * wE[comp][1][yy] * wE[comp][2][zz];
-                    double const wb = wB[comp][0][xx] * wB[comp][1][yy] * wB[comp][2][zz];
-                    atomicAdd(a.lE[comp] + gidx(a.g, baseE[comp][0] + xx, baseE[comp][1] + yy, baseE[comp][2] + zz),
-                              lEp[comp] * we);
-                    atomicAdd(a.lB[comp] + gidx(a.g, baseB[comp][0] + xx, baseB[comp][1] + yy, baseB[comp][2] + zz),
-                              lBp[comp] * wb);
-                }
+    D3 lE, lB, lp0;
+    {
+        D3 const p0o{opaque(p0.x), opaque(p0.y), opaque(p0.z)};
+        D3 const Eo{opaque(E.x), opaque(E.y), opaque(E.z)};
+        D3 const Bo{opaque(B.x), opaque(B.y), opaque(B.z)};
+        BorisFwd const b = borisFwd(p0o, Eo, Bo, m, q, dt, a.rc2);
+        lE = scale3(b.hq, l1);                   // p1 = pl + hq E
+        D3 lpm = l1;                             // pl = pm + pp x s
+        D3 const lpp = crossD(b.s, l1);
+        D3 const ls = crossD(l1, b.pp);
+        lpm = add3(lpm, add3(lpp, crossD(b.t, lpp)));  // pp = pm + pm x t
+        D3 const lt1 = crossD(lpp, b.pm);
+        double const tls = dotD(b.t, ls);              // s = 2 f t, f = 1/(1+|t|^2)
+        D3 const lt = add3(lt1, add3(scale3(2.0 * b.f, ls), scale3(-4.0 * b.f * b.f * tls, b.t)));
+        lB = scale3(b.beta / b.g, lt);                 // t = beta B / g
+        double const lg = -b.beta * dotD(B, lt) / (b.g * b.g);
+        lpm = axpy(lg * a.rc2 / (m * m * b.g), b.pm, lpm);  // g = sqrt(1 + |pm|^2 rc2/m^2)
+        lE = axpy(b.hq, lpm, lE);                      // pm = p0 + hq E
+        lp0 = lpm;
     }
-    for (int d = 0; d < 3; ++d) {
-        S.lX[d * n + idx] = lX0[d];
-        S.lP[d * n + idx] = lp0[d];
+
+    phaseBarrier();
+    // ---- reverse gather (weights recomputed, not kept from the forward gather) ----
+    double lX0[3] = {lX1[0], lX1[1], lX1[2]};
+    int const co[3] = {opaqueI(c[0]), opaqueI(c[1]), opaqueI(c[2])};
+    double const ro[3] = {opaque(r[0]), opaque(r[1]), opaque(r[2])};
+    if (lds) {
+        gatherAdj<0, 0, true>(a.E[0], sL, a.g, win, co, rel, ro, lE.x, lX0);
+        phaseBarrier();
+        gatherAdj<0, 1, true>(a.E[1], sL + win.vol, a.g, win, co, rel, ro, lE.y, lX0);
+        phaseBarrier();
+        gatherAdj<0, 2, true>(a.E[2], sL + 2 * win.vol, a.g, win, co, rel, ro, lE.z, lX0);
+        phaseBarrier();
+        gatherAdj<1, 0, true>(a.B[0], sL + 3 * win.vol, a.g, win, co, rel, ro, lB.x, lX0);
+        phaseBarrier();
+        gatherAdj<1, 1, true>(a.B[1], sL + 4 * win.vol, a.g, win, co, rel, ro, lB.y, lX0);
+        phaseBarrier();
+        gatherAdj<1, 2, true>(a.B[2], sL + 5 * win.vol, a.g, win, co, rel, ro, lB.z, lX0);
+    } else {
+        gatherAdj<0, 0, false>(a.E[0], a.lE[0], a.g, win, co, rel, ro, lE.x, lX0);
+        phaseBarrier();
+        gatherAdj<0, 1, false>(a.E[1], a.lE[1], a.g, win, co, rel, ro, lE.y, lX0);
+        phaseBarrier();
+        gatherAdj<0, 2, false>(a.E[2], a.lE[2], a.g, win, co, rel, ro, lE.z, lX0);
+        phaseBarrier();
+        gatherAdj<1, 0, false>(a.B[0], a.lB[0], a.g, win, co, rel, ro, lB.x, lX0);
+        phaseBarrier();
+        gatherAdj<1, 1, false>(a.B[1], a.lB[1], a.g, win, co, rel, ro, lB.y, lX0);
+        phaseBarrier();
+        gatherAdj<1, 2, false>(a.B[2], a.lB[2], a.g, win, co, rel, ro, lB.z, lX0);
+    }
+    S.lX[idx] = lX0[0];
+    S.lX[n + idx] = lX0[1];
+    S.lX[2 * n + idx] = lX0[2];
+    S.lP[idx] = lp0.x;
+    S.lP[n + idx] = lp0.y;
+    S.lP[2 * n + idx] = lp0.z;
+}
+
+
+// FP64 LDS window of the field adjoint scatter: 6 components of this many cells
+constexpr int kAdjLdsCells = 640;
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void kAdjParticles(AdjPartArgs a,
+                                                                                                 AdjState S) {
+    __shared__ double sL[6 * kAdjLdsCells];
+    __shared__ int sBox[9];
+    int64_t const idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool const active = idx < S.n;
+    int64_t const n = S.n;
+    int c[3] = {0, 0, 0};
+    if (active) {
+        c[0] = S.cell[idx];
+        c[1] = S.cell[n + idx];
+        c[2] = S.cell[2 * n + idx];
+    }
+    // interpolation nodes of a particle in cell c: c-1 .. c+1 per axis
+    int rel[3];
+    BlockWindow const win = blockWindow<1, 1>(a.g, active, c, rel, sBox, kAdjLdsCells);
+    if (win.lds) {
+        for (int e = threadIdx.x; e < 6 * win.vol; e += blockDim.x) sL[e] = 0.0;
+        __syncthreads();
+        if (active) adjointParticle(a, S, idx, c, win, rel, sL, windowHolds<1, 1>(win, rel));
+        __syncthreads();
+        for (int e = threadIdx.x; e < 6 * win.vol; e += blockDim.x) {
+            double const v = sL[e];
+            if (v != 0.0) {
+                int const plane = e / win.vol;
+                double* const dst = plane < 3 ? a.lE[plane] : a.lB[plane - 3];
+                unsafeAtomicAdd(dst + windowToGlobal(a.g, win, e - plane * win.vol), v);
+            }
+        }
     }
 }
 

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void kChargeDensity(SpeciesDev S, GridDev g, d
                     if (v == 0.0) continue;
                     int xx = c[0] + begin + ox;
                     xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
-                    atomicAdd(rho + (size_t)xx + (size_t)g.nx * ((size_t)yy + (size_t)g.ny * zz), v);
+                    unsafeAtomicAdd(rho + (size_t)xx + (size_t)g.nx * ((size_t)yy + (size_t)g.ny * zz), v);
                 }
             }
         }

@@ -1114,6 +1114,7 @@ int pic_adjoint_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* 
     if (r != PIC_OK) return r;
     if (n <= 0 || !cell || !pos || !mom || !w) return fail(PIC_E_INVALID, "particle arrays and n > 0 are required");
     if (c->cfg.solver != PIC_SOLVER_YEE) return fail(PIC_E_INVALID, "the differentiable form uses the Yee solver");
+    if (c->ncells > (size_t)INT32_MAX) return fail(PIC_E_INVALID, "the differentiable form needs fewer than 2^31 cells");
     const SpeciesMem& sm = c->sp[s];
     if (sm.desc.shape != PIC_SHAPE_CIC || sm.desc.pusher != PIC_PUSHER_BORIS)
         return fail(PIC_E_INVALID, "the differentiable form supports the CIC shape with the Boris pusher");
