@@ -25,10 +25,23 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per launch of the fused push/deposit kernel measured with
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch) on this
+# workload, scaled by what the counters read for a 1 GiB device copy
+# (FETCH_SIZE reads 0.5 GiB for it: factor 2.0; WRITE_SIZE 1 GiB: factor 1.0).
+# Command: rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -- python3 bench.py
+# --steps 1 --warmup 1 --no-cpu-baseline --no-adjoint --calibrate-copy
+MEASURED_TRAFFIC = {
+    ("kelvin_helmholtz", (256, 256, 256)): {"fetch_kib": 598069.4, "write_kib": 924442.5, "fetch_scale": 2.0,
+                                            "write_scale": 1.0},
+}
 BYTES_PER_PARTICLE = 55.0
 BYTES_PER_CELL_STEP = 96.0
 BYTES_PER_CELL_PUSH = 36.0
@@ -100,6 +113,41 @@ def adjoint_timing(steps=200, every=20, grid=(64, 64, 64), ppc=8):
             "forward_particle_steps_per_s": n * steps / t_fwd}
 
 
+def ensemble_timing(steps=10, warmup=2):
+    """Batched ensemble (64 two-stream boxes of 32..96 cells per axis, 2..16
+    ppc, TSC, Boris, Yee, seed 7): particle-steps/s of the autotuned batched
+    step replayed from graphs, against the same members advanced one at a time
+    with their own graph-replayed pic_step."""
+    from picongpu_amd import abi, setups
+
+    sts = setups.ensemble()
+    ens = abi.Ensemble([st.config() for st in sts])
+    for m, st in zip(ens.members, sts):
+        setups.init_device(m, st)
+    particles = sum(m.particle_count(0) for m in ens.members)
+    tuned_ms, taken = ens.autotune(trial_steps=2)
+    ens.step_graph(warmup)
+    t0 = time.perf_counter()
+    ens.step_graph(steps)
+    t_batch = time.perf_counter() - t0
+    for m in ens.members:
+        m.step_graph(warmup)
+    t0 = time.perf_counter()
+    for m in ens.members:
+        m.step_graph(steps)
+    t_seq = time.perf_counter() - t0
+    order, bpc = ens.schedule()
+    ens.close()
+    names = {0: "sim-major", 1: "interleaved", 2: "heavy-first"}
+    return {"workload": f"{len(sts)} two-stream boxes (32..96 cells per axis, 2..16 ppc, TSC, Boris, Yee, seed 7)",
+            "particles": particles, "cells": int(sum(int(np.prod(st.grid)) for st in sts)), "steps": steps,
+            "batched_ms_per_step": t_batch * 1e3 / steps, "one_at_a_time_ms_per_step": t_seq * 1e3 / steps,
+            "batched_particle_steps_per_s": particles * steps / t_batch,
+            "one_at_a_time_particle_steps_per_s": particles * steps / t_seq, "speedup": t_seq / t_batch,
+            "schedule": {"order": names.get(order, order), "blocks_per_cu": bpc, "autotune_ms_per_step": tuned_ms,
+                         "autotune_steps": taken}}
+
+
 def copy_calibration(nbytes=1 << 30):
     """One device-to-device copy of nbytes (a copy kernel of known size) so
     rocprofv3's FETCH_SIZE/WRITE_SIZE can be scaled to bytes."""
@@ -126,6 +174,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-grid", default="128,128,64")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -177,6 +226,9 @@ def main():
     adj = None
     if not args.no_adjoint:
         adj = adjoint_timing()
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_timing()
 
     baseline = None
     if not args.no_cpu_baseline:
@@ -184,6 +236,11 @@ def main():
             baseline = cpu_baseline(setup_fn, tuple(int(v) for v in args.cpu_grid.split(",")))
         except Exception as exc:  # the baseline is reported, never required
             baseline = {"error": str(exc)}
+
+    traffic = None
+    t = MEASURED_TRAFFIC.get((st.name, tuple(st.grid)))
+    if t is not None:
+        traffic = (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
 
     out = {
         "metric": "particle-steps/s (push+deposit+FDTD) on 1 GPU; adjoint/forward time ratio",
@@ -204,7 +261,7 @@ def main():
                    else "Vay", "solver": "Yee" if st.solver == 0 else "Lehe", "particles": n_particles,
                    "cells": ncells},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "kPushDeposit (fused gather+push+move+Esirkepov)",
                      "avg_launch_ms": push_ms / max(push_launches, 1), "launches": push_launches},
         "step_roofline": {"achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS,
@@ -212,6 +269,7 @@ def main():
                           "field_ms": field_ms / args.steps},
         "particles_after": counts,
         "adjoint": adj,
+        "ensemble": ens,
         "cpu_baseline": baseline,
     }
     print(json.dumps(out))

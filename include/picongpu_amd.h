@@ -130,7 +130,9 @@ typedef struct pic_constants {
 typedef struct pic_init_desc {
     uint64_t seed;
     int32_t ppc;              /* macro-particles per cell */
-    int32_t drift_band;       /* 0: uniform drift; 1: drift sign +1 for y in [ny/4,3ny/4), -1 outside */
+    int32_t drift_band;       /* 0: uniform drift; 1: drift sign +1 for y in [ny/4,3ny/4), -1 outside;
+                                 2: sign +1 for even, -1 for odd particle index k in the cell (two
+                                 counter-streaming beams) */
     double density_si;        /* physical density [1/m^3] */
     double u_drift[3];        /* gamma*beta of the drift */
     double u_th[3];           /* thermal spread of gamma*beta per component */
@@ -253,6 +255,44 @@ int pic_adjoint_forward(pic_ctx* ctx, const float* E0, const float* B0, int32_t 
  * (double, 3 planes of n); recomputes each checkpoint segment forward */
 int pic_adjoint_backward(pic_ctx* ctx, const double* dE, const double* dB, double* dE0, double* dB0, double* dX0,
                          double* dP0);
+
+/* --- batched ensembles -------------------------------------------------- */
+/* Many independent periodic simulations (each a pic_ctx, grids and particle
+ * counts may differ) advanced together: one launch per checkerboard phase of
+ * the fused push/deposit over every member's supercells, one launch each for
+ * the shift's insert and compact passes, the fused field step and the current
+ * reset, optionally replayed from a captured hipGraph. The reference runs one
+ * simulation per process; there is no counterpart interface. Members are
+ * created from the configurations and are then initialised, loaded, read and
+ * diagnosed through the single-simulation calls on pic_ensemble_member();
+ * stepping a member on its own is allowed between ensemble steps only in even
+ * numbers (the fused field step alternates two E buffers). Every member must
+ * use the same shape, pusher and solver, at most 32 frames per supercell. */
+typedef struct pic_ensemble pic_ensemble;
+
+/* order of the work items of a batched phase */
+#define PIC_ENSEMBLE_ORDER_SIM_MAJOR 0    /* member by member */
+#define PIC_ENSEMBLE_ORDER_INTERLEAVED 1  /* one supercell of every member in turn */
+#define PIC_ENSEMBLE_ORDER_HEAVY_FIRST 2  /* member by member, largest particle capacity first */
+
+int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out);
+int pic_ensemble_destroy(pic_ensemble* ens);
+int32_t pic_ensemble_size(const pic_ensemble* ens);
+/* the member context i (owned by the ensemble), or null with pic_last_error set */
+pic_ctx* pic_ensemble_member(pic_ensemble* ens, int32_t i);
+/* nsteps batched steps of every member; each member's results equal its own pic_step */
+int pic_ensemble_step(pic_ensemble* ens, int32_t nsteps);
+/* the same, replayed from captured graphs (one per E buffer parity) */
+int pic_ensemble_step_graph(pic_ensemble* ens, int32_t nsteps);
+/* schedule of the batched push: work-item order and workgroups per CU
+ * (0 = one workgroup per item; k > 0 = k*CUs persistent workgroups looping over items) */
+int pic_ensemble_set_schedule(pic_ensemble* ens, int32_t order, int32_t blocks_per_cu);
+int pic_ensemble_get_schedule(const pic_ensemble* ens, int32_t* order, int32_t* blocks_per_cu);
+/* times every (order, blocks per CU) candidate over trial_steps batched steps
+ * (after one warm-up step each) and keeps the fastest. The schedule does not
+ * change results, so these are ordinary steps of every member: steps_taken
+ * reports how many were run. best_ms_per_step receives the winner's time. */
+int pic_ensemble_autotune(pic_ensemble* ens, int32_t trial_steps, double* best_ms_per_step, int32_t* steps_taken);
 
 #ifdef __cplusplus
 }

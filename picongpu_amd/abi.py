@@ -133,6 +133,15 @@ SIGNATURES = {
     "pic_adjoint_read_particles": (C.c_int, [_P, _I32, _F, _F]),
     "pic_adjoint_forward": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P]),
     "pic_adjoint_backward": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "pic_ensemble_create": (C.c_int, [C.POINTER(Config), C.c_int32, C.POINTER(_P)]),
+    "pic_ensemble_destroy": (C.c_int, [_P]),
+    "pic_ensemble_size": (C.c_int32, [_P]),
+    "pic_ensemble_member": (_P, [_P, C.c_int32]),
+    "pic_ensemble_step": (C.c_int, [_P, C.c_int32]),
+    "pic_ensemble_step_graph": (C.c_int, [_P, C.c_int32]),
+    "pic_ensemble_set_schedule": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "pic_ensemble_get_schedule": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "pic_ensemble_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
 }
 
 _lib = None
@@ -210,11 +219,15 @@ def constants_from_config(cfg: Config) -> Constants:
 class Simulation:
     """One periodic simulation box on the GPU (a pic_ctx)."""
 
-    def __init__(self, cfg: Config):
+    def __init__(self, cfg: Config, handle=None):
         self.lib = load_library()
         self.cfg = cfg
-        h = C.c_void_p()
-        check(self.lib.pic_create(C.byref(cfg), C.byref(h)), "pic_create")
+        self.owned = handle is None
+        if handle is None:
+            h = C.c_void_p()
+            check(self.lib.pic_create(C.byref(cfg), C.byref(h)), "pic_create")
+        else:
+            h = C.c_void_p(handle)
         self.h = h
         self.grid = tuple(cfg.grid)
         self.ncells = int(np.prod(self.grid))
@@ -222,7 +235,8 @@ class Simulation:
 
     def close(self):
         if self.h:
-            self.lib.pic_destroy(self.h)
+            if self.owned:
+                self.lib.pic_destroy(self.h)
             self.h = None
 
     def __enter__(self):
@@ -411,3 +425,60 @@ class Simulation:
     def adjoint_backward(self, dE_addr, dB_addr, dE0_addr, dB0_addr, dX0_addr=None, dP0_addr=None):
         check(self.lib.pic_adjoint_backward(self.h, dE_addr, dB_addr, dE0_addr, dB0_addr, dX0_addr, dP0_addr),
               "pic_adjoint_backward")
+
+
+ENSEMBLE_ORDER_SIM_MAJOR, ENSEMBLE_ORDER_INTERLEAVED, ENSEMBLE_ORDER_HEAVY_FIRST = 0, 1, 2
+
+
+class Ensemble:
+    """Independent simulations advanced by batched launches (a pic_ensemble)."""
+
+    def __init__(self, cfgs):
+        self.lib = load_library()
+        n = len(cfgs)
+        arr = (Config * n)(*cfgs)
+        h = C.c_void_p()
+        check(self.lib.pic_ensemble_create(arr, n, C.byref(h)), "pic_ensemble_create")
+        self.h = h
+        self.members = []
+        for i in range(n):
+            m = self.lib.pic_ensemble_member(h, i)
+            if not m:
+                raise PicError(-1, "pic_ensemble_member", self.lib.pic_last_error().decode())
+            self.members.append(Simulation(cfgs[i], handle=m))
+
+    def __len__(self):
+        return len(self.members)
+
+    def close(self):
+        if self.h:
+            for m in self.members:
+                m.h = None
+            self.lib.pic_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        check(self.lib.pic_ensemble_step(self.h, int(n)), "pic_ensemble_step")
+
+    def step_graph(self, n=1):
+        check(self.lib.pic_ensemble_step_graph(self.h, int(n)), "pic_ensemble_step_graph")
+
+    def set_schedule(self, order, blocks_per_cu):
+        check(self.lib.pic_ensemble_set_schedule(self.h, int(order), int(blocks_per_cu)), "pic_ensemble_set_schedule")
+
+    def schedule(self):
+        o, g = C.c_int32(), C.c_int32()
+        check(self.lib.pic_ensemble_get_schedule(self.h, C.byref(o), C.byref(g)), "pic_ensemble_get_schedule")
+        return o.value, g.value
+
+    def autotune(self, trial_steps=2):
+        ms, n = C.c_double(), C.c_int32()
+        check(self.lib.pic_ensemble_autotune(self.h, int(trial_steps), C.byref(ms), C.byref(n)),
+              "pic_ensemble_autotune")
+        return ms.value, n.value

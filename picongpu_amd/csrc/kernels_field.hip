@@ -106,6 +106,129 @@ __global__ __launch_bounds__(256) void kEFull(FieldArgs a) {
     a.E[2][i] = a.E[2][i] + cz * a.c2 * a.dt;
 }
 
+// ---------------------------------------------------------------------------
+// Fused field step (the same per-cell float operations as kBHalf, kEFull,
+// kAddCurrent, kBHalf in that order, so the results are bit-identical).
+// kFieldBeforeAdd: per 8x8x4 tile, B' = B - curlE(E)*0.5*dt on the tile and
+// its lower halo (kept in LDS only), then E'' = (E + curlB(B')*c2*dt) +
+// coeff*J, written to Enew. kFieldAfter: B'' = (B - curlE(E)*0.5*dt) -
+// curlE(Enew)*0.5*dt per cell, in place. E is read-only in both kernels and
+// B is not written by the first, so no block reads a value another block of
+// the same launch has overwritten; the caller then swaps E and Enew.
+// ---------------------------------------------------------------------------
+constexpr int kFTX = 8, kFTY = 8, kFTZ = 4;
+constexpr int kFHX = kFTX + 1, kFHY = kFTY + 1, kFHZ = kFTZ + 1;
+constexpr int kFHV = kFHX * kFHY * kFHZ;
+
+template <int SOLVER>
+__device__ __forceinline__ void curlE(const FieldArgs& a, float* const* E, const Idx& I, float& cx, float& cy,
+                                      float& cz) {
+    cx = derivE<SOLVER>(E[2], I, 1, a) - derivE<SOLVER>(E[1], I, 2, a);
+    cy = derivE<SOLVER>(E[0], I, 2, a) - derivE<SOLVER>(E[2], I, 0, a);
+    cz = derivE<SOLVER>(E[1], I, 0, a) - derivE<SOLVER>(E[0], I, 1, a);
+}
+
+// tile b of the grid: every thread of the 256-thread workgroup calls it
+template <int SOLVER>
+__device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* const* Enew, int b,
+                                                   float (*sB)[kFHV]) {
+    int const ntx = a.g.nx / kFTX, nty = a.g.ny / kFTY;
+    int const x0 = (b % ntx) * kFTX, y0 = ((b / ntx) % nty) * kFTY, z0 = (b / (ntx * nty)) * kFTZ;
+    for (int t = threadIdx.x; t < kFHV; t += blockDim.x) {
+        int const hx = t % kFHX, hy = (t / kFHX) % kFHY, hz = t / (kFHX * kFHY);
+        Idx const I0{a.g.nx, a.g.ny, a.g.nz, x0, y0, z0};
+        size_t const i = I0.at(hx - 1, hy - 1, hz - 1);
+        Idx const I{a.g.nx, a.g.ny, a.g.nz, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny),
+                    (int)(i / ((size_t)a.g.nx * a.g.ny))};
+        float cx, cy, cz;
+        curlE<SOLVER>(a, a.E, I, cx, cy, cz);
+        sB[0][t] = a.B[0][i] - cx * 0.5f * a.dt;
+        sB[1][t] = a.B[1][i] - cy * 0.5f * a.dt;
+        sB[2][t] = a.B[2][i] - cz * 0.5f * a.dt;
+    }
+    __syncthreads();
+    int const lx = threadIdx.x % kFTX, ly = (threadIdx.x / kFTX) % kFTY, lz = threadIdx.x / (kFTX * kFTY);
+    size_t const i = (size_t)(x0 + lx) + (size_t)a.g.nx * ((size_t)(y0 + ly) + (size_t)a.g.ny * (size_t)(z0 + lz));
+    int const h = (lx + 1) + kFHX * ((ly + 1) + kFHY * (lz + 1));
+    constexpr int SY = kFHX, SZ = kFHX * kFHY;
+    // backward differences of B' (BackwardDerivative.hpp:44-65), curl as kEFull
+    float const cx = (sB[2][h] - sB[2][h - SY]) / a.cell[1] - (sB[1][h] - sB[1][h - SZ]) / a.cell[2];
+    float const cy = (sB[0][h] - sB[0][h - SZ]) / a.cell[2] - (sB[2][h] - sB[2][h - 1]) / a.cell[0];
+    float const cz = (sB[1][h] - sB[1][h - 1]) / a.cell[0] - (sB[0][h] - sB[0][h - SY]) / a.cell[1];
+    float e0 = a.E[0][i] + cx * a.c2 * a.dt;
+    float e1 = a.E[1][i] + cy * a.c2 * a.dt;
+    float e2 = a.E[2][i] + cz * a.c2 * a.dt;
+    e0 = e0 + a.coeff * a.J[0][i];
+    e1 = e1 + a.coeff * a.J[1][i];
+    e2 = e2 + a.coeff * a.J[2][i];
+    Enew[0][i] = e0;
+    Enew[1][i] = e1;
+    Enew[2][i] = e2;
+}
+
+template <int SOLVER>
+__device__ __forceinline__ void fieldAfterCell(const FieldArgs& a, float* const* Enew, size_t i) {
+    Idx I{a.g.nx, a.g.ny, a.g.nz, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    float cx, cy, cz;
+    curlE<SOLVER>(a, a.E, I, cx, cy, cz);
+    float const b0 = a.B[0][i] - cx * 0.5f * a.dt;
+    float const b1 = a.B[1][i] - cy * 0.5f * a.dt;
+    float const b2 = a.B[2][i] - cz * 0.5f * a.dt;
+    curlE<SOLVER>(a, Enew, I, cx, cy, cz);
+    a.B[0][i] = b0 - cx * 0.5f * a.dt;
+    a.B[1][i] = b1 - cy * 0.5f * a.dt;
+    a.B[2][i] = b2 - cz * 0.5f * a.dt;
+}
+
+template <int SOLVER>
+__global__ __launch_bounds__(256) void kFieldBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2) {
+    __shared__ float sB[3][kFHV];
+    float* const En[3] = {Enew0, Enew1, Enew2};
+    fieldBeforeAddTile<SOLVER>(a, En, blockIdx.x, sB);
+}
+
+template <int SOLVER>
+__global__ __launch_bounds__(256) void kFieldAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float* const En[3] = {Enew0, Enew1, Enew2};
+    fieldAfterCell<SOLVER>(a, En, i);
+}
+
+// batched forms for ensembles
+template <int SOLVER>
+__global__ __launch_bounds__(256) void kFieldBeforeAddBatch(const FieldBatchEntry* __restrict__ table,
+                                                            const int2* __restrict__ tiles) {
+    __shared__ float sB[3][kFHV];
+    int2 const it = tiles[blockIdx.x];
+    const FieldBatchEntry& e = table[it.x];
+    fieldBeforeAddTile<SOLVER>(e.a, e.Enew, it.y, sB);
+}
+
+template <int SOLVER>
+__global__ __launch_bounds__(256) void kFieldAfterBatch(const FieldBatchEntry* __restrict__ table,
+                                                        const int2* __restrict__ blocks) {
+    int2 const it = blocks[blockIdx.x];
+    const FieldBatchEntry& e = table[it.x];
+    size_t const n = (size_t)e.a.g.nx * e.a.g.ny * e.a.g.nz;
+    size_t const i = (size_t)it.y * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fieldAfterCell<SOLVER>(e.a, e.Enew, i);
+}
+
+__global__ __launch_bounds__(256) void kCurrentResetBatch(const FieldBatchEntry* __restrict__ table,
+                                                          const int2* __restrict__ blocks) {
+    int2 const it = blocks[blockIdx.x];
+    const FieldBatchEntry& e = table[it.x];
+    size_t const n = (size_t)e.a.g.nx * e.a.g.ny * e.a.g.nz;
+    size_t const i = (size_t)it.y * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    e.a.J[0][i] = 0.0f;
+    e.a.J[1][i] = 0.0f;
+    e.a.J[2][i] = 0.0f;
+}
+
 __global__ __launch_bounds__(256) void kAddCurrent(FieldArgs a) {
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -175,6 +298,40 @@ hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s) {
 hipError_t launchEFull(const FieldArgs& a, hipStream_t s) {
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     hipLaunchKernelGGL(kEFull, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    int const tiles = (a.g.nx / kFTX) * (a.g.ny / kFTY) * (a.g.nz / kFTZ);
+    if (solver == PIC_SOLVER_LEHE) {
+        hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_LEHE>, dim3(blocksFor(n)), dim3(256), 0, s, a, Enew[0], Enew[1],
+                           Enew[2]);
+    } else {
+        hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_YEE>, dim3(blocksFor(n)), dim3(256), 0, s, a, Enew[0], Enew[1],
+                           Enew[2]);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchFieldStepBatch(int solver, const FieldBatchEntry* table, const int2* tiles, int ntiles,
+                                const int2* blocks, int nblocks, hipStream_t s) {
+    if (ntiles <= 0 || nblocks <= 0) return hipSuccess;
+    if (solver == PIC_SOLVER_LEHE) {
+        hipLaunchKernelGGL(kFieldBeforeAddBatch<PIC_SOLVER_LEHE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
+        hipLaunchKernelGGL(kFieldAfterBatch<PIC_SOLVER_LEHE>, dim3(nblocks), dim3(256), 0, s, table, blocks);
+    } else {
+        hipLaunchKernelGGL(kFieldBeforeAddBatch<PIC_SOLVER_YEE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
+        hipLaunchKernelGGL(kFieldAfterBatch<PIC_SOLVER_YEE>, dim3(nblocks), dim3(256), 0, s, table, blocks);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchCurrentResetBatch(const FieldBatchEntry* table, const int2* blocks, int nblocks, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kCurrentResetBatch, dim3(nblocks), dim3(256), 0, s, table, blocks);
     return hipGetLastError();
 }
 

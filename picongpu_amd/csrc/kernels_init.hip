@@ -3,8 +3,10 @@
 // supercell gets ppc particles per cell, placed round-robin over its 256
 // cells (slot = k*256 + localCellIdx), at uniform random in-cell positions
 // (startPosition/RandomImpl.hpp:46-56) with weighting density*V/ppc
-// (WeightMacroParticles.hpp:30-42) and momentum mass*(u_drift + u_th*N(0,1))
-// (manipulators/unary/Drift.hpp:53-76, Temperature.hpp:60-84).
+// (WeightMacroParticles.hpp:30-42) and momentum mass*(sign*u_drift + u_th*N(0,1))
+// (manipulators/unary/Drift.hpp:53-76, Temperature.hpp:60-84); the drift sign
+// is +1, or set by the cell's y band (shear flow), or alternates with the
+// particle's index in its cell (two counter-streaming beams).
 #include "pic_kernels.hpp"
 #include "pic_rng.hpp"
 
@@ -37,7 +39,8 @@ __global__ __launch_bounds__(256) void kInit(InitArgs a) {
         normalPair(a.seed, spc, cell, (uint32_t)k, 0u, &n0, &n1);
         normalPair(a.seed, spc, cell, (uint32_t)k, 1u, &n2, &n3);
         double sign = 1.0;
-        if (a.drift_band) sign = (gy >= a.g.ny / 4 && gy < 3 * a.g.ny / 4) ? 1.0 : -1.0;
+        if (a.drift_band == 1) sign = (gy >= a.g.ny / 4 && gy < 3 * a.g.ny / 4) ? 1.0 : -1.0;
+        if (a.drift_band == 2) sign = (k & 1) ? -1.0 : 1.0;  // counter-streaming beams
         double const m = (double)a.mass;
         a.sp.mom[0][g] = (float)(m * (sign * a.u_drift[0] + a.u_th[0] * n0));
         a.sp.mom[1][g] = (float)(m * (sign * a.u_drift[1] + a.u_th[1] * n1));

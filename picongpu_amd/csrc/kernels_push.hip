@@ -23,8 +23,30 @@ __device__ __forceinline__ int wrapIdx(int v, int n) {
 constexpr int kPushThreads = kPushThreadsPerBlock;
 constexpr int kPushWaves = kPushThreads / 64;
 
+// LDS bytes of one supercell's work: the E/B tile (or the epilogue's
+// scratch), then the int64 J tile
+template <int SUPP, int MODE>
+struct PushLds {
+    static constexpr int GLO = SUPP / 2;
+    static constexpr int GUP = (SUPP + 1) / 2;
+    static constexpr int GV = (SCX + GLO + GUP) * (SCY + GLO + GUP) * (SCZ + GLO + GUP);
+    static constexpr int DLO = Esirkepov<SUPP>::lowerMargin + (MODE == 1 ? 1 : 0);
+    static constexpr int DUP = Esirkepov<SUPP>::upperMargin + (MODE == 1 ? 1 : 0);
+    static constexpr int JV = (SCX + DLO + DUP) * (SCY + DLO + DUP) * (SCZ + DLO + DUP);
+    static constexpr int EB_ELEMS = MODE == 2 ? 1 : 6 * GV;
+    static constexpr int J_ELEMS = MODE == 0 ? 1 : 3 * JV;
+    static constexpr int EB_BYTES = ((EB_ELEMS * 4 + 15) / 16) * 16;
+    static constexpr int EPI_BYTES =
+        (kPushMaxIters * kPushWaves + 1) * 4 + kPushWaves * kNumDirs * 4 + (int)sizeof(GroupScratch);
+    // 16-byte multiple: the int64 J tile that follows needs 8-byte aligned LDS addresses
+    static constexpr int LO_BYTES = (((EB_BYTES > EPI_BYTES ? EB_BYTES : EPI_BYTES) + 15) / 16) * 16;
+    static constexpr int BYTES = LO_BYTES + J_ELEMS * 8;
+};
+
+// The work of one supercell `sc` of the simulation described by `a`; every
+// thread of the workgroup calls it.
 template <int SUPP, int PUSHER, int MODE>
-__global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
+__device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigned char* smem) {
     constexpr int GLO = SUPP / 2;          // FieldToParticleInterpolation.hpp:49
     constexpr int GUP = (SUPP + 1) / 2;    // FieldToParticleInterpolation.hpp:50
     constexpr int GX = SCX + GLO + GUP, GY = SCY + GLO + GUP, GZ = SCZ + GLO + GUP;
@@ -40,19 +62,15 @@ __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
 
     // One LDS buffer: the E/B tile, then the J tile. After the particle loop
     // the E/B tile is dead and its space holds the epilogue's scratch.
-    constexpr int EB_BYTES = ((EB_ELEMS * 4 + 15) / 16) * 16;
-    constexpr int EPI_BYTES = (kPushMaxIters * kPushWaves + 1) * 4 + kPushWaves * kNumDirs * 4 + (int)sizeof(GroupScratch);
-    // 16-byte multiple: the int64 J tile that follows needs 8-byte aligned LDS addresses
-    constexpr int LO_BYTES = (((EB_BYTES > EPI_BYTES ? EB_BYTES : EPI_BYTES) + 15) / 16) * 16;
+    constexpr int LO_BYTES = PushLds<SUPP, MODE>::LO_BYTES;
     static_assert(LO_BYTES % 16 == 0, "J tile must be 16-byte aligned");
-    __shared__ __align__(16) unsigned char smem[LO_BYTES + J_ELEMS * 8];
+    static_assert(EB_ELEMS == PushLds<SUPP, MODE>::EB_ELEMS && J_ELEMS == PushLds<SUPP, MODE>::J_ELEMS, "LDS plan");
     float* const sEB = reinterpret_cast<float*>(smem);
     unsigned long long* const sJ = reinterpret_cast<unsigned long long*>(smem + LO_BYTES);
     uint32_t* const sPre = reinterpret_cast<uint32_t*>(smem);
     uint32_t* const sWaveCnt = sPre + kPushMaxIters * kPushWaves + 1;
     GroupScratch& sGroup = *reinterpret_cast<GroupScratch*>(sWaveCnt + kPushWaves * kNumDirs);
 
-    int const sc = a.scList[blockIdx.x];
     int const sx = sc % a.g.nscx;
     int const sy = (sc / a.g.nscx) % a.g.nscy;
     int const sz = sc / (a.g.nscx * a.g.nscy);
@@ -214,6 +232,49 @@ __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
             groupLeavers<kPushWaves>(S, sc, L, sGroup, sWaveCnt);
         }
     }
+}
+
+template <int SUPP, int PUSHER, int MODE>
+__global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
+    __shared__ __align__(16) unsigned char smem[PushLds<SUPP, MODE>::BYTES];
+    pushDepositSC<SUPP, PUSHER, MODE>(a, a.scList[blockIdx.x], smem);
+}
+
+// Batched form for ensembles: work items (simulation, supercell) of one
+// checkerboard phase of every simulation in one launch; a workgroup takes
+// items blockIdx.x, blockIdx.x + gridDim.x, ... so a grid smaller than the
+// item count runs persistently.
+template <int SUPP, int PUSHER, int MODE>
+__global__ __launch_bounds__(kPushThreads) void kPushDepositBatch(const PushArgs* __restrict__ table,
+                                                                  const int2* __restrict__ work, int nwork) {
+    __shared__ __align__(16) unsigned char smem[PushLds<SUPP, MODE>::BYTES];
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+        int2 const it = work[w];
+        pushDepositSC<SUPP, PUSHER, MODE>(table[it.x], it.y, smem);
+        __syncthreads();  // the next item reuses the LDS
+    }
+}
+
+hipError_t launchPushBatch(int supp, int pusher, const PushArgs* table, const int2* work, int nwork, int nblocks,
+                           hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    if (nblocks <= 0 || nblocks > nwork) nblocks = nwork;
+    if (pusher == PIC_PUSHER_VAY) {
+        switch (supp) {
+            case 2: hipLaunchKernelGGL((kPushDepositBatch<2, PIC_PUSHER_VAY, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
+            case 3: hipLaunchKernelGGL((kPushDepositBatch<3, PIC_PUSHER_VAY, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
+            case 4: hipLaunchKernelGGL((kPushDepositBatch<4, PIC_PUSHER_VAY, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (supp) {
+            case 2: hipLaunchKernelGGL((kPushDepositBatch<2, PIC_PUSHER_BORIS, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
+            case 3: hipLaunchKernelGGL((kPushDepositBatch<3, PIC_PUSHER_BORIS, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
+            case 4: hipLaunchKernelGGL((kPushDepositBatch<4, PIC_PUSHER_BORIS, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
+            default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
 }
 
 template <int SUPP, int PUSHER, int MODE>

@@ -67,10 +67,9 @@ __device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(kThreads) void kInsert(SpeciesDev S, GridDev g, int nsc, uint32_t* errFlags) {
+// insert for destination supercell d; called by one whole wave
+__device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, int d, uint32_t* errFlags) {
     int const lane = threadIdx.x & 63;
-    int const d = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
-    if (d >= nsc) return;  // whole wave
     int const dx = d % g.nscx;
     int const dy = (d / g.nscx) % g.nscy;
     int const dz = d / (g.nscx * g.nscy);
@@ -135,10 +134,9 @@ __global__ __launch_bounds__(kThreads) void kInsert(SpeciesDev S, GridDev g, int
     }
 }
 
-__global__ __launch_bounds__(kThreads) void kCompact(SpeciesDev S, int nsc) {
+// compaction of supercell d; called by one whole wave
+__device__ __forceinline__ void compactSC(const SpeciesDev& S, int d) {
     int const lane = threadIdx.x & 63;
-    int const d = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
-    if (d >= nsc) return;  // whole wave
     size_t const base = (size_t)d * (size_t)S.cap;
     uint32_t const n = S.count[d];
     uint32_t const tot = S.occ[d];
@@ -183,6 +181,47 @@ __global__ __launch_bounds__(kThreads) void kCompact(SpeciesDev S, int nsc) {
     // gaps are below nn; the vacated tail [nn, tot) is cleared
     for (uint32_t i = nn + lane; i < tot; i += 64) S.mask[base + i] = 0;
     if (lane == 0) S.count[d] = nn;
+}
+
+__global__ __launch_bounds__(kThreads) void kInsert(SpeciesDev S, GridDev g, int nsc, uint32_t* errFlags) {
+    int const d = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
+    if (d >= nsc) return;  // whole wave
+    insertSC(S, g, d, errFlags);
+}
+
+__global__ __launch_bounds__(kThreads) void kCompact(SpeciesDev S, int nsc) {
+    int const d = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
+    if (d >= nsc) return;  // whole wave
+    compactSC(S, d);
+}
+
+// Batched forms for ensembles: work items (table entry, supercell), one wave
+// per item; the table holds one entry per (simulation, species).
+__global__ __launch_bounds__(kThreads) void kInsertBatch(const ShiftEntry* __restrict__ table,
+                                                        const int2* __restrict__ work, int nwork) {
+    int const w = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
+    if (w >= nwork) return;  // whole wave
+    int2 const it = work[w];
+    const ShiftEntry& e = table[it.x];
+    insertSC(e.S, e.g, it.y, e.errFlags);
+}
+
+__global__ __launch_bounds__(kThreads) void kCompactBatch(const ShiftEntry* __restrict__ table,
+                                                         const int2* __restrict__ work, int nwork) {
+    int const w = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
+    if (w >= nwork) return;  // whole wave
+    int2 const it = work[w];
+    compactSC(table[it.x].S, it.y);
+}
+
+hipError_t launchShiftBatch(const ShiftEntry* table, const int2* work, int nwork, hipStream_t s) {
+    if (nwork <= 0) return hipSuccess;
+    int const blocks = (nwork + kScPerBlock - 1) / kScPerBlock;
+    hipLaunchKernelGGL(kInsertBatch, dim3(blocks), dim3(kThreads), 0, s, table, work, nwork);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kCompactBatch, dim3(blocks), dim3(kThreads), 0, s, table, work, nwork);
+    return hipGetLastError();
 }
 
 hipError_t launchCollect(const GridDev& g, const SpeciesDev& sp, hipStream_t s) {

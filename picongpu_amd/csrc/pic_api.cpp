@@ -151,6 +151,7 @@ struct PhasePlan {
     int ncol[3] = {0, 0, 0};
     std::vector<int> offset;  // per phase into list
     std::vector<int> count;   // per phase
+    std::vector<int> host;    // the list on the host, launch order per phase
     int32_t* dList = nullptr;
 };
 
@@ -206,6 +207,8 @@ struct pic_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     float* E[3] = {nullptr, nullptr, nullptr};
+    float* Ealt[3] = {nullptr, nullptr, nullptr};  // receives the new E of the fused field step
+    int eParity = 0;                               // flips with every swap of E and Ealt
     float* B[3] = {nullptr, nullptr, nullptr};
     float* J[3] = {nullptr, nullptr, nullptr};
     std::vector<SpeciesMem> sp;
@@ -225,10 +228,10 @@ struct pic_ctx {
     double totalMs[PIC_TIMER_COUNT] = {0, 0, 0};
     int64_t launches[PIC_TIMER_COUNT] = {0, 0, 0};
     std::vector<void*> allocations;
-    // captured step for pic_step_graph
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
-    int graphJexp = 0;
+    // captured steps for pic_step_graph, one per E buffer parity
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graphExec[2] = {nullptr, nullptr};
+    int graphJexp[2] = {0, 0};
     // differentiable form
     AdjointMem adj;
 };
@@ -331,6 +334,7 @@ int buildPlan(pic_ctx* c, int supp, bool fused, PhasePlan& plan) {
         plan.count.push_back(n);
         list.insert(list.end(), order.begin(), order.end());
     }
+    plan.host = list;
     if (devAlloc(c, &plan.dList, list.size()) != PIC_OK) return PIC_E_HIP;
     PIC_HIP(hipMemcpyAsync(plan.dList, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
@@ -525,7 +529,8 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(fail(PIC_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
     for (int d = 0; d < 3; ++d) {
-        if (devAlloc(c, &c->E[d], c->ncells) || devAlloc(c, &c->B[d], c->ncells) || devAlloc(c, &c->J[d], c->ncells))
+        if (devAlloc(c, &c->E[d], c->ncells) || devAlloc(c, &c->B[d], c->ncells) || devAlloc(c, &c->J[d], c->ncells) ||
+            devAlloc(c, &c->Ealt[d], c->ncells))
             return bail(PIC_E_HIP);
     }
     if (devAlloc(c, &c->errFlags, 1)) return bail(PIC_E_HIP);
@@ -575,8 +580,10 @@ int pic_destroy(pic_ctx* c) {
             (void)hipEventDestroy(pr.second);
         }
     for (hipEvent_t e : c->eventPool) (void)hipEventDestroy(e);
-    if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (int p = 0; p < 2; ++p) {
+        if (c->graphExec[p]) (void)hipGraphExecDestroy(c->graphExec[p]);
+        if (c->graph[p]) (void)hipGraphDestroy(c->graph[p]);
+    }
     for (void* p : c->allocations) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -883,17 +890,43 @@ int pic_fdtd_after_current(pic_ctx* c) {
     return PIC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// update_beforeCurrent + addCurrent + update_afterCurrent as the two fused
+// kernels; the new E lands in Ealt and the buffers swap
+int fieldStep(pic_ctx* c) {
+    pic::FieldArgs a = makeFieldArgs(c);
+    {
+        TimerScope ts(c, PIC_TIMER_FIELD);
+        PIC_HIP(pic::launchFieldStep(c->cfg.solver, a, c->Ealt, c->stream));
+    }
+    for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
+    c->eParity ^= 1;
+    return PIC_OK;
+}
+
+// one full step in the reference stage order, push and deposit fused, the
+// field stages fused
+int oneStep(pic_ctx* c) {
+    int r;
+    if ((r = pic_current_reset(c)) != PIC_OK) return r;
+    if ((r = pic_push(c, -1, 1)) != PIC_OK) return r;
+    if ((r = pic_shift(c, -1)) != PIC_OK) return r;
+    return fieldStep(c);
+}
+
+}  // namespace
+
+extern "C" {
+
 int pic_step(pic_ctx* c, int32_t nsteps) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
     for (int32_t i = 0; i < nsteps; ++i) {
-        int r;
-        if ((r = pic_current_reset(c)) != PIC_OK) return r;
-        if ((r = pic_push(c, -1, 1)) != PIC_OK) return r;
-        if ((r = pic_shift(c, -1)) != PIC_OK) return r;
-        if ((r = pic_fdtd_before_current(c)) != PIC_OK) return r;
-        if ((r = pic_add_current(c)) != PIC_OK) return r;
-        if ((r = pic_fdtd_after_current(c)) != PIC_OK) return r;
+        int const r = oneStep(c);
+        if (r != PIC_OK) return r;
     }
     return checkErrors(c);
 }
@@ -902,39 +935,42 @@ int pic_step_graph(pic_ctx* c, int32_t nsteps) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
     if (c->timing) return fail(PIC_E_STATE, "pic_step_graph cannot run with kernel timing enabled");
-    if (c->graphExec && c->graphJexp != c->jexp) {
-        (void)hipGraphExecDestroy(c->graphExec);
-        (void)hipGraphDestroy(c->graph);
-        c->graphExec = nullptr;
-        c->graph = nullptr;
-    }
-    if (!c->graphExec) {
-        // every launch plan must exist before capture (building one copies and synchronises)
-        for (const SpeciesMem& m : c->sp) {
-            if (m.desc.pusher == PIC_PUSHER_NONE) continue;
-            PhasePlan* p = nullptr;
-            int const r = getPlan(c, supportOf(m.desc.shape), true, &p);
-            if (r != PIC_OK) return r;
+    for (int p = 0; p < 2; ++p) {
+        if (c->graphExec[p] && c->graphJexp[p] != c->jexp) {
+            (void)hipGraphExecDestroy(c->graphExec[p]);
+            (void)hipGraphDestroy(c->graph[p]);
+            c->graphExec[p] = nullptr;
+            c->graph[p] = nullptr;
         }
-        PIC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int r = pic_current_reset(c);
-        if (r == PIC_OK) r = pic_push(c, -1, 1);
-        if (r == PIC_OK) r = pic_shift(c, -1);
-        if (r == PIC_OK) r = pic_fdtd_before_current(c);
-        if (r == PIC_OK) r = pic_add_current(c);
-        if (r == PIC_OK) r = pic_fdtd_after_current(c);
-        hipGraph_t g = nullptr;
-        hipError_t const e = hipStreamEndCapture(c->stream, &g);
-        if (r != PIC_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return r;
-        }
-        if (e != hipSuccess) return fail(PIC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        c->graph = g;
-        PIC_HIP(hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
-        c->graphJexp = c->jexp;
     }
-    for (int32_t i = 0; i < nsteps; ++i) PIC_HIP(hipGraphLaunch(c->graphExec, c->stream));
+    for (int32_t i = 0; i < nsteps; ++i) {
+        int const p = c->eParity;
+        if (!c->graphExec[p]) {
+            // every launch plan must exist before capture (building one copies and synchronises)
+            for (const SpeciesMem& m : c->sp) {
+                if (m.desc.pusher == PIC_PUSHER_NONE) continue;
+                PhasePlan* plan = nullptr;
+                int const r = getPlan(c, supportOf(m.desc.shape), true, &plan);
+                if (r != PIC_OK) return r;
+            }
+            PIC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int const r = oneStep(c);  // records the step and swaps E/Ealt as its replay will
+            hipGraph_t g = nullptr;
+            hipError_t const e = hipStreamEndCapture(c->stream, &g);
+            if (r != PIC_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return r;
+            }
+            if (e != hipSuccess) return fail(PIC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            c->graph[p] = g;
+            PIC_HIP(hipGraphInstantiate(&c->graphExec[p], c->graph[p], nullptr, nullptr, 0));
+            c->graphJexp[p] = c->jexp;
+        } else {
+            for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
+            c->eParity ^= 1;
+        }
+        PIC_HIP(hipGraphLaunch(c->graphExec[p], c->stream));
+    }
     for (SpeciesMem& m : c->sp) m.listsValid = false;
     return checkErrors(c);
 }
@@ -1291,6 +1327,436 @@ int pic_adjoint_backward(pic_ctx* c, const double* dE, const double* dB, double*
     if (dX0) PIC_HIP(hipMemcpyAsync(dX0, m.lX, np * 8, hipMemcpyDeviceToDevice, c->stream));
     if (dP0) PIC_HIP(hipMemcpyAsync(dP0, m.lP, np * 8, hipMemcpyDeviceToDevice, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched ensembles of independent simulations
+// ---------------------------------------------------------------------------
+struct pic_ensemble {
+    std::vector<pic_ctx*> sims;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int supp = 0, pusher = 0, solver = 0;
+    int parity = 0;  // every member's E/Ealt parity
+    // device tables; PushArgs and field entries per parity (E and Ealt swap)
+    pic::PushArgs* dPush[2] = {nullptr, nullptr};
+    pic::FieldBatchEntry* dField[2] = {nullptr, nullptr};
+    pic::ShiftEntry* dShift = nullptr;
+    int nShiftEntries = 0;
+    // work lists
+    std::vector<int> phaseOffset, phaseCount;
+    int2* dPhaseWork = nullptr;
+    int2* dShiftWork = nullptr;
+    int nShiftWork = 0;
+    int2* dTiles = nullptr;
+    int nTiles = 0;
+    int2* dBlocks = nullptr;
+    int nBlocks = 0;
+    // schedule
+    int order = 1;         // PIC_ENSEMBLE_ORDER_*
+    int blocksPerCu = 0;   // workgroups per CU of the batched push (0: one per work item)
+    int cus = 256;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graphExec[2] = {nullptr, nullptr};
+    std::vector<void*> allocations;
+};
+
+namespace {
+
+template <typename T>
+int ensAlloc(pic_ensemble* e, T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t const r = hipMalloc(&q, std::max<size_t>(n * sizeof(T), 16));
+    if (r != hipSuccess) return fail(PIC_E_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(r));
+    e->allocations.push_back(q);
+    *p = static_cast<T*>(q);
+    return PIC_OK;
+}
+
+void ensDropGraphs(pic_ensemble* e) {
+    for (int p = 0; p < 2; ++p) {
+        if (e->graphExec[p]) (void)hipGraphExecDestroy(e->graphExec[p]);
+        if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+        e->graphExec[p] = nullptr;
+        e->graph[p] = nullptr;
+    }
+}
+
+// per-simulation tables for both parities (contents change with the current
+// scale exponent; the buffers stay, so captured graphs stay valid)
+int ensUploadTables(pic_ensemble* e) {
+    int const n = static_cast<int>(e->sims.size());
+    for (int par = 0; par < 2; ++par) {
+        std::vector<pic::PushArgs> push(n);
+        std::vector<pic::FieldBatchEntry> field(n);
+        for (int i = 0; i < n; ++i) {
+            pic_ctx* c = e->sims[i];
+            // members share one parity; parity `par` has E in E if par == parity, else in Ealt
+            bool const swap = par != e->parity;
+            float* E[3];
+            float* Ealt[3];
+            for (int d = 0; d < 3; ++d) {
+                E[d] = swap ? c->Ealt[d] : c->E[d];
+                Ealt[d] = swap ? c->E[d] : c->Ealt[d];
+            }
+            pic::PushArgs a = makePushArgs(c);
+            for (int d = 0; d < 3; ++d) a.E[d] = E[d];
+            int k = 0;
+            for (const SpeciesMem& m : c->sp)
+                if (m.desc.pusher != PIC_PUSHER_NONE) a.sp[k++] = m.dev;
+            a.nsp = k;
+            a.scList = nullptr;
+            push[i] = a;
+            pic::FieldArgs f = makeFieldArgs(c);
+            for (int d = 0; d < 3; ++d) {
+                f.E[d] = E[d];
+                field[i].Enew[d] = Ealt[d];
+            }
+            field[i].a = f;
+        }
+        PIC_HIP(hipMemcpyAsync(e->dPush[par], push.data(), n * sizeof(pic::PushArgs), hipMemcpyHostToDevice, e->stream));
+        PIC_HIP(hipMemcpyAsync(e->dField[par], field.data(), n * sizeof(pic::FieldBatchEntry), hipMemcpyHostToDevice,
+                               e->stream));
+    }
+    std::vector<pic::ShiftEntry> shift;
+    for (pic_ctx* c : e->sims)
+        for (const SpeciesMem& m : c->sp)
+            if (m.desc.pusher != PIC_PUSHER_NONE) shift.push_back(pic::ShiftEntry{m.dev, c->g, c->errFlags});
+    PIC_HIP(hipMemcpyAsync(e->dShift, shift.data(), shift.size() * sizeof(pic::ShiftEntry), hipMemcpyHostToDevice,
+                           e->stream));
+    PIC_HIP(hipStreamSynchronize(e->stream));
+    return PIC_OK;
+}
+
+int64_t ensParticles(pic_ensemble* e, int i) {
+    pic_ctx* c = e->sims[i];
+    int64_t n = 0;
+    for (const SpeciesMem& m : c->sp)
+        if (m.desc.pusher != PIC_PUSHER_NONE) n += (int64_t)c->nsc * m.cap;
+    return n;
+}
+
+// work lists of the batched push in the chosen order, XCD-interleaved per phase
+int ensBuildWork(pic_ensemble* e) {
+    int const n = static_cast<int>(e->sims.size());
+    std::vector<PhasePlan*> plans(n);
+    size_t nph = 0;
+    for (int i = 0; i < n; ++i) {
+        int const r = getPlan(e->sims[i], e->supp, true, &plans[i]);
+        if (r != PIC_OK) return r;
+        nph = std::max(nph, plans[i]->count.size());
+    }
+    std::vector<int> simOrder(n);
+    for (int i = 0; i < n; ++i) simOrder[i] = i;
+    if (e->order == PIC_ENSEMBLE_ORDER_HEAVY_FIRST)
+        std::stable_sort(simOrder.begin(), simOrder.end(),
+                         [&](int a, int b) { return ensParticles(e, a) > ensParticles(e, b); });
+    std::vector<int2> work;
+    e->phaseOffset.assign(nph, 0);
+    e->phaseCount.assign(nph, 0);
+    for (size_t ph = 0; ph < nph; ++ph) {
+        std::vector<int2> items;
+        if (e->order == PIC_ENSEMBLE_ORDER_INTERLEAVED) {
+            size_t longest = 0;
+            for (int i = 0; i < n; ++i)
+                if (ph < plans[i]->count.size()) longest = std::max(longest, (size_t)plans[i]->count[ph]);
+            for (size_t k = 0; k < longest; ++k)
+                for (int i : simOrder)
+                    if (ph < plans[i]->count.size() && k < (size_t)plans[i]->count[ph])
+                        items.push_back(int2{i, plans[i]->host[plans[i]->offset[ph] + k]});
+        } else {
+            for (int i : simOrder)
+                if (ph < plans[i]->count.size())
+                    for (int k = 0; k < plans[i]->count[ph]; ++k)
+                        items.push_back(int2{i, plans[i]->host[plans[i]->offset[ph] + k]});
+        }
+        // workgroup b runs on XCD b % 8: contiguous runs of items per XCD
+        int const m = static_cast<int>(items.size());
+        std::vector<int2> ordered(m);
+        int pos = 0;
+        for (int x = 0; x < 8; ++x)
+            for (int b = x; b < m; b += 8) ordered[b] = items[pos++];
+        e->phaseOffset[ph] = static_cast<int>(work.size());
+        e->phaseCount[ph] = m;
+        work.insert(work.end(), ordered.begin(), ordered.end());
+    }
+    if (!e->dPhaseWork && ensAlloc(e, &e->dPhaseWork, work.size()) != PIC_OK) return PIC_E_HIP;
+    PIC_HIP(hipMemcpyAsync(e->dPhaseWork, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice, e->stream));
+    PIC_HIP(hipStreamSynchronize(e->stream));
+    return PIC_OK;
+}
+
+int ensSyncMembers(pic_ensemble* e) {
+    for (pic_ctx* c : e->sims) PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+// one batched step: J reset, push + deposit per phase over all members, shift,
+// fused field step; the members' E/Ealt swap as in pic_step
+int ensOneStep(pic_ensemble* e) {
+    int const p = e->parity;
+    PIC_HIP(pic::launchCurrentResetBatch(e->dField[p], e->dBlocks, e->nBlocks, e->stream));
+    for (size_t ph = 0; ph < e->phaseCount.size(); ++ph) {
+        int const m = e->phaseCount[ph];
+        int const grid = e->blocksPerCu > 0 ? std::min(m, e->blocksPerCu * e->cus) : m;
+        PIC_HIP(pic::launchPushBatch(e->supp, e->pusher, e->dPush[p], e->dPhaseWork + e->phaseOffset[ph], m, grid,
+                                     e->stream));
+    }
+    PIC_HIP(pic::launchShiftBatch(e->dShift, e->dShiftWork, e->nShiftWork, e->stream));
+    PIC_HIP(pic::launchFieldStepBatch(e->solver, e->dField[p], e->dTiles, e->nTiles, e->dBlocks, e->nBlocks,
+                                      e->stream));
+    for (pic_ctx* c : e->sims) {
+        for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
+        c->eParity ^= 1;
+    }
+    e->parity ^= 1;
+    return PIC_OK;
+}
+
+int ensFinish(pic_ensemble* e) {
+    PIC_HIP(hipStreamSynchronize(e->stream));
+    for (pic_ctx* c : e->sims) {
+        for (SpeciesMem& m : c->sp) m.listsValid = false;
+        int const r = checkErrors(c);
+        if (r != PIC_OK) return r;
+    }
+    return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out) {
+    if (!cfgs || !out || n <= 0) return fail(PIC_E_INVALID, "need n > 0 configurations and an output pointer");
+    *out = nullptr;
+    pic_ensemble* e = new pic_ensemble();
+    auto bail = [&](int code) {
+        pic_ensemble_destroy(e);
+        return code;
+    };
+    e->device = cfgs[0].device;
+    for (int32_t i = 0; i < n; ++i) {
+        const pic_config& c = cfgs[i];
+        if (c.device != e->device) return bail(fail(PIC_E_INVALID, "ensemble members must share one device"));
+        int supp = 0, pusher = 0;
+        for (int s = 0; s < c.n_species; ++s) {
+            if (c.species[s].pusher == PIC_PUSHER_NONE) continue;
+            int const sp = supportOf(c.species[s].shape);
+            if ((supp && sp != supp) || (pusher && c.species[s].pusher != pusher))
+                return bail(fail(PIC_E_INVALID, "ensemble mobile species must share one shape and pusher"));
+            supp = sp;
+            pusher = c.species[s].pusher;
+            if (c.species[s].frames_per_sc * pic::kSCV > pic::kListMaxSlots)
+                return bail(fail(PIC_E_INVALID, "ensemble species need at most 32 frames per supercell"));
+        }
+        if (i == 0) {
+            e->supp = supp;
+            e->pusher = pusher;
+            e->solver = c.solver;
+        } else if (supp != e->supp || pusher != e->pusher || c.solver != e->solver) {
+            return bail(fail(PIC_E_INVALID, "ensemble members must share shape, pusher and solver"));
+        }
+        if (supp == 0) return bail(fail(PIC_E_INVALID, "every ensemble member needs a mobile species"));
+        pic_ctx* ctx = nullptr;
+        int const r = pic_create(&c, &ctx);
+        if (r != PIC_OK) return bail(r);
+        e->sims.push_back(ctx);
+    }
+    hipError_t he = hipSetDevice(e->device);
+    if (he != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(he)));
+    he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(he)));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) == hipSuccess && prop.multiProcessorCount > 0)
+        e->cus = prop.multiProcessorCount;
+    int nEntries = 0;
+    for (pic_ctx* c : e->sims)
+        for (const SpeciesMem& m : c->sp)
+            if (m.desc.pusher != PIC_PUSHER_NONE) ++nEntries;
+    e->nShiftEntries = nEntries;
+    if (ensAlloc(e, &e->dPush[0], n) || ensAlloc(e, &e->dPush[1], n) || ensAlloc(e, &e->dField[0], n) ||
+        ensAlloc(e, &e->dField[1], n) || ensAlloc(e, &e->dShift, nEntries))
+        return bail(PIC_E_HIP);
+    // shift items (entry, supercell), field tiles and 256-cell blocks per member
+    std::vector<int2> shiftWork, tiles, blocks;
+    int entry = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        pic_ctx* c = e->sims[i];
+        for (const SpeciesMem& m : c->sp) {
+            if (m.desc.pusher == PIC_PUSHER_NONE) continue;
+            for (int sc = 0; sc < c->nsc; ++sc) shiftWork.push_back(int2{entry, sc});
+            ++entry;
+        }
+        int const nt = (c->g.nx / 8) * (c->g.ny / 8) * (c->g.nz / 4);
+        for (int t = 0; t < nt; ++t) tiles.push_back(int2{i, t});
+        int const nb = static_cast<int>((c->ncells + 255) / 256);
+        for (int b = 0; b < nb; ++b) blocks.push_back(int2{i, b});
+    }
+    e->nShiftWork = static_cast<int>(shiftWork.size());
+    e->nTiles = static_cast<int>(tiles.size());
+    e->nBlocks = static_cast<int>(blocks.size());
+    if (ensAlloc(e, &e->dShiftWork, shiftWork.size()) || ensAlloc(e, &e->dTiles, tiles.size()) ||
+        ensAlloc(e, &e->dBlocks, blocks.size()))
+        return bail(PIC_E_HIP);
+    he = hipMemcpyAsync(e->dShiftWork, shiftWork.data(), shiftWork.size() * sizeof(int2), hipMemcpyHostToDevice,
+                        e->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(e->dTiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice, e->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(e->dBlocks, blocks.data(), blocks.size() * sizeof(int2), hipMemcpyHostToDevice,
+                            e->stream);
+    if (he != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(he)));
+    int r = ensBuildWork(e);
+    if (r != PIC_OK) return bail(r);
+    r = ensUploadTables(e);
+    if (r != PIC_OK) return bail(r);
+    *out = e;
+    return PIC_OK;
+}
+
+int pic_ensemble_destroy(pic_ensemble* e) {
+    if (!e) return PIC_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    ensDropGraphs(e);
+    for (void* p : e->allocations) (void)hipFree(p);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    for (pic_ctx* c : e->sims) pic_destroy(c);
+    delete e;
+    return PIC_OK;
+}
+
+int32_t pic_ensemble_size(const pic_ensemble* e) { return e ? static_cast<int32_t>(e->sims.size()) : -1; }
+
+pic_ctx* pic_ensemble_member(pic_ensemble* e, int32_t i) {
+    if (!e || i < 0 || i >= static_cast<int32_t>(e->sims.size())) {
+        fail(PIC_E_INVALID, "ensemble member index out of range");
+        return nullptr;
+    }
+    return e->sims[i];
+}
+
+int pic_ensemble_step(pic_ensemble* e, int32_t nsteps) {
+    if (!e) return fail(PIC_E_INVALID, "null ensemble");
+    if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
+    for (pic_ctx* c : e->sims)
+        if (c->eParity != e->parity) return fail(PIC_E_STATE, "an ensemble member was stepped on its own");
+    int r = ensSyncMembers(e);
+    if (r != PIC_OK) return r;
+    for (pic_ctx* c : e->sims) updateJexp(c);
+    if ((r = ensUploadTables(e)) != PIC_OK) return r;
+    for (int32_t i = 0; i < nsteps; ++i)
+        if ((r = ensOneStep(e)) != PIC_OK) return r;
+    return ensFinish(e);
+}
+
+int pic_ensemble_step_graph(pic_ensemble* e, int32_t nsteps) {
+    if (!e) return fail(PIC_E_INVALID, "null ensemble");
+    if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
+    for (pic_ctx* c : e->sims)
+        if (c->eParity != e->parity) return fail(PIC_E_STATE, "an ensemble member was stepped on its own");
+    int r = ensSyncMembers(e);
+    if (r != PIC_OK) return r;
+    for (pic_ctx* c : e->sims) updateJexp(c);
+    if ((r = ensUploadTables(e)) != PIC_OK) return r;
+    for (int32_t i = 0; i < nsteps; ++i) {
+        int const p = e->parity;
+        if (!e->graphExec[p]) {
+            PIC_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            r = ensOneStep(e);  // records the step and swaps parity as its replay will
+            hipGraph_t g = nullptr;
+            hipError_t const he = hipStreamEndCapture(e->stream, &g);
+            if (r != PIC_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return r;
+            }
+            if (he != hipSuccess) return fail(PIC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+            e->graph[p] = g;
+            PIC_HIP(hipGraphInstantiate(&e->graphExec[p], e->graph[p], nullptr, nullptr, 0));
+        } else {
+            for (pic_ctx* c : e->sims) {
+                for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
+                c->eParity ^= 1;
+            }
+            e->parity ^= 1;
+        }
+        PIC_HIP(hipGraphLaunch(e->graphExec[p], e->stream));
+    }
+    return ensFinish(e);
+}
+
+int pic_ensemble_set_schedule(pic_ensemble* e, int32_t order, int32_t blocks_per_cu) {
+    if (!e) return fail(PIC_E_INVALID, "null ensemble");
+    if (order < PIC_ENSEMBLE_ORDER_SIM_MAJOR || order > PIC_ENSEMBLE_ORDER_HEAVY_FIRST || blocks_per_cu < 0 ||
+        blocks_per_cu > 64)
+        return fail(PIC_E_INVALID, "unknown schedule");
+    PIC_HIP(hipStreamSynchronize(e->stream));
+    e->order = order;
+    e->blocksPerCu = blocks_per_cu;
+    ensDropGraphs(e);
+    return ensBuildWork(e);
+}
+
+int pic_ensemble_get_schedule(const pic_ensemble* e, int32_t* order, int32_t* blocks_per_cu) {
+    if (!e) return fail(PIC_E_INVALID, "null ensemble");
+    if (order) *order = e->order;
+    if (blocks_per_cu) *blocks_per_cu = e->blocksPerCu;
+    return PIC_OK;
+}
+
+int pic_ensemble_autotune(pic_ensemble* e, int32_t trial_steps, double* best_ms_per_step, int32_t* steps_taken) {
+    if (!e) return fail(PIC_E_INVALID, "null ensemble");
+    if (trial_steps < 1) return fail(PIC_E_INVALID, "trial_steps must be >= 1");
+    int const orders[3] = {PIC_ENSEMBLE_ORDER_SIM_MAJOR, PIC_ENSEMBLE_ORDER_INTERLEAVED,
+                           PIC_ENSEMBLE_ORDER_HEAVY_FIRST};
+    int const grids[4] = {0, 1, 2, 4};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    PIC_HIP(hipEventCreate(&t0));
+    PIC_HIP(hipEventCreate(&t1));
+    double best = 1e300;
+    int bestOrder = e->order, bestGrid = e->blocksPerCu;
+    int taken = 0;
+    int r = PIC_OK;
+    for (int o : orders) {
+        for (int gpc : grids) {
+            if ((r = pic_ensemble_set_schedule(e, o, gpc)) != PIC_OK) break;
+            if ((r = pic_ensemble_step(e, 1)) != PIC_OK) break;  // warm-up of this schedule
+            ++taken;
+            if ((r = ensSyncMembers(e)) != PIC_OK) break;
+            for (pic_ctx* c : e->sims) updateJexp(c);
+            if ((r = ensUploadTables(e)) != PIC_OK) break;
+            if (hipEventRecord(t0, e->stream) != hipSuccess) {
+                r = fail(PIC_E_HIP, "hipEventRecord");
+                break;
+            }
+            for (int k = 0; k < trial_steps && r == PIC_OK; ++k) r = ensOneStep(e);
+            if (r != PIC_OK) break;
+            taken += trial_steps;
+            float ms = 0.0f;
+            if (hipEventRecord(t1, e->stream) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+                r = fail(PIC_E_HIP, "autotune timing");
+                break;
+            }
+            if ((r = ensFinish(e)) != PIC_OK) break;
+            double const per = ms / trial_steps;
+            if (per < best) {
+                best = per;
+                bestOrder = o;
+                bestGrid = gpc;
+            }
+        }
+        if (r != PIC_OK) break;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (r != PIC_OK) return r;
+    if ((r = pic_ensemble_set_schedule(e, bestOrder, bestGrid)) != PIC_OK) return r;
+    if (best_ms_per_step) *best_ms_per_step = best;
+    if (steps_taken) *steps_taken = taken;
     return PIC_OK;
 }
 

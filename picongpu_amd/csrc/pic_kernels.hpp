@@ -76,6 +76,10 @@ hipError_t launchFill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s);
 hipError_t launchEFull(const FieldArgs& a, hipStream_t s);
 hipError_t launchAddCurrent(const FieldArgs& a, hipStream_t s);
+// fused update_beforeCurrent + addCurrent + update_afterCurrent; the new E is
+// written to Enew (the caller swaps E and Enew afterwards); the grid must be a
+// multiple of 8x8x4 cells
+hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], hipStream_t s);
 hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, hipStream_t s);
 hipError_t launchCountSum(const uint32_t* counts, int n, unsigned long long* out, hipStream_t s);
 hipError_t launchGaussResidual(const FieldArgs& f, const SpeciesDev* sp, const int* supp, int nsp, double qScale,
@@ -95,6 +99,30 @@ struct InitArgs {
 };
 hipError_t launchInit(const InitArgs& a, hipStream_t s);
 hipError_t launchNoise(float* f, size_t n, uint64_t seed, int comp, float amp, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// batched ensembles: one launch per phase over every simulation
+// ---------------------------------------------------------------------------
+struct ShiftEntry {
+    SpeciesDev S;
+    GridDev g;
+    uint32_t* errFlags;
+};
+// items (simulation, supercell) into a table of PushArgs (fused push + deposit)
+hipError_t launchPushBatch(int supp, int pusher, const PushArgs* table, const int2* work, int nwork, int nblocks,
+                           hipStream_t s);
+// items (entry, supercell) into a table of (species, grid)
+hipError_t launchShiftBatch(const ShiftEntry* table, const int2* work, int nwork, hipStream_t s);
+// items (simulation, tile) for the tiled kernel and (simulation, block of 256
+// cells) for the per-cell kernels, into tables of FieldArgs and new-E planes
+struct FieldBatchEntry {
+    FieldArgs a;
+    float* Enew[3];
+};
+hipError_t launchFieldStepBatch(int solver, const FieldBatchEntry* table, const int2* tiles, int ntiles,
+                                const int2* blocks, int nblocks, hipStream_t s);
+// J = 0 for items (simulation, block of 256 cells)
+hipError_t launchCurrentResetBatch(const FieldBatchEntry* table, const int2* blocks, int nblocks, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // differentiable form (one CIC/Boris species in identity order) and its adjoint

@@ -122,6 +122,36 @@ CONFIGS = {
 }
 
 
+def two_stream(grid=(64, 64, 64), ppc=8, beta=0.25, steps=300, seed=7) -> Setup:
+    """Two counter-streaming electron beams along z (drift +-beta c, alternating
+    with the particle's index in its cell), u_th = 0.01 c, 1e25 m^-3,
+    dx = 1e-7 m, dt at 0.99 of the Yee limit, TSC, Boris, Yee."""
+    cell = (1e-7, 1e-7, 1e-7)
+    e = SpeciesSetup("e", shape=abi.SHAPE_TSC, pusher=abi.PUSHER_BORIS, ppc=ppc, density_si=1e25,
+                     u_drift=(0.0, 0.0, gamma_beta(beta)), u_th=(0.01, 0.01, 0.01), drift_band=2, seed=seed)
+    return Setup("two_stream", tuple(grid), cell, yee_dt(cell), 1e25, ppc, [e], steps=steps)
+
+
+CONFIGS["two_stream"] = two_stream
+
+
+def ensemble(n=64, lo=32, hi=96, ppc_range=(2, 16), beta_range=(0.1, 0.4), steps=300, seed=7) -> list:
+    """Batched-ensemble workload: n independent two-stream boxes whose grid per
+    axis is drawn from the supercell multiples in [lo, hi], ppc from
+    ppc_range and beam drift from beta_range, all from one seeded generator."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        gx = int(rng.choice(np.arange(lo, hi + 1, 8)))
+        gy = int(rng.choice(np.arange(lo, hi + 1, 8)))
+        gz = int(rng.choice(np.arange(lo, hi + 1, 4)))
+        ppc = int(rng.integers(ppc_range[0], ppc_range[1] + 1))
+        beta = float(rng.uniform(*beta_range))
+        st = two_stream(grid=(gx, gy, gz), ppc=ppc, beta=beta, steps=steps, seed=seed * 1000 + i)
+        out.append(st)
+    return out
+
+
 # -- initial state --------------------------------------------------------
 def differentiable(grid=(64, 64, 64), ppc=8, steps=200, seed=0) -> Setup:
     """Cold electron plasma for the differentiable form: dx = dy = dz = 1e-7 m,
@@ -156,9 +186,12 @@ def host_particles(setup: Setup, sp: int, seed: int = None):
     w = np.float32(s.density_si * vol / s.ppc)
     mass = np.float32(k["base_mass"] * np.float32(s.mass_ratio)) * w
     u = np.array(s.u_drift, np.float64)[None, :] * np.ones((n, 1))
-    if s.drift_band:
+    if s.drift_band == 1:
         band = (cells[:, 1] >= ny // 4) & (cells[:, 1] < 3 * ny // 4)
         u = u * np.where(band, 1.0, -1.0)[:, None]
+    elif s.drift_band == 2:  # counter-streaming: sign alternates with the particle's index in its cell
+        k = np.tile(np.arange(s.ppc), nx * ny * nz)
+        u = u * np.where(k % 2 == 0, 1.0, -1.0)[:, None]
     u = u + rng.standard_normal((n, 3)) * np.array(s.u_th)[None, :]
     mom = (np.float64(mass) * u).astype(np.float32)
     weights = np.full(n, w, np.float32)

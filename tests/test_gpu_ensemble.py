@@ -1,0 +1,105 @@
+"""Batched ensembles: every member advanced by the batched launches must equal
+the same simulation advanced on its own with pic_step, bit for bit (frames,
+E, B, J), whatever the schedule, eager or replayed from graphs."""
+import numpy as np
+import pytest
+
+from picongpu_amd import abi, setups
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def members():
+    return [setups.two_stream(grid=(16, 16, 8), ppc=2, beta=0.3, seed=11),
+            setups.two_stream(grid=(24, 16, 12), ppc=3, beta=0.2, seed=12),
+            setups.two_stream(grid=(16, 24, 8), ppc=4, beta=0.35, seed=13)]
+
+
+def init(sim, st):
+    setups.init_device(sim, st)
+
+
+def compare(ens_member, single, what):
+    H.assert_frames_equal(ens_member.read_frames(0), single.read_frames(0), f"{what} frames")
+    for f in (abi.FIELD_E, abi.FIELD_B, abi.FIELD_J):
+        H.assert_field_equal(ens_member.read_field(f), single.read_field(f).astype(np.float64), f"{what} field {f}")
+
+
+def singles(sts, nsteps):
+    out = []
+    for st in sts:
+        s = abi.Simulation(st.config())
+        init(s, st)
+        s.step(nsteps)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("mode", ["eager", "graph"])
+def test_ensemble_equals_single_runs(mode):
+    sts = members()
+    nsteps = 6
+    print(f"case ensemble {mode} members={[s.grid for s in sts]} steps={nsteps}", flush=True)
+    ens = abi.Ensemble([st.config() for st in sts])
+    for m, st in zip(ens.members, sts):
+        init(m, st)
+    if mode == "eager":
+        ens.step(nsteps)
+    else:
+        ens.step_graph(nsteps)
+    ref = singles(sts, nsteps)
+    for i, (m, s) in enumerate(zip(ens.members, ref)):
+        compare(m, s, f"member {i}")
+        assert m.error_flags() == 0
+        s.close()
+    ens.close()
+
+
+def test_ensemble_schedules_and_autotune_do_not_change_results():
+    sts = members()
+    print("case ensemble schedules + autotune", flush=True)
+    ens = abi.Ensemble([st.config() for st in sts])
+    for m, st in zip(ens.members, sts):
+        init(m, st)
+    taken = 0
+    for order in (abi.ENSEMBLE_ORDER_SIM_MAJOR, abi.ENSEMBLE_ORDER_INTERLEAVED, abi.ENSEMBLE_ORDER_HEAVY_FIRST):
+        for bpc in (0, 1):
+            ens.set_schedule(order, bpc)
+            ens.step(1)
+            taken += 1
+    ms, n = ens.autotune(trial_steps=1)
+    taken += n
+    assert n == 12 * 2 and ms > 0.0
+    ens.step_graph(1)
+    taken += 1
+    ref = singles(sts, taken)
+    for i, (m, s) in enumerate(zip(ens.members, ref)):
+        compare(m, s, f"member {i} after {taken} steps")
+        s.close()
+    ens.close()
+
+
+def test_member_stepped_alone_in_pairs():
+    """A member may be stepped on its own between ensemble steps in even
+    numbers; an odd number is refused."""
+    sts = members()[:2]
+    print("case ensemble member stepped alone", flush=True)
+    ens = abi.Ensemble([st.config() for st in sts])
+    for m, st in zip(ens.members, sts):
+        init(m, st)
+    ens.step(1)
+    ens.members[0].step(2)
+    ens.step_graph(1)
+    ens.members[1].step(1)
+    with pytest.raises(abi.PicError):
+        ens.step(1)
+    ens.members[1].step(1)
+    ens.step(1)
+    ref = singles(sts, 0)
+    ref[0].step(5)
+    ref[1].step(5)
+    for i, (m, s) in enumerate(zip(ens.members, ref)):
+        compare(m, s, f"member {i}")
+        s.close()
+    ens.close()

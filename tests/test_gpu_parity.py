@@ -100,8 +100,17 @@ def test_fields_bit_exact(solver):
     sim.close()
 
 
+def small_lehe():
+    st = small_thermal(abi.SHAPE_TSC, abi.PUSHER_VAY)
+    st.solver = abi.SOLVER_LEHE
+    st.lehe_dir = 1
+    st.dt_si = 0.99 * st.cell_si[1] / setups.SPEED_OF_LIGHT_SI / 1.8
+    return st
+
+
 STEP_CASES = [
     ("thermal-cic-boris", lambda: small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS)),
+    ("thermal-tsc-vay-lehe", small_lehe),
     ("thermal-pqs-vay", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_VAY)),
     ("kh-tsc-2species", lambda: small_two_species(abi.SHAPE_TSC)),
 ]
