@@ -199,6 +199,7 @@ def main():
     sim = abi.Simulation(st.config())
     setups.init_device(sim, st)
     sim.sync()
+    n_particles = sum(sim.particle_count(i) for i, s in enumerate(st.species) if s.pusher != abi.PUSHER_NONE)
     for _ in range(args.warmup):
         sim.step(1)
     sim.sync()

@@ -136,7 +136,31 @@ typedef struct pic_init_desc {
     double density_si;        /* physical density [1/m^3] */
     double u_drift[3];        /* gamma*beta of the drift */
     double u_th[3];           /* thermal spread of gamma*beta per component */
+    int32_t profile;          /* 0: uniform; 1: linear ramp along profile_axis, evaluated at the cell
+                                 origin: density_si * clamp((x - ramp_start_si)/ramp_length_si, 0, 1);
+                                 cells at zero density get no particles, the others ppc particles of
+                                 weighting density*V/ppc (densityProfiles, WeightMacroParticles.hpp:30-42) */
+    int32_t profile_axis;
+    double ramp_start_si;
+    double ramp_length_si;
 } pic_init_desc;
+
+/* Gaussian laser pulse set as the initial field (the reference injects it
+ * through a Huygens surface, incidentField/profiles/GaussianPulse.hpp; here
+ * the pulse starts inside the box at its focus): E along polarisation_axis
+ * = E0 exp(-r^2/w0^2) exp(-xi^2/(2 sigma^2)) cos(k xi), B = (1/c) k_hat x E,
+ * xi = distance from center_si along propagation_axis, r the distance from
+ * that axis, E0 = a0 m_e c omega / e, sigma = c * fwhm_si / (2 sqrt(ln 2))
+ * (intensity FWHM), each component sampled at its Yee position. */
+typedef struct pic_laser_desc {
+    double wavelength_si;
+    double a0;
+    double waist_si;
+    double fwhm_si;
+    double center_si[3];
+    int32_t propagation_axis;
+    int32_t polarisation_axis;
+} pic_laser_desc;
 
 typedef struct pic_ctx pic_ctx;
 
@@ -178,6 +202,8 @@ int pic_load_field(pic_ctx* ctx, int32_t field, const float* data);
 int pic_read_field(pic_ctx* ctx, int32_t field, float* data);
 /* B += amp*U(-1,1) per component, seeded (initial noise of the shear-flow setup) */
 int pic_add_field_noise(pic_ctx* ctx, int32_t field, uint64_t seed, float amp);
+/* adds the laser pulse to E and B */
+int pic_add_laser(pic_ctx* ctx, const pic_laser_desc* laser);
 
 /* --- stages of one PIC step (Simulation::runOneStep, Simulation.hpp:522-542) --- */
 /* CurrentReset (simulation/stage/CurrentReset.hpp:39-53): J = 0 */

@@ -82,6 +82,22 @@ class InitDesc(C.Structure):
         ("density_si", C.c_double),
         ("u_drift", C.c_double * 3),
         ("u_th", C.c_double * 3),
+        ("profile", C.c_int32),
+        ("profile_axis", C.c_int32),
+        ("ramp_start_si", C.c_double),
+        ("ramp_length_si", C.c_double),
+    ]
+
+
+class LaserDesc(C.Structure):
+    _fields_ = [
+        ("wavelength_si", C.c_double),
+        ("a0", C.c_double),
+        ("waist_si", C.c_double),
+        ("fwhm_si", C.c_double),
+        ("center_si", C.c_double * 3),
+        ("propagation_axis", C.c_int32),
+        ("polarisation_axis", C.c_int32),
     ]
 
 
@@ -113,6 +129,7 @@ SIGNATURES = {
     "pic_load_field": (C.c_int, [_P, C.c_int32, _F]),
     "pic_read_field": (C.c_int, [_P, C.c_int32, _F]),
     "pic_add_field_noise": (C.c_int, [_P, C.c_int32, C.c_uint64, C.c_float]),
+    "pic_add_laser": (C.c_int, [_P, C.POINTER(LaserDesc)]),
     "pic_current_reset": (C.c_int, [_P]),
     "pic_push": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "pic_shift": (C.c_int, [_P, C.c_int32]),
@@ -309,7 +326,8 @@ class Simulation:
                                         ptr(f["mom"], C.c_float), ptr(f["w"], C.c_float), ptr(f["cell"], C.c_uint16),
                                         ptr(f["mask"], C.c_uint8)), "pic_write_frames")
 
-    def init_species(self, sp, seed, ppc, density_si, u_drift=(0, 0, 0), u_th=(0, 0, 0), drift_band=0):
+    def init_species(self, sp, seed, ppc, density_si, u_drift=(0, 0, 0), u_th=(0, 0, 0), drift_band=0, ramp=None):
+        """ramp: None (uniform) or (axis, start_si, length_si) of a linear density ramp"""
         d = InitDesc()
         d.seed = int(seed)
         d.ppc = int(ppc)
@@ -318,7 +336,24 @@ class Simulation:
         for i in range(3):
             d.u_drift[i] = float(u_drift[i])
             d.u_th[i] = float(u_th[i])
+        if ramp is not None:
+            d.profile = 1
+            d.profile_axis = int(ramp[0])
+            d.ramp_start_si = float(ramp[1])
+            d.ramp_length_si = float(ramp[2])
         check(self.lib.pic_init_species(self.h, sp, C.byref(d)), "pic_init_species")
+
+    def add_laser(self, wavelength_si, a0, waist_si, fwhm_si, center_si, propagation_axis, polarisation_axis):
+        d = LaserDesc()
+        d.wavelength_si = float(wavelength_si)
+        d.a0 = float(a0)
+        d.waist_si = float(waist_si)
+        d.fwhm_si = float(fwhm_si)
+        for i in range(3):
+            d.center_si[i] = float(center_si[i])
+        d.propagation_axis = int(propagation_axis)
+        d.polarisation_axis = int(polarisation_axis)
+        check(self.lib.pic_add_laser(self.h, C.byref(d)), "pic_add_laser")
 
     def load_field(self, which, data):
         data = np.ascontiguousarray(data, dtype=np.float32).reshape(3, -1)

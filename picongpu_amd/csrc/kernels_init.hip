@@ -12,21 +12,51 @@
 
 namespace pic {
 
+// density factor of the profile at the origin of global cell (gx, gy, gz)
+__device__ __forceinline__ double densityFactor(const InitArgs& a, int gx, int gy, int gz) {
+    if (a.profile == 0) return 1.0;
+    double const x = (double)(a.axis == 0 ? gx : (a.axis == 1 ? gy : gz));
+    double const f = (x - a.rampStart) / a.rampLength;
+    return f <= 0.0 ? 0.0 : (f >= 1.0 ? 1.0 : f);
+}
+
 __global__ __launch_bounds__(256) void kInit(InitArgs a) {
+    __shared__ uint32_t sWave[4];
+    __shared__ int16_t sCellOfRank[kSCV];
     int const sc = blockIdx.x;
     int const sx = sc % a.g.nscx;
     int const sy = (sc / a.g.nscx) % a.g.nscy;
     int const sz = sc / (a.g.nscx * a.g.nscy);
     size_t const base = (size_t)sc * (size_t)a.sp.cap;
-    int const n = a.ppc * kSCV;
+    // cells with a positive density, ranked in cell order (256 threads = 256 cells)
+    int nActive;
+    {
+        int const lc = threadIdx.x;
+        bool const active = densityFactor(a, sx * kSCX + lc % kSCX, sy * kSCY + (lc / kSCX) % kSCY,
+                                          sz * kSCZ + lc / (kSCX * kSCY)) > 0.0;
+        int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        unsigned long long const b = __ballot(active);
+        if (lane == 0) sWave[wave] = (uint32_t)__popcll(b);
+        __syncthreads();
+        int before = 0;
+        nActive = 0;
+        for (int w = 0; w < 4; ++w) {
+            before += w < wave ? (int)sWave[w] : 0;
+            nActive += (int)sWave[w];
+        }
+        if (active) sCellOfRank[before + __popcll(b & ((1ull << lane) - 1ull))] = (int16_t)lc;
+        __syncthreads();
+    }
+    // slot = k * nActive + rank (round-robin over the populated cells)
+    int const n = a.ppc * nActive;
     for (int i = threadIdx.x; i < a.sp.cap; i += blockDim.x) {
         size_t const g = base + i;
         if (i >= n) {
             a.sp.mask[g] = 0;
             continue;
         }
-        int const k = i / kSCV;
-        int const lc = i % kSCV;
+        int const k = i / nActive;
+        int const lc = sCellOfRank[i % nActive];
         int const gx = sx * kSCX + lc % kSCX;
         int const gy = sy * kSCY + (lc / kSCX) % kSCY;
         int const gz = sz * kSCZ + lc / (kSCX * kSCY);
@@ -41,16 +71,45 @@ __global__ __launch_bounds__(256) void kInit(InitArgs a) {
         double sign = 1.0;
         if (a.drift_band == 1) sign = (gy >= a.g.ny / 4 && gy < 3 * a.g.ny / 4) ? 1.0 : -1.0;
         if (a.drift_band == 2) sign = (k & 1) ? -1.0 : 1.0;  // counter-streaming beams
-        double const m = (double)a.mass;
+        float const w = (float)(a.weighting * densityFactor(a, gx, gy, gz));
+        double const m = (double)(a.frameMass * w);
         a.sp.mom[0][g] = (float)(m * (sign * a.u_drift[0] + a.u_th[0] * n0));
         a.sp.mom[1][g] = (float)(m * (sign * a.u_drift[1] + a.u_th[1] * n1));
         a.sp.mom[2][g] = (float)(m * (sign * a.u_drift[2] + a.u_th[2] * n2));
         (void)n3;
-        a.sp.w[g] = a.weighting;
+        a.sp.w[g] = w;
         a.sp.cell[g] = (uint16_t)lc;
         a.sp.mask[g] = 1;
     }
     if (threadIdx.x == 0) a.sp.count[sc] = (uint32_t)(n < a.sp.cap ? n : a.sp.cap);
+}
+
+// Gaussian pulse added to E and B at the Yee positions (YeeCell.hpp:69-123)
+__global__ __launch_bounds__(256) void kLaser(LaserArgs a) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int const idx[3] = {(int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    int const P = a.prop, Q = a.pol, R = 3 - a.prop - a.pol;  // R: the magnetic component's axis
+    double const sgnB = ((Q + 1) % 3 == P) ? -1.0 : 1.0;      // (k_hat x e_Q) along e_R
+    for (int f = 0; f < 2; ++f) {
+        int const comp = f == 0 ? Q : R;
+        double x[3];
+        for (int d = 0; d < 3; ++d) {
+            // E_c sits at +1/2 along c; B_c at +1/2 along the two other axes
+            double const off = f == 0 ? (d == comp ? 0.5 : 0.0) : (d == comp ? 0.0 : 0.5);
+            x[d] = ((double)idx[d] + off) * a.cell[d] - a.center[d];
+        }
+        double r2 = 0.0;
+        for (int d = 0; d < 3; ++d)
+            if (d != P) r2 += x[d] * x[d];
+        double const xi = x[P];
+        double const v = a.e0 * exp(-r2 / (a.w0 * a.w0)) * exp(-xi * xi / (2.0 * a.sigma * a.sigma)) * cos(a.k * xi);
+        if (f == 0)
+            a.E[comp][i] = a.E[comp][i] + (float)v;
+        else
+            a.B[comp][i] = a.B[comp][i] + (float)(sgnB * v * a.invc);
+    }
 }
 
 __global__ __launch_bounds__(256) void kNoise(float* f, size_t n, uint64_t seed, int comp, float amp) {
@@ -63,6 +122,12 @@ __global__ __launch_bounds__(256) void kNoise(float* f, size_t n, uint64_t seed,
 hipError_t launchInit(const InitArgs& a, hipStream_t s) {
     int const nsc = a.g.nscx * a.g.nscy * a.g.nscz;
     hipLaunchKernelGGL(kInit, dim3(nsc), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchLaser(const LaserArgs& a, hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    hipLaunchKernelGGL(kLaser, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -786,17 +786,27 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
     a.seed = desc->seed;
     a.species = s;
     a.ppc = desc->ppc;
+    if (desc->drift_band < 0 || desc->drift_band > 2) return fail(PIC_E_INVALID, "drift_band must be 0, 1 or 2");
+    if (desc->profile < 0 || desc->profile > 1) return fail(PIC_E_INVALID, "unknown density profile");
+    if (desc->profile == 1 && (desc->profile_axis < 0 || desc->profile_axis > 2 || !(desc->ramp_length_si > 0.0)))
+        return fail(PIC_E_INVALID, "a ramp needs an axis in 0..2 and a positive length");
     a.drift_band = desc->drift_band;
-    a.weighting = static_cast<float>(desc->density_si * vol / static_cast<double>(desc->ppc));
-    a.mass = m.mass * a.weighting;
+    a.weighting = desc->density_si * vol / static_cast<double>(desc->ppc);
+    a.frameMass = m.mass;
     for (int d = 0; d < 3; ++d) {
         a.u_drift[d] = desc->u_drift[d];
         a.u_th[d] = desc->u_th[d];
     }
+    a.profile = desc->profile;
+    a.axis = desc->profile == 1 ? desc->profile_axis : 0;
+    if (desc->profile == 1) {
+        a.rampStart = desc->ramp_start_si / c->cfg.cell_si[a.axis];
+        a.rampLength = desc->ramp_length_si / c->cfg.cell_si[a.axis];
+    }
     m.listsValid = false;
     PIC_HIP(pic::launchInit(a, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
-    m.wmax = std::fabs(static_cast<double>(a.weighting));
+    m.wmax = std::fabs(static_cast<double>(static_cast<float>(a.weighting)));
     updateJexp(c);
     return PIC_OK;
 }
@@ -826,6 +836,37 @@ int pic_add_field_noise(pic_ctx* c, int32_t field, uint64_t seed, float amp) {
     float** f = field == PIC_FIELD_E ? c->E : (field == PIC_FIELD_B ? c->B : (field == PIC_FIELD_J ? c->J : nullptr));
     if (!f) return fail(PIC_E_INVALID, "unknown field");
     for (int a = 0; a < 3; ++a) PIC_HIP(pic::launchNoise(f[a], c->ncells, seed, a, amp, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_add_laser(pic_ctx* c, const pic_laser_desc* l) {
+    if (!c || !l) return fail(PIC_E_INVALID, "null argument");
+    if (l->propagation_axis < 0 || l->propagation_axis > 2 || l->polarisation_axis < 0 || l->polarisation_axis > 2 ||
+        l->propagation_axis == l->polarisation_axis)
+        return fail(PIC_E_INVALID, "propagation and polarisation must be two different axes");
+    if (!(l->wavelength_si > 0.0) || !(l->waist_si > 0.0) || !(l->fwhm_si > 0.0))
+        return fail(PIC_E_INVALID, "wavelength, waist and duration must be positive");
+    double const cSi = 2.99792458e8, meSi = 9.1093837139e-31, qeSi = 1.602176634e-19;
+    double const L = c->k.unit_length;
+    double const unitE = c->k.unit_mass * L / (c->k.unit_time * c->k.unit_time * c->k.unit_charge);
+    double const omega = 2.0 * 3.14159265358979323846 * cSi / l->wavelength_si;
+    pic::LaserArgs a{};
+    a.g = c->g;
+    for (int d = 0; d < 3; ++d) {
+        a.E[d] = c->E[d];
+        a.B[d] = c->B[d];
+        a.cell[d] = static_cast<double>(c->k.cell[d]);
+        a.center[d] = l->center_si[d] / L;
+    }
+    a.e0 = l->a0 * meSi * cSi * omega / qeSi / unitE;
+    a.k = 2.0 * 3.14159265358979323846 / (l->wavelength_si / L);
+    a.w0 = l->waist_si / L;
+    a.sigma = cSi * l->fwhm_si / (2.0 * std::sqrt(std::log(2.0))) / L;
+    a.invc = 1.0 / static_cast<double>(c->k.c);
+    a.prop = l->propagation_axis;
+    a.pol = l->polarisation_axis;
+    PIC_HIP(pic::launchLaser(a, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }

@@ -92,12 +92,29 @@ struct InitArgs {
     int32_t species;
     int32_t ppc;
     int32_t drift_band;
-    float weighting;
-    float mass;          // frame mass * weighting
+    double weighting;    // density*V/ppc at full density
+    float frameMass;     // frame::getMass()
     double u_drift[3];
     double u_th[3];
+    int32_t profile;     // 0 uniform, 1 linear ramp
+    int32_t axis;
+    double rampStart;    // in cells
+    double rampLength;   // in cells
 };
 hipError_t launchInit(const InitArgs& a, hipStream_t s);
+struct LaserArgs {
+    GridDev g;
+    float* E[3];
+    float* B[3];
+    double e0;           // peak field in PIC units
+    double k;            // 2 pi / wavelength, per cell of the propagation axis... in PIC length units
+    double w0, sigma;    // in PIC length units
+    double center[3];    // in PIC length units
+    double cell[3];      // in PIC length units
+    double invc;         // 1 / c
+    int32_t prop, pol;
+};
+hipError_t launchLaser(const LaserArgs& a, hipStream_t s);
 hipError_t launchNoise(float* f, size_t n, uint64_t seed, int comp, float amp, hipStream_t s);
 
 // ---------------------------------------------------------------------------

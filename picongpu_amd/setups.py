@@ -32,6 +32,7 @@ class SpeciesSetup:
     u_th: tuple = (0.0, 0.0, 0.0)
     drift_band: int = 0
     seed: int = 0
+    ramp: tuple = None  # (axis, start_si, length_si) of a linear density ramp
     frames_per_sc: int = 0  # 0: derived from ppc
 
     def frames(self) -> int:
@@ -54,6 +55,7 @@ class Setup:
     steps: int = 100
     b_noise: float = 0.0
     b_noise_seed: int = 1
+    laser: dict = None  # keyword arguments of Simulation.add_laser
 
     def config(self, device: int = 0, j_scale_exp: int = 0) -> abi.Config:
         sp = [dict(mass_ratio=s.mass_ratio, charge_ratio=s.charge_ratio, shape=s.shape, pusher=s.pusher,
@@ -103,16 +105,24 @@ def kelvin_helmholtz(grid=(256, 256, 256), ppc=16, shape=abi.SHAPE_TSC, steps=50
 
 def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0) -> Setup:
     """Laser-plasma geometry: PQS shape, Vay pusher, Lehe solver Cherenkov-free
-    along y, dx = dz = 0.1 um, dy = 0.05 um, cold electrons with immobile
-    protons. The density ramp and laser initial field are not part of this
-    setup yet; the electrons are uniform at 3e24 m^-3."""
+    along y, dx = dz = 0.1 um, dy = 0.05 um, cold electrons and immobile
+    protons on a linear density ramp along y rising over 100 um to 3e24 m^-3
+    from y = 20 um (the plateau lies beyond the 76.8 um box), and a Gaussian
+    laser pulse as the initial field: 0.8 um, a0 = 2, waist 8 um, intensity
+    FWHM 25 fs, polarised along x, travelling along +y, centred at y = 10 um
+    on the transverse axis."""
     cell = (1e-7, 5e-8, 1e-7)
     dt = 0.995 * cell[1] / SPEED_OF_LIGHT_SI
-    e = SpeciesSetup("e", shape=abi.SHAPE_PQS, pusher=abi.PUSHER_VAY, ppc=ppc, density_si=3e24, seed=seed)
+    ramp = (1, 20e-6, 100e-6)
+    e = SpeciesSetup("e", shape=abi.SHAPE_PQS, pusher=abi.PUSHER_VAY, ppc=ppc, density_si=3e24, seed=seed,
+                     ramp=ramp)
     p = SpeciesSetup("p", mass_ratio=MASS_RATIO_PROTON, charge_ratio=-1.0, shape=abi.SHAPE_PQS,
-                     pusher=abi.PUSHER_NONE, ppc=ppc, density_si=3e24, seed=seed + 1)
+                     pusher=abi.PUSHER_NONE, ppc=ppc, density_si=3e24, seed=seed + 1, ramp=ramp)
+    laser = dict(wavelength_si=0.8e-6, a0=2.0, waist_si=8e-6, fwhm_si=25e-15,
+                 center_si=(grid[0] * cell[0] / 2, 10e-6, grid[2] * cell[2] / 2), propagation_axis=1,
+                 polarisation_axis=0)
     return Setup("laser_plasma", tuple(grid), cell, dt, 3e24, ppc, [e, p], solver=abi.SOLVER_LEHE, lehe_dir=1,
-                 steps=steps)
+                 steps=steps, laser=laser)
 
 
 CONFIGS = {
@@ -165,9 +175,11 @@ def differentiable(grid=(64, 64, 64), ppc=8, steps=200, seed=0) -> Setup:
 def init_device(sim: "abi.Simulation", setup: Setup) -> None:
     """Initial particles (and B noise) generated on the device."""
     for i, s in enumerate(setup.species):
-        sim.init_species(i, s.seed, s.ppc, s.density_si, s.u_drift, s.u_th, s.drift_band)
+        sim.init_species(i, s.seed, s.ppc, s.density_si, s.u_drift, s.u_th, s.drift_band, ramp=s.ramp)
     if setup.b_noise:
         sim.add_field_noise(abi.FIELD_B, setup.b_noise_seed, setup.b_noise)
+    if setup.laser:
+        sim.add_laser(**setup.laser)
 
 
 def host_particles(setup: Setup, sp: int, seed: int = None):

@@ -235,3 +235,67 @@ def test_full_size_kelvin_helmholtz_properties():
     print("Gauss residual drift / rho_species:", drift)
     assert drift < 2e-6
     sim.close()
+
+
+def test_density_ramp_init():
+    """Linear ramp along y: cells whose origin lies before the ramp start get
+    no particles, the others ppc particles of weighting w0*clamp(f,0,1)."""
+    st = setups.two_stream(grid=(16, 32, 8), ppc=3)
+    dy = st.cell_si[1]
+    st.species[0].ramp = (1, 8 * dy, 16 * dy)
+    print("case density ramp", flush=True)
+    sim = abi.Simulation(st.config())
+    setups.init_device(sim, st)
+    cells, pos, mom, w = sim.read_particles(0)
+    vol = st.cell_si[0] * st.cell_si[1] * st.cell_si[2]
+    w0 = st.species[0].density_si * vol / st.species[0].ppc
+    start, length = (8 * dy) / dy, (16 * dy) / dy  # the ramp in cells, as the library converts it
+    gy = cells[:, 1].astype(np.int64)
+    f = np.clip((gy - start) / length, 0.0, 1.0)
+    assert np.all(f > 0.0)
+    np.testing.assert_array_equal(H.bits(w), H.bits((w0 * f).astype(np.float32)))
+    flat = cells[:, 0] + 16 * (cells[:, 1] + 32 * cells[:, 2])
+    per_cell = np.bincount(flat, minlength=16 * 32 * 8).reshape(8, 32, 16)
+    expect = np.where((np.arange(32) - start) / length > 0.0, 3, 0)
+    assert np.all(per_cell == expect[None, :, None])
+    sim.close()
+
+
+def test_laser_initial_field():
+    """The Gaussian pulse at the Yee positions against its formula (FP64
+    evaluated on the host, rounded once to FP32)."""
+    st = setups.two_stream(grid=(16, 64, 16), ppc=1)
+    cell = np.array(st.cell_si)
+    laser = dict(wavelength_si=0.8e-6, a0=1.5, waist_si=0.6e-6, fwhm_si=5e-15, center_si=(0.8e-6, 3.2e-6, 0.8e-6),
+                 propagation_axis=1, polarisation_axis=0)
+    print("case laser initial field", flush=True)
+    sim = abi.Simulation(st.config())
+    sim.add_laser(**laser)
+    E = sim.read_field(abi.FIELD_E).reshape(3, 16, 64, 16)
+    B = sim.read_field(abi.FIELD_B).reshape(3, 16, 64, 16)
+    k = sim.constants()
+    L = k.unit_length
+    unit_e = k.unit_mass * L / (k.unit_time ** 2 * k.unit_charge)
+    c_si, me, qe = setups.SPEED_OF_LIGHT_SI, 9.1093837139e-31, 1.602176634e-19
+    omega = 2 * math.pi * c_si / laser["wavelength_si"]
+    e0 = laser["a0"] * me * c_si * omega / qe / unit_e
+    kk = 2 * math.pi / (laser["wavelength_si"] / L)
+    w0 = laser["waist_si"] / L
+    sigma = c_si * laser["fwhm_si"] / (2 * math.sqrt(math.log(2))) / L
+    cpic = [float(np.float32(v)) for v in k.cell]
+    ctr = np.array(laser["center_si"]) / L
+    z, y, x = np.meshgrid(np.arange(16), np.arange(64), np.arange(16), indexing="ij")
+
+    def pulse(off):
+        px = (x + off[0]) * cpic[0] - ctr[0]
+        py = (y + off[1]) * cpic[1] - ctr[1]
+        pz = (z + off[2]) * cpic[2] - ctr[2]
+        return e0 * np.exp(-(px * px + pz * pz) / (w0 * w0)) * np.exp(-py * py / (2 * sigma * sigma)) * np.cos(kk * py)
+
+    ex = pulse((0.5, 0.0, 0.0)).astype(np.float32)
+    bz = (-pulse((0.5, 0.5, 0.0))).astype(np.float32)
+    assert np.abs(E[0] - ex).max() <= 2e-7 * np.abs(ex).max()
+    assert np.abs(B[2] - bz).max() <= 2e-7 * np.abs(bz).max()
+    assert np.abs(E[1]).max() == 0.0 and np.abs(E[2]).max() == 0.0 and np.abs(B[0]).max() == 0.0
+    assert np.abs(ex).max() > 0.5 * e0
+    sim.close()
