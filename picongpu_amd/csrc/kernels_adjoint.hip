@@ -307,15 +307,15 @@ __global__ __launch_bounds__(256) void kDiffPush(DiffArgs a, DiffParticles P) {
         nc[0] = wrapI(cx + dir[0], a.g.nx);
         nc[1] = wrapI(cy + dir[1], a.g.ny);
         nc[2] = wrapI(cz + dir[2], a.g.nz);
-        P.cell[i] = nc[0];
-        P.cell[n + i] = nc[1];
-        P.cell[2 * n + i] = nc[2];
-        P.pos[i] = p[0];
-        P.pos[n + i] = p[1];
-        P.pos[2 * n + i] = p[2];
-        P.mom[i] = mom.x;
-        P.mom[n + i] = mom.y;
-        P.mom[2 * n + i] = mom.z;
+        P.cellOut[i] = nc[0];
+        P.cellOut[n + i] = nc[1];
+        P.cellOut[2 * n + i] = nc[2];
+        P.posOut[i] = p[0];
+        P.posOut[n + i] = p[1];
+        P.posOut[2 * n + i] = p[2];
+        P.momOut[i] = mom.x;
+        P.momOut[n + i] = mom.y;
+        P.momOut[2 * n + i] = mom.z;
     }
     // deposit nodes of a particle in cell c: c-1 .. c+2 per axis
     int rel[3];
@@ -353,12 +353,15 @@ __global__ __launch_bounds__(256) void kDiffPush(DiffArgs a, DiffParticles P) {
     }
 }
 
-__global__ __launch_bounds__(256) void kFixToFloat(unsigned long long* fix, float* J, size_t n, double jinv) {
-    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    long long const acc = (long long)fix[i];
+__global__ __launch_bounds__(256) void kFixToFloat(unsigned long long* fix, float* J0, float* J1, float* J2, size_t n,
+                                                   double jinv) {
+    size_t const t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n) return;
+    size_t const c = t / n, i = t - c * n;
+    float* const J = c == 0 ? J0 : (c == 1 ? J1 : J2);
+    long long const acc = (long long)fix[t];
     J[i] = acc == 0 ? 0.0f : (float)((double)acc * jinv);
-    fix[i] = 0ull;
+    fix[t] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -368,7 +371,8 @@ __global__ __launch_bounds__(256) void kFixToFloat(unsigned long long* fix, floa
 // dst += coef * curl(src): forward differences (fwd = 1) or backward (fwd = 0)
 template <int FWD>
 __global__ __launch_bounds__(256) void kCurlAdd(AdjFieldArgs a, double* d0, double* d1, double* d2, const double* s0,
-                                                const double* s1, const double* s2, double coef) {
+                                                const double* s1, const double* s2, double coef, double* lJ,
+                                                double jcoef) {
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -382,15 +386,19 @@ __global__ __launch_bounds__(256) void kCurlAdd(AdjFieldArgs a, double* d0, doub
     double const cx = der(2, 1) - der(1, 2);
     double const cy = der(0, 2) - der(2, 0);
     double const cz = der(1, 0) - der(0, 1);
-    d0[i] += coef * cx;
-    d1[i] += coef * cy;
-    d2[i] += coef * cz;
+    double const v0 = d0[i] + coef * cx;
+    double const v1 = d1[i] + coef * cy;
+    double const v2 = d2[i] + coef * cz;
+    d0[i] = v0;
+    d1[i] = v1;
+    d2[i] = v2;
+    if (lJ) {  // the current's adjoint, lambda_J = coeff * lambda_E'
+        lJ[i] = jcoef * v0;
+        lJ[n + i] = jcoef * v1;
+        lJ[2 * n + i] = jcoef * v2;
+    }
 }
 
-__global__ __launch_bounds__(256) void kScaleCopy(const double* src, double* dst, size_t n, double coef) {
-    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = coef * src[i];
-}
 
 
 
@@ -781,8 +789,8 @@ hipError_t launchDiffPush(const DiffArgs& a, const DiffParticles& P, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launchFixToFloat(unsigned long long* fix, float* J, size_t n, double jinv, hipStream_t s) {
-    hipLaunchKernelGGL(kFixToFloat, dim3(blocksOf(n)), dim3(256), 0, s, fix, J, n, jinv);
+hipError_t launchFixToFloat(unsigned long long* fix, float* const J[3], size_t n, double jinv, hipStream_t s) {
+    hipLaunchKernelGGL(kFixToFloat, dim3(blocksOf(3 * n)), dim3(256), 0, s, fix, J[0], J[1], J[2], n, jinv);
     return hipGetLastError();
 }
 
@@ -792,17 +800,15 @@ hipError_t launchFixToFloat(unsigned long long* fix, float* J, size_t n, double 
 hipError_t launchAdjFields(const AdjFieldArgs& a, double dt, double c2, double coeff, double* lJ[3], hipStream_t s) {
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     double const h = 0.5 * dt;
-    // B'' = B' - h C+ E'  ->  lE' = lE - h C- lB
+    // B'' = B' - h C+ E'  ->  lE' = lE - h C- lB, and lambda_J = coeff lE'
     hipLaunchKernelGGL(kCurlAdd<0>, dim3(blocksOf(n)), dim3(256), 0, s, a, a.lE[0], a.lE[1], a.lE[2], a.lB[0],
-                       a.lB[1], a.lB[2], -h);
-    for (int c = 0; c < 3; ++c)
-        hipLaunchKernelGGL(kScaleCopy, dim3(blocksOf(n)), dim3(256), 0, s, a.lE[c], lJ[c], n, coeff);
+                       a.lB[1], a.lB[2], -h, lJ[0], coeff);
     // E' = E + dt c^2 C- B' + coeff J  ->  lB' = lB + dt c^2 C+ lE'
     hipLaunchKernelGGL(kCurlAdd<1>, dim3(blocksOf(n)), dim3(256), 0, s, a, a.lB[0], a.lB[1], a.lB[2], a.lE[0],
-                       a.lE[1], a.lE[2], dt * c2);
+                       a.lE[1], a.lE[2], dt * c2, (double*)nullptr, 0.0);
     // B' = B - h C+ E  ->  lE = lE' - h C- lB'
     hipLaunchKernelGGL(kCurlAdd<0>, dim3(blocksOf(n)), dim3(256), 0, s, a, a.lE[0], a.lE[1], a.lE[2], a.lB[0],
-                       a.lB[1], a.lB[2], -h);
+                       a.lB[1], a.lB[2], -h, (double*)nullptr, 0.0);
     return hipGetLastError();
 }
 

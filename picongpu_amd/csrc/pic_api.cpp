@@ -1150,8 +1150,7 @@ int diffStep(pic_ctx* c, const pic::DiffParticles& P, float* const E[3], float* 
     a.charge = m.charge;
     PIC_HIP(pic::launchDiffPush(a, P, c->stream));
     double const jinv = std::ldexp(1.0, -m.jexp);
-    for (int d = 0; d < 3; ++d)
-        PIC_HIP(pic::launchFixToFloat(m.Jfix + d * c->ncells, c->J[d], c->ncells, jinv, c->stream));
+    PIC_HIP(pic::launchFixToFloat(m.Jfix, c->J, c->ncells, jinv, c->stream));
     pic::FieldArgs f = makeFieldArgs(c);
     for (int d = 0; d < 3; ++d) {
         f.E[d] = E[d];
@@ -1164,20 +1163,29 @@ int diffStep(pic_ctx* c, const pic::DiffParticles& P, float* const E[3], float* 
     return PIC_OK;
 }
 
-pic::DiffParticles particlesOf(const AdjointMem& m, const DiffState& st) {
+// particles of `from` stepped into `to` (the same state for an in-place step)
+pic::DiffParticles particlesOf(const AdjointMem& m, const DiffState& from, const DiffState& to) {
     pic::DiffParticles P{};
-    P.cell = st.cell;
-    P.pos = st.pos;
-    P.mom = st.mom;
+    P.cell = from.cell;
+    P.pos = from.pos;
+    P.mom = from.mom;
     P.w = m.w;
+    P.cellOut = to.cell;
+    P.posOut = to.pos;
+    P.momOut = to.mom;
     P.n = m.n;
     return P;
 }
 
-int stepState(pic_ctx* c, const DiffState& st) {
-    float* const E[3] = {st.E, st.E + c->ncells, st.E + 2 * c->ncells};
-    float* const B[3] = {st.B, st.B + c->ncells, st.B + 2 * c->ncells};
-    return diffStep(c, particlesOf(c->adj, st), E, B);
+// state `to` = one step of state `from`: particles out of place, fields
+// copied and then updated in place
+int stepState(pic_ctx* c, const DiffState& from, const DiffState& to) {
+    int r;
+    if ((r = copyFields(c, to.E, from.E)) != PIC_OK) return r;
+    if ((r = copyFields(c, to.B, from.B)) != PIC_OK) return r;
+    float* const E[3] = {to.E, to.E + c->ncells, to.E + 2 * c->ncells};
+    float* const B[3] = {to.B, to.B + c->ncells, to.B + 2 * c->ncells};
+    return diffStep(c, particlesOf(c->adj, from, to), E, B);
 }
 
 }  // namespace
@@ -1270,7 +1278,7 @@ int pic_adjoint_forward(pic_ctx* c, const float* E0, const float* B0, int32_t ns
         if (allocState(c, st, m.n) != PIC_OK) return PIC_E_HIP;
         m.checkpoints.push_back(st);
     }
-    pic::DiffParticles const P = particlesOf(m, m.work);
+    pic::DiffParticles const P = particlesOf(m, m.work, m.work);
     for (int32_t i = 0; i < nsteps; ++i) {
         if (i % every == 0) {
             DiffState& cp = m.checkpoints[(size_t)(i / every)];
@@ -1339,12 +1347,8 @@ int pic_adjoint_backward(pic_ctx* c, const double* dE, const double* dB, double*
         if ((r = copyParticles(c, m.tape[0], cp, m.n)) != PIC_OK) return r;
         if ((r = copyFields(c, m.tape[0].E, cp.E)) != PIC_OK) return r;
         if ((r = copyFields(c, m.tape[0].B, cp.B)) != PIC_OK) return r;
-        for (int t = 1; t < len; ++t) {
-            if ((r = copyParticles(c, m.tape[(size_t)t], m.tape[(size_t)t - 1], m.n)) != PIC_OK) return r;
-            if ((r = copyFields(c, m.tape[(size_t)t].E, m.tape[(size_t)t - 1].E)) != PIC_OK) return r;
-            if ((r = copyFields(c, m.tape[(size_t)t].B, m.tape[(size_t)t - 1].B)) != PIC_OK) return r;
-            if ((r = stepState(c, m.tape[(size_t)t])) != PIC_OK) return r;
-        }
+        for (int t = 1; t < len; ++t)
+            if ((r = stepState(c, m.tape[(size_t)t - 1], m.tape[(size_t)t])) != PIC_OK) return r;
         for (int t = len - 1; t >= 0; --t) {
             const DiffState& st = m.tape[(size_t)t];
             PIC_HIP(pic::launchAdjFields(fa, dt, c2, coeff, lJ, c->stream));

@@ -145,10 +145,13 @@ hipError_t launchCurrentResetBatch(const FieldBatchEntry* table, const int2* blo
 // differentiable form (one CIC/Boris species in identity order) and its adjoint
 // ---------------------------------------------------------------------------
 struct DiffParticles {
-    int32_t* cell;  // global cell index, 3 planes of n
-    float* pos;     // in-cell position, 3 planes of n
-    float* mom;     // 3 planes of n
-    float* w;       // n
+    const int32_t* cell;  // global cell index, 3 planes of n
+    const float* pos;     // in-cell position, 3 planes of n
+    const float* mom;     // 3 planes of n
+    const float* w;       // n
+    int32_t* cellOut;     // the state after the step (may equal the input)
+    float* posOut;
+    float* momOut;
     int64_t n;
 };
 
@@ -198,7 +201,7 @@ struct AdjState {
 };
 
 hipError_t launchDiffPush(const DiffArgs& a, const DiffParticles& P, hipStream_t s);
-hipError_t launchFixToFloat(unsigned long long* fix, float* J, size_t n, double jinv, hipStream_t s);
+hipError_t launchFixToFloat(unsigned long long* fix, float* const J[3], size_t n, double jinv, hipStream_t s);
 hipError_t launchAdjFields(const AdjFieldArgs& a, double dt, double c2, double coeff, double* lJ[3], hipStream_t s);
 hipError_t launchAdjParticles(const AdjPartArgs& a, const AdjState& st, hipStream_t s);
 hipError_t launchFillD(double* p, size_t n, double v, hipStream_t s);
