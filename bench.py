@@ -188,6 +188,11 @@ def main():
               f"spatial domain decomposition, which this build does not run)", file=sys.stderr)
         sys.exit(2)
 
+    # torch first: its HIP device discovery must run before the library
+    # initialises the runtime in this process (the adjoint harness uses torch)
+    import torch
+
+    torch.cuda.is_available()
     from picongpu_amd import abi, setups
 
     if args.calibrate_copy:

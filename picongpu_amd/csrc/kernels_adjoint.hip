@@ -18,6 +18,8 @@
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 namespace pic {
 
 __device__ __forceinline__ int wrapI(int v, int n) {
@@ -402,20 +404,27 @@ __global__ __launch_bounds__(256) void kCurlAdd(AdjFieldArgs a, double* d0, doub
 
 
 
-struct D3 {
-    double x, y, z;
+template <typename R>
+struct V3 {
+    R x, y, z;
 };
-__device__ __forceinline__ D3 crossD(D3 a, D3 b) {
-    return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+template <typename R>
+__device__ __forceinline__ V3<R> crossD(V3<R> a, V3<R> b) {
+    return V3<R>{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ double dotD(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ D3 axpy(double s, D3 x, D3 y) { return D3{s * x.x + y.x, s * x.y + y.y, s * x.z + y.z}; }
-__device__ __forceinline__ D3 add3(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ D3 scale3(double s, D3 a) { return D3{s * a.x, s * a.y, s * a.z}; }
+template <typename R>
+__device__ __forceinline__ R dotD(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename R>
+__device__ __forceinline__ V3<R> axpy(R s, V3<R> x, V3<R> y) { return V3<R>{s * x.x + y.x, s * x.y + y.y, s * x.z + y.z}; }
+template <typename R>
+__device__ __forceinline__ V3<R> add3(V3<R> a, V3<R> b) { return V3<R>{a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename R>
+__device__ __forceinline__ V3<R> scale3(R s, V3<R> a) { return V3<R>{s * a.x, s * a.y, s * a.z}; }
 
 // Values the compiler must treat as new: recomputing from them is not merged
 // with earlier identical work, so intermediates are not kept live across phases.
-__device__ __forceinline__ double opaque(double x) {
+template <typename R>
+__device__ __forceinline__ R opaque(R x) {
     asm volatile("" : "+v"(x));
     return x;
 }
@@ -432,13 +441,15 @@ __device__ __forceinline__ void phaseBarrier() {
 }
 
 // CIC assignment 1-|k-p| on |k-p| < 1 and its derivative with respect to p
-__device__ __forceinline__ double cicD(double k, double p) {
-    double const u = k - p;
-    return fabs(u) < 1.0 ? 1.0 - fabs(u) : 0.0;
+template <typename R>
+__device__ __forceinline__ R cicD(R k, R p) {
+    R const u = k - p;
+    return fabs(u) < R(1.0) ? R(1.0) - fabs(u) : R(0.0);
 }
-__device__ __forceinline__ double cicDp(double k, double p) {
-    double const u = k - p;
-    return fabs(u) < 1.0 ? (u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0)) : 0.0;
+template <typename R>
+__device__ __forceinline__ R cicDp(R k, R p) {
+    R const u = k - p;
+    return fabs(u) < R(1.0) ? (u > R(0.0) ? R(1.0) : (u < R(0.0) ? -R(1.0) : R(0.0))) : R(0.0);
 }
 
 // Yee offsets of the E and B components (YeeCell.hpp:69-123)
@@ -451,36 +462,36 @@ struct YeeOff {
 
 // Node offsets and CIC weights of the interpolation of one Yee component at
 // in-cell position r of cell c (ShiftCoordinateSystem for an even support).
-template <int FIELD, int COMP>
-__device__ __forceinline__ void interpNodes(const GridDev& g, const int* c, const double* r, uint32_t (&o)[3][2],
-                                            double (&w)[3][2]) {
+template <int FIELD, int COMP, typename R>
+__device__ __forceinline__ void interpNodes(const GridDev& g, const int* c, const R* r, uint32_t (&o)[3][2],
+                                            R (&w)[3][2]) {
     int const n[3] = {g.nx, g.ny, g.nz};
     uint32_t const stride[3] = {1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        double const v = r[d] - YeeOff<FIELD, COMP>::f(d) - 0.5;
-        int const s = v >= -0.5 ? 0 : -1;
-        double const p = v - (double)s + 0.5;
-        w[d][0] = 1.0 - p;
+        R const v = r[d] - R(YeeOff<FIELD, COMP>::f(d)) - R(0.5);
+        int const s = v >= -R(0.5) ? 0 : -1;
+        R const p = v - (R)s + R(0.5);
+        w[d][0] = R(1.0) - p;
         w[d][1] = p;
         o[d][0] = (uint32_t)wrapI(c[d] + s, n[d]) * stride[d];
         o[d][1] = (uint32_t)wrapI(c[d] + s + 1, n[d]) * stride[d];
     }
 }
 
-template <int FIELD, int COMP>
-__device__ __forceinline__ double gatherD(const float* __restrict__ F, const GridDev& g, const int* c, const double* r) {
+template <int FIELD, int COMP, typename R>
+__device__ __forceinline__ R gatherD(const float* __restrict__ F, const GridDev& g, const int* c, const R* r) {
     uint32_t o[3][2];
-    double w[3][2];
+    R w[3][2];
     interpNodes<FIELD, COMP>(g, c, r, o, w);
-    double v = 0.0;
+    R v = R(0.0);
 #pragma unroll
     for (int zz = 0; zz < 2; ++zz)
 #pragma unroll
         for (int yy = 0; yy < 2; ++yy)
 #pragma unroll
             for (int xx = 0; xx < 2; ++xx)
-                v += (double)F[o[0][xx] + o[1][yy] + o[2][zz]] * (w[0][xx] * w[1][yy] * w[2][zz]);
+                v += (R)F[o[0][xx] + o[1][yy] + o[2][zz]] * (w[0][xx] * w[1][yy] * w[2][zz]);
     return v;
 }
 
@@ -488,24 +499,24 @@ __device__ __forceinline__ double gatherD(const float* __restrict__ F, const Gri
 // field adjoint at the 8 nodes (the block's LDS window plane L when LDS, with
 // rel the cell relative to the window reference; the global plane otherwise)
 // and lam * d(value)/dX to gX.
-template <int FIELD, int COMP, bool LDS>
+template <int FIELD, int COMP, bool LDS, typename R>
 __device__ __forceinline__ void gatherAdj(const float* __restrict__ F, double* __restrict__ L, const GridDev& g,
-                                          const BlockWindow& win, const int* c, const int* rel, const double* r,
-                                          double lam, double* gX) {
+                                          const BlockWindow& win, const int* c, const int* rel, const R* r,
+                                          R lam, R* gX) {
     uint32_t o[3][2];
-    double w[3][2];
+    R w[3][2];
     interpNodes<FIELD, COMP>(g, c, r, o, w);
     uint32_t lo[3][2];
     if (LDS) {
         uint32_t const ls[3] = {1u, (uint32_t)win.ext[0], (uint32_t)(win.ext[0] * win.ext[1])};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            int const sh = (r[d] - YeeOff<FIELD, COMP>::f(d) - 0.5) >= -0.5 ? 0 : -1;
+            int const sh = (r[d] - R(YeeOff<FIELD, COMP>::f(d)) - R(0.5)) >= -R(0.5) ? 0 : -1;
             lo[d][0] = (uint32_t)(rel[d] + sh - win.lo[d]) * ls[d];
             lo[d][1] = lo[d][0] + ls[d];
         }
     }
-    double gx = 0.0, gy = 0.0, gz = 0.0;
+    R gx = R(0.0), gy = R(0.0), gz = R(0.0);
 #pragma unroll
     for (int zz = 0; zz < 2; ++zz)
 #pragma unroll
@@ -513,12 +524,12 @@ __device__ __forceinline__ void gatherAdj(const float* __restrict__ F, double* _
 #pragma unroll
             for (int xx = 0; xx < 2; ++xx) {
                 uint32_t const at = o[0][xx] + o[1][yy] + o[2][zz];
-                double const f = (double)F[at];
-                double const sx = xx ? 1.0 : -1.0, sy = yy ? 1.0 : -1.0, sz = zz ? 1.0 : -1.0;
+                R const f = (R)F[at];
+                R const sx = xx ? R(1.0) : -R(1.0), sy = yy ? R(1.0) : -R(1.0), sz = zz ? R(1.0) : -R(1.0);
                 gx += f * sx * w[1][yy] * w[2][zz];
                 gy += f * w[0][xx] * sy * w[2][zz];
                 gz += f * w[0][xx] * w[1][yy] * sz;
-                double const v = lam * (w[0][xx] * w[1][yy] * w[2][zz]);
+                double const v = (double)(lam * (w[0][xx] * w[1][yy] * w[2][zz]));
                 if (LDS)
                     __hip_atomic_fetch_add(L + (lo[0][xx] + lo[1][yy] + lo[2][zz]), v, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -538,10 +549,10 @@ __device__ __forceinline__ void gatherAdj(const float* __restrict__ F, double* _
 // -csd W(i,j) C(k), W = S0a S0b + (DSa S0b + S0a DSb)/2 + DSa DSb/3 and C(k)
 // the prefix sum of DS along c (Esirkepov.hpp:140-240). Accumulates
 // dF/dX0 into l0 and dF/dX1 into l1 per axis.
-template <int COMP>
+template <int COMP, typename R>
 __device__ __forceinline__ void esirkepovAdjComp(const double* __restrict__ lJ, const GridDev& g, const int* bc,
-                                                 const double* X0, const double* X1, double csd, double* l0,
-                                                 double* l1) {
+                                                 const R* X0, const R* X1, R csd, R* l0,
+                                                 R* l1) {
     constexpr int A = (COMP + 1) % 3;  // rotated i axis
     constexpr int B = (COMP + 2) % 3;  // rotated j axis
     int const n[3] = {g.nx, g.ny, g.nz};
@@ -551,41 +562,41 @@ __device__ __forceinline__ void esirkepovAdjComp(const double* __restrict__ lJ, 
     uint32_t ob[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) ob[j] = (uint32_t)wrapI(bc[B] + j, n[B]) * stride[B];
-    double const C0 = cicD(0.0, X1[COMP]) - cicD(0.0, X0[COMP]);
-    double const C1 = C0 + (cicD(1.0, X1[COMP]) - cicD(1.0, X0[COMP]));
-    double dC0 = 0.0, dC1 = 0.0;
-    double l0a = 0.0, l1a = 0.0, l0b = 0.0, l1b = 0.0;
+    R const C0 = cicD(R(0.0), X1[COMP]) - cicD(R(0.0), X0[COMP]);
+    R const C1 = C0 + (cicD(R(1.0), X1[COMP]) - cicD(R(1.0), X0[COMP]));
+    R dC0 = R(0.0), dC1 = R(0.0);
+    R l0a = R(0.0), l1a = R(0.0), l0b = R(0.0), l1b = R(0.0);
 #pragma unroll 1
     for (int i = 0; i < 3; ++i) {
         uint32_t const oa = (uint32_t)wrapI(bc[A] + i, n[A]) * stride[A];
         // the six current adjoints of this row, loaded together
-        double m[3][2];
+        R m[3][2];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             m[j][0] = lJ[oa + ob[j] + k0];
             m[j][1] = lJ[oa + ob[j] + k1];
         }
-        double const s0a = cicD((double)i, X0[A]);
-        double const dsa = cicD((double)i, X1[A]) - s0a;
-        double const d0a = cicDp((double)i, X0[A]);
-        double const d1a = cicDp((double)i, X1[A]);
+        R const s0a = cicD((R)i, X0[A]);
+        R const dsa = cicD((R)i, X1[A]) - s0a;
+        R const d0a = cicDp((R)i, X0[A]);
+        R const d1a = cicDp((R)i, X1[A]);
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            double const s0b = cicD((double)j, X0[B]);
-            double const dsb = cicD((double)j, X1[B]) - s0b;
-            double const W = s0a * s0b + 0.5 * (dsa * s0b + s0a * dsb) + (1.0 / 3.0) * dsa * dsb;
-            double const m0 = -csd * m[j][0];
-            double const m1 = -csd * m[j][1];
-            double const G = m0 * C0 + m1 * C1;
+            R const s0b = cicD((R)j, X0[B]);
+            R const dsb = cicD((R)j, X1[B]) - s0b;
+            R const W = s0a * s0b + R(0.5) * (dsa * s0b + s0a * dsb) + (R(1.0) / R(3.0)) * dsa * dsb;
+            R const m0 = -csd * m[j][0];
+            R const m1 = -csd * m[j][1];
+            R const G = m0 * C0 + m1 * C1;
             dC0 += m0 * W;
             dC1 += m1 * W;
             // dW/dS0a = S0b + DSb/2 - dW/dDSa, dW/dS1a = dW/dDSa = S0b/2 + DSb/3 (and a <-> b)
-            double const wa = 0.5 * s0b + (1.0 / 3.0) * dsb;
-            double const wb = 0.5 * s0a + (1.0 / 3.0) * dsa;
-            l0a += G * (s0b + 0.5 * dsb - wa) * d0a;
+            R const wa = R(0.5) * s0b + (R(1.0) / R(3.0)) * dsb;
+            R const wb = R(0.5) * s0a + (R(1.0) / R(3.0)) * dsa;
+            l0a += G * (s0b + R(0.5) * dsb - wa) * d0a;
             l1a += G * wa * d1a;
-            l0b += G * (s0a + 0.5 * dsa - wb) * cicDp((double)j, X0[B]);
-            l1b += G * wb * cicDp((double)j, X1[B]);
+            l0b += G * (s0a + R(0.5) * dsa - wb) * cicDp((R)j, X0[B]);
+            l1b += G * wb * cicDp((R)j, X1[B]);
         }
     }
     l0[A] += l0a;
@@ -593,117 +604,123 @@ __device__ __forceinline__ void esirkepovAdjComp(const double* __restrict__ lJ, 
     l0[B] += l0b;
     l1[B] += l1b;
     // C0 = DS_0, C1 = DS_0 + DS_1 along the current's own axis
-    l1[COMP] += (dC0 + dC1) * cicDp(0.0, X1[COMP]) + dC1 * cicDp(1.0, X1[COMP]);
-    l0[COMP] -= (dC0 + dC1) * cicDp(0.0, X0[COMP]) + dC1 * cicDp(1.0, X0[COMP]);
+    l1[COMP] += (dC0 + dC1) * cicDp(R(0.0), X1[COMP]) + dC1 * cicDp(R(1.0), X1[COMP]);
+    l0[COMP] -= (dC0 + dC1) * cicDp(R(0.0), X0[COMP]) + dC1 * cicDp(R(1.0), X0[COMP]);
 }
 
+template <typename R>
 struct BorisFwd {
-    D3 pm, t, s, pp, p1;
-    double g, f, beta, hq;
+    V3<R> pm, t, s, pp, p1;
+    R g, f, beta, hq;
 };
 
-// Boris push in FP64 (particlePusherBoris.hpp:41-90)
-__device__ __forceinline__ BorisFwd borisFwd(D3 p0, D3 E, D3 B, double m, double q, double dt, double rc2) {
-    BorisFwd b;
-    b.hq = 0.5 * q * dt;
+// Boris push in the adjoint's precision (particlePusherBoris.hpp:41-90)
+template <typename R>
+__device__ __forceinline__ BorisFwd<R> borisFwd(V3<R> p0, V3<R> E, V3<R> B, R m, R q, R dt, R rc2) {
+    BorisFwd<R> b;
+    b.hq = R(0.5) * q * dt;
     b.pm = axpy(b.hq, E, p0);
-    b.g = sqrt(1.0 + dotD(b.pm, b.pm) * rc2 / (m * m));
-    b.beta = 0.5 * (q / m) * dt;
+    b.g = sqrt(R(1.0) + dotD(b.pm, b.pm) * rc2 / (m * m));
+    b.beta = R(0.5) * (q / m) * dt;
     b.t = scale3(b.beta / b.g, B);
-    b.f = 1.0 / (1.0 + dotD(b.t, b.t));
-    b.s = scale3(2.0 * b.f, b.t);
+    b.f = R(1.0) / (R(1.0) + dotD(b.t, b.t));
+    b.s = scale3(R(2.0) * b.f, b.t);
     b.pp = add3(b.pm, crossD(b.pm, b.t));
     b.p1 = axpy(b.hq, E, add3(b.pm, crossD(b.pp, b.s)));
     return b;
 }
 
+template <typename R>
 __device__ __forceinline__ void adjointParticle(const AdjPartArgs& a, const AdjState& S, int64_t idx, const int* c,
                                                 const BlockWindow& win, const int* rel, double* sL, bool lds) {
     int64_t const n = S.n;
-    double const r[3] = {(double)S.pos[idx], (double)S.pos[n + idx], (double)S.pos[2 * n + idx]};
-    D3 const p0{(double)S.mom[idx], (double)S.mom[n + idx], (double)S.mom[2 * n + idx]};
-    double const w = (double)S.w[idx];
-    double const m = a.mass * w;
-    double const q = a.charge * w;
-    double const dt = a.dt;
+    R const r[3] = {(R)S.pos[idx], (R)S.pos[n + idx], (R)S.pos[2 * n + idx]};
+    V3<R> const p0{(R)S.mom[idx], (R)S.mom[n + idx], (R)S.mom[2 * n + idx]};
+    R const w = (R)S.w[idx];
+    R const m = (R)a.mass * w;
+    R const q = (R)a.charge * w;
+    R const dt = (R)a.dt;
+    R const rc2 = (R)a.rc2;
+    R const cellR[3] = {(R)a.cell[0], (R)a.cell[1], (R)a.cell[2]};
+    R const ivdt = (R)a.inv_vdt;
 
     // ---- forward recompute: gather, Boris, velocity ----
-    D3 const E{gatherD<0, 0>(a.E[0], a.g, c, r), gatherD<0, 1>(a.E[1], a.g, c, r), gatherD<0, 2>(a.E[2], a.g, c, r)};
+    V3<R> const E{gatherD<0, 0>(a.E[0], a.g, c, r), gatherD<0, 1>(a.E[1], a.g, c, r), gatherD<0, 2>(a.E[2], a.g, c, r)};
     phaseBarrier();
-    D3 const B{gatherD<1, 0>(a.B[0], a.g, c, r), gatherD<1, 1>(a.B[1], a.g, c, r), gatherD<1, 2>(a.B[2], a.g, c, r)};
+    V3<R> const B{gatherD<1, 0>(a.B[0], a.g, c, r), gatherD<1, 1>(a.B[1], a.g, c, r), gatherD<1, 2>(a.B[2], a.g, c, r)};
     phaseBarrier();
-    D3 p1;
+    V3<R> p1;
     {
-        BorisFwd const b = borisFwd(p0, E, B, m, q, dt, a.rc2);
+        BorisFwd<R> const b = borisFwd(p0, E, B, m, q, dt, rc2);
         p1 = b.p1;
     }
-    double const G = sqrt(m * m + dotD(p1, p1) * a.rc2);
-    double const v[3] = {p1.x / G, p1.y / G, p1.z / G};
+    R const G = sqrt(m * m + dotD(p1, p1) * rc2);
+    R const v[3] = {p1.x / G, p1.y / G, p1.z / G};
 
     // ---- reverse Esirkepov ----
-    double lX1[3], lv[3];
+    R lX1[3], lv[3];
     {
-        double X0[3], X1[3];
+        R X0[3], X1[3];
         int bc[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            X1[d] = r[d] + v[d] * dt / a.cell[d];
-            X0[d] = X1[d] - v[d] * dt / a.cell[d];
+            X1[d] = r[d] + v[d] * dt / cellR[d];
+            X0[d] = X1[d] - v[d] * dt / cellR[d];
             int const gs = (int)fmin(floor(X0[d]), floor(X1[d]));
-            X0[d] -= (double)gs;
-            X1[d] -= (double)gs;
+            X0[d] -= (R)gs;
+            X1[d] -= (R)gs;
             bc[d] = c[d] + gs;
         }
-        double l0[3] = {0.0, 0.0, 0.0}, l1[3] = {0.0, 0.0, 0.0};
+        R l0[3] = {R(0.0), R(0.0), R(0.0)}, l1[3] = {R(0.0), R(0.0), R(0.0)};
         phaseBarrier();
-        esirkepovAdjComp<0>(a.lJ[0], a.g, bc, X0, X1, q * a.inv_vdt * a.cell[0], l0, l1);
+        esirkepovAdjComp<0>(a.lJ[0], a.g, bc, X0, X1, q * ivdt * cellR[0], l0, l1);
         phaseBarrier();
-        esirkepovAdjComp<1>(a.lJ[1], a.g, bc, X0, X1, q * a.inv_vdt * a.cell[1], l0, l1);
+        esirkepovAdjComp<1>(a.lJ[1], a.g, bc, X0, X1, q * ivdt * cellR[1], l0, l1);
         phaseBarrier();
-        esirkepovAdjComp<2>(a.lJ[2], a.g, bc, X0, X1, q * a.inv_vdt * a.cell[2], l0, l1);
+        esirkepovAdjComp<2>(a.lJ[2], a.g, bc, X0, X1, q * ivdt * cellR[2], l0, l1);
         phaseBarrier();
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             // X1 = X + v dt/cell, X0 = X1 - v dt/cell
-            lX1[d] = S.lX[d * n + idx] + l0[d] + l1[d];
-            lv[d] = (lX1[d] - l0[d]) * dt / a.cell[d];
+            lX1[d] = (R)S.lX[d * n + idx] + l0[d] + l1[d];
+            lv[d] = (lX1[d] - l0[d]) * dt / cellR[d];
         }
     }
 
     phaseBarrier();
     // ---- reverse velocity: v = p1 / sqrt(m^2 + |p1|^2 rc2) ----
-    D3 const LV{lv[0], lv[1], lv[2]};
-    double const kv = dotD(p1, LV) * a.rc2 / (G * G * G);
-    D3 const l1{S.lP[idx] + LV.x / G - p1.x * kv, S.lP[n + idx] + LV.y / G - p1.y * kv,
-                S.lP[2 * n + idx] + LV.z / G - p1.z * kv};
+    V3<R> const LV{lv[0], lv[1], lv[2]};
+    R const kv = dotD(p1, LV) * rc2 / (G * G * G);
+    V3<R> const l1{(R)S.lP[idx] + LV.x / G - p1.x * kv, (R)S.lP[n + idx] + LV.y / G - p1.y * kv,
+                   (R)S.lP[2 * n + idx] + LV.z / G - p1.z * kv};
 
     // ---- reverse Boris ----
-    D3 lE, lB, lp0;
+    V3<R> lE, lB, lp0;
     {
-        D3 const p0o{opaque(p0.x), opaque(p0.y), opaque(p0.z)};
-        D3 const Eo{opaque(E.x), opaque(E.y), opaque(E.z)};
-        D3 const Bo{opaque(B.x), opaque(B.y), opaque(B.z)};
-        BorisFwd const b = borisFwd(p0o, Eo, Bo, m, q, dt, a.rc2);
+        V3<R> const p0o{opaque(p0.x), opaque(p0.y), opaque(p0.z)};
+        V3<R> const Eo{opaque(E.x), opaque(E.y), opaque(E.z)};
+        V3<R> const Bo{opaque(B.x), opaque(B.y), opaque(B.z)};
+        BorisFwd<R> const b = borisFwd(p0o, Eo, Bo, m, q, dt, rc2);
         lE = scale3(b.hq, l1);                   // p1 = pl + hq E
-        D3 lpm = l1;                             // pl = pm + pp x s
-        D3 const lpp = crossD(b.s, l1);
-        D3 const ls = crossD(l1, b.pp);
+        V3<R> lpm = l1;                             // pl = pm + pp x s
+        V3<R> const lpp = crossD(b.s, l1);
+        V3<R> const ls = crossD(l1, b.pp);
         lpm = add3(lpm, add3(lpp, crossD(b.t, lpp)));  // pp = pm + pm x t
-        D3 const lt1 = crossD(lpp, b.pm);
-        double const tls = dotD(b.t, ls);              // s = 2 f t, f = 1/(1+|t|^2)
-        D3 const lt = add3(lt1, add3(scale3(2.0 * b.f, ls), scale3(-4.0 * b.f * b.f * tls, b.t)));
+        V3<R> const lt1 = crossD(lpp, b.pm);
+        R const tls = dotD(b.t, ls);              // s = 2 f t, f = 1/(1+|t|^2)
+        V3<R> const lt = add3(lt1, add3(scale3(R(2.0) * b.f, ls), scale3(-R(4.0) * b.f * b.f * tls, b.t)));
         lB = scale3(b.beta / b.g, lt);                 // t = beta B / g
-        double const lg = -b.beta * dotD(B, lt) / (b.g * b.g);
-        lpm = axpy(lg * a.rc2 / (m * m * b.g), b.pm, lpm);  // g = sqrt(1 + |pm|^2 rc2/m^2)
+        R const lg = -b.beta * dotD(B, lt) / (b.g * b.g);
+        lpm = axpy(lg * rc2 / (m * m * b.g), b.pm, lpm);  // g = sqrt(1 + |pm|^2 rc2/m^2)
         lE = axpy(b.hq, lpm, lE);                      // pm = p0 + hq E
         lp0 = lpm;
     }
 
     phaseBarrier();
     // ---- reverse gather (weights recomputed, not kept from the forward gather) ----
-    double lX0[3] = {lX1[0], lX1[1], lX1[2]};
+    R lX0[3] = {lX1[0], lX1[1], lX1[2]};
     int const co[3] = {opaqueI(c[0]), opaqueI(c[1]), opaqueI(c[2])};
-    double const ro[3] = {opaque(r[0]), opaque(r[1]), opaque(r[2])};
+    R const ro[3] = {opaque(r[0]), opaque(r[1]), opaque(r[2])};
     if (lds) {
         gatherAdj<0, 0, true>(a.E[0], sL, a.g, win, co, rel, ro, lE.x, lX0);
         phaseBarrier();
@@ -741,7 +758,12 @@ __device__ __forceinline__ void adjointParticle(const AdjPartArgs& a, const AdjS
 // FP64 LDS window of the field adjoint scatter: 6 components of this many cells
 constexpr int kAdjLdsCells = 640;
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void kAdjParticles(AdjPartArgs a,
+// Precision of the per-particle adjoint arithmetic (Jacobian-vector
+// products); the adjoint state (lambda of particles and fields) is always
+// accumulated in FP64.
+using AdjReal = float;
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void kAdjParticles(AdjPartArgs a,
                                                                                                  AdjState S) {
     __shared__ double sL[6 * kAdjLdsCells];
     __shared__ int sBox[9];
@@ -760,7 +782,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     if (win.lds) {
         for (int e = threadIdx.x; e < 6 * win.vol; e += blockDim.x) sL[e] = 0.0;
         __syncthreads();
-        if (active) adjointParticle(a, S, idx, c, win, rel, sL, windowHolds<1, 1>(win, rel));
+        if (active) adjointParticle<AdjReal>(a, S, idx, c, win, rel, sL, windowHolds<1, 1>(win, rel));
         __syncthreads();
         for (int e = threadIdx.x; e < 6 * win.vol; e += blockDim.x) {
             double const v = sL[e];
@@ -776,6 +798,83 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 __global__ __launch_bounds__(256) void kFillD(double* p, size_t n, double v) {
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// re-tiling of the particle order at checkpoints
+// ---------------------------------------------------------------------------
+// key = tile of 8x4x1 cells of the particle's cell (x-fastest tile order), value = slot
+__global__ __launch_bounds__(256) void kTileKeys(const int32_t* __restrict__ cell, int64_t n, GridDev g,
+                                                 uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    int64_t const i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t const ntx = (uint32_t)g.nx / kTileX, nty = (uint32_t)g.ny / kTileY;
+    uint32_t const tx = (uint32_t)cell[i] / kTileX, ty = (uint32_t)cell[n + i] / kTileY;
+    uint32_t const tz = (uint32_t)cell[2 * n + i] / kTileZ;
+    keys[i] = tx + ntx * (ty + nty * tz);
+    vals[i] = (int32_t)i;
+}
+
+// dst slot j takes the particle of src slot perm[j]; ids follow the particles
+__global__ __launch_bounds__(256) void kGatherParticles(const int32_t* __restrict__ perm, DiffParticles src,
+                                                        DiffParticles dst, const int32_t* __restrict__ idsIn,
+                                                        int32_t* __restrict__ idsOut, float* __restrict__ wOut) {
+    int64_t const j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t const n = src.n;
+    if (j >= n) return;
+    int64_t const i = perm[j];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        dst.cellOut[d * n + j] = src.cell[d * n + i];
+        dst.posOut[d * n + j] = src.pos[d * n + i];
+        dst.momOut[d * n + j] = src.mom[d * n + i];
+    }
+    wOut[j] = src.w[i];
+    idsOut[j] = idsIn[i];
+}
+
+// adjoint of the re-tiling: dst[perm[j]] = src[j] for 3 planes of n
+__global__ __launch_bounds__(256) void kScatterAdjoint(const int32_t* __restrict__ perm, const double* __restrict__ src,
+                                                       double* __restrict__ dst, int64_t n) {
+    int64_t const j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int64_t const i = perm[j];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dst[d * n + i] = src[d * n + j];
+}
+
+hipError_t sortTempBytes(int64_t n, size_t* bytes) {
+    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (const int32_t*)nullptr, (int32_t*)nullptr, (int)n, 0, 32, nullptr);
+}
+
+hipError_t launchTileSort(const int32_t* cell, int64_t n, const GridDev& g, uint32_t* keys[2], int32_t* vals[2],
+                          void* temp, size_t tempBytes, int32_t** perm, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kTileKeys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cell, n, g, keys[0], vals[0]);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    uint32_t const tiles = (uint32_t)(g.nx / kTileX) * (uint32_t)(g.ny / kTileY) * (uint32_t)(g.nz / kTileZ);
+    int bits = 1;
+    while (bits < 32 && (1u << bits) < tiles) ++bits;
+    size_t b = tempBytes;
+    e = hipcub::DeviceRadixSort::SortPairs(temp, b, keys[0], keys[1], vals[0], vals[1], (int)n, 0, bits, s);
+    *perm = vals[1];
+    return e;
+}
+
+hipError_t launchGatherParticles(const int32_t* perm, const DiffParticles& src, const DiffParticles& dst,
+                                 const int32_t* idsIn, int32_t* idsOut, float* wOut, hipStream_t s) {
+    if (src.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kGatherParticles, dim3((unsigned)((src.n + 255) / 256)), dim3(256), 0, s, perm, src, dst,
+                       idsIn, idsOut, wOut);
+    return hipGetLastError();
+}
+
+hipError_t launchScatterAdjoint(const int32_t* perm, const double* src, double* dst, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kScatterAdjoint, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, perm, src, dst, n);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -119,6 +119,16 @@ __global__ __launch_bounds__(256) void kEFull(FieldArgs a) {
 constexpr int kFTX = 8, kFTY = 8, kFTZ = 4;
 constexpr int kFHX = kFTX + 1, kFHY = kFTY + 1, kFHZ = kFTZ + 1;
 constexpr int kFHV = kFHX * kFHY * kFHZ;
+// E staged for the curl of B' on the tile and its lower halo: 2 cells of halo
+// on each side cover the Yee stencil and the Lehe stencil along any axis
+// (Lehe/Derivative.hpp:132-145: f(i+2) - f(i-1) along the Cherenkov-free axis,
+// shifts of one cell across it)
+constexpr int kEH = 2;
+constexpr int kEX = kFTX + 2 * kEH, kEY = kFTY + 2 * kEH, kEZ = kFTZ + 2 * kEH;
+constexpr int kEV = kEX * kEY * kEZ;
+// E and the new E staged for kFieldAfter: halo 1 below, 2 above
+constexpr int kAX = kFTX + 3, kAY = kFTY + 3, kAZ = kFTZ + 3;
+constexpr int kAV = kAX * kAY * kAZ;
 
 template <int SOLVER>
 __device__ __forceinline__ void curlE(const FieldArgs& a, float* const* E, const Idx& I, float& cx, float& cy,
@@ -128,36 +138,100 @@ __device__ __forceinline__ void curlE(const FieldArgs& a, float* const* E, const
     cz = derivE<SOLVER>(E[1], I, 0, a) - derivE<SOLVER>(E[0], I, 1, a);
 }
 
+// Derivatives of a component staged in LDS with strides (1, SY, SZ): the
+// float operations of fwd/leheDeriv on the global arrays
+template <int SY, int SZ>
+struct LdsStencil {
+    static __device__ __forceinline__ int off(int d) { return d == 0 ? 1 : (d == 1 ? SY : SZ); }
+    static __device__ __forceinline__ float fwd(const float* f, int h, int d, float step) {
+        return (f[h + off(d)] - f[h]) / step;
+    }
+    static __device__ __forceinline__ float lehe(const float* f, int h, int d, const FieldArgs& a) {
+        int const L = a.lehe_dir;
+        float const step = a.cell[d];
+        if (d == L) {
+            int const o1 = off((d + 1) % 3), o2 = off((d + 2) % 3), o0 = off(d);
+            float r = a.lehe_alpha * fwd(f, h, d, step);
+            r = r + a.lehe_beta1 * fwd(f, h + o1, d, step);
+            r = r + a.lehe_beta1 * fwd(f, h - o1, d, step);
+            r = r + a.lehe_beta2 * fwd(f, h + o2, d, step);
+            r = r + a.lehe_beta2 * fwd(f, h - o2, d, step);
+            r = r + a.lehe_delta * (f[h + 2 * o0] - f[h - o0]) / step;
+            return r;
+        }
+        int const oL = off(L);
+        float const beta = 0.125f;
+        float const alpha = 1.0f - 2.0f * beta;
+        float r = alpha * fwd(f, h, d, step);
+        r = r + beta * fwd(f, h + oL, d, step);
+        r = r + beta * fwd(f, h - oL, d, step);
+        return r;
+    }
+    template <int SOLVER>
+    static __device__ __forceinline__ float deriv(const float* f, int h, int d, const FieldArgs& a) {
+        if (SOLVER == PIC_SOLVER_LEHE) return lehe(f, h, d, a);
+        return fwd(f, h, d, a.cell[d]);
+    }
+    // curl of E with components at f0, f1, f2 (each a plane of the staged tile)
+    template <int SOLVER>
+    static __device__ __forceinline__ void curl(const float* f0, const float* f1, const float* f2, int h,
+                                                const FieldArgs& a, float& cx, float& cy, float& cz) {
+        cx = deriv<SOLVER>(f2, h, 1, a) - deriv<SOLVER>(f1, h, 2, a);
+        cy = deriv<SOLVER>(f0, h, 2, a) - deriv<SOLVER>(f2, h, 0, a);
+        cz = deriv<SOLVER>(f1, h, 0, a) - deriv<SOLVER>(f0, h, 1, a);
+    }
+};
+
+// global index of the cell at tile-relative offset (tx, ty, tz) from (x0, y0, z0), wrapped
+__device__ __forceinline__ size_t wrapAt(const GridDev& g, int x0, int y0, int z0, int tx, int ty, int tz) {
+    int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+    y = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
+    z = z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z);
+    return (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * (size_t)z);
+}
+
+struct FieldLds {
+    float E[3][kEV];
+    float B[3][kFHV];
+};
+
 // tile b of the grid: every thread of the 256-thread workgroup calls it
 template <int SOLVER>
-__device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* const* Enew, int b,
-                                                   float (*sB)[kFHV]) {
+__device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* const* Enew, int b, FieldLds& L) {
     int const ntx = a.g.nx / kFTX, nty = a.g.ny / kFTY;
     int const x0 = (b % ntx) * kFTX, y0 = ((b / ntx) % nty) * kFTY, z0 = (b / (ntx * nty)) * kFTZ;
+    for (int t = threadIdx.x; t < 3 * kEV; t += blockDim.x) {
+        int const c = t / kEV, r = t - c * kEV;
+        int const ex = r % kEX, ey = (r / kEX) % kEY, ez = r / (kEX * kEY);
+        L.E[c][r] = a.E[c][wrapAt(a.g, x0, y0, z0, ex - kEH, ey - kEH, ez - kEH)];
+    }
+    __syncthreads();
+    using SE = LdsStencil<kEX, kEX * kEY>;
     for (int t = threadIdx.x; t < kFHV; t += blockDim.x) {
         int const hx = t % kFHX, hy = (t / kFHX) % kFHY, hz = t / (kFHX * kFHY);
-        Idx const I0{a.g.nx, a.g.ny, a.g.nz, x0, y0, z0};
-        size_t const i = I0.at(hx - 1, hy - 1, hz - 1);
-        Idx const I{a.g.nx, a.g.ny, a.g.nz, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny),
-                    (int)(i / ((size_t)a.g.nx * a.g.ny))};
+        // B' at tile offset (hx-1, hy-1, hz-1)
+        int const he = (hx - 1 + kEH) + kEX * ((hy - 1 + kEH) + kEY * (hz - 1 + kEH));
+        size_t const i = wrapAt(a.g, x0, y0, z0, hx - 1, hy - 1, hz - 1);
         float cx, cy, cz;
-        curlE<SOLVER>(a, a.E, I, cx, cy, cz);
-        sB[0][t] = a.B[0][i] - cx * 0.5f * a.dt;
-        sB[1][t] = a.B[1][i] - cy * 0.5f * a.dt;
-        sB[2][t] = a.B[2][i] - cz * 0.5f * a.dt;
+        SE::template curl<SOLVER>(L.E[0], L.E[1], L.E[2], he, a, cx, cy, cz);
+        L.B[0][t] = a.B[0][i] - cx * 0.5f * a.dt;
+        L.B[1][t] = a.B[1][i] - cy * 0.5f * a.dt;
+        L.B[2][t] = a.B[2][i] - cz * 0.5f * a.dt;
     }
     __syncthreads();
     int const lx = threadIdx.x % kFTX, ly = (threadIdx.x / kFTX) % kFTY, lz = threadIdx.x / (kFTX * kFTY);
     size_t const i = (size_t)(x0 + lx) + (size_t)a.g.nx * ((size_t)(y0 + ly) + (size_t)a.g.ny * (size_t)(z0 + lz));
     int const h = (lx + 1) + kFHX * ((ly + 1) + kFHY * (lz + 1));
+    int const he = (lx + kEH) + kEX * ((ly + kEH) + kEY * (lz + kEH));
     constexpr int SY = kFHX, SZ = kFHX * kFHY;
     // backward differences of B' (BackwardDerivative.hpp:44-65), curl as kEFull
-    float const cx = (sB[2][h] - sB[2][h - SY]) / a.cell[1] - (sB[1][h] - sB[1][h - SZ]) / a.cell[2];
-    float const cy = (sB[0][h] - sB[0][h - SZ]) / a.cell[2] - (sB[2][h] - sB[2][h - 1]) / a.cell[0];
-    float const cz = (sB[1][h] - sB[1][h - 1]) / a.cell[0] - (sB[0][h] - sB[0][h - SY]) / a.cell[1];
-    float e0 = a.E[0][i] + cx * a.c2 * a.dt;
-    float e1 = a.E[1][i] + cy * a.c2 * a.dt;
-    float e2 = a.E[2][i] + cz * a.c2 * a.dt;
+    float const cx = (L.B[2][h] - L.B[2][h - SY]) / a.cell[1] - (L.B[1][h] - L.B[1][h - SZ]) / a.cell[2];
+    float const cy = (L.B[0][h] - L.B[0][h - SZ]) / a.cell[2] - (L.B[2][h] - L.B[2][h - 1]) / a.cell[0];
+    float const cz = (L.B[1][h] - L.B[1][h - 1]) / a.cell[0] - (L.B[0][h] - L.B[0][h - SY]) / a.cell[1];
+    float e0 = L.E[0][he] + cx * a.c2 * a.dt;
+    float e1 = L.E[1][he] + cy * a.c2 * a.dt;
+    float e2 = L.E[2][he] + cz * a.c2 * a.dt;
     e0 = e0 + a.coeff * a.J[0][i];
     e1 = e1 + a.coeff * a.J[1][i];
     e2 = e2 + a.coeff * a.J[2][i];
@@ -166,15 +240,34 @@ __device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* co
     Enew[2][i] = e2;
 }
 
+struct AfterLds {
+    float E[3][kAV];
+    float En[3][kAV];
+};
+
+// B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt on tile b
 template <int SOLVER>
-__device__ __forceinline__ void fieldAfterCell(const FieldArgs& a, float* const* Enew, size_t i) {
-    Idx I{a.g.nx, a.g.ny, a.g.nz, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+__device__ __forceinline__ void fieldAfterTile(const FieldArgs& a, float* const* Enew, int b, AfterLds& L) {
+    int const ntx = a.g.nx / kFTX, nty = a.g.ny / kFTY;
+    int const x0 = (b % ntx) * kFTX, y0 = ((b / ntx) % nty) * kFTY, z0 = (b / (ntx * nty)) * kFTZ;
+    for (int t = threadIdx.x; t < 3 * kAV; t += blockDim.x) {
+        int const c = t / kAV, r = t - c * kAV;
+        int const ex = r % kAX, ey = (r / kAX) % kAY, ez = r / (kAX * kAY);
+        size_t const gi = wrapAt(a.g, x0, y0, z0, ex - 1, ey - 1, ez - 1);
+        L.E[c][r] = a.E[c][gi];
+        L.En[c][r] = Enew[c][gi];
+    }
+    __syncthreads();
+    using SA = LdsStencil<kAX, kAX * kAY>;
+    int const lx = threadIdx.x % kFTX, ly = (threadIdx.x / kFTX) % kFTY, lz = threadIdx.x / (kFTX * kFTY);
+    size_t const i = (size_t)(x0 + lx) + (size_t)a.g.nx * ((size_t)(y0 + ly) + (size_t)a.g.ny * (size_t)(z0 + lz));
+    int const h = (lx + 1) + kAX * ((ly + 1) + kAY * (lz + 1));
     float cx, cy, cz;
-    curlE<SOLVER>(a, a.E, I, cx, cy, cz);
+    SA::template curl<SOLVER>(L.E[0], L.E[1], L.E[2], h, a, cx, cy, cz);
     float const b0 = a.B[0][i] - cx * 0.5f * a.dt;
     float const b1 = a.B[1][i] - cy * 0.5f * a.dt;
     float const b2 = a.B[2][i] - cz * 0.5f * a.dt;
-    curlE<SOLVER>(a, Enew, I, cx, cy, cz);
+    SA::template curl<SOLVER>(L.En[0], L.En[1], L.En[2], h, a, cx, cy, cz);
     a.B[0][i] = b0 - cx * 0.5f * a.dt;
     a.B[1][i] = b1 - cy * 0.5f * a.dt;
     a.B[2][i] = b2 - cz * 0.5f * a.dt;
@@ -182,39 +275,35 @@ __device__ __forceinline__ void fieldAfterCell(const FieldArgs& a, float* const*
 
 template <int SOLVER>
 __global__ __launch_bounds__(256) void kFieldBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2) {
-    __shared__ float sB[3][kFHV];
+    __shared__ FieldLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
-    fieldBeforeAddTile<SOLVER>(a, En, blockIdx.x, sB);
+    fieldBeforeAddTile<SOLVER>(a, En, blockIdx.x, L);
 }
 
 template <int SOLVER>
 __global__ __launch_bounds__(256) void kFieldAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2) {
-    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
-    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ AfterLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
-    fieldAfterCell<SOLVER>(a, En, i);
+    fieldAfterTile<SOLVER>(a, En, blockIdx.x, L);
 }
 
-// batched forms for ensembles
+// batched forms for ensembles: items (member, tile)
 template <int SOLVER>
 __global__ __launch_bounds__(256) void kFieldBeforeAddBatch(const FieldBatchEntry* __restrict__ table,
                                                             const int2* __restrict__ tiles) {
-    __shared__ float sB[3][kFHV];
+    __shared__ FieldLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
-    fieldBeforeAddTile<SOLVER>(e.a, e.Enew, it.y, sB);
+    fieldBeforeAddTile<SOLVER>(e.a, e.Enew, it.y, L);
 }
 
 template <int SOLVER>
 __global__ __launch_bounds__(256) void kFieldAfterBatch(const FieldBatchEntry* __restrict__ table,
-                                                        const int2* __restrict__ blocks) {
-    int2 const it = blocks[blockIdx.x];
+                                                        const int2* __restrict__ tiles) {
+    __shared__ AfterLds L;
+    int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
-    size_t const n = (size_t)e.a.g.nx * e.a.g.ny * e.a.g.nz;
-    size_t const i = (size_t)it.y * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    fieldAfterCell<SOLVER>(e.a, e.Enew, i);
+    fieldAfterTile<SOLVER>(e.a, e.Enew, it.y, L);
 }
 
 __global__ __launch_bounds__(256) void kCurrentResetBatch(const FieldBatchEntry* __restrict__ table,
@@ -302,29 +391,28 @@ hipError_t launchEFull(const FieldArgs& a, hipStream_t s) {
 }
 
 hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], hipStream_t s) {
-    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     int const tiles = (a.g.nx / kFTX) * (a.g.ny / kFTY) * (a.g.nz / kFTZ);
     if (solver == PIC_SOLVER_LEHE) {
         hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
-        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_LEHE>, dim3(blocksFor(n)), dim3(256), 0, s, a, Enew[0], Enew[1],
-                           Enew[2]);
+        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
     } else {
         hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
-        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_YEE>, dim3(blocksFor(n)), dim3(256), 0, s, a, Enew[0], Enew[1],
-                           Enew[2]);
+        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
     }
     return hipGetLastError();
 }
 
 hipError_t launchFieldStepBatch(int solver, const FieldBatchEntry* table, const int2* tiles, int ntiles,
                                 const int2* blocks, int nblocks, hipStream_t s) {
-    if (ntiles <= 0 || nblocks <= 0) return hipSuccess;
+    (void)blocks;
+    (void)nblocks;
+    if (ntiles <= 0) return hipSuccess;
     if (solver == PIC_SOLVER_LEHE) {
         hipLaunchKernelGGL(kFieldBeforeAddBatch<PIC_SOLVER_LEHE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
-        hipLaunchKernelGGL(kFieldAfterBatch<PIC_SOLVER_LEHE>, dim3(nblocks), dim3(256), 0, s, table, blocks);
+        hipLaunchKernelGGL(kFieldAfterBatch<PIC_SOLVER_LEHE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
     } else {
         hipLaunchKernelGGL(kFieldBeforeAddBatch<PIC_SOLVER_YEE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
-        hipLaunchKernelGGL(kFieldAfterBatch<PIC_SOLVER_YEE>, dim3(nblocks), dim3(256), 0, s, table, blocks);
+        hipLaunchKernelGGL(kFieldAfterBatch<PIC_SOLVER_YEE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
     }
     return hipGetLastError();
 }

@@ -172,6 +172,8 @@ struct DiffState {
     float* mom = nullptr;     // 3 planes of n
     float* E = nullptr;       // 3 planes of ncells
     float* B = nullptr;       // 3 planes of ncells
+    float* w = nullptr;       // checkpoints: weightings in this state's particle order
+    int32_t* perm = nullptr;  // checkpoints: slot -> slot of the previous order (the re-tiling)
 };
 
 struct AdjointMem {
@@ -180,9 +182,21 @@ struct AdjointMem {
     float mass = 0.0f;    // frame mass
     float charge = 0.0f;  // frame charge
     int jexp = 40;
-    float* w = nullptr;
+    float* w = nullptr;      // weightings in the order of `work`
+    float* wInit = nullptr;  // weightings as loaded
+    int32_t* ids = nullptr;  // original index of each slot of `work`
+    int32_t* idsTmp = nullptr;
+    int32_t* iota = nullptr;
+    uint32_t* keys[2] = {nullptr, nullptr};  // tile sort
+    int32_t* vals[2] = {nullptr, nullptr};
+    void* sortTemp = nullptr;
+    size_t sortTempBytes = 0;
+    double* lXtmp = nullptr;  // adjoint state while it is permuted
+    double* lPtmp = nullptr;
     DiffState init;   // particle part: loaded initial particles
     DiffState work;   // particle part: particles of the running forward
+    DiffState scratch;              // particle part and w: target of a re-tiling gather
+    std::vector<const float*> tapeW;  // weightings of each tape state's particle order
     std::vector<DiffState> checkpoints;  // state at steps 0, K, 2K, ...
     std::vector<DiffState> tape;         // states of the segment being reversed
     unsigned long long* Jfix = nullptr;  // 3 planes of ncells
@@ -1164,12 +1178,12 @@ int diffStep(pic_ctx* c, const pic::DiffParticles& P, float* const E[3], float* 
 }
 
 // particles of `from` stepped into `to` (the same state for an in-place step)
-pic::DiffParticles particlesOf(const AdjointMem& m, const DiffState& from, const DiffState& to) {
+pic::DiffParticles particlesOf(const AdjointMem& m, const DiffState& from, const DiffState& to, const float* w) {
     pic::DiffParticles P{};
     P.cell = from.cell;
     P.pos = from.pos;
     P.mom = from.mom;
-    P.w = m.w;
+    P.w = w;
     P.cellOut = to.cell;
     P.posOut = to.pos;
     P.momOut = to.mom;
@@ -1179,13 +1193,39 @@ pic::DiffParticles particlesOf(const AdjointMem& m, const DiffState& from, const
 
 // state `to` = one step of state `from`: particles out of place, fields
 // copied and then updated in place
-int stepState(pic_ctx* c, const DiffState& from, const DiffState& to) {
+int stepState(pic_ctx* c, const DiffState& from, const DiffState& to, const float* w) {
     int r;
     if ((r = copyFields(c, to.E, from.E)) != PIC_OK) return r;
     if ((r = copyFields(c, to.B, from.B)) != PIC_OK) return r;
     float* const E[3] = {to.E, to.E + c->ncells, to.E + 2 * c->ncells};
     float* const B[3] = {to.B, to.B + c->ncells, to.B + 2 * c->ncells};
-    return diffStep(c, particlesOf(c->adj, from, to), E, B);
+    return diffStep(c, particlesOf(c->adj, from, to, w), E, B);
+}
+
+// re-tiling at a checkpoint: sort the running particles by tile into `cp`
+// (particles, weightings, the permutation), continue from the sorted order
+// Particles are sorted by tile again every kRetileEvery steps (and at every
+// checkpoint) so the particles of a thread block stay in a compact box of
+// cells while they drift.
+constexpr int kRetileEvery = 4;
+
+// sorts the particles of `st` (weightings w, identities ids if not null) by
+// tile, in place through the scratch state; the permutation (new slot -> old
+// slot) is copied to permOut if not null
+int retileInPlace(pic_ctx* c, DiffState& st, float* w, int32_t* ids, int32_t* permOut) {
+    AdjointMem& m = c->adj;
+    int32_t* perm = nullptr;
+    PIC_HIP(pic::launchTileSort(st.cell, m.n, c->g, m.keys, m.vals, m.sortTemp, m.sortTempBytes, &perm, c->stream));
+    pic::DiffParticles src = particlesOf(m, st, st, w);
+    pic::DiffParticles dst = particlesOf(m, m.scratch, m.scratch, m.scratch.w);
+    const int32_t* idsIn = ids ? ids : m.iota;
+    PIC_HIP(pic::launchGatherParticles(perm, src, dst, idsIn, m.idsTmp, m.scratch.w, c->stream));
+    int r;
+    if ((r = copyParticles(c, st, m.scratch, m.n)) != PIC_OK) return r;
+    PIC_HIP(hipMemcpyAsync(w, m.scratch.w, (size_t)m.n * 4, hipMemcpyDeviceToDevice, c->stream));
+    if (ids) PIC_HIP(hipMemcpyAsync(ids, m.idsTmp, (size_t)m.n * 4, hipMemcpyDeviceToDevice, c->stream));
+    if (permOut) PIC_HIP(hipMemcpyAsync(permOut, perm, (size_t)m.n * 4, hipMemcpyDeviceToDevice, c->stream));
+    return PIC_OK;
 }
 
 }  // namespace
@@ -1225,8 +1265,25 @@ int pic_adjoint_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* 
         if (devAlloc(c, &m.lE, 3 * c->ncells) != PIC_OK || devAlloc(c, &m.lB, 3 * c->ncells) != PIC_OK ||
             devAlloc(c, &m.lJ, 3 * c->ncells) != PIC_OK)
             return PIC_E_HIP;
-        if (devAlloc(c, &m.lX, 3 * (size_t)n) != PIC_OK || devAlloc(c, &m.lP, 3 * (size_t)n) != PIC_OK)
+        if (devAlloc(c, &m.lX, 3 * (size_t)n) != PIC_OK || devAlloc(c, &m.lP, 3 * (size_t)n) != PIC_OK ||
+            devAlloc(c, &m.lXtmp, 3 * (size_t)n) != PIC_OK || devAlloc(c, &m.lPtmp, 3 * (size_t)n) != PIC_OK)
             return PIC_E_HIP;
+        if (devAlloc(c, &m.wInit, (size_t)n) != PIC_OK || devAlloc(c, &m.ids, (size_t)n) != PIC_OK ||
+            devAlloc(c, &m.idsTmp, (size_t)n) != PIC_OK || devAlloc(c, &m.iota, (size_t)n) != PIC_OK)
+            return PIC_E_HIP;
+        for (int k = 0; k < 2; ++k)
+            if (devAlloc(c, &m.keys[k], (size_t)n) != PIC_OK || devAlloc(c, &m.vals[k], (size_t)n) != PIC_OK)
+                return PIC_E_HIP;
+        if (allocParticles(c, m.scratch, n) != PIC_OK || devAlloc(c, &m.scratch.w, (size_t)n) != PIC_OK)
+            return PIC_E_HIP;
+        PIC_HIP(pic::sortTempBytes(n, &m.sortTempBytes));
+        unsigned char* t = nullptr;
+        if (devAlloc(c, &t, m.sortTempBytes) != PIC_OK) return PIC_E_HIP;
+        m.sortTemp = t;
+        std::vector<int32_t> iota((size_t)n);
+        for (int64_t i = 0; i < n; ++i) iota[(size_t)i] = (int32_t)i;
+        PIC_HIP(hipMemcpyAsync(m.iota, iota.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(hipStreamSynchronize(c->stream));
     }
     m.species = s;
     m.mass = sm.mass;
@@ -1242,7 +1299,9 @@ int pic_adjoint_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* 
     PIC_HIP(hipMemcpyAsync(m.init.cell, cell, b, hipMemcpyHostToDevice, c->stream));
     PIC_HIP(hipMemcpyAsync(m.init.pos, pos, b, hipMemcpyHostToDevice, c->stream));
     PIC_HIP(hipMemcpyAsync(m.init.mom, mom, b, hipMemcpyHostToDevice, c->stream));
-    PIC_HIP(hipMemcpyAsync(m.w, w, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(m.wInit, w, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(m.w, m.wInit, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(m.ids, m.iota, (size_t)n * 4, hipMemcpyDeviceToDevice, c->stream));
     r = copyParticles(c, m.work, m.init, n);
     if (r != PIC_OK) return r;
     PIC_HIP(hipStreamSynchronize(c->stream));
@@ -1254,11 +1313,23 @@ int pic_adjoint_read_particles(pic_ctx* c, int32_t* cell, float* pos, float* mom
     if (!c) return fail(PIC_E_INVALID, "null context");
     AdjointMem& m = c->adj;
     if (m.n == 0) return fail(PIC_E_STATE, "no differentiable particles loaded");
-    size_t const b = 3 * (size_t)m.n * 4;
-    if (cell) PIC_HIP(hipMemcpyAsync(cell, m.work.cell, b, hipMemcpyDeviceToHost, c->stream));
-    if (pos) PIC_HIP(hipMemcpyAsync(pos, m.work.pos, b, hipMemcpyDeviceToHost, c->stream));
-    if (mom) PIC_HIP(hipMemcpyAsync(mom, m.work.mom, b, hipMemcpyDeviceToHost, c->stream));
+    // the running order is tiled; hand the particles back in the loaded order
+    size_t const n = (size_t)m.n;
+    std::vector<int32_t> ids(n), hc(3 * n);
+    std::vector<float> hp(3 * n), hm(3 * n);
+    PIC_HIP(hipMemcpyAsync(ids.data(), m.ids, n * 4, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemcpyAsync(hc.data(), m.work.cell, 3 * n * 4, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemcpyAsync(hp.data(), m.work.pos, 3 * n * 4, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipMemcpyAsync(hm.data(), m.work.mom, 3 * n * 4, hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
+    for (size_t j = 0; j < n; ++j) {
+        size_t const i = (size_t)ids[j];
+        for (int d = 0; d < 3; ++d) {
+            if (cell) cell[d * n + i] = hc[d * n + j];
+            if (pos) pos[d * n + i] = hp[d * n + j];
+            if (mom) mom[d * n + i] = hm[d * n + j];
+        }
+    }
     return PIC_OK;
 }
 
@@ -1272,19 +1343,27 @@ int pic_adjoint_forward(pic_ctx* c, const float* E0, const float* B0, int32_t ns
     if (E0 && (r = scatterPlanes(c, c->E, E0)) != PIC_OK) return r;
     if (B0 && (r = scatterPlanes(c, c->B, B0)) != PIC_OK) return r;
     if ((r = copyParticles(c, m.work, m.init, m.n)) != PIC_OK) return r;
+    PIC_HIP(hipMemcpyAsync(m.w, m.wInit, (size_t)m.n * 4, hipMemcpyDeviceToDevice, c->stream));
+    PIC_HIP(hipMemcpyAsync(m.ids, m.iota, (size_t)m.n * 4, hipMemcpyDeviceToDevice, c->stream));
     size_t const ncp = (size_t)(nsteps + every - 1) / (size_t)every;
     while (m.checkpoints.size() < ncp) {
         DiffState st;
         if (allocState(c, st, m.n) != PIC_OK) return PIC_E_HIP;
+        if (devAlloc(c, &st.w, (size_t)m.n) != PIC_OK || devAlloc(c, &st.perm, (size_t)m.n) != PIC_OK)
+            return PIC_E_HIP;
         m.checkpoints.push_back(st);
     }
-    pic::DiffParticles const P = particlesOf(m, m.work, m.work);
+    pic::DiffParticles const P = particlesOf(m, m.work, m.work, m.w);
     for (int32_t i = 0; i < nsteps; ++i) {
         if (i % every == 0) {
             DiffState& cp = m.checkpoints[(size_t)(i / every)];
+            if ((r = retileInPlace(c, m.work, m.w, m.ids, cp.perm)) != PIC_OK) return r;
             if ((r = copyParticles(c, cp, m.work, m.n)) != PIC_OK) return r;
+            PIC_HIP(hipMemcpyAsync(cp.w, m.w, (size_t)m.n * 4, hipMemcpyDeviceToDevice, c->stream));
             if ((r = gatherPlanes(c, cp.E, c->E)) != PIC_OK) return r;
             if ((r = gatherPlanes(c, cp.B, c->B)) != PIC_OK) return r;
+        } else if (i % kRetileEvery == 0) {
+            if ((r = retileInPlace(c, m.work, m.w, m.ids, nullptr)) != PIC_OK) return r;
         }
         if ((r = diffStep(c, P, c->E, c->B)) != PIC_OK) return r;
     }
@@ -1313,8 +1392,11 @@ int pic_adjoint_backward(pic_ctx* c, const double* dE, const double* dB, double*
     while (m.tape.size() < (size_t)std::min(K, std::max(m.nsteps, 1))) {
         DiffState st;
         if (allocState(c, st, m.n) != PIC_OK) return PIC_E_HIP;
+        if (devAlloc(c, &st.w, (size_t)m.n) != PIC_OK || devAlloc(c, &st.perm, (size_t)m.n) != PIC_OK)
+            return PIC_E_HIP;
         m.tape.push_back(st);
     }
+    m.tapeW.assign(m.tape.size(), nullptr);
     pic::AdjFieldArgs fa{};
     fa.g = c->g;
     for (int d = 0; d < 3; ++d) {
@@ -1347,8 +1429,19 @@ int pic_adjoint_backward(pic_ctx* c, const double* dE, const double* dB, double*
         if ((r = copyParticles(c, m.tape[0], cp, m.n)) != PIC_OK) return r;
         if ((r = copyFields(c, m.tape[0].E, cp.E)) != PIC_OK) return r;
         if ((r = copyFields(c, m.tape[0].B, cp.B)) != PIC_OK) return r;
-        for (int t = 1; t < len; ++t)
-            if ((r = stepState(c, m.tape[(size_t)t - 1], m.tape[(size_t)t])) != PIC_OK) return r;
+        // replay the segment with the forward's re-tilings (deterministic), keeping their permutations
+        m.tapeW[0] = cp.w;
+        for (int t = 1; t < len; ++t) {
+            DiffState& st = m.tape[(size_t)t];
+            if ((r = stepState(c, m.tape[(size_t)t - 1], st, m.tapeW[(size_t)t - 1])) != PIC_OK) return r;
+            m.tapeW[(size_t)t] = m.tapeW[(size_t)t - 1];
+            if ((seg * K + t) % kRetileEvery == 0) {
+                PIC_HIP(hipMemcpyAsync(st.w, m.tapeW[(size_t)t - 1], (size_t)m.n * 4, hipMemcpyDeviceToDevice,
+                                       c->stream));
+                if ((r = retileInPlace(c, st, st.w, nullptr, st.perm)) != PIC_OK) return r;
+                m.tapeW[(size_t)t] = st.w;
+            }
+        }
         for (int t = len - 1; t >= 0; --t) {
             const DiffState& st = m.tape[(size_t)t];
             PIC_HIP(pic::launchAdjFields(fa, dt, c2, coeff, lJ, c->stream));
@@ -1360,17 +1453,35 @@ int pic_adjoint_backward(pic_ctx* c, const double* dE, const double* dB, double*
             as.cell = st.cell;
             as.pos = st.pos;
             as.mom = st.mom;
-            as.w = m.w;
+            as.w = m.tapeW[(size_t)t];
             as.lX = m.lX;
             as.lP = m.lP;
             as.n = m.n;
             PIC_HIP(pic::launchAdjParticles(pa, as, c->stream));
+            if (t > 0 && (seg * K + t) % kRetileEvery == 0) {  // undo the re-tiling of state t
+                PIC_HIP(pic::launchScatterAdjoint(st.perm, m.lX, m.lXtmp, m.n, c->stream));
+                PIC_HIP(pic::launchScatterAdjoint(st.perm, m.lP, m.lPtmp, m.n, c->stream));
+                std::swap(m.lX, m.lXtmp);
+                std::swap(m.lP, m.lPtmp);
+            }
+        }
+        if (seg > 0) {  // undo this checkpoint's re-tiling: the previous segment's order
+            PIC_HIP(pic::launchScatterAdjoint(cp.perm, m.lX, m.lXtmp, m.n, c->stream));
+            PIC_HIP(pic::launchScatterAdjoint(cp.perm, m.lP, m.lPtmp, m.n, c->stream));
+            std::swap(m.lX, m.lXtmp);
+            std::swap(m.lP, m.lPtmp);
         }
     }
     PIC_HIP(hipMemcpyAsync(dE0, m.lE, nf * 8, hipMemcpyDeviceToDevice, c->stream));
     PIC_HIP(hipMemcpyAsync(dB0, m.lB, nf * 8, hipMemcpyDeviceToDevice, c->stream));
-    if (dX0) PIC_HIP(hipMemcpyAsync(dX0, m.lX, np * 8, hipMemcpyDeviceToDevice, c->stream));
-    if (dP0) PIC_HIP(hipMemcpyAsync(dP0, m.lP, np * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (nseg == 0) {  // no step: the loaded particles are the final ones
+        if (dX0) PIC_HIP(hipMemcpyAsync(dX0, m.lX, np * 8, hipMemcpyDeviceToDevice, c->stream));
+        if (dP0) PIC_HIP(hipMemcpyAsync(dP0, m.lP, np * 8, hipMemcpyDeviceToDevice, c->stream));
+    } else {  // the first checkpoint's permutation maps to the loaded order
+        const int32_t* perm0 = m.checkpoints[0].perm;
+        if (dX0) PIC_HIP(pic::launchScatterAdjoint(perm0, m.lX, dX0, m.n, c->stream));
+        if (dP0) PIC_HIP(pic::launchScatterAdjoint(perm0, m.lP, dP0, m.n, c->stream));
+    }
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }

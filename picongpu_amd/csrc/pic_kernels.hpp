@@ -205,5 +205,15 @@ hipError_t launchFixToFloat(unsigned long long* fix, float* const J[3], size_t n
 hipError_t launchAdjFields(const AdjFieldArgs& a, double dt, double c2, double coeff, double* lJ[3], hipStream_t s);
 hipError_t launchAdjParticles(const AdjPartArgs& a, const AdjState& st, hipStream_t s);
 hipError_t launchFillD(double* p, size_t n, double v, hipStream_t s);
+// particle order of the differentiable form: tiles of 8x4x1 cells (grids are
+// multiples of the supercell, so of the tile)
+constexpr int kTileX = 8, kTileY = 4, kTileZ = 1;
+hipError_t sortTempBytes(int64_t n, size_t* bytes);
+// stable sort of the slots by tile; *perm receives vals[1] (new slot -> old slot)
+hipError_t launchTileSort(const int32_t* cell, int64_t n, const GridDev& g, uint32_t* keys[2], int32_t* vals[2],
+                          void* temp, size_t tempBytes, int32_t** perm, hipStream_t s);
+hipError_t launchGatherParticles(const int32_t* perm, const DiffParticles& src, const DiffParticles& dst,
+                                 const int32_t* idsIn, int32_t* idsOut, float* wOut, hipStream_t s);
+hipError_t launchScatterAdjoint(const int32_t* perm, const double* src, double* dst, int64_t n, hipStream_t s);
 
 }  // namespace pic

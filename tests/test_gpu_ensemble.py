@@ -11,9 +11,10 @@ pytestmark = pytest.mark.gpu
 
 
 def members():
-    return [setups.two_stream(grid=(16, 16, 8), ppc=2, beta=0.3, seed=11),
-            setups.two_stream(grid=(24, 16, 12), ppc=3, beta=0.2, seed=12),
-            setups.two_stream(grid=(16, 24, 8), ppc=4, beta=0.35, seed=13)]
+    # at least 2x2x3 supercells: the TSC fused checkerboard needs stride 2, 2, 3
+    return [setups.two_stream(grid=(16, 16, 12), ppc=2, beta=0.3, seed=11),
+            setups.two_stream(grid=(24, 16, 16), ppc=3, beta=0.2, seed=12),
+            setups.two_stream(grid=(16, 24, 12), ppc=4, beta=0.35, seed=13)]
 
 
 def init(sim, st):
