@@ -226,6 +226,10 @@ int pic_fdtd_before_current(pic_ctx* ctx);
 int pic_add_current(pic_ctx* ctx);
 /* fields::Solver::update_afterCurrent (FDTDBase.hpp:151-183): B half step */
 int pic_fdtd_after_current(pic_ctx* ctx);
+/* update_beforeCurrent, addCurrent and update_afterCurrent back to back, as
+ * Simulation::runOneStep calls them (Simulation.hpp:522-542), in two fused
+ * LDS-tiled kernels; E, B bit-identical to the three calls above */
+int pic_fdtd_step(pic_ctx* ctx);
 /* nsteps full PIC steps in the reference stage order, push and deposit fused */
 int pic_step(pic_ctx* ctx, int32_t nsteps);
 /* the same steps replayed from one captured hipGraph of a step (captured on

@@ -137,6 +137,7 @@ SIGNATURES = {
     "pic_fdtd_before_current": (C.c_int, [_P]),
     "pic_add_current": (C.c_int, [_P]),
     "pic_fdtd_after_current": (C.c_int, [_P]),
+    "pic_fdtd_step": (C.c_int, [_P]),
     "pic_step": (C.c_int, [_P, C.c_int32]),
     "pic_step_graph": (C.c_int, [_P, C.c_int32]),
     "pic_field_energy": (C.c_int, [_P, _D]),
@@ -389,6 +390,10 @@ class Simulation:
 
     def update_after_current(self):
         check(self.lib.pic_fdtd_after_current(self.h), "pic_fdtd_after_current")
+
+    def fdtd_step(self):
+        """update_beforeCurrent + addCurrent + update_afterCurrent in the fused kernels."""
+        check(self.lib.pic_fdtd_step(self.h), "pic_fdtd_step")
 
     def step(self, n=1):
         check(self.lib.pic_step(self.h, int(n)), "pic_step")

@@ -240,6 +240,205 @@ __device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* co
     Enew[2][i] = e2;
 }
 
+// ---------------------------------------------------------------------------
+// Yee: z-marching form. A workgroup owns an xy tile of kZX x kZY cells and a
+// chunk of zc planes; it walks up in z with two planes of E and of B' in
+// LDS (each with the one-cell xy halo the stencil needs), so E is read once
+// per plane plus the halo (two extra planes per chunk) and nothing is
+// recomputed along z. The chunk is kZC for ensembles (many members fill the
+// chip) and is halved for a single small grid until the launch has
+// kZFill workgroups, down to kZCMin planes.
+// ---------------------------------------------------------------------------
+constexpr int kZX = 32, kZY = 8, kZC = 16, kZCMin = 2, kZFill = 2048;
+constexpr int kZEX = kZX + 2, kZEY = kZY + 2;  // E plane: x, y in [-1, T+1)
+constexpr int kZBX = kZX + 1, kZBY = kZY + 1;  // B' plane: x, y in [-1, T)
+
+// periodic index for any v (a partial tile near the box edge may reach more than one box length away)
+__device__ __forceinline__ int wrap1(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+struct ZTile {
+    int x0, y0, z0, z1;
+};
+
+__device__ __forceinline__ ZTile zTile(const GridDev& g, int b, int zc) {
+    int const ntx = (g.nx + kZX - 1) / kZX, nty = (g.ny + kZY - 1) / kZY;
+    ZTile t;
+    t.x0 = (b % ntx) * kZX;
+    t.y0 = ((b / ntx) % nty) * kZY;
+    int const c = b / (ntx * nty);
+    t.z0 = c * zc;
+    t.z1 = min(t.z0 + zc, g.nz);
+    return t;
+}
+
+__host__ __device__ __forceinline__ int zTileCount(const GridDev& g, int zc) {
+    return ((g.nx + kZX - 1) / kZX) * ((g.ny + kZY - 1) / kZY) * ((g.nz + zc - 1) / zc);
+}
+
+struct YeeBeforeLds {
+    float E[2][3][kZEY][kZEX];
+    float B[2][3][kZBY][kZBX];
+};
+
+// update_beforeCurrent + addCurrent for one z-marching tile (Yee)
+__device__ __forceinline__ void yeeBeforeAddTile(const FieldArgs& a, float* const* Enew, int b, int zc,
+                                                 YeeBeforeLds& L) {
+    ZTile const T = zTile(a.g, b, zc);
+    int const nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    size_t const plane = (size_t)nx * ny;
+    // per thread, once per tile: the wrapped xy offsets of the plane elements it stages and updates
+    constexpr int NE = (3 * kZEX * kZEY + 255) / 256;
+    constexpr int NB = (kZBX * kZBY + 255) / 256;
+    int eoff[NE];
+    for (int j = 0; j < NE; ++j) {
+        int const t = threadIdx.x + 256 * j;
+        int const r = t % (kZEX * kZEY);
+        eoff[j] = wrap1(T.x0 + r % kZEX - 1, nx) + nx * wrap1(T.y0 + r / kZEX - 1, ny);
+    }
+    int boff[NB];
+    for (int j = 0; j < NB; ++j) {
+        int const t = threadIdx.x + 256 * j;
+        boff[j] = wrap1(T.x0 + t % kZBX - 1, nx) + nx * wrap1(T.y0 + t / kZBX - 1, ny);
+    }
+    float* const Ebase = &L.E[0][0][0][0];
+    auto loadE = [&](int k) {  // E plane k (wrapped) into slot k & 1
+        size_t const zoff = plane * (size_t)wrap1(k, nz);
+        float* const dst = Ebase + (k & 1) * (3 * kZEX * kZEY);
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            int const t = threadIdx.x + 256 * j;
+            if (t < 3 * kZEX * kZEY) dst[t] = a.E[t / (kZEX * kZEY)][zoff + (size_t)eoff[j]];
+        }
+    };
+    // B' = B - curlE(E)*0.5*dt at plane k on x, y in [-1, T): forward differences
+    auto bPrime = [&](int k) {
+        size_t const zoff = plane * (size_t)wrap1(k, nz);
+        int const s = k & 1, s1 = (k + 1) & 1;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            int const t = threadIdx.x + 256 * j;
+            if (t >= kZBX * kZBY) continue;
+            int const bx = t % kZBX, by = t / kZBX;  // cell (bx-1, by-1) of the tile
+            int const ex = bx, ey = by;              // its E-plane index (E plane starts at -1)
+            float const e0 = L.E[s][0][ey][ex], e1 = L.E[s][1][ey][ex], e2 = L.E[s][2][ey][ex];
+            float const dEzdy = (L.E[s][2][ey + 1][ex] - e2) / a.cell[1];
+            float const dEydz = (L.E[s1][1][ey][ex] - e1) / a.cell[2];
+            float const dExdz = (L.E[s1][0][ey][ex] - e0) / a.cell[2];
+            float const dEzdx = (L.E[s][2][ey][ex + 1] - e2) / a.cell[0];
+            float const dEydx = (L.E[s][1][ey][ex + 1] - e1) / a.cell[0];
+            float const dExdy = (L.E[s][0][ey + 1][ex] - e0) / a.cell[1];
+            float const cx = dEzdy - dEydz;
+            float const cy = dExdz - dEzdx;
+            float const cz = dEydx - dExdy;
+            size_t const gi = zoff + (size_t)boff[j];
+            L.B[s][0][by][bx] = a.B[0][gi] - cx * 0.5f * a.dt;
+            L.B[s][1][by][bx] = a.B[1][gi] - cy * 0.5f * a.dt;
+            L.B[s][2][by][bx] = a.B[2][gi] - cz * 0.5f * a.dt;
+        }
+    };
+    int const lx = threadIdx.x % kZX, ly = threadIdx.x / kZX;
+    bool const inside = T.x0 + lx < nx && T.y0 + ly < ny;
+    size_t const ixy = (size_t)(T.x0 + lx) + (size_t)nx * (T.y0 + ly);
+    loadE(T.z0 - 1);
+    for (int k = T.z0 - 1; k < T.z1; ++k) {
+        __syncthreads();  // slot (k+1)&1 is no longer read
+        loadE(k + 1);
+        __syncthreads();
+        bPrime(k);
+        __syncthreads();
+        if (k >= T.z0 && inside) {
+            int const s = k & 1, sm = (k - 1) & 1;
+            int const bx = lx + 1, by = ly + 1;
+            // backward differences of B' (BackwardDerivative.hpp:44-65), curl as kEFull
+            float const cx = (L.B[s][2][by][bx] - L.B[s][2][by - 1][bx]) / a.cell[1] -
+                             (L.B[s][1][by][bx] - L.B[sm][1][by][bx]) / a.cell[2];
+            float const cy = (L.B[s][0][by][bx] - L.B[sm][0][by][bx]) / a.cell[2] -
+                             (L.B[s][2][by][bx] - L.B[s][2][by][bx - 1]) / a.cell[0];
+            float const cz = (L.B[s][1][by][bx] - L.B[s][1][by][bx - 1]) / a.cell[0] -
+                             (L.B[s][0][by][bx] - L.B[s][0][by - 1][bx]) / a.cell[1];
+            size_t const i = ixy + plane * k;
+            float e0 = L.E[s][0][ly + 1][lx + 1] + cx * a.c2 * a.dt;
+            float e1 = L.E[s][1][ly + 1][lx + 1] + cy * a.c2 * a.dt;
+            float e2 = L.E[s][2][ly + 1][lx + 1] + cz * a.c2 * a.dt;
+            e0 = e0 + a.coeff * a.J[0][i];
+            e1 = e1 + a.coeff * a.J[1][i];
+            e2 = e2 + a.coeff * a.J[2][i];
+            Enew[0][i] = e0;
+            Enew[1][i] = e1;
+            Enew[2][i] = e2;
+        }
+    }
+}
+
+struct YeeAfterLds {
+    float E[2][3][kZY + 1][kZX + 1];
+    float En[2][3][kZY + 1][kZX + 1];
+};
+
+// update_afterCurrent for one z-marching tile (Yee): B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt
+__device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* Enew, int b, int zc, YeeAfterLds& L) {
+    ZTile const T = zTile(a.g, b, zc);
+    int const nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    size_t const plane = (size_t)nx * ny;
+    constexpr int PX = kZX + 1, PY = kZY + 1;
+    constexpr int NE = (3 * PX * PY + 255) / 256;
+    int eoff[NE];
+    for (int j = 0; j < NE; ++j) {
+        int const t = threadIdx.x + 256 * j;
+        int const r = t % (PX * PY);
+        eoff[j] = wrap1(T.x0 + r % PX, nx) + nx * wrap1(T.y0 + r / PX, ny);
+    }
+    float* const Eb = &L.E[0][0][0][0];
+    float* const Nb = &L.En[0][0][0][0];
+    auto load = [&](int k) {  // planes k of E and Enew on x, y in [0, T+1)
+        size_t const zoff = plane * (size_t)wrap1(k, nz);
+        int const so = (k & 1) * (3 * PX * PY);
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            int const t = threadIdx.x + 256 * j;
+            if (t < 3 * PX * PY) {
+                int const c = t / (PX * PY);
+                size_t const gi = zoff + (size_t)eoff[j];
+                Eb[so + t] = a.E[c][gi];
+                Nb[so + t] = Enew[c][gi];
+            }
+        }
+    };
+    int const lx = threadIdx.x % kZX, ly = threadIdx.x / kZX;
+    bool const inside = T.x0 + lx < nx && T.y0 + ly < ny;
+    size_t const ixy = (size_t)(T.x0 + lx) + (size_t)nx * (T.y0 + ly);
+    load(T.z0);
+    for (int k = T.z0; k < T.z1; ++k) {
+        __syncthreads();
+        load(k + 1);
+        __syncthreads();
+        if (inside) {
+            int const s = k & 1, s1 = (k + 1) & 1;
+            auto curl = [&](float (*F)[kZY + 1][kZX + 1], float (*G)[kZY + 1][kZX + 1], float& cx, float& cy,
+                            float& cz) {
+                // F: plane k, G: plane k+1 (forward differences, curl as kBHalf)
+                float const f0 = F[0][ly][lx], f1 = F[1][ly][lx], f2 = F[2][ly][lx];
+                cx = (F[2][ly + 1][lx] - f2) / a.cell[1] - (G[1][ly][lx] - f1) / a.cell[2];
+                cy = (G[0][ly][lx] - f0) / a.cell[2] - (F[2][ly][lx + 1] - f2) / a.cell[0];
+                cz = (F[1][ly][lx + 1] - f1) / a.cell[0] - (F[0][ly + 1][lx] - f0) / a.cell[1];
+            };
+            size_t const i = ixy + plane * k;
+            float cx, cy, cz;
+            curl(L.E[s], L.E[s1], cx, cy, cz);
+            float const b0 = a.B[0][i] - cx * 0.5f * a.dt;
+            float const b1 = a.B[1][i] - cy * 0.5f * a.dt;
+            float const b2 = a.B[2][i] - cz * 0.5f * a.dt;
+            curl(L.En[s], L.En[s1], cx, cy, cz);
+            a.B[0][i] = b0 - cx * 0.5f * a.dt;
+            a.B[1][i] = b1 - cy * 0.5f * a.dt;
+            a.B[2][i] = b2 - cz * 0.5f * a.dt;
+        }
+    }
+}
+
 struct AfterLds {
     float E[3][kAV];
     float En[3][kAV];
@@ -278,6 +477,34 @@ __global__ __launch_bounds__(256) void kFieldBeforeAdd(FieldArgs a, float* Enew0
     __shared__ FieldLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
     fieldBeforeAddTile<SOLVER>(a, En, blockIdx.x, L);
+}
+
+__global__ __launch_bounds__(256) void kYeeBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
+    __shared__ YeeBeforeLds L;
+    float* const En[3] = {Enew0, Enew1, Enew2};
+    yeeBeforeAddTile(a, En, blockIdx.x, zc, L);
+}
+
+__global__ __launch_bounds__(256) void kYeeAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
+    __shared__ YeeAfterLds L;
+    float* const En[3] = {Enew0, Enew1, Enew2};
+    yeeAfterTile(a, En, blockIdx.x, zc, L);
+}
+
+__global__ __launch_bounds__(256) void kYeeBeforeAddBatch(const FieldBatchEntry* __restrict__ table,
+                                                          const int2* __restrict__ tiles) {
+    __shared__ YeeBeforeLds L;
+    int2 const it = tiles[blockIdx.x];
+    const FieldBatchEntry& e = table[it.x];
+    yeeBeforeAddTile(e.a, e.Enew, it.y, kZC, L);
+}
+
+__global__ __launch_bounds__(256) void kYeeAfterBatch(const FieldBatchEntry* __restrict__ table,
+                                                      const int2* __restrict__ tiles) {
+    __shared__ YeeAfterLds L;
+    int2 const it = tiles[blockIdx.x];
+    const FieldBatchEntry& e = table[it.x];
+    yeeAfterTile(e.a, e.Enew, it.y, kZC, L);
 }
 
 template <int SOLVER>
@@ -390,16 +617,33 @@ hipError_t launchEFull(const FieldArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+int yeeChunk(const GridDev& g) {
+    int zc = kZC;
+    while (zc > kZCMin && zTileCount(g, zc) < kZFill) zc /= 2;
+    return zTileCount(g, zc) >= kZFill / 2 ? zc : 0;
+}
+
 hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], hipStream_t s) {
     int const tiles = (a.g.nx / kFTX) * (a.g.ny / kFTY) * (a.g.nz / kFTZ);
+    int const zc = solver == PIC_SOLVER_YEE ? yeeChunk(a.g) : 0;
     if (solver == PIC_SOLVER_LEHE) {
         hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
         hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
-    } else {
+    } else if (zc == 0) {
+        // grids too small for the z-marching form to fill the chip: 3D tiles of 8x8x4
         hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
         hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+    } else {
+        int const zt = zTileCount(a.g, zc);
+        hipLaunchKernelGGL(kYeeBeforeAdd, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
+        hipLaunchKernelGGL(kYeeAfter, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
     }
     return hipGetLastError();
+}
+
+int fieldStepTiles(int solver, const GridDev& g) {
+    if (solver == PIC_SOLVER_LEHE) return (g.nx / kFTX) * (g.ny / kFTY) * (g.nz / kFTZ);
+    return zTileCount(g, kZC);
 }
 
 hipError_t launchFieldStepBatch(int solver, const FieldBatchEntry* table, const int2* tiles, int ntiles,
@@ -411,8 +655,8 @@ hipError_t launchFieldStepBatch(int solver, const FieldBatchEntry* table, const 
         hipLaunchKernelGGL(kFieldBeforeAddBatch<PIC_SOLVER_LEHE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
         hipLaunchKernelGGL(kFieldAfterBatch<PIC_SOLVER_LEHE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
     } else {
-        hipLaunchKernelGGL(kFieldBeforeAddBatch<PIC_SOLVER_YEE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
-        hipLaunchKernelGGL(kFieldAfterBatch<PIC_SOLVER_YEE>, dim3(ntiles), dim3(256), 0, s, table, tiles);
+        hipLaunchKernelGGL(kYeeBeforeAddBatch, dim3(ntiles), dim3(256), 0, s, table, tiles);
+        hipLaunchKernelGGL(kYeeAfterBatch, dim3(ntiles), dim3(256), 0, s, table, tiles);
     }
     return hipGetLastError();
 }

@@ -974,6 +974,11 @@ int oneStep(pic_ctx* c) {
 
 }  // namespace
 
+extern "C" int pic_fdtd_step(pic_ctx* c) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    return fieldStep(c);
+}
+
 extern "C" {
 
 int pic_step(pic_ctx* c, int32_t nsteps) {
@@ -1747,7 +1752,7 @@ int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out) {
             for (int sc = 0; sc < c->nsc; ++sc) shiftWork.push_back(int2{entry, sc});
             ++entry;
         }
-        int const nt = (c->g.nx / 8) * (c->g.ny / 8) * (c->g.nz / 4);
+        int const nt = pic::fieldStepTiles(c->cfg.solver, c->g);
         for (int t = 0; t < nt; ++t) tiles.push_back(int2{i, t});
         int const nb = static_cast<int>((c->ncells + 255) / 256);
         for (int b = 0; b < nb; ++b) blocks.push_back(int2{i, b});

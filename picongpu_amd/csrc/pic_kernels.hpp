@@ -80,6 +80,8 @@ hipError_t launchAddCurrent(const FieldArgs& a, hipStream_t s);
 // written to Enew (the caller swaps E and Enew afterwards); the grid must be a
 // multiple of 8x8x4 cells
 hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], hipStream_t s);
+// tiles of the fused field step of one grid (work items of the batched form)
+int fieldStepTiles(int solver, const GridDev& g);
 hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, hipStream_t s);
 hipError_t launchCountSum(const uint32_t* counts, int n, unsigned long long* out, hipStream_t s);
 hipError_t launchGaussResidual(const FieldArgs& f, const SpeciesDev* sp, const int* supp, int nsp, double qScale,

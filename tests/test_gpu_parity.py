@@ -100,6 +100,48 @@ def test_fields_bit_exact(solver):
     sim.close()
 
 
+# launch forms of the fused field step: 64^3 runs 8x8x4 tiles; the others
+# z-march 32x8 xy tiles over chunks of 4 planes with partial xy tiles
+# (200x104x96), 2 planes (128x128x68), 8 planes with a partial last chunk
+# (256x256x68) and 16 planes (256x256x128)
+FUSED_FIELD_CASES = [((64, 64, 64), abi.SOLVER_YEE), ((200, 104, 96), abi.SOLVER_YEE),
+                     ((128, 128, 68), abi.SOLVER_YEE), ((256, 256, 68), abi.SOLVER_YEE),
+                     ((256, 256, 128), abi.SOLVER_YEE), ((64, 64, 64), abi.SOLVER_LEHE),
+                     ((200, 104, 96), abi.SOLVER_LEHE)]
+
+
+@pytest.mark.parametrize("grid,solver", FUSED_FIELD_CASES,
+                         ids=[f"{'yee' if s == abi.SOLVER_YEE else 'lehe'}-{'x'.join(map(str, g))}"
+                              for g, s in FUSED_FIELD_CASES])
+def test_fused_field_step_equals_stage_calls(grid, solver):
+    """pic_fdtd_step (two LDS-tiled kernels) against update_beforeCurrent,
+    addCurrent and update_afterCurrent called one by one, which
+    test_fields_bit_exact pins to the oracle; twice, so the second call runs
+    on the swapped E buffers."""
+    st = setups.thermal(grid=grid, ppc=1)
+    st.solver = solver
+    st.lehe_dir = 1
+    if solver == abi.SOLVER_LEHE:
+        st.dt_si = 0.99 * st.cell_si[1] / setups.SPEED_OF_LIGHT_SI / 1.8
+    print(f"case fused field step grid={grid} solver={solver}", flush=True)
+    E, B = H.random_fields(st, 3, 1.0, 1.0)
+    J = H.random_fields(st, 4, 0.1, 0.1)[0]
+    fused, staged = abi.Simulation(st.config()), abi.Simulation(st.config())
+    for s in (fused, staged):
+        s.load_field(abi.FIELD_E, E)
+        s.load_field(abi.FIELD_B, B)
+        s.load_field(abi.FIELD_J, J)
+    for it in range(2):
+        fused.fdtd_step()
+        staged.update_before_current()
+        staged.add_current()
+        staged.update_after_current()
+        for f in (abi.FIELD_E, abi.FIELD_B):
+            H.assert_field_equal(fused.read_field(f), staged.read_field(f), f"call {it} field {f}")
+    fused.close()
+    staged.close()
+
+
 def small_lehe():
     st = small_thermal(abi.SHAPE_TSC, abi.PUSHER_VAY)
     st.solver = abi.SOLVER_LEHE
