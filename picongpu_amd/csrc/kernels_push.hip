@@ -25,6 +25,63 @@ constexpr int kPushWaves = kPushThreads / 64;
 
 // LDS bytes of one supercell's work: the E/B tile (or the epilogue's
 // scratch), then the int64 J tile
+// A particle whose move changes its shifted cell on some axis needs the
+// general Esirkepov form (up to (SUPP+1)^2*SUPP cells per component); the rest
+// need SUPP^2*(SUPP-1). The push deposits the latter at once and queues the
+// former per wave: lane l holds queued item l in A (items 64..127 in B), and
+// a full A is deposited by all 64 lanes together. An item carries what the
+// deposit reads: the tile cell, the position after the move, the velocity
+// and the charge, so the contribution is the one the particle would have
+// made in place.
+struct SlowItem {
+    int cell;  // tile cell x | y << 8 | z << 16
+    float x, y, z, vx, vy, vz, q;
+};
+
+__device__ __forceinline__ int permuteLane(int dstLane, int v) { return __builtin_amdgcn_ds_permute(dstLane << 2, v); }
+__device__ __forceinline__ float permuteLane(int dstLane, float v) {
+    return __int_as_float(__builtin_amdgcn_ds_permute(dstLane << 2, __float_as_int(v)));
+}
+
+// Append the items of the lanes with `slow` set after the P queued ones
+// (wave-uniform P). Every lane of the wave calls it: the ds_permute moves
+// slow lanes to queue positions P.. and the other lanes to the free ones, a
+// permutation of the 64 lanes.
+__device__ __forceinline__ void queueSlow(bool slow, const SlowItem& mine, SlowItem& A, SlowItem& B, int& P) {
+    unsigned long long const m = __ballot(slow);
+    int const k = __popcll(m);
+    if (k == 0) return;
+    int const lane = threadIdx.x & 63;
+    int const below = __popcll(m & laneMaskLt());
+    int const dst = (P + (slow ? below : k + (lane - below))) & 63;
+    int const off = (lane - P) & 63;
+    bool const toA = off < k && lane >= P;
+    bool const toB = off < k && lane < P;
+#define PIC_QUEUE_MOVE(f)                          \
+    {                                              \
+        auto const r = permuteLane(dst, mine.f);   \
+        if (toA) A.f = r;                          \
+        if (toB) B.f = r;                          \
+    }
+    PIC_QUEUE_MOVE(cell)
+    PIC_QUEUE_MOVE(x)
+    PIC_QUEUE_MOVE(y)
+    PIC_QUEUE_MOVE(z)
+    PIC_QUEUE_MOVE(vx)
+    PIC_QUEUE_MOVE(vy)
+    PIC_QUEUE_MOVE(vz)
+    PIC_QUEUE_MOVE(q)
+#undef PIC_QUEUE_MOVE
+    P += k;
+}
+
+template <int SUPP, int TX, int TY, int TV>
+__device__ __forceinline__ void depositQueued(unsigned long long* __restrict__ sJ, const SlowItem& t,
+                                              const DepositConsts& dk) {
+    Esirkepov<SUPP>::template deposit<TX, TY, TV>(sJ, t.cell & 255, (t.cell >> 8) & 255, t.cell >> 16,
+                                                  Vec3{t.x, t.y, t.z}, Vec3{t.vx, t.vy, t.vz}, t.q, dk);
+}
+
 template <int SUPP, int MODE>
 struct PushLds {
     static constexpr int GLO = SUPP / 2;
@@ -123,51 +180,75 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     dk.inv_vdt = a.inv_vdt;
     dk.jscale = a.jscale;
 
+    SlowItem qA{}, qB{};
+    int queued = 0;  // wave-uniform
     for (int s = 0; s < a.nsp; ++s) {
         const SpeciesDev& S = a.sp[s];
         int const n = (int)S.count[sc];
         size_t const base = (size_t)sc * (size_t)S.cap;
         uint32_t leaveBits = 0u;  // bit `it`: the particle of iteration `it` left the supercell
-        for (int i = tid; i < n; i += blockDim.x) {
-            size_t const gidx = base + i;
-            Vec3 const pos{S.pos[0][gidx], S.pos[1][gidx], S.pos[2][gidx]};
-            Vec3 mom{S.mom[0][gidx], S.mom[1][gidx], S.mom[2][gidx]};
-            float const w = S.w[gidx];
-            int const lidx = S.cell[gidx];
-            int const lx = lidx % SCX, ly = (lidx / SCX) % SCY, lz = lidx / (SCX * SCY);
-            float const mass = S.mass * w;     // attribute::getMass (GetMass.hpp:40-44)
-            float const charge = S.charge * w; // attribute::getCharge (GetCharge.hpp:45-66)
-            if (MODE != 2) {
-                Vec3 const e = gatherE<SUPP, GX, GY, GV>(sEB, lx + GLO, ly + GLO, lz + GLO, pos);
-                Vec3 const b = gatherB<SUPP, GX, GY, GV>(sEB + 3 * GV, lx + GLO, ly + GLO, lz + GLO, pos);
-                Vec3 vel;
-                if (PUSHER == PIC_PUSHER_VAY)
-                    pushVay(e, b, mom, vel, mass, charge, pk);
-                else
-                    pushBoris(e, b, mom, vel, mass, charge, pk);
-                float p[3] = {pos.x, pos.y, pos.z};
-                p[0] += (vel.x * pk.dt) / pk.cell[0];
-                p[1] += (vel.y * pk.dt) / pk.cell[1];
-                p[2] += (vel.z * pk.dt) / pk.cell[2];
-                int dir[3], newIdx, newMask;
-                moveParticle(p, lidx, dir, newIdx, newMask);
-                S.pos[0][gidx] = p[0];
-                S.pos[1][gidx] = p[1];
-                S.pos[2][gidx] = p[2];
-                S.mom[0][gidx] = mom.x;
-                S.mom[1][gidx] = mom.y;
-                S.mom[2][gidx] = mom.z;
-                if ((dir[0] | dir[1] | dir[2]) != 0) S.cell[gidx] = (uint16_t)newIdx;
-                if (newMask >= 2) S.mask[gidx] = (uint8_t)newMask;
-                leaveBits |= (newMask >= 2 ? 1u : 0u) << (i / kPushThreads);
-                if (MODE == 1) {
-                    Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + dir[0] + DLO, ly + dir[1] + DLO,
-                                                                  lz + dir[2] + DLO, Vec3{p[0], p[1], p[2]}, vel,
-                                                                  charge, dk);
+        int const iters = (n + (int)blockDim.x - 1) / (int)blockDim.x;  // uniform across the workgroup
+        // the last species runs one iteration more, without particles, that drains the queue
+        bool const lastSpecies = s == a.nsp - 1;
+        for (int it = 0; it < iters + (MODE == 1 && lastSpecies ? 1 : 0); ++it) {
+            int const i = it * (int)blockDim.x + tid;
+            bool slow = false;
+            SlowItem mine{};
+            if (i < n) {
+                size_t const gidx = base + i;
+                Vec3 const pos{S.pos[0][gidx], S.pos[1][gidx], S.pos[2][gidx]};
+                Vec3 mom{S.mom[0][gidx], S.mom[1][gidx], S.mom[2][gidx]};
+                float const w = S.w[gidx];
+                int const lidx = S.cell[gidx];
+                int const lx = lidx % SCX, ly = (lidx / SCX) % SCY, lz = lidx / (SCX * SCY);
+                float const mass = S.mass * w;     // attribute::getMass (GetMass.hpp:40-44)
+                float const charge = S.charge * w; // attribute::getCharge (GetCharge.hpp:45-66)
+                if (MODE != 2) {
+                    Vec3 const e = gatherE<SUPP, GX, GY, GV>(sEB, lx + GLO, ly + GLO, lz + GLO, pos);
+                    Vec3 const b = gatherB<SUPP, GX, GY, GV>(sEB + 3 * GV, lx + GLO, ly + GLO, lz + GLO, pos);
+                    Vec3 vel;
+                    if (PUSHER == PIC_PUSHER_VAY)
+                        pushVay(e, b, mom, vel, mass, charge, pk);
+                    else
+                        pushBoris(e, b, mom, vel, mass, charge, pk);
+                    float p[3] = {pos.x, pos.y, pos.z};
+                    p[0] += (vel.x * pk.dt) / pk.cell[0];
+                    p[1] += (vel.y * pk.dt) / pk.cell[1];
+                    p[2] += (vel.z * pk.dt) / pk.cell[2];
+                    int dir[3], newIdx, newMask;
+                    moveParticle(p, lidx, dir, newIdx, newMask);
+                    S.pos[0][gidx] = p[0];
+                    S.pos[1][gidx] = p[1];
+                    S.pos[2][gidx] = p[2];
+                    S.mom[0][gidx] = mom.x;
+                    S.mom[1][gidx] = mom.y;
+                    S.mom[2][gidx] = mom.z;
+                    if ((dir[0] | dir[1] | dir[2]) != 0) S.cell[gidx] = (uint16_t)newIdx;
+                    if (newMask >= 2) S.mask[gidx] = (uint8_t)newMask;
+                    leaveBits |= (newMask >= 2 ? 1u : 0u) << (i / kPushThreads);
+                    if (MODE == 1) {
+                        int const cx = lx + dir[0] + DLO, cy = ly + dir[1] + DLO, cz = lz + dir[2] + DLO;
+                        Vec3 const np{p[0], p[1], p[2]};
+                        auto const G = Esirkepov<SUPP>::geometry(np, vel, dk);
+                        if (Esirkepov<SUPP>::changesCell(G)) {
+                            slow = true;
+                            mine = SlowItem{cx | (cy << 8) | (cz << 16), np.x, np.y, np.z, vel.x, vel.y, vel.z, charge};
+                        } else {
+                            Esirkepov<SUPP>::template depositGeometry<JX, JY, JV, false>(sJ, cx, cy, cz, G, charge, dk);
+                        }
+                    }
+                } else {
+                    Vec3 const vel = velocityOf(mom, mass, a.rc2);
+                    Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge, dk);
                 }
-            } else {
-                Vec3 const vel = velocityOf(mom, mass, a.rc2);
-                Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge, dk);
+            }
+            if (MODE == 1) {
+                queueSlow(slow, mine, qA, qB, queued);
+                if (queued >= 64 || (it == iters && queued > 0)) {
+                    if ((int)(tid & 63) < queued) depositQueued<SUPP, JX, JY, JV>(sJ, qA, dk);
+                    qA = qB;
+                    queued = queued > 64 ? queued - 64 : 0;
+                }
             }
         }
         if (MODE != 2 && S.lists) S.leaveBits[(size_t)sc * kPushThreads + tid] = (uint16_t)leaveBits;

@@ -375,18 +375,22 @@ struct Esirkepov {
     // positions, st the rotated status bits. `jt` points at the tile element
     // of rotated offset (begin,begin,begin); SI/SJ/SK are the tile strides of
     // the rotated i/j/k axes, so every address is jt + a compile-time offset.
-    template <int SI, int SJ, int SK>
+    // GENERAL = false is the form for a particle that stays in its (shifted)
+    // cell on every axis: then st = 6 on every axis, the leaveCell rows are
+    // absent and the cached shapes need no shift, so the loops visit exactly
+    // the cells the general form gives a contribution, with the same values.
+    template <int SI, int SJ, int SK, bool GENERAL>
     __device__ __forceinline__ static void cpt1D(unsigned long long* __restrict__ jt, const float* p0, const float* p1,
                                                  const int* st, float csd, double jscale) {
         if (p0[2] == p1[2]) return;
         float s0j[SUPP + 1], s1j[SUPP + 1], s0k[SUPP + 1], s1k[SUPP + 1];
-        cachedShape<SUPP>(p0[1], !(st[1] & 2), s0j);
-        cachedShape<SUPP>(p1[1], !(st[1] & 4), s1j);
-        cachedShape<SUPP>(p0[2], !(st[2] & 2), s0k);
-        cachedShape<SUPP>(p1[2], !(st[2] & 4), s1k);
-        int const leaveCellI = st[0] & 1;
-        int const leaveCellJ = st[1] & 1;
-        int const leaveCellK = st[2] & 1;
+        cachedShape<SUPP>(p0[1], GENERAL && !(st[1] & 2), s0j);
+        cachedShape<SUPP>(p1[1], GENERAL && !(st[1] & 4), s1j);
+        cachedShape<SUPP>(p0[2], GENERAL && !(st[2] & 2), s0k);
+        cachedShape<SUPP>(p1[2], GENERAL && !(st[2] & 4), s1k);
+        int const leaveCellI = GENERAL ? st[0] & 1 : 0;
+        int const leaveCellJ = GENERAL ? st[1] & 1 : 0;
+        int const leaveCellK = GENERAL ? st[2] & 1 : 0;
 #pragma unroll
         for (int i = begin; i < end + 1; ++i) {
             if (i < end + leaveCellI) {
@@ -416,60 +420,84 @@ struct Esirkepov {
         }
     }
 
-    // Deposit one particle. (cx,cy,cz) is the particle's cell in J-tile
-    // coordinates, pos its in-cell position after the move, vel its velocity.
-    // Tile layout: component-major planes of TV elements, [z][y][x] with pitches TX, TX*TY.
-    template <int TX, int TY, int TV>
-    __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
-                                                   Vec3 vel, float charge, const DepositConsts& k) {
-        constexpr bool isEven = (SUPP % 2) == 0;
-        constexpr int PX = 1, PY = TX, PZ = TX * TY;
-        float const np[3] = {pos.x, pos.y, pos.z};
-        float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
+    // Start/end points of the move relative to the grid shift, and the status
+    // bits per axis (bit 0: the shifted cell changes, bits 1/2: start/end lie
+    // in the shifted cell gs).
+    struct Geometry {
         float p0[3], p1[3];
         int gs[3], st[3];
+    };
+
+    __device__ __forceinline__ static Geometry geometry(Vec3 pos, Vec3 vel, const DepositConsts& k) {
+        constexpr bool isEven = (SUPP % 2) == 0;
+        float const np[3] = {pos.x, pos.y, pos.z};
+        float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
+        Geometry G;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            p0[d] = np[d] - dp[d];
-            p1[d] = np[d];
+            G.p0[d] = np[d] - dp[d];
+            G.p1[d] = np[d];
             int iS, iE;
             if (isEven) {
-                iS = (int)floorf(p0[d]);
-                iE = (int)floorf(p1[d]);
+                iS = (int)floorf(G.p0[d]);
+                iE = (int)floorf(G.p1[d]);
             } else {
-                iS = (int)floorf(p0[d] + 0.5f);
-                iE = (int)floorf(p1[d] + 0.5f);
+                iS = (int)floorf(G.p0[d] + 0.5f);
+                iE = (int)floorf(G.p1[d] + 0.5f);
             }
-            gs[d] = iS < iE ? iS : iE;
-            st[d] = (gs[d] == iS ? 2 : 0) | (gs[d] == iE ? 4 : 0) | (iS != iE ? 1 : 0);
-            p0[d] -= (float)gs[d];
-            p1[d] -= (float)gs[d];
+            G.gs[d] = iS < iE ? iS : iE;
+            G.st[d] = (G.gs[d] == iS ? 2 : 0) | (G.gs[d] == iE ? 4 : 0) | (iS != iE ? 1 : 0);
+            G.p0[d] -= (float)G.gs[d];
+            G.p1[d] -= (float)G.gs[d];
         }
+        return G;
+    }
+
+    // true when the move changes the shifted cell on some axis (the general form is needed)
+    __device__ __forceinline__ static bool changesCell(const Geometry& G) {
+        return ((G.st[0] | G.st[1] | G.st[2]) & 1) != 0;
+    }
+
+    // Deposit one particle. (cx,cy,cz) is the particle's cell in J-tile
+    // coordinates, G its move geometry. Tile layout: component-major planes
+    // of TV elements, [z][y][x] with pitches TX, TX*TY. GENERAL = false only
+    // for a geometry with changesCell(G) == false.
+    template <int TX, int TY, int TV, bool GENERAL>
+    __device__ __forceinline__ static void depositGeometry(unsigned long long* __restrict__ jt, int cx, int cy, int cz,
+                                                           const Geometry& G, float charge, const DepositConsts& k) {
+        constexpr int PX = 1, PY = TX, PZ = TX * TY;
         // tile element of cell (gridShift + begin) in every axis
         unsigned long long* const corner =
-            jt + ((cz + gs[2] + begin) * TY + (cy + gs[1] + begin)) * TX + (cx + gs[0] + begin);
+            jt + ((cz + G.gs[2] + begin) * TY + (cy + G.gs[1] + begin)) * TX + (cx + G.gs[0] + begin);
         // J_x: rotation (1,2,0): i->y, j->z, k->x
         {
-            float const q0[3] = {p0[1], p0[2], p0[0]};
-            float const q1[3] = {p1[1], p1[2], p1[0]};
-            int const s3[3] = {st[1], st[2], st[0]};
+            float const q0[3] = {G.p0[1], G.p0[2], G.p0[0]};
+            float const q1[3] = {G.p1[1], G.p1[2], G.p1[0]};
+            int const s3[3] = {G.st[1], G.st[2], G.st[0]};
             float const csd = charge * k.inv_vdt * k.cell[0];
-            cpt1D<PY, PZ, PX>(corner, q0, q1, s3, csd, k.jscale);
+            cpt1D<PY, PZ, PX, GENERAL>(corner, q0, q1, s3, csd, k.jscale);
         }
         // J_y: rotation (2,0,1): i->z, j->x, k->y
         {
-            float const q0[3] = {p0[2], p0[0], p0[1]};
-            float const q1[3] = {p1[2], p1[0], p1[1]};
-            int const s3[3] = {st[2], st[0], st[1]};
+            float const q0[3] = {G.p0[2], G.p0[0], G.p0[1]};
+            float const q1[3] = {G.p1[2], G.p1[0], G.p1[1]};
+            int const s3[3] = {G.st[2], G.st[0], G.st[1]};
             float const csd = charge * k.inv_vdt * k.cell[1];
-            cpt1D<PZ, PX, PY>(corner + TV, q0, q1, s3, csd, k.jscale);
+            cpt1D<PZ, PX, PY, GENERAL>(corner + TV, q0, q1, s3, csd, k.jscale);
         }
         // J_z: identity: i->x, j->y, k->z
         {
-            int const s3[3] = {st[0], st[1], st[2]};
             float const csd = charge * k.inv_vdt * k.cell[2];
-            cpt1D<PX, PY, PZ>(corner + 2 * TV, p0, p1, s3, csd, k.jscale);
+            cpt1D<PX, PY, PZ, GENERAL>(corner + 2 * TV, G.p0, G.p1, G.st, csd, k.jscale);
         }
+    }
+
+    // Deposit one particle. (cx,cy,cz) is the particle's cell in J-tile
+    // coordinates, pos its in-cell position after the move, vel its velocity.
+    template <int TX, int TY, int TV>
+    __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
+                                                   Vec3 vel, float charge, const DepositConsts& k) {
+        depositGeometry<TX, TY, TV, true>(jt, cx, cy, cz, geometry(pos, vel, k), charge, k);
     }
 };
 
