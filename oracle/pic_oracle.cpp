@@ -807,29 +807,64 @@ void deposit(State<T>& st, int species) {
 }
 
 // Canonical supercell shift (same order as the device):
-// collect leavers per exchange code in slot order; each supercell appends the
-// leavers its neighbours send to it for codes 1..26; gaps below the new count
-// are filled, k-th ascending gap by the k-th ascending valid slot above it.
+// 1. per supercell, the records of its leavers, grouped by exchange code and
+//    ascending in slot inside a code, go to its outbox;
+// 2. every supercell is compacted on its own: with nn = count - L (L leavers),
+//    the r-th leaver slot below nn (ascending) receives the r-th remaining
+//    particle at or above nn (ascending); count becomes nn;
+// 3. every supercell appends, for codes 1..26 in order, the outbox records of
+//    the neighbour that sends towards it with that code.
 template <typename T>
 void shift(State<T>& st, int species) {
+    struct Rec {
+        T pos[3], mom[3], w;
+        uint16_t cell;
+        uint32_t id;
+    };
     for (int s = 0; s < static_cast<int>(st.sp.size()); ++s) {
         if (species >= 0 && s != species) continue;
         Species<T>& S = st.sp[s];
         if (S.desc.pusher == 0) continue;
-        std::vector<std::vector<std::vector<int>>> out(st.nsc, std::vector<std::vector<int>>(27));
-        std::vector<uint32_t> nLeave(st.nsc, 0);
+        std::vector<std::vector<std::vector<Rec>>> out(st.nsc, std::vector<std::vector<Rec>>(27));
 #pragma omp parallel for schedule(static)
         for (int sc = 0; sc < st.nsc; ++sc) {
             size_t const base = static_cast<size_t>(sc) * S.cap;
-            for (uint32_t i = 0; i < S.count[sc]; ++i) {
+            uint32_t const n = S.count[sc];
+            std::vector<uint32_t> leavers;
+            for (uint32_t i = 0; i < n; ++i) {
                 int const m = S.mask[base + i];
-                if (m >= 2) {
-                    out[sc][m - 1].push_back(static_cast<int>(i));
-                    ++nLeave[sc];
+                if (m < 2) continue;
+                size_t const b = base + i;
+                Rec r;
+                for (int c = 0; c < 3; ++c) {
+                    r.pos[c] = S.pos[c][b];
+                    r.mom[c] = S.mom[c][b];
                 }
+                r.w = S.w[b];
+                r.cell = S.cell[b];
+                r.id = S.id[b];
+                out[sc][m - 1].push_back(r);
+                leavers.push_back(i);
             }
+            uint32_t const L = static_cast<uint32_t>(leavers.size());
+            if (L == 0) continue;
+            uint32_t const nn = n - L;
+            size_t q = 0;
+            for (uint32_t i = nn; i < n; ++i) {
+                if (S.mask[base + i] != 1) continue;
+                size_t const a = base + leavers[q++], b = base + i;
+                for (int c = 0; c < 3; ++c) {
+                    S.pos[c][a] = S.pos[c][b];
+                    S.mom[c][a] = S.mom[c][b];
+                }
+                S.w[a] = S.w[b];
+                S.cell[a] = S.cell[b];
+                S.id[a] = S.id[b];
+                S.mask[a] = 1;
+            }
+            for (uint32_t i = nn; i < n; ++i) S.mask[base + i] = 0;
+            S.count[sc] = nn;
         }
-        std::vector<uint32_t> occ(st.nsc);
         bool overflow = false;
 #pragma omp parallel for schedule(static) reduction(|| : overflow)
         for (int d = 0; d < st.nsc; ++d) {
@@ -844,58 +879,27 @@ void shift(State<T>& st, int species) {
                 int const sy = ((dy - ty) % st.nscy + st.nscy) % st.nscy;
                 int const sz = ((dz - tz) % st.nscz + st.nscz) % st.nscz;
                 int const src = sx + st.nscx * (sy + st.nscy * sz);
-                size_t const sbase = static_cast<size_t>(src) * S.cap;
-                for (int slot : out[src][code]) {
+                for (const Rec& r : out[src][code]) {
                     if (run >= static_cast<uint32_t>(S.cap)) {
                         overflow = true;
                         ++run;
                         continue;
                     }
-                    size_t const a = dbase + run, b = sbase + static_cast<size_t>(slot);
+                    size_t const a = dbase + run;
                     for (int c = 0; c < 3; ++c) {
-                        S.pos[c][a] = S.pos[c][b];
-                        S.mom[c][a] = S.mom[c][b];
+                        S.pos[c][a] = r.pos[c];
+                        S.mom[c][a] = r.mom[c];
                     }
-                    S.w[a] = S.w[b];
-                    S.cell[a] = S.cell[b];
-                    S.id[a] = S.id[b];
+                    S.w[a] = r.w;
+                    S.cell[a] = r.cell;
+                    S.id[a] = r.id;
                     S.mask[a] = 1;
                     ++run;
                 }
             }
-            occ[d] = std::min<uint32_t>(run, static_cast<uint32_t>(S.cap));
+            S.count[d] = std::min<uint32_t>(run, static_cast<uint32_t>(S.cap));
         }
         if (overflow) st.capacityError = true;
-#pragma omp parallel for schedule(static)
-        for (int d = 0; d < st.nsc; ++d) {
-            size_t const base = static_cast<size_t>(d) * S.cap;
-            uint32_t const tot = occ[d];
-            uint32_t const L = nLeave[d];
-            if (L == 0) {
-                S.count[d] = tot;
-                continue;
-            }
-            uint32_t const nn = tot > L ? tot - L : 0;
-            std::vector<uint32_t> gaps, srcs;
-            for (uint32_t i = 0; i < nn; ++i)
-                if (S.mask[base + i] >= 2) gaps.push_back(i);
-            for (uint32_t i = nn; i < tot; ++i)
-                if (S.mask[base + i] == 1) srcs.push_back(i);
-            size_t const M = std::min(gaps.size(), srcs.size());
-            for (size_t q = 0; q < M; ++q) {
-                size_t const a = base + gaps[q], b = base + srcs[q];
-                for (int c = 0; c < 3; ++c) {
-                    S.pos[c][a] = S.pos[c][b];
-                    S.mom[c][a] = S.mom[c][b];
-                }
-                S.w[a] = S.w[b];
-                S.cell[a] = S.cell[b];
-                S.id[a] = S.id[b];
-                S.mask[a] = 1;
-            }
-            for (uint32_t i = nn; i < tot; ++i) S.mask[base + i] = 0;
-            S.count[d] = nn;
-        }
     }
 }
 

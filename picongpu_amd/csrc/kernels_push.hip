@@ -274,7 +274,8 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     }
 
     // Epilogue: slot-ordered leaver list from the ballots, then grouped by
-    // exchange code (the input of the supercell shift).
+    // exchange code (the input of the supercell shift); in a full step also
+    // the staging of the shift (outbox records, compaction of the supercell).
     if (MODE != 2) {
         for (int s = 0; s < a.nsp; ++s) {
             const SpeciesDev& S = a.sp[s];
@@ -311,6 +312,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             }
             __syncthreads();
             groupLeavers<kPushWaves>(S, sc, L, sGroup, sWaveCnt);
+            if (a.stageShift) stageSC<kPushWaves>(S, sc, L, sWaveCnt);
         }
     }
 }

@@ -8,10 +8,15 @@
 // of scheduling:
 //   leaver lists (built by the push kernel's epilogue, or by kCollect):
 //            slot-ordered, and grouped by exchange code ascending in slot;
-//   insert:  every supercell appends, for codes 1..26 in order, the leavers
-//            its neighbours send towards it behind its own particles;
-//   compact: the k-th gap (ascending slot) below the new count is filled with
-//            the k-th valid particle (ascending slot) above it.
+//   stage:   the grouped leavers' records go to the supercell's outbox and the
+//            supercell is compacted on its own (stageSC): the k-th leaver slot
+//            below count-L is filled with the k-th remaining particle at or
+//            above it, ascending; count becomes count-L;
+//   insert:  every supercell appends, for codes 1..26 in order, the outbox
+//            records its neighbours send towards it behind its own particles.
+// In a full step the push epilogue stages (its supercell's lines are still in
+// L2), so the shift itself is the insert: coalesced reads of 32-byte records
+// and coalesced appends.
 #include "pic_shift_common.hpp"
 
 namespace pic {
@@ -42,15 +47,11 @@ __global__ __launch_bounds__(kThreads) void kCollect(SpeciesDev S) {
     groupLeavers<kWaves>(S, sc, run, g, sWaveCnt);
 }
 
-__device__ __forceinline__ void copyParticle(const SpeciesDev& S, size_t dst, size_t src) {
-    S.pos[0][dst] = S.pos[0][src];
-    S.pos[1][dst] = S.pos[1][src];
-    S.pos[2][dst] = S.pos[2][src];
-    S.mom[0][dst] = S.mom[0][src];
-    S.mom[1][dst] = S.mom[1][src];
-    S.mom[2][dst] = S.mom[2][src];
-    S.w[dst] = S.w[src];
-    S.cell[dst] = S.cell[src];
+// Staging of every supercell from the leaver lists (when the push did not stage).
+__global__ __launch_bounds__(kThreads) void kStage(SpeciesDev S) {
+    __shared__ uint32_t sWave[kWaves];
+    int const sc = blockIdx.x;
+    stageSC<kWaves>(S, sc, (int)S.outOff[(size_t)sc * kOffStride + kNumDirs], sWave);
 }
 
 // One wave per destination supercell, four supercells per workgroup.
@@ -122,77 +123,30 @@ __device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, 
             atomicOr(errFlags, 1u);
             continue;
         }
-        size_t const sbase = (size_t)sc * (size_t)S.cap;
-        size_t const from = sbase + (size_t)S.outIdx[sbase + bg + (k - st)];
+        size_t const rec = (size_t)sc * (size_t)S.cap + bg + (k - st);
+        float4 const r0 = S.outbox[2 * rec];
+        float4 const r1 = S.outbox[2 * rec + 1];
         size_t const dst = dbase + slot;
-        copyParticle(S, dst, from);
+        S.pos[0][dst] = r0.x;
+        S.pos[1][dst] = r0.y;
+        S.pos[2][dst] = r0.z;
+        S.mom[0][dst] = r0.w;
+        S.mom[1][dst] = r1.x;
+        S.mom[2][dst] = r1.y;
+        S.w[dst] = r1.z;
+        S.cell[dst] = (uint16_t)__float_as_int(r1.w);
         S.mask[dst] = 1;
     }
     if (lane == 0) {
-        uint32_t const occ = n + total;
-        S.occ[d] = occ < (uint32_t)S.cap ? occ : (uint32_t)S.cap;
+        uint32_t const nt = n + total;
+        S.count[d] = nt < (uint32_t)S.cap ? nt : (uint32_t)S.cap;
     }
-}
-
-// compaction of supercell d; called by one whole wave
-__device__ __forceinline__ void compactSC(const SpeciesDev& S, int d) {
-    int const lane = threadIdx.x & 63;
-    size_t const base = (size_t)d * (size_t)S.cap;
-    uint32_t const n = S.count[d];
-    uint32_t const tot = S.occ[d];
-    uint32_t const L = S.outOff[(size_t)d * kOffStride + kNumDirs];
-    if (L == 0u) {
-        if (lane == 0) S.count[d] = tot;
-        return;
-    }
-    uint32_t const nn = tot > L ? tot - L : 0u;  // tot < L only after a capacity error
-    const int32_t* gaps = S.scratch + base;  // slot-ordered leavers, written by an earlier kernel
-    // G: leavers below the new count (the gaps)
-    uint32_t G = 0;
-    for (uint32_t c0 = 0; c0 < L; c0 += 64) {
-        uint32_t const k = c0 + lane;
-        G += (uint32_t)__popcll(__ballot(k < L && (uint32_t)gaps[k] < nn));
-    }
-    // sources in order: own non-leavers in [nn, n) ascending, then the
-    // incoming particles from max(nn, n); source r fills gap r
-    uint32_t a = 0;
-    for (uint32_t c0 = nn; c0 < n; c0 += 64) {
-        uint32_t const i = c0 + lane;
-        bool const pred = i < n && S.mask[base + i] == 1;
-        unsigned long long const b = __ballot(pred);
-        if (pred) {
-            uint32_t const r = a + (uint32_t)__popcll(b & laneMaskLt());
-            if (r < G) {
-                size_t const dst = base + (size_t)gaps[r];
-                copyParticle(S, dst, base + i);
-                S.mask[dst] = 1;
-            }
-        }
-        a += (uint32_t)__popcll(b);
-    }
-    uint32_t const inStart = nn > n ? nn : n;
-    for (uint32_t k = a + lane; k < G; k += 64) {
-        uint32_t const srcSlot = inStart + (k - a);
-        if (srcSlot >= tot) continue;  // only after a capacity error
-        size_t const dst = base + (size_t)gaps[k];
-        copyParticle(S, dst, base + srcSlot);
-        S.mask[dst] = 1;
-    }
-    // gaps are below nn; the vacated tail [nn, tot) is cleared
-    for (uint32_t i = nn + lane; i < tot; i += 64) S.mask[base + i] = 0;
-    if (lane == 0) S.count[d] = nn;
 }
 
 __global__ __launch_bounds__(kThreads) void kInsert(SpeciesDev S, GridDev g, int nsc, uint32_t* errFlags) {
     int const d = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
     if (d >= nsc) return;  // whole wave
     insertSC(S, g, d, errFlags);
-}
-
-__global__ __launch_bounds__(kThreads) void kCompact(SpeciesDev S, int nsc) {
-    int const d = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
-    if (d >= nsc) return;  // whole wave
-    compactSC(S, d);
 }
 
 // Batched forms for ensembles: work items (table entry, supercell), one wave
@@ -206,21 +160,10 @@ __global__ __launch_bounds__(kThreads) void kInsertBatch(const ShiftEntry* __res
     insertSC(e.S, e.g, it.y, e.errFlags);
 }
 
-__global__ __launch_bounds__(kThreads) void kCompactBatch(const ShiftEntry* __restrict__ table,
-                                                         const int2* __restrict__ work, int nwork) {
-    int const w = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
-    if (w >= nwork) return;  // whole wave
-    int2 const it = work[w];
-    compactSC(table[it.x].S, it.y);
-}
-
 hipError_t launchShiftBatch(const ShiftEntry* table, const int2* work, int nwork, hipStream_t s) {
     if (nwork <= 0) return hipSuccess;
     int const blocks = (nwork + kScPerBlock - 1) / kScPerBlock;
     hipLaunchKernelGGL(kInsertBatch, dim3(blocks), dim3(kThreads), 0, s, table, work, nwork);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kCompactBatch, dim3(blocks), dim3(kThreads), 0, s, table, work, nwork);
     return hipGetLastError();
 }
 
@@ -230,13 +173,15 @@ hipError_t launchCollect(const GridDev& g, const SpeciesDev& sp, hipStream_t s) 
     return hipGetLastError();
 }
 
-hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, hipStream_t s) {
+hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, bool staged, hipStream_t s) {
     int const nsc = g.nscx * g.nscy * g.nscz;
+    if (!staged) {
+        hipLaunchKernelGGL(kStage, dim3(nsc), dim3(kThreads), 0, s, sp);
+        hipError_t const e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     int const blocks = (nsc + kScPerBlock - 1) / kScPerBlock;
     hipLaunchKernelGGL(kInsert, dim3(blocks), dim3(kThreads), 0, s, sp, g, nsc, errFlags);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kCompact, dim3(blocks), dim3(kThreads), 0, s, sp, nsc);
     return hipGetLastError();
 }
 

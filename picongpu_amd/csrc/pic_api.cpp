@@ -161,7 +161,10 @@ struct SpeciesMem {
     float mass = 0.0f;
     float charge = 0.0f;
     double wmax = 0.0;
-    bool listsValid = false;  // leaver lists of the last push are in outIdx/scratch/outOff
+    // what the last push left for the shift: 0 nothing (the shift collects the
+    // leaver lists itself), 1 the leaver lists in scratch/outIdx/outOff, 2 the
+    // lists plus the staged outbox and compaction (only the insert remains)
+    int shiftReady = 0;
     pic::SpeciesDev dev{};
 };
 
@@ -459,7 +462,7 @@ std::vector<std::vector<int>> pushGroups(pic_ctx* c, int32_t species, bool byPus
     return groups;
 }
 
-int runPush(pic_ctx* c, int32_t species, int mode) {
+int runPush(pic_ctx* c, int32_t species, int mode, bool stageShift = false) {
     bool const byPusher = mode != 2;
     auto groups = pushGroups(c, species, byPusher);
     for (const auto& grp : groups) {
@@ -468,8 +471,9 @@ int runPush(pic_ctx* c, int32_t species, int mode) {
         pic::PushArgs a = makePushArgs(c);
         a.nsp = static_cast<int>(grp.size());
         for (size_t i = 0; i < grp.size(); ++i) a.sp[i] = c->sp[grp[i]].dev;
+        a.stageShift = stageShift ? 1 : 0;
         if (mode != 2)
-            for (int s : grp) c->sp[s].listsValid = c->sp[s].dev.lists != 0;
+            for (int s : grp) c->sp[s].shiftReady = c->sp[s].dev.lists != 0 ? (stageShift ? 2 : 1) : 0;
         if (mode == 0) {
             a.scList = c->dAll;
             TimerScope ts(c, PIC_TIMER_PUSH);
@@ -568,7 +572,7 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
             if (devAlloc(c, &d.pos[a], slots) || devAlloc(c, &d.mom[a], slots)) return bail(PIC_E_HIP);
         }
         if (devAlloc(c, &d.w, slots) || devAlloc(c, &d.cell, slots) || devAlloc(c, &d.mask, slots) ||
-            devAlloc(c, &d.count, c->nsc) || devAlloc(c, &d.occ, c->nsc) || devAlloc(c, &d.outIdx, slots) ||
+            devAlloc(c, &d.count, c->nsc) || devAlloc(c, &d.outbox, 2 * slots) || devAlloc(c, &d.outIdx, slots) ||
             devAlloc(c, &d.scratch, slots) || devAlloc(c, &d.outOff, static_cast<size_t>(c->nsc) * pic::kOffStride) ||
             devAlloc(c, &d.leaveBits, static_cast<size_t>(c->nsc) * pic::kPushThreadsPerBlock))
             return bail(PIC_E_HIP);
@@ -700,7 +704,7 @@ int pic_write_frames(pic_ctx* c, int32_t s, const uint32_t* counts, const float*
     SpeciesMem& m = c->sp[s];
     pic::SpeciesDev& d = m.dev;
     size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
-    m.listsValid = false;
+    m.shiftReady = 0;
     if (counts) {
         for (int i = 0; i < c->nsc; ++i)
             if (counts[i] > static_cast<uint32_t>(m.cap)) return fail(PIC_E_CAPACITY, "count above frame capacity");
@@ -817,7 +821,7 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
         a.rampStart = desc->ramp_start_si / c->cfg.cell_si[a.axis];
         a.rampLength = desc->ramp_length_si / c->cfg.cell_si[a.axis];
     }
-    m.listsValid = false;
+    m.shiftReady = 0;
     PIC_HIP(pic::launchInit(a, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     m.wmax = std::fabs(static_cast<double>(static_cast<float>(a.weighting)));
@@ -913,9 +917,10 @@ int pic_shift(pic_ctx* c, int32_t species) {
         if (species >= 0 && s != species) continue;
         if (c->sp[s].desc.pusher == PIC_PUSHER_NONE) continue;
         TimerScope ts(c, PIC_TIMER_SHIFT);
-        if (!c->sp[s].listsValid) PIC_HIP(pic::launchCollect(c->g, c->sp[s].dev, c->stream));
-        PIC_HIP(pic::launchShift(c->g, c->sp[s].dev, c->errFlags, c->stream));
-        c->sp[s].listsValid = false;
+        int const ready = c->sp[s].shiftReady;
+        if (ready == 0) PIC_HIP(pic::launchCollect(c->g, c->sp[s].dev, c->stream));
+        PIC_HIP(pic::launchShift(c->g, c->sp[s].dev, c->errFlags, ready == 2, c->stream));
+        c->sp[s].shiftReady = 0;
     }
     return PIC_OK;
 }
@@ -967,7 +972,7 @@ int fieldStep(pic_ctx* c) {
 int oneStep(pic_ctx* c) {
     int r;
     if ((r = pic_current_reset(c)) != PIC_OK) return r;
-    if ((r = pic_push(c, -1, 1)) != PIC_OK) return r;
+    if ((r = runPush(c, -1, 1, true)) != PIC_OK) return r;
     if ((r = pic_shift(c, -1)) != PIC_OK) return r;
     return fieldStep(c);
 }
@@ -1031,7 +1036,7 @@ int pic_step_graph(pic_ctx* c, int32_t nsteps) {
         }
         PIC_HIP(hipGraphLaunch(c->graphExec[p], c->stream));
     }
-    for (SpeciesMem& m : c->sp) m.listsValid = false;
+    for (SpeciesMem& m : c->sp) m.shiftReady = 0;
     return checkErrors(c);
 }
 
@@ -1565,6 +1570,7 @@ int ensUploadTables(pic_ensemble* e) {
             }
             pic::PushArgs a = makePushArgs(c);
             for (int d = 0; d < 3; ++d) a.E[d] = E[d];
+            a.stageShift = 1;
             int k = 0;
             for (const SpeciesMem& m : c->sp)
                 if (m.desc.pusher != PIC_PUSHER_NONE) a.sp[k++] = m.dev;
@@ -1680,7 +1686,7 @@ int ensOneStep(pic_ensemble* e) {
 int ensFinish(pic_ensemble* e) {
     PIC_HIP(hipStreamSynchronize(e->stream));
     for (pic_ctx* c : e->sims) {
-        for (SpeciesMem& m : c->sp) m.listsValid = false;
+        for (SpeciesMem& m : c->sp) m.shiftReady = 0;
         int const r = checkErrors(c);
         if (r != PIC_OK) return r;
     }

@@ -20,7 +20,7 @@ struct SpeciesDev {
     uint16_t* cell;
     uint8_t* mask;
     uint32_t* count;     // particles per supercell
-    uint32_t* occ;       // occupied slots per supercell during a shift
+    float4* outbox;      // per supercell: records of the leavers in outIdx order, 2 float4 each
     int32_t* outIdx;     // per supercell: leaver slots grouped by exchange code
     int32_t* scratch;    // per supercell: gap/source lists of the compaction
     uint32_t* outOff;    // per supercell: 28 offsets into outIdx (code 1..26, [27]=total)
@@ -50,6 +50,7 @@ struct PushArgs {
     SpeciesDev sp[PIC_MAX_SPECIES];
     const int32_t* scList;  // supercells of this launch, in launch order
     uint32_t* errFlags;
+    int stageShift;  // 1: the epilogue also moves the leavers to the outbox and compacts the supercell
 };
 
 struct FieldArgs {
@@ -66,7 +67,9 @@ struct FieldArgs {
 // launchers (return hipError_t of the launch)
 // mode 0: push only, 1: push + fused deposit, 2: deposit only
 hipError_t launchPush(int supp, int pusher, int mode, const PushArgs& a, int nblocks, hipStream_t s);
-hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, hipStream_t s);
+// shift of one species: `staged` = the push epilogue already moved the leavers
+// to the outbox and compacted every supercell (then only the insert runs)
+hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, bool staged, hipStream_t s);
 hipError_t launchCollect(const GridDev& g, const SpeciesDev& sp, hipStream_t s);
 // the push kernel builds leaver lists for species with at most this many slots per supercell
 constexpr int kPushThreadsPerBlock = 512;
@@ -130,7 +133,8 @@ struct ShiftEntry {
 // items (simulation, supercell) into a table of PushArgs (fused push + deposit)
 hipError_t launchPushBatch(int supp, int pusher, const PushArgs* table, const int2* work, int nwork, int nblocks,
                            hipStream_t s);
-// items (entry, supercell) into a table of (species, grid)
+// items (entry, supercell) into a table of (species, grid); the batched push
+// staged the shift, so this is the insert
 hipError_t launchShiftBatch(const ShiftEntry* table, const int2* work, int nwork, hipStream_t s);
 // items (simulation, tile) for the tiled kernel and (simulation, block of 256
 // cells) for the per-cell kernels, into tables of FieldArgs and new-E planes

@@ -99,4 +99,60 @@ __device__ void groupLeavers(const SpeciesDev& S, int sc, int L, GroupScratch& g
     }
 }
 
+__device__ __forceinline__ void copyParticle(const SpeciesDev& S, size_t dst, size_t src) {
+    S.pos[0][dst] = S.pos[0][src];
+    S.pos[1][dst] = S.pos[1][src];
+    S.pos[2][dst] = S.pos[2][src];
+    S.mom[0][dst] = S.mom[0][src];
+    S.mom[1][dst] = S.mom[1][src];
+    S.mom[2][dst] = S.mom[2][src];
+    S.w[dst] = S.w[src];
+    S.cell[dst] = S.cell[src];
+}
+
+// The leavers of supercell sc leave it. With the slot-ordered leaver list in
+// S.scratch and the grouped one in S.outIdx/S.outOff (L leavers):
+//  1. the record of the k-th grouped leaver goes to outbox entry k (position,
+//     momentum, weighting, localCellIdx: 32 bytes, two float4);
+//  2. the supercell is compacted on its own: with nn = count - L, the r-th
+//     leaver slot below nn (ascending) receives the r-th remaining particle at
+//     or above nn (ascending); multiMask of [nn, count) is cleared and count
+//     becomes nn.
+// The insert of the neighbours' outbox records then appends at nn. Called by
+// the whole workgroup (NW waves); sWave is NW words of LDS.
+template <int NW>
+__device__ void stageSC(const SpeciesDev& S, int sc, int L, uint32_t* sWave) {
+    if (L == 0) return;  // uniform
+    constexpr int NT = NW * 64;
+    int const tid = threadIdx.x;
+    size_t const base = (size_t)sc * (size_t)S.cap;
+    uint32_t const n = S.count[sc];
+    for (int k = tid; k < L; k += NT) {
+        size_t const from = base + (size_t)S.outIdx[base + k];
+        float4 const r0 = make_float4(S.pos[0][from], S.pos[1][from], S.pos[2][from], S.mom[0][from]);
+        float4 const r1 = make_float4(S.mom[1][from], S.mom[2][from], S.w[from], __int_as_float((int)S.cell[from]));
+        S.outbox[2 * (base + k)] = r0;
+        S.outbox[2 * (base + k) + 1] = r1;
+    }
+    __syncthreads();  // every leaver record is out before its slot is overwritten
+    uint32_t const nn = n - (uint32_t)L;
+    const int32_t* leavers = S.scratch + base;
+    int run = 0;
+    for (uint32_t c0 = nn; c0 < n; c0 += NT) {
+        uint32_t const i = c0 + tid;
+        bool const src = i < n && S.mask[base + i] == 1;
+        int total;
+        int const r = blockRank<NW>(src, sWave, &total);
+        if (src) {
+            size_t const dst = base + (size_t)leavers[run + r];
+            copyParticle(S, dst, base + i);
+            S.mask[dst] = 1;
+        }
+        run += total;
+    }
+    __syncthreads();  // multiMask of [nn, n) has been read
+    for (uint32_t i = nn + tid; i < n; i += NT) S.mask[base + i] = 0;
+    if (tid == 0) S.count[sc] = nn;
+}
+
 }  // namespace pic
