@@ -56,6 +56,7 @@ extern "C" {
 #define PIC_PUSHER_NONE 0
 #define PIC_PUSHER_BORIS 1
 #define PIC_PUSHER_VAY 2
+#define PIC_PUSHER_HIGUERA_CARY 3 /* particlePusherHigueraCary.hpp:49-142 */
 
 /* field solvers: fields/MaxwellSolver/Yee/Yee.def:36-47, Lehe/Lehe.def:57-73 */
 #define PIC_SOLVER_YEE 0

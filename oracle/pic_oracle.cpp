@@ -51,7 +51,7 @@ typedef struct oracle_species {
     double mass_ratio;
     double charge_ratio;
     int32_t shape;   // 1 CIC, 2 TSC, 3 PQS
-    int32_t pusher;  // 0 none, 1 Boris, 2 Vay
+    int32_t pusher;  // 0 none, 1 Boris, 2 Vay, 3 Higuera-Cary
     int32_t frames_per_sc;
     int32_t reserved;
 } oracle_species;
@@ -389,6 +389,45 @@ void vay(const V3<T>& e, const V3<T>& b, V3<T>& mom, V3<T>& vel, T mass, T charg
     vel = velocityOf(mom, mass, k.mue0_eps0);
 }
 
+// particlePusherHigueraCary.hpp:61-142 (second half acceleration with mom_plus),
+// auxiliary quantities in FP64 (param/pusher.param:71), Gamma<> in T
+template <typename T>
+void higueraCary(const V3<T>& e, const V3<T>& b, V3<T>& mom, V3<T>& vel, T mass, T charge, const Consts<T>& k) {
+    T const deltaT = k.dt;
+    T const hq = T(0.5) * charge;
+    V3<T> const de{hq * e.x * deltaT, hq * e.y * deltaT, hq * e.z * deltaT};
+    V3<T> const mm{mom.x + de.x, mom.y + de.y, mom.z + de.z};
+    double const mx = static_cast<double>(mm.x), my = static_cast<double>(mm.y), mz = static_cast<double>(mm.z);
+    double const gm = static_cast<double>(gammaOf(mm, mass, k.c));
+    double const tx = static_cast<double>(T(0.5) * b.x * charge * deltaT / mass);
+    double const ty = static_cast<double>(T(0.5) * b.y * charge * deltaT / mass);
+    double const tz = static_cast<double>(T(0.5) * b.z * charge * deltaT / mass);
+    double tau2 = tx * tx;
+    tau2 += ty * ty;
+    tau2 += tz * tz;
+    double const sigma = gm * gm - tau2;
+    double dmt = mx * tx;
+    dmt += my * ty;
+    dmt += mz * tz;
+    double const u_star = dmt / static_cast<double>(mass * k.c);
+    double const gp = std::sqrt(0.5 * (sigma + std::sqrt(sigma * sigma + 4.0 * (tau2 + u_star * u_star))));
+    double const t0 = tx / gp, t1 = ty / gp, t2 = tz / gp;
+    double tt = t0 * t0;
+    tt += t1 * t1;
+    tt += t2 * t2;
+    double const s = 1.0 / (1.0 + tt);
+    double dm = mx * t0;
+    dm += my * t1;
+    dm += mz * t2;
+    double const px = s * (mx + dm * t0 + (my * t2 - mz * t1));
+    double const py = s * (my + dm * t1 + (mz * t0 - mx * t2));
+    double const pz = s * (mz + dm * t2 + (mx * t1 - my * t0));
+    V3<T> const d2{static_cast<T>(py * t2 - pz * t1), static_cast<T>(pz * t0 - px * t2),
+                   static_cast<T>(px * t1 - py * t0)};
+    mom = {static_cast<T>(px) + (de.x + d2.x), static_cast<T>(py) + (de.y + d2.y), static_cast<T>(pz) + (de.z + d2.z)};
+    vel = velocityOf(mom, mass, k.mue0_eps0);
+}
+
 // ---- moveParticle: particles/MoveParticle.hpp:47-160 -------------------------
 template <typename T>
 int moveParticle(T* newPos, uint16_t& localCellIdx, uint8_t& multiMask, int* cellStep) {
@@ -574,6 +613,8 @@ V3<T> pushOne(State<T>& st, Species<T>& S, size_t g, int ox, int oy, int oz, int
     V3<T> vel;
     if (S.desc.pusher == 2)
         vay(e, b, mom, vel, mass, charge, st.k);
+    else if (S.desc.pusher == 3)
+        higueraCary(e, b, mom, vel, mass, charge, st.k);
     else
         boris(e, b, mom, vel, mass, charge, st.k);
     for (int d = 0; d < 3; ++d) pos[d] += (vel[d] * st.k.dt) / st.k.cell[d];

@@ -17,7 +17,7 @@ from . import build as _build
 MAX_SPECIES = 4
 
 SHAPE_CIC, SHAPE_TSC, SHAPE_PQS = 1, 2, 3
-PUSHER_NONE, PUSHER_BORIS, PUSHER_VAY = 0, 1, 2
+PUSHER_NONE, PUSHER_BORIS, PUSHER_VAY, PUSHER_HIGUERA_CARY = 0, 1, 2, 3
 SOLVER_YEE, SOLVER_LEHE = 0, 1
 FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
 TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2

@@ -209,6 +209,8 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                     Vec3 vel;
                     if (PUSHER == PIC_PUSHER_VAY)
                         pushVay(e, b, mom, vel, mass, charge, pk);
+                    else if (PUSHER == PIC_PUSHER_HIGUERA_CARY)
+                        pushHigueraCary(e, b, mom, vel, mass, charge, pk);
                     else
                         pushBoris(e, b, mom, vel, mass, charge, pk);
                     float p[3] = {pos.x, pos.y, pos.z};
@@ -342,20 +344,22 @@ hipError_t launchPushBatch(int supp, int pusher, const PushArgs* table, const in
                            hipStream_t s) {
     if (nwork <= 0) return hipSuccess;
     if (nblocks <= 0 || nblocks > nwork) nblocks = nwork;
-    if (pusher == PIC_PUSHER_VAY) {
-        switch (supp) {
-            case 2: hipLaunchKernelGGL((kPushDepositBatch<2, PIC_PUSHER_VAY, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
-            case 3: hipLaunchKernelGGL((kPushDepositBatch<3, PIC_PUSHER_VAY, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
-            case 4: hipLaunchKernelGGL((kPushDepositBatch<4, PIC_PUSHER_VAY, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
-            default: return hipErrorInvalidValue;
-        }
-    } else {
-        switch (supp) {
-            case 2: hipLaunchKernelGGL((kPushDepositBatch<2, PIC_PUSHER_BORIS, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
-            case 3: hipLaunchKernelGGL((kPushDepositBatch<3, PIC_PUSHER_BORIS, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
-            case 4: hipLaunchKernelGGL((kPushDepositBatch<4, PIC_PUSHER_BORIS, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); break;
-            default: return hipErrorInvalidValue;
-        }
+    switch (pusher * 8 + supp) {
+#define PIC_BATCH_CASE(P, S)                                                                                        \
+    case P * 8 + S:                                                                                                 \
+        hipLaunchKernelGGL((kPushDepositBatch<S, P, 1>), dim3(nblocks), dim3(kPushThreads), 0, s, table, work, nwork); \
+        break;
+        PIC_BATCH_CASE(PIC_PUSHER_BORIS, 2)
+        PIC_BATCH_CASE(PIC_PUSHER_BORIS, 3)
+        PIC_BATCH_CASE(PIC_PUSHER_BORIS, 4)
+        PIC_BATCH_CASE(PIC_PUSHER_VAY, 2)
+        PIC_BATCH_CASE(PIC_PUSHER_VAY, 3)
+        PIC_BATCH_CASE(PIC_PUSHER_VAY, 4)
+        PIC_BATCH_CASE(PIC_PUSHER_HIGUERA_CARY, 2)
+        PIC_BATCH_CASE(PIC_PUSHER_HIGUERA_CARY, 3)
+        PIC_BATCH_CASE(PIC_PUSHER_HIGUERA_CARY, 4)
+#undef PIC_BATCH_CASE
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -371,6 +375,9 @@ static hipError_t launchS(int pusher, int mode, const PushArgs& a, int nblocks, 
     if (mode == 2) return launchT<SUPP, PIC_PUSHER_BORIS, 2>(a, nblocks, s);
     if (pusher == PIC_PUSHER_VAY)
         return mode == 1 ? launchT<SUPP, PIC_PUSHER_VAY, 1>(a, nblocks, s) : launchT<SUPP, PIC_PUSHER_VAY, 0>(a, nblocks, s);
+    if (pusher == PIC_PUSHER_HIGUERA_CARY)
+        return mode == 1 ? launchT<SUPP, PIC_PUSHER_HIGUERA_CARY, 1>(a, nblocks, s)
+                         : launchT<SUPP, PIC_PUSHER_HIGUERA_CARY, 0>(a, nblocks, s);
     return mode == 1 ? launchT<SUPP, PIC_PUSHER_BORIS, 1>(a, nblocks, s) : launchT<SUPP, PIC_PUSHER_BORIS, 0>(a, nblocks, s);
 }
 

@@ -60,7 +60,7 @@ int validateConfig(const pic_config& c) {
     for (int s = 0; s < c.n_species; ++s) {
         const pic_species_desc& d = c.species[s];
         if (d.shape < PIC_SHAPE_CIC || d.shape > PIC_SHAPE_PQS) return fail(PIC_E_INVALID, "unknown shape");
-        if (d.pusher < PIC_PUSHER_NONE || d.pusher > PIC_PUSHER_VAY) return fail(PIC_E_INVALID, "unknown pusher");
+        if (d.pusher < PIC_PUSHER_NONE || d.pusher > PIC_PUSHER_HIGUERA_CARY) return fail(PIC_E_INVALID, "unknown pusher");
         if (d.frames_per_sc <= 0 || d.frames_per_sc > 255)
             return fail(PIC_E_INVALID, "frames_per_sc must be in [1,255]");
         if (!(d.mass_ratio > 0.0)) return fail(PIC_E_INVALID, "mass_ratio must be positive");

@@ -294,6 +294,48 @@ __device__ __forceinline__ void pushVay(Vec3 e, Vec3 b, Vec3& mom, Vec3& vel, fl
 }
 
 // ---------------------------------------------------------------------------
+// Higuera-Cary (particlePusherHigueraCary.hpp:61-142, with the WarpX
+// correction mom_minus -> mom_plus in the second half acceleration), the
+// auxiliary quantities in FP64 (param/pusher.param:71 sqrt_HigueraCary =
+// precision64Bit); Gamma<> stays FP32 (Gamma.def:31, Gamma.hpp:28-38).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void pushHigueraCary(Vec3 e, Vec3 b, Vec3& mom, Vec3& vel, float mass, float charge,
+                                                const PushConsts& k) {
+    float const deltaT = k.dt;
+    float const hq = 0.5f * charge;
+    Vec3 const de{hq * e.x * deltaT, hq * e.y * deltaT, hq * e.z * deltaT};
+    Vec3 const mmf{mom.x + de.x, mom.y + de.y, mom.z + de.z};
+    double const mx = mmf.x, my = mmf.y, mz = mmf.z;
+    double const gamma_minus = (double)gammaOf(mmf, mass, k.c2);
+    double const tx = (double)(0.5f * b.x * charge * deltaT / mass);
+    double const ty = (double)(0.5f * b.y * charge * deltaT / mass);
+    double const tz = (double)(0.5f * b.z * charge * deltaT / mass);
+    double tau2 = tx * tx;
+    tau2 += ty * ty;
+    tau2 += tz * tz;
+    double const sigma = gamma_minus * gamma_minus - tau2;
+    double dmt = mx * tx;
+    dmt += my * ty;
+    dmt += mz * tz;
+    double const u_star = dmt / (double)(mass * k.c);
+    double const gamma_plus = sqrt(0.5 * (sigma + sqrt(sigma * sigma + 4.0 * (tau2 + u_star * u_star))));
+    double const t0 = tx / gamma_plus, t1 = ty / gamma_plus, t2 = tz / gamma_plus;
+    double tt = t0 * t0;
+    tt += t1 * t1;
+    tt += t2 * t2;
+    double const s = 1.0 / (1.0 + tt);
+    double dm = mx * t0;
+    dm += my * t1;
+    dm += mz * t2;
+    double const px = s * (mx + dm * t0 + (my * t2 - mz * t1));
+    double const py = s * (my + dm * t1 + (mz * t0 - mx * t2));
+    double const pz = s * (mz + dm * t2 + (mx * t1 - my * t0));
+    Vec3 const d2{(float)(py * t2 - pz * t1), (float)(pz * t0 - px * t2), (float)(px * t1 - py * t0)};
+    mom = Vec3{(float)px + (de.x + d2.x), (float)py + (de.y + d2.y), (float)pz + (de.z + d2.z)};
+    vel = velocityOf(mom, mass, k.rc2);
+}
+
+// ---------------------------------------------------------------------------
 // moveParticle (particles/MoveParticle.hpp:47-160). pos in [-1,2) per axis on
 // entry, [0,1) on exit; dir receives the cell step per axis; returns the new
 // local cell index (wrapped into the destination supercell) and multiMask.

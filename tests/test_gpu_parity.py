@@ -33,6 +33,8 @@ CASES = [
     ("pqs-boris", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_BORIS)),
     ("cic-vay", lambda: small_thermal(abi.SHAPE_CIC, abi.PUSHER_VAY)),
     ("pqs-vay", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_VAY)),
+    ("cic-hc", lambda: small_thermal(abi.SHAPE_CIC, abi.PUSHER_HIGUERA_CARY, u=0.3)),
+    ("tsc-hc", lambda: small_thermal(abi.SHAPE_TSC, abi.PUSHER_HIGUERA_CARY, u=0.3)),
 ]
 
 
@@ -154,6 +156,7 @@ STEP_CASES = [
     ("thermal-cic-boris", lambda: small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS)),
     ("thermal-tsc-vay-lehe", small_lehe),
     ("thermal-pqs-vay", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_VAY)),
+    ("thermal-tsc-hc", lambda: small_thermal(abi.SHAPE_TSC, abi.PUSHER_HIGUERA_CARY, u=0.3)),
     ("kh-tsc-2species", lambda: small_two_species(abi.SHAPE_TSC)),
 ]
 

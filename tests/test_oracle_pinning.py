@@ -290,7 +290,7 @@ def track(st, ora, k):
     return g * np.array(k.cell, np.float64), f["mom"][:, slot].copy()
 
 
-@pytest.mark.parametrize("pusher", [abi.PUSHER_BORIS, abi.PUSHER_VAY])
+@pytest.mark.parametrize("pusher", [abi.PUSHER_BORIS, abi.PUSHER_VAY, abi.PUSHER_HIGUERA_CARY])
 @pytest.mark.parametrize("precision", [32, 64])
 def test_pusher_gyration(pusher, precision):
     st, ora, k, b0, charge, mass, gamma = single_particle_oracle(pusher, precision=precision)
@@ -324,6 +324,41 @@ def test_pusher_gyration(pusher, precision):
     ang = np.unwrap(np.arctan2(moms[:, 1], moms[:, 0]))
     per_turn = abs(abs(ang[n] - ang[0]) - 2 * math.pi)
     assert per_turn < 0.16
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_relativistic_exb_drift(precision):
+    """Crossed E (y) and B (z) with |E| = 0.6 c |B|: a particle moving at the
+    drift velocity E x B / B^2 = 0.6 c x with its Lorentz factor feels no net
+    force. Vay and Higuera-Cary keep that momentum (both are built to give the
+    relativistic E x B drift exactly); Boris does not (its half-step velocity
+    from the averaged momentum misses the drift at relativistic speed), which
+    shows the check tells the pushers apart."""
+    err = {}
+    for pusher in (abi.PUSHER_BORIS, abi.PUSHER_VAY, abi.PUSHER_HIGUERA_CARY):
+        st, ora, k, b0, charge, mass, gamma = single_particle_oracle(pusher, precision=precision)
+        bd = 0.6
+        gd = 1.0 / math.sqrt(1.0 - bd * bd)
+        E = np.zeros((3, st.ncells), np.float32)
+        E[1, :] = bd * k.c * b0
+        ora.set_field(abi.FIELD_E, E)
+        f = ora.get_frames(0)
+        cap = ora.cap(0)
+        sc = int(np.nonzero(f["counts"])[0][0])
+        f["mom"][:, sc * cap] = [gd * mass * bd * k.c, 0.0, 0.0]
+        ora.set_frames(0, {key: (v.astype(np.float32) if v.dtype == np.float64 else v) for key, v in f.items()})
+        _, p0 = track(st, ora, k)
+        worst = 0.0
+        for _ in range(200):
+            ora.push(-1, False)
+            ora.shift(-1)
+            _, p = track(st, ora, k)
+            worst = max(worst, float(np.abs(p - p0).max() / np.abs(p0).max()))
+        err[pusher] = worst
+        ora.close()
+    tol = 1e-6 if precision == 64 else 5e-6
+    assert err[abi.PUSHER_VAY] < tol and err[abi.PUSHER_HIGUERA_CARY] < tol, err
+    assert err[abi.PUSHER_BORIS] > 1e-3, err
 
 
 def test_boris_phase_lag_scales_with_dt_squared():
