@@ -62,6 +62,23 @@ extern "C" {
 #define PIC_SOLVER_YEE 0
 #define PIC_SOLVER_LEHE 1
 
+/* outer boundary per axis. PERIODIC: fields and particles wrap. OPEN: the
+ * reference with that axis not periodic and every species' particle boundary
+ * Kind::Absorbing with offset 0 (particles/boundary/Description.hpp:36-44):
+ * particles crossing the border are removed (boundary/Absorbing.hpp:63-92,
+ * before the deposit sees them); fields read zero beyond it (the guard cells
+ * of a non-periodic border are never written nor exchanged) and current
+ * deposited beyond it is dropped (FieldJ.x.cpp:156-174 folds guards only
+ * between neighbours). */
+#define PIC_BOUNDARY_PERIODIC 0
+#define PIC_BOUNDARY_OPEN 1
+
+/* field absorber at open borders (fields/absorber/Absorber.hpp): NONE, or the
+ * exponential damping of absorber/exponential/Exponential.kernel:63-112 */
+#define PIC_ABSORBER_NONE 0
+#define PIC_ABSORBER_EXPONENTIAL 1
+#define PIC_ABSORBER_MAX_CELLS 64
+
 /* fields */
 #define PIC_FIELD_E 0
 #define PIC_FIELD_B 1
@@ -101,6 +118,11 @@ typedef struct pic_config {
     pic_species_desc species[PIC_MAX_SPECIES];
     int32_t j_scale_exp;      /* fixed-point exponent of the current tile, 0 = automatic */
     int32_t device;           /* HIP device ordinal */
+    int32_t boundary[3];      /* PIC_BOUNDARY_* per axis; zero-initialised = periodic */
+    int32_t absorber;         /* PIC_ABSORBER_* (acts on open axes only) */
+    int32_t absorber_cells[3][2];  /* param/fieldAbsorber.param NUM_CELLS [axis][negative, positive],
+                                    * at most PIC_ABSORBER_MAX_CELLS */
+    float absorber_strength[3][2]; /* param/fieldAbsorber.param exponential::STRENGTH */
 } pic_config;
 
 /* FP32 constants in PIC units derived from a pic_config, computed as

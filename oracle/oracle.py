@@ -24,7 +24,9 @@ class OracleConfig(C.Structure):
     _fields_ = [("grid", C.c_int32 * 3), ("cell_si", C.c_double * 3), ("dt_si", C.c_double),
                 ("base_density_si", C.c_double), ("typical_ppc", C.c_int32), ("solver", C.c_int32),
                 ("lehe_dir", C.c_int32), ("n_species", C.c_int32), ("species", OracleSpecies * 4),
-                ("j_scale_exp", C.c_int32), ("device", C.c_int32)]
+                ("j_scale_exp", C.c_int32), ("device", C.c_int32), ("boundary", C.c_int32 * 3),
+                ("absorber", C.c_int32), ("absorber_cells", (C.c_int32 * 2) * 3),
+                ("absorber_strength", (C.c_float * 2) * 3)]
 
 
 class OracleConstants(C.Structure):
@@ -112,6 +114,12 @@ def config_from(setup) -> OracleConfig:
         c.species[i].shape = s.shape
         c.species[i].pusher = s.pusher
         c.species[i].frames_per_sc = s.frames()
+    for d in range(3):
+        c.boundary[d] = int(setup.boundary[d])
+        for side in range(2):
+            c.absorber_cells[d][side] = int(setup.absorber_cells[d][side])
+            c.absorber_strength[d][side] = float(setup.absorber_strength[d][side])
+    c.absorber = int(setup.absorber)
     return c
 
 

@@ -68,6 +68,10 @@ typedef struct oracle_config {
     oracle_species species[4];
     int32_t j_scale_exp;
     int32_t device;
+    int32_t boundary[3];            // 0 periodic, 1 open (absorbing particles, zero fields beyond)
+    int32_t absorber;               // 0 none, 1 exponential
+    int32_t absorber_cells[3][2];   // fieldAbsorber.param NUM_CELLS [axis][negative, positive]
+    float absorber_strength[3][2];  // fieldAbsorber.param exponential::STRENGTH
 } oracle_config;
 
 typedef struct oracle_constants {
@@ -277,6 +281,15 @@ struct State {
         z = ((z % nz) + nz) % nz;
         return static_cast<size_t>(x) + static_cast<size_t>(nx) * (static_cast<size_t>(y) + static_cast<size_t>(ny) * z);
     }
+    // beyond the border of an open axis: the reference's guard cells there are
+    // never written nor exchanged, so fields read zero and current deposited
+    // there never reaches the core (FieldJ.x.cpp:156-174 exchanges with
+    // neighbours only)
+    bool outside(int x, int y, int z) const {
+        return (cfg.boundary[0] && (x < 0 || x >= nx)) || (cfg.boundary[1] && (y < 0 || y >= ny)) ||
+               (cfg.boundary[2] && (z < 0 || z >= nz));
+    }
+    T at(const std::vector<T>& f, int x, int y, int z) const { return outside(x, y, z) ? T(0) : f[idx(x, y, z)]; }
 };
 
 int supportOf(int shape) { return shape == 1 ? 2 : (shape == 2 ? 3 : 4); }
@@ -308,7 +321,7 @@ T gather1(const State<T>& st, const std::vector<T>& F, int supp, int gx, int gy,
         for (int y = 0; y < supp; ++y) {
             T rx = T(0.0);
             for (int x = 0; x < supp; ++x)
-                rx += F[st.idx(gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)] * sx[x];
+                rx += st.at(F, gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z) * sx[x];
             ry += rx * sy[y];
         }
         rz += ry * sz[z];
@@ -656,7 +669,7 @@ void flushTile(State<T>& st, const Tile& t, int ox, int oy, int oz) {
             for (int y = 0; y < t.ty; ++y)
                 for (int x = 0; x < t.tx; ++x) {
                     int64_t const a = t.acc[c * tv + (static_cast<size_t>(z) * t.ty + y) * t.tx + x];
-                    if (a == 0) continue;
+                    if (a == 0 || st.outside(ox - t.lo + x, oy - t.lo + y, oz - t.lo + z)) continue;
                     size_t const gi = st.idx(ox - t.lo + x, oy - t.lo + y, oz - t.lo + z);
                     st.J[c][gi] += static_cast<T>(static_cast<double>(a) * inv);
                 }
@@ -736,6 +749,9 @@ void push(State<T>& st, int species, bool fused) {
                         int const nz = lc0 / (SCX * SCY) + step[2] + lo;
                         T const pos[3] = {S.pos[0][g], S.pos[1][g], S.pos[2][g]};
                         T const charge = S.charge * S.w[g];
+                        // a particle that left through an open border is absorbed by the
+                        // boundary before the reference's deposit stage sees it
+                        if (st.outside(ox - lo + nx, oy - lo + ny, oz - lo + nz)) continue;
                         if (fixedPoint) {
                             esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
                                 int64_t const q = toFixed(v, jscale);
@@ -745,6 +761,7 @@ void push(State<T>& st, int species, bool fused) {
                         } else {
                             // FP64 twin: plain sums straight into J (each supercell of a phase is disjoint)
                             esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
+                                if (st.outside(ox - lo + nx + x, oy - lo + ny + y, oz - lo + nz + z)) return;
                                 st.J[c][st.idx(ox - lo + nx + x, oy - lo + ny + y, oz - lo + nz + z)] += v;
                             });
                         }
@@ -838,9 +855,11 @@ void deposit(State<T>& st, int species) {
                     for (int c = 0; c < 3; ++c)
                         for (int z = 0; z < tz; ++z)
                             for (int y = 0; y < ty; ++y)
-                                for (int x = 0; x < tx; ++x)
+                                for (int x = 0; x < tx; ++x) {
+                                    if (st.outside(ox - lo + x, oy - lo + y, oz - lo + z)) continue;
                                     st.J[c][st.idx(ox - lo + x, oy - lo + y, oz - lo + z)] +=
                                         cache[c * tv + (static_cast<size_t>(z) * ty + y) * tx + x];
+                                }
                 }
             }
         }
@@ -916,6 +935,11 @@ void shift(State<T>& st, int species) {
             for (int code = 1; code < 27; ++code) {
                 int const ex = code % 3, ey = (code / 3) % 3, ez = code / 9;
                 int const tx = ex == 2 ? -1 : ex, ty = ey == 2 ? -1 : ey, tz = ez == 2 ? -1 : ez;
+                // leavers crossing an open border are absorbed (boundary/Absorbing.hpp:63-92)
+                if ((st.cfg.boundary[0] && (dx - tx < 0 || dx - tx >= st.nscx)) ||
+                    (st.cfg.boundary[1] && (dy - ty < 0 || dy - ty >= st.nscy)) ||
+                    (st.cfg.boundary[2] && (dz - tz < 0 || dz - tz >= st.nscz)))
+                    continue;
                 int const sx = ((dx - tx) % st.nscx + st.nscx) % st.nscx;
                 int const sy = ((dy - ty) % st.nscy + st.nscy) % st.nscy;
                 int const sz = ((dz - tz) % st.nscz + st.nscz) % st.nscz;
@@ -949,7 +973,7 @@ void shift(State<T>& st, int species) {
 template <typename T>
 T fwd(const State<T>& st, const std::vector<T>& f, int d, int x, int y, int z) {
     int const ux = d == 0, uy = d == 1, uz = d == 2;
-    return (f[st.idx(x + ux, y + uy, z + uz)] - f[st.idx(x, y, z)]) / st.k.cell[d];
+    return (st.at(f, x + ux, y + uy, z + uz) - st.at(f, x, y, z)) / st.k.cell[d];
 }
 
 template <typename T>
@@ -967,7 +991,7 @@ T derivE(const State<T>& st, const std::vector<T>& f, int d, int x, int y, int z
         r = r + st.k.lehe_beta2 * fwd(st, f, d, x + u2[0], y + u2[1], z + u2[2]);
         r = r + st.k.lehe_beta2 * fwd(st, f, d, x - u2[0], y - u2[1], z - u2[2]);
         r = r + st.k.lehe_delta *
-                    (f[st.idx(x + 2 * u0[0], y + 2 * u0[1], z + 2 * u0[2])] - f[st.idx(x - u0[0], y - u0[1], z - u0[2])]) /
+                    (st.at(f, x + 2 * u0[0], y + 2 * u0[1], z + 2 * u0[2]) - st.at(f, x - u0[0], y - u0[1], z - u0[2])) /
                     st.k.cell[d];
         return r;
     }
@@ -983,7 +1007,7 @@ T derivE(const State<T>& st, const std::vector<T>& f, int d, int x, int y, int z
 template <typename T>
 T bwd(const State<T>& st, const std::vector<T>& f, int d, int x, int y, int z) {
     int const ux = d == 0, uy = d == 1, uz = d == 2;
-    return (f[st.idx(x, y, z)] - f[st.idx(x - ux, y - uy, z - uz)]) / st.k.cell[d];
+    return (st.at(f, x, y, z) - st.at(f, x - ux, y - uy, z - uz)) / st.k.cell[d];
 }
 
 template <typename T>
@@ -1029,6 +1053,47 @@ void addCurrent(State<T>& st) {
         for (int c = 0; c < 3; ++c) st.E[c][i] = st.E[c][i] + st.k.coeff * st.J[c][i];
 }
 
+// Exponential absorber (absorber/exponential/Exponential.hpp:52-95,
+// Exponential.kernel:63-112): for the plane exchanges LEFT, RIGHT, BOTTOM,
+// TOP, BACK, FRONT in this order, on open axes, the cell at distance d from
+// that border (d = 0 outermost) is multiplied by exp(-strength * (T-1-d)) for
+// T-1-d > 0. Thickness and strength are read at [axis][exchange % 2], so the
+// negative side uses entry 1 and the positive side entry 0, as the reference
+// indexes them.
+template <typename T>
+void absorb(State<T>& st, std::vector<T>* F) {
+    if (st.cfg.absorber != 1) return;
+    int const n[3] = {st.nx, st.ny, st.nz};
+    int const exchanges[6] = {1, 2, 3, 6, 9, 18};
+    for (int e : exchanges) {
+        int const axis = e >= 9 ? 2 : (e >= 3 ? 1 : 0);
+        bool const minSide = e == 1 || e == 3 || e == 9;
+        int const th = st.cfg.absorber_cells[axis][e % 2];
+        T const strength = static_cast<T>(st.cfg.absorber_strength[axis][e % 2]);
+        if (!st.cfg.boundary[axis] || th <= 1) continue;
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < st.nz; ++z)
+            for (int y = 0; y < st.ny; ++y)
+                for (int x = 0; x < st.nx; ++x) {
+                    int const c[3] = {x, y, z};
+                    int const dist = minSide ? c[axis] : n[axis] - 1 - c[axis];
+                    int const factor = th - 1 - dist;
+                    if (factor <= 0) continue;
+                    T const a = std::exp(-strength * static_cast<T>(factor));
+                    size_t const i = st.idx(x, y, z);
+                    for (int k = 0; k < 3; ++k) F[k][i] = F[k][i] * a;
+                }
+    }
+}
+
+// update_afterCurrent (FDTDBase.hpp:151-183): absorber on E, B half step, absorber on B
+template <typename T>
+void afterCurrent(State<T>& st) {
+    absorb(st, st.E);
+    bHalf(st);
+    absorb(st, st.B);
+}
+
 template <typename T>
 void step(State<T>& st, int n) {
     for (int it = 0; it < n; ++it) {
@@ -1040,7 +1105,7 @@ void step(State<T>& st, int n) {
         eFull(st);
         if (!fused) deposit(st, -1);  // CurrentDeposition
         addCurrent(st);  // CurrentInterpolationAndAdditionToEMF
-        bHalf(st);  // update_afterCurrent
+        afterCurrent(st);  // update_afterCurrent
     }
 }
 
@@ -1070,7 +1135,8 @@ void chargeDensity(const State<T>& st, double* rho) {
                     for (int oy = 0; oy <= S.supp; ++oy)
                         for (int ox = 0; ox <= S.supp; ++ox) {
                             double const v = q * wgt[0][ox] * wgt[1][oy] * wgt[2][oz];
-                            if (v != 0.0) rho[st.idx(c[0] + begin + ox, c[1] + begin + oy, c[2] + begin + oz)] += v;
+                            if (v != 0.0 && !st.outside(c[0] + begin + ox, c[1] + begin + oy, c[2] + begin + oz))
+                                rho[st.idx(c[0] + begin + ox, c[1] + begin + oy, c[2] + begin + oz)] += v;
                         }
             }
         }
@@ -1310,7 +1376,7 @@ void oracle_add_current(void* p) {
 }
 
 void oracle_fdtd_after_current(void* p) {
-    auto impl = [&](auto* st) { bHalf(*st); };
+    auto impl = [&](auto* st) { afterCurrent(*st); };
     dispatch<void>(static_cast<Handle*>(p), impl, impl);
 }
 
@@ -1351,8 +1417,8 @@ void oracle_div_e(void* p, double* div) {
                     double r = 0.0;
                     for (int d = 0; d < 3; ++d) {
                         int const ux = d == 0, uy = d == 1, uz = d == 2;
-                        r += (static_cast<double>(st->E[d][st->idx(x, y, z)]) -
-                              static_cast<double>(st->E[d][st->idx(x - ux, y - uy, z - uz)])) /
+                        r += (static_cast<double>(st->at(st->E[d], x, y, z)) -
+                              static_cast<double>(st->at(st->E[d], x - ux, y - uy, z - uz))) /
                              static_cast<double>(st->k.cell[d]);
                     }
                     div[st->idx(x, y, z)] = r;

@@ -18,6 +18,8 @@ MAX_SPECIES = 4
 
 SHAPE_CIC, SHAPE_TSC, SHAPE_PQS = 1, 2, 3
 PUSHER_NONE, PUSHER_BORIS, PUSHER_VAY, PUSHER_HIGUERA_CARY = 0, 1, 2, 3
+BOUNDARY_PERIODIC, BOUNDARY_OPEN = 0, 1
+ABSORBER_NONE, ABSORBER_EXPONENTIAL = 0, 1
 SOLVER_YEE, SOLVER_LEHE = 0, 1
 FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
 TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2
@@ -49,6 +51,10 @@ class Config(C.Structure):
         ("species", SpeciesDesc * MAX_SPECIES),
         ("j_scale_exp", C.c_int32),
         ("device", C.c_int32),
+        ("boundary", C.c_int32 * 3),
+        ("absorber", C.c_int32),
+        ("absorber_cells", (C.c_int32 * 2) * 3),
+        ("absorber_strength", (C.c_float * 2) * 3),
     ]
 
 
@@ -205,8 +211,11 @@ def ptr(a: np.ndarray, ctype):
 
 
 def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YEE, lehe_dir=1, j_scale_exp=0,
-                device=0) -> Config:
-    """species: list of dicts with mass_ratio, charge_ratio, shape, pusher, frames_per_sc."""
+                device=0, boundary=(0, 0, 0), absorber=0, absorber_cells=((0, 0), (0, 0), (0, 0)),
+                absorber_strength=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0))) -> Config:
+    """species: list of dicts with mass_ratio, charge_ratio, shape, pusher, frames_per_sc;
+    boundary: BOUNDARY_* per axis; absorber: ABSORBER_*, with thickness and
+    strength per [axis][negative, positive] side as in fieldAbsorber.param."""
     cfg = Config()
     for d in range(3):
         cfg.grid[d] = int(grid[d])
@@ -225,6 +234,12 @@ def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YE
         cfg.species[i].frames_per_sc = int(s["frames_per_sc"])
     cfg.j_scale_exp = int(j_scale_exp)
     cfg.device = int(device)
+    for d in range(3):
+        cfg.boundary[d] = int(boundary[d])
+        for side in range(2):
+            cfg.absorber_cells[d][side] = int(absorber_cells[d][side])
+            cfg.absorber_strength[d][side] = float(absorber_strength[d][side])
+    cfg.absorber = int(absorber)
     return cfg
 
 

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void kChargeDensity(SpeciesDev S, GridDev g, d
                 yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
                 for (int ox = 0; ox <= SUPP; ++ox) {
                     double const v = q * wgt[0][ox] * wgt[1][oy] * wgt[2][oz];
-                    if (v == 0.0) continue;
+                    if (v == 0.0 || outsideOpen(g, c[0] + begin + ox, c[1] + begin + oy, c[2] + begin + oz)) continue;
                     int xx = c[0] + begin + ox;
                     xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
                     unsafeAtomicAdd(rho + (size_t)xx + (size_t)g.nx * ((size_t)yy + (size_t)g.ny * zz), v);
@@ -63,9 +63,13 @@ __global__ __launch_bounds__(256) void kGaussResidual(FieldArgs a, double eps0, 
     size_t const ix = (size_t)xm + (size_t)a.g.nx * ((size_t)y + (size_t)a.g.ny * z);
     size_t const iy = (size_t)x + (size_t)a.g.nx * ((size_t)ym + (size_t)a.g.ny * z);
     size_t const iz = (size_t)x + (size_t)a.g.nx * ((size_t)y + (size_t)a.g.ny * zm);
-    double div = ((double)a.E[0][i] - (double)a.E[0][ix]) / (double)a.cell[0];
-    div += ((double)a.E[1][i] - (double)a.E[1][iy]) / (double)a.cell[1];
-    div += ((double)a.E[2][i] - (double)a.E[2][iz]) / (double)a.cell[2];
+    // beyond an open border E reads zero
+    double const ex = (a.g.open & 1) && x == 0 ? 0.0 : (double)a.E[0][ix];
+    double const ey = (a.g.open & 2) && y == 0 ? 0.0 : (double)a.E[1][iy];
+    double const ez = (a.g.open & 4) && z == 0 ? 0.0 : (double)a.E[2][iz];
+    double div = ((double)a.E[0][i] - ex) / (double)a.cell[0];
+    div += ((double)a.E[1][i] - ey) / (double)a.cell[1];
+    div += ((double)a.E[2][i] - ez) / (double)a.cell[2];
     inout[i] = eps0 * div - inout[i];
 }
 

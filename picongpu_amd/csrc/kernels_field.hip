@@ -7,20 +7,26 @@
 // BackwardDerivative.hpp:44-65) or the Lehe curl of E
 // (fields/MaxwellSolver/Lehe/Derivative.hpp:65-228), and AddCurrentDensity
 // with currentInterpolation::None (E += coeff * J, None.hpp:56-65).
-// Fields are SoA planes, x fastest; neighbours wrap periodically.
+// Fields are SoA planes, x fastest; neighbours wrap periodically, except
+// beyond the border of an open axis, where every field reads zero (the
+// reference's guard cells of a non-periodic border are never written).
 #include "pic_kernels.hpp"
 
 namespace pic {
 
 struct Idx {
-    int nx, ny, nz;
+    GridDev g;
     int x, y, z;
     __device__ __forceinline__ size_t at(int dx, int dy, int dz) const {
         int xx = x + dx, yy = y + dy, zz = z + dz;
-        xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
-        yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
-        zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
-        return (size_t)xx + (size_t)nx * ((size_t)yy + (size_t)ny * zz);
+        xx = xx < 0 ? xx + g.nx : (xx >= g.nx ? xx - g.nx : xx);
+        yy = yy < 0 ? yy + g.ny : (yy >= g.ny ? yy - g.ny : yy);
+        zz = zz < 0 ? zz + g.nz : (zz >= g.nz ? zz - g.nz : zz);
+        return (size_t)xx + (size_t)g.nx * ((size_t)yy + (size_t)g.ny * zz);
+    }
+    // value of f at the offset cell; zero beyond an open border
+    __device__ __forceinline__ float get(const float* f, int dx, int dy, int dz) const {
+        return outsideOpen(g, x + dx, y + dy, z + dz) ? 0.0f : f[at(dx, dy, dz)];
     }
 };
 
@@ -34,7 +40,7 @@ __device__ __forceinline__ void unit(int d, int& ux, int& uy, int& uz) {
 __device__ __forceinline__ float fwd(const float* f, const Idx& I, int d, int ox, int oy, int oz, float step) {
     int ux, uy, uz;
     unit(d, ux, uy, uz);
-    return (f[I.at(ox + ux, oy + uy, oz + uz)] - f[I.at(ox, oy, oz)]) / step;
+    return (I.get(f, ox + ux, oy + uy, oz + uz) - I.get(f, ox, oy, oz)) / step;
 }
 
 // Lehe derivative along axis d of component f (Lehe/Derivative.hpp:132-145 for
@@ -53,7 +59,7 @@ __device__ __forceinline__ float leheDeriv(const float* f, const Idx& I, int d, 
         r = r + a.lehe_beta1 * fwd(f, I, d, -u1x, -u1y, -u1z, step);
         r = r + a.lehe_beta2 * fwd(f, I, d, u2x, u2y, u2z, step);
         r = r + a.lehe_beta2 * fwd(f, I, d, -u2x, -u2y, -u2z, step);
-        r = r + a.lehe_delta * (f[I.at(2 * u0x, 2 * u0y, 2 * u0z)] - f[I.at(-u0x, -u0y, -u0z)]) / step;
+        r = r + a.lehe_delta * (I.get(f, 2 * u0x, 2 * u0y, 2 * u0z) - I.get(f, -u0x, -u0y, -u0z)) / step;
         return r;
     }
     int ulx, uly, ulz;
@@ -77,7 +83,7 @@ __global__ __launch_bounds__(256) void kBHalf(FieldArgs a) {
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Idx I{a.g.nx, a.g.ny, a.g.nz, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    Idx I{a.g, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
     // curl = (dFdy.z - dFdz.y, dFdz.x - dFdx.z, dFdx.y - dFdy.x)
     float const cx = derivE<SOLVER>(a.E[2], I, 1, a) - derivE<SOLVER>(a.E[1], I, 2, a);
     float const cy = derivE<SOLVER>(a.E[0], I, 2, a) - derivE<SOLVER>(a.E[2], I, 0, a);
@@ -90,14 +96,14 @@ __global__ __launch_bounds__(256) void kBHalf(FieldArgs a) {
 __device__ __forceinline__ float bwd(const float* f, const Idx& I, int d, float step) {
     int ux, uy, uz;
     unit(d, ux, uy, uz);
-    return (f[I.at(0, 0, 0)] - f[I.at(-ux, -uy, -uz)]) / step;
+    return (I.get(f, 0, 0, 0) - I.get(f, -ux, -uy, -uz)) / step;
 }
 
 __global__ __launch_bounds__(256) void kEFull(FieldArgs a) {
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Idx I{a.g.nx, a.g.ny, a.g.nz, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    Idx I{a.g, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
     float const cx = bwd(a.B[2], I, 1, a.cell[1]) - bwd(a.B[1], I, 2, a.cell[2]);
     float const cy = bwd(a.B[0], I, 2, a.cell[2]) - bwd(a.B[2], I, 0, a.cell[0]);
     float const cz = bwd(a.B[1], I, 0, a.cell[0]) - bwd(a.B[0], I, 1, a.cell[1]);
@@ -204,7 +210,8 @@ __device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* co
     for (int t = threadIdx.x; t < 3 * kEV; t += blockDim.x) {
         int const c = t / kEV, r = t - c * kEV;
         int const ex = r % kEX, ey = (r / kEX) % kEY, ez = r / (kEX * kEY);
-        L.E[c][r] = a.E[c][wrapAt(a.g, x0, y0, z0, ex - kEH, ey - kEH, ez - kEH)];
+        bool const out = outsideOpen(a.g, x0 + ex - kEH, y0 + ey - kEH, z0 + ez - kEH);
+        L.E[c][r] = out ? 0.0f : a.E[c][wrapAt(a.g, x0, y0, z0, ex - kEH, ey - kEH, ez - kEH)];
     }
     __syncthreads();
     using SE = LdsStencil<kEX, kEX * kEY>;
@@ -213,6 +220,10 @@ __device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* co
         // B' at tile offset (hx-1, hy-1, hz-1)
         int const he = (hx - 1 + kEH) + kEX * ((hy - 1 + kEH) + kEY * (hz - 1 + kEH));
         size_t const i = wrapAt(a.g, x0, y0, z0, hx - 1, hy - 1, hz - 1);
+        if (outsideOpen(a.g, x0 + hx - 1, y0 + hy - 1, z0 + hz - 1)) {  // B beyond an open border stays zero
+            L.B[0][t] = L.B[1][t] = L.B[2][t] = 0.0f;
+            continue;
+        }
         float cx, cy, cz;
         SE::template curl<SOLVER>(L.E[0], L.E[1], L.E[2], he, a, cx, cy, cz);
         L.B[0][t] = a.B[0][i] - cx * 0.5f * a.dt;
@@ -289,39 +300,50 @@ __device__ __forceinline__ void yeeBeforeAddTile(const FieldArgs& a, float* cons
     ZTile const T = zTile(a.g, b, zc);
     int const nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
     size_t const plane = (size_t)nx * ny;
-    // per thread, once per tile: the wrapped xy offsets of the plane elements it stages and updates
+    // per thread, once per tile: the wrapped xy offsets of the plane elements
+    // it stages and updates, -1 beyond an open border (the value there is zero)
     constexpr int NE = (3 * kZEX * kZEY + 255) / 256;
     constexpr int NB = (kZBX * kZBY + 255) / 256;
     int eoff[NE];
     for (int j = 0; j < NE; ++j) {
         int const t = threadIdx.x + 256 * j;
         int const r = t % (kZEX * kZEY);
-        eoff[j] = wrap1(T.x0 + r % kZEX - 1, nx) + nx * wrap1(T.y0 + r / kZEX - 1, ny);
+        int const x = T.x0 + r % kZEX - 1, y = T.y0 + r / kZEX - 1;
+        eoff[j] = outsideOpen(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
     }
     int boff[NB];
     for (int j = 0; j < NB; ++j) {
         int const t = threadIdx.x + 256 * j;
-        boff[j] = wrap1(T.x0 + t % kZBX - 1, nx) + nx * wrap1(T.y0 + t / kZBX - 1, ny);
+        int const x = T.x0 + t % kZBX - 1, y = T.y0 + t / kZBX - 1;
+        boff[j] = outsideOpen(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
     }
+    auto planeOut = [&](int k) { return (a.g.open & 4) && (k < 0 || k >= nz); };
     float* const Ebase = &L.E[0][0][0][0];
     auto loadE = [&](int k) {  // E plane k (wrapped) into slot k & 1
         size_t const zoff = plane * (size_t)wrap1(k, nz);
+        bool const zout = planeOut(k);
         float* const dst = Ebase + (k & 1) * (3 * kZEX * kZEY);
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             int const t = threadIdx.x + 256 * j;
-            if (t < 3 * kZEX * kZEY) dst[t] = a.E[t / (kZEX * kZEY)][zoff + (size_t)eoff[j]];
+            if (t < 3 * kZEX * kZEY)
+                dst[t] = (zout || eoff[j] < 0) ? 0.0f : a.E[t / (kZEX * kZEY)][zoff + (size_t)eoff[j]];
         }
     };
     // B' = B - curlE(E)*0.5*dt at plane k on x, y in [-1, T): forward differences
     auto bPrime = [&](int k) {
         size_t const zoff = plane * (size_t)wrap1(k, nz);
+        bool const zout = planeOut(k);
         int const s = k & 1, s1 = (k + 1) & 1;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             int const t = threadIdx.x + 256 * j;
             if (t >= kZBX * kZBY) continue;
             int const bx = t % kZBX, by = t / kZBX;  // cell (bx-1, by-1) of the tile
+            if (zout || boff[j] < 0) {  // B beyond an open border stays zero
+                L.B[s][0][by][bx] = L.B[s][1][by][bx] = L.B[s][2][by][bx] = 0.0f;
+                continue;
+            }
             int const ex = bx, ey = by;              // its E-plane index (E plane starts at -1)
             float const e0 = L.E[s][0][ey][ex], e1 = L.E[s][1][ey][ex], e2 = L.E[s][2][ey][ex];
             float const dEzdy = (L.E[s][2][ey + 1][ex] - e2) / a.cell[1];
@@ -389,21 +411,24 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
     for (int j = 0; j < NE; ++j) {
         int const t = threadIdx.x + 256 * j;
         int const r = t % (PX * PY);
-        eoff[j] = wrap1(T.x0 + r % PX, nx) + nx * wrap1(T.y0 + r / PX, ny);
+        int const x = T.x0 + r % PX, y = T.y0 + r / PX;
+        eoff[j] = outsideOpen(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
     }
     float* const Eb = &L.E[0][0][0][0];
     float* const Nb = &L.En[0][0][0][0];
     auto load = [&](int k) {  // planes k of E and Enew on x, y in [0, T+1)
         size_t const zoff = plane * (size_t)wrap1(k, nz);
+        bool const zout = (a.g.open & 4) && (k < 0 || k >= nz);
         int const so = (k & 1) * (3 * PX * PY);
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             int const t = threadIdx.x + 256 * j;
             if (t < 3 * PX * PY) {
                 int const c = t / (PX * PY);
-                size_t const gi = zoff + (size_t)eoff[j];
-                Eb[so + t] = a.E[c][gi];
-                Nb[so + t] = Enew[c][gi];
+                bool const out = zout || eoff[j] < 0;
+                size_t const gi = zoff + (size_t)(out ? 0 : eoff[j]);
+                Eb[so + t] = out ? 0.0f : a.E[c][gi];
+                Nb[so + t] = out ? 0.0f : Enew[c][gi];
             }
         }
     };
@@ -453,8 +478,9 @@ __device__ __forceinline__ void fieldAfterTile(const FieldArgs& a, float* const*
         int const c = t / kAV, r = t - c * kAV;
         int const ex = r % kAX, ey = (r / kAX) % kAY, ez = r / (kAX * kAY);
         size_t const gi = wrapAt(a.g, x0, y0, z0, ex - 1, ey - 1, ez - 1);
-        L.E[c][r] = a.E[c][gi];
-        L.En[c][r] = Enew[c][gi];
+        bool const out = outsideOpen(a.g, x0 + ex - 1, y0 + ey - 1, z0 + ez - 1);
+        L.E[c][r] = out ? 0.0f : a.E[c][gi];
+        L.En[c][r] = out ? 0.0f : Enew[c][gi];
     }
     __syncthreads();
     using SA = LdsStencil<kAX, kAX * kAY>;
@@ -617,27 +643,68 @@ hipError_t launchEFull(const FieldArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Exponential absorber, one layer (absorber/exponential/Exponential.kernel:63-112):
+// the cell at distance d from the border of `axis` (minSide: the low border)
+// is multiplied by table[depth - d] for d < depth, each component. Threads
+// walk the slab x fastest, so rows along x stay coalesced.
+__global__ __launch_bounds__(256) void kAbsorb(float* f0, float* f1, float* f2, GridDev g, int axis, int minSide,
+                                               int depth, const float* __restrict__ table) {
+    int const sx = axis == 0 ? depth : g.nx, sy = axis == 1 ? depth : g.ny, sz = axis == 2 ? depth : g.nz;
+    size_t const t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)sx * sy * sz) return;
+    int c[3] = {(int)(t % sx), (int)((t / sx) % sy), (int)(t / ((size_t)sx * sy))};
+    int const n[3] = {g.nx, g.ny, g.nz};
+    int const d = c[axis];
+    if (!minSide) c[axis] = n[axis] - 1 - d;
+    float const a = table[depth - d];
+    size_t const i = (size_t)c[0] + (size_t)g.nx * ((size_t)c[1] + (size_t)g.ny * c[2]);
+    f0[i] = f0[i] * a;
+    f1[i] = f1[i] * a;
+    f2[i] = f2[i] * a;
+}
+
+hipError_t launchAbsorb(const GridDev& g, float* const F[3], const AbsorbPlan& p, hipStream_t s) {
+    for (int l = 0; l < p.n; ++l) {
+        const AbsorbLayer& L = p.layer[l];
+        size_t const other = L.axis == 0 ? (size_t)g.ny * g.nz : (L.axis == 1 ? (size_t)g.nx * g.nz : (size_t)g.nx * g.ny);
+        size_t const cells = other * (size_t)L.depth;
+        if (cells == 0) continue;
+        hipLaunchKernelGGL(kAbsorb, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, F[0], F[1], F[2], g, L.axis,
+                           L.minSide, L.depth, L.table);
+    }
+    return hipGetLastError();
+}
+
 int yeeChunk(const GridDev& g) {
     int zc = kZC;
     while (zc > kZCMin && zTileCount(g, zc) < kZFill) zc /= 2;
     return zTileCount(g, zc) >= kZFill / 2 ? zc : 0;
 }
 
-hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], hipStream_t s) {
+hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], const AbsorbPlan* ab, hipStream_t s) {
+    // update_afterCurrent damps E before its B half step and B after it
+    // (FDTDBase.hpp:151-183): between the two kernels and after the second
+    bool const absorb = ab && ab->n > 0;
     int const tiles = (a.g.nx / kFTX) * (a.g.ny / kFTY) * (a.g.nz / kFTZ);
     int const zc = solver == PIC_SOLVER_YEE ? yeeChunk(a.g) : 0;
-    if (solver == PIC_SOLVER_LEHE) {
+    int const zt = zc > 0 ? zTileCount(a.g, zc) : 0;
+    if (solver == PIC_SOLVER_LEHE)
         hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
-        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
-    } else if (zc == 0) {
-        // grids too small for the z-marching form to fill the chip: 3D tiles of 8x8x4
+    else if (zc == 0)  // grids too small for the z-marching form to fill the chip: 3D tiles of 8x8x4
         hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
-        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
-    } else {
-        int const zt = zTileCount(a.g, zc);
+    else
         hipLaunchKernelGGL(kYeeBeforeAdd, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
-        hipLaunchKernelGGL(kYeeAfter, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
+    if (absorb) {
+        hipError_t const e = launchAbsorb(a.g, Enew, *ab, s);
+        if (e != hipSuccess) return e;
     }
+    if (solver == PIC_SOLVER_LEHE)
+        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+    else if (zc == 0)
+        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+    else
+        hipLaunchKernelGGL(kYeeAfter, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
+    if (absorb) return launchAbsorb(a.g, a.B, *ab, s);
     return hipGetLastError();
 }
 

@@ -156,7 +156,8 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             int const gz = wrapIdx(oz - GLO + tz, a.g.nz);
             size_t const gi = (size_t)gx + (size_t)a.g.nx * ((size_t)gy + (size_t)a.g.ny * gz);
             const float* src = c < 3 ? a.E[c] : a.B[c - 3];
-            sEB[t] = src[gi];
+            // fields beyond an open border read zero
+            sEB[t] = outsideOpen(a.g, ox - GLO + tx, oy - GLO + ty, oz - GLO + tz) ? 0.0f : src[gi];
         }
     }
     if (MODE != 0) {
@@ -228,7 +229,9 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                     if ((dir[0] | dir[1] | dir[2]) != 0) S.cell[gidx] = (uint16_t)newIdx;
                     if (newMask >= 2) S.mask[gidx] = (uint8_t)newMask;
                     leaveBits |= (newMask >= 2 ? 1u : 0u) << (i / kPushThreads);
-                    if (MODE == 1) {
+                    // a particle leaving through an open border is absorbed by the
+                    // boundary before the reference's deposit stage sees it
+                    if (MODE == 1 && !outsideOpen(a.g, ox + lx + dir[0], oy + ly + dir[1], oz + lz + dir[2])) {
                         int const cx = lx + dir[0] + DLO, cy = ly + dir[1] + DLO, cz = lz + dir[2] + DLO;
                         Vec3 const np{p[0], p[1], p[2]};
                         auto const G = Esirkepov<SUPP>::geometry(np, vel, dk);
@@ -266,6 +269,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             int const tx = r % JX;
             int const ty = (r / JX) % JY;
             int const tz = r / (JX * JY);
+            if (outsideOpen(a.g, ox - DLO + tx, oy - DLO + ty, oz - DLO + tz)) continue;  // dropped beyond the border
             int const gx = wrapIdx(ox - DLO + tx, a.g.nx);
             int const gy = wrapIdx(oy - DLO + ty, a.g.ny);
             int const gz = wrapIdx(oz - DLO + tz, a.g.nz);

@@ -82,13 +82,16 @@ __device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, 
         int const ex = code % 3, ey = (code / 3) % 3, ez = code / 9;
         int const stx = ex == 2 ? -1 : ex, sty = ey == 2 ? -1 : ey, stz = ez == 2 ? -1 : ez;
         int sx = dx - stx, sy = dy - sty, sz = dz - stz;
+        // leavers crossing an open border are absorbed (boundary/Absorbing.hpp:63-92)
+        bool const absorbed = ((g.open & 1) && (sx < 0 || sx >= g.nscx)) || ((g.open & 2) && (sy < 0 || sy >= g.nscy)) ||
+                              ((g.open & 4) && (sz < 0 || sz >= g.nscz));
         sx = sx < 0 ? sx + g.nscx : (sx >= g.nscx ? sx - g.nscx : sx);
         sy = sy < 0 ? sy + g.nscy : (sy >= g.nscy ? sy - g.nscy : sy);
         sz = sz < 0 ? sz + g.nscz : (sz >= g.nscz ? sz - g.nscz : sz);
         src = sx + g.nscx * (sy + g.nscy * sz);
         const uint32_t* off = S.outOff + (size_t)src * kOffStride;
         beg = off[code];
-        cnt = off[code + 1] - beg;
+        cnt = absorbed ? 0u : off[code + 1] - beg;
     }
     uint32_t const incl = waveInclusiveScan(cnt);
     uint32_t const start = incl - cnt;  // exclusive prefix: first incoming index of this code

@@ -65,7 +65,27 @@ int validateConfig(const pic_config& c) {
             return fail(PIC_E_INVALID, "frames_per_sc must be in [1,255]");
         if (!(d.mass_ratio > 0.0)) return fail(PIC_E_INVALID, "mass_ratio must be positive");
     }
+    for (int d = 0; d < 3; ++d) {
+        if (c.boundary[d] != PIC_BOUNDARY_PERIODIC && c.boundary[d] != PIC_BOUNDARY_OPEN)
+            return fail(PIC_E_INVALID, "boundary must be PIC_BOUNDARY_PERIODIC or PIC_BOUNDARY_OPEN");
+        for (int side = 0; side < 2; ++side) {
+            if (c.absorber_cells[d][side] < 0 || c.absorber_cells[d][side] > PIC_ABSORBER_MAX_CELLS)
+                return fail(PIC_E_INVALID, "absorber_cells must be in [0, PIC_ABSORBER_MAX_CELLS]");
+            if (!(c.absorber_strength[d][side] >= 0.0f) || !std::isfinite(c.absorber_strength[d][side]))
+                return fail(PIC_E_INVALID, "absorber_strength must be finite and >= 0");
+        }
+        if (c.boundary[d] == PIC_BOUNDARY_OPEN && c.absorber == PIC_ABSORBER_EXPONENTIAL &&
+            c.absorber_cells[d][0] + c.absorber_cells[d][1] > c.grid[d])
+            return fail(PIC_E_INVALID, "absorber layers are thicker than the grid");
+    }
+    if (c.absorber != PIC_ABSORBER_NONE && c.absorber != PIC_ABSORBER_EXPONENTIAL)
+        return fail(PIC_E_INVALID, "absorber must be PIC_ABSORBER_NONE or PIC_ABSORBER_EXPONENTIAL");
     return PIC_OK;
+}
+
+bool allPeriodic(const pic_config& c) {
+    return c.boundary[0] == PIC_BOUNDARY_PERIODIC && c.boundary[1] == PIC_BOUNDARY_PERIODIC &&
+           c.boundary[2] == PIC_BOUNDARY_PERIODIC;
 }
 
 void computeConstants(const pic_config& c, pic_constants& k) {
@@ -226,6 +246,7 @@ struct pic_ctx {
     float* E[3] = {nullptr, nullptr, nullptr};
     float* Ealt[3] = {nullptr, nullptr, nullptr};  // receives the new E of the fused field step
     int eParity = 0;                               // flips with every swap of E and Ealt
+    pic::AbsorbPlan absorb{};                      // exponential absorber layers, exchange order
     float* B[3] = {nullptr, nullptr, nullptr};
     float* J[3] = {nullptr, nullptr, nullptr};
     std::vector<SpeciesMem> sp;
@@ -535,6 +556,8 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
     c->g.nscx = cfg->grid[0] / pic::kSCX;
     c->g.nscy = cfg->grid[1] / pic::kSCY;
     c->g.nscz = cfg->grid[2] / pic::kSCZ;
+    for (int d = 0; d < 3; ++d)
+        if (cfg->boundary[d] == PIC_BOUNDARY_OPEN) c->g.open |= 1 << d;
     c->nsc = c->g.nscx * c->g.nscy * c->g.nscz;
     c->ncells = static_cast<size_t>(cfg->grid[0]) * cfg->grid[1] * cfg->grid[2];
     c->device = cfg->device;
@@ -552,6 +575,32 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
             return bail(PIC_E_HIP);
     }
     if (devAlloc(c, &c->errFlags, 1)) return bail(PIC_E_HIP);
+    if (cfg->absorber == PIC_ABSORBER_EXPONENTIAL) {
+        // ExponentialImpl::run (Exponential.hpp:52-95): plane exchanges LEFT,
+        // RIGHT, BOTTOM, TOP, BACK, FRONT in this order, on borders without a
+        // neighbour (open axes), thickness and strength read at [axis][exchange % 2];
+        // the factors exp(-strength * f) of Exponential.kernel:99 in float
+        constexpr int kExchanges[6] = {1, 2, 3, 6, 9, 18};
+        constexpr int kStride = PIC_ABSORBER_MAX_CELLS + 1;
+        std::vector<float> table(6 * kStride, 0.0f);
+        float* dTable = nullptr;
+        if (devAlloc(c, &dTable, table.size())) return bail(PIC_E_HIP);
+        for (int l = 0; l < 6; ++l) {
+            int const ex = kExchanges[l];
+            int const axis = ex >= 9 ? 2 : (ex >= 3 ? 1 : 0);
+            int const th = cfg->absorber_cells[axis][ex % 2];
+            float const strength = cfg->absorber_strength[axis][ex % 2];
+            if (cfg->boundary[axis] != PIC_BOUNDARY_OPEN || th <= 1) continue;
+            for (int f = 1; f < th; ++f) table[l * kStride + f] = std::exp(-strength * static_cast<float>(f));
+            pic::AbsorbLayer& L = c->absorb.layer[c->absorb.n++];
+            L.axis = axis;
+            L.minSide = (ex == 1 || ex == 3 || ex == 9) ? 1 : 0;
+            L.depth = th - 1;
+            L.table = dTable + l * kStride;
+        }
+        e = hipMemcpyAsync(dTable, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(e)));
+    }
     c->energyBlocks = 1024;
     if (devAlloc(c, &c->energyPartials, 2 * c->energyBlocks)) return bail(PIC_E_HIP);
     if (devAlloc(c, &c->countOut, 1)) return bail(PIC_E_HIP);
@@ -946,7 +995,10 @@ int pic_fdtd_after_current(pic_ctx* c) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     pic::FieldArgs a = makeFieldArgs(c);
     TimerScope ts(c, PIC_TIMER_FIELD);
+    // FDTDBase.hpp:151-183: absorber on E, B half step, absorber on B
+    if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->E, c->absorb, c->stream));
     PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
+    if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->B, c->absorb, c->stream));
     return PIC_OK;
 }
 
@@ -960,7 +1012,7 @@ int fieldStep(pic_ctx* c) {
     pic::FieldArgs a = makeFieldArgs(c);
     {
         TimerScope ts(c, PIC_TIMER_FIELD);
-        PIC_HIP(pic::launchFieldStep(c->cfg.solver, a, c->Ealt, c->stream));
+        PIC_HIP(pic::launchFieldStep(c->cfg.solver, a, c->Ealt, &c->absorb, c->stream));
     }
     for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
     c->eParity ^= 1;
@@ -1249,6 +1301,8 @@ int pic_adjoint_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* 
     if (r != PIC_OK) return r;
     if (n <= 0 || !cell || !pos || !mom || !w) return fail(PIC_E_INVALID, "particle arrays and n > 0 are required");
     if (c->cfg.solver != PIC_SOLVER_YEE) return fail(PIC_E_INVALID, "the differentiable form uses the Yee solver");
+    if (!allPeriodic(c->cfg) || c->cfg.absorber != PIC_ABSORBER_NONE)
+        return fail(PIC_E_INVALID, "the differentiable form needs a periodic box without absorber");
     if (c->ncells > (size_t)INT32_MAX) return fail(PIC_E_INVALID, "the differentiable form needs fewer than 2^31 cells");
     const SpeciesMem& sm = c->sp[s];
     if (sm.desc.shape != PIC_SHAPE_CIC || sm.desc.pusher != PIC_PUSHER_BORIS)
@@ -1709,6 +1763,8 @@ int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out) {
     for (int32_t i = 0; i < n; ++i) {
         const pic_config& c = cfgs[i];
         if (c.device != e->device) return bail(fail(PIC_E_INVALID, "ensemble members must share one device"));
+        if (!allPeriodic(c) || c.absorber != PIC_ABSORBER_NONE)
+            return bail(fail(PIC_E_INVALID, "ensemble members need periodic boxes without absorber"));
         int supp = 0, pusher = 0;
         for (int s = 0; s < c.n_species; ++s) {
             if (c.species[s].pusher == PIC_PUSHER_NONE) continue;

@@ -34,7 +34,14 @@ struct SpeciesDev {
 struct GridDev {
     int nx, ny, nz;
     int nscx, nscy, nscz;
+    int open;  // bit d: axis d has open borders (fields read zero beyond, particles leaving are absorbed)
 };
+
+// (x, y, z) lies beyond the border of an open axis
+__host__ __device__ __forceinline__ bool outsideOpen(const GridDev& g, int x, int y, int z) {
+    return ((g.open & 1) && (x < 0 || x >= g.nx)) || ((g.open & 2) && (y < 0 || y >= g.ny)) ||
+           ((g.open & 4) && (z < 0 || z >= g.nz));
+}
 
 struct PushArgs {
     GridDev g;
@@ -82,7 +89,18 @@ hipError_t launchAddCurrent(const FieldArgs& a, hipStream_t s);
 // fused update_beforeCurrent + addCurrent + update_afterCurrent; the new E is
 // written to Enew (the caller swaps E and Enew afterwards); the grid must be a
 // multiple of 8x8x4 cells
-hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], hipStream_t s);
+// layers of the exponential absorber, in the reference's exchange order
+struct AbsorbLayer {
+    int axis, minSide;
+    int depth;            // damped cells: distance 0 .. depth-1 from the border (thickness - 1)
+    const float* table;   // table[f] = exp(-strength * f), f = 1 .. depth
+};
+struct AbsorbPlan {
+    int n;
+    AbsorbLayer layer[6];
+};
+hipError_t launchAbsorb(const GridDev& g, float* const F[3], const AbsorbPlan& p, hipStream_t s);
+hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], const AbsorbPlan* ab, hipStream_t s);
 // tiles of the fused field step of one grid (work items of the batched form)
 int fieldStepTiles(int solver, const GridDev& g);
 hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, hipStream_t s);

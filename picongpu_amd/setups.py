@@ -56,12 +56,18 @@ class Setup:
     b_noise: float = 0.0
     b_noise_seed: int = 1
     laser: dict = None  # keyword arguments of Simulation.add_laser
+    boundary: tuple = (0, 0, 0)  # abi.BOUNDARY_* per axis
+    absorber: int = 0  # abi.ABSORBER_*
+    absorber_cells: tuple = ((0, 0), (0, 0), (0, 0))  # [axis][negative, positive]
+    absorber_strength: tuple = ((0.0, 0.0), (0.0, 0.0), (0.0, 0.0))
 
     def config(self, device: int = 0, j_scale_exp: int = 0) -> abi.Config:
         sp = [dict(mass_ratio=s.mass_ratio, charge_ratio=s.charge_ratio, shape=s.shape, pusher=s.pusher,
                    frames_per_sc=s.frames()) for s in self.species]
         return abi.make_config(self.grid, self.cell_si, self.dt_si, self.density_si, self.typical_ppc, sp,
-                               solver=self.solver, lehe_dir=self.lehe_dir, j_scale_exp=j_scale_exp, device=device)
+                               solver=self.solver, lehe_dir=self.lehe_dir, j_scale_exp=j_scale_exp, device=device,
+                               boundary=self.boundary, absorber=self.absorber, absorber_cells=self.absorber_cells,
+                               absorber_strength=self.absorber_strength)
 
     @property
     def ncells(self) -> int:

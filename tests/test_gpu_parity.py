@@ -20,6 +20,15 @@ def small_thermal(shape=abi.SHAPE_CIC, pusher=abi.PUSHER_BORIS, grid=(16, 16, 12
     return st
 
 
+def open_box(st, boundary=(1, 0, 1), cells=((4, 6), (3, 5), (2, 4)), strength=((0.05, 0.1), (0.2, 0.03), (0.1, 0.1))):
+    """Open borders on the given axes with the exponential absorber."""
+    st.boundary = boundary
+    st.absorber = abi.ABSORBER_EXPONENTIAL
+    st.absorber_cells = cells
+    st.absorber_strength = strength
+    return st
+
+
 def small_two_species(shape=abi.SHAPE_TSC, grid=(16, 16, 12), ppc=3):
     st = setups.kelvin_helmholtz(grid=grid, ppc=ppc, shape=shape)
     for s in st.species:
@@ -65,9 +74,12 @@ def test_deposit_unfused_bit_exact(name, make):
     sim.close()
 
 
-def test_shift_bit_exact():
+@pytest.mark.parametrize("open_", [False, True], ids=["periodic", "open"])
+def test_shift_bit_exact(open_):
     st = small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS, u=0.3)
-    print(f"case shift grid={st.grid}", flush=True)
+    if open_:
+        open_box(st, boundary=(1, 1, 0))
+    print(f"case shift grid={st.grid} open={open_}", flush=True)
     sim, ora, _ = H.make_pair(st)
     for it in range(3):
         sim.push(-1, False)
@@ -79,9 +91,12 @@ def test_shift_bit_exact():
     sim.close()
 
 
-@pytest.mark.parametrize("solver", [abi.SOLVER_YEE, abi.SOLVER_LEHE], ids=["yee", "lehe"])
-def test_fields_bit_exact(solver):
+@pytest.mark.parametrize("solver,open_", [(abi.SOLVER_YEE, False), (abi.SOLVER_LEHE, False), (abi.SOLVER_YEE, True),
+                                          (abi.SOLVER_LEHE, True)], ids=["yee", "lehe", "yee-open", "lehe-open"])
+def test_fields_bit_exact(solver, open_):
     st = small_thermal()
+    if open_:
+        open_box(st, boundary=(1, 1, 1))
     st.solver = solver
     st.lehe_dir = 1
     if solver == abi.SOLVER_LEHE:
@@ -110,22 +125,28 @@ FUSED_FIELD_CASES = [((64, 64, 64), abi.SOLVER_YEE), ((200, 104, 96), abi.SOLVER
                      ((128, 128, 68), abi.SOLVER_YEE), ((256, 256, 68), abi.SOLVER_YEE),
                      ((256, 256, 128), abi.SOLVER_YEE), ((64, 64, 64), abi.SOLVER_LEHE),
                      ((200, 104, 96), abi.SOLVER_LEHE)]
+# the same launch forms with open borders on all axes and the absorber
+FUSED_FIELD_CASES = [(g, s, False) for g, s in FUSED_FIELD_CASES] + [
+    ((64, 64, 64), abi.SOLVER_YEE, True), ((200, 104, 96), abi.SOLVER_YEE, True),
+    ((256, 256, 68), abi.SOLVER_YEE, True), ((200, 104, 96), abi.SOLVER_LEHE, True)]
 
 
-@pytest.mark.parametrize("grid,solver", FUSED_FIELD_CASES,
-                         ids=[f"{'yee' if s == abi.SOLVER_YEE else 'lehe'}-{'x'.join(map(str, g))}"
-                              for g, s in FUSED_FIELD_CASES])
-def test_fused_field_step_equals_stage_calls(grid, solver):
+@pytest.mark.parametrize("grid,solver,open_", FUSED_FIELD_CASES,
+                         ids=[f"{'yee' if s == abi.SOLVER_YEE else 'lehe'}-{'x'.join(map(str, g))}{'-open' if o else ''}"
+                              for g, s, o in FUSED_FIELD_CASES])
+def test_fused_field_step_equals_stage_calls(grid, solver, open_):
     """pic_fdtd_step (two LDS-tiled kernels) against update_beforeCurrent,
     addCurrent and update_afterCurrent called one by one, which
     test_fields_bit_exact pins to the oracle; twice, so the second call runs
     on the swapped E buffers."""
     st = setups.thermal(grid=grid, ppc=1)
+    if open_:
+        open_box(st, boundary=(1, 1, 1), cells=((12, 7), (5, 9), (3, 8)))
     st.solver = solver
     st.lehe_dir = 1
     if solver == abi.SOLVER_LEHE:
         st.dt_si = 0.99 * st.cell_si[1] / setups.SPEED_OF_LIGHT_SI / 1.8
-    print(f"case fused field step grid={grid} solver={solver}", flush=True)
+    print(f"case fused field step grid={grid} solver={solver} open={open_}", flush=True)
     E, B = H.random_fields(st, 3, 1.0, 1.0)
     J = H.random_fields(st, 4, 0.1, 0.1)[0]
     fused, staged = abi.Simulation(st.config()), abi.Simulation(st.config())
@@ -158,6 +179,9 @@ STEP_CASES = [
     ("thermal-pqs-vay", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_VAY)),
     ("thermal-tsc-hc", lambda: small_thermal(abi.SHAPE_TSC, abi.PUSHER_HIGUERA_CARY, u=0.3)),
     ("kh-tsc-2species", lambda: small_two_species(abi.SHAPE_TSC)),
+    ("open-xz-cic-boris", lambda: open_box(small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS, u=0.3))),
+    ("open-xyz-tsc-vay-lehe", lambda: open_box(small_lehe(), boundary=(1, 1, 1))),
+    ("open-y-kh-tsc", lambda: open_box(small_two_species(abi.SHAPE_TSC), boundary=(0, 1, 0))),
 ]
 
 
@@ -167,10 +191,13 @@ def test_steps_bit_exact(name, make):
     nsteps = 10
     print(f"case steps {name} grid={st.grid} steps={nsteps}", flush=True)
     sim, ora, _ = H.make_pair(st, e_amp=1e-3, b_amp=1e-3)
+    n0 = sum(int(sim.read_frames(sp)["counts"].sum()) for sp in range(len(st.species)))
     sim.step(nsteps)
     ora.step(nsteps)
     for sp in range(len(st.species)):
         H.assert_frames_equal(sim.read_frames(sp), ora.get_frames(sp), f"{name} species {sp}")
+    if any(st.boundary):  # the case must absorb particles at its open borders
+        assert sum(int(sim.read_frames(sp)["counts"].sum()) for sp in range(len(st.species))) < n0
     for f in (abi.FIELD_E, abi.FIELD_B, abi.FIELD_J):
         H.assert_field_equal(sim.read_field(f), ora.get_field(f), f"{name} field {f}")
     sim.close()

@@ -1,0 +1,131 @@
+"""Open borders and the exponential field absorber in the oracle, pinned to
+closed forms and to independent numpy statements.
+
+Open axis (the reference with the axis not periodic and the default
+absorbing particle boundary, offset 0): fields read zero beyond the border,
+current deposited there is dropped, particles crossing it are removed.
+Exponential absorber (absorber/exponential/Exponential.hpp:52-95,
+Exponential.kernel:63-112): at distance d from an open border, fields are
+multiplied by exp(-strength * (T - 1 - d)) for d < T - 1, thickness T and
+strength read at [axis][exchange % 2] (the negative side uses entry 1)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from picongpu_amd import abi, setups
+
+
+def field_only(grid, boundary=(1, 0, 0), absorber=0, cells=((0, 0), (0, 0), (0, 0)),
+               strength=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0))):
+    st = setups.thermal(grid=grid, ppc=1)
+    st.species = []
+    st.boundary = boundary
+    st.absorber = absorber
+    st.absorber_cells = cells
+    st.absorber_strength = strength
+    return st
+
+
+def test_absorber_factors_closed_form():
+    nx = 64
+    st = field_only((nx, 8, 4), absorber=abi.ABSORBER_EXPONENTIAL, cells=((6, 10), (0, 0), (0, 0)),
+                    strength=((0.05, 0.02), (0.0, 0.0), (0.0, 0.0)))
+    ora = O.Oracle(st, 32, 1)
+    E = np.ones((3, st.ncells), np.float32)
+    ora.set_field(abi.FIELD_E, E)
+    ora.update_after_current()  # absorber on E, B half step, absorber on B
+    got = ora.get_field(abi.FIELD_E).reshape(3, 4, 8, nx)
+    want = np.ones(nx, np.float32)
+    # LEFT is exchange 1: entry [x][1] = 10 cells, strength 0.02
+    for x in range(0, 9):
+        want[x] = np.exp(np.float32(-0.02) * np.float32(10 - 1 - x))
+    # RIGHT is exchange 2: entry [x][0] = 6 cells, strength 0.05
+    for d in range(0, 5):
+        want[nx - 1 - d] = np.exp(np.float32(-0.05) * np.float32(6 - 1 - d))
+    for c in range(3):
+        np.testing.assert_allclose(got[c], np.broadcast_to(want, (4, 8, nx)), rtol=2e-7, atol=0)
+    assert np.all(got[:, :, :, 9:nx - 5] == 1.0)
+    ora.close()
+
+
+def test_open_border_curl_reads_zero_beyond():
+    """update_afterCurrent's B half step against a numpy Yee curl with zero
+    padding along the open axis and periodic wrap on the others."""
+    nx, ny, nz = 32, 16, 8
+    st = field_only((nx, ny, nz))
+    ora = O.Oracle(st, 32, 1)
+    k = O.constants(st)
+    rng = np.random.default_rng(5)
+    E = rng.standard_normal((3, st.ncells)).astype(np.float32)
+    B = rng.standard_normal((3, st.ncells)).astype(np.float32)
+    ora.set_field(abi.FIELD_E, E)
+    ora.set_field(abi.FIELD_B, B)
+    ora.update_after_current()
+    e = E.reshape(3, nz, ny, nx)
+    b = B.reshape(3, nz, ny, nx)
+
+    def fwd(f, axis):  # (f[i+1] - f[i]) / dx, zero beyond the open x border
+        cell = np.float32(k.cell[axis])
+        if axis == 0:
+            nxt = np.concatenate([f[:, :, 1:], np.zeros_like(f[:, :, :1])], axis=2)
+        elif axis == 1:
+            nxt = np.roll(f, -1, axis=1)
+        else:
+            nxt = np.roll(f, -1, axis=0)
+        return (nxt - f) / cell
+
+    cx = fwd(e[2], 1) - fwd(e[1], 2)
+    cy = fwd(e[0], 2) - fwd(e[2], 0)
+    cz = fwd(e[1], 0) - fwd(e[0], 1)
+    half = np.float32(0.5) * np.float32(k.dt)
+    want = np.stack([b[0] - cx * half, b[1] - cy * half, b[2] - cz * half]).reshape(3, -1)
+    np.testing.assert_array_equal(ora.get_field(abi.FIELD_B).astype(np.float32), want)
+    ora.close()
+
+
+def test_absorber_removes_a_pulse_an_open_border_reflects_it():
+    def run(absorber):
+        st = field_only((256, 8, 4), absorber=absorber, cells=((32, 32), (0, 0), (0, 0)),
+                        strength=((0.1, 0.1), (0.0, 0.0), (0.0, 0.0)))
+        ora = O.Oracle(st, 32, 1)
+        k = O.constants(st)
+        x = np.arange(256)
+        E = np.zeros((3, st.ncells), np.float32)
+        B = np.zeros((3, st.ncells), np.float32)
+        E[1] = np.tile(np.exp(-((x - 128) / 10.0) ** 2), 32).astype(np.float32)
+        B[2] = (np.tile(np.exp(-((x + 0.5 - 128) / 10.0) ** 2), 32) / k.c).astype(np.float32)
+        ora.set_field(abi.FIELD_E, E)
+        ora.set_field(abi.FIELD_B, B)
+        e0 = sum(ora.field_energy())
+        for _ in range(400):  # the pulse travels about 230 cells: into the border and back
+            ora.update_before_current()
+            ora.add_current()
+            ora.update_after_current()
+        e1 = sum(ora.field_energy())
+        ora.close()
+        return e1 / e0
+    assert run(abi.ABSORBER_NONE) > 0.99
+    assert run(abi.ABSORBER_EXPONENTIAL) < 0.05
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_particle_leaving_an_open_border_is_removed(boundary):
+    st = setups.thermal(grid=(32, 8, 4), ppc=1, pusher=abi.PUSHER_BORIS)
+    st.boundary = (boundary, 0, 0)
+    ora = O.Oracle(st, 32, 1)
+    k = O.constants(st)
+    w = np.float32(k.typical_weighting)
+    mass = float(np.float32(k.species_mass[0]) * w)
+    beta = 0.5
+    p = np.float32(beta / math.sqrt(1 - beta * beta) * mass * k.c)
+    cells = np.array([[29, 4, 2]], np.int32)
+    f = setups.bin_frames(st, 0, cells, np.array([[0.5, 0.5, 0.5]], np.float32), np.array([[p, 0.0, 0.0]], np.float32),
+                          np.array([w], np.float32))
+    ora.set_frames(0, f)
+    for _ in range(20):  # about 5.7 cells: past the +x border at x = 32
+        ora.step(1)
+    n = int(ora.get_frames(0)["counts"].sum())
+    assert n == (1 if boundary == 0 else 0)
+    ora.close()
