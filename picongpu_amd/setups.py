@@ -116,7 +116,10 @@ def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0) -> Setup:
     from y = 20 um (the plateau lies beyond the 76.8 um box), and a Gaussian
     laser pulse as the initial field: 0.8 um, a0 = 2, waist 8 um, intensity
     FWHM 25 fs, polarised along x, travelling along +y, centred at y = 10 um
-    on the transverse axis."""
+    on the transverse axis. Borders as the reference's LaserWakefield example
+    (etc/picongpu/1.cfg: --periodic 0 0 1): x and y open, z periodic, with the
+    exponential absorber of param/fieldAbsorber.param (12 cells, strength
+    1e-3 on every side)."""
     cell = (1e-7, 5e-8, 1e-7)
     dt = 0.995 * cell[1] / SPEED_OF_LIGHT_SI
     ramp = (1, 20e-6, 100e-6)
@@ -128,7 +131,9 @@ def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0) -> Setup:
                  center_si=(grid[0] * cell[0] / 2, 10e-6, grid[2] * cell[2] / 2), propagation_axis=1,
                  polarisation_axis=0)
     return Setup("laser_plasma", tuple(grid), cell, dt, 3e24, ppc, [e, p], solver=abi.SOLVER_LEHE, lehe_dir=1,
-                 steps=steps, laser=laser)
+                 steps=steps, laser=laser, boundary=(abi.BOUNDARY_OPEN, abi.BOUNDARY_OPEN, abi.BOUNDARY_PERIODIC),
+                 absorber=abi.ABSORBER_EXPONENTIAL, absorber_cells=((12, 12), (12, 12), (12, 12)),
+                 absorber_strength=((1e-3, 1e-3), (1e-3, 1e-3), (1e-3, 1e-3)))
 
 
 CONFIGS = {
