@@ -26,7 +26,7 @@ class OracleConfig(C.Structure):
                 ("lehe_dir", C.c_int32), ("n_species", C.c_int32), ("species", OracleSpecies * 4),
                 ("j_scale_exp", C.c_int32), ("device", C.c_int32), ("boundary", C.c_int32 * 3),
                 ("absorber", C.c_int32), ("absorber_cells", (C.c_int32 * 2) * 3),
-                ("absorber_strength", (C.c_float * 2) * 3)]
+                ("absorber_strength", (C.c_float * 2) * 3), ("current_interpolation", C.c_int32)]
 
 
 class OracleConstants(C.Structure):
@@ -120,6 +120,7 @@ def config_from(setup) -> OracleConfig:
             c.absorber_cells[d][side] = int(setup.absorber_cells[d][side])
             c.absorber_strength[d][side] = float(setup.absorber_strength[d][side])
     c.absorber = int(setup.absorber)
+    c.current_interpolation = int(setup.current_interpolation)
     return c
 
 

@@ -72,6 +72,7 @@ typedef struct oracle_config {
     int32_t absorber;               // 0 none, 1 exponential
     int32_t absorber_cells[3][2];   // fieldAbsorber.param NUM_CELLS [axis][negative, positive]
     float absorber_strength[3][2];  // fieldAbsorber.param exponential::STRENGTH
+    int32_t current_interpolation;  // 0 None, 1 Binomial
 } oracle_config;
 
 typedef struct oracle_constants {
@@ -285,7 +286,9 @@ struct State {
     // never written nor exchanged, so fields read zero and current deposited
     // there never reaches the core (FieldJ.x.cpp:156-174 exchanges with
     // neighbours only)
+    bool anyOpen = false;
     bool outside(int x, int y, int z) const {
+        if (!anyOpen) return false;
         return (cfg.boundary[0] && (x < 0 || x >= nx)) || (cfg.boundary[1] && (y < 0 || y >= ny)) ||
                (cfg.boundary[2] && (z < 0 || z >= nz));
     }
@@ -1046,11 +1049,34 @@ void eFull(State<T>& st) {
             }
 }
 
+// E += coeff * J (None.hpp:56-65), or with the 3D binomial filter
+// (Binomial.hpp:40-104): corner, edge, face sums in the reference's order
+// with weights 1, 2, 4, the centre with 8, times 1/64, on a periodic box
 template <typename T>
 void addCurrent(State<T>& st) {
+    if (st.cfg.current_interpolation != 1) {
 #pragma omp parallel for schedule(static)
-    for (size_t i = 0; i < st.ncells; ++i)
-        for (int c = 0; c < 3; ++c) st.E[c][i] = st.E[c][i] + st.k.coeff * st.J[c][i];
+        for (size_t i = 0; i < st.ncells; ++i)
+            for (int c = 0; c < 3; ++c) st.E[c][i] = st.E[c][i] + st.k.coeff * st.J[c][i];
+        return;
+    }
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < st.nz; ++z)
+        for (int y = 0; y < st.ny; ++y)
+            for (int x = 0; x < st.nx; ++x)
+                for (int c = 0; c < 3; ++c) {
+                    auto J = [&](int dx, int dy, int dz) { return st.J[c][st.idx(x + dx, y + dy, z + dz)]; };
+                    T const corner = J(-1, -1, -1) + J(+1, -1, -1) + J(-1, +1, -1) + J(+1, +1, -1) + J(-1, -1, +1) +
+                                     J(+1, -1, +1) + J(-1, +1, +1) + J(+1, +1, +1);
+                    T const edge = J(-1, -1, 0) + J(+1, -1, 0) + J(-1, +1, 0) + J(+1, +1, 0) + J(-1, 0, -1) +
+                                   J(+1, 0, -1) + J(-1, 0, +1) + J(+1, 0, +1) + J(0, -1, -1) + J(0, +1, -1) +
+                                   J(0, -1, +1) + J(0, +1, +1);
+                    T const face = J(-1, 0, 0) + J(+1, 0, 0) + J(0, -1, 0) + J(0, +1, 0) + J(0, 0, -1) + J(0, 0, +1);
+                    T avg = T(1.0) * corner + T(2.0) * edge + T(4.0) * face + T(8.0) * J(0, 0, 0);
+                    avg = avg * (T(1.0) / (T(8.0) + T(6.0) * T(4.0) + T(12.0) * T(2.0) + T(8.0) * T(1.0)));
+                    size_t const i = st.idx(x, y, z);
+                    st.E[c][i] = st.E[c][i] + st.k.coeff * avg;
+                }
 }
 
 // Exponential absorber (absorber/exponential/Exponential.hpp:52-95,
@@ -1157,6 +1183,7 @@ State<T>* create(const oracle_config& cfg, int order) {
     st->nsc = st->nscx * st->nscy * st->nscz;
     st->ncells = static_cast<size_t>(st->nx) * st->ny * st->nz;
     st->order = order;
+    st->anyOpen = cfg.boundary[0] || cfg.boundary[1] || cfg.boundary[2];
     for (int c = 0; c < 3; ++c) {
         st->E[c].assign(st->ncells, T(0));
         st->B[c].assign(st->ncells, T(0));

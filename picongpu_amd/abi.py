@@ -20,6 +20,7 @@ SHAPE_CIC, SHAPE_TSC, SHAPE_PQS = 1, 2, 3
 PUSHER_NONE, PUSHER_BORIS, PUSHER_VAY, PUSHER_HIGUERA_CARY = 0, 1, 2, 3
 BOUNDARY_PERIODIC, BOUNDARY_OPEN = 0, 1
 ABSORBER_NONE, ABSORBER_EXPONENTIAL = 0, 1
+CURRENT_NONE, CURRENT_BINOMIAL = 0, 1
 SOLVER_YEE, SOLVER_LEHE = 0, 1
 FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
 TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2
@@ -55,6 +56,7 @@ class Config(C.Structure):
         ("absorber", C.c_int32),
         ("absorber_cells", (C.c_int32 * 2) * 3),
         ("absorber_strength", (C.c_float * 2) * 3),
+        ("current_interpolation", C.c_int32),
     ]
 
 
@@ -212,7 +214,7 @@ def ptr(a: np.ndarray, ctype):
 
 def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YEE, lehe_dir=1, j_scale_exp=0,
                 device=0, boundary=(0, 0, 0), absorber=0, absorber_cells=((0, 0), (0, 0), (0, 0)),
-                absorber_strength=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0))) -> Config:
+                absorber_strength=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0)), current_interpolation=0) -> Config:
     """species: list of dicts with mass_ratio, charge_ratio, shape, pusher, frames_per_sc;
     boundary: BOUNDARY_* per axis; absorber: ABSORBER_*, with thickness and
     strength per [axis][negative, positive] side as in fieldAbsorber.param."""
@@ -240,6 +242,7 @@ def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YE
             cfg.absorber_cells[d][side] = int(absorber_cells[d][side])
             cfg.absorber_strength[d][side] = float(absorber_strength[d][side])
     cfg.absorber = int(absorber)
+    cfg.current_interpolation = int(current_interpolation)
     return cfg
 
 

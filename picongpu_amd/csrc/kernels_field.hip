@@ -197,20 +197,27 @@ __device__ __forceinline__ size_t wrapAt(const GridDev& g, int x0, int y0, int z
     return (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * (size_t)z);
 }
 
+// beyond an open border, for kernels compiled with OPEN (periodic boxes
+// compile the checks away)
+template <bool OPEN>
+__device__ __forceinline__ bool outsideIf(const GridDev& g, int x, int y, int z) {
+    return OPEN && outsideOpen(g, x, y, z);
+}
+
 struct FieldLds {
     float E[3][kEV];
     float B[3][kFHV];
 };
 
 // tile b of the grid: every thread of the 256-thread workgroup calls it
-template <int SOLVER>
+template <int SOLVER, bool OPEN>
 __device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* const* Enew, int b, FieldLds& L) {
     int const ntx = a.g.nx / kFTX, nty = a.g.ny / kFTY;
     int const x0 = (b % ntx) * kFTX, y0 = ((b / ntx) % nty) * kFTY, z0 = (b / (ntx * nty)) * kFTZ;
     for (int t = threadIdx.x; t < 3 * kEV; t += blockDim.x) {
         int const c = t / kEV, r = t - c * kEV;
         int const ex = r % kEX, ey = (r / kEX) % kEY, ez = r / (kEX * kEY);
-        bool const out = outsideOpen(a.g, x0 + ex - kEH, y0 + ey - kEH, z0 + ez - kEH);
+        bool const out = outsideIf<OPEN>(a.g, x0 + ex - kEH, y0 + ey - kEH, z0 + ez - kEH);
         L.E[c][r] = out ? 0.0f : a.E[c][wrapAt(a.g, x0, y0, z0, ex - kEH, ey - kEH, ez - kEH)];
     }
     __syncthreads();
@@ -220,7 +227,7 @@ __device__ __forceinline__ void fieldBeforeAddTile(const FieldArgs& a, float* co
         // B' at tile offset (hx-1, hy-1, hz-1)
         int const he = (hx - 1 + kEH) + kEX * ((hy - 1 + kEH) + kEY * (hz - 1 + kEH));
         size_t const i = wrapAt(a.g, x0, y0, z0, hx - 1, hy - 1, hz - 1);
-        if (outsideOpen(a.g, x0 + hx - 1, y0 + hy - 1, z0 + hz - 1)) {  // B beyond an open border stays zero
+        if (outsideIf<OPEN>(a.g, x0 + hx - 1, y0 + hy - 1, z0 + hz - 1)) {  // B beyond an open border stays zero
             L.B[0][t] = L.B[1][t] = L.B[2][t] = 0.0f;
             continue;
         }
@@ -295,6 +302,7 @@ struct YeeBeforeLds {
 };
 
 // update_beforeCurrent + addCurrent for one z-marching tile (Yee)
+template <bool OPEN>
 __device__ __forceinline__ void yeeBeforeAddTile(const FieldArgs& a, float* const* Enew, int b, int zc,
                                                  YeeBeforeLds& L) {
     ZTile const T = zTile(a.g, b, zc);
@@ -309,15 +317,15 @@ __device__ __forceinline__ void yeeBeforeAddTile(const FieldArgs& a, float* cons
         int const t = threadIdx.x + 256 * j;
         int const r = t % (kZEX * kZEY);
         int const x = T.x0 + r % kZEX - 1, y = T.y0 + r / kZEX - 1;
-        eoff[j] = outsideOpen(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
+        eoff[j] = outsideIf<OPEN>(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
     }
     int boff[NB];
     for (int j = 0; j < NB; ++j) {
         int const t = threadIdx.x + 256 * j;
         int const x = T.x0 + t % kZBX - 1, y = T.y0 + t / kZBX - 1;
-        boff[j] = outsideOpen(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
+        boff[j] = outsideIf<OPEN>(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
     }
-    auto planeOut = [&](int k) { return (a.g.open & 4) && (k < 0 || k >= nz); };
+    auto planeOut = [&](int k) { return OPEN && (a.g.open & 4) && (k < 0 || k >= nz); };
     float* const Ebase = &L.E[0][0][0][0];
     auto loadE = [&](int k) {  // E plane k (wrapped) into slot k & 1
         size_t const zoff = plane * (size_t)wrap1(k, nz);
@@ -327,7 +335,7 @@ __device__ __forceinline__ void yeeBeforeAddTile(const FieldArgs& a, float* cons
         for (int j = 0; j < NE; ++j) {
             int const t = threadIdx.x + 256 * j;
             if (t < 3 * kZEX * kZEY)
-                dst[t] = (zout || eoff[j] < 0) ? 0.0f : a.E[t / (kZEX * kZEY)][zoff + (size_t)eoff[j]];
+                dst[t] = (OPEN && (zout || eoff[j] < 0)) ? 0.0f : a.E[t / (kZEX * kZEY)][zoff + (size_t)eoff[j]];
         }
     };
     // B' = B - curlE(E)*0.5*dt at plane k on x, y in [-1, T): forward differences
@@ -340,7 +348,7 @@ __device__ __forceinline__ void yeeBeforeAddTile(const FieldArgs& a, float* cons
             int const t = threadIdx.x + 256 * j;
             if (t >= kZBX * kZBY) continue;
             int const bx = t % kZBX, by = t / kZBX;  // cell (bx-1, by-1) of the tile
-            if (zout || boff[j] < 0) {  // B beyond an open border stays zero
+            if (OPEN && (zout || boff[j] < 0)) {  // B beyond an open border stays zero
                 L.B[s][0][by][bx] = L.B[s][1][by][bx] = L.B[s][2][by][bx] = 0.0f;
                 continue;
             }
@@ -401,6 +409,7 @@ struct YeeAfterLds {
 };
 
 // update_afterCurrent for one z-marching tile (Yee): B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt
+template <bool OPEN>
 __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* Enew, int b, int zc, YeeAfterLds& L) {
     ZTile const T = zTile(a.g, b, zc);
     int const nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
@@ -412,20 +421,20 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
         int const t = threadIdx.x + 256 * j;
         int const r = t % (PX * PY);
         int const x = T.x0 + r % PX, y = T.y0 + r / PX;
-        eoff[j] = outsideOpen(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
+        eoff[j] = outsideIf<OPEN>(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
     }
     float* const Eb = &L.E[0][0][0][0];
     float* const Nb = &L.En[0][0][0][0];
     auto load = [&](int k) {  // planes k of E and Enew on x, y in [0, T+1)
         size_t const zoff = plane * (size_t)wrap1(k, nz);
-        bool const zout = (a.g.open & 4) && (k < 0 || k >= nz);
+        bool const zout = OPEN && (a.g.open & 4) && (k < 0 || k >= nz);
         int const so = (k & 1) * (3 * PX * PY);
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             int const t = threadIdx.x + 256 * j;
             if (t < 3 * PX * PY) {
                 int const c = t / (PX * PY);
-                bool const out = zout || eoff[j] < 0;
+                bool const out = OPEN && (zout || eoff[j] < 0);
                 size_t const gi = zoff + (size_t)(out ? 0 : eoff[j]);
                 Eb[so + t] = out ? 0.0f : a.E[c][gi];
                 Nb[so + t] = out ? 0.0f : Enew[c][gi];
@@ -470,7 +479,7 @@ struct AfterLds {
 };
 
 // B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt on tile b
-template <int SOLVER>
+template <int SOLVER, bool OPEN>
 __device__ __forceinline__ void fieldAfterTile(const FieldArgs& a, float* const* Enew, int b, AfterLds& L) {
     int const ntx = a.g.nx / kFTX, nty = a.g.ny / kFTY;
     int const x0 = (b % ntx) * kFTX, y0 = ((b / ntx) % nty) * kFTY, z0 = (b / (ntx * nty)) * kFTZ;
@@ -478,7 +487,7 @@ __device__ __forceinline__ void fieldAfterTile(const FieldArgs& a, float* const*
         int const c = t / kAV, r = t - c * kAV;
         int const ex = r % kAX, ey = (r / kAX) % kAY, ez = r / (kAX * kAY);
         size_t const gi = wrapAt(a.g, x0, y0, z0, ex - 1, ey - 1, ez - 1);
-        bool const out = outsideOpen(a.g, x0 + ex - 1, y0 + ey - 1, z0 + ez - 1);
+        bool const out = outsideIf<OPEN>(a.g, x0 + ex - 1, y0 + ey - 1, z0 + ez - 1);
         L.E[c][r] = out ? 0.0f : a.E[c][gi];
         L.En[c][r] = out ? 0.0f : Enew[c][gi];
     }
@@ -498,23 +507,25 @@ __device__ __forceinline__ void fieldAfterTile(const FieldArgs& a, float* const*
     a.B[2][i] = b2 - cz * 0.5f * a.dt;
 }
 
-template <int SOLVER>
+template <int SOLVER, bool OPEN>
 __global__ __launch_bounds__(256) void kFieldBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2) {
     __shared__ FieldLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
-    fieldBeforeAddTile<SOLVER>(a, En, blockIdx.x, L);
+    fieldBeforeAddTile<SOLVER, OPEN>(a, En, blockIdx.x, L);
 }
 
+template <bool OPEN>
 __global__ __launch_bounds__(256) void kYeeBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
     __shared__ YeeBeforeLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
-    yeeBeforeAddTile(a, En, blockIdx.x, zc, L);
+    yeeBeforeAddTile<OPEN>(a, En, blockIdx.x, zc, L);
 }
 
+template <bool OPEN>
 __global__ __launch_bounds__(256) void kYeeAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
     __shared__ YeeAfterLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
-    yeeAfterTile(a, En, blockIdx.x, zc, L);
+    yeeAfterTile<OPEN>(a, En, blockIdx.x, zc, L);
 }
 
 __global__ __launch_bounds__(256) void kYeeBeforeAddBatch(const FieldBatchEntry* __restrict__ table,
@@ -522,7 +533,7 @@ __global__ __launch_bounds__(256) void kYeeBeforeAddBatch(const FieldBatchEntry*
     __shared__ YeeBeforeLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
-    yeeBeforeAddTile(e.a, e.Enew, it.y, kZC, L);
+    yeeBeforeAddTile<false>(e.a, e.Enew, it.y, kZC, L);
 }
 
 __global__ __launch_bounds__(256) void kYeeAfterBatch(const FieldBatchEntry* __restrict__ table,
@@ -530,14 +541,14 @@ __global__ __launch_bounds__(256) void kYeeAfterBatch(const FieldBatchEntry* __r
     __shared__ YeeAfterLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
-    yeeAfterTile(e.a, e.Enew, it.y, kZC, L);
+    yeeAfterTile<false>(e.a, e.Enew, it.y, kZC, L);
 }
 
-template <int SOLVER>
+template <int SOLVER, bool OPEN>
 __global__ __launch_bounds__(256) void kFieldAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2) {
     __shared__ AfterLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
-    fieldAfterTile<SOLVER>(a, En, blockIdx.x, L);
+    fieldAfterTile<SOLVER, OPEN>(a, En, blockIdx.x, L);
 }
 
 // batched forms for ensembles: items (member, tile)
@@ -547,7 +558,7 @@ __global__ __launch_bounds__(256) void kFieldBeforeAddBatch(const FieldBatchEntr
     __shared__ FieldLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
-    fieldBeforeAddTile<SOLVER>(e.a, e.Enew, it.y, L);
+    fieldBeforeAddTile<SOLVER, false>(e.a, e.Enew, it.y, L);
 }
 
 template <int SOLVER>
@@ -556,7 +567,7 @@ __global__ __launch_bounds__(256) void kFieldAfterBatch(const FieldBatchEntry* _
     __shared__ AfterLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
-    fieldAfterTile<SOLVER>(e.a, e.Enew, it.y, L);
+    fieldAfterTile<SOLVER, false>(e.a, e.Enew, it.y, L);
 }
 
 __global__ __launch_bounds__(256) void kCurrentResetBatch(const FieldBatchEntry* __restrict__ table,
@@ -578,6 +589,41 @@ __global__ __launch_bounds__(256) void kAddCurrent(FieldArgs a) {
     a.E[0][i] = a.E[0][i] + a.coeff * a.J[0][i];
     a.E[1][i] = a.E[1][i] + a.coeff * a.J[1][i];
     a.E[2][i] = a.E[2][i] + a.coeff * a.J[2][i];
+}
+
+// Binomial current filter (currentInterpolation/Binomial.hpp:40-104) of one
+// component at cell I on a periodic box: corner, edge and face sums in the
+// reference's order with weights 1, 2, 4, the centre with 8, times 1/64
+__device__ __forceinline__ float binomialJ(const float* J, const Idx& I) {
+    auto j = [&](int dx, int dy, int dz) { return J[I.at(dx, dy, dz)]; };
+    float const corner = j(-1, -1, -1) + j(+1, -1, -1) + j(-1, +1, -1) + j(+1, +1, -1) + j(-1, -1, +1) +
+                         j(+1, -1, +1) + j(-1, +1, +1) + j(+1, +1, +1);
+    float const edge = j(-1, -1, 0) + j(+1, -1, 0) + j(-1, +1, 0) + j(+1, +1, 0) + j(-1, 0, -1) + j(+1, 0, -1) +
+                       j(-1, 0, +1) + j(+1, 0, +1) + j(0, -1, -1) + j(0, +1, -1) + j(0, -1, +1) + j(0, +1, +1);
+    float const face = j(-1, 0, 0) + j(+1, 0, 0) + j(0, -1, 0) + j(0, +1, 0) + j(0, 0, -1) + j(0, 0, +1);
+    float const avg = 1.0f * corner + 2.0f * edge + 4.0f * face + 8.0f * j(0, 0, 0);
+    return avg * (1.0f / 64.0f);
+}
+
+__global__ __launch_bounds__(256) void kAddCurrentBinomial(FieldArgs a) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Idx const I{a.g, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    a.E[0][i] = a.E[0][i] + a.coeff * binomialJ(a.J[0], I);
+    a.E[1][i] = a.E[1][i] + a.coeff * binomialJ(a.J[1], I);
+    a.E[2][i] = a.E[2][i] + a.coeff * binomialJ(a.J[2], I);
+}
+
+// the filtered current into out (the fused field step then adds it as J)
+__global__ __launch_bounds__(256) void kBinomialJ(FieldArgs a, float* o0, float* o1, float* o2) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Idx const I{a.g, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    o0[i] = binomialJ(a.J[0], I);
+    o1[i] = binomialJ(a.J[1], I);
+    o2[i] = binomialJ(a.J[2], I);
 }
 
 __global__ __launch_bounds__(256) void kFill(float* p, size_t n, float v) {
@@ -681,7 +727,8 @@ int yeeChunk(const GridDev& g) {
     return zTileCount(g, zc) >= kZFill / 2 ? zc : 0;
 }
 
-hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], const AbsorbPlan* ab, hipStream_t s) {
+template <bool OPEN>
+static hipError_t fieldStepT(int solver, const FieldArgs& a, float* const Enew[3], const AbsorbPlan* ab, hipStream_t s) {
     // update_afterCurrent damps E before its B half step and B after it
     // (FDTDBase.hpp:151-183): between the two kernels and after the second
     bool const absorb = ab && ab->n > 0;
@@ -689,23 +736,31 @@ hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3],
     int const zc = solver == PIC_SOLVER_YEE ? yeeChunk(a.g) : 0;
     int const zt = zc > 0 ? zTileCount(a.g, zc) : 0;
     if (solver == PIC_SOLVER_LEHE)
-        hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+        hipLaunchKernelGGL((kFieldBeforeAdd<PIC_SOLVER_LEHE, OPEN>), dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1],
+                           Enew[2]);
     else if (zc == 0)  // grids too small for the z-marching form to fill the chip: 3D tiles of 8x8x4
-        hipLaunchKernelGGL(kFieldBeforeAdd<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+        hipLaunchKernelGGL((kFieldBeforeAdd<PIC_SOLVER_YEE, OPEN>), dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1],
+                           Enew[2]);
     else
-        hipLaunchKernelGGL(kYeeBeforeAdd, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
+        hipLaunchKernelGGL(kYeeBeforeAdd<OPEN>, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
     if (absorb) {
         hipError_t const e = launchAbsorb(a.g, Enew, *ab, s);
         if (e != hipSuccess) return e;
     }
     if (solver == PIC_SOLVER_LEHE)
-        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_LEHE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+        hipLaunchKernelGGL((kFieldAfter<PIC_SOLVER_LEHE, OPEN>), dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1],
+                           Enew[2]);
     else if (zc == 0)
-        hipLaunchKernelGGL(kFieldAfter<PIC_SOLVER_YEE>, dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+        hipLaunchKernelGGL((kFieldAfter<PIC_SOLVER_YEE, OPEN>), dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1],
+                           Enew[2]);
     else
-        hipLaunchKernelGGL(kYeeAfter, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
+        hipLaunchKernelGGL(kYeeAfter<OPEN>, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
     if (absorb) return launchAbsorb(a.g, a.B, *ab, s);
     return hipGetLastError();
+}
+
+hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], const AbsorbPlan* ab, hipStream_t s) {
+    return a.g.open ? fieldStepT<true>(solver, a, Enew, ab, s) : fieldStepT<false>(solver, a, Enew, ab, s);
 }
 
 int fieldStepTiles(int solver, const GridDev& g) {
@@ -734,9 +789,18 @@ hipError_t launchCurrentResetBatch(const FieldBatchEntry* table, const int2* blo
     return hipGetLastError();
 }
 
-hipError_t launchAddCurrent(const FieldArgs& a, hipStream_t s) {
+hipError_t launchAddCurrent(const FieldArgs& a, bool binomial, hipStream_t s) {
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
-    hipLaunchKernelGGL(kAddCurrent, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    if (binomial)
+        hipLaunchKernelGGL(kAddCurrentBinomial, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(kAddCurrent, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchBinomialJ(const FieldArgs& a, float* const out[3], hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    hipLaunchKernelGGL(kBinomialJ, dim3(blocksFor(n)), dim3(256), 0, s, a, out[0], out[1], out[2]);
     return hipGetLastError();
 }
 

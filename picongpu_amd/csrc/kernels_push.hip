@@ -9,6 +9,8 @@
 // that is flushed once to HBM. Launches run over a checkerboard of
 // supercells whose J tiles do not overlap, so the flush is a plain
 // read-add-write in a fixed phase order.
+#include <type_traits>
+
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 #include "pic_shift_common.hpp"
@@ -144,7 +146,9 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
         return;
     }
 
-    if (MODE != 2) {
+    // the open-border checks are compiled only into the loop a box with open
+    // borders runs (the branch is uniform)
+    auto stageEB = [&](auto openBorders) {
         for (int t = tid; t < 6 * GV; t += blockDim.x) {
             int const c = t / GV;
             int const r = t - c * GV;
@@ -157,8 +161,15 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             size_t const gi = (size_t)gx + (size_t)a.g.nx * ((size_t)gy + (size_t)a.g.ny * gz);
             const float* src = c < 3 ? a.E[c] : a.B[c - 3];
             // fields beyond an open border read zero
-            sEB[t] = outsideOpen(a.g, ox - GLO + tx, oy - GLO + ty, oz - GLO + tz) ? 0.0f : src[gi];
+            bool const out = decltype(openBorders)::value && outsideOpen(a.g, ox - GLO + tx, oy - GLO + ty, oz - GLO + tz);
+            sEB[t] = out ? 0.0f : src[gi];
         }
+    };
+    if (MODE != 2) {
+        if (a.g.open)
+            stageEB(std::true_type{});
+        else
+            stageEB(std::false_type{});
     }
     if (MODE != 0) {
         for (int t = tid; t < 3 * JV; t += blockDim.x) sJ[t] = 0ull;
@@ -231,7 +242,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                     leaveBits |= (newMask >= 2 ? 1u : 0u) << (i / kPushThreads);
                     // a particle leaving through an open border is absorbed by the
                     // boundary before the reference's deposit stage sees it
-                    if (MODE == 1 && !outsideOpen(a.g, ox + lx + dir[0], oy + ly + dir[1], oz + lz + dir[2])) {
+                    if (MODE == 1 && !(a.g.open && outsideOpen(a.g, ox + lx + dir[0], oy + ly + dir[1], oz + lz + dir[2]))) {
                         int const cx = lx + dir[0] + DLO, cy = ly + dir[1] + DLO, cz = lz + dir[2] + DLO;
                         Vec3 const np{p[0], p[1], p[2]};
                         auto const G = Esirkepov<SUPP>::geometry(np, vel, dk);
@@ -260,7 +271,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     }
     __syncthreads();
 
-    if (MODE != 0) {
+    auto flushJ = [&](auto openBorders) {
         for (int t = tid; t < 3 * JV; t += blockDim.x) {
             long long const acc = (long long)sJ[t];
             if (acc == 0) continue;
@@ -269,7 +280,8 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             int const tx = r % JX;
             int const ty = (r / JX) % JY;
             int const tz = r / (JX * JY);
-            if (outsideOpen(a.g, ox - DLO + tx, oy - DLO + ty, oz - DLO + tz)) continue;  // dropped beyond the border
+            if (decltype(openBorders)::value && outsideOpen(a.g, ox - DLO + tx, oy - DLO + ty, oz - DLO + tz))
+                continue;  // dropped beyond the border
             int const gx = wrapIdx(ox - DLO + tx, a.g.nx);
             int const gy = wrapIdx(oy - DLO + ty, a.g.ny);
             int const gz = wrapIdx(oz - DLO + tz, a.g.nz);
@@ -277,6 +289,12 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             float const v = (float)((double)acc * a.jinv);
             a.J[c][gi] += v;
         }
+    };
+    if (MODE != 0) {
+        if (a.g.open)
+            flushJ(std::true_type{});
+        else
+            flushJ(std::false_type{});
     }
 
     // Epilogue: slot-ordered leaver list from the ballots, then grouped by

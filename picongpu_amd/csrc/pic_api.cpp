@@ -80,6 +80,12 @@ int validateConfig(const pic_config& c) {
     }
     if (c.absorber != PIC_ABSORBER_NONE && c.absorber != PIC_ABSORBER_EXPONENTIAL)
         return fail(PIC_E_INVALID, "absorber must be PIC_ABSORBER_NONE or PIC_ABSORBER_EXPONENTIAL");
+    if (c.current_interpolation != PIC_CURRENT_NONE && c.current_interpolation != PIC_CURRENT_BINOMIAL)
+        return fail(PIC_E_INVALID, "current_interpolation must be PIC_CURRENT_NONE or PIC_CURRENT_BINOMIAL");
+    if (c.current_interpolation == PIC_CURRENT_BINOMIAL &&
+        (c.boundary[0] != PIC_BOUNDARY_PERIODIC || c.boundary[1] != PIC_BOUNDARY_PERIODIC ||
+         c.boundary[2] != PIC_BOUNDARY_PERIODIC))
+        return fail(PIC_E_INVALID, "the binomial current filter needs a periodic box");
     return PIC_OK;
 }
 
@@ -247,6 +253,7 @@ struct pic_ctx {
     float* Ealt[3] = {nullptr, nullptr, nullptr};  // receives the new E of the fused field step
     int eParity = 0;                               // flips with every swap of E and Ealt
     pic::AbsorbPlan absorb{};                      // exponential absorber layers, exchange order
+    float* Jf[3] = {nullptr, nullptr, nullptr};    // binomially filtered J for the fused field step
     float* B[3] = {nullptr, nullptr, nullptr};
     float* J[3] = {nullptr, nullptr, nullptr};
     std::vector<SpeciesMem> sp;
@@ -575,6 +582,9 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
             return bail(PIC_E_HIP);
     }
     if (devAlloc(c, &c->errFlags, 1)) return bail(PIC_E_HIP);
+    if (cfg->current_interpolation == PIC_CURRENT_BINOMIAL)
+        for (int d = 0; d < 3; ++d)
+            if (devAlloc(c, &c->Jf[d], c->ncells)) return bail(PIC_E_HIP);
     if (cfg->absorber == PIC_ABSORBER_EXPONENTIAL) {
         // ExponentialImpl::run (Exponential.hpp:52-95): plane exchanges LEFT,
         // RIGHT, BOTTOM, TOP, BACK, FRONT in this order, on borders without a
@@ -987,7 +997,7 @@ int pic_add_current(pic_ctx* c) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     pic::FieldArgs a = makeFieldArgs(c);
     TimerScope ts(c, PIC_TIMER_FIELD);
-    PIC_HIP(pic::launchAddCurrent(a, c->stream));
+    PIC_HIP(pic::launchAddCurrent(a, c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL, c->stream));
     return PIC_OK;
 }
 
@@ -1012,6 +1022,10 @@ int fieldStep(pic_ctx* c) {
     pic::FieldArgs a = makeFieldArgs(c);
     {
         TimerScope ts(c, PIC_TIMER_FIELD);
+        if (c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL) {
+            PIC_HIP(pic::launchBinomialJ(a, c->Jf, c->stream));
+            for (int d = 0; d < 3; ++d) a.J[d] = c->Jf[d];
+        }
         PIC_HIP(pic::launchFieldStep(c->cfg.solver, a, c->Ealt, &c->absorb, c->stream));
     }
     for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
@@ -1234,7 +1248,7 @@ int diffStep(pic_ctx* c, const pic::DiffParticles& P, float* const E[3], float* 
     }
     PIC_HIP(pic::launchBHalf(PIC_SOLVER_YEE, f, c->stream));
     PIC_HIP(pic::launchEFull(f, c->stream));
-    PIC_HIP(pic::launchAddCurrent(f, c->stream));
+    PIC_HIP(pic::launchAddCurrent(f, false, c->stream));
     PIC_HIP(pic::launchBHalf(PIC_SOLVER_YEE, f, c->stream));
     return PIC_OK;
 }
@@ -1301,8 +1315,8 @@ int pic_adjoint_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* 
     if (r != PIC_OK) return r;
     if (n <= 0 || !cell || !pos || !mom || !w) return fail(PIC_E_INVALID, "particle arrays and n > 0 are required");
     if (c->cfg.solver != PIC_SOLVER_YEE) return fail(PIC_E_INVALID, "the differentiable form uses the Yee solver");
-    if (!allPeriodic(c->cfg) || c->cfg.absorber != PIC_ABSORBER_NONE)
-        return fail(PIC_E_INVALID, "the differentiable form needs a periodic box without absorber");
+    if (!allPeriodic(c->cfg) || c->cfg.absorber != PIC_ABSORBER_NONE || c->cfg.current_interpolation != PIC_CURRENT_NONE)
+        return fail(PIC_E_INVALID, "the differentiable form needs a periodic box without absorber or current filter");
     if (c->ncells > (size_t)INT32_MAX) return fail(PIC_E_INVALID, "the differentiable form needs fewer than 2^31 cells");
     const SpeciesMem& sm = c->sp[s];
     if (sm.desc.shape != PIC_SHAPE_CIC || sm.desc.pusher != PIC_PUSHER_BORIS)
@@ -1763,8 +1777,8 @@ int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out) {
     for (int32_t i = 0; i < n; ++i) {
         const pic_config& c = cfgs[i];
         if (c.device != e->device) return bail(fail(PIC_E_INVALID, "ensemble members must share one device"));
-        if (!allPeriodic(c) || c.absorber != PIC_ABSORBER_NONE)
-            return bail(fail(PIC_E_INVALID, "ensemble members need periodic boxes without absorber"));
+        if (!allPeriodic(c) || c.absorber != PIC_ABSORBER_NONE || c.current_interpolation != PIC_CURRENT_NONE)
+            return bail(fail(PIC_E_INVALID, "ensemble members need periodic boxes without absorber or current filter"));
         int supp = 0, pusher = 0;
         for (int s = 0; s < c.n_species; ++s) {
             if (c.species[s].pusher == PIC_PUSHER_NONE) continue;

@@ -85,7 +85,9 @@ constexpr int kListMaxSlots = kPushThreadsPerBlock * kPushMaxIters;
 hipError_t launchFill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s);
 hipError_t launchEFull(const FieldArgs& a, hipStream_t s);
-hipError_t launchAddCurrent(const FieldArgs& a, hipStream_t s);
+// binomial: the 27-point filter of currentInterpolation/Binomial.hpp (periodic box)
+hipError_t launchAddCurrent(const FieldArgs& a, bool binomial, hipStream_t s);
+hipError_t launchBinomialJ(const FieldArgs& a, float* const out[3], hipStream_t s);
 // fused update_beforeCurrent + addCurrent + update_afterCurrent; the new E is
 // written to Enew (the caller swaps E and Enew afterwards); the grid must be a
 // multiple of 8x8x4 cells

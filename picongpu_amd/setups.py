@@ -60,6 +60,7 @@ class Setup:
     absorber: int = 0  # abi.ABSORBER_*
     absorber_cells: tuple = ((0, 0), (0, 0), (0, 0))  # [axis][negative, positive]
     absorber_strength: tuple = ((0.0, 0.0), (0.0, 0.0), (0.0, 0.0))
+    current_interpolation: int = 0  # abi.CURRENT_*
 
     def config(self, device: int = 0, j_scale_exp: int = 0) -> abi.Config:
         sp = [dict(mass_ratio=s.mass_ratio, charge_ratio=s.charge_ratio, shape=s.shape, pusher=s.pusher,
@@ -67,7 +68,8 @@ class Setup:
         return abi.make_config(self.grid, self.cell_si, self.dt_si, self.density_si, self.typical_ppc, sp,
                                solver=self.solver, lehe_dir=self.lehe_dir, j_scale_exp=j_scale_exp, device=device,
                                boundary=self.boundary, absorber=self.absorber, absorber_cells=self.absorber_cells,
-                               absorber_strength=self.absorber_strength)
+                               absorber_strength=self.absorber_strength,
+                               current_interpolation=self.current_interpolation)
 
     @property
     def ncells(self) -> int:

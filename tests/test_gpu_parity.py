@@ -4,6 +4,8 @@ Bar: bit-exact for particle state (positions, momenta, cell indices,
 multiMask, supercell counts) and for E, B, J, against the oracle's device
 summation order. Every case prints what it runs before the GPU work.
 """
+from dataclasses import replace
+
 import numpy as np
 import pytest
 
@@ -91,12 +93,16 @@ def test_shift_bit_exact(open_):
     sim.close()
 
 
-@pytest.mark.parametrize("solver,open_", [(abi.SOLVER_YEE, False), (abi.SOLVER_LEHE, False), (abi.SOLVER_YEE, True),
-                                          (abi.SOLVER_LEHE, True)], ids=["yee", "lehe", "yee-open", "lehe-open"])
-def test_fields_bit_exact(solver, open_):
+@pytest.mark.parametrize("solver,open_,binomial", [(abi.SOLVER_YEE, False, False), (abi.SOLVER_LEHE, False, False),
+                                                   (abi.SOLVER_YEE, True, False), (abi.SOLVER_LEHE, True, False),
+                                                   (abi.SOLVER_YEE, False, True)],
+                         ids=["yee", "lehe", "yee-open", "lehe-open", "yee-binomial"])
+def test_fields_bit_exact(solver, open_, binomial):
     st = small_thermal()
     if open_:
         open_box(st, boundary=(1, 1, 1))
+    if binomial:
+        st.current_interpolation = abi.CURRENT_BINOMIAL
     st.solver = solver
     st.lehe_dir = 1
     if solver == abi.SOLVER_LEHE:
@@ -126,27 +132,31 @@ FUSED_FIELD_CASES = [((64, 64, 64), abi.SOLVER_YEE), ((200, 104, 96), abi.SOLVER
                      ((256, 256, 128), abi.SOLVER_YEE), ((64, 64, 64), abi.SOLVER_LEHE),
                      ((200, 104, 96), abi.SOLVER_LEHE)]
 # the same launch forms with open borders on all axes and the absorber
-FUSED_FIELD_CASES = [(g, s, False) for g, s in FUSED_FIELD_CASES] + [
-    ((64, 64, 64), abi.SOLVER_YEE, True), ((200, 104, 96), abi.SOLVER_YEE, True),
-    ((256, 256, 68), abi.SOLVER_YEE, True), ((200, 104, 96), abi.SOLVER_LEHE, True)]
+# (the last two: the binomial current filter)
+FUSED_FIELD_CASES = [(g, s, o) for g, s in FUSED_FIELD_CASES for o in ("",)] + [
+    ((64, 64, 64), abi.SOLVER_YEE, "open"), ((200, 104, 96), abi.SOLVER_YEE, "open"),
+    ((256, 256, 68), abi.SOLVER_YEE, "open"), ((200, 104, 96), abi.SOLVER_LEHE, "open"),
+    ((64, 64, 64), abi.SOLVER_YEE, "binomial"), ((256, 256, 68), abi.SOLVER_YEE, "binomial")]
 
 
-@pytest.mark.parametrize("grid,solver,open_", FUSED_FIELD_CASES,
-                         ids=[f"{'yee' if s == abi.SOLVER_YEE else 'lehe'}-{'x'.join(map(str, g))}{'-open' if o else ''}"
+@pytest.mark.parametrize("grid,solver,variant", FUSED_FIELD_CASES,
+                         ids=[f"{'yee' if s == abi.SOLVER_YEE else 'lehe'}-{'x'.join(map(str, g))}{'-' + o if o else ''}"
                               for g, s, o in FUSED_FIELD_CASES])
-def test_fused_field_step_equals_stage_calls(grid, solver, open_):
+def test_fused_field_step_equals_stage_calls(grid, solver, variant):
     """pic_fdtd_step (two LDS-tiled kernels) against update_beforeCurrent,
     addCurrent and update_afterCurrent called one by one, which
     test_fields_bit_exact pins to the oracle; twice, so the second call runs
     on the swapped E buffers."""
     st = setups.thermal(grid=grid, ppc=1)
-    if open_:
+    if variant == "open":
         open_box(st, boundary=(1, 1, 1), cells=((12, 7), (5, 9), (3, 8)))
+    if variant == "binomial":
+        st.current_interpolation = abi.CURRENT_BINOMIAL
     st.solver = solver
     st.lehe_dir = 1
     if solver == abi.SOLVER_LEHE:
         st.dt_si = 0.99 * st.cell_si[1] / setups.SPEED_OF_LIGHT_SI / 1.8
-    print(f"case fused field step grid={grid} solver={solver} open={open_}", flush=True)
+    print(f"case fused field step grid={grid} solver={solver} variant={variant}", flush=True)
     E, B = H.random_fields(st, 3, 1.0, 1.0)
     J = H.random_fields(st, 4, 0.1, 0.1)[0]
     fused, staged = abi.Simulation(st.config()), abi.Simulation(st.config())
@@ -182,6 +192,7 @@ STEP_CASES = [
     ("open-xz-cic-boris", lambda: open_box(small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS, u=0.3))),
     ("open-xyz-tsc-vay-lehe", lambda: open_box(small_lehe(), boundary=(1, 1, 1))),
     ("open-y-kh-tsc", lambda: open_box(small_two_species(abi.SHAPE_TSC), boundary=(0, 1, 0))),
+    ("binomial-kh-tsc", lambda: replace(small_two_species(abi.SHAPE_TSC), current_interpolation=abi.CURRENT_BINOMIAL)),
 ]
 
 

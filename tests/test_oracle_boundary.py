@@ -129,3 +129,38 @@ def test_particle_leaving_an_open_border_is_removed(boundary):
     n = int(ora.get_frames(0)["counts"].sum())
     assert n == (1 if boundary == 0 else 0)
     ora.close()
+
+
+def test_binomial_filter_weights_and_conservation():
+    """Binomial.hpp:40-104: a unit impulse spreads with weights 8 (centre),
+    4 (faces), 2 (edges), 1 (corners) over 64; the filter keeps the total."""
+    grid = (16, 16, 12)
+    st = setups.thermal(grid=grid, ppc=1)
+    st.species = []
+    st.current_interpolation = abi.CURRENT_BINOMIAL
+    ora = O.Oracle(st, 32, 1)
+    k = O.constants(st)
+    J = np.zeros((3, st.ncells), np.float32)
+    nx, ny, nz = grid
+    c0 = (5, 7, 3)
+    J[1, c0[0] + nx * (c0[1] + ny * c0[2])] = 64.0
+    ora.set_field(abi.FIELD_J, J)
+    ora.add_current()
+    E = ora.get_field(abi.FIELD_E).reshape(3, nz, ny, nx)
+    coeff = np.float32(k.add_current_coeff)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                w = 8 >> (abs(dx) + abs(dy) + abs(dz))
+                assert E[1, c0[2] + dz, c0[1] + dy, c0[0] + dx] == np.float32(coeff * np.float32(w))
+    assert np.count_nonzero(E[1]) == 27 and not E[0].any() and not E[2].any()
+    ora.close()
+    rng = np.random.default_rng(2)
+    J = rng.standard_normal((3, st.ncells)).astype(np.float32)
+    ora = O.Oracle(st, 64, 1)
+    ora.set_field(abi.FIELD_J, J)
+    ora.add_current()
+    E = ora.get_field(abi.FIELD_E)
+    np.testing.assert_allclose(E.sum(axis=1), float(k.add_current_coeff) * J.astype(np.float64).sum(axis=1),
+                               rtol=1e-9, atol=1e-9 * abs(float(k.add_current_coeff)) * st.ncells)
+    ora.close()
