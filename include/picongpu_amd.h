@@ -58,6 +58,12 @@ extern "C" {
 #define PIC_PUSHER_VAY 2
 #define PIC_PUSHER_HIGUERA_CARY 3 /* particlePusherHigueraCary.hpp:49-142 */
 
+/* current deposition per species (species.param UsedParticleCurrentSolver):
+ * Esirkepov (currentDeposition/Esirkepov/Esirkepov.hpp:61-242) or EmZ
+ * (currentDeposition/EmZ/EmZ.hpp:61-137, DepositCurrent.hpp:46-112) */
+#define PIC_CURRENT_SOLVER_ESIRKEPOV 0
+#define PIC_CURRENT_SOLVER_EMZ 1
+
 /* field solvers: fields/MaxwellSolver/Yee/Yee.def:36-47, Lehe/Lehe.def:57-73 */
 #define PIC_SOLVER_YEE 0
 #define PIC_SOLVER_LEHE 1
@@ -79,6 +85,13 @@ extern "C" {
 #define PIC_ABSORBER_EXPONENTIAL 1
 #define PIC_ABSORBER_MAX_CELLS 64
 
+/* current interpolation before J is added to E (fields/currentInterpolation):
+ * NONE (None.hpp:56-65) or the 27-point BINOMIAL filter (Binomial.hpp:40-104),
+ * the latter on periodic boxes (the reference reads neighbour-refreshed J
+ * guards, which an open border does not have) */
+#define PIC_CURRENT_NONE 0
+#define PIC_CURRENT_BINOMIAL 1
+
 /* fields */
 #define PIC_FIELD_E 0
 #define PIC_FIELD_B 1
@@ -99,7 +112,7 @@ typedef struct pic_species_desc {
     int32_t shape;          /* PIC_SHAPE_* */
     int32_t pusher;         /* PIC_PUSHER_* */
     int32_t frames_per_sc;  /* frames of 256 slots preallocated per supercell */
-    int32_t reserved;
+    int32_t current_solver; /* PIC_CURRENT_SOLVER_*: the species' current<> flag */
 } pic_species_desc;
 
 /* Simulation description: the .param surface of the hot path
@@ -123,6 +136,7 @@ typedef struct pic_config {
     int32_t absorber_cells[3][2];  /* param/fieldAbsorber.param NUM_CELLS [axis][negative, positive],
                                     * at most PIC_ABSORBER_MAX_CELLS */
     float absorber_strength[3][2]; /* param/fieldAbsorber.param exponential::STRENGTH */
+    int32_t current_interpolation; /* PIC_CURRENT_* */
 } pic_config;
 
 /* FP32 constants in PIC units derived from a pic_config, computed as

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 class OracleSpecies(C.Structure):
     _fields_ = [("mass_ratio", C.c_double), ("charge_ratio", C.c_double), ("shape", C.c_int32),
-                ("pusher", C.c_int32), ("frames_per_sc", C.c_int32), ("reserved", C.c_int32)]
+                ("pusher", C.c_int32), ("frames_per_sc", C.c_int32), ("current_solver", C.c_int32)]
 
 
 class OracleConfig(C.Structure):
@@ -81,6 +81,8 @@ def lib() -> C.CDLL:
             "oracle_charge_density": (None, [_P, _D]),
             "oracle_div_e": (None, [_P, _D]),
             "oracle_esirkepov_single": (None, [C.POINTER(OracleConfig), C.c_int32, _I32, _F, _F, C.c_float, _D]),
+            "oracle_deposit_single": (None, [C.POINTER(OracleConfig), C.c_int32, C.c_int32, _I32, _F, _F, C.c_float,
+                                             _D]),
             "oracle_move_particle": (C.c_int32, [_F, _U16, _U8]),
             "oracle_shape": (None, [C.c_int32, C.c_int32, C.c_float, C.c_int32, _F]),
         }
@@ -114,6 +116,7 @@ def config_from(setup) -> OracleConfig:
         c.species[i].shape = s.shape
         c.species[i].pusher = s.pusher
         c.species[i].frames_per_sc = s.frames()
+        c.species[i].current_solver = s.current_solver
     for d in range(3):
         c.boundary[d] = int(setup.boundary[d])
         for side in range(2):
@@ -247,6 +250,19 @@ def esirkepov_single(setup, shape, cell, pos, vel, charge):
     vel = np.ascontiguousarray(vel, np.float32)
     lib().oracle_esirkepov_single(C.byref(cfg), shape, _p(cell, C.c_int32), _p(pos, C.c_float), _p(vel, C.c_float),
                                   float(charge), _p(J, C.c_double))
+    return J
+
+
+def deposit_single(setup, shape, scheme, cell, pos, vel, charge):
+    """One particle's current with the Esirkepov (scheme 0) or EmZ (1) solver."""
+    cfg = config_from(setup)
+    n = int(np.prod(setup.grid))
+    J = np.zeros((3, n))
+    cell = np.ascontiguousarray(cell, np.int32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    lib().oracle_deposit_single(C.byref(cfg), shape, scheme, _p(cell, C.c_int32), _p(pos, C.c_float),
+                                _p(vel, C.c_float), float(charge), _p(J, C.c_double))
     return J
 
 

@@ -23,6 +23,7 @@
 // Supercell shifts follow the device's canonical order (see pic_shift) so
 // that the frame layouts can be compared slot by slot.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -53,7 +54,7 @@ typedef struct oracle_species {
     int32_t shape;   // 1 CIC, 2 TSC, 3 PQS
     int32_t pusher;  // 0 none, 1 Boris, 2 Vay, 3 Higuera-Cary
     int32_t frames_per_sc;
-    int32_t reserved;
+    int32_t current_solver;  // 0 Esirkepov, 1 EmZ
 } oracle_species;
 
 typedef struct oracle_config {
@@ -238,6 +239,13 @@ struct ShapeFns {
                 r = r2;
         }
         return r;
+    }
+    // ChargeAssignmentOnSupport::operator() (CIC.hpp:139-147, TSC.hpp:166-185, PQS.hpp:166-185)
+    T onSupportAt(T x) const {
+        T const a = std::fabs(x);
+        if (supp == 2) return T(1.0) - a;
+        if (supp == 3) return a < T(0.5) ? tscFF1(a) : tscFF2(a);
+        return a < T(1.0) ? pqsFF1(a) : pqsFF2(a);
     }
     // ChargeAssignment::shapeArray(pos, isOutOfRange) -> supp+1 values
     void cached(T pos, bool out, T* s) const {
@@ -518,6 +526,97 @@ void cptCurrent1D(int supp, const int* st, const T* p0, const T* p1, T csd, int 
     }
 }
 
+// EmZ, one virtual particle (EmZ/DepositCurrent.hpp:46-112): on-support
+// shapes (Jit outer, Cached middle and inner), supp x supp x (supp-1) cells
+// of its own shifted cell, no leave rows
+template <typename T, typename Add>
+void emzCpt1D(int supp, const T* p0, const T* p1, T csd, int comp, const int* perm, Add add) {
+    if (p0[2] == p1[2]) return;
+    ShapeFns<T> sh{supp};
+    int const begin = sh.begin();
+    int const end = begin + supp;
+    T s0j[4], s1j[4], s0k[4], s1k[4];
+    sh.onSupport(p0[1], s0j);
+    sh.onSupport(p1[1], s1j);
+    sh.onSupport(p0[2], s0k);
+    sh.onSupport(p1[2], s1k);
+    for (int i = begin; i < end; ++i) {
+        T const s0i = sh.onSupportAt(T(i) - p0[0]);
+        T const dsi = sh.onSupportAt(T(i) - p1[0]) - s0i;
+        for (int j = begin; j < end; ++j) {
+            T const sj = s0j[j - begin];
+            T const dsj = s1j[j - begin] - sj;
+            T const tmp = -csd * (s0i * sj + T(0.5) * (dsi * sj + s0i * dsj) + (T(1.0) / T(3.0)) * dsj * dsi);
+            T acc = T(0.0);
+            for (int kk = begin; kk < end - 1; ++kk) {
+                T const W = (s1k[kk - begin] - s0k[kk - begin]) * tmp;
+                acc += W;
+                int const ijk[3] = {i, j, kk};
+                int rel[3];
+                for (int a = 0; a < 3; ++a) rel[perm[a]] = ijk[a];
+                add(comp, rel[0], rel[1], rel[2], acc);
+            }
+        }
+    }
+}
+
+// EmZ (currentDeposition/EmZ/EmZ.hpp:61-137): the move splits at the relay
+// point (relayPoint.hpp:47-64) into one or two virtual particles
+template <typename T, typename Add>
+void emz(int supp, const T* pos, const V3<T>& vel, T charge, const Consts<T>& k, Add add) {
+    bool const isEven = supp % 2 == 0;
+    T const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
+    T p0[3], relay[3];
+    int s0[3], s1[3];
+    for (int d = 0; d < 3; ++d) {
+        p0[d] = pos[d] - dp[d];
+        if (isEven) {
+            s0[d] = static_cast<int>(std::floor(p0[d]));
+            s1[d] = static_cast<int>(std::floor(pos[d]));
+            relay[d] = s0[d] == s1[d] ? pos[d] : static_cast<T>(std::max(s0[d], s1[d]));
+        } else {
+            s0[d] = static_cast<int>(std::floor(p0[d] + T(0.5)));
+            s1[d] = static_cast<int>(std::floor(pos[d] + T(0.5)));
+            relay[d] = s0[d] == s1[d] ? pos[d] : static_cast<T>(s0[d] + s1[d]) / T(2.0);
+        }
+    }
+    T const chargeDensity = charge / (k.cell[0] * k.cell[1] * k.cell[2]);
+    auto virtualParticle = [&](const int* sh, const T* a, const T* b) {
+        auto shifted = [&](int comp, int x, int y, int z, T v) { add(comp, x + sh[0], y + sh[1], z + sh[2], v); };
+        {  // J_x: rotateOrigin<1,2,0>
+            int const perm[3] = {1, 2, 0};
+            T const q0[3] = {a[1], a[2], a[0]}, q1[3] = {b[1], b[2], b[0]};
+            emzCpt1D(supp, q0, q1, k.cell[0] * chargeDensity / k.dt, 0, perm, shifted);
+        }
+        {  // J_y: rotateOrigin<2,0,1>
+            int const perm[3] = {2, 0, 1};
+            T const q0[3] = {a[2], a[0], a[1]}, q1[3] = {b[2], b[0], b[1]};
+            emzCpt1D(supp, q0, q1, k.cell[1] * chargeDensity / k.dt, 1, perm, shifted);
+        }
+        {  // J_z
+            int const perm[3] = {0, 1, 2};
+            emzCpt1D(supp, a, b, k.cell[2] * chargeDensity / k.dt, 2, perm, shifted);
+        }
+    };
+    T a[3], b[3];
+    for (int d = 0; d < 3; ++d) {
+        a[d] = p0[d] - T(s0[d]);
+        b[d] = relay[d] - T(s0[d]);
+    }
+    virtualParticle(s0, a, b);
+    if (s0[0] != s1[0] || s0[1] != s1[1] || s0[2] != s1[2]) {
+        for (int d = 0; d < 3; ++d) {
+            b[d] = pos[d] - T(s1[d]);
+            a[d] = relay[d] - T(s1[d]);
+        }
+        virtualParticle(s1, a, b);
+    }
+}
+
+// the species' current solver: Esirkepov or EmZ
+template <typename T, typename Add>
+void depositParticle(int scheme, int supp, const T* pos, const V3<T>& vel, T charge, const Consts<T>& k, Add add);
+
 template <typename T, typename Add>
 void esirkepov(int supp, const T* pos, const V3<T>& vel, T charge, const Consts<T>& k, Add add) {
     bool const isEven = supp % 2 == 0;
@@ -563,6 +662,14 @@ void esirkepov(int supp, const T* pos, const V3<T>& vel, T charge, const Consts<
         T const csd = charge * k.inv_vdt * k.cell[2];
         cptCurrent1D(supp, st, p0, p1, csd, 2, perm, shifted);
     }
+}
+
+template <typename T, typename Add>
+void depositParticle(int scheme, int supp, const T* pos, const V3<T>& vel, T charge, const Consts<T>& k, Add add) {
+    if (scheme == 1)
+        emz(supp, pos, vel, charge, k, add);
+    else
+        esirkepov(supp, pos, vel, charge, k, add);
 }
 
 int depLower(int supp) { return supp / 2 + 1 - (supp + 1) % 2; }  // Esirkepov.hpp:42
@@ -681,11 +788,14 @@ void flushTile(State<T>& st, const Tile& t, int ox, int oy, int oz) {
 template <typename T>
 std::vector<std::vector<int>> groupsOf(State<T>& st, int species, bool byPusher) {
     std::vector<std::vector<int>> groups;
-    std::vector<std::pair<int, int>> keys;
+    // the device runs one launch per (support, pusher, current solver): the
+    // species of a launch share one tile per supercell, so the grouping sets
+    // the order of the float adds into J
+    std::vector<std::array<int, 3>> keys;
     for (int s = 0; s < static_cast<int>(st.sp.size()); ++s) {
         if (species >= 0 && s != species) continue;
         if (st.sp[s].desc.pusher == 0) continue;
-        std::pair<int, int> key{st.sp[s].supp, byPusher ? st.sp[s].desc.pusher : 0};
+        std::array<int, 3> key{st.sp[s].supp, byPusher ? st.sp[s].desc.pusher : 0, st.sp[s].desc.current_solver};
         size_t gi = 0;
         for (; gi < keys.size(); ++gi)
             if (keys[gi] == key) break;
@@ -756,14 +866,16 @@ void push(State<T>& st, int species, bool fused) {
                         // boundary before the reference's deposit stage sees it
                         if (st.outside(ox - lo + nx, oy - lo + ny, oz - lo + nz)) continue;
                         if (fixedPoint) {
-                            esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
+                            depositParticle(S.desc.current_solver, supp, pos, vel, charge, st.k,
+                                            [&](int c, int x, int y, int z, T v) {
                                 int64_t const q = toFixed(v, jscale);
                                 tile.acc[c * tv + (static_cast<size_t>(nz + z) * tile.ty + (ny + y)) * tile.tx +
                                          (nx + x)] += q;
                             });
                         } else {
                             // FP64 twin: plain sums straight into J (each supercell of a phase is disjoint)
-                            esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
+                            depositParticle(S.desc.current_solver, supp, pos, vel, charge, st.k,
+                                            [&](int c, int x, int y, int z, T v) {
                                 if (st.outside(ox - lo + nx + x, oy - lo + ny + y, oz - lo + nz + z)) return;
                                 st.J[c][st.idx(ox - lo + nx + x, oy - lo + ny + y, oz - lo + nz + z)] += v;
                             });
@@ -834,7 +946,8 @@ void deposit(State<T>& st, int species) {
                             V3<T> const mom{S.mom[0][g], S.mom[1][g], S.mom[2][g]};
                             V3<T> const vel = velocityOf(mom, S.mass * w, st.k.mue0_eps0);
                             T const charge = S.charge * w;
-                            esirkepov(supp, pos, vel, charge, st.k, [&](int c, int x, int y, int z, T v) {
+                            depositParticle(S.desc.current_solver, supp, pos, vel, charge, st.k,
+                                            [&](int c, int x, int y, int z, T v) {
                                 size_t const ti = c * tv + (static_cast<size_t>(cz + z) * ty + (cy + y)) * tx + (cx + x);
                                 if (fixedPoint)
                                     acc[ti] += toFixed(v, jscale);
@@ -1468,6 +1581,23 @@ void oracle_esirkepov_single(const oracle_config* cfg, int32_t shape, const int3
     float const p[3] = {pos[0], pos[1], pos[2]};
     V3<float> const v{vel[0], vel[1], vel[2]};
     esirkepov(supportOf(shape), p, v, charge, tmp.k, [&](int c, int x, int y, int z, float val) {
+        J[c * n + tmp.idx(cell[0] + x, cell[1] + y, cell[2] + z)] += static_cast<double>(val);
+    });
+}
+
+// the same with the current solver chosen: 0 Esirkepov, 1 EmZ
+void oracle_deposit_single(const oracle_config* cfg, int32_t shape, int32_t scheme, const int32_t* cell,
+                           const float* pos, const float* vel, float charge, double* J) {
+    State<float> tmp;
+    tmp.cfg = *cfg;
+    tmp.k = makeConsts<float>(*cfg);
+    tmp.nx = cfg->grid[0];
+    tmp.ny = cfg->grid[1];
+    tmp.nz = cfg->grid[2];
+    size_t const n = static_cast<size_t>(tmp.nx) * tmp.ny * tmp.nz;
+    float const p[3] = {pos[0], pos[1], pos[2]};
+    V3<float> const v{vel[0], vel[1], vel[2]};
+    depositParticle(scheme, supportOf(shape), p, v, charge, tmp.k, [&](int c, int x, int y, int z, float val) {
         J[c * n + tmp.idx(cell[0] + x, cell[1] + y, cell[2] + z)] += static_cast<double>(val);
     });
 }

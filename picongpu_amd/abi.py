@@ -21,6 +21,7 @@ PUSHER_NONE, PUSHER_BORIS, PUSHER_VAY, PUSHER_HIGUERA_CARY = 0, 1, 2, 3
 BOUNDARY_PERIODIC, BOUNDARY_OPEN = 0, 1
 ABSORBER_NONE, ABSORBER_EXPONENTIAL = 0, 1
 CURRENT_NONE, CURRENT_BINOMIAL = 0, 1
+CURRENT_SOLVER_ESIRKEPOV, CURRENT_SOLVER_EMZ = 0, 1
 SOLVER_YEE, SOLVER_LEHE = 0, 1
 FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
 TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2
@@ -35,7 +36,7 @@ class SpeciesDesc(C.Structure):
         ("shape", C.c_int32),
         ("pusher", C.c_int32),
         ("frames_per_sc", C.c_int32),
-        ("reserved", C.c_int32),
+        ("current_solver", C.c_int32),
     ]
 
 
@@ -234,6 +235,7 @@ def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YE
         cfg.species[i].shape = int(s["shape"])
         cfg.species[i].pusher = int(s["pusher"])
         cfg.species[i].frames_per_sc = int(s["frames_per_sc"])
+        cfg.species[i].current_solver = int(s.get("current_solver", 0))
     cfg.j_scale_exp = int(j_scale_exp)
     cfg.device = int(device)
     for d in range(3):

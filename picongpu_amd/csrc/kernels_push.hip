@@ -104,7 +104,7 @@ struct PushLds {
 
 // The work of one supercell `sc` of the simulation described by `a`; every
 // thread of the workgroup calls it.
-template <int SUPP, int PUSHER, int MODE>
+template <int SUPP, int PUSHER, int MODE, bool EMZ = false>
 __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigned char* smem) {
     constexpr int GLO = SUPP / 2;          // FieldToParticleInterpolation.hpp:49
     constexpr int GUP = (SUPP + 1) / 2;    // FieldToParticleInterpolation.hpp:50
@@ -202,7 +202,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
         int const iters = (n + (int)blockDim.x - 1) / (int)blockDim.x;  // uniform across the workgroup
         // the last species runs one iteration more, without particles, that drains the queue
         bool const lastSpecies = s == a.nsp - 1;
-        for (int it = 0; it < iters + (MODE == 1 && lastSpecies ? 1 : 0); ++it) {
+        for (int it = 0; it < iters + (MODE == 1 && !EMZ && lastSpecies ? 1 : 0); ++it) {
             int const i = it * (int)blockDim.x + tid;
             bool slow = false;
             SlowItem mine{};
@@ -246,7 +246,9 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                         int const cx = lx + dir[0] + DLO, cy = ly + dir[1] + DLO, cz = lz + dir[2] + DLO;
                         Vec3 const np{p[0], p[1], p[2]};
                         auto const G = Esirkepov<SUPP>::geometry(np, vel, dk);
-                        if (Esirkepov<SUPP>::changesCell(G)) {
+                        if (EMZ) {
+                            EmZ<SUPP>::template deposit<JX, JY, JV>(sJ, cx, cy, cz, np, vel, charge, dk);
+                        } else if (Esirkepov<SUPP>::changesCell(G)) {
                             slow = true;
                             mine = SlowItem{cx | (cy << 8) | (cz << 16), np.x, np.y, np.z, vel.x, vel.y, vel.z, charge};
                         } else {
@@ -255,10 +257,14 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                     }
                 } else {
                     Vec3 const vel = velocityOf(mom, mass, a.rc2);
-                    Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge, dk);
+                    if (EMZ)
+                        EmZ<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge, dk);
+                    else
+                        Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge,
+                                                                      dk);
                 }
             }
-            if (MODE == 1) {
+            if (MODE == 1 && !EMZ) {
                 queueSlow(slow, mine, qA, qB, queued);
                 if (queued >= 64 || (it == iters && queued > 0)) {
                     if ((int)(tid & 63) < queued) depositQueued<SUPP, JX, JY, JV>(sJ, qA, dk);
@@ -341,17 +347,17 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     }
 }
 
-template <int SUPP, int PUSHER, int MODE>
+template <int SUPP, int PUSHER, int MODE, bool EMZ = false>
 __global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
     __shared__ __align__(16) unsigned char smem[PushLds<SUPP, MODE>::BYTES];
-    pushDepositSC<SUPP, PUSHER, MODE>(a, a.scList[blockIdx.x], smem);
+    pushDepositSC<SUPP, PUSHER, MODE, EMZ>(a, a.scList[blockIdx.x], smem);
 }
 
 // Batched form for ensembles: work items (simulation, supercell) of one
 // checkerboard phase of every simulation in one launch; a workgroup takes
 // items blockIdx.x, blockIdx.x + gridDim.x, ... so a grid smaller than the
 // item count runs persistently.
-template <int SUPP, int PUSHER, int MODE>
+template <int SUPP, int PUSHER, int MODE, bool EMZ = false>
 __global__ __launch_bounds__(kPushThreads) void kPushDepositBatch(const PushArgs* __restrict__ table,
                                                                   const int2* __restrict__ work, int nwork) {
     __shared__ __align__(16) unsigned char smem[PushLds<SUPP, MODE>::BYTES];
@@ -386,29 +392,33 @@ hipError_t launchPushBatch(int supp, int pusher, const PushArgs* table, const in
     return hipGetLastError();
 }
 
-template <int SUPP, int PUSHER, int MODE>
+template <int SUPP, int PUSHER, int MODE, bool EMZ>
 static hipError_t launchT(const PushArgs& a, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((kPushDeposit<SUPP, PUSHER, MODE>), dim3(nblocks), dim3(kPushThreads), 0, s, a);
+    hipLaunchKernelGGL((kPushDeposit<SUPP, PUSHER, MODE, EMZ>), dim3(nblocks), dim3(kPushThreads), 0, s, a);
     return hipGetLastError();
 }
 
-template <int SUPP>
+template <int SUPP, bool EMZ>
 static hipError_t launchS(int pusher, int mode, const PushArgs& a, int nblocks, hipStream_t s) {
-    if (mode == 2) return launchT<SUPP, PIC_PUSHER_BORIS, 2>(a, nblocks, s);
-    if (pusher == PIC_PUSHER_VAY)
-        return mode == 1 ? launchT<SUPP, PIC_PUSHER_VAY, 1>(a, nblocks, s) : launchT<SUPP, PIC_PUSHER_VAY, 0>(a, nblocks, s);
-    if (pusher == PIC_PUSHER_HIGUERA_CARY)
-        return mode == 1 ? launchT<SUPP, PIC_PUSHER_HIGUERA_CARY, 1>(a, nblocks, s)
-                         : launchT<SUPP, PIC_PUSHER_HIGUERA_CARY, 0>(a, nblocks, s);
-    return mode == 1 ? launchT<SUPP, PIC_PUSHER_BORIS, 1>(a, nblocks, s) : launchT<SUPP, PIC_PUSHER_BORIS, 0>(a, nblocks, s);
+    if (mode == 2) return launchT<SUPP, PIC_PUSHER_BORIS, 2, EMZ>(a, nblocks, s);
+    if (mode == 0) {  // no deposit: the current solver does not matter
+        if (pusher == PIC_PUSHER_VAY) return launchT<SUPP, PIC_PUSHER_VAY, 0, false>(a, nblocks, s);
+        if (pusher == PIC_PUSHER_HIGUERA_CARY) return launchT<SUPP, PIC_PUSHER_HIGUERA_CARY, 0, false>(a, nblocks, s);
+        return launchT<SUPP, PIC_PUSHER_BORIS, 0, false>(a, nblocks, s);
+    }
+    if (pusher == PIC_PUSHER_VAY) return launchT<SUPP, PIC_PUSHER_VAY, 1, EMZ>(a, nblocks, s);
+    if (pusher == PIC_PUSHER_HIGUERA_CARY) return launchT<SUPP, PIC_PUSHER_HIGUERA_CARY, 1, EMZ>(a, nblocks, s);
+    return launchT<SUPP, PIC_PUSHER_BORIS, 1, EMZ>(a, nblocks, s);
 }
 
-hipError_t launchPush(int supp, int pusher, int mode, const PushArgs& a, int nblocks, hipStream_t s) {
+hipError_t launchPush(int supp, int pusher, int mode, int currentSolver, const PushArgs& a, int nblocks,
+                      hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
+    bool const emz = currentSolver == PIC_CURRENT_SOLVER_EMZ;
     switch (supp) {
-        case 2: return launchS<2>(pusher, mode, a, nblocks, s);
-        case 3: return launchS<3>(pusher, mode, a, nblocks, s);
-        case 4: return launchS<4>(pusher, mode, a, nblocks, s);
+        case 2: return emz ? launchS<2, true>(pusher, mode, a, nblocks, s) : launchS<2, false>(pusher, mode, a, nblocks, s);
+        case 3: return emz ? launchS<3, true>(pusher, mode, a, nblocks, s) : launchS<3, false>(pusher, mode, a, nblocks, s);
+        case 4: return emz ? launchS<4, true>(pusher, mode, a, nblocks, s) : launchS<4, false>(pusher, mode, a, nblocks, s);
         default: return hipErrorInvalidValue;
     }
 }

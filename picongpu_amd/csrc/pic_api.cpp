@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -61,6 +62,8 @@ int validateConfig(const pic_config& c) {
         const pic_species_desc& d = c.species[s];
         if (d.shape < PIC_SHAPE_CIC || d.shape > PIC_SHAPE_PQS) return fail(PIC_E_INVALID, "unknown shape");
         if (d.pusher < PIC_PUSHER_NONE || d.pusher > PIC_PUSHER_HIGUERA_CARY) return fail(PIC_E_INVALID, "unknown pusher");
+        if (d.current_solver != PIC_CURRENT_SOLVER_ESIRKEPOV && d.current_solver != PIC_CURRENT_SOLVER_EMZ)
+            return fail(PIC_E_INVALID, "unknown current solver");
         if (d.frames_per_sc <= 0 || d.frames_per_sc > 255)
             return fail(PIC_E_INVALID, "frames_per_sc must be in [1,255]");
         if (!(d.mass_ratio > 0.0)) return fail(PIC_E_INVALID, "mass_ratio must be positive");
@@ -469,15 +472,15 @@ int checkSpecies(pic_ctx* c, int32_t s) {
     return PIC_OK;
 }
 
-// Mobile species grouped by (support, pusher), groups ordered by their first species.
+// Mobile species grouped by (support, pusher, current solver), groups ordered by their first species.
 std::vector<std::vector<int>> pushGroups(pic_ctx* c, int32_t species, bool byPusher) {
     std::vector<std::vector<int>> groups;
-    std::vector<std::pair<int, int>> keys;
+    std::vector<std::array<int, 3>> keys;
     for (int s = 0; s < static_cast<int>(c->sp.size()); ++s) {
         if (species >= 0 && s != species) continue;
         const SpeciesMem& m = c->sp[s];
         if (m.desc.pusher == PIC_PUSHER_NONE) continue;
-        std::pair<int, int> key{supportOf(m.desc.shape), byPusher ? m.desc.pusher : 0};
+        std::array<int, 3> key{supportOf(m.desc.shape), byPusher ? m.desc.pusher : 0, m.desc.current_solver};
         size_t gi = 0;
         for (; gi < keys.size(); ++gi)
             if (keys[gi] == key) break;
@@ -496,6 +499,7 @@ int runPush(pic_ctx* c, int32_t species, int mode, bool stageShift = false) {
     for (const auto& grp : groups) {
         int const supp = supportOf(c->sp[grp[0]].desc.shape);
         int const pusher = c->sp[grp[0]].desc.pusher;
+        int const scheme = c->sp[grp[0]].desc.current_solver;
         pic::PushArgs a = makePushArgs(c);
         a.nsp = static_cast<int>(grp.size());
         for (size_t i = 0; i < grp.size(); ++i) a.sp[i] = c->sp[grp[i]].dev;
@@ -505,7 +509,7 @@ int runPush(pic_ctx* c, int32_t species, int mode, bool stageShift = false) {
         if (mode == 0) {
             a.scList = c->dAll;
             TimerScope ts(c, PIC_TIMER_PUSH);
-            PIC_HIP(pic::launchPush(supp, pusher, 0, a, c->nsc, c->stream));
+            PIC_HIP(pic::launchPush(supp, pusher, 0, scheme, a, c->nsc, c->stream));
             continue;
         }
         PhasePlan* plan = nullptr;
@@ -514,7 +518,7 @@ int runPush(pic_ctx* c, int32_t species, int mode, bool stageShift = false) {
         for (size_t ph = 0; ph < plan->count.size(); ++ph) {
             a.scList = plan->dList + plan->offset[ph];
             TimerScope ts(c, PIC_TIMER_PUSH);
-            PIC_HIP(pic::launchPush(supp, pusher, mode, a, plan->count[ph], c->stream));
+            PIC_HIP(pic::launchPush(supp, pusher, mode, scheme, a, plan->count[ph], c->stream));
         }
     }
     return PIC_OK;
@@ -1319,8 +1323,10 @@ int pic_adjoint_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* 
         return fail(PIC_E_INVALID, "the differentiable form needs a periodic box without absorber or current filter");
     if (c->ncells > (size_t)INT32_MAX) return fail(PIC_E_INVALID, "the differentiable form needs fewer than 2^31 cells");
     const SpeciesMem& sm = c->sp[s];
-    if (sm.desc.shape != PIC_SHAPE_CIC || sm.desc.pusher != PIC_PUSHER_BORIS)
-        return fail(PIC_E_INVALID, "the differentiable form supports the CIC shape with the Boris pusher");
+    if (sm.desc.shape != PIC_SHAPE_CIC || sm.desc.pusher != PIC_PUSHER_BORIS ||
+        sm.desc.current_solver != PIC_CURRENT_SOLVER_ESIRKEPOV)
+        return fail(PIC_E_INVALID,
+                    "the differentiable form supports the CIC shape with the Boris pusher and Esirkepov current");
     int const ng[3] = {c->g.nx, c->g.ny, c->g.nz};
     double wmax = 0.0;
     for (int64_t i = 0; i < n; ++i) {
@@ -1785,6 +1791,8 @@ int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out) {
             int const sp = supportOf(c.species[s].shape);
             if ((supp && sp != supp) || (pusher && c.species[s].pusher != pusher))
                 return bail(fail(PIC_E_INVALID, "ensemble mobile species must share one shape and pusher"));
+            if (c.species[s].current_solver != PIC_CURRENT_SOLVER_ESIRKEPOV)
+                return bail(fail(PIC_E_INVALID, "ensemble species use the Esirkepov current solver"));
             supp = sp;
             pusher = c.species[s].pusher;
             if (c.species[s].frames_per_sc * pic::kSCV > pic::kListMaxSlots)

@@ -63,6 +63,8 @@ struct Shape<2> {  // CIC.hpp:38-161
         if (below_1) result = onSupport;
         return result;
     }
+    // ChargeAssignmentOnSupport::operator() (CIC.hpp:139-147)
+    __device__ __forceinline__ static float onSupportAt(float x) { return 1.0f - fabsf(x); }
 };
 
 template <>
@@ -96,6 +98,14 @@ struct Shape<3> {  // TSC.hpp:39-201
         else if (below_1_5)
             result = rad2;
         return result;
+    }
+    // ChargeAssignmentOnSupport::operator() (TSC.hpp:166-185)
+    __device__ __forceinline__ static float onSupportAt(float x) {
+        float const abs_x = fabsf(x);
+        bool const below_05 = abs_x < 0.5f;
+        float const rad1 = ff1(abs_x);
+        float const rad2 = ff2(abs_x);
+        return below_05 ? rad1 : rad2;
     }
 };
 
@@ -132,6 +142,14 @@ struct Shape<4> {  // PQS.hpp:38-205
         else if (below_2)
             result = rad2;
         return result;
+    }
+    // ChargeAssignmentOnSupport::operator() (PQS.hpp:166-185)
+    __device__ __forceinline__ static float onSupportAt(float x) {
+        float const abs_x = fabsf(x);
+        bool const below_1 = abs_x < 1.0f;
+        float const rad1 = ff1(abs_x);
+        float const rad2 = ff2(abs_x);
+        return below_1 ? rad1 : rad2;
     }
 };
 
@@ -540,6 +558,111 @@ struct Esirkepov {
     __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
                                                    Vec3 vel, float charge, const DepositConsts& k) {
         depositGeometry<TX, TY, TV, true>(jt, cx, cy, cz, geometry(pos, vel, k), charge, k);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// EmZ current deposition (currentDeposition/EmZ/EmZ.hpp:61-137,
+// EmZ/DepositCurrent.hpp:46-112): the move splits at the relay point
+// (relayPoint.hpp:47-64) into one or two virtual particles, each inside one
+// assignment cell, deposited with on-support shapes (Jit outer, Cached
+// middle and inner) over SUPP x SUPP x (SUPP-1) cells; same fixed-point tile
+// as Esirkepov.
+// ---------------------------------------------------------------------------
+template <int SUPP>
+struct EmZ {
+    static constexpr int begin = Shape<SUPP>::begin;
+    static constexpr int end = begin + SUPP;
+
+    template <int SI, int SJ, int SK>
+    __device__ __forceinline__ static void cpt1D(unsigned long long* __restrict__ jt, const float* p0, const float* p1,
+                                                 float csd, double jscale) {
+        if (p0[2] == p1[2]) return;
+        float s0j[SUPP], s1j[SUPP], s0k[SUPP], s1k[SUPP];
+        Shape<SUPP>::onSupport(p0[1], s0j);
+        Shape<SUPP>::onSupport(p1[1], s1j);
+        Shape<SUPP>::onSupport(p0[2], s0k);
+        Shape<SUPP>::onSupport(p1[2], s1k);
+#pragma unroll
+        for (int i = begin; i < end; ++i) {
+            float const s0i = Shape<SUPP>::onSupportAt((float)i - p0[0]);
+            float const dsi = Shape<SUPP>::onSupportAt((float)i - p1[0]) - s0i;
+#pragma unroll
+            for (int j = begin; j < end; ++j) {
+                float const sj0 = s0j[j - begin];
+                float const dsj = s1j[j - begin] - sj0;
+                float const tmp = -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
+                float accumulated_J = 0.0f;
+#pragma unroll
+                for (int k = begin; k < end - 1; ++k) {
+                    float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
+                    accumulated_J += W;
+                    __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
+                                           toFixed(accumulated_J, jscale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+    }
+
+    // one virtual particle from a to b (positions relative to shifted cell s)
+    template <int TX, int TY, int TV>
+    __device__ __forceinline__ static void virtualParticle(unsigned long long* __restrict__ jt, int cx, int cy, int cz,
+                                                           const int* s, const float* a, const float* b,
+                                                           float chargeDensity, const DepositConsts& k) {
+        constexpr int PX = 1, PY = TX, PZ = TX * TY;
+        unsigned long long* const corner = jt + ((cz + s[2] + begin) * TY + (cy + s[1] + begin)) * TX + (cx + s[0] + begin);
+        {  // J_x: rotation (1,2,0)
+            float const q0[3] = {a[1], a[2], a[0]};
+            float const q1[3] = {b[1], b[2], b[0]};
+            cpt1D<PY, PZ, PX>(corner, q0, q1, k.cell[0] * chargeDensity / k.dt, k.jscale);
+        }
+        {  // J_y: rotation (2,0,1)
+            float const q0[3] = {a[2], a[0], a[1]};
+            float const q1[3] = {b[2], b[0], b[1]};
+            cpt1D<PZ, PX, PY>(corner + TV, q0, q1, k.cell[1] * chargeDensity / k.dt, k.jscale);
+        }
+        cpt1D<PX, PY, PZ>(corner + 2 * TV, a, b, k.cell[2] * chargeDensity / k.dt, k.jscale);  // J_z
+    }
+
+    // Deposit one particle: (cx,cy,cz) its cell in J-tile coordinates, pos
+    // its in-cell position after the move, vel its velocity.
+    template <int TX, int TY, int TV>
+    __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
+                                                   Vec3 vel, float charge, const DepositConsts& k) {
+        constexpr bool isEven = (SUPP % 2) == 0;
+        float const p1[3] = {pos.x, pos.y, pos.z};
+        float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
+        float p0[3], relay[3];
+        int s0[3], s1[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            p0[d] = p1[d] - dp[d];
+            if (isEven) {
+                s0[d] = (int)floorf(p0[d]);
+                s1[d] = (int)floorf(p1[d]);
+                relay[d] = s0[d] == s1[d] ? p1[d] : (float)(s0[d] > s1[d] ? s0[d] : s1[d]);
+            } else {
+                s0[d] = (int)floorf(p0[d] + 0.5f);
+                s1[d] = (int)floorf(p1[d] + 0.5f);
+                relay[d] = s0[d] == s1[d] ? p1[d] : (float)(s0[d] + s1[d]) / 2.0f;
+            }
+        }
+        float const chargeDensity = charge / (k.cell[0] * k.cell[1] * k.cell[2]);
+        float a[3], b[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a[d] = p0[d] - (float)s0[d];
+            b[d] = relay[d] - (float)s0[d];
+        }
+        virtualParticle<TX, TY, TV>(jt, cx, cy, cz, s0, a, b, chargeDensity, k);
+        if (s0[0] != s1[0] || s0[1] != s1[1] || s0[2] != s1[2]) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                b[d] = p1[d] - (float)s1[d];
+                a[d] = relay[d] - (float)s1[d];
+            }
+            virtualParticle<TX, TY, TV>(jt, cx, cy, cz, s1, a, b, chargeDensity, k);
+        }
     }
 };
 

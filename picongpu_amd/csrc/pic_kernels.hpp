@@ -73,7 +73,9 @@ struct FieldArgs {
 
 // launchers (return hipError_t of the launch)
 // mode 0: push only, 1: push + fused deposit, 2: deposit only
-hipError_t launchPush(int supp, int pusher, int mode, const PushArgs& a, int nblocks, hipStream_t s);
+// currentSolver: PIC_CURRENT_SOLVER_* of the launch's species (one per launch)
+hipError_t launchPush(int supp, int pusher, int mode, int currentSolver, const PushArgs& a, int nblocks,
+                      hipStream_t s);
 // shift of one species: `staged` = the push epilogue already moved the leavers
 // to the outbox and compacted every supercell (then only the insert runs)
 hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, bool staged, hipStream_t s);

@@ -22,6 +22,14 @@ def small_thermal(shape=abi.SHAPE_CIC, pusher=abi.PUSHER_BORIS, grid=(16, 16, 12
     return st
 
 
+def emz(st, species=None):
+    """The EmZ current solver for the given species (all by default)."""
+    for i, s in enumerate(st.species):
+        if species is None or i in species:
+            s.current_solver = abi.CURRENT_SOLVER_EMZ
+    return st
+
+
 def open_box(st, boundary=(1, 0, 1), cells=((4, 6), (3, 5), (2, 4)), strength=((0.05, 0.1), (0.2, 0.03), (0.1, 0.1))):
     """Open borders on the given axes with the exponential absorber."""
     st.boundary = boundary
@@ -46,6 +54,9 @@ CASES = [
     ("pqs-vay", lambda: small_thermal(abi.SHAPE_PQS, abi.PUSHER_VAY)),
     ("cic-hc", lambda: small_thermal(abi.SHAPE_CIC, abi.PUSHER_HIGUERA_CARY, u=0.3)),
     ("tsc-hc", lambda: small_thermal(abi.SHAPE_TSC, abi.PUSHER_HIGUERA_CARY, u=0.3)),
+    ("cic-emz", lambda: emz(small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS, u=0.3))),
+    ("tsc-emz", lambda: emz(small_thermal(abi.SHAPE_TSC, abi.PUSHER_HIGUERA_CARY, u=0.3))),
+    ("pqs-emz", lambda: emz(small_thermal(abi.SHAPE_PQS, abi.PUSHER_VAY, u=0.3))),
 ]
 
 
@@ -63,7 +74,10 @@ def test_push_fused_bit_exact(name, make):
     sim.close()
 
 
-@pytest.mark.parametrize("name,make", CASES[:3], ids=[c[0] for c in CASES[:3]])
+DEPOSIT_CASES = CASES[:3] + CASES[-3:]
+
+
+@pytest.mark.parametrize("name,make", DEPOSIT_CASES, ids=[c[0] for c in DEPOSIT_CASES])
 def test_deposit_unfused_bit_exact(name, make):
     st = make()
     print(f"case deposit {name} grid={st.grid}", flush=True)
@@ -193,6 +207,8 @@ STEP_CASES = [
     ("open-xyz-tsc-vay-lehe", lambda: open_box(small_lehe(), boundary=(1, 1, 1))),
     ("open-y-kh-tsc", lambda: open_box(small_two_species(abi.SHAPE_TSC), boundary=(0, 1, 0))),
     ("binomial-kh-tsc", lambda: replace(small_two_species(abi.SHAPE_TSC), current_interpolation=abi.CURRENT_BINOMIAL)),
+    ("emz-kh-tsc-electrons", lambda: emz(small_two_species(abi.SHAPE_TSC), species=(0,))),
+    ("emz-open-cic", lambda: emz(open_box(small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS, u=0.3)))),
 ]
 
 
