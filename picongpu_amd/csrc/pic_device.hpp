@@ -468,7 +468,9 @@ struct Esirkepov {
                         for (int k = begin; k < end; ++k) {
                             if (k < end + leaveCellK - 1) {
                                 float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
-                                accumulated_J += W;
+                                // 0 + W differs from W only in the sign of a zero, which the
+                                // fixed-point conversion does not see: start the sum at W
+                                accumulated_J = k == begin ? W : accumulated_J + W;
                                 __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
                                                        toFixed(accumulated_J, jscale), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -596,7 +598,7 @@ struct EmZ {
 #pragma unroll
                 for (int k = begin; k < end - 1; ++k) {
                     float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
-                    accumulated_J += W;
+                    accumulated_J = k == begin ? W : accumulated_J + W;  // as in Esirkepov::cpt1D
                     __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
                                            toFixed(accumulated_J, jscale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
