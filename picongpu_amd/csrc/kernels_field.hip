@@ -473,6 +473,236 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
     }
 }
 
+// ---------------------------------------------------------------------------
+// Lehe: z-marching form. The Lehe curl of E (Lehe/Derivative.hpp:65-228)
+// reaches one cell below and two above along the Cherenkov-free axis and one
+// cell across it, so E is staged in a ring of four planes (k-1 .. k+2) with
+// an xy halo of two cells; B' keeps two planes as in the Yee form. The float
+// operations per cell are those of leheDeriv/kBHalf/kEFull/kAddCurrent.
+// ---------------------------------------------------------------------------
+constexpr int kLH = 2;
+constexpr int kLEX = kZX + 2 * kLH, kLEY = kZY + 2 * kLH;  // E plane: x, y in [-2, T+2)
+
+// staged field planes: P[k & 3][component][y + kLH][x + kLH]
+using LehePlanes = float[4][3][kLEY][kLEX];
+
+__device__ __forceinline__ float lAt(const LehePlanes& P, int c, int x, int y, int k) {
+    return P[k & 3][c][y + kLH][x + kLH];
+}
+
+// fwd/leheDeriv of component c along d at tile cell (x, y) of plane k
+__device__ __forceinline__ float lFwd(const LehePlanes& P, int c, int d, int x, int y, int k, float step) {
+    return (lAt(P, c, x + (d == 0), y + (d == 1), k + (d == 2)) - lAt(P, c, x, y, k)) / step;
+}
+
+__device__ __forceinline__ float lDeriv(const LehePlanes& P, int c, int d, int x, int y, int k, const FieldArgs& a) {
+    int const L = a.lehe_dir;
+    float const step = a.cell[d];
+    if (d == L) {
+        int const d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+        int const u1x = d1 == 0, u1y = d1 == 1, u1z = d1 == 2;
+        int const u2x = d2 == 0, u2y = d2 == 1, u2z = d2 == 2;
+        int const u0x = d == 0, u0y = d == 1, u0z = d == 2;
+        float r = a.lehe_alpha * lFwd(P, c, d, x, y, k, step);
+        r = r + a.lehe_beta1 * lFwd(P, c, d, x + u1x, y + u1y, k + u1z, step);
+        r = r + a.lehe_beta1 * lFwd(P, c, d, x - u1x, y - u1y, k - u1z, step);
+        r = r + a.lehe_beta2 * lFwd(P, c, d, x + u2x, y + u2y, k + u2z, step);
+        r = r + a.lehe_beta2 * lFwd(P, c, d, x - u2x, y - u2y, k - u2z, step);
+        r = r + a.lehe_delta * (lAt(P, c, x + 2 * u0x, y + 2 * u0y, k + 2 * u0z) - lAt(P, c, x - u0x, y - u0y, k - u0z)) /
+                    step;
+        return r;
+    }
+    int const ulx = L == 0, uly = L == 1, ulz = L == 2;
+    float const beta = 0.125f;
+    float const alpha = 1.0f - 2.0f * beta;
+    float r = alpha * lFwd(P, c, d, x, y, k, step);
+    r = r + beta * lFwd(P, c, d, x + ulx, y + uly, k + ulz, step);
+    r = r + beta * lFwd(P, c, d, x - ulx, y - uly, k - ulz, step);
+    return r;
+}
+
+__device__ __forceinline__ void lCurl(const LehePlanes& P, int x, int y, int k, const FieldArgs& a, float& cx, float& cy,
+                                      float& cz) {
+    cx = lDeriv(P, 2, 1, x, y, k, a) - lDeriv(P, 1, 2, x, y, k, a);
+    cy = lDeriv(P, 0, 2, x, y, k, a) - lDeriv(P, 2, 0, x, y, k, a);
+    cz = lDeriv(P, 1, 0, x, y, k, a) - lDeriv(P, 0, 1, x, y, k, a);
+}
+
+// per-thread wrapped xy offsets of the staged plane elements (-1: beyond an open border)
+template <bool OPEN, int N>
+__device__ __forceinline__ void lehePlaneOffsets(const GridDev& g, const ZTile& T, int (&off)[N]) {
+    for (int j = 0; j < N; ++j) {
+        int const t = threadIdx.x + 256 * j;
+        int const r = t % (kLEX * kLEY);
+        int const x = T.x0 + r % kLEX - kLH, y = T.y0 + r / kLEX - kLH;
+        off[j] = outsideIf<OPEN>(g, x, y, 0) ? -1 : wrap1(x, g.nx) + g.nx * wrap1(y, g.ny);
+    }
+}
+
+// this thread's elements of plane k of the three components of F (zero beyond an open border)
+template <bool OPEN, int N>
+__device__ __forceinline__ void lehePlaneFetch(const GridDev& g, float* const* F, const int (&off)[N], int k,
+                                               float (&v)[N]) {
+    size_t const zoff = (size_t)g.nx * g.ny * (size_t)wrap1(k, g.nz);
+    bool const zout = OPEN && (g.open & 4) && (k < 0 || k >= g.nz);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        int const t = threadIdx.x + 256 * j;
+        v[j] = 0.0f;
+        if (t < 3 * kLEX * kLEY && !(OPEN && (zout || off[j] < 0)))
+            v[j] = F[t / (kLEX * kLEY)][zoff + (size_t)off[j]];
+    }
+}
+
+// fetched elements into the ring slot of plane k
+template <int N>
+__device__ __forceinline__ void lehePlaneStore(const float (&v)[N], int k, LehePlanes& P) {
+    float* const dst = &P[k & 3][0][0][0];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        int const t = threadIdx.x + 256 * j;
+        if (t < 3 * kLEX * kLEY) dst[t] = v[j];
+    }
+}
+
+template <bool OPEN, int N>
+__device__ __forceinline__ void lehePlaneLoad(const GridDev& g, float* const* F, const int (&off)[N], int k,
+                                              LehePlanes& P) {
+    float v[N];
+    lehePlaneFetch<OPEN>(g, F, off, k, v);
+    lehePlaneStore(v, k, P);
+}
+
+struct LeheBeforeLds {
+    LehePlanes E;
+    float B[2][3][kZBY][kZBX];
+};
+
+// update_beforeCurrent + addCurrent for one z-marching tile (Lehe)
+template <bool OPEN>
+__device__ __forceinline__ void leheBeforeAddTile(const FieldArgs& a, float* const* Enew, int b, int zc,
+                                                  LeheBeforeLds& L) {
+    ZTile const T = zTile(a.g, b, zc);
+    int const nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+    size_t const plane = (size_t)nx * ny;
+    constexpr int NE = (3 * kLEX * kLEY + 255) / 256;
+    constexpr int NB = (kZBX * kZBY + 255) / 256;
+    int eoff[NE];
+    lehePlaneOffsets<OPEN>(a.g, T, eoff);
+    int boff[NB];
+    for (int j = 0; j < NB; ++j) {
+        int const t = threadIdx.x + 256 * j;
+        int const x = T.x0 + t % kZBX - 1, y = T.y0 + t / kZBX - 1;
+        boff[j] = outsideIf<OPEN>(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
+    }
+    float* const* const E = a.E;
+    lehePlaneLoad<OPEN>(a.g, E, eoff, T.z0 - 2, L.E);
+    lehePlaneLoad<OPEN>(a.g, E, eoff, T.z0 - 1, L.E);
+    lehePlaneLoad<OPEN>(a.g, E, eoff, T.z0, L.E);
+    int const lx = threadIdx.x % kZX, ly = threadIdx.x / kZX;
+    bool const inside = T.x0 + lx < nx && T.y0 + ly < ny;
+    size_t const ixy = (size_t)(T.x0 + lx) + (size_t)nx * (T.y0 + ly);
+    float next[NE];  // plane k+2, fetched while plane k-1 was being computed
+    lehePlaneFetch<OPEN>(a.g, E, eoff, T.z0 + 1, next);
+    for (int k = T.z0 - 1; k < T.z1; ++k) {
+        __syncthreads();  // the slot of plane k+2 (plane k-2) is no longer read
+        lehePlaneStore(next, k + 2, L.E);
+        if (k + 3 <= T.z1 + 1) lehePlaneFetch<OPEN>(a.g, E, eoff, k + 3, next);  // in flight during this step
+        __syncthreads();
+        {  // B' = B - curlE(E)*0.5*dt at plane k on x, y in [-1, T)
+            size_t const zoff = plane * (size_t)wrap1(k, nz);
+            bool const zout = OPEN && (a.g.open & 4) && (k < 0 || k >= nz);
+            int const s = k & 1;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                int const t = threadIdx.x + 256 * j;
+                if (t >= kZBX * kZBY) continue;
+                int const bx = t % kZBX, by = t / kZBX;  // cell (bx-1, by-1) of the tile
+                if (OPEN && (zout || boff[j] < 0)) {     // B beyond an open border stays zero
+                    L.B[s][0][by][bx] = L.B[s][1][by][bx] = L.B[s][2][by][bx] = 0.0f;
+                    continue;
+                }
+                float cx, cy, cz;
+                lCurl(L.E, bx - 1, by - 1, k, a, cx, cy, cz);
+                size_t const gi = zoff + (size_t)boff[j];
+                L.B[s][0][by][bx] = a.B[0][gi] - cx * 0.5f * a.dt;
+                L.B[s][1][by][bx] = a.B[1][gi] - cy * 0.5f * a.dt;
+                L.B[s][2][by][bx] = a.B[2][gi] - cz * 0.5f * a.dt;
+            }
+        }
+        __syncthreads();
+        if (k >= T.z0 && inside) {
+            int const s = k & 1, sm = (k - 1) & 1;
+            int const bx = lx + 1, by = ly + 1;
+            // backward differences of B' (BackwardDerivative.hpp:44-65), curl as kEFull
+            float const cx = (L.B[s][2][by][bx] - L.B[s][2][by - 1][bx]) / a.cell[1] -
+                             (L.B[s][1][by][bx] - L.B[sm][1][by][bx]) / a.cell[2];
+            float const cy = (L.B[s][0][by][bx] - L.B[sm][0][by][bx]) / a.cell[2] -
+                             (L.B[s][2][by][bx] - L.B[s][2][by][bx - 1]) / a.cell[0];
+            float const cz = (L.B[s][1][by][bx] - L.B[s][1][by][bx - 1]) / a.cell[0] -
+                             (L.B[s][0][by][bx] - L.B[s][0][by - 1][bx]) / a.cell[1];
+            size_t const i = ixy + plane * k;
+            float e0 = lAt(L.E, 0, lx, ly, k) + cx * a.c2 * a.dt;
+            float e1 = lAt(L.E, 1, lx, ly, k) + cy * a.c2 * a.dt;
+            float e2 = lAt(L.E, 2, lx, ly, k) + cz * a.c2 * a.dt;
+            e0 = e0 + a.coeff * a.J[0][i];
+            e1 = e1 + a.coeff * a.J[1][i];
+            e2 = e2 + a.coeff * a.J[2][i];
+            Enew[0][i] = e0;
+            Enew[1][i] = e1;
+            Enew[2][i] = e2;
+        }
+    }
+}
+
+struct LeheAfterLds {
+    LehePlanes E;
+    LehePlanes En;
+};
+
+// update_afterCurrent for one z-marching tile (Lehe): B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt
+template <bool OPEN>
+__device__ __forceinline__ void leheAfterTile(const FieldArgs& a, float* const* Enew, int b, int zc, LeheAfterLds& L) {
+    ZTile const T = zTile(a.g, b, zc);
+    int const nx = a.g.nx, ny = a.g.ny;
+    size_t const plane = (size_t)nx * ny;
+    constexpr int NE = (3 * kLEX * kLEY + 255) / 256;
+    int eoff[NE];
+    lehePlaneOffsets<OPEN>(a.g, T, eoff);
+    for (int k = T.z0 - 1; k <= T.z0 + 1; ++k) {
+        lehePlaneLoad<OPEN>(a.g, a.E, eoff, k, L.E);
+        lehePlaneLoad<OPEN>(a.g, Enew, eoff, k, L.En);
+    }
+    int const lx = threadIdx.x % kZX, ly = threadIdx.x / kZX;
+    bool const inside = T.x0 + lx < nx && T.y0 + ly < ny;
+    size_t const ixy = (size_t)(T.x0 + lx) + (size_t)nx * (T.y0 + ly);
+    float nextE[NE], nextN[NE];  // plane k+2, fetched while plane k-1 was being computed
+    lehePlaneFetch<OPEN>(a.g, a.E, eoff, T.z0 + 2, nextE);
+    lehePlaneFetch<OPEN>(a.g, Enew, eoff, T.z0 + 2, nextN);
+    for (int k = T.z0; k < T.z1; ++k) {
+        __syncthreads();  // the slot of plane k+2 (plane k-2) is no longer read
+        lehePlaneStore(nextE, k + 2, L.E);
+        lehePlaneStore(nextN, k + 2, L.En);
+        if (k + 3 <= T.z1 + 1) {  // in flight during this step
+            lehePlaneFetch<OPEN>(a.g, a.E, eoff, k + 3, nextE);
+            lehePlaneFetch<OPEN>(a.g, Enew, eoff, k + 3, nextN);
+        }
+        __syncthreads();
+        if (inside) {
+            size_t const i = ixy + plane * k;
+            float cx, cy, cz;
+            lCurl(L.E, lx, ly, k, a, cx, cy, cz);
+            float const b0 = a.B[0][i] - cx * 0.5f * a.dt;
+            float const b1 = a.B[1][i] - cy * 0.5f * a.dt;
+            float const b2 = a.B[2][i] - cz * 0.5f * a.dt;
+            lCurl(L.En, lx, ly, k, a, cx, cy, cz);
+            a.B[0][i] = b0 - cx * 0.5f * a.dt;
+            a.B[1][i] = b1 - cy * 0.5f * a.dt;
+            a.B[2][i] = b2 - cz * 0.5f * a.dt;
+        }
+    }
+}
+
 struct AfterLds {
     float E[3][kAV];
     float En[3][kAV];
@@ -519,6 +749,20 @@ __global__ __launch_bounds__(256) void kYeeBeforeAdd(FieldArgs a, float* Enew0, 
     __shared__ YeeBeforeLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
     yeeBeforeAddTile<OPEN>(a, En, blockIdx.x, zc, L);
+}
+
+template <bool OPEN>
+__global__ __launch_bounds__(256) void kLeheBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
+    __shared__ LeheBeforeLds L;
+    float* const En[3] = {Enew0, Enew1, Enew2};
+    leheBeforeAddTile<OPEN>(a, En, blockIdx.x, zc, L);
+}
+
+template <bool OPEN>
+__global__ __launch_bounds__(256) void kLeheAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
+    __shared__ LeheAfterLds L;
+    float* const En[3] = {Enew0, Enew1, Enew2};
+    leheAfterTile<OPEN>(a, En, blockIdx.x, zc, L);
 }
 
 template <bool OPEN>
@@ -733,9 +977,11 @@ static hipError_t fieldStepT(int solver, const FieldArgs& a, float* const Enew[3
     // (FDTDBase.hpp:151-183): between the two kernels and after the second
     bool const absorb = ab && ab->n > 0;
     int const tiles = (a.g.nx / kFTX) * (a.g.ny / kFTY) * (a.g.nz / kFTZ);
-    int const zc = solver == PIC_SOLVER_YEE ? yeeChunk(a.g) : 0;
+    int const zc = yeeChunk(a.g);  // both solvers z-march when the grid fills the chip
     int const zt = zc > 0 ? zTileCount(a.g, zc) : 0;
-    if (solver == PIC_SOLVER_LEHE)
+    if (solver == PIC_SOLVER_LEHE && zc > 0)
+        hipLaunchKernelGGL(kLeheBeforeAdd<OPEN>, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
+    else if (solver == PIC_SOLVER_LEHE)
         hipLaunchKernelGGL((kFieldBeforeAdd<PIC_SOLVER_LEHE, OPEN>), dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1],
                            Enew[2]);
     else if (zc == 0)  // grids too small for the z-marching form to fill the chip: 3D tiles of 8x8x4
@@ -747,7 +993,9 @@ static hipError_t fieldStepT(int solver, const FieldArgs& a, float* const Enew[3
         hipError_t const e = launchAbsorb(a.g, Enew, *ab, s);
         if (e != hipSuccess) return e;
     }
-    if (solver == PIC_SOLVER_LEHE)
+    if (solver == PIC_SOLVER_LEHE && zc > 0)
+        hipLaunchKernelGGL(kLeheAfter<OPEN>, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
+    else if (solver == PIC_SOLVER_LEHE)
         hipLaunchKernelGGL((kFieldAfter<PIC_SOLVER_LEHE, OPEN>), dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1],
                            Enew[2]);
     else if (zc == 0)

@@ -150,7 +150,9 @@ FUSED_FIELD_CASES = [((64, 64, 64), abi.SOLVER_YEE), ((200, 104, 96), abi.SOLVER
 FUSED_FIELD_CASES = [(g, s, o) for g, s in FUSED_FIELD_CASES for o in ("",)] + [
     ((64, 64, 64), abi.SOLVER_YEE, "open"), ((200, 104, 96), abi.SOLVER_YEE, "open"),
     ((256, 256, 68), abi.SOLVER_YEE, "open"), ((200, 104, 96), abi.SOLVER_LEHE, "open"),
-    ((64, 64, 64), abi.SOLVER_YEE, "binomial"), ((256, 256, 68), abi.SOLVER_YEE, "binomial")]
+    ((64, 64, 64), abi.SOLVER_YEE, "binomial"), ((256, 256, 68), abi.SOLVER_YEE, "binomial"),
+    ((256, 256, 68), abi.SOLVER_LEHE, "dir0"), ((200, 104, 96), abi.SOLVER_LEHE, "dir2"),
+    ((256, 256, 68), abi.SOLVER_LEHE, "open")]
 
 
 @pytest.mark.parametrize("grid,solver,variant", FUSED_FIELD_CASES,
@@ -167,7 +169,7 @@ def test_fused_field_step_equals_stage_calls(grid, solver, variant):
     if variant == "binomial":
         st.current_interpolation = abi.CURRENT_BINOMIAL
     st.solver = solver
-    st.lehe_dir = 1
+    st.lehe_dir = {"dir0": 0, "dir2": 2}.get(variant, 1)
     if solver == abi.SOLVER_LEHE:
         st.dt_si = 0.99 * st.cell_si[1] / setups.SPEED_OF_LIGHT_SI / 1.8
     print(f"case fused field step grid={grid} solver={solver} variant={variant}", flush=True)
