@@ -327,15 +327,23 @@ __device__ __forceinline__ void yeeBeforeAddTile(const FieldArgs& a, float* cons
     }
     auto planeOut = [&](int k) { return OPEN && (a.g.open & 4) && (k < 0 || k >= nz); };
     float* const Ebase = &L.E[0][0][0][0];
-    auto loadE = [&](int k) {  // E plane k (wrapped) into slot k & 1
+    auto fetchE = [&](int k, float (&v)[NE]) {  // this thread's elements of E plane k (wrapped)
         size_t const zoff = plane * (size_t)wrap1(k, nz);
         bool const zout = planeOut(k);
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            int const t = threadIdx.x + 256 * j;
+            v[j] = 0.0f;
+            if (t < 3 * kZEX * kZEY && !(OPEN && (zout || eoff[j] < 0)))
+                v[j] = a.E[t / (kZEX * kZEY)][zoff + (size_t)eoff[j]];
+        }
+    };
+    auto storeE = [&](int k, const float (&v)[NE]) {  // into slot k & 1
         float* const dst = Ebase + (k & 1) * (3 * kZEX * kZEY);
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             int const t = threadIdx.x + 256 * j;
-            if (t < 3 * kZEX * kZEY)
-                dst[t] = (OPEN && (zout || eoff[j] < 0)) ? 0.0f : a.E[t / (kZEX * kZEY)][zoff + (size_t)eoff[j]];
+            if (t < 3 * kZEX * kZEY) dst[t] = v[j];
         }
     };
     // B' = B - curlE(E)*0.5*dt at plane k on x, y in [-1, T): forward differences
@@ -372,10 +380,14 @@ __device__ __forceinline__ void yeeBeforeAddTile(const FieldArgs& a, float* cons
     int const lx = threadIdx.x % kZX, ly = threadIdx.x / kZX;
     bool const inside = T.x0 + lx < nx && T.y0 + ly < ny;
     size_t const ixy = (size_t)(T.x0 + lx) + (size_t)nx * (T.y0 + ly);
-    loadE(T.z0 - 1);
+    float next[NE];  // plane k+1, fetched while plane k-1 was being computed
+    fetchE(T.z0 - 1, next);
+    storeE(T.z0 - 1, next);
+    fetchE(T.z0, next);
     for (int k = T.z0 - 1; k < T.z1; ++k) {
         __syncthreads();  // slot (k+1)&1 is no longer read
-        loadE(k + 1);
+        storeE(k + 1, next);
+        if (k + 2 <= T.z1) fetchE(k + 2, next);  // in flight during this step
         __syncthreads();
         bPrime(k);
         __syncthreads();
@@ -425,29 +437,44 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
     }
     float* const Eb = &L.E[0][0][0][0];
     float* const Nb = &L.En[0][0][0][0];
-    auto load = [&](int k) {  // planes k of E and Enew on x, y in [0, T+1)
+    // this thread's elements of planes k of E and Enew on x, y in [0, T+1)
+    auto fetch = [&](int k, float (&ve)[NE], float (&vn)[NE]) {
         size_t const zoff = plane * (size_t)wrap1(k, nz);
         bool const zout = OPEN && (a.g.open & 4) && (k < 0 || k >= nz);
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            int const t = threadIdx.x + 256 * j;
+            ve[j] = vn[j] = 0.0f;
+            if (t < 3 * PX * PY && !(OPEN && (zout || eoff[j] < 0))) {
+                int const c = t / (PX * PY);
+                size_t const gi = zoff + (size_t)eoff[j];
+                ve[j] = a.E[c][gi];
+                vn[j] = Enew[c][gi];
+            }
+        }
+    };
+    auto store = [&](int k, const float (&ve)[NE], const float (&vn)[NE]) {
         int const so = (k & 1) * (3 * PX * PY);
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             int const t = threadIdx.x + 256 * j;
             if (t < 3 * PX * PY) {
-                int const c = t / (PX * PY);
-                bool const out = OPEN && (zout || eoff[j] < 0);
-                size_t const gi = zoff + (size_t)(out ? 0 : eoff[j]);
-                Eb[so + t] = out ? 0.0f : a.E[c][gi];
-                Nb[so + t] = out ? 0.0f : Enew[c][gi];
+                Eb[so + t] = ve[j];
+                Nb[so + t] = vn[j];
             }
         }
     };
     int const lx = threadIdx.x % kZX, ly = threadIdx.x / kZX;
     bool const inside = T.x0 + lx < nx && T.y0 + ly < ny;
     size_t const ixy = (size_t)(T.x0 + lx) + (size_t)nx * (T.y0 + ly);
-    load(T.z0);
+    float nextE[NE], nextN[NE];  // plane k+1, fetched while plane k-1 was being computed
+    fetch(T.z0, nextE, nextN);
+    store(T.z0, nextE, nextN);
+    fetch(T.z0 + 1, nextE, nextN);
     for (int k = T.z0; k < T.z1; ++k) {
         __syncthreads();
-        load(k + 1);
+        store(k + 1, nextE, nextN);
+        if (k + 2 <= T.z1) fetch(k + 2, nextE, nextN);  // in flight during this step
         __syncthreads();
         if (inside) {
             int const s = k & 1, s1 = (k + 1) & 1;
