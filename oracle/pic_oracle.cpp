@@ -307,8 +307,9 @@ int supportOf(int shape) { return shape == 1 ? 2 : (shape == 2 ? 3 : 4); }
 
 // ---- field gather: FieldToParticleInterpolation.hpp:100-124,
 // ShiftCoordinateSystem.hpp:53-79, AssignedTrilinearInterpolation.hpp:53-86
-template <typename T>
-T gather1(const State<T>& st, const std::vector<T>& F, int supp, int gx, int gy, int gz, const T* pos, const T* fpos) {
+template <typename T, bool OPEN>
+T gather1Impl(const State<T>& st, const std::vector<T>& F, int supp, int gx, int gy, int gz, const T* pos,
+              const T* fpos) {
     ShapeFns<T> sh{supp};
     bool const isEven = supp % 2 == 0;
     int const begin = -supp / 2 + (supp + 1) % 2;  // FieldToParticleInterpolation.hpp:60
@@ -332,12 +333,21 @@ T gather1(const State<T>& st, const std::vector<T>& F, int supp, int gx, int gy,
         for (int y = 0; y < supp; ++y) {
             T rx = T(0.0);
             for (int x = 0; x < supp; ++x)
-                rx += st.at(F, gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z) * sx[x];
+                rx += (OPEN ? st.at(F, gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)
+                            : F[st.idx(gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)]) *
+                      sx[x];
             ry += rx * sy[y];
         }
         rz += ry * sz[z];
     }
     return rz;
+}
+
+// the periodic form is kept free of border checks (they slow the loop down)
+template <typename T>
+T gather1(const State<T>& st, const std::vector<T>& F, int supp, int gx, int gy, int gz, const T* pos, const T* fpos) {
+    return st.anyOpen ? gather1Impl<T, true>(st, F, supp, gx, gy, gz, pos, fpos)
+                      : gather1Impl<T, false>(st, F, supp, gx, gy, gz, pos, fpos);
 }
 
 // ---- pushers ------------------------------------------------------------------
@@ -1001,17 +1011,28 @@ void shift(State<T>& st, int species) {
         if (species >= 0 && s != species) continue;
         Species<T>& S = st.sp[s];
         if (S.desc.pusher == 0) continue;
-        std::vector<std::vector<std::vector<Rec>>> out(st.nsc, std::vector<std::vector<Rec>>(27));
+        // per supercell: leaver records grouped by code (offsets off[sc][code .. code+1])
+        std::vector<std::vector<Rec>> out(st.nsc);
+        std::vector<std::array<uint32_t, 28>> off(st.nsc);
 #pragma omp parallel for schedule(static)
         for (int sc = 0; sc < st.nsc; ++sc) {
             size_t const base = static_cast<size_t>(sc) * S.cap;
             uint32_t const n = S.count[sc];
+            std::array<uint32_t, 28>& o = off[sc];
+            o.fill(0);
             std::vector<uint32_t> leavers;
-            for (uint32_t i = 0; i < n; ++i) {
-                int const m = S.mask[base + i];
-                if (m < 2) continue;
+            for (uint32_t i = 0; i < n; ++i)
+                if (S.mask[base + i] >= 2) {
+                    ++o[S.mask[base + i]];  // count of code m-1 at o[m]
+                    leavers.push_back(i);
+                }
+            uint32_t const L = static_cast<uint32_t>(leavers.size());
+            for (int code = 1; code < 28; ++code) o[code] += o[code - 1];  // o[code] = start of code
+            out[sc].resize(L);
+            std::array<uint32_t, 28> run = o;
+            for (uint32_t i : leavers) {
                 size_t const b = base + i;
-                Rec r;
+                Rec& r = out[sc][run[S.mask[b] - 1]++];
                 for (int c = 0; c < 3; ++c) {
                     r.pos[c] = S.pos[c][b];
                     r.mom[c] = S.mom[c][b];
@@ -1019,10 +1040,7 @@ void shift(State<T>& st, int species) {
                 r.w = S.w[b];
                 r.cell = S.cell[b];
                 r.id = S.id[b];
-                out[sc][m - 1].push_back(r);
-                leavers.push_back(i);
             }
-            uint32_t const L = static_cast<uint32_t>(leavers.size());
             if (L == 0) continue;
             uint32_t const nn = n - L;
             size_t q = 0;
@@ -1060,7 +1078,8 @@ void shift(State<T>& st, int species) {
                 int const sy = ((dy - ty) % st.nscy + st.nscy) % st.nscy;
                 int const sz = ((dz - tz) % st.nscz + st.nscz) % st.nscz;
                 int const src = sx + st.nscx * (sy + st.nscy * sz);
-                for (const Rec& r : out[src][code]) {
+                for (uint32_t q = off[src][code]; q < off[src][code + 1]; ++q) {
+                    const Rec& r = out[src][q];
                     if (run >= static_cast<uint32_t>(S.cap)) {
                         overflow = true;
                         ++run;
