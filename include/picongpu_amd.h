@@ -92,6 +92,20 @@ extern "C" {
 #define PIC_CURRENT_NONE 0
 #define PIC_CURRENT_BINOMIAL 1
 
+/* incident field (fields/incidentField): a profile per face of the Huygens
+ * surface (param/incidentField.param XMin..ZMax), NONE (profiles/None.def),
+ * GAUSSIAN_PULSE (profiles/GaussianPulse.hpp with GaussianPulseEnvelope) or
+ * PLANE_WAVE (profiles/PlaneWave.hpp); polarisation and focus origin as
+ * profiles/BaseParam.def:39-56 */
+#define PIC_INCIDENT_NONE 0
+#define PIC_INCIDENT_GAUSSIAN_PULSE 1
+#define PIC_INCIDENT_PLANE_WAVE 2
+#define PIC_POLARISATION_LINEAR 0
+#define PIC_POLARISATION_CIRCULAR 1
+#define PIC_ORIGIN_ZERO 0
+#define PIC_ORIGIN_CENTER 1
+#define PIC_INCIDENT_MAX_MODES 16
+
 /* fields */
 #define PIC_FIELD_E 0
 #define PIC_FIELD_B 1
@@ -115,6 +129,33 @@ typedef struct pic_species_desc {
     int32_t current_solver; /* PIC_CURRENT_SOLVER_*: the species' current<> flag */
 } pic_species_desc;
 
+/* One incident-field profile: the members of profiles::BaseParam
+ * (profiles/BaseParam.def:60-200) and of the GaussianPulse and PlaneWave
+ * parameter structs (GaussianPulse.def:36-70, PlaneWave.def:36-49), in SI.
+ * Zero-initialised = PIC_INCIDENT_NONE. */
+typedef struct pic_incident_profile {
+    int32_t kind;                      /* PIC_INCIDENT_* */
+    int32_t polarisation;              /* Polarisation: PIC_POLARISATION_* */
+    double wavelength_si;              /* WAVE_LENGTH_SI [m] */
+    double amplitude_si;               /* AMPLITUDE_SI [V/m] */
+    double pulse_duration_si;          /* PULSE_DURATION_SI [s] (sigma of the intensity) */
+    double time_delay_si;              /* TIME_DELAY_SI [s], >= 0 */
+    float laser_phase;                 /* LASER_PHASE [rad] */
+    double direction[3];               /* DIRECTION_X/Y/Z, unit vector */
+    double polarisation_direction[3];  /* POLARISATION_DIRECTION_X/Y/Z, unit, orthogonal to direction */
+    double focus_position_si[3];       /* FOCUS_POSITION_X/Y/Z_SI [m] */
+    int32_t focus_origin[3];           /* FOCUS_ORIGIN_X/Y/Z: PIC_ORIGIN_* */
+    /* GaussianPulse */
+    double w0_si;                      /* W0_SI [m] */
+    double pulse_init;                 /* PULSE_INIT */
+    int32_t num_modes;                 /* numModes: Laguerre modes 0..num_modes */
+    float laguerre_modes[PIC_INCIDENT_MAX_MODES];
+    float laguerre_phases[PIC_INCIDENT_MAX_MODES];
+    /* PlaneWave */
+    double laser_nofocus_constant_si;  /* LASER_NOFOCUS_CONSTANT_SI [s] */
+    double ramp_init;                  /* RAMP_INIT */
+} pic_incident_profile;
+
 /* Simulation description: the .param surface of the hot path
  * (param/simulation.param DELTA_T_SI/CELL_*_SI, param/density.param
  * BASE_DENSITY_SI, param/particle.param TYPICAL_PARTICLES_PER_CELL,
@@ -137,6 +178,12 @@ typedef struct pic_config {
                                     * at most PIC_ABSORBER_MAX_CELLS */
     float absorber_strength[3][2]; /* param/fieldAbsorber.param exponential::STRENGTH */
     int32_t current_interpolation; /* PIC_CURRENT_* */
+    /* incidentField.param XMin, XMax, YMin, YMax, ZMin, ZMax as [axis][negative, positive] */
+    pic_incident_profile incident[3][2];
+    /* incidentField.param POSITION[axis][negative, positive]: Huygens surface
+     * cell index from the negative border, and from the positive border when
+     * the entry is <= 0 (a positive entry is an absolute index) */
+    int32_t incident_position[3][2];
 } pic_config;
 
 /* FP32 constants in PIC units derived from a pic_config, computed as
@@ -256,17 +303,28 @@ int pic_push(pic_ctx* ctx, int32_t species, int32_t fuse_deposit);
 int pic_shift(pic_ctx* ctx, int32_t species);
 /* CurrentDeposition from the current particle state (unfused form) */
 int pic_deposit(pic_ctx* ctx, int32_t species);
-/* fields::Solver::update_beforeCurrent (FDTDBase.hpp:97-121): B half step, E full step */
+/* fields::Solver::update_beforeCurrent (FDTDBase.hpp:97-121): B half step,
+ * incident field on B (at the field step counter) and on E (counter + 0.5),
+ * E full step */
 int pic_fdtd_before_current(pic_ctx* ctx);
 /* CurrentInterpolationAndAdditionToEMF with currentInterpolation::None
  * (CurrentInterpolationAndAdditionToEMF.hpp:99-148, FDTD.hpp:78-86): E += -(dt/eps0) J */
 int pic_add_current(pic_ctx* ctx);
-/* fields::Solver::update_afterCurrent (FDTDBase.hpp:151-183): B half step */
+/* fields::Solver::update_afterCurrent (FDTDBase.hpp:151-183): absorber on E,
+ * incident field on B (counter + 1), B half step, absorber on B; then the
+ * field step counter advances by one */
 int pic_fdtd_after_current(pic_ctx* ctx);
 /* update_beforeCurrent, addCurrent and update_afterCurrent back to back, as
  * Simulation::runOneStep calls them (Simulation.hpp:522-542), in two fused
- * LDS-tiled kernels; E, B bit-identical to the three calls above */
+ * LDS-tiled kernels (the stage kernels when an incident field is set); E, B
+ * bit-identical to the three calls above */
 int pic_fdtd_step(pic_ctx* ctx);
+/* the field step counter: the currentStep argument of update_beforeCurrent /
+ * update_afterCurrent (FDTDBase.hpp:97,151), which times the incident field
+ * (incidentField/Functors.hpp BaseFunctorE: t = currentStep * dt). It starts
+ * at 0 and lives on the device, so captured steps advance it too. */
+int pic_set_step(pic_ctx* ctx, int64_t step);
+int pic_get_step(pic_ctx* ctx, int64_t* step);
 /* nsteps full PIC steps in the reference stage order, push and deposit fused */
 int pic_step(pic_ctx* ctx, int32_t nsteps);
 /* the same steps replayed from one captured hipGraph of a step (captured on

@@ -20,13 +20,53 @@ class OracleSpecies(C.Structure):
                 ("pusher", C.c_int32), ("frames_per_sc", C.c_int32), ("current_solver", C.c_int32)]
 
 
+class OracleIncidentProfile(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("polarisation", C.c_int32), ("wavelength_si", C.c_double),
+                ("amplitude_si", C.c_double), ("pulse_duration_si", C.c_double), ("time_delay_si", C.c_double),
+                ("laser_phase", C.c_float), ("direction", C.c_double * 3), ("polarisation_direction", C.c_double * 3),
+                ("focus_position_si", C.c_double * 3), ("focus_origin", C.c_int32 * 3), ("w0_si", C.c_double),
+                ("pulse_init", C.c_double), ("num_modes", C.c_int32), ("laguerre_modes", C.c_float * 16),
+                ("laguerre_phases", C.c_float * 16), ("laser_nofocus_constant_si", C.c_double),
+                ("ramp_init", C.c_double)]
+
+
+# parameter defaults of profiles::BaseParam / GaussianPulseParam / PlaneWaveParam
+# as the setups spell them (keyword names of the incident profile dicts)
+_INCIDENT_DEFAULTS = dict(focus_position_si=(0.0, 0.0, 0.0), focus_origin=(0, 0, 0), polarisation=0,
+                          time_delay_si=0.0, laser_phase=0.0, w0_si=0.0, pulse_init=20.0, laguerre_modes=(1.0,),
+                          laguerre_phases=None, laser_nofocus_constant_si=0.0, ramp_init=20.6146)
+
+
+def _fill_incident(dst, prof):
+    p = dict(_INCIDENT_DEFAULTS)
+    p.update(prof)
+    modes = list(p["laguerre_modes"])
+    phases = list(p["laguerre_phases"]) if p["laguerre_phases"] is not None else [0.0] * len(modes)
+    dst.kind = int(p["kind"])
+    dst.polarisation = int(p["polarisation"])
+    for name in ("wavelength_si", "amplitude_si", "pulse_duration_si", "time_delay_si", "w0_si", "pulse_init",
+                 "laser_nofocus_constant_si", "ramp_init"):
+        setattr(dst, name, float(p[name]))
+    dst.laser_phase = float(p["laser_phase"])
+    for d in range(3):
+        dst.direction[d] = float(p["direction"][d])
+        dst.polarisation_direction[d] = float(p["polarisation_direction"][d])
+        dst.focus_position_si[d] = float(p["focus_position_si"][d])
+        dst.focus_origin[d] = int(p["focus_origin"][d])
+    dst.num_modes = len(modes) - 1
+    for m in range(len(modes)):
+        dst.laguerre_modes[m] = float(modes[m])
+        dst.laguerre_phases[m] = float(phases[m])
+
+
 class OracleConfig(C.Structure):
     _fields_ = [("grid", C.c_int32 * 3), ("cell_si", C.c_double * 3), ("dt_si", C.c_double),
                 ("base_density_si", C.c_double), ("typical_ppc", C.c_int32), ("solver", C.c_int32),
                 ("lehe_dir", C.c_int32), ("n_species", C.c_int32), ("species", OracleSpecies * 4),
                 ("j_scale_exp", C.c_int32), ("device", C.c_int32), ("boundary", C.c_int32 * 3),
                 ("absorber", C.c_int32), ("absorber_cells", (C.c_int32 * 2) * 3),
-                ("absorber_strength", (C.c_float * 2) * 3), ("current_interpolation", C.c_int32)]
+                ("absorber_strength", (C.c_float * 2) * 3), ("current_interpolation", C.c_int32),
+                ("incident", (OracleIncidentProfile * 2) * 3), ("incident_position", (C.c_int32 * 2) * 3)]
 
 
 class OracleConstants(C.Structure):
@@ -77,6 +117,11 @@ def lib() -> C.CDLL:
             "oracle_add_current": (None, [_P]),
             "oracle_fdtd_after_current": (None, [_P]),
             "oracle_step": (C.c_int32, [_P, C.c_int32]),
+            "oracle_set_step": (None, [_P, C.c_int64]),
+            "oracle_incident_field": (None, [C.POINTER(OracleConfig), C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                             _D, C.c_int64, _D]),
+            "oracle_incident_info": (None, [C.POINTER(OracleConfig), C.c_int32, C.c_int32, _D]),
+            "oracle_get_step": (C.c_int64, [_P]),
             "oracle_field_energy": (None, [_P, _D]),
             "oracle_charge_density": (None, [_P, _D]),
             "oracle_div_e": (None, [_P, _D]),
@@ -124,6 +169,11 @@ def config_from(setup) -> OracleConfig:
             c.absorber_strength[d][side] = float(setup.absorber_strength[d][side])
     c.absorber = int(setup.absorber)
     c.current_interpolation = int(setup.current_interpolation)
+    for d in range(3):
+        for side in range(2):
+            c.incident_position[d][side] = int(setup.incident_position[d][side])
+    for (axis, side), prof in (setup.incident or {}).items():
+        _fill_incident(c.incident[axis][side], prof)
     return c
 
 
@@ -216,6 +266,12 @@ class Oracle:
     def update_after_current(self):
         lib().oracle_fdtd_after_current(self.h)
 
+    def set_step(self, step):
+        lib().oracle_set_step(self.h, int(step))
+
+    def get_step(self) -> int:
+        return int(lib().oracle_get_step(self.h))
+
     def step(self, n=1):
         r = lib().oracle_step(self.h, int(n))
         if r != 0:
@@ -278,3 +334,22 @@ def shape(shape_id, kind, x, flag=0):
     out = np.zeros(5, np.float32)
     lib().oracle_shape(shape_id, kind, float(x), int(flag), _p(out, C.c_float))
     return out
+
+
+def incident_field(setup, axis, side, which, current_step, pos):
+    """FP64 incident field (which 0: E, 1: B) of face (axis, side) at total
+    cell positions pos (n, 3) and source step current_step; returns (n, 3)."""
+    cfg = config_from(setup)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros_like(pos)
+    lib().oracle_incident_field(C.byref(cfg), axis, side, which, float(current_step), _p(pos, C.c_double),
+                                pos.shape[0], _p(out, C.c_double))
+    return out
+
+
+def incident_info(setup, axis, side):
+    """(phase velocity / c, origin[3], focus[3]) of face (axis, side), FP64, PIC units."""
+    cfg = config_from(setup)
+    out = np.zeros(7)
+    lib().oracle_incident_info(C.byref(cfg), axis, side, _p(out, C.c_double))
+    return out[0], out[1:4], out[4:7]

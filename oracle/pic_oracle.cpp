@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -57,6 +58,27 @@ typedef struct oracle_species {
     int32_t current_solver;  // 0 Esirkepov, 1 EmZ
 } oracle_species;
 
+typedef struct oracle_incident_profile {
+    int32_t kind;  // 0 none, 1 GaussianPulse, 2 PlaneWave
+    int32_t polarisation;  // 0 linear, 1 circular
+    double wavelength_si;
+    double amplitude_si;
+    double pulse_duration_si;
+    double time_delay_si;
+    float laser_phase;
+    double direction[3];
+    double polarisation_direction[3];
+    double focus_position_si[3];
+    int32_t focus_origin[3];  // 0 zero, 1 center
+    double w0_si;
+    double pulse_init;
+    int32_t num_modes;
+    float laguerre_modes[16];
+    float laguerre_phases[16];
+    double laser_nofocus_constant_si;
+    double ramp_init;
+} oracle_incident_profile;
+
 typedef struct oracle_config {
     int32_t grid[3];
     double cell_si[3];
@@ -74,6 +96,8 @@ typedef struct oracle_config {
     int32_t absorber_cells[3][2];   // fieldAbsorber.param NUM_CELLS [axis][negative, positive]
     float absorber_strength[3][2];  // fieldAbsorber.param exponential::STRENGTH
     int32_t current_interpolation;  // 0 None, 1 Binomial
+    oracle_incident_profile incident[3][2];  // incidentField.param XMin..ZMax as [axis][side]
+    int32_t incident_position[3][2];         // incidentField.param POSITION
 } oracle_config;
 
 typedef struct oracle_constants {
@@ -301,6 +325,7 @@ struct State {
                (cfg.boundary[2] && (z < 0 || z >= nz));
     }
     T at(const std::vector<T>& f, int x, int y, int z) const { return outside(x, y, z) ? T(0) : f[idx(x, y, z)]; }
+    int64_t step = 0;  // currentStep of the field solver calls
 };
 
 int supportOf(int shape) { return shape == 1 ? 2 : (shape == 2 ? 3 : 4); }
@@ -1244,12 +1269,510 @@ void absorb(State<T>& st, std::vector<T>* F) {
     }
 }
 
-// update_afterCurrent (FDTDBase.hpp:151-183): absorber on E, B half step, absorber on B
+// ---- incident field through the Huygens surface (fields/incidentField) ----
+// exp, cos and atan of the FP32 functors are taken as the FP64 function
+// rounded to T (the reference calls the platform's float_X functions, which
+// agree with this to an ulp or two).
+template <typename T>
+T incExp(T x) { return static_cast<T>(std::exp(static_cast<double>(x))); }
+template <typename T>
+T incCos(T x) { return static_cast<T>(std::cos(static_cast<double>(x))); }
+template <typename T>
+T incAtan(T x) { return static_cast<T>(std::atan(static_cast<double>(x))); }
+
+// DispersionRelation<Yee> (Yee.hpp:68-119) / <Lehe> (Lehe.hpp:77-215):
+// f(|k|) and df/d|k|, FP64
+template <typename T>
+struct Dispersion {
+    int solver, d0;
+    double dt, c, step[3], omega, dir[3];
+    double relation(double absK) const {
+        double const lhsTerm = std::sin(0.5 * omega * dt) / (c * dt);
+        double const lhs = lhsTerm * lhsTerm;
+        if (solver == 0) {
+            double rhs = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                double const term = std::sin(0.5 * absK * dir[d] * step[d]) / step[d];
+                rhs += term * term;
+            }
+            return rhs - lhs;
+        }
+        double sq[3], st2[3];
+        for (int d = 0; d < 3; ++d) {
+            double const s = std::sin(0.5 * absK * dir[d] * step[d]);
+            sq[d] = s * s;
+            st2[d] = step[d] * step[d];
+        }
+        int const d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
+        double const ratio = step[d0] / (c * dt);
+        double const co = ratio * std::sin(kPi / 2.0 * (c * dt) / step[d0]);
+        double const delta = 0.25 * (1.0 - co * co);
+        double const r1 = step[d0] / step[d1], r2 = step[d0] / step[d2];
+        double const b0 = 0.125, b1 = 0.125 * r1 * r1, b2 = 0.125 * r2 * r2;
+        double rhs = 0.0;
+        for (int d = 0; d < 3; ++d) rhs += sq[d] / st2[d];
+        rhs -= 4.0 * delta * sq[d0] * sq[d0] / st2[d0];
+        rhs -= 4.0 * (b1 / st2[d1] + b0 / st2[d0]) * sq[d0] * sq[d1];
+        rhs -= 4.0 * (b2 / st2[d2] + b0 / st2[d0]) * sq[d0] * sq[d2];
+        return rhs - lhs;
+    }
+    double derivative(double absK) const {
+        if (solver == 0) {
+            double r = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                double const arg = 0.5 * absK * dir[d] * step[d];
+                double const term = std::sin(arg) / step[d];
+                r += 2.0 * term * (0.5 * dir[d] * std::cos(arg));
+            }
+            return r;
+        }
+        double sq[3], sqd[3], st2[3];
+        for (int d = 0; d < 3; ++d) {
+            double const arg = 0.5 * absK * dir[d] * step[d];
+            double const s = std::sin(arg);
+            sqd[d] = 2.0 * s * (0.5 * dir[d] * step[d] * std::cos(arg));
+            double const s2 = std::sin(0.5 * absK * dir[d] * step[d]);
+            sq[d] = s2 * s2;
+            st2[d] = step[d] * step[d];
+        }
+        int const d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
+        double const ratio = step[d0] / (c * dt);
+        double const co = ratio * std::sin(kPi / 2.0 * (c * dt) / step[d0]);
+        double const delta = 0.25 * (1.0 - co * co);
+        double const r1 = step[d0] / step[d1];
+        double const b0 = 0.125, b1 = 0.125 * r1 * r1;
+        double r = 0.0;
+        for (int d = 0; d < 3; ++d) r += sqd[d] / st2[d];
+        r -= 4.0 * delta * sq[d0] * sqd[d0] / st2[d0];
+        r -= 4.0 * (b1 / st2[d1] + b0 / st2[d0]) * (sq[d0] * sqd[d1] + sqd[d0] * sq[d1]);
+        // Lehe.hpp:190-191 uses the dir1 coefficient for the dir2 term too
+        r -= 4.0 * (b1 / st2[d1] + b0 / st2[d0]) * (sq[d0] * sqd[d2] + sqd[d0] * sq[d2]);
+        return r;
+    }
+};
+
+// One profile: BaseFunctorE (Functors.hpp:80-230) with the constants of
+// BaseParamUnitless (BaseParam.hpp:40-140), GaussianPulseUnitless
+// (GaussianPulse.hpp:60-75) and PlaneWaveUnitless (PlaneWave.hpp:60-75)
+template <typename T>
+struct IncidentFunctor {
+    int kind = 0, polarisation = 0;
+    T cell[3], c, dt;
+    T WAVE_LENGTH, w, AMPLITUDE, PULSE_DURATION, LASER_PHASE, TIME_DELAY;
+    T DIR[3], POL[3], AX2[3];
+    T focus[3], origin[3];
+    T phaseVelocity;
+    T W0, rayleighLength, TIME_SHIFT;
+    int nModes = 1;
+    T modes[16], modePhases[16];
+    double RAMP_INIT = 0.0;
+    T LASER_NOFOCUS_CONSTANT;
+
+    void init(const oracle_config& cfg, const Consts<T>& k, const oracle_incident_profile& p) {
+        kind = p.kind;
+        polarisation = p.polarisation;
+        for (int d = 0; d < 3; ++d) cell[d] = k.cell[d];
+        c = k.c;
+        dt = k.dt;
+        double const uT = k.unit_time, uL = k.unit_length;
+        double const eUnit = 1.0 / (uT * uT / k.unit_mass / uL * k.unit_charge);  // simulation.unitless:461-464
+        WAVE_LENGTH = static_cast<T>(p.wavelength_si / uL);
+        T const f = static_cast<T>(c / WAVE_LENGTH);
+        w = T(2) * static_cast<T>(kPi) * f;
+        AMPLITUDE = static_cast<T>(p.amplitude_si / eUnit);
+        PULSE_DURATION = static_cast<T>(p.pulse_duration_si / uT);
+        LASER_PHASE = static_cast<T>(p.laser_phase);
+        T const delayParam = static_cast<T>(p.time_delay_si);
+        TIME_DELAY = static_cast<T>(static_cast<double>(delayParam) / uT);
+        for (int d = 0; d < 3; ++d) {
+            DIR[d] = static_cast<T>(p.direction[d]);
+            POL[d] = static_cast<T>(p.polarisation_direction[d]);
+            focus[d] = static_cast<T>(p.focus_position_si[d] / uL);
+        }
+        AX2[0] = DIR[1] * POL[2] - DIR[2] * POL[1];
+        AX2[1] = DIR[2] * POL[0] - DIR[0] * POL[2];
+        AX2[2] = DIR[0] * POL[1] - DIR[1] * POL[0];
+        for (int d = 0; d < 3; ++d)  // getFocus, Functors.hpp:158-175
+            if (p.focus_origin[d] == 1) focus[d] += static_cast<T>(static_cast<uint32_t>(cfg.grid[d]) / 2u) * cell[d];
+        T originP = -std::numeric_limits<T>::infinity();  // getOrigin, Functors.hpp:177-205
+        for (int a = 0; a < 3; ++a) {
+            if (std::abs(DIR[a]) > std::numeric_limits<T>::epsilon()) {
+                T const minPos = (static_cast<T>(cfg.incident_position[a][0]) + T(0.75)) * cell[a];
+                int const maxIdx = cfg.incident_position[a][1] > 0 ? cfg.incident_position[a][1]
+                                                                   : cfg.grid[a] + cfg.incident_position[a][1];
+                T const maxPos = (static_cast<T>(maxIdx) - T(0.75)) * cell[a];
+                T const axisP = std::min((minPos - focus[a]) / DIR[a], (maxPos - focus[a]) / DIR[a]);
+                originP = std::max(originP, axisP);
+            }
+        }
+        for (int d = 0; d < 3; ++d) origin[d] = focus[d] + originP * DIR[d];
+        // calculatePhaseVelocity + DispersionRelationSolver (FP64 Newton)
+        Dispersion<T> dr;
+        dr.solver = cfg.solver;
+        dr.d0 = cfg.lehe_dir;
+        dr.dt = static_cast<double>(dt);
+        dr.c = static_cast<double>(c);
+        for (int d = 0; d < 3; ++d) {
+            dr.step[d] = static_cast<double>(cell[d]);
+            dr.dir[d] = static_cast<double>(DIR[d]);
+        }
+        dr.omega = 2.0 * kPi * static_cast<double>(c / WAVE_LENGTH);
+        double absK = dr.omega * static_cast<double>(c);
+        for (int it = 0; it < 100; ++it) {
+            double const der = dr.derivative(absK);
+            if (std::abs(der) < 1e-5) break;
+            absK = absK - dr.relation(absK) / der;
+        }
+        phaseVelocity = static_cast<T>(dr.omega / absK / static_cast<double>(c));
+        if (kind == 1) {
+            W0 = static_cast<T>(p.w0_si / uL);
+            rayleighLength = static_cast<T>(kPi) * W0 * W0 / WAVE_LENGTH;
+            TIME_SHIFT = static_cast<T>(-0.5 * p.pulse_init * static_cast<double>(PULSE_DURATION));
+            nModes = p.num_modes + 1;
+            for (int m = 0; m < nModes; ++m) {
+                modes[m] = static_cast<T>(p.laguerre_modes[m]);
+                modePhases[m] = static_cast<T>(p.laguerre_phases[m]);
+            }
+        } else {
+            RAMP_INIT = p.ramp_init;
+            LASER_NOFOCUS_CONSTANT = static_cast<T>(p.laser_nofocus_constant_si / uT);
+        }
+    }
+
+    static T dot(const T* a, const T* b) {
+        T r = a[0] * b[0];
+        r += a[1] * b[1];
+        r += a[2] * b[2];
+        return r;
+    }
+    // getCurrentTime (Functors.hpp:103-110)
+    T currentTime(T timeOrigin, const T* x) const {
+        T sh[3];
+        for (int d = 0; d < 3; ++d) sh[d] = x[d] * cell[d] - origin[d];
+        T const distance = dot(sh, DIR);
+        T const delay = distance / phaseVelocity + TIME_DELAY;
+        return timeOrigin - delay;
+    }
+    // simpleLaguerre (GaussianPulse.hpp:230-250)
+    static T laguerre(int n, T x) {
+        if (n == 0) return T(1);
+        T prev = T(1), cur = T(1) - x, next = T(0);
+        for (int k = 1; k < n; ++k) {
+            next = ((T(2) * static_cast<T>(k) + T(1) - x) * cur - static_cast<T>(k) * prev) / static_cast<T>(k + 1);
+            prev = cur;
+            cur = next;
+        }
+        return cur;
+    }
+    // GaussianPulseFunctorIncidentE::getValue (GaussianPulse.hpp:130-228)
+    T gaussian(T timeOrigin, const T* x, T phaseShift) const {
+        T sh[3];
+        for (int d = 0; d < 3; ++d) sh[d] = x[d] * cell[d] - origin[d];
+        T pos[3] = {dot(sh, DIR), dot(sh, POL), dot(sh, AX2)};  // getInternalCoordinates
+        T time = currentTime(timeOrigin, x);
+        if (time < T(0)) return T(0);
+        time += TIME_SHIFT;
+        T const fo[3] = {focus[0] - origin[0], focus[1] - origin[1], focus[2] - origin[2]};
+        T const focusPos = dot(fo, DIR) - pos[0];
+        T const beamW = W0 * std::sqrt(T(1) + (focusPos / rayleighLength) * (focusPos / rayleighLength));
+        T const phase = w * (time - focusPos / c) + LASER_PHASE + phaseShift;
+        T const t0 = pos[0] * T(0), t1 = pos[1] * T(1), t2 = pos[2] * T(1);  // pos * planeNoNormal
+        T rho2 = t0 * t0;
+        rho2 += t1 * t1;
+        rho2 += t2 * t2;
+        T const Rinv = -focusPos / (rayleighLength * rayleighLength + focusPos * focusPos);
+        T const xi = incAtan(-focusPos / rayleighLength);
+        T const r2OverW2 = rho2 / beamW / beamW;
+        T const r = T(0.5) * rho2 * Rinv;
+        T const twoPi = T(2) * static_cast<T>(kPi);
+        T etrans = T(0);
+        for (int m = 0; m < nModes; ++m)
+            etrans += modes[m] * laguerre(m, T(2) * r2OverW2) * incExp(-r2OverW2) *
+                      incCos(twoPi / WAVE_LENGTH * focusPos - twoPi / WAVE_LENGTH * r +
+                             (T(2) * static_cast<T>(m) + T(1)) * xi + phase + modePhases[m]);
+        T const shiftedTime = time - r / c;
+        T const ex = shiftedTime / (T(2) * PULSE_DURATION);  // GaussianPulseEnvelope::getEnvelope
+        etrans *= incExp(-ex * ex);
+        T norm = T(0);
+        for (int m = 0; m < nModes; ++m) norm += modes[m];
+        T env = AMPLITUDE;
+        env *= W0 / beamW;
+        return env * etrans / norm;
+    }
+    // PlaneWaveFunctorIncidentE::getLongitudinal (PlaneWave.hpp:90-125); the
+    // FP64 RAMP_INIT makes the ramp arithmetic FP64
+    T planeLongitudinal(T time, T phaseShift) const {
+        T env = AMPLITUDE;
+        double const mue = 0.5 * RAMP_INIT * static_cast<double>(PULSE_DURATION);
+        T const tau = PULSE_DURATION * std::sqrt(T(2));
+        double const up = mue, down = mue + static_cast<double>(LASER_NOFOCUS_CONSTANT);
+        T corr = T(0);
+        double const t = static_cast<double>(time);
+        if (t > down) {
+            double const e = (t - down) / static_cast<double>(tau);
+            env = static_cast<T>(static_cast<double>(env) * std::exp(-0.5 * e * e));
+            corr = static_cast<T>((t - down) / static_cast<double>(w * tau * tau));
+        } else if (t < up) {
+            double const e = (t - up) / static_cast<double>(tau);
+            env = static_cast<T>(static_cast<double>(env) * std::exp(-0.5 * e * e));
+            corr = static_cast<T>((t - up) / static_cast<double>(w * tau * tau));
+        }
+        double const ph = static_cast<double>(w) * (t - up) + static_cast<double>(LASER_PHASE) +
+                          static_cast<double>(phaseShift);
+        return static_cast<T>((std::sin(ph) + std::cos(ph) * static_cast<double>(corr)) * static_cast<double>(env));
+    }
+    // E_inc (GaussianPulse operator(), GaussianPulse.hpp:112-126; separable
+    // functor operator(), Functors.hpp:262-283)
+    void fieldE(T timeOrigin, const T* x, T* e) const {
+        T const halfPi = static_cast<T>(kPi) / T(2);
+        T cp1[3], cp2[3];
+        T const root2 = std::sqrt(T(2));
+        for (int d = 0; d < 3; ++d) cp1[d] = POL[d] / root2;
+        cp2[0] = DIR[1] * cp1[2] - DIR[2] * cp1[1];
+        cp2[1] = DIR[2] * cp1[0] - DIR[0] * cp1[2];
+        cp2[2] = DIR[0] * cp1[1] - DIR[1] * cp1[0];
+        T v1, v2;
+        if (kind == 1) {
+            if (polarisation == 0) {
+                v1 = gaussian(timeOrigin, x, T(0));
+                for (int d = 0; d < 3; ++d) e[d] = POL[d] * v1;
+                return;
+            }
+            v1 = gaussian(timeOrigin, x, halfPi);
+            v2 = gaussian(timeOrigin, x, T(0));
+        } else {
+            T const time = currentTime(timeOrigin, x);
+            if (time < T(0)) {
+                e[0] = e[1] = e[2] = T(0);
+                return;
+            }
+            if (polarisation == 0) {
+                v1 = planeLongitudinal(time, T(0)) * T(1);
+                for (int d = 0; d < 3; ++d) e[d] = POL[d] * v1;
+                return;
+            }
+            v1 = planeLongitudinal(time, halfPi) * T(1);
+            v2 = planeLongitudinal(time, T(0)) * T(1);
+        }
+        for (int d = 0; d < 3; ++d) e[d] = cp1[d] * v1 + cp2[d] * v2;
+    }
+    // ApproximateIncidentB: cross(direction, E_inc) / c (Functors.hpp:540-570)
+    void fieldB(T timeOrigin, const T* x, T* b) const {
+        T e[3];
+        fieldE(timeOrigin, x, e);
+        b[0] = (DIR[1] * e[2] - DIR[2] * e[1]) / c;
+        b[1] = (DIR[2] * e[0] - DIR[0] * e[2]) / c;
+        b[2] = (DIR[0] * e[1] - DIR[1] * e[0]) / c;
+    }
+};
+
+// DerivativeCoefficients (DerivativeCoefficients.hpp:35-200) of the curl the
+// updated field's solver step uses along `axis`: the E update reads curl B
+// (Yee's derivative for both solvers, Lehe.def:73), the B update curl E
+// (Lehe's derivative under Lehe)
+template <typename T>
+void derivativeCoefficients(const State<T>& st, bool curlE, int axis, T value[2][2][2], int size[3]) {
+    for (int i = 0; i < 8; ++i) (&value[0][0][0])[i] = T(0);
+    size[0] = size[1] = size[2] = 1;
+    if (!curlE || st.cfg.solver != 1) {
+        value[0][0][0] = T(1);
+        return;
+    }
+    int const dir = st.cfg.lehe_dir;
+    const Consts<T>& k = st.k;
+    if (dir == axis) {
+        int const d0 = axis, d1 = (axis + 1) % 3, d2 = (axis + 2) % 3;
+        double const stepRatio = static_cast<double>(k.cell[d0] / (k.c * k.dt));
+        double const coeff = stepRatio * std::sin(kPi / 2.0 * static_cast<double>(k.c) * static_cast<double>(k.dt) /
+                                                  static_cast<double>(k.cell[d0]));
+        T const delta = static_cast<T>(0.25 * (1.0 - coeff * coeff));
+        double const sr1 = static_cast<double>(k.cell[d0] / k.cell[d1]);
+        double const sr2 = static_cast<double>(k.cell[d0] / k.cell[d2]);
+        double const betaDir1 = 0.125 * sr1 * sr1;
+        double const betaDir2 = 0.125 * sr2 * sr2;
+        size[0] = size[1] = size[2] = 2;
+        value[0][0][0] = static_cast<T>(1.0 - 2.0 * betaDir1 - 2.0 * betaDir2 - 3.0 * static_cast<double>(delta));
+        int a[3] = {0, 0, 0};
+        a[d0] = 1;
+        value[a[0]][a[1]][a[2]] = delta;
+        a[d0] = 0;
+        a[d1] = 1;
+        value[a[0]][a[1]][a[2]] = static_cast<T>(betaDir1);
+        a[d1] = 0;
+        a[d2] = 1;
+        value[a[0]][a[1]][a[2]] = static_cast<T>(betaDir2);
+    } else {
+        T const beta = T(0.125);
+        size[dir] = 2;
+        value[0][0][0] = T(1) - T(2) * beta;
+        int a[3] = {0, 0, 0};
+        a[dir] = 1;
+        value[a[0]][a[1]][a[2]] = beta;
+    }
+}
+
+// updateField (Solver.hpp:120-258) with UpdateFunctor (Solver.kernel:60-262)
+// for one face; updateB: the B half update from E_inc, else the E update from B_inc
+template <typename T>
+void incidentFace(State<T>& st, int axis, int side, bool updateB, T currentStep, T timeIncrement) {
+    const oracle_config& cfg = st.cfg;
+    const oracle_incident_profile& prof = cfg.incident[axis][side];
+    if (prof.kind == 0) return;
+    IncidentFunctor<T> F;
+    F.init(cfg, st.k, prof);
+    T const direction = side == 0 ? T(1) : T(-1);
+    T const curlCoefficient = updateB ? -timeIncrement : timeIncrement * (st.k.c * st.k.c);
+    bool const isTotal = !updateB;  // E_x at (0.5, 0, 0) is a total field, B_x at (0, .5, .5) is not
+    int n[3] = {st.nx, st.ny, st.nz};
+    int minB[3], maxB[3];
+    for (int d = 0; d < 3; ++d) {
+        minB[d] = cfg.incident_position[d][0];
+        maxB[d] = cfg.incident_position[d][1] > 0 ? cfg.incident_position[d][1] : n[d] + cfg.incident_position[d][1];
+    }
+    int beginUser[3], endUser[3];
+    for (int d = 0; d < 3; ++d) {
+        beginUser[d] = minB[d] + 1;
+        endUser[d] = maxB[d];
+    }
+    if (!isTotal && direction > T(0)) beginUser[axis] -= 1;
+    bool lastDomain[3];
+    bool const contiguous = prof.kind == 2;  // MakePeriodicTransversalHuygensSurfaceContiguous<PlaneWave>
+    for (int d = 0; d < 3; ++d) {
+        if (contiguous && cfg.boundary[d] == 0 && d != axis) {
+            beginUser[d] = 0;
+            endUser[d] = n[d];
+            lastDomain[d] = false;
+        } else {
+            lastDomain[d] = true;
+        }
+    }
+    T coeff[2][2][2];
+    int size[3];
+    derivativeCoefficients(st, updateB, axis, coeff, size);
+    int const margin = size[axis];
+    if (direction > T(0)) {
+        beginUser[axis] -= margin - 1;
+        endUser[axis] = beginUser[axis] + 2 * margin - 1;
+    } else {
+        endUser[axis] += margin - 1;
+        beginUser[axis] = endUser[axis] - (2 * margin - 1);
+    }
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = std::max(beginUser[d], 0);
+        hi[d] = std::min(endUser[d], n[d]);
+        if (lo[d] >= hi[d]) return;
+    }
+    T baseShift[3] = {T(0), T(0), T(0)};
+    if (direction > T(0)) baseShift[axis] = isTotal ? T(-1) : T(1);
+    // Yee positions of the incident field's components (YeeCell.hpp:69-141)
+    static const double yeeE[3][3] = {{0.5, 0, 0}, {0, 0.5, 0}, {0, 0, 0.5}};
+    static const double yeeB[3][3] = {{0, 0.5, 0.5}, {0.5, 0, 0.5}, {0.5, 0.5, 0}};
+    const double(*ypos)[3] = updateB ? yeeE : yeeB;
+    int const comp1 = (axis + 2) % 3, comp2 = (axis + 1) % 3;  // incidentComponent1 / 2
+    int const dir1 = comp2, dir2 = comp1;
+    T inCell1[3], inCell2[3];
+    for (int d = 0; d < 3; ++d) {
+        inCell1[d] = baseShift[d] + static_cast<T>(ypos[comp1][d]);
+        inCell2[d] = baseShift[d] + static_cast<T>(ypos[comp2][d]);
+    }
+    T const sign = direction > T(0) ? T(1) : T(-1);
+    T const baseCoefficient = curlCoefficient / st.k.cell[axis] * sign;
+    T const timeOrigin = currentStep * st.k.dt;
+    std::vector<T>* dest = updateB ? st.B : st.E;
+    int const huygens = lo[axis] + margin - 1;
+    int const sizeDir1 = size[dir1], sizeDir2 = size[dir2];
+    auto incident = [&](const T* x, int comp) {
+        T v[3];
+        if (updateB)
+            F.fieldE(timeOrigin, x, v);
+        else
+            F.fieldB(timeOrigin, x, v);
+        return v[comp];
+    };
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int z = lo[2]; z < hi[2]; ++z)
+        for (int y = lo[1]; y < hi[1]; ++y)
+            for (int x = lo[0]; x < hi[0]; ++x) {
+                int const cellIdx[3] = {x, y, z};
+                bool last[3];
+                for (int d = 0; d < 3; ++d) last[d] = lastDomain[d] && cellIdx[d] == hi[d] - 1;
+                int surf[3] = {x, y, z};
+                surf[axis] = huygens;
+                T s1[3], s2[3];
+                for (int d = 0; d < 3; ++d) {
+                    s1[d] = static_cast<T>(surf[d]) + inCell1[d];
+                    s2[d] = static_cast<T>(surf[d]) + inCell2[d];
+                }
+                int const shiftAlong = cellIdx[axis] - huygens;
+                bool const apply1 = (isTotal && !last[comp2]) || (!isTotal && !last[comp1]);
+                bool const apply2 = (isTotal && !last[comp1]) || (!isTotal && !last[comp2]);
+                T result[3] = {T(0), T(0), T(0)};
+                T idxShift[3] = {T(0), T(0), T(0)};
+                T initial = static_cast<T>(-shiftAlong);
+                if (isTotal && static_cast<T>(shiftAlong) * direction > T(0)) initial += direction;
+                if (!isTotal && static_cast<T>(shiftAlong) * direction < T(0)) initial -= direction;
+                idxShift[axis] = initial;
+                T const increment = shiftAlong < 0 ? T(1) : (shiftAlong > 0 ? T(-1) : (isTotal ? -direction : direction));
+                for (int as = std::abs(shiftAlong); as < margin; ++as) {
+                    int ci[3] = {0, 0, 0};
+                    ci[axis] = as;
+                    idxShift[dir1] = static_cast<T>(1 - sizeDir1);
+                    for (int a1 = 1 - sizeDir1; a1 < sizeDir1; ++a1) {
+                        ci[dir1] = std::abs(a1);
+                        idxShift[dir2] = static_cast<T>(1 - sizeDir2);
+                        for (int a2 = 1 - sizeDir2; a2 < sizeDir2; ++a2) {
+                            ci[dir2] = std::abs(a2);
+                            T const dc = coeff[ci[0]][ci[1]][ci[2]];
+                            if (apply1) {
+                                T const p[3] = {s1[0] + idxShift[0], s1[1] + idxShift[1], s1[2] + idxShift[2]};
+                                result[dir1] += dc * incident(p, comp1);
+                            }
+                            if (apply2) {
+                                T const p[3] = {s2[0] + idxShift[0], s2[1] + idxShift[1], s2[2] + idxShift[2]};
+                                result[dir2] += dc * incident(p, comp2);
+                            }
+                            idxShift[dir2] += T(1);
+                        }
+                        idxShift[dir1] += T(1);
+                    }
+                    idxShift[axis] += increment;
+                }
+                result[dir1] *= baseCoefficient;
+                result[dir2] *= -baseCoefficient;
+                size_t const at = st.idx(x, y, z);
+                for (int d = 0; d < 3; ++d) dest[d][at] += result[d];
+            }
+}
+
+// Solver::updateBHalf / updateE (Solver.hpp:270-330): faces x, y, z, the
+// negative border's profile before the positive one's
+template <typename T>
+void incidentUpdate(State<T>& st, bool updateB, T currentStep) {
+    T const timeIncrement = updateB ? T(0.5) * st.k.dt : st.k.dt;
+    for (int axis = 0; axis < 3; ++axis)
+        for (int side = 0; side < 2; ++side) incidentFace(st, axis, side, updateB, currentStep, timeIncrement);
+}
+
+// update_beforeCurrent (FDTDBase.hpp:97-121)
+template <typename T>
+void beforeCurrent(State<T>& st) {
+    bHalf(st);
+    T const cs = static_cast<T>(st.step);
+    incidentUpdate(st, true, cs);
+    incidentUpdate(st, false, cs + T(0.5) * st.k.dt / st.k.dt);
+    eFull(st);
+}
+
+// update_afterCurrent (FDTDBase.hpp:151-183): absorber on E, incident B,
+// B half step, absorber on B; the step counter then advances
 template <typename T>
 void afterCurrent(State<T>& st) {
     absorb(st, st.E);
+    incidentUpdate(st, true, static_cast<T>(st.step) + st.k.dt / st.k.dt);
     bHalf(st);
     absorb(st, st.B);
+    ++st.step;
 }
 
 template <typename T>
@@ -1259,8 +1782,7 @@ void step(State<T>& st, int n) {
         bool const fused = st.order == 0 && std::is_same<T, float>::value;
         push(st, -1, fused);  // ParticlePush
         shift(st, -1);
-        bHalf(st);  // update_beforeCurrent
-        eFull(st);
+        beforeCurrent(st);  // update_beforeCurrent
         if (!fused) deposit(st, -1);  // CurrentDeposition
         addCurrent(st);  // CurrentInterpolationAndAdditionToEMF
         afterCurrent(st);  // update_afterCurrent
@@ -1408,6 +1930,43 @@ void oracle_set_jexp(void* p, int32_t e) {
                    [&](State<double>* s) { s->jexp = e; });
 }
 
+// The incident field of face (axis, side) at n total-cell positions pos[3n]
+// and source step `current_step` (time current_step * dt), FP64 functor:
+// which = 0 E_inc, 1 B_inc (ApproximateIncidentB); out[3n]
+void oracle_incident_field(const oracle_config* cfg, int32_t axis, int32_t side, int32_t which, double current_step,
+                           const double* pos, int64_t n, double* out) {
+    Consts<double> k = makeConsts<double>(*cfg);
+    IncidentFunctor<double> F;
+    F.init(*cfg, k, cfg->incident[axis][side]);
+    double const t0 = current_step * k.dt;
+    for (int64_t i = 0; i < n; ++i) {
+        if (which == 0)
+            F.fieldE(t0, pos + 3 * i, out + 3 * i);
+        else
+            F.fieldB(t0, pos + 3 * i, out + 3 * i);
+    }
+}
+
+// phase velocity (units of c) and origin (PIC units) of face (axis, side), FP64
+void oracle_incident_info(const oracle_config* cfg, int32_t axis, int32_t side, double* out) {
+    Consts<double> k = makeConsts<double>(*cfg);
+    IncidentFunctor<double> F;
+    F.init(*cfg, k, cfg->incident[axis][side]);
+    out[0] = F.phaseVelocity;
+    for (int d = 0; d < 3; ++d) out[1 + d] = F.origin[d];
+    for (int d = 0; d < 3; ++d) out[4 + d] = F.focus[d];
+}
+
+void oracle_set_step(void* p, int64_t step) {
+    dispatch<void>(static_cast<Handle*>(p), [&](State<float>* s) { s->step = step; },
+                   [&](State<double>* s) { s->step = step; });
+}
+
+int64_t oracle_get_step(void* p) {
+    return dispatch<int64_t>(static_cast<Handle*>(p), [&](State<float>* s) { return s->step; },
+                             [&](State<double>* s) { return s->step; });
+}
+
 int32_t oracle_set_threads(int32_t n) {
 #ifdef _OPENMP
     if (n > 0) omp_set_num_threads(n);
@@ -1522,10 +2081,7 @@ int32_t oracle_shift(void* p, int32_t species) {
 }
 
 void oracle_fdtd_before_current(void* p) {
-    auto impl = [&](auto* st) {
-        bHalf(*st);
-        eFull(*st);
-    };
+    auto impl = [&](auto* st) { beforeCurrent(*st); };
     dispatch<void>(static_cast<Handle*>(p), impl, impl);
 }
 

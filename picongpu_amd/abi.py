@@ -23,6 +23,10 @@ ABSORBER_NONE, ABSORBER_EXPONENTIAL = 0, 1
 CURRENT_NONE, CURRENT_BINOMIAL = 0, 1
 CURRENT_SOLVER_ESIRKEPOV, CURRENT_SOLVER_EMZ = 0, 1
 SOLVER_YEE, SOLVER_LEHE = 0, 1
+INCIDENT_NONE, INCIDENT_GAUSSIAN_PULSE, INCIDENT_PLANE_WAVE = 0, 1, 2
+POLARISATION_LINEAR, POLARISATION_CIRCULAR = 0, 1
+ORIGIN_ZERO, ORIGIN_CENTER = 0, 1
+INCIDENT_MAX_MODES = 16
 FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
 TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2
 
@@ -38,6 +42,64 @@ class SpeciesDesc(C.Structure):
         ("frames_per_sc", C.c_int32),
         ("current_solver", C.c_int32),
     ]
+
+
+class IncidentProfile(C.Structure):
+    """pic_incident_profile: one face's profile of incidentField.param, in SI."""
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("polarisation", C.c_int32),
+        ("wavelength_si", C.c_double),
+        ("amplitude_si", C.c_double),
+        ("pulse_duration_si", C.c_double),
+        ("time_delay_si", C.c_double),
+        ("laser_phase", C.c_float),
+        ("direction", C.c_double * 3),
+        ("polarisation_direction", C.c_double * 3),
+        ("focus_position_si", C.c_double * 3),
+        ("focus_origin", C.c_int32 * 3),
+        ("w0_si", C.c_double),
+        ("pulse_init", C.c_double),
+        ("num_modes", C.c_int32),
+        ("laguerre_modes", C.c_float * INCIDENT_MAX_MODES),
+        ("laguerre_phases", C.c_float * INCIDENT_MAX_MODES),
+        ("laser_nofocus_constant_si", C.c_double),
+        ("ramp_init", C.c_double),
+    ]
+
+
+def fill_incident_profile(dst, kind, wavelength_si, amplitude_si, pulse_duration_si, direction,
+                          polarisation_direction, focus_position_si=(0.0, 0.0, 0.0), focus_origin=(0, 0, 0),
+                          polarisation=POLARISATION_LINEAR, time_delay_si=0.0, laser_phase=0.0, w0_si=0.0,
+                          pulse_init=20.0, laguerre_modes=(1.0,), laguerre_phases=None,
+                          laser_nofocus_constant_si=0.0, ramp_init=20.6146):
+    """Fill a pic_incident_profile-shaped ctypes struct from the parameter
+    names of profiles::BaseParam / GaussianPulseParam / PlaneWaveParam (SI)."""
+    modes = list(laguerre_modes)
+    phases = list(laguerre_phases) if laguerre_phases is not None else [0.0] * len(modes)
+    if not 1 <= len(modes) <= INCIDENT_MAX_MODES or len(phases) != len(modes):
+        raise ValueError("laguerre_modes / laguerre_phases: 1..16 entries of equal length")
+    dst.kind = int(kind)
+    dst.polarisation = int(polarisation)
+    dst.wavelength_si = float(wavelength_si)
+    dst.amplitude_si = float(amplitude_si)
+    dst.pulse_duration_si = float(pulse_duration_si)
+    dst.time_delay_si = float(time_delay_si)
+    dst.laser_phase = float(laser_phase)
+    for d in range(3):
+        dst.direction[d] = float(direction[d])
+        dst.polarisation_direction[d] = float(polarisation_direction[d])
+        dst.focus_position_si[d] = float(focus_position_si[d])
+        dst.focus_origin[d] = int(focus_origin[d])
+    dst.w0_si = float(w0_si)
+    dst.pulse_init = float(pulse_init)
+    dst.num_modes = len(modes) - 1
+    for m in range(len(modes)):
+        dst.laguerre_modes[m] = float(modes[m])
+        dst.laguerre_phases[m] = float(phases[m])
+    dst.laser_nofocus_constant_si = float(laser_nofocus_constant_si)
+    dst.ramp_init = float(ramp_init)
+    return dst
 
 
 class Config(C.Structure):
@@ -58,6 +120,8 @@ class Config(C.Structure):
         ("absorber_cells", (C.c_int32 * 2) * 3),
         ("absorber_strength", (C.c_float * 2) * 3),
         ("current_interpolation", C.c_int32),
+        ("incident", (IncidentProfile * 2) * 3),
+        ("incident_position", (C.c_int32 * 2) * 3),
     ]
 
 
@@ -147,6 +211,8 @@ SIGNATURES = {
     "pic_add_current": (C.c_int, [_P]),
     "pic_fdtd_after_current": (C.c_int, [_P]),
     "pic_fdtd_step": (C.c_int, [_P]),
+    "pic_set_step": (C.c_int, [_P, C.c_int64]),
+    "pic_get_step": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "pic_step": (C.c_int, [_P, C.c_int32]),
     "pic_step_graph": (C.c_int, [_P, C.c_int32]),
     "pic_field_energy": (C.c_int, [_P, _D]),
@@ -215,10 +281,14 @@ def ptr(a: np.ndarray, ctype):
 
 def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YEE, lehe_dir=1, j_scale_exp=0,
                 device=0, boundary=(0, 0, 0), absorber=0, absorber_cells=((0, 0), (0, 0), (0, 0)),
-                absorber_strength=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0)), current_interpolation=0) -> Config:
+                absorber_strength=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0)), current_interpolation=0, incident=None,
+                incident_position=((16, -16), (16, -16), (16, -16))) -> Config:
     """species: list of dicts with mass_ratio, charge_ratio, shape, pusher, frames_per_sc;
     boundary: BOUNDARY_* per axis; absorber: ABSORBER_*, with thickness and
-    strength per [axis][negative, positive] side as in fieldAbsorber.param."""
+    strength per [axis][negative, positive] side as in fieldAbsorber.param;
+    incident: {(axis, side): keyword arguments of fill_incident_profile} for
+    the faces of incidentField.param that carry a profile, incident_position
+    its POSITION table."""
     cfg = Config()
     for d in range(3):
         cfg.grid[d] = int(grid[d])
@@ -245,6 +315,11 @@ def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YE
             cfg.absorber_strength[d][side] = float(absorber_strength[d][side])
     cfg.absorber = int(absorber)
     cfg.current_interpolation = int(current_interpolation)
+    for d in range(3):
+        for side in range(2):
+            cfg.incident_position[d][side] = int(incident_position[d][side])
+    for (axis, side), prof in (incident or {}).items():
+        fill_incident_profile(cfg.incident[axis][side], **prof)
     return cfg
 
 
@@ -414,6 +489,15 @@ class Simulation:
     def fdtd_step(self):
         """update_beforeCurrent + addCurrent + update_afterCurrent in the fused kernels."""
         check(self.lib.pic_fdtd_step(self.h), "pic_fdtd_step")
+
+    def set_step(self, step):
+        """Set the field step counter (the solver's currentStep, which times the incident field)."""
+        check(self.lib.pic_set_step(self.h, int(step)), "pic_set_step")
+
+    def get_step(self) -> int:
+        v = C.c_int64()
+        check(self.lib.pic_get_step(self.h, C.byref(v)), "pic_get_step")
+        return int(v.value)
 
     def step(self, n=1):
         check(self.lib.pic_step(self.h, int(n)), "pic_step")

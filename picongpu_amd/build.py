@@ -20,7 +20,7 @@ LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libpicongpu_amd.so")
 
 SOURCES = ["kernels_push.hip", "kernels_shift.hip", "kernels_field.hip", "kernels_init.hip", "kernels_diag.hip",
-           "kernels_adjoint.hip",
+           "kernels_adjoint.hip", "kernels_incident.hip",
            "pic_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

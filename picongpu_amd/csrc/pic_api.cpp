@@ -10,10 +10,12 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
 #include "../../include/picongpu_amd.h"
+#include "pic_incident.hpp"
 #include "pic_kernels.hpp"
 
 namespace {
@@ -43,6 +45,8 @@ int supportOf(int shape) { return shape == PIC_SHAPE_CIC ? 2 : (shape == PIC_SHA
 // Esirkepov current margins (Esirkepov.hpp:42-43)
 int depLower(int supp) { return supp / 2 + 1 - (supp + 1) % 2; }
 int depUpper(int supp) { return (supp + 1) / 2 + 1; }
+
+int validateIncident(const pic_config& c);
 
 int validateConfig(const pic_config& c) {
     const int sc[3] = {pic::kSCX, pic::kSCY, pic::kSCZ};
@@ -89,7 +93,7 @@ int validateConfig(const pic_config& c) {
         (c.boundary[0] != PIC_BOUNDARY_PERIODIC || c.boundary[1] != PIC_BOUNDARY_PERIODIC ||
          c.boundary[2] != PIC_BOUNDARY_PERIODIC))
         return fail(PIC_E_INVALID, "the binomial current filter needs a periodic box");
-    return PIC_OK;
+    return validateIncident(c);
 }
 
 bool allPeriodic(const pic_config& c) {
@@ -157,6 +161,373 @@ int checkCfl(const pic_config& c, const pic_constants& k) {
             return fail(PIC_E_INVALID, "Courant-Friedrichs-Lewy condition failure (Lehe)");
     }
     return PIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Incident field set-up (fields/incidentField/Solver.hpp, Functors.hpp,
+// profiles/*, DerivativeCoefficients.hpp, calculatePhaseVelocity.hpp)
+// ---------------------------------------------------------------------------
+constexpr float kPiF = 3.14159265358979323846f;  // pmacc Pi<float_X>::value
+constexpr float kTwoPiF = 2.0f * kPiF;            // Pi<float_X>::doubleValue
+// Yee-cell positions of the E and B components (fields/YeeCell.hpp:69-141)
+constexpr float kYeeE[3][3] = {{0.5f, 0.0f, 0.0f}, {0.0f, 0.5f, 0.0f}, {0.0f, 0.0f, 0.5f}};
+constexpr float kYeeB[3][3] = {{0.0f, 0.5f, 0.5f}, {0.5f, 0.0f, 0.5f}, {0.5f, 0.5f, 0.0f}};
+
+// Coefficients of the derivative along `axis` in the curl of the field the
+// update reads: DerivativeCoefficients (DerivativeCoefficients.hpp:35-200).
+// curlE: the B update (the Lehe derivative under Lehe), else the E update
+// (Yee's backward derivative, also under Lehe: Lehe.def:73).
+void incidentCoefficients(const pic_config& cfg, const pic_constants& k, bool curlE, int axis, float value[2][2][2],
+                          int size[3]) {
+    std::memset(value, 0, sizeof(float) * 8);
+    size[0] = size[1] = size[2] = 1;
+    if (!(curlE && cfg.solver == PIC_SOLVER_LEHE)) {
+        value[0][0][0] = 1.0f;
+        return;
+    }
+    int const dir = cfg.lehe_dir;
+    if (dir == axis) {
+        // DerivativeCoefficients<lehe::DerivativeFunctor<axis, axis>> (:95-165)
+        int const d0 = axis, d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
+        double const stepRatio = static_cast<double>(k.cell[d0] / (k.c * k.dt));
+        double const coeff = stepRatio * std::sin(kPi / 2.0 * static_cast<double>(k.c) * static_cast<double>(k.dt) /
+                                                  static_cast<double>(k.cell[d0]));
+        float const delta = static_cast<float>(0.25 * (1.0 - coeff * coeff));
+        double const stepRatio1 = static_cast<double>(k.cell[d0] / k.cell[d1]);
+        double const stepRatio2 = static_cast<double>(k.cell[d0] / k.cell[d2]);
+        double const betaDir1 = 0.125 * stepRatio1 * stepRatio1;
+        double const betaDir2 = 0.125 * stepRatio2 * stepRatio2;
+        float const alpha = static_cast<float>(1.0 - 2.0 * betaDir1 - 2.0 * betaDir2 - 3.0 * delta);
+        size[0] = size[1] = size[2] = 2;
+        value[0][0][0] = alpha;
+        int i0[3] = {0, 0, 0}, i1[3] = {0, 0, 0}, i2[3] = {0, 0, 0};
+        i0[d0] = 1;
+        i1[d1] = 1;
+        i2[d2] = 1;
+        value[i0[0]][i0[1]][i0[2]] = delta;
+        value[i1[0]][i1[1]][i1[2]] = static_cast<float>(betaDir1);
+        value[i2[0]][i2[1]][i2[2]] = static_cast<float>(betaDir2);
+    } else {
+        // DerivativeCoefficients<lehe::DerivativeFunctor<dir, axis>> (:167-200)
+        float const beta = 0.125f;
+        float const alpha = 1.0f - 2.0f * beta;
+        size[dir] = 2;
+        value[0][0][0] = alpha;
+        int i[3] = {0, 0, 0};
+        i[dir] = 1;
+        value[i[0]][i[1]][i[2]] = beta;
+    }
+}
+
+// margin of the B and E updates along `axis` (Solver.kernel:123)
+int incidentMargin(const pic_config& cfg, bool curlE, int axis) {
+    return (curlE && cfg.solver == PIC_SOLVER_LEHE && cfg.lehe_dir == axis) ? 2 : 1;
+}
+
+// DispersionRelation<Solver>::relation / relationDerivative (DispersionRelation.hpp:60-100,
+// Yee.hpp:68-119, Lehe.hpp:77-200) in FP64
+double dispersion(const pic_config& cfg, const pic_constants& k, double omega, const double* dir, double absK,
+                  bool derivative) {
+    double const timeStep = static_cast<double>(k.dt);
+    double const step[3] = {static_cast<double>(k.cell[0]), static_cast<double>(k.cell[1]),
+                            static_cast<double>(k.cell[2])};
+    double const cc = static_cast<double>(k.c);
+    double const lhsTerm = std::sin(0.5 * omega * timeStep) / (cc * timeStep);
+    if (cfg.solver == PIC_SOLVER_YEE) {
+        double r = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            double const arg = 0.5 * absK * dir[d] * step[d];
+            double const term = std::sin(arg) / step[d];
+            if (derivative) {
+                double const termDerivative = 0.5 * dir[d] * std::cos(arg);
+                r += 2.0 * term * termDerivative;
+            } else {
+                r += term * term;
+            }
+        }
+        return derivative ? r : r - lhsTerm * lhsTerm;
+    }
+    int const d0 = cfg.lehe_dir, d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
+    double const stepSq[3] = {step[0] * step[0], step[1] * step[1], step[2] * step[2]};
+    double const stepRatio = step[d0] / (cc * timeStep);
+    double const coeff = stepRatio * std::sin(kPi / 2.0 * (cc * timeStep) / step[d0]);
+    double const delta = 0.25 * (1.0 - coeff * coeff);
+    double const stepRatio1 = step[d0] / step[d1];
+    double const stepRatio2 = step[d0] / step[d2];
+    double const betaDir0 = 0.125;
+    double const betaDir1 = 0.125 * stepRatio1 * stepRatio1;
+    double const betaDir2 = 0.125 * stepRatio2 * stepRatio2;
+    double sSq[3];
+    for (int d = 0; d < 3; ++d) {
+        double const s = std::sin(0.5 * absK * dir[d] * step[d]);
+        sSq[d] = s * s;
+    }
+    if (!derivative) {
+        double rhs = 0.0;
+        for (int d = 0; d < 3; ++d) rhs += sSq[d] / stepSq[d];
+        rhs -= 4.0 * delta * sSq[d0] * sSq[d0] / stepSq[d0];
+        rhs -= 4.0 * (betaDir1 / stepSq[d1] + betaDir0 / stepSq[d0]) * sSq[d0] * sSq[d1];
+        rhs -= 4.0 * (betaDir2 / stepSq[d2] + betaDir0 / stepSq[d0]) * sSq[d0] * sSq[d2];
+        return rhs - lhsTerm * lhsTerm;
+    }
+    double sSqD[3];
+    for (int d = 0; d < 3; ++d) {
+        double const arg = 0.5 * absK * dir[d] * step[d];
+        double const s = std::sin(arg);
+        double const sD = 0.5 * dir[d] * step[d] * std::cos(arg);
+        sSqD[d] = 2.0 * s * sD;
+    }
+    double r = 0.0;
+    for (int d = 0; d < 3; ++d) r += sSqD[d] / stepSq[d];
+    r -= 4.0 * delta * sSq[d0] * sSqD[d0] / stepSq[d0];
+    r -= 4.0 * (betaDir1 / stepSq[d1] + betaDir0 / stepSq[d0]) * (sSq[d0] * sSqD[d1] + sSqD[d0] * sSq[d1]);
+    // the reference takes betaDir1 in this term as well (Lehe.hpp:190-191)
+    r -= 4.0 * (betaDir1 / stepSq[d1] + betaDir0 / stepSq[d0]) * (sSq[d0] * sSqD[d2] + sSqD[d0] * sSq[d2]);
+    return r;
+}
+
+// calculatePhaseVelocity (traits/GetPhaseVelocity.hpp:31-45) with
+// DispersionRelationSolver (DispersionRelationSolver.hpp:36-55): Newton from
+// |k| = omega * c, at most 100 steps, stopping on a flat derivative
+float incidentPhaseVelocity(const pic_config& cfg, const pic_constants& k, float waveLength, const float* dirF) {
+    double const omega = 2.0 * kPi * static_cast<double>(k.c / waveLength);
+    double const dir[3] = {static_cast<double>(dirF[0]), static_cast<double>(dirF[1]), static_cast<double>(dirF[2])};
+    double absK = omega * static_cast<double>(k.c);
+    for (int i = 0; i < 100; ++i) {
+        double const der = dispersion(cfg, k, omega, dir, absK, true);
+        if (std::abs(der) < 1e-5) break;
+        absK = absK - dispersion(cfg, k, omega, dir, absK, false) / der;
+    }
+    return static_cast<float>(omega / absK / static_cast<double>(k.c));
+}
+
+// Huygens surface cell index along `axis` at the negative (side 0) and
+// positive (side 1) border (Solver.hpp:343-354)
+int huygensBorder(const pic_config& cfg, int axis, int side) {
+    if (side == 0) return cfg.incident_position[axis][0];
+    int const p = cfg.incident_position[axis][1];
+    return p > 0 ? p : cfg.grid[axis] + p;
+}
+
+int validateIncident(const pic_config& c) {
+    bool any = false;
+    for (int axis = 0; axis < 3; ++axis)
+        for (int side = 0; side < 2; ++side) {
+            const pic_incident_profile& p = c.incident[axis][side];
+            if (p.kind == PIC_INCIDENT_NONE) continue;
+            if (p.kind != PIC_INCIDENT_GAUSSIAN_PULSE && p.kind != PIC_INCIDENT_PLANE_WAVE)
+                return fail(PIC_E_INVALID, "unknown incident-field profile kind");
+            any = true;
+            if (p.polarisation != PIC_POLARISATION_LINEAR && p.polarisation != PIC_POLARISATION_CIRCULAR)
+                return fail(PIC_E_INVALID, "polarisation must be PIC_POLARISATION_LINEAR or _CIRCULAR");
+            if (!(p.wavelength_si > 0.0) || !(p.pulse_duration_si > 0.0))
+                return fail(PIC_E_INVALID, "incident field: wavelength and pulse duration must be positive");
+            if (!(p.time_delay_si >= 0.0))
+                return fail(PIC_E_INVALID, "incident field: the time delay must be >= 0 (BaseParam.hpp:90-93)");
+            // BaseParam.hpp:95-120: unit direction, unit polarisation, orthogonal
+            float const dir[3] = {static_cast<float>(p.direction[0]), static_cast<float>(p.direction[1]),
+                                  static_cast<float>(p.direction[2])};
+            float const pol[3] = {static_cast<float>(p.polarisation_direction[0]),
+                                  static_cast<float>(p.polarisation_direction[1]),
+                                  static_cast<float>(p.polarisation_direction[2])};
+            double const n2 = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
+            double const p2 = pol[0] * pol[0] + pol[1] * pol[1] + pol[2] * pol[2];
+            double const dp = dir[0] * pol[0] + dir[1] * pol[1] + dir[2] * pol[2];
+            if (!(n2 > 0.9999 && n2 < 1.0001)) return fail(PIC_E_INVALID, "incident field: direction must be a unit vector");
+            if (!(p2 > 0.9999 && p2 < 1.0001))
+                return fail(PIC_E_INVALID, "incident field: polarisation direction must be a unit vector");
+            if (!(dp > -0.0001 && dp < 0.0001))
+                return fail(PIC_E_INVALID, "incident field: polarisation must be orthogonal to the direction");
+            for (int d = 0; d < 3; ++d)
+                if (p.focus_origin[d] != PIC_ORIGIN_ZERO && p.focus_origin[d] != PIC_ORIGIN_CENTER)
+                    return fail(PIC_E_INVALID, "incident field: focus origin must be PIC_ORIGIN_ZERO or _CENTER");
+            if (p.kind == PIC_INCIDENT_GAUSSIAN_PULSE) {
+                if (!(p.w0_si > 0.0)) return fail(PIC_E_INVALID, "incident field: W0 must be positive");
+                if (p.num_modes < 0 || p.num_modes >= PIC_INCIDENT_MAX_MODES)
+                    return fail(PIC_E_INVALID, "incident field: num_modes must be in [0, PIC_INCIDENT_MAX_MODES)");
+                float norm = 0.0f;
+                for (int m = 0; m <= p.num_modes; ++m) norm += p.laguerre_modes[m];
+                if (!(std::abs(norm) > std::numeric_limits<float>::epsilon()))
+                    return fail(PIC_E_INVALID, "Sum of laguerreModes can not be 0.");
+            }
+            // checkRequirements (Solver.hpp:60-100): the surface keeps the
+            // absorber plus the update margin from the border, and fits the domain
+            int const margin = std::max(incidentMargin(c, true, axis), incidentMargin(c, false, axis));
+            int const th = c.absorber == PIC_ABSORBER_EXPONENTIAL ? c.absorber_cells[axis][side] : 0;
+            int offset;
+            if (side == 0)
+                offset = c.incident_position[axis][0];
+            else if (c.incident_position[axis][1] > 0)
+                offset = c.grid[axis] - c.incident_position[axis][1];
+            else
+                offset = -c.incident_position[axis][1];
+            if (offset < th + margin - 1)
+                return fail(PIC_E_INVALID, "Incident field POSITION[" + std::to_string(axis) + "][" +
+                                               std::to_string(side) +
+                                               "] is too close to the boundary for used field solver and absorber, "
+                                               "must be at least " + std::to_string(th + margin - 1) + " cells away");
+            for (int updateB = 0; updateB < 2; ++updateB) {
+                int const m = incidentMargin(c, updateB != 0, axis);
+                int begin = huygensBorder(c, axis, 0) + 1, end = huygensBorder(c, axis, 1);
+                if (updateB && side == 0) begin -= 1;
+                if (side == 0) {
+                    begin -= m - 1;
+                } else {
+                    end += m - 1;
+                    begin = end - (2 * m - 1);
+                }
+                begin = std::max(begin, 0);
+                if (begin + 1 < m || begin + m > c.grid[axis])
+                    return fail(PIC_E_INVALID, "The Huygens surface for incident field generation is too close to a "
+                                               "local domain border. Adjust POSITION.");
+            }
+        }
+    // an empty volume inside the surface generates no field (the reference
+    // only warns, Solver.hpp:380-390): its launches have no cells
+    (void)any;
+    return PIC_OK;
+}
+
+// Profile constants in PIC units (BaseParamUnitless, BaseParam.hpp:40-140;
+// GaussianPulseUnitless, GaussianPulse.hpp:60-75; PlaneWaveUnitless,
+// PlaneWave.hpp:60-75; BaseFunctorE, Functors.hpp:80-230)
+pic::IncidentProfile incidentProfile(const pic_config& cfg, const pic_constants& k, const pic_incident_profile& p) {
+    pic::IncidentProfile P{};
+    double const uL = k.unit_length, uT = k.unit_time, uM = k.unit_mass, uQ = k.unit_charge;
+    double const unitE = 1.0 / (uT * uT / uM / uL * uQ);  // simulation.unitless:461-464
+    P.kind = p.kind;
+    P.polarisation = p.polarisation;
+    for (int d = 0; d < 3; ++d) P.cell[d] = k.cell[d];
+    P.c = k.c;
+    P.dt = k.dt;
+    P.waveLength = static_cast<float>(p.wavelength_si / uL);
+    float const f = static_cast<float>(k.c / P.waveLength);
+    P.omega = kTwoPiF * f;
+    P.amplitude = static_cast<float>(p.amplitude_si / unitE);
+    P.pulseDuration = static_cast<float>(p.pulse_duration_si / uT);
+    P.laserPhase = p.laser_phase;
+    float const delaySi = static_cast<float>(p.time_delay_si);  // GetTimeDelay::value is a float_X
+    P.timeDelay = static_cast<float>(static_cast<double>(delaySi) / uT);
+    float dir[3], pol[3], focus[3];
+    for (int d = 0; d < 3; ++d) {
+        dir[d] = static_cast<float>(p.direction[d]);
+        pol[d] = static_cast<float>(p.polarisation_direction[d]);
+        focus[d] = static_cast<float>(p.focus_position_si[d] / uL);
+        P.axis0[d] = dir[d];
+        P.axis1[d] = pol[d];
+    }
+    P.axis2[0] = dir[1] * pol[2] - dir[2] * pol[1];
+    P.axis2[1] = dir[2] * pol[0] - dir[0] * pol[2];
+    P.axis2[2] = dir[0] * pol[1] - dir[1] * pol[0];
+    // getFocus (Functors.hpp:158-175)
+    for (int d = 0; d < 3; ++d)
+        if (p.focus_origin[d] == PIC_ORIGIN_CENTER)
+            focus[d] += static_cast<float>(static_cast<uint32_t>(cfg.grid[d]) / 2u) * k.cell[d];
+    // getOrigin (Functors.hpp:177-205)
+    float originP = -std::numeric_limits<float>::infinity();
+    for (int axis = 0; axis < 3; ++axis) {
+        if (std::abs(dir[axis]) > std::numeric_limits<float>::epsilon()) {
+            float const minPosition = (static_cast<float>(cfg.incident_position[axis][0]) + 0.75f) * k.cell[axis];
+            int const maxIdx = cfg.incident_position[axis][1] > 0 ? cfg.incident_position[axis][1]
+                                                                  : cfg.grid[axis] + cfg.incident_position[axis][1];
+            float const maxPosition = (static_cast<float>(maxIdx) - 0.75f) * k.cell[axis];
+            float const axisP = std::min((minPosition - focus[axis]) / dir[axis], (maxPosition - focus[axis]) / dir[axis]);
+            originP = std::max(originP, axisP);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        P.focus[d] = focus[d];
+        P.origin[d] = focus[d] + originP * dir[d];
+    }
+    P.phaseVelocity = incidentPhaseVelocity(cfg, k, P.waveLength, dir);
+    if (p.kind == PIC_INCIDENT_GAUSSIAN_PULSE) {
+        P.w0 = static_cast<float>(p.w0_si / uL);
+        P.rayleighLength = kPiF * P.w0 * P.w0 / P.waveLength;
+        P.timeShift = static_cast<float>(-0.5 * p.pulse_init * static_cast<double>(P.pulseDuration));
+        P.numModes = p.num_modes + 1;
+        for (int m = 0; m < P.numModes; ++m) {
+            P.modes[m] = p.laguerre_modes[m];
+            P.phases[m] = p.laguerre_phases[m];
+        }
+    } else {
+        P.rampInit = p.ramp_init;
+        P.nofocus = static_cast<float>(p.laser_nofocus_constant_si / uT);
+    }
+    return P;
+}
+
+// The launches of updateBHalf (updateB = true) or updateE in the reference's
+// order: axes x, y, z, each the negative border's profile (direction +1)
+// then the positive one's (updateField, Solver.hpp:120-258)
+std::vector<pic::IncidentUpdate> incidentUpdates(const pic_config& cfg, const pic_constants& k, bool updateB) {
+    std::vector<pic::IncidentUpdate> out;
+    // UpdateB: curlCoefficient = -timeIncrement with timeIncrement = 0.5 dt;
+    // UpdateE: timeIncrement * c^2 with timeIncrement = dt (Solver.hpp:290-330, 380-420)
+    float const c2 = k.c * k.c;
+    float const curlCoefficient = updateB ? -(0.5f * k.dt) : k.dt * c2;
+    for (int axis = 0; axis < 3; ++axis) {
+        for (int side = 0; side < 2; ++side) {
+            const pic_incident_profile& p = cfg.incident[axis][side];
+            if (p.kind == PIC_INCIDENT_NONE) continue;
+            pic::IncidentUpdate u{};
+            u.prof = incidentProfile(cfg, k, p);
+            u.axis = axis;
+            u.direction = side == 0 ? 1.0f : -1.0f;
+            u.updatedTotal = updateB ? 0 : 1;  // E_x sits at x = 0.5, B_x at x = 0
+            u.incidentIsB = updateB ? 0 : 1;
+            int size[3];
+            incidentCoefficients(cfg, k, updateB, axis, u.value, size);
+            u.margin = size[axis];
+            u.size1 = size[(axis + 1) % 3];
+            u.size2 = size[(axis + 2) % 3];
+            float const sign = u.direction > 0.0f ? 1.0f : -1.0f;
+            u.baseCoefficient = curlCoefficient / k.cell[axis] * sign;
+            // updated cells (Solver.hpp:130-200)
+            int begin[3], end[3];
+            for (int d = 0; d < 3; ++d) {
+                begin[d] = huygensBorder(cfg, d, 0) + 1;
+                end[d] = huygensBorder(cfg, d, 1);
+            }
+            if (!u.updatedTotal && u.direction > 0.0f) begin[axis] -= 1;
+            bool const contiguous = p.kind == PIC_INCIDENT_PLANE_WAVE;  // PlaneWave.def:70-74
+            for (int d = 0; d < 3; ++d) {
+                if (contiguous && cfg.boundary[d] == PIC_BOUNDARY_PERIODIC && d != axis) {
+                    begin[d] = 0;
+                    end[d] = cfg.grid[d];
+                    u.isLast[d] = 0;
+                } else {
+                    u.isLast[d] = 1;
+                }
+            }
+            int const sizeAlongAxis = 2 * u.margin - 1;
+            if (u.direction > 0.0f) {
+                begin[axis] -= u.margin - 1;
+                end[axis] = begin[axis] + sizeAlongAxis;
+            } else {
+                end[axis] += u.margin - 1;
+                begin[axis] = end[axis] - sizeAlongAxis;
+            }
+            bool anyCells = true;
+            for (int d = 0; d < 3; ++d) {
+                u.begin[d] = std::max(begin[d], 0);
+                u.end[d] = std::min(end[d], cfg.grid[d]);
+                anyCells = anyCells && u.begin[d] < u.end[d];
+            }
+            if (!anyCells) continue;
+            u.huygens = u.begin[axis] + u.margin - 1;
+            // incidentFieldBaseShift + the incident component's Yee position (Solver.hpp:226-240)
+            float baseShift[3] = {0.0f, 0.0f, 0.0f};
+            if (u.direction > 0.0f) baseShift[axis] = u.updatedTotal ? -1.0f : 1.0f;
+            const float(*pos)[3] = updateB ? kYeeE : kYeeB;
+            int const c1 = (axis + 2) % 3, c2i = (axis + 1) % 3;
+            for (int d = 0; d < 3; ++d) {
+                u.inCellShift1[d] = baseShift[d] + pos[c1][d];
+                u.inCellShift2[d] = baseShift[d] + pos[c2i][d];
+            }
+            out.push_back(u);
+        }
+    }
+    return out;
 }
 
 // Colours of a periodic ring of n supercells such that equal colours are at
@@ -257,6 +628,8 @@ struct pic_ctx {
     int eParity = 0;                               // flips with every swap of E and Ealt
     pic::AbsorbPlan absorb{};                      // exponential absorber layers, exchange order
     float* Jf[3] = {nullptr, nullptr, nullptr};    // binomially filtered J for the fused field step
+    std::vector<pic::IncidentUpdate> incB, incE;   // updateBHalf / updateE launches, face order
+    int64_t* stepDev = nullptr;                    // field step counter (the solver's currentStep)
     float* B[3] = {nullptr, nullptr, nullptr};
     float* J[3] = {nullptr, nullptr, nullptr};
     std::vector<SpeciesMem> sp;
@@ -441,6 +814,12 @@ pic::PushArgs makePushArgs(pic_ctx* c) {
     return a;
 }
 
+int applyIncident(pic_ctx* c, const std::vector<pic::IncidentUpdate>& list, float* const F[3], float stepOffset) {
+    for (const pic::IncidentUpdate& u : list)
+        PIC_HIP(pic::launchIncident(u, c->g, F, c->stepDev, stepOffset, c->stream));
+    return PIC_OK;
+}
+
 pic::FieldArgs makeFieldArgs(pic_ctx* c) {
     pic::FieldArgs a{};
     a.g = c->g;
@@ -615,6 +994,9 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
         e = hipMemcpyAsync(dTable, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(e)));
     }
+    if (devAlloc(c, &c->stepDev, 1)) return bail(PIC_E_HIP);
+    c->incB = incidentUpdates(*cfg, c->k, true);
+    c->incE = incidentUpdates(*cfg, c->k, false);
     c->energyBlocks = 1024;
     if (devAlloc(c, &c->energyPartials, 2 * c->energyBlocks)) return bail(PIC_E_HIP);
     if (devAlloc(c, &c->countOut, 1)) return bail(PIC_E_HIP);
@@ -992,7 +1374,12 @@ int pic_fdtd_before_current(pic_ctx* c) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     pic::FieldArgs a = makeFieldArgs(c);
     TimerScope ts(c, PIC_TIMER_FIELD);
+    // FDTDBase.hpp:97-121: B half step, incident B at currentStep, incident E
+    // at currentStep + 0.5 * timeStep / dt, E full step
     PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
+    int r;
+    if ((r = applyIncident(c, c->incB, c->B, 0.0f)) != PIC_OK) return r;
+    if ((r = applyIncident(c, c->incE, c->E, 0.5f * c->k.dt / c->k.dt)) != PIC_OK) return r;
     PIC_HIP(pic::launchEFull(a, c->stream));
     return PIC_OK;
 }
@@ -1009,10 +1396,29 @@ int pic_fdtd_after_current(pic_ctx* c) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     pic::FieldArgs a = makeFieldArgs(c);
     TimerScope ts(c, PIC_TIMER_FIELD);
-    // FDTDBase.hpp:151-183: absorber on E, B half step, absorber on B
+    // FDTDBase.hpp:151-183: absorber on E, incident B at currentStep +
+    // timeStep / dt, B half step, absorber on B
     if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->E, c->absorb, c->stream));
+    int const r = applyIncident(c, c->incB, c->B, c->k.dt / c->k.dt);
+    if (r != PIC_OK) return r;
     PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
     if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->B, c->absorb, c->stream));
+    PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->stream));
+    return PIC_OK;
+}
+
+int pic_set_step(pic_ctx* c, int64_t step) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (step < 0) return fail(PIC_E_INVALID, "the step counter must be >= 0");
+    PIC_HIP(hipMemcpyAsync(c->stepDev, &step, sizeof(step), hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_get_step(pic_ctx* c, int64_t* step) {
+    if (!c || !step) return fail(PIC_E_INVALID, "null argument");
+    PIC_HIP(hipMemcpyAsync(step, c->stepDev, sizeof(*step), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
 
@@ -1022,7 +1428,16 @@ namespace {
 
 // update_beforeCurrent + addCurrent + update_afterCurrent as the two fused
 // kernels; the new E lands in Ealt and the buffers swap
+bool fusedFieldStep(const pic_ctx* c) { return c->incB.empty() && c->incE.empty(); }
+
 int fieldStep(pic_ctx* c) {
+    if (!fusedFieldStep(c)) {
+        // the incident field adds between the half steps the fused kernels join
+        int r;
+        if ((r = pic_fdtd_before_current(c)) != PIC_OK) return r;
+        if ((r = pic_add_current(c)) != PIC_OK) return r;
+        return pic_fdtd_after_current(c);
+    }
     pic::FieldArgs a = makeFieldArgs(c);
     {
         TimerScope ts(c, PIC_TIMER_FIELD);
@@ -1031,6 +1446,7 @@ int fieldStep(pic_ctx* c) {
             for (int d = 0; d < 3; ++d) a.J[d] = c->Jf[d];
         }
         PIC_HIP(pic::launchFieldStep(c->cfg.solver, a, c->Ealt, &c->absorb, c->stream));
+        PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->stream));
     }
     for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
     c->eParity ^= 1;
@@ -1100,7 +1516,8 @@ int pic_step_graph(pic_ctx* c, int32_t nsteps) {
             c->graph[p] = g;
             PIC_HIP(hipGraphInstantiate(&c->graphExec[p], c->graph[p], nullptr, nullptr, 0));
             c->graphJexp[p] = c->jexp;
-        } else {
+        } else if (fusedFieldStep(c)) {
+            // the replayed step writes the new E into the other buffer
             for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
             c->eParity ^= 1;
         }
@@ -1319,8 +1736,10 @@ int pic_adjoint_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* 
     if (r != PIC_OK) return r;
     if (n <= 0 || !cell || !pos || !mom || !w) return fail(PIC_E_INVALID, "particle arrays and n > 0 are required");
     if (c->cfg.solver != PIC_SOLVER_YEE) return fail(PIC_E_INVALID, "the differentiable form uses the Yee solver");
-    if (!allPeriodic(c->cfg) || c->cfg.absorber != PIC_ABSORBER_NONE || c->cfg.current_interpolation != PIC_CURRENT_NONE)
-        return fail(PIC_E_INVALID, "the differentiable form needs a periodic box without absorber or current filter");
+    if (!allPeriodic(c->cfg) || c->cfg.absorber != PIC_ABSORBER_NONE || c->cfg.current_interpolation != PIC_CURRENT_NONE ||
+        !c->incB.empty() || !c->incE.empty())
+        return fail(PIC_E_INVALID,
+                    "the differentiable form needs a periodic box without absorber, current filter or incident field");
     if (c->ncells > (size_t)INT32_MAX) return fail(PIC_E_INVALID, "the differentiable form needs fewer than 2^31 cells");
     const SpeciesMem& sm = c->sp[s];
     if (sm.desc.shape != PIC_SHAPE_CIC || sm.desc.pusher != PIC_PUSHER_BORIS ||
@@ -1783,8 +2202,12 @@ int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out) {
     for (int32_t i = 0; i < n; ++i) {
         const pic_config& c = cfgs[i];
         if (c.device != e->device) return bail(fail(PIC_E_INVALID, "ensemble members must share one device"));
-        if (!allPeriodic(c) || c.absorber != PIC_ABSORBER_NONE || c.current_interpolation != PIC_CURRENT_NONE)
-            return bail(fail(PIC_E_INVALID, "ensemble members need periodic boxes without absorber or current filter"));
+        bool incident = false;
+        for (int a = 0; a < 3; ++a)
+            for (int side = 0; side < 2; ++side) incident = incident || c.incident[a][side].kind != PIC_INCIDENT_NONE;
+        if (!allPeriodic(c) || c.absorber != PIC_ABSORBER_NONE || c.current_interpolation != PIC_CURRENT_NONE || incident)
+            return bail(fail(PIC_E_INVALID,
+                             "ensemble members need periodic boxes without absorber, current filter or incident field"));
         int supp = 0, pusher = 0;
         for (int s = 0; s < c.n_species; ++s) {
             if (c.species[s].pusher == PIC_PUSHER_NONE) continue;

@@ -62,6 +62,9 @@ class Setup:
     absorber_cells: tuple = ((0, 0), (0, 0), (0, 0))  # [axis][negative, positive]
     absorber_strength: tuple = ((0.0, 0.0), (0.0, 0.0), (0.0, 0.0))
     current_interpolation: int = 0  # abi.CURRENT_*
+    # incidentField.param: {(axis, side): keyword arguments of abi.fill_incident_profile}
+    incident: dict = None
+    incident_position: tuple = ((16, -16), (16, -16), (16, -16))
 
     def config(self, device: int = 0, j_scale_exp: int = 0) -> abi.Config:
         sp = [dict(mass_ratio=s.mass_ratio, charge_ratio=s.charge_ratio, shape=s.shape, pusher=s.pusher,
@@ -70,7 +73,8 @@ class Setup:
                                solver=self.solver, lehe_dir=self.lehe_dir, j_scale_exp=j_scale_exp, device=device,
                                boundary=self.boundary, absorber=self.absorber, absorber_cells=self.absorber_cells,
                                absorber_strength=self.absorber_strength,
-                               current_interpolation=self.current_interpolation)
+                               current_interpolation=self.current_interpolation, incident=self.incident,
+                               incident_position=self.incident_position)
 
     @property
     def ncells(self) -> int:
@@ -112,7 +116,18 @@ def kelvin_helmholtz(grid=(256, 256, 256), ppc=16, shape=abi.SHAPE_TSC, steps=50
     return Setup("kelvin_helmholtz", tuple(grid), cell, dt, 1e25, ppc, [e, p], steps=steps, b_noise=1e-6)
 
 
-def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0) -> Setup:
+ELECTRON_MASS_SI = 9.1093837139e-31
+ELECTRON_CHARGE_SI = -1.602176634e-19
+
+
+def a0_amplitude_si(a0: float, wavelength_si: float) -> float:
+    """E amplitude [V/m] of normalised vector potential a0, as the
+    reference's UNITCONV_A0_to_Amplitude_SI (LaserWakefield incidentField.param:101-103)."""
+    return a0 * (-2.0 * math.pi / wavelength_si * ELECTRON_MASS_SI * SPEED_OF_LIGHT_SI * SPEED_OF_LIGHT_SI
+                 / ELECTRON_CHARGE_SI)
+
+
+def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0, laser="initial") -> Setup:
     """Laser-plasma geometry: PQS shape, Vay pusher, Lehe solver Cherenkov-free
     along y, dx = dz = 0.1 um, dy = 0.05 um, cold electrons and immobile
     protons on a linear density ramp along y rising over 100 um to 3e24 m^-3
@@ -122,7 +137,13 @@ def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0) -> Setup:
     on the transverse axis. Borders as the reference's LaserWakefield example
     (etc/picongpu/1.cfg: --periodic 0 0 1): x and y open, z periodic, with the
     exponential absorber of param/fieldAbsorber.param (12 cells, strength
-    1e-3 on every side)."""
+    1e-3 on every side).
+
+    laser="incident" injects the same pulse as the reference's
+    LaserWakefield example does instead (its incidentField.param): a
+    GaussianPulse from YMin through the Huygens surface 16 cells inside the
+    borders, PULSE_DURATION_SI = FWHM / (2 sqrt(2 ln 2)), PULSE_INIT = 15,
+    focused at y = 10 um on the transverse centre (FOCUS_ORIGIN Center)."""
     cell = (1e-7, 5e-8, 1e-7)
     dt = 0.995 * cell[1] / SPEED_OF_LIGHT_SI
     ramp = (1, 20e-6, 100e-6)
@@ -130,13 +151,25 @@ def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0) -> Setup:
                      ramp=ramp)
     p = SpeciesSetup("p", mass_ratio=MASS_RATIO_PROTON, charge_ratio=-1.0, shape=abi.SHAPE_PQS,
                      pusher=abi.PUSHER_NONE, ppc=ppc, density_si=3e24, seed=seed + 1, ramp=ramp)
-    laser = dict(wavelength_si=0.8e-6, a0=2.0, waist_si=8e-6, fwhm_si=25e-15,
-                 center_si=(grid[0] * cell[0] / 2, 10e-6, grid[2] * cell[2] / 2), propagation_axis=1,
-                 polarisation_axis=0)
+    initial = incident = None
+    if laser == "initial":
+        initial = dict(wavelength_si=0.8e-6, a0=2.0, waist_si=8e-6, fwhm_si=25e-15,
+                       center_si=(grid[0] * cell[0] / 2, 10e-6, grid[2] * cell[2] / 2), propagation_axis=1,
+                       polarisation_axis=0)
+    elif laser == "incident":
+        incident = {(1, 0): dict(kind=abi.INCIDENT_GAUSSIAN_PULSE, wavelength_si=0.8e-6,
+                                 amplitude_si=a0_amplitude_si(2.0, 0.8e-6),
+                                 pulse_duration_si=25e-15 / (2.0 * math.sqrt(2.0 * math.log(2.0))),
+                                 direction=(0, 1, 0), polarisation_direction=(1, 0, 0),
+                                 focus_position_si=(0.0, 10e-6, 0.0),
+                                 focus_origin=(abi.ORIGIN_CENTER, abi.ORIGIN_ZERO, abi.ORIGIN_CENTER),
+                                 w0_si=8e-6, pulse_init=15.0)}
+    elif laser is not None:
+        raise ValueError("laser must be 'initial', 'incident' or None")
     return Setup("laser_plasma", tuple(grid), cell, dt, 3e24, ppc, [e, p], solver=abi.SOLVER_LEHE, lehe_dir=1,
-                 steps=steps, laser=laser, boundary=(abi.BOUNDARY_OPEN, abi.BOUNDARY_OPEN, abi.BOUNDARY_PERIODIC),
+                 steps=steps, laser=initial, boundary=(abi.BOUNDARY_OPEN, abi.BOUNDARY_OPEN, abi.BOUNDARY_PERIODIC),
                  absorber=abi.ABSORBER_EXPONENTIAL, absorber_cells=((12, 12), (12, 12), (12, 12)),
-                 absorber_strength=((1e-3, 1e-3), (1e-3, 1e-3), (1e-3, 1e-3)))
+                 absorber_strength=((1e-3, 1e-3), (1e-3, 1e-3), (1e-3, 1e-3)), incident=incident)
 
 
 CONFIGS = {

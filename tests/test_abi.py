@@ -101,3 +101,51 @@ def test_bin_frames_layout():
     for sc in range(m.shape[0]):
         n = f["counts"][sc]
         assert np.all(m[sc, :n] == 1) and np.all(m[sc, n:] == 0)
+
+
+def _incident_setup(**prof_over):
+    st = setups.laser_plasma(grid=(32, 64, 32), ppc=1, laser="incident")
+    prof = dict(st.incident[(1, 0)])
+    prof.update(prof_over)
+    st.incident = {(1, 0): prof}
+    return st
+
+
+def test_incident_setup_is_accepted():
+    rc, msg = _create_status(_incident_setup().config())
+    # no device here: creation fails only after validation, at the first HIP call
+    assert rc in (0, -2), msg
+
+
+@pytest.mark.parametrize("over,fragment", [
+    (dict(direction=(0, 1.1, 0)), "direction"),
+    (dict(polarisation_direction=(0, 1, 0)), "orthogonal"),
+    (dict(polarisation_direction=(0.5, 0, 0)), "polarisation direction"),
+    (dict(laguerre_modes=(1.0, -1.0)), "laguerreModes"),
+    (dict(time_delay_si=-1e-15), "time delay"),
+    (dict(w0_si=0.0), "W0"),
+    (dict(kind=7), "profile kind"),
+])
+def test_incident_rejects_bad_profile(over, fragment):
+    rc, msg = _create_status(_incident_setup(**over).config())
+    assert rc == -1 and fragment in msg, msg
+
+
+def test_incident_rejects_surface_inside_absorber():
+    st = _incident_setup()
+    # Lehe along y: the B update's margin is 2, so the surface must keep 12 + 2 - 1 cells from the border
+    st.incident_position = ((16, -16), (12, -16), (16, -16))
+    rc, msg = _create_status(st.config())
+    assert rc == -1 and "too close to the boundary" in msg and "13" in msg
+    st.incident_position = ((16, -16), (13, -16), (16, -16))
+    rc, msg = _create_status(st.config())
+    assert rc in (0, -2), msg
+
+
+def test_incident_rejected_by_ensemble_and_differentiable_forms():
+    st = _incident_setup()
+    cfgs = (abi.Config * 1)(st.config())
+    lib = abi.load_library()
+    h = C.c_void_p()
+    rc = lib.pic_ensemble_create(cfgs, 1, C.byref(h))
+    assert rc == -1 and "incident" in lib.pic_last_error().decode()
