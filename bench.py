@@ -177,6 +177,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="kelvin_helmholtz")
     ap.add_argument("--grid", default="", help="override grid, e.g. 128,128,128")
+    ap.add_argument("--laser", default="initial", choices=["initial", "incident"],
+                    help="laser_plasma only: the pulse as the initial field or injected through the Huygens surface")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-grid", default="128,128,64")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
@@ -203,6 +205,8 @@ def main():
     kw = {}
     if args.grid:
         kw["grid"] = tuple(int(v) for v in args.grid.split(","))
+    if args.config == "laser_plasma":
+        kw["laser"] = args.laser
     st = setup_fn(**kw)
     n_particles = st.mobile_particles()
     ncells = st.ncells
@@ -278,7 +282,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (device-seeded initial state)",
-        "config": {"workload": st.name, "grid": list(st.grid), "species": [s.name for s in st.species],
+        "config": {"workload": st.name + ("+incident" if st.incident else ""), "grid": list(st.grid), "species": [s.name for s in st.species],
                    "ppc": [s.ppc for s in st.species], "shape": "TSC" if st.species[0].shape == 2 else
                    ("CIC" if st.species[0].shape == 1 else "PQS"), "pusher": "Boris" if st.species[0].pusher == 1
                    else "Vay", "solver": "Yee" if st.solver == 0 else "Lehe", "particles": n_particles,
