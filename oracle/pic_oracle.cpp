@@ -352,18 +352,24 @@ T gather1Impl(const State<T>& st, const std::vector<T>& F, int supp, int gx, int
     sh.onSupport(p[0], sx);
     sh.onSupport(p[1], sy);
     sh.onSupport(p[2], sz);
+    // AssignedTrilinearInterpolation.hpp:53-86 sums into zero-initialised
+    // partial sums; here each starts at its first term, which differs only in
+    // the sign of an all-zero sum
     T rz = T(0.0);
     for (int z = 0; z < supp; ++z) {
         T ry = T(0.0);
         for (int y = 0; y < supp; ++y) {
             T rx = T(0.0);
-            for (int x = 0; x < supp; ++x)
-                rx += (OPEN ? st.at(F, gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)
-                            : F[st.idx(gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)]) *
-                      sx[x];
-            ry += rx * sy[y];
+            for (int x = 0; x < supp; ++x) {
+                T const term =
+                    (OPEN ? st.at(F, gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)
+                          : F[st.idx(gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)]) *
+                    sx[x];
+                rx = x == 0 ? term : rx + term;
+            }
+            ry = y == 0 ? rx * sy[y] : ry + rx * sy[y];
         }
-        rz += ry * sz[z];
+        rz = z == 0 ? ry * sz[z] : rz + ry * sz[z];
     }
     return rz;
 }

@@ -144,19 +144,18 @@ __device__ __forceinline__ float gatherGlobalCIC(const float* __restrict__ F, co
     Shape<2>::onSupport(p[0], sx);
     Shape<2>::onSupport(p[1], sy);
     Shape<2>::onSupport(p[2], sz);
+    // partial sums start at their first term, as gather1
     float result_z = 0.0f;
 #pragma unroll
     for (int z = 0; z < 2; ++z) {
         float result_y = 0.0f;
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
-            float result_x = 0.0f;
-#pragma unroll
-            for (int x = 0; x < 2; ++x)
-                result_x += F[gidx(g, cx + is[0] + x, cy + is[1] + y, cz + is[2] + z)] * sx[x];
-            result_y += result_x * sy[y];
+            float const result_x = F[gidx(g, cx + is[0], cy + is[1] + y, cz + is[2] + z)] * sx[0] +
+                                   F[gidx(g, cx + is[0] + 1, cy + is[1] + y, cz + is[2] + z)] * sx[1];
+            result_y = y == 0 ? result_x * sy[0] : result_y + result_x * sy[y];
         }
-        result_z += result_y * sz[z];
+        result_z = z == 0 ? result_y * sz[0] : result_z + result_y * sz[z];
     }
     return result_z;
 }

@@ -112,6 +112,23 @@ __global__ __launch_bounds__(256) void kEFull(FieldArgs a) {
     a.E[2][i] = a.E[2][i] + cz * a.c2 * a.dt;
 }
 
+// kEFull then kAddCurrent (None) per cell in one pass, the new E into Enew
+__global__ __launch_bounds__(256) void kEFullJ(FieldArgs a, float* En0, float* En1, float* En2) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Idx I{a.g, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny), (int)(i / ((size_t)a.g.nx * a.g.ny))};
+    float const cx = bwd(a.B[2], I, 1, a.cell[1]) - bwd(a.B[1], I, 2, a.cell[2]);
+    float const cy = bwd(a.B[0], I, 2, a.cell[2]) - bwd(a.B[2], I, 0, a.cell[0]);
+    float const cz = bwd(a.B[1], I, 0, a.cell[0]) - bwd(a.B[0], I, 1, a.cell[1]);
+    float const e0 = a.E[0][i] + cx * a.c2 * a.dt;
+    float const e1 = a.E[1][i] + cy * a.c2 * a.dt;
+    float const e2 = a.E[2][i] + cz * a.c2 * a.dt;
+    En0[i] = e0 + a.coeff * a.J[0][i];
+    En1[i] = e1 + a.coeff * a.J[1][i];
+    En2[i] = e2 + a.coeff * a.J[2][i];
+}
+
 // ---------------------------------------------------------------------------
 // Fused field step (the same per-cell float operations as kBHalf, kEFull,
 // kAddCurrent, kBHalf in that order, so the results are bit-identical).
@@ -420,8 +437,9 @@ struct YeeAfterLds {
     float En[2][3][kZY + 1][kZX + 1];
 };
 
-// update_afterCurrent for one z-marching tile (Yee): B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt
-template <bool OPEN>
+// update_afterCurrent for one z-marching tile (Yee): B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt;
+// with TWO = false one B half step alone, B' = B - curlE(E)*0.5*dt (kBHalf's arithmetic)
+template <bool OPEN, bool TWO>
 __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* Enew, int b, int zc, YeeAfterLds& L) {
     ZTile const T = zTile(a.g, b, zc);
     int const nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
@@ -449,7 +467,7 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
                 int const c = t / (PX * PY);
                 size_t const gi = zoff + (size_t)eoff[j];
                 ve[j] = a.E[c][gi];
-                vn[j] = Enew[c][gi];
+                if (TWO) vn[j] = Enew[c][gi];
             }
         }
     };
@@ -460,7 +478,7 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
             int const t = threadIdx.x + 256 * j;
             if (t < 3 * PX * PY) {
                 Eb[so + t] = ve[j];
-                Nb[so + t] = vn[j];
+                if (TWO) Nb[so + t] = vn[j];
             }
         }
     };
@@ -492,6 +510,12 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
             float const b0 = a.B[0][i] - cx * 0.5f * a.dt;
             float const b1 = a.B[1][i] - cy * 0.5f * a.dt;
             float const b2 = a.B[2][i] - cz * 0.5f * a.dt;
+            if (!TWO) {
+                a.B[0][i] = b0;
+                a.B[1][i] = b1;
+                a.B[2][i] = b2;
+                continue;
+            }
             curl(L.En[s], L.En[s1], cx, cy, cz);
             a.B[0][i] = b0 - cx * 0.5f * a.dt;
             a.B[1][i] = b1 - cy * 0.5f * a.dt;
@@ -501,11 +525,13 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
 }
 
 // ---------------------------------------------------------------------------
-// Lehe: z-marching form. The Lehe curl of E (Lehe/Derivative.hpp:65-228)
+// Lehe: z-marching B half step. The Lehe curl of E (Lehe/Derivative.hpp:65-228)
 // reaches one cell below and two above along the Cherenkov-free axis and one
 // cell across it, so E is staged in a ring of four planes (k-1 .. k+2) with
-// an xy halo of two cells; B' keeps two planes as in the Yee form. The float
-// operations per cell are those of leheDeriv/kBHalf/kEFull/kAddCurrent.
+// an xy halo of two cells. The float operations per cell are those of
+// leheDeriv/kBHalf. The Lehe field step writes B' back rather than keeping it
+// in LDS as the Yee form does: each Lehe curl costs about 24 correctly
+// rounded divisions per cell, and the fused pair would take the first one twice.
 // ---------------------------------------------------------------------------
 constexpr int kLH = 2;
 constexpr int kLEX = kZX + 2 * kLH, kLEY = kZY + 2 * kLH;  // E plane: x, y in [-2, T+2)
@@ -600,132 +626,34 @@ __device__ __forceinline__ void lehePlaneLoad(const GridDev& g, float* const* F,
     lehePlaneStore(v, k, P);
 }
 
-struct LeheBeforeLds {
-    LehePlanes E;
-    float B[2][3][kZBY][kZBX];
-};
-
-// update_beforeCurrent + addCurrent for one z-marching tile (Lehe)
+// one B half step for one z-marching tile (Lehe): B' = B - curlE(E)*0.5*dt,
+// the arithmetic of kBHalf
 template <bool OPEN>
-__device__ __forceinline__ void leheBeforeAddTile(const FieldArgs& a, float* const* Enew, int b, int zc,
-                                                  LeheBeforeLds& L) {
-    ZTile const T = zTile(a.g, b, zc);
-    int const nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
-    size_t const plane = (size_t)nx * ny;
-    constexpr int NE = (3 * kLEX * kLEY + 255) / 256;
-    constexpr int NB = (kZBX * kZBY + 255) / 256;
-    int eoff[NE];
-    lehePlaneOffsets<OPEN>(a.g, T, eoff);
-    int boff[NB];
-    for (int j = 0; j < NB; ++j) {
-        int const t = threadIdx.x + 256 * j;
-        int const x = T.x0 + t % kZBX - 1, y = T.y0 + t / kZBX - 1;
-        boff[j] = outsideIf<OPEN>(a.g, x, y, 0) ? -1 : wrap1(x, nx) + nx * wrap1(y, ny);
-    }
-    float* const* const E = a.E;
-    lehePlaneLoad<OPEN>(a.g, E, eoff, T.z0 - 2, L.E);
-    lehePlaneLoad<OPEN>(a.g, E, eoff, T.z0 - 1, L.E);
-    lehePlaneLoad<OPEN>(a.g, E, eoff, T.z0, L.E);
-    int const lx = threadIdx.x % kZX, ly = threadIdx.x / kZX;
-    bool const inside = T.x0 + lx < nx && T.y0 + ly < ny;
-    size_t const ixy = (size_t)(T.x0 + lx) + (size_t)nx * (T.y0 + ly);
-    float next[NE];  // plane k+2, fetched while plane k-1 was being computed
-    lehePlaneFetch<OPEN>(a.g, E, eoff, T.z0 + 1, next);
-    for (int k = T.z0 - 1; k < T.z1; ++k) {
-        __syncthreads();  // the slot of plane k+2 (plane k-2) is no longer read
-        lehePlaneStore(next, k + 2, L.E);
-        if (k + 3 <= T.z1 + 1) lehePlaneFetch<OPEN>(a.g, E, eoff, k + 3, next);  // in flight during this step
-        __syncthreads();
-        {  // B' = B - curlE(E)*0.5*dt at plane k on x, y in [-1, T)
-            size_t const zoff = plane * (size_t)wrap1(k, nz);
-            bool const zout = OPEN && (a.g.open & 4) && (k < 0 || k >= nz);
-            int const s = k & 1;
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                int const t = threadIdx.x + 256 * j;
-                if (t >= kZBX * kZBY) continue;
-                int const bx = t % kZBX, by = t / kZBX;  // cell (bx-1, by-1) of the tile
-                if (OPEN && (zout || boff[j] < 0)) {     // B beyond an open border stays zero
-                    L.B[s][0][by][bx] = L.B[s][1][by][bx] = L.B[s][2][by][bx] = 0.0f;
-                    continue;
-                }
-                float cx, cy, cz;
-                lCurl(L.E, bx - 1, by - 1, k, a, cx, cy, cz);
-                size_t const gi = zoff + (size_t)boff[j];
-                L.B[s][0][by][bx] = a.B[0][gi] - cx * 0.5f * a.dt;
-                L.B[s][1][by][bx] = a.B[1][gi] - cy * 0.5f * a.dt;
-                L.B[s][2][by][bx] = a.B[2][gi] - cz * 0.5f * a.dt;
-            }
-        }
-        __syncthreads();
-        if (k >= T.z0 && inside) {
-            int const s = k & 1, sm = (k - 1) & 1;
-            int const bx = lx + 1, by = ly + 1;
-            // backward differences of B' (BackwardDerivative.hpp:44-65), curl as kEFull
-            float const cx = (L.B[s][2][by][bx] - L.B[s][2][by - 1][bx]) / a.cell[1] -
-                             (L.B[s][1][by][bx] - L.B[sm][1][by][bx]) / a.cell[2];
-            float const cy = (L.B[s][0][by][bx] - L.B[sm][0][by][bx]) / a.cell[2] -
-                             (L.B[s][2][by][bx] - L.B[s][2][by][bx - 1]) / a.cell[0];
-            float const cz = (L.B[s][1][by][bx] - L.B[s][1][by][bx - 1]) / a.cell[0] -
-                             (L.B[s][0][by][bx] - L.B[s][0][by - 1][bx]) / a.cell[1];
-            size_t const i = ixy + plane * k;
-            float e0 = lAt(L.E, 0, lx, ly, k) + cx * a.c2 * a.dt;
-            float e1 = lAt(L.E, 1, lx, ly, k) + cy * a.c2 * a.dt;
-            float e2 = lAt(L.E, 2, lx, ly, k) + cz * a.c2 * a.dt;
-            e0 = e0 + a.coeff * a.J[0][i];
-            e1 = e1 + a.coeff * a.J[1][i];
-            e2 = e2 + a.coeff * a.J[2][i];
-            Enew[0][i] = e0;
-            Enew[1][i] = e1;
-            Enew[2][i] = e2;
-        }
-    }
-}
-
-struct LeheAfterLds {
-    LehePlanes E;
-    LehePlanes En;
-};
-
-// update_afterCurrent for one z-marching tile (Lehe): B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt
-template <bool OPEN>
-__device__ __forceinline__ void leheAfterTile(const FieldArgs& a, float* const* Enew, int b, int zc, LeheAfterLds& L) {
+__device__ __forceinline__ void leheBHalfTile(const FieldArgs& a, int b, int zc, LehePlanes& LE) {
     ZTile const T = zTile(a.g, b, zc);
     int const nx = a.g.nx, ny = a.g.ny;
     size_t const plane = (size_t)nx * ny;
     constexpr int NE = (3 * kLEX * kLEY + 255) / 256;
     int eoff[NE];
     lehePlaneOffsets<OPEN>(a.g, T, eoff);
-    for (int k = T.z0 - 1; k <= T.z0 + 1; ++k) {
-        lehePlaneLoad<OPEN>(a.g, a.E, eoff, k, L.E);
-        lehePlaneLoad<OPEN>(a.g, Enew, eoff, k, L.En);
-    }
+    for (int k = T.z0 - 1; k <= T.z0 + 1; ++k) lehePlaneLoad<OPEN>(a.g, a.E, eoff, k, LE);
     int const lx = threadIdx.x % kZX, ly = threadIdx.x / kZX;
     bool const inside = T.x0 + lx < nx && T.y0 + ly < ny;
     size_t const ixy = (size_t)(T.x0 + lx) + (size_t)nx * (T.y0 + ly);
-    float nextE[NE], nextN[NE];  // plane k+2, fetched while plane k-1 was being computed
+    float nextE[NE];  // plane k+2, fetched while plane k-1 was being computed
     lehePlaneFetch<OPEN>(a.g, a.E, eoff, T.z0 + 2, nextE);
-    lehePlaneFetch<OPEN>(a.g, Enew, eoff, T.z0 + 2, nextN);
     for (int k = T.z0; k < T.z1; ++k) {
         __syncthreads();  // the slot of plane k+2 (plane k-2) is no longer read
-        lehePlaneStore(nextE, k + 2, L.E);
-        lehePlaneStore(nextN, k + 2, L.En);
-        if (k + 3 <= T.z1 + 1) {  // in flight during this step
-            lehePlaneFetch<OPEN>(a.g, a.E, eoff, k + 3, nextE);
-            lehePlaneFetch<OPEN>(a.g, Enew, eoff, k + 3, nextN);
-        }
+        lehePlaneStore(nextE, k + 2, LE);
+        if (k + 3 <= T.z1 + 1) lehePlaneFetch<OPEN>(a.g, a.E, eoff, k + 3, nextE);  // in flight during this step
         __syncthreads();
         if (inside) {
             size_t const i = ixy + plane * k;
             float cx, cy, cz;
-            lCurl(L.E, lx, ly, k, a, cx, cy, cz);
-            float const b0 = a.B[0][i] - cx * 0.5f * a.dt;
-            float const b1 = a.B[1][i] - cy * 0.5f * a.dt;
-            float const b2 = a.B[2][i] - cz * 0.5f * a.dt;
-            lCurl(L.En, lx, ly, k, a, cx, cy, cz);
-            a.B[0][i] = b0 - cx * 0.5f * a.dt;
-            a.B[1][i] = b1 - cy * 0.5f * a.dt;
-            a.B[2][i] = b2 - cz * 0.5f * a.dt;
+            lCurl(LE, lx, ly, k, a, cx, cy, cz);
+            a.B[0][i] = a.B[0][i] - cx * 0.5f * a.dt;
+            a.B[1][i] = a.B[1][i] - cy * 0.5f * a.dt;
+            a.B[2][i] = a.B[2][i] - cz * 0.5f * a.dt;
         }
     }
 }
@@ -779,24 +707,23 @@ __global__ __launch_bounds__(256) void kYeeBeforeAdd(FieldArgs a, float* Enew0, 
 }
 
 template <bool OPEN>
-__global__ __launch_bounds__(256) void kLeheBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
-    __shared__ LeheBeforeLds L;
-    float* const En[3] = {Enew0, Enew1, Enew2};
-    leheBeforeAddTile<OPEN>(a, En, blockIdx.x, zc, L);
-}
-
-template <bool OPEN>
-__global__ __launch_bounds__(256) void kLeheAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
-    __shared__ LeheAfterLds L;
-    float* const En[3] = {Enew0, Enew1, Enew2};
-    leheAfterTile<OPEN>(a, En, blockIdx.x, zc, L);
-}
-
-template <bool OPEN>
 __global__ __launch_bounds__(256) void kYeeAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
     __shared__ YeeAfterLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
-    yeeAfterTile<OPEN>(a, En, blockIdx.x, zc, L);
+    yeeAfterTile<OPEN, true>(a, En, blockIdx.x, zc, L);
+}
+
+// one B half step (the stage call kBHalf) in the z-marching form
+template <bool OPEN>
+__global__ __launch_bounds__(256) void kLeheBHalfZ(FieldArgs a, int zc) {
+    __shared__ LehePlanes L;
+    leheBHalfTile<OPEN>(a, blockIdx.x, zc, L);
+}
+
+template <bool OPEN>
+__global__ __launch_bounds__(256) void kYeeBHalfZ(FieldArgs a, int zc) {
+    __shared__ YeeAfterLds L;
+    yeeAfterTile<OPEN, false>(a, nullptr, blockIdx.x, zc, L);
 }
 
 __global__ __launch_bounds__(256) void kYeeBeforeAddBatch(const FieldBatchEntry* __restrict__ table,
@@ -812,7 +739,7 @@ __global__ __launch_bounds__(256) void kYeeAfterBatch(const FieldBatchEntry* __r
     __shared__ YeeAfterLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
-    yeeAfterTile<false>(e.a, e.Enew, it.y, kZC, L);
+    yeeAfterTile<false, true>(e.a, e.Enew, it.y, kZC, L);
 }
 
 template <int SOLVER, bool OPEN>
@@ -945,12 +872,36 @@ __global__ void kCountSum(const uint32_t* c, int n, unsigned long long* out) {
 
 static inline int blocksFor(size_t n) { return (int)((n + 255) / 256); }
 
+int yeeChunk(const GridDev& g);
+
 hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s) {
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    int const zc = yeeChunk(a.g);  // the z-marching form when the grid fills the chip
+    if (zc > 0) {
+        unsigned const zt = (unsigned)zTileCount(a.g, zc);
+        if (solver == PIC_SOLVER_LEHE) {
+            if (a.g.open)
+                hipLaunchKernelGGL(kLeheBHalfZ<true>, dim3(zt), dim3(256), 0, s, a, zc);
+            else
+                hipLaunchKernelGGL(kLeheBHalfZ<false>, dim3(zt), dim3(256), 0, s, a, zc);
+        } else {
+            if (a.g.open)
+                hipLaunchKernelGGL(kYeeBHalfZ<true>, dim3(zt), dim3(256), 0, s, a, zc);
+            else
+                hipLaunchKernelGGL(kYeeBHalfZ<false>, dim3(zt), dim3(256), 0, s, a, zc);
+        }
+        return hipGetLastError();
+    }
     if (solver == PIC_SOLVER_LEHE)
         hipLaunchKernelGGL(kBHalf<PIC_SOLVER_LEHE>, dim3(blocksFor(n)), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(kBHalf<PIC_SOLVER_YEE>, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchEFullJ(const FieldArgs& a, float* const Enew[3], hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    hipLaunchKernelGGL(kEFullJ, dim3(blocksFor(n)), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
     return hipGetLastError();
 }
 
@@ -1006,9 +957,24 @@ static hipError_t fieldStepT(int solver, const FieldArgs& a, float* const Enew[3
     int const tiles = (a.g.nx / kFTX) * (a.g.ny / kFTY) * (a.g.nz / kFTZ);
     int const zc = yeeChunk(a.g);  // both solvers z-march when the grid fills the chip
     int const zt = zc > 0 ? zTileCount(a.g, zc) : 0;
-    if (solver == PIC_SOLVER_LEHE && zc > 0)
-        hipLaunchKernelGGL(kLeheBeforeAdd<OPEN>, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
-    else if (solver == PIC_SOLVER_LEHE)
+    if (solver == PIC_SOLVER_LEHE && zc > 0) {
+        // Lehe: B' is written back so that each of the expensive Lehe curls is
+        // taken once (the fused pair would take the first one twice): B half,
+        // E full step and J into Enew, absorber, B half from Enew, absorber
+        hipLaunchKernelGGL(kLeheBHalfZ<OPEN>, dim3(zt), dim3(256), 0, s, a, zc);
+        size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+        hipLaunchKernelGGL(kEFullJ, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2]);
+        if (absorb) {
+            hipError_t const e = launchAbsorb(a.g, Enew, *ab, s);
+            if (e != hipSuccess) return e;
+        }
+        FieldArgs an = a;
+        for (int d = 0; d < 3; ++d) an.E[d] = Enew[d];
+        hipLaunchKernelGGL(kLeheBHalfZ<OPEN>, dim3(zt), dim3(256), 0, s, an, zc);
+        if (absorb) return launchAbsorb(a.g, a.B, *ab, s);
+        return hipGetLastError();
+    }
+    if (solver == PIC_SOLVER_LEHE)
         hipLaunchKernelGGL((kFieldBeforeAdd<PIC_SOLVER_LEHE, OPEN>), dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1],
                            Enew[2]);
     else if (zc == 0)  // grids too small for the z-marching form to fill the chip: 3D tiles of 8x8x4
@@ -1020,9 +986,7 @@ static hipError_t fieldStepT(int solver, const FieldArgs& a, float* const Enew[3
         hipError_t const e = launchAbsorb(a.g, Enew, *ab, s);
         if (e != hipSuccess) return e;
     }
-    if (solver == PIC_SOLVER_LEHE && zc > 0)
-        hipLaunchKernelGGL(kLeheAfter<OPEN>, dim3(zt), dim3(256), 0, s, a, Enew[0], Enew[1], Enew[2], zc);
-    else if (solver == PIC_SOLVER_LEHE)
+    if (solver == PIC_SOLVER_LEHE)
         hipLaunchKernelGGL((kFieldAfter<PIC_SOLVER_LEHE, OPEN>), dim3(tiles), dim3(256), 0, s, a, Enew[0], Enew[1],
                            Enew[2]);
     else if (zc == 0)

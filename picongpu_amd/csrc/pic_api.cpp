@@ -1432,11 +1432,25 @@ bool fusedFieldStep(const pic_ctx* c) { return c->incB.empty() && c->incE.empty(
 
 int fieldStep(pic_ctx* c) {
     if (!fusedFieldStep(c)) {
-        // the incident field adds between the half steps the fused kernels join
+        // the incident field adds between the half steps the fused kernels
+        // join: the stage kernels in place, E full step and J in one pass
+        pic::FieldArgs a = makeFieldArgs(c);
+        TimerScope ts(c, PIC_TIMER_FIELD);
+        PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
         int r;
-        if ((r = pic_fdtd_before_current(c)) != PIC_OK) return r;
-        if ((r = pic_add_current(c)) != PIC_OK) return r;
-        return pic_fdtd_after_current(c);
+        if ((r = applyIncident(c, c->incB, c->B, 0.0f)) != PIC_OK) return r;
+        if ((r = applyIncident(c, c->incE, c->E, 0.5f * c->k.dt / c->k.dt)) != PIC_OK) return r;
+        if (c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL) {
+            PIC_HIP(pic::launchBinomialJ(a, c->Jf, c->stream));
+            for (int d = 0; d < 3; ++d) a.J[d] = c->Jf[d];
+        }
+        PIC_HIP(pic::launchEFullJ(a, c->E, c->stream));
+        if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->E, c->absorb, c->stream));
+        if ((r = applyIncident(c, c->incB, c->B, c->k.dt / c->k.dt)) != PIC_OK) return r;
+        PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
+        if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->B, c->absorb, c->stream));
+        PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->stream));
+        return PIC_OK;
     }
     pic::FieldArgs a = makeFieldArgs(c);
     {

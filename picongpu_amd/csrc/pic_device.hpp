@@ -199,19 +199,22 @@ __device__ __forceinline__ float gather1(const float* __restrict__ tile, int cx,
     int const bx = cx + ishift[0] + begin;
     int const by = cy + ishift[1] + begin;
     int const bz = cz + ishift[2] + begin;
+    // each partial sum starts at its first term; the reference adds that term
+    // to 0.0, which differs only in the sign of an all-zero sum (a zero field
+    // value adds nothing to a momentum, whatever its sign)
     float result_z = 0.0f;
 #pragma unroll
     for (int z = 0; z < SUPP; ++z) {
         float result_y = 0.0f;
 #pragma unroll
         for (int y = 0; y < SUPP; ++y) {
-            float result_x = 0.0f;
             const float* row = tile + ((bz + z) * TY + (by + y)) * TX + bx;
+            float result_x = row[0] * sx[0];
 #pragma unroll
-            for (int x = 0; x < SUPP; ++x) result_x += row[x] * sx[x];
-            result_y += result_x * sy[y];
+            for (int x = 1; x < SUPP; ++x) result_x += row[x] * sx[x];
+            result_y = y == 0 ? result_x * sy[0] : result_y + result_x * sy[y];
         }
-        result_z += result_y * sz[z];
+        result_z = z == 0 ? result_y * sz[0] : result_z + result_y * sz[z];
     }
     return result_z;
 }

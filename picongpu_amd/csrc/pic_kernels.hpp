@@ -86,6 +86,8 @@ constexpr int kPushMaxIters = 16;
 constexpr int kListMaxSlots = kPushThreadsPerBlock * kPushMaxIters;
 hipError_t launchFill(float* p, size_t n, float v, hipStream_t s);
 hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s);
+// E full step and J -> E (None) per cell in one pass into Enew (may be a.E)
+hipError_t launchEFullJ(const FieldArgs& a, float* const Enew[3], hipStream_t s);
 hipError_t launchEFull(const FieldArgs& a, hipStream_t s);
 // binomial: the 27-point filter of currentInterpolation/Binomial.hpp (periodic box)
 hipError_t launchAddCurrent(const FieldArgs& a, bool binomial, hipStream_t s);

@@ -107,12 +107,20 @@ def test_shift_bit_exact(open_):
     sim.close()
 
 
-@pytest.mark.parametrize("solver,open_,binomial", [(abi.SOLVER_YEE, False, False), (abi.SOLVER_LEHE, False, False),
-                                                   (abi.SOLVER_YEE, True, False), (abi.SOLVER_LEHE, True, False),
-                                                   (abi.SOLVER_YEE, False, True)],
-                         ids=["yee", "lehe", "yee-open", "lehe-open", "yee-binomial"])
-def test_fields_bit_exact(solver, open_, binomial):
-    st = small_thermal()
+# the stage kernels: per-cell kernels on 16x16x12, the z-marching B half on
+# 128x128x32 (1024 tiles of 32x8 over chunks of 2 planes)
+FIELD_STAGE_CASES = [(abi.SOLVER_YEE, False, False, (16, 16, 12)), (abi.SOLVER_LEHE, False, False, (16, 16, 12)),
+                     (abi.SOLVER_YEE, True, False, (16, 16, 12)), (abi.SOLVER_LEHE, True, False, (16, 16, 12)),
+                     (abi.SOLVER_YEE, False, True, (16, 16, 12)), (abi.SOLVER_YEE, False, False, (128, 128, 32)),
+                     (abi.SOLVER_LEHE, False, False, (128, 128, 32)), (abi.SOLVER_YEE, True, False, (128, 128, 32)),
+                     (abi.SOLVER_LEHE, True, False, (128, 128, 32))]
+
+
+@pytest.mark.parametrize("solver,open_,binomial,grid", FIELD_STAGE_CASES,
+                         ids=[f"{'yee' if s == abi.SOLVER_YEE else 'lehe'}{'-open' if o else ''}"
+                              f"{'-binomial' if b else ''}-{'x'.join(map(str, g))}" for s, o, b, g in FIELD_STAGE_CASES])
+def test_fields_bit_exact(solver, open_, binomial, grid):
+    st = small_thermal(grid=grid, ppc=1)
     if open_:
         open_box(st, boundary=(1, 1, 1))
     if binomial:
@@ -121,7 +129,7 @@ def test_fields_bit_exact(solver, open_, binomial):
     st.lehe_dir = 1
     if solver == abi.SOLVER_LEHE:
         st.dt_si = 0.99 * st.cell_si[1] / setups.SPEED_OF_LIGHT_SI / 1.8
-    print(f"case fields solver={solver}", flush=True)
+    print(f"case fields solver={solver} open={open_} binomial={binomial} grid={grid}", flush=True)
     sim, ora, _ = H.make_pair(st, e_amp=1.0, b_amp=1.0)
     J = H.random_fields(st, 5, 0.1, 0.1)[0]
     sim.load_field(abi.FIELD_J, J)
