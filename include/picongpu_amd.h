@@ -307,6 +307,13 @@ int pic_deposit(pic_ctx* ctx, int32_t species);
  * incident field on B (at the field step counter) and on E (counter + 0.5),
  * E full step */
 int pic_fdtd_before_current(pic_ctx* ctx);
+/* FieldJ::asyncCommunication (FieldJ.x.cpp:156-174): the fold of the J
+ * guards into the opposite border. The deposit's tile flush already adds
+ * every contribution at its wrapped cell on a periodic axis and drops it
+ * beyond an open border, so J is complete after pic_deposit / a fused
+ * pic_push; this call only checks the context. It is the counterpart for a
+ * caller that keeps the reference's stage sequence. */
+int pic_fold_current(pic_ctx* ctx);
 /* CurrentInterpolationAndAdditionToEMF with currentInterpolation::None
  * (CurrentInterpolationAndAdditionToEMF.hpp:99-148, FDTD.hpp:78-86): E += -(dt/eps0) J */
 int pic_add_current(pic_ctx* ctx);

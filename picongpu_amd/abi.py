@@ -208,6 +208,7 @@ SIGNATURES = {
     "pic_shift": (C.c_int, [_P, C.c_int32]),
     "pic_deposit": (C.c_int, [_P, C.c_int32]),
     "pic_fdtd_before_current": (C.c_int, [_P]),
+    "pic_fold_current": (C.c_int, [_P]),
     "pic_add_current": (C.c_int, [_P]),
     "pic_fdtd_after_current": (C.c_int, [_P]),
     "pic_fdtd_step": (C.c_int, [_P]),
@@ -479,6 +480,10 @@ class Simulation:
 
     def update_before_current(self):
         check(self.lib.pic_fdtd_before_current(self.h), "pic_fdtd_before_current")
+
+    def fold_current(self):
+        """FieldJ::asyncCommunication: the fold is part of the deposit's flush; this only checks the context."""
+        check(self.lib.pic_fold_current(self.h), "pic_fold_current")
 
     def add_current(self):
         check(self.lib.pic_add_current(self.h), "pic_add_current")

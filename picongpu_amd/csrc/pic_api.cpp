@@ -1384,6 +1384,11 @@ int pic_fdtd_before_current(pic_ctx* c) {
     return PIC_OK;
 }
 
+int pic_fold_current(pic_ctx* c) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    return PIC_OK;
+}
+
 int pic_add_current(pic_ctx* c) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     pic::FieldArgs a = makeFieldArgs(c);
