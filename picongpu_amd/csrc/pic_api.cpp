@@ -1111,7 +1111,9 @@ int pic_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* cell, co
         for (int a = 0; a < 3; ++a) {
             float const p = pos[3 * i + a];
             if (!(p >= 0.0f && p < 1.0f)) return fail(PIC_E_INVALID, "in-cell position outside [0,1)");
+            if (!std::isfinite(mom[3 * i + a])) return fail(PIC_E_INVALID, "non-finite momentum");
         }
+        if (!std::isfinite(w[i])) return fail(PIC_E_INVALID, "non-finite weighting");
         int const sc = x / pic::kSCX + c->g.nscx * (y / pic::kSCY + c->g.nscy * (z / pic::kSCZ));
         int const lc = x % pic::kSCX + pic::kSCX * (y % pic::kSCY + pic::kSCY * (z % pic::kSCZ));
         if (counts[sc] >= static_cast<uint32_t>(m.cap))
@@ -1149,12 +1151,37 @@ int pic_write_frames(pic_ctx* c, int32_t s, const uint32_t* counts, const float*
     SpeciesMem& m = c->sp[s];
     pic::SpeciesDev& d = m.dev;
     size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
-    m.shiftReady = 0;
     if (counts) {
         for (int i = 0; i < c->nsc; ++i)
             if (counts[i] > static_cast<uint32_t>(m.cap)) return fail(PIC_E_CAPACITY, "count above frame capacity");
-        PIC_HIP(hipMemcpyAsync(d.count, counts, c->nsc * 4, hipMemcpyHostToDevice, c->stream));
     }
+    // the occupied slots of what is written must hold a particle the kernels
+    // can take: local cell in the supercell, position in [0,1), finite
+    // momentum and weighting, multiMask 1 or an exchange code 2..27
+    if (pos || mom || w || lc || mask) {
+        std::vector<uint32_t> n(c->nsc);
+        if (counts) {
+            std::copy(counts, counts + c->nsc, n.begin());
+        } else {
+            PIC_HIP(hipMemcpyAsync(n.data(), d.count, c->nsc * 4, hipMemcpyDeviceToHost, c->stream));
+            PIC_HIP(hipStreamSynchronize(c->stream));
+        }
+        for (int sc = 0; sc < c->nsc; ++sc)
+            for (uint32_t k = 0; k < n[sc]; ++k) {
+                size_t const i = static_cast<size_t>(sc) * m.cap + k;
+                if (lc && lc[i] >= pic::kSCV) return fail(PIC_E_INVALID, "localCellIdx outside the supercell");
+                if (mask && (mask[i] < 1 || mask[i] > 27))
+                    return fail(PIC_E_INVALID, "multiMask of an occupied slot outside 1..27");
+                if (w && !std::isfinite(w[i])) return fail(PIC_E_INVALID, "non-finite weighting");
+                for (int a = 0; a < 3; ++a) {
+                    if (pos && !(pos[a * slots + i] >= 0.0f && pos[a * slots + i] < 1.0f))
+                        return fail(PIC_E_INVALID, "in-cell position outside [0,1)");
+                    if (mom && !std::isfinite(mom[a * slots + i])) return fail(PIC_E_INVALID, "non-finite momentum");
+                }
+            }
+    }
+    m.shiftReady = 0;
+    if (counts) PIC_HIP(hipMemcpyAsync(d.count, counts, c->nsc * 4, hipMemcpyHostToDevice, c->stream));
     for (int a = 0; a < 3; ++a) {
         if (pos) PIC_HIP(hipMemcpyAsync(d.pos[a], pos + a * slots, slots * 4, hipMemcpyHostToDevice, c->stream));
         if (mom) PIC_HIP(hipMemcpyAsync(d.mom[a], mom + a * slots, slots * 4, hipMemcpyHostToDevice, c->stream));

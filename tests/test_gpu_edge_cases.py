@@ -137,3 +137,25 @@ def test_capacity_error_leaves_supercells_full():
     with pytest.raises(RuntimeError, match="capacity"):
         ora.step(1)
     ora.close()
+
+
+@pytest.mark.parametrize("field,index,value,fragment", [
+    ("cell", None, 300, "localCellIdx"), ("mask", None, 0, "multiMask"), ("pos", 1, 1.0, "position"),
+    ("pos", 0, -1e-7, "position"), ("mom", 2, np.nan, "momentum"), ("w", None, np.inf, "weighting")])
+def test_frames_that_kernels_cannot_take_are_rejected(field, index, value, fragment):
+    """pic_write_frames checks the occupied slots (local cell, multiMask,
+    position, finite momentum and weighting) before anything reaches a kernel."""
+    st = thermal(frames=2)
+    print(f"case rejected frames {field}={value}", flush=True)
+    cells, pos, mom, w = crafted(st, [10] * 12, seed=19)
+    f = setups.bin_frames(st, 0, cells, pos, mom, w, frames_per_sc=2)
+    slot = 3 * 512 + 4  # supercell 3, slot 4: occupied
+    arr = f[field]
+    if index is None:
+        arr[slot] = value
+    else:
+        arr[index, slot] = value
+    sim = abi.Simulation(st.config())
+    with pytest.raises(abi.PicError, match=fragment):
+        sim.write_frames(0, f)
+    sim.close()
