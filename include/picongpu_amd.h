@@ -79,6 +79,14 @@ extern "C" {
 #define PIC_BOUNDARY_PERIODIC 0
 #define PIC_BOUNDARY_OPEN 1
 
+/* particle boundary of a species on an open axis (particles/boundary/Kind.hpp,
+ * the species' boundaryDescription()[axis].kind with offset 0): ABSORBING
+ * (boundary/Absorbing.hpp:63-92, the default) or REFLECTING
+ * (boundary/Reflecting.hpp:38-66: the in-cell position is mirrored at the
+ * border and the momentum component negated) */
+#define PIC_PARTICLE_ABSORBING 0
+#define PIC_PARTICLE_REFLECTING 1
+
 /* field absorber at open borders (fields/absorber/Absorber.hpp): NONE, or the
  * exponential damping of absorber/exponential/Exponential.kernel:63-112 */
 #define PIC_ABSORBER_NONE 0
@@ -127,6 +135,7 @@ typedef struct pic_species_desc {
     int32_t pusher;         /* PIC_PUSHER_* */
     int32_t frames_per_sc;  /* frames of 256 slots preallocated per supercell */
     int32_t current_solver; /* PIC_CURRENT_SOLVER_*: the species' current<> flag */
+    int32_t particle_boundary[3]; /* PIC_PARTICLE_* per axis, used on open axes */
 } pic_species_desc;
 
 /* One incident-field profile: the members of profiles::BaseParam

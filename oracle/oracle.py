@@ -17,7 +17,8 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 class OracleSpecies(C.Structure):
     _fields_ = [("mass_ratio", C.c_double), ("charge_ratio", C.c_double), ("shape", C.c_int32),
-                ("pusher", C.c_int32), ("frames_per_sc", C.c_int32), ("current_solver", C.c_int32)]
+                ("pusher", C.c_int32), ("frames_per_sc", C.c_int32), ("current_solver", C.c_int32),
+                ("particle_boundary", C.c_int32 * 3)]
 
 
 class OracleIncidentProfile(C.Structure):
@@ -162,6 +163,8 @@ def config_from(setup) -> OracleConfig:
         c.species[i].pusher = s.pusher
         c.species[i].frames_per_sc = s.frames()
         c.species[i].current_solver = s.current_solver
+        for d in range(3):
+            c.species[i].particle_boundary[d] = int(s.particle_boundary[d])
     for d in range(3):
         c.boundary[d] = int(setup.boundary[d])
         for side in range(2):

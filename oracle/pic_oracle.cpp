@@ -56,6 +56,7 @@ typedef struct oracle_species {
     int32_t pusher;  // 0 none, 1 Boris, 2 Vay, 3 Higuera-Cary
     int32_t frames_per_sc;
     int32_t current_solver;  // 0 Esirkepov, 1 EmZ
+    int32_t particle_boundary[3];  // 0 absorbing, 1 reflecting (open axes)
 } oracle_species;
 
 typedef struct oracle_incident_profile {
@@ -494,21 +495,11 @@ void higueraCary(const V3<T>& e, const V3<T>& b, V3<T>& mom, V3<T>& vel, T mass,
 }
 
 // ---- moveParticle: particles/MoveParticle.hpp:47-160 -------------------------
-template <typename T>
-int moveParticle(T* newPos, uint16_t& localCellIdx, uint8_t& multiMask, int* cellStep) {
-    volatile T shift = T(0.5);
+// the cell part of moveParticle (MoveParticle.hpp:139-157): localCellIdx and
+// multiMask after the cell steps dir; returns the multiMask value
+int cellAfterSteps(const int* dirIn, uint16_t& localCellIdx, uint8_t& multiMask) {
     int const sc[3] = {SCX, SCY, SCZ};
-    int dir[3];
-    for (int i = 0; i < 3; ++i) {
-        T pos = newPos[i] - shift;
-        T moveDir = T(0.0);
-        if (pos < T(-0.5)) moveDir = T(-1.0);
-        if (pos >= T(0.5)) moveDir = T(1.0);
-        pos -= moveDir;
-        newPos[i] = pos + shift;
-        dir[i] = static_cast<int>(moveDir);
-        cellStep[i] = dir[i];
-    }
+    int dir[3] = {dirIn[0], dirIn[1], dirIn[2]};
     int newMultimask = 1;
     if (dir[0] != 0 || dir[1] != 0 || dir[2] != 0) {
         int const idx = localCellIdx;
@@ -527,6 +518,24 @@ int moveParticle(T* newPos, uint16_t& localCellIdx, uint8_t& multiMask, int* cel
     }
     return newMultimask;
 }
+
+template <typename T>
+int moveParticle(T* newPos, uint16_t& localCellIdx, uint8_t& multiMask, int* cellStep) {
+    volatile T shift = T(0.5);
+    int dir[3];
+    for (int i = 0; i < 3; ++i) {
+        T pos = newPos[i] - shift;
+        T moveDir = T(0.0);
+        if (pos < T(-0.5)) moveDir = T(-1.0);
+        if (pos >= T(0.5)) moveDir = T(1.0);
+        pos -= moveDir;
+        newPos[i] = pos + shift;
+        dir[i] = static_cast<int>(moveDir);
+        cellStep[i] = dir[i];
+    }
+    return cellAfterSteps(dir, localCellIdx, multiMask);
+}
+
 
 // ---- Esirkepov: fields/currentDeposition/Esirkepov/Esirkepov.hpp:61-242 ----
 // `add(comp, x, y, z, value)` receives each accumulated_J at cell (x,y,z)
@@ -758,6 +767,49 @@ void scCoords(const State<T>& st, int sc, int& sx, int& sy, int& sz) {
 
 // One particle: gather, push, move (PushParticlePerFrame, Particles.kernel:275-316).
 // Returns the velocity and writes the cell step of the move.
+// boundary::ReflectParticleIfOutside (boundary/Reflecting.hpp:38-66, offset
+// 0) as boundary::apply runs it after the push, exchange by exchange in the
+// order +x, -x, +y, -y, +z, -z (Apply.hpp, Utility.hpp:35-48): a particle
+// whose cell lies beyond a reflecting border of an open axis gets the in-cell
+// position -pos - epsilon (upper border) or 2 - pos (lower border) on that
+// axis, the momentum component negated, and moveParticle (all three axes)
+// moves it back. The deposit stage then sees the reflected particle, whose
+// velocity component is the negated one. cell: global cell after the push;
+// step: accumulated cell steps. Returns whether anything was reflected.
+template <typename T>
+bool reflectParticle(const State<T>& st, const Species<T>& S, int* cell, T* pos, V3<T>& mom, V3<T>& vel, int* step) {
+    int const n[3] = {st.nx, st.ny, st.nz};
+    bool any = false;
+    for (int d = 0; d < 3; ++d) {
+        if (S.desc.particle_boundary[d] != 1 || st.cfg.boundary[d] != 1) continue;
+        bool const beyondEnd = cell[d] >= n[d];
+        bool const beforeBegin = cell[d] < 0;
+        if (!beyondEnd && !beforeBegin) continue;
+        if (beyondEnd)
+            pos[d] = -pos[d] - std::numeric_limits<T>::epsilon();
+        else
+            pos[d] = T(2.0) - pos[d];
+        T* const m = d == 0 ? &mom.x : (d == 1 ? &mom.y : &mom.z);
+        T* const v = d == 0 ? &vel.x : (d == 1 ? &vel.y : &vel.z);
+        *m = -*m;
+        *v = -*v;
+        // moveParticle (MoveParticle.hpp:47-74) on the whole position
+        volatile T shift = T(0.5);
+        for (int i = 0; i < 3; ++i) {
+            T p = pos[i] - shift;
+            T moveDir = T(0.0);
+            if (p < T(-0.5)) moveDir = T(-1.0);
+            if (p >= T(0.5)) moveDir = T(1.0);
+            p -= moveDir;
+            pos[i] = p + shift;
+            step[i] += static_cast<int>(moveDir);
+            cell[i] += static_cast<int>(moveDir);
+        }
+        any = true;
+    }
+    return any;
+}
+
 template <typename T>
 V3<T> pushOne(State<T>& st, Species<T>& S, size_t g, int ox, int oy, int oz, int* step) {
     int const lc = S.cell[g];
@@ -782,7 +834,16 @@ V3<T> pushOne(State<T>& st, Species<T>& S, size_t g, int ox, int oy, int oz, int
     else
         boris(e, b, mom, vel, mass, charge, st.k);
     for (int d = 0; d < 3; ++d) pos[d] += (vel[d] * st.k.dt) / st.k.cell[d];
+    uint16_t const lc0 = S.cell[g];
+    uint8_t const mask0 = S.mask[g];
     moveParticle(pos, S.cell[g], S.mask[g], step);
+    int cellNow[3] = {cx + step[0], cy + step[1], cz + step[2]};
+    if (reflectParticle(st, S, cellNow, pos, mom, vel, step)) {
+        // the particle's cell and multiMask after both moves, from where it started
+        S.cell[g] = lc0;
+        S.mask[g] = mask0;
+        cellAfterSteps(step, S.cell[g], S.mask[g]);
+    }
     for (int d = 0; d < 3; ++d) S.pos[d][g] = pos[d];
     S.mom[0][g] = mom.x;
     S.mom[1][g] = mom.y;

@@ -19,6 +19,7 @@ MAX_SPECIES = 4
 SHAPE_CIC, SHAPE_TSC, SHAPE_PQS = 1, 2, 3
 PUSHER_NONE, PUSHER_BORIS, PUSHER_VAY, PUSHER_HIGUERA_CARY = 0, 1, 2, 3
 BOUNDARY_PERIODIC, BOUNDARY_OPEN = 0, 1
+PARTICLE_ABSORBING, PARTICLE_REFLECTING = 0, 1
 ABSORBER_NONE, ABSORBER_EXPONENTIAL = 0, 1
 CURRENT_NONE, CURRENT_BINOMIAL = 0, 1
 CURRENT_SOLVER_ESIRKEPOV, CURRENT_SOLVER_EMZ = 0, 1
@@ -41,6 +42,7 @@ class SpeciesDesc(C.Structure):
         ("pusher", C.c_int32),
         ("frames_per_sc", C.c_int32),
         ("current_solver", C.c_int32),
+        ("particle_boundary", C.c_int32 * 3),
     ]
 
 
@@ -307,6 +309,8 @@ def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YE
         cfg.species[i].pusher = int(s["pusher"])
         cfg.species[i].frames_per_sc = int(s["frames_per_sc"])
         cfg.species[i].current_solver = int(s.get("current_solver", 0))
+        for d, kind in enumerate(s.get("particle_boundary", (0, 0, 0))):
+            cfg.species[i].particle_boundary[d] = int(kind)
     cfg.j_scale_exp = int(j_scale_exp)
     cfg.device = int(device)
     for d in range(3):

@@ -231,6 +231,12 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                     p[2] += (vel.z * pk.dt) / pk.cell[2];
                     int dir[3], newIdx, newMask;
                     moveParticle(p, lidx, dir, newIdx, newMask);
+                    if (S.reflect) {  // the species' reflecting borders (boundary::apply after the push)
+                        int cellNow[3] = {ox + lx + dir[0], oy + ly + dir[1], oz + lz + dir[2]};
+                        int const nGrid[3] = {a.g.nx, a.g.ny, a.g.nz};
+                        if (reflectParticle(nGrid, S.reflect, cellNow, p, mom, vel, dir))
+                            cellAfterMove(lidx, dir, newIdx, newMask);
+                    }
                     S.pos[0][gidx] = p[0];
                     S.pos[1][gidx] = p[1];
                     S.pos[2][gidx] = p[2];

@@ -71,6 +71,12 @@ int validateConfig(const pic_config& c) {
         if (d.frames_per_sc <= 0 || d.frames_per_sc > 255)
             return fail(PIC_E_INVALID, "frames_per_sc must be in [1,255]");
         if (!(d.mass_ratio > 0.0)) return fail(PIC_E_INVALID, "mass_ratio must be positive");
+        for (int a = 0; a < 3; ++a) {
+            if (d.particle_boundary[a] != PIC_PARTICLE_ABSORBING && d.particle_boundary[a] != PIC_PARTICLE_REFLECTING)
+                return fail(PIC_E_INVALID, "particle_boundary must be PIC_PARTICLE_ABSORBING or _REFLECTING");
+            if (d.particle_boundary[a] == PIC_PARTICLE_REFLECTING && c.boundary[a] != PIC_BOUNDARY_OPEN)
+                return fail(PIC_E_INVALID, "a reflecting particle boundary needs an open axis");
+        }
     }
     for (int d = 0; d < 3; ++d) {
         if (c.boundary[d] != PIC_BOUNDARY_PERIODIC && c.boundary[d] != PIC_BOUNDARY_OPEN)
@@ -1023,6 +1029,9 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
             return bail(PIC_E_HIP);
         d.cap = m.cap;
         d.lists = m.cap <= pic::kListMaxSlots ? 1 : 0;
+        d.reflect = 0;
+        for (int a = 0; a < 3; ++a)
+            if (cfg->species[s].particle_boundary[a] == PIC_PARTICLE_REFLECTING) d.reflect |= 1 << a;
         d.mass = m.mass;
         d.charge = m.charge;
         c->sp.push_back(m);

@@ -361,13 +361,13 @@ __device__ __forceinline__ void pushHigueraCary(Vec3 e, Vec3 b, Vec3& mom, Vec3&
 // entry, [0,1) on exit; dir receives the cell step per axis; returns the new
 // local cell index (wrapped into the destination supercell) and multiMask.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void moveParticle(float* pos, int localIdx, int* dir, int& newIdx, int& newMask) {
+// the position part of moveParticle: pos in [-1,2) per axis to [0,1), the
+// cell step per axis into dir
+__device__ __forceinline__ void moveAxes(float* pos, int* dir) {
     // The reference marks `shift` volatile on HIP to keep (p-0.5)+0.5 from
     // being folded; the asm barrier below has the same effect.
     float shift = 0.5f;
     asm volatile("" : "+v"(shift));
-    const int sc[3] = {SCX, SCY, SCZ};
-    bool moved = false;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         float p = pos[i] - shift;
@@ -377,11 +377,16 @@ __device__ __forceinline__ void moveParticle(float* pos, int localIdx, int* dir,
         p -= moveDir;
         pos[i] = p + shift;
         dir[i] = (int)moveDir;
-        moved |= dir[i] != 0;
     }
+}
+
+// the cell part of moveParticle: local cell index after the steps dir
+// (wrapped into the destination supercell) and the multiMask towards it
+__device__ __forceinline__ void cellAfterMove(int localIdx, const int* dir, int& newIdx, int& newMask) {
+    const int sc[3] = {SCX, SCY, SCZ};
     newIdx = localIdx;
     newMask = 1;
-    if (moved) {
+    if ((dir[0] | dir[1] | dir[2]) != 0) {
         int lc[3] = {localIdx % SCX, (localIdx / SCX) % SCY, localIdx / (SCX * SCY)};
         int ex[3];
 #pragma unroll
@@ -398,6 +403,51 @@ __device__ __forceinline__ void moveParticle(float* pos, int localIdx, int* dir,
             exchangeType *= 3;
         }
     }
+}
+
+__device__ __forceinline__ void moveParticle(float* pos, int localIdx, int* dir, int& newIdx, int& newMask) {
+    moveAxes(pos, dir);
+    cellAfterMove(localIdx, dir, newIdx, newMask);
+}
+
+// boundary::ReflectParticleIfOutside (boundary/Reflecting.hpp:38-66) with
+// offset 0, as boundary::apply runs it over the exchanges +x, -x, +y, -y, +z,
+// -z (Apply.hpp, Utility.hpp getAllAxisAlignedExchanges): for each axis in
+// `reflect` whose border the particle's cell `cell` (after the move) lies
+// beyond, the in-cell position is mirrored (-pos - epsilon above the upper
+// border, 2 - pos below the lower one), the momentum and velocity components
+// are negated (Velocity is odd in each momentum component) and moveParticle's
+// position arithmetic runs on all three axes; dir accumulates the steps.
+// n: the grid's cells per axis. Returns whether anything was reflected.
+__device__ __forceinline__ bool reflectParticle(const int* n, int reflect, int* cell, float* pos, Vec3& mom, Vec3& vel,
+                                                int* dir) {
+    bool any = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!((reflect >> d) & 1)) continue;
+        bool const above = cell[d] >= n[d], below = cell[d] < 0;
+        if (!(above || below)) continue;
+        pos[d] = above ? -pos[d] - 1.1920929e-7f : 2.0f - pos[d];  // std::numeric_limits<float>::epsilon()
+        if (d == 0) {
+            mom.x = -mom.x;
+            vel.x = -vel.x;
+        } else if (d == 1) {
+            mom.y = -mom.y;
+            vel.y = -vel.y;
+        } else {
+            mom.z = -mom.z;
+            vel.z = -vel.z;
+        }
+        int step[3];
+        moveAxes(pos, step);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            dir[i] += step[i];
+            cell[i] += step[i];
+        }
+        any = true;
+    }
+    return any;
 }
 
 // ---------------------------------------------------------------------------

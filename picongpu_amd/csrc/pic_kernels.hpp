@@ -27,6 +27,7 @@ struct SpeciesDev {
     uint16_t* leaveBits; // per supercell and push thread: leaver flags per loop iteration
     int32_t cap;         // slots per supercell
     int32_t lists;       // 1: the push kernel builds the leaver lists of this species
+    int32_t reflect;     // bit d: particles crossing an open border of axis d are reflected
     float mass;          // frame::getMass()
     float charge;        // frame::getCharge()
 };

@@ -35,6 +35,7 @@ class SpeciesSetup:
     ramp: tuple = None  # (axis, start_si, length_si) of a linear density ramp
     frames_per_sc: int = 0  # 0: derived from ppc
     current_solver: int = 0  # abi.CURRENT_SOLVER_*
+    particle_boundary: tuple = (0, 0, 0)  # abi.PARTICLE_* per axis (open axes)
 
     def frames(self) -> int:
         if self.frames_per_sc:
@@ -68,7 +69,8 @@ class Setup:
 
     def config(self, device: int = 0, j_scale_exp: int = 0) -> abi.Config:
         sp = [dict(mass_ratio=s.mass_ratio, charge_ratio=s.charge_ratio, shape=s.shape, pusher=s.pusher,
-                   frames_per_sc=s.frames(), current_solver=s.current_solver) for s in self.species]
+                   frames_per_sc=s.frames(), current_solver=s.current_solver,
+                   particle_boundary=s.particle_boundary) for s in self.species]
         return abi.make_config(self.grid, self.cell_si, self.dt_si, self.density_si, self.typical_ppc, sp,
                                solver=self.solver, lehe_dir=self.lehe_dir, j_scale_exp=j_scale_exp, device=device,
                                boundary=self.boundary, absorber=self.absorber, absorber_cells=self.absorber_cells,

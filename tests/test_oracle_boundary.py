@@ -164,3 +164,117 @@ def test_binomial_filter_weights_and_conservation():
     np.testing.assert_allclose(E.sum(axis=1), float(k.add_current_coeff) * J.astype(np.float64).sum(axis=1),
                                rtol=1e-9, atol=1e-9 * abs(float(k.add_current_coeff)) * st.ncells)
     ora.close()
+
+
+# ---- reflecting particle boundary (boundary/Reflecting.hpp:38-66) ----------
+def reflecting_box(axes, grid=(16, 16, 12)):
+    st = setups.thermal(grid=grid, ppc=1, pusher=abi.PUSHER_BORIS)
+    st.boundary = (1, 1, 1)
+    st.species[0].particle_boundary = tuple(abi.PARTICLE_REFLECTING if d in axes else abi.PARTICLE_ABSORBING
+                                            for d in range(3))
+    return st
+
+
+def one_particle(st, cell, pos, beta_dir):
+    """One electron at `cell`/`pos` moving with velocity beta_dir (units of c)
+    in zero fields; returns the oracle, its momentum and the cells it moves
+    per step along each axis (float64)."""
+    ora = O.Oracle(st, 32, 0)
+    k = O.constants(st)
+    w = np.float32(k.typical_weighting)
+    mass = float(np.float32(k.species_mass[0]) * w)
+    beta = np.asarray(beta_dir, np.float64)
+    g = 1.0 / math.sqrt(1.0 - float(beta @ beta))
+    p = (g * beta * mass * float(k.c)).astype(np.float32)
+    f = setups.bin_frames(st, 0, np.array([cell], np.int32), np.array([pos], np.float32), p[None, :],
+                          np.array([w], np.float32))
+    ora.set_frames(0, f)
+    p64 = p.astype(np.float64)
+    v = p64 / np.sqrt(mass * mass + (p64 @ p64) / float(k.c) ** 2)  # Velocity.hpp
+    per_step = v * float(k.dt) / np.array([k.cell[0], k.cell[1], k.cell[2]], np.float64)
+    return ora, p, per_step
+
+
+def particle_state(ora, st):
+    f = ora.get_frames(0)
+    sc = int(np.nonzero(f["counts"])[0][0])
+    cap = f["w"].shape[0] // f["counts"].shape[0]
+    slot = sc * cap
+    nscx, nscy = st.grid[0] // 8, st.grid[1] // 8
+    lc = int(f["cell"][slot])
+    cell = np.array([(sc % nscx) * 8 + lc % 8, ((sc // nscx) % nscy) * 8 + (lc // 8) % 8,
+                     (sc // (nscx * nscy)) * 4 + lc // 64])
+    return cell, f["pos"][:, slot].astype(np.float64), f["mom"][:, slot].astype(np.float64), int(f["counts"].sum())
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("side", [1, -1])
+def test_reflecting_border_mirrors_the_particle(axis, side):
+    """Crossing a reflecting border the particle comes back mirrored: global
+    coordinate 2 X_b - x of where the unreflected move ends (X_b the border),
+    to FP32 round-off plus the epsilon the upper border subtracts; the momentum
+    component is negated, the other components and coordinates unchanged."""
+    st = reflecting_box(axes=(axis,))
+    n = st.grid[axis]
+    cell = [5, 5, 5]
+    pos = [0.5, 0.5, 0.5]
+    cell[axis] = n - 1 if side > 0 else 0
+    pos[axis] = 0.9 if side > 0 else 0.1
+    beta = [0.0, 0.0, 0.0]
+    beta[axis] = 0.5 * side
+    ora, p, per_step = one_particle(st, cell, pos, beta)
+    x0 = cell[axis] + pos[axis]
+    x_unreflected = x0 + per_step[axis]
+    assert (x_unreflected > n) if side > 0 else (x_unreflected < 0)  # the move crosses the border
+    ora.step(1)
+    c1, pos1, mom1, count = particle_state(ora, st)
+    assert count == 1
+    border = n if side > 0 else 0
+    x1 = c1[axis] + pos1[axis]
+    assert x1 == pytest.approx(2 * border - x_unreflected, abs=4e-7)
+    assert c1[axis] == (n - 1 if side > 0 else 0)
+    assert mom1[axis] == -float(p[axis])
+    for d in range(3):
+        if d != axis:
+            assert c1[d] == 5 and pos1[d] == 0.5 and mom1[d] == float(p[d])
+    ora.close()
+
+
+def test_reflecting_corner_mirrors_both_axes():
+    st = reflecting_box(axes=(0, 1))
+    n = st.grid
+    ora, p, per_step = one_particle(st, [n[0] - 1, 0, 5], [0.95, 0.05, 0.5], [0.4, -0.4, 0.0])
+    x_unref = np.array([n[0] - 1 + 0.95, 0.05, 5.5]) + per_step
+    ora.step(1)
+    c1, pos1, mom1, count = particle_state(ora, st)
+    assert count == 1
+    assert c1[0] == n[0] - 1 and c1[1] == 0 and c1[2] == 5
+    assert c1[0] + pos1[0] == pytest.approx(2 * n[0] - x_unref[0], abs=4e-7)
+    assert c1[1] + pos1[1] == pytest.approx(-x_unref[1], abs=4e-7)
+    assert mom1[0] == -float(p[0]) and mom1[1] == -float(p[1]) and mom1[2] == float(p[2])
+    ora.close()
+
+
+def test_reflecting_box_keeps_every_particle():
+    """A thermal plasma in a box whose borders all reflect keeps its particles,
+    every one inside the box with in-cell positions in [0, 1]; the same box with
+    absorbing borders loses particles."""
+    counts = {}
+    for kind in (abi.PARTICLE_REFLECTING, abi.PARTICLE_ABSORBING):
+        st = setups.thermal(grid=(16, 16, 12), ppc=2, pusher=abi.PUSHER_BORIS)
+        st.boundary = (1, 1, 1)
+        st.species[0].u_th = (0.3, 0.3, 0.3)
+        st.species[0].particle_boundary = (kind, kind, kind)
+        ora = O.Oracle(st, 32, 0)
+        f0 = setups.bin_frames(st, 0, *setups.host_particles(st, 0))
+        ora.set_frames(0, f0)
+        ora.step(12)
+        f = ora.get_frames(0)
+        counts[kind] = int(f["counts"].sum())
+        cap = f["w"].shape[0] // f["counts"].shape[0]
+        valid = (np.arange(cap)[None, :] < f["counts"][:, None]).ravel()
+        assert np.all((f["pos"][:, valid] >= 0.0) & (f["pos"][:, valid] <= 1.0))
+        n0 = int(f0["counts"].sum())
+        ora.close()
+    assert counts[abi.PARTICLE_REFLECTING] == n0
+    assert counts[abi.PARTICLE_ABSORBING] < n0
