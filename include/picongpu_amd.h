@@ -215,7 +215,7 @@ typedef struct pic_constants {
 } pic_constants;
 
 /* Device-side initial state in the manner of KernelFillGridWithParticles +
- * startPosition::Random + Drift + Temperature (particles/ParticlesInit.kernel:66-302,
+ * startPosition::Random (or Quiet) + Drift + Temperature (particles/ParticlesInit.kernel:66-302,
  * startPosition/RandomImpl.hpp:46-56, manipulators/unary/Drift.hpp:53-76,
  * Temperature.hpp:60-84): ppc particles per cell at uniform random in-cell
  * positions, weighting density*V/ppc, momentum mass*(u_drift + u_th*N(0,1)).
@@ -236,6 +236,11 @@ typedef struct pic_init_desc {
     int32_t profile_axis;
     double ramp_start_si;
     double ramp_length_si;
+    /* start position: all zero = startPosition::Random; otherwise
+     * startPosition::Quiet (startPosition/QuietImpl.hpp:34-76) with these
+     * particles per dimension (their product = ppc): the k-th particle of a
+     * cell sits at lattice point ppc-1-k (x fastest), position (i + 1/2)/n_i */
+    int32_t quiet[3];
 } pic_init_desc;
 
 /* Gaussian laser pulse set as the initial field (the reference injects it

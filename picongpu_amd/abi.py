@@ -161,6 +161,7 @@ class InitDesc(C.Structure):
         ("profile_axis", C.c_int32),
         ("ramp_start_si", C.c_double),
         ("ramp_length_si", C.c_double),
+        ("quiet", C.c_int32 * 3),
     ]
 
 
@@ -427,8 +428,10 @@ class Simulation:
                                         ptr(f["mom"], C.c_float), ptr(f["w"], C.c_float), ptr(f["cell"], C.c_uint16),
                                         ptr(f["mask"], C.c_uint8)), "pic_write_frames")
 
-    def init_species(self, sp, seed, ppc, density_si, u_drift=(0, 0, 0), u_th=(0, 0, 0), drift_band=0, ramp=None):
-        """ramp: None (uniform) or (axis, start_si, length_si) of a linear density ramp"""
+    def init_species(self, sp, seed, ppc, density_si, u_drift=(0, 0, 0), u_th=(0, 0, 0), drift_band=0, ramp=None,
+                     quiet=None):
+        """ramp: None (uniform) or (axis, start_si, length_si) of a linear density ramp;
+        quiet: None (random positions) or particles per dimension of the quiet start"""
         d = InitDesc()
         d.seed = int(seed)
         d.ppc = int(ppc)
@@ -442,6 +445,9 @@ class Simulation:
             d.profile_axis = int(ramp[0])
             d.ramp_start_si = float(ramp[1])
             d.ramp_length_si = float(ramp[2])
+        if quiet is not None:
+            for i in range(3):
+                d.quiet[i] = int(quiet[i])
         check(self.lib.pic_init_species(self.h, sp, C.byref(d)), "pic_init_species")
 
     def add_laser(self, wavelength_si, a0, waist_si, fwhm_si, center_si, propagation_axis, polarisation_axis):

@@ -62,9 +62,21 @@ __global__ __launch_bounds__(256) void kInit(InitArgs a) {
         int const gz = sz * kSCZ + lc / (kSCX * kSCY);
         uint64_t const cell = (uint64_t)gx + (uint64_t)a.g.nx * ((uint64_t)gy + (uint64_t)a.g.ny * (uint64_t)gz);
         uint32_t const spc = (uint32_t)a.species;
-        a.sp.pos[0][g] = uniformF(rngKey(a.seed, spc, cell, (uint32_t)k, 0u));
-        a.sp.pos[1][g] = uniformF(rngKey(a.seed, spc, cell, (uint32_t)k, 1u));
-        a.sp.pos[2][g] = uniformF(rngKey(a.seed, spc, cell, (uint32_t)k, 2u));
+        if (a.quiet[0] > 0) {
+            // QuietImpl (startPosition/QuietImpl.hpp:34-49): particle k of the
+            // cell takes lattice point ppc-1-k, mapToND with x fastest
+            int const idx = a.ppc - 1 - k;
+            int const q[3] = {idx % a.quiet[0], (idx / a.quiet[0]) % a.quiet[1], idx / (a.quiet[0] * a.quiet[1])};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                float const spacing = 1.0f / (float)a.quiet[d];
+                a.sp.pos[d][g] = (float)q[d] * spacing + spacing * 0.5f;
+            }
+        } else {
+            a.sp.pos[0][g] = uniformF(rngKey(a.seed, spc, cell, (uint32_t)k, 0u));
+            a.sp.pos[1][g] = uniformF(rngKey(a.seed, spc, cell, (uint32_t)k, 1u));
+            a.sp.pos[2][g] = uniformF(rngKey(a.seed, spc, cell, (uint32_t)k, 2u));
+        }
         double n0, n1, n2, n3;
         normalPair(a.seed, spc, cell, (uint32_t)k, 0u, &n0, &n1);
         normalPair(a.seed, spc, cell, (uint32_t)k, 1u, &n2, &n3);

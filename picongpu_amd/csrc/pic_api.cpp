@@ -1296,6 +1296,13 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
         a.u_drift[d] = desc->u_drift[d];
         a.u_th[d] = desc->u_th[d];
     }
+    bool const quiet = desc->quiet[0] != 0 || desc->quiet[1] != 0 || desc->quiet[2] != 0;
+    if (quiet) {
+        if (desc->quiet[0] <= 0 || desc->quiet[1] <= 0 || desc->quiet[2] <= 0 ||
+            desc->quiet[0] * desc->quiet[1] * desc->quiet[2] != desc->ppc)
+            return fail(PIC_E_INVALID, "quiet start: positive particles per dimension whose product is ppc");
+        for (int d = 0; d < 3; ++d) a.quiet[d] = desc->quiet[d];
+    }
     a.profile = desc->profile;
     a.axis = desc->profile == 1 ? desc->profile_axis : 0;
     if (desc->profile == 1) {

@@ -130,6 +130,7 @@ struct InitArgs {
     int32_t axis;
     double rampStart;    // in cells
     double rampLength;   // in cells
+    int32_t quiet[3];    // particles per dimension of the quiet start, 0: random positions
 };
 hipError_t launchInit(const InitArgs& a, hipStream_t s);
 struct LaserArgs {

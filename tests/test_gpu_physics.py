@@ -196,6 +196,38 @@ def test_device_init_matches_generator_restatement():
     sim.close()
 
 
+def test_quiet_start_lattice():
+    """Quiet start (startPosition/QuietImpl.hpp:47-72): particle k of every cell
+    sits on lattice point ppc-1-k of a 3 x 2 x 2 in-cell lattice (x fastest),
+    at q * (1/n) + (1/n) * 0.5 per axis in FP32; the momenta are those of the
+    random start of the same seed."""
+    quiet = (3, 2, 2)
+    st = setups.thermal(grid=(16, 16, 8), ppc=12)
+    print("case quiet start", flush=True)
+    sims = []
+    for q in (quiet, None):
+        st.species[0].quiet = q
+        sim = abi.Simulation(st.config())
+        setups.init_device(sim, st)
+        sims.append(sim)
+    fq, fr = (s.read_frames(0) for s in sims)
+    cap = sims[0].slots(0)
+    spacing = (np.float32(1) / np.array(quiet, np.float32)).astype(np.float32)
+    for sc in range(sims[0].nsc):
+        n = int(fq["counts"][sc])
+        assert n == 12 * 256
+        k = np.arange(n) // 256
+        idx = 12 - 1 - k
+        q = np.stack([idx % quiet[0], (idx // quiet[0]) % quiet[1], idx // (quiet[0] * quiet[1])])
+        expect = (q.astype(np.float32) * spacing[:, None] + spacing[:, None] * np.float32(0.5)).astype(np.float32)
+        sl = slice(sc * cap, sc * cap + n)
+        np.testing.assert_array_equal(H.bits(fq["pos"][:, sl]), H.bits(expect))
+        np.testing.assert_array_equal(H.bits(fq["mom"][:, sl]), H.bits(fr["mom"][:, sl]))
+        np.testing.assert_array_equal(fq["cell"][sl], fr["cell"][sl])
+    for s in sims:
+        s.close()
+
+
 def test_gauss_residual_matches_oracle():
     st = small_thermal(abi.SHAPE_TSC)
     print("case gauss residual", flush=True)
@@ -299,3 +331,40 @@ def test_laser_initial_field():
     assert np.abs(E[1]).max() == 0.0 and np.abs(E[2]).max() == 0.0 and np.abs(B[0]).max() == 0.0
     assert np.abs(ex).max() > 0.5 * e0
     sim.close()
+
+
+def test_khi_growth_rate_matches_theory():
+    """The reference's KHI growth-rate test (share/picongpu/tests/KHI_growthRate,
+    validated by lib/python/test/setups/ESKHI/config.py): on its setup (192 x 512
+    x 12 cells, 2 x 25 quiet-start particles per cell, gamma 1.021 shear flow,
+    3000 steps) the growth rate of the B_x field energy f,
+    Gamma(t_k) = 0.5 ln(f(t_k+1) / f(t_k-1)) / (t_k+1 - t_k-1) with f every 10
+    steps and t in units of the relativistic plasma frequency
+    sqrt(n e^2 / (eps0 gamma m_e)) (testsuite Math/math.py growthRate,
+    Math/physics.py plasmafrequence), must reach 1 / (sqrt(8) gamma) within the
+    acceptance of 20 %, judged on the maximum of Gamma (Math/deviation.py
+    getTestResult)."""
+    st = setups.khi_growth_rate()
+    gamma = 1.021
+    steps, period = 3000, 10
+    print(f"case KHI growth rate grid={st.grid} steps={steps}", flush=True)
+    sim = abi.Simulation(st.config())
+    setups.init_device(sim, st)
+    f = []
+    for k in range(steps // period + 1):
+        if k:
+            sim.step(period)
+        bx = sim.read_field(abi.FIELD_B)[0].astype(np.float64)
+        f.append(float(np.sum(bx * bx)))
+        if k % 50 == 0:
+            print(f"  step {k * period} sum Bx^2 {f[-1]:.4e}", flush=True)
+    sim.close()
+    f = np.array(f)
+    e, eps0, me = 1.602176634e-19, 8.8541878188e-12, 9.1093837139e-31
+    omega = math.sqrt(1e25 * e * e / (eps0 * gamma * me))
+    t = np.arange(len(f)) * period * st.dt_si * omega
+    growth = 0.5 * np.log(f[3:] / f[1:-2]) / (t[3:] - t[1:-2])
+    theory = 1.0 / (math.sqrt(8.0) * gamma)
+    deviation = abs(theory - growth.max()) / theory
+    print(f"  max growth rate {growth.max():.4f} (theory {theory:.4f}), deviation {100 * deviation:.1f} %", flush=True)
+    assert deviation <= 0.2
