@@ -1,8 +1,9 @@
 """GPU checks beyond slot-exact parity: agreement with the reference CPU's
 sequential summation order and with the FP64 twin (
 max-norm relative), fused against unfused deposition, the device initial
-state against an independent restatement of its generator, and
-size-independent properties at the full Kelvin-Helmholtz size."""
+state against an independent restatement of its generator,
+size-independent properties at the full Kelvin-Helmholtz size, and the
+Kelvin-Helmholtz and filamentation growth rates against linear theory."""
 import math
 
 import numpy as np
@@ -367,4 +368,71 @@ def test_khi_growth_rate_matches_theory():
     theory = 1.0 / (math.sqrt(8.0) * gamma)
     deviation = abs(theory - growth.max()) / theory
     print(f"  max growth rate {growth.max():.4f} (theory {theory:.4f}), deviation {100 * deviation:.1f} %", flush=True)
+    assert deviation <= 0.2
+
+
+def filamentation_rate(k_c_over_omega, beta):
+    """Growth rate (units of omega_pe, the non-relativistic plasma frequency
+    of the total density) of the purely transverse filamentation mode of two
+    symmetric cold beams at +-beta c: the positive root of
+    G^4 + G^2 (K^2 + 1/gamma^3) - K^2 beta^2 / gamma = 0, K = k c / omega_pe
+    (cold relativistic two-fluid beams with an immobile neutralising
+    background; large K gives beta / sqrt(gamma))."""
+    gamma = 1.0 / math.sqrt(1.0 - beta * beta)
+    K2 = k_c_over_omega ** 2
+    b = K2 + gamma ** -3
+    return math.sqrt((-b + math.sqrt(b * b + 4.0 * K2 * beta * beta / gamma)) / 2.0)
+
+
+def test_filamentation_growth_rate_matches_theory():
+    """Filamentation (Weibel) growth of counter-streaming beams, judged in the
+    form of the reference's MI test (lib/python/test/setups/MI/config.py:
+    theory v / (c sqrt(gamma)) in units of the non-relativistic plasma
+    frequency, acceptance 20 %). Two electron beams at +-0.3 c along z, each
+    of half the 1e25 m^-3 density, cold (0.001 c spread, quiet start), on
+    64 x 64 x 16 cells: the 16-cell z extent leaves the two-stream modes
+    stable (they need k_z v < omega_pe / gamma^1.5 = 0.93 omega_pe; the
+    longest z mode has k_z v = 1.98 omega_pe). The power of each k_z = 0
+    transverse mode of (Bx, By) is fitted log-linearly over its linear phase
+    (steps 350..700); every fitted rate must lie within 10 % of the cold
+    dispersion relation's root at that k, and the fastest within 20 % of
+    beta / sqrt(gamma)."""
+    beta = 0.3
+    st = setups.two_stream(grid=(64, 64, 16), ppc=8, beta=beta, seed=21)
+    # the theory is the cold-beam rate: a quiet start (the beams' particles
+    # paired on a 2 x 2 x 2 lattice, so the initial current noise is small)
+    # and a 0.001 c thermal spread
+    st.species[0].quiet = (2, 2, 2)
+    st.species[0].u_th = (1e-3, 1e-3, 1e-3)
+    steps, period = 700, 10
+    print(f"case filamentation growth rate grid={st.grid} steps={steps}", flush=True)
+    sim = abi.Simulation(st.config())
+    setups.init_device(sim, st)
+    modes = []
+    for k in range(steps // period + 1):
+        if k:
+            sim.step(period)
+        b = sim.read_field(abi.FIELD_B).astype(np.float64).reshape(3, st.grid[2], st.grid[1], st.grid[0])
+        # power of the k_z = 0 transverse modes: x-y Fourier plane of the z mean, [ky, kx]
+        modes.append(np.sum(np.abs(np.fft.fft2(b[:2].mean(axis=1))) ** 2, axis=0))
+    sim.close()
+    e, eps0, me, c = 1.602176634e-19, 8.8541878188e-12, 9.1093837139e-31, 2.99792458e8
+    omega = math.sqrt(1e25 * e * e / (eps0 * me))
+    t = np.arange(len(modes)) * period * st.dt_si * omega
+    window = slice(350 // period, 700 // period + 1)
+    P = np.log(np.array(modes[window]))  # B starts at zero: the log only over the window
+    t = t[window]
+    L = st.grid[0] * st.cell_si[0]
+    fastest = 0.0
+    for ny, nx in ((0, 1), (1, 0), (1, 1), (0, 2), (2, 0), (2, 1), (0, 3), (3, 0), (2, 2), (0, 4), (4, 0)):
+        rate = np.polyfit(t, P[:, ny, nx], 1)[0] / 2.0
+        kc = 2.0 * math.pi * math.hypot(nx, ny) / L * c / omega
+        expect = filamentation_rate(kc, beta)
+        print(f"  mode (kx={nx}, ky={ny}) k c/omega_pe {kc:.2f}: rate {rate:.4f}, cold theory {expect:.4f}", flush=True)
+        assert abs(rate - expect) <= 0.1 * expect
+        fastest = max(fastest, rate)
+    gamma = 1.0 / math.sqrt(1.0 - beta * beta)
+    theory = beta / math.sqrt(gamma)
+    deviation = abs(theory - fastest) / theory
+    print(f"  fastest mode {fastest:.4f} (beta/sqrt(gamma) {theory:.4f}), deviation {100 * deviation:.1f} %", flush=True)
     assert deviation <= 0.2
