@@ -189,7 +189,7 @@ def gradient_all(g: Grid, X, p, E0, B0, nsteps, lo, hi):
     p = p.clone().detach().requires_grad_(True)
     f = run(g, X, p, E0, B0, nsteps, lo, hi)
     f.backward()
-    return float(f), E0.grad.detach(), B0.grad.detach(), X.grad.detach(), p.grad.detach()
+    return float(f.detach()), E0.grad.detach(), B0.grad.detach(), X.grad.detach(), p.grad.detach()
 
 
 def grid_from_constants(shape, k, w):

@@ -218,6 +218,12 @@ class Oracle:
         lib().oracle_set_jexp(self.h, int(e))
 
     def set_frames(self, sp, f):
+        slots = self.nsc * self.cap(sp)
+        if len(f["counts"]) != self.nsc or len(f["cell"]) != slots or len(f["mask"]) != slots \
+                or np.shape(f["pos"]) != (3, slots) or np.shape(f["mom"]) != (3, slots) or len(f["w"]) != slots:
+            raise ValueError(f"frames of species {sp} do not match the layout ({self.nsc} supercells x {self.cap(sp)} slots)")
+        if np.asarray(f["counts"]).max(initial=0) > self.cap(sp):
+            raise ValueError(f"a supercell count of species {sp} exceeds its {self.cap(sp)} slots")
         lib().oracle_set_frames(self.h, sp, _p(f["counts"], C.c_uint32), _p(np.ascontiguousarray(f["pos"], np.float32), C.c_float),
                                 _p(np.ascontiguousarray(f["mom"], np.float32), C.c_float),
                                 _p(np.ascontiguousarray(f["w"], np.float32), C.c_float),

@@ -436,3 +436,50 @@ def test_filamentation_growth_rate_matches_theory():
     deviation = abs(theory - fastest) / theory
     print(f"  fastest mode {fastest:.4f} (beta/sqrt(gamma) {theory:.4f}), deviation {100 * deviation:.1f} %", flush=True)
     assert deviation <= 0.2
+
+
+@pytest.mark.parametrize("shape", [abi.SHAPE_CIC, abi.SHAPE_TSC, abi.SHAPE_PQS])
+@pytest.mark.parametrize("direction", [(1, 0, 0), (1, 1, 0), (1, 1, 1)])
+def test_current_deposition_setup_closed_form(shape, direction):
+    """The reference's CurrentDeposition test (share/picongpu/tests/CurrentDeposition)
+    through the GPU step: 24^3 cells, dt = dx / (2c), one electron at cell
+    (8,8,8) offset (0.9,0.8,0.7) moving at beta = 0.999 along the direction,
+    zero fields. J after one pic_step must equal Esirkepov's FP64 closed form
+    for that move within 1e-5 of its maximum (the reference's tolerance)."""
+    from tests.test_oracle_pinning import esirkepov_closed_form
+
+    st = setups.thermal(grid=(24, 24, 24), shape=shape)
+    st.species[0].frames_per_sc = 1
+    st.cell_si = (1e-7, 1e-7, 1e-7)
+    st.dt_si = 0.5 * st.cell_si[0] / setups.SPEED_OF_LIGHT_SI
+    print(f"case CurrentDeposition setup shape={shape} direction={direction}", flush=True)
+    sim = abi.Simulation(st.config())
+    k = sim.constants()
+    vol = st.cell_si[0] ** 3
+    w = np.float32(st.species[0].density_si * vol / st.species[0].ppc)
+    mass = float(np.float32(k.species_mass[0]) * w)
+    u = np.array(direction, np.float64)
+    beta = 0.999 * u / np.linalg.norm(u)
+    gamma = 1.0 / math.sqrt(1.0 - 0.999 ** 2)
+    cells = np.array([[8, 8, 8]], np.int32)
+    pos = np.array([[0.9, 0.8, 0.7]], np.float32)
+    mom = (gamma * mass * float(k.c) * beta)[None, :].astype(np.float32)
+    f = setups.bin_frames(st, 0, cells, pos, mom, np.array([w], np.float32))
+    sim.write_frames(0, f)
+    zero = np.zeros((3, sim.ncells), np.float32)
+    sim.load_field(abi.FIELD_E, zero)
+    sim.load_field(abi.FIELD_B, zero)
+    sim.step(1)
+    J = sim.read_field(abi.FIELD_J).astype(np.float64)
+    sim.close()
+    cell = np.array(k.cell, np.float64)
+    dt = float(k.dt)
+    d = beta * float(k.c) * dt / cell
+    end = np.array([8.9, 8.8, 8.7]) + d
+    c = np.floor(end).astype(np.int64)
+    q = float(np.float32(k.species_charge[0]) * w)
+    Jref = esirkepov_closed_form(shape, c, end - c, d, q, cell, dt, st.grid)
+    scale = np.abs(Jref).max()
+    err = np.abs(J - Jref).max() / scale
+    print(f"  max |J - closed form| / max |J| = {err:.2e}", flush=True)
+    assert err <= 1e-5
