@@ -520,7 +520,7 @@ def test_charge_conservation_over_steps(shape, order, precision):
 
 
 def test_fp32_against_fp64_twin_100_steps():
-    """North star (ii)/(iv): FP32 state against the FP64 twin after 100 steps."""
+    """FP32 state against the FP64 twin after 100 steps (1e-4, max-norm relative)."""
     st = setups.thermal(grid=(16, 16, 12), ppc=4, shape=abi.SHAPE_CIC)
     cells, pos, mom, w = setups.host_particles(st, 0)
     f = setups.bin_frames(st, 0, cells, pos, mom, w)
