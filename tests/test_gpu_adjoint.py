@@ -122,7 +122,7 @@ def test_parameter_gradient_matches_central_differences():
             tm[i] -= h
             fd[i] = (objective(tp) - objective(tm)) / (2 * h)
     err = np.abs(grad - fd) / np.abs(fd)
-    print("dF/dtheta HIP adjoint:", grad, "FP64 central differences:", fd, "rel err:", err, "F:", float(F))
+    print("dF/dtheta HIP adjoint:", grad, "FP64 central differences:", fd, "rel err:", err, "F:", float(F.detach()))
     assert np.all(err < 1e-3)
     pic.close()
 
