@@ -485,3 +485,80 @@ def test_current_deposition_setup_closed_form(shape, direction):
     err = np.abs(J - Jref).max() / scale
     print(f"  max |J - closed form| / max |J| = {err:.2e}", flush=True)
     assert err <= 1e-5
+
+
+def one_particle_position(st, frames):
+    """Global position (cells, FP64) of the only particle in the frames."""
+    cap = len(frames["cell"]) // len(frames["counts"])
+    sc = int(np.flatnonzero(frames["counts"])[0])
+    g = sc * cap
+    nscx, nscy = st.grid[0] // 8, st.grid[1] // 8
+    lc = int(frames["cell"][g])
+    cell = np.array([(sc % nscx) * 8 + lc % 8, ((sc // nscx) % nscy) * 8 + (lc // 8) % 8,
+                     (sc // (nscx * nscy)) * 4 + lc // 64], np.float64)
+    return cell + np.asarray(frames["pos"], np.float64)[:, g]
+
+
+@pytest.mark.parametrize("pusher", [abi.PUSHER_BORIS, abi.PUSHER_VAY, abi.PUSHER_HIGUERA_CARY])
+def test_single_particle_parallel_e_and_b(pusher):
+    """The reference's SingleParticle setup (share/picongpu/tests/SingleParticle,
+    fieldBackground.param: E = (0, 0, -10 MV/m), B = (0, 0, 50 T)): one
+    electron starting with u = (0.5, 0, 0.2) gyrates about z while E
+    accelerates it along z. Its weighting is 1e-6, so its own current leaves
+    the uniform fields unchanged (q/m and the trajectory do not depend on it).
+    After 1000 steps its position must match the FP64 solution of
+    dp/dt = q (E + v x B), dx/dt = v (scipy DOP853, rtol 1e-12) within the
+    FP32 bound N 2^-24 x the distance travelled (each step's momentum update
+    rounds at 2^-24 of |p|); the momentum starts half a step before the
+    position, as the leapfrog keeps it."""
+    from scipy.integrate import solve_ivp
+
+    st = setups.thermal(grid=(16, 16, 12), ppc=1, shape=abi.SHAPE_CIC, pusher=pusher)
+    st.species[0].frames_per_sc = 1
+    print(f"case single particle E || B pusher={pusher}", flush=True)
+    sim = abi.Simulation(st.config())
+    k = sim.constants()
+    uL, uT, uM, uQ = k.unit_length, k.unit_time, k.unit_mass, k.unit_charge
+    E_si, B_si = np.array([0.0, 0.0, -10.0e6]), np.array([0.0, 0.0, 50.0])
+    m_si = float(k.species_mass[0]) * uM
+    q_si = float(k.species_charge[0]) * uQ
+    c_si = setups.SPEED_OF_LIGHT_SI
+    u0 = np.array([0.5, 0.0, 0.2])
+    w = 1e-6
+    p_int = (u0 * m_si * w * c_si / (uM * uL / uT)).astype(np.float32)
+    start = np.array([[8, 8, 6]], np.int32)
+    pos0 = np.array([[0.25, 0.5, 0.75]], np.float32)
+    f = setups.bin_frames(st, 0, start, pos0, p_int[None, :], np.array([w], np.float32))
+    sim.write_frames(0, f)
+    ones = np.ones(sim.ncells, np.float32)
+    sim.load_field(abi.FIELD_E, np.stack([0 * ones, 0 * ones, ones * np.float32(E_si[2] / (uM * uL / (uT * uT * uQ)))]))
+    sim.load_field(abi.FIELD_B, np.stack([0 * ones, 0 * ones, ones * np.float32(B_si[2] / (uM / (uT * uQ)))]))
+    nsteps = 1000
+    sim.step(nsteps)
+    x_gpu = one_particle_position(st, sim.read_frames(0))
+    sim.close()
+
+    dt = float(np.float32(k.dt)) * uT
+    p_start = np.asarray(p_int, np.float64) * (uM * uL / uT) / float(np.float32(w))  # at t = -dt/2, per electron
+
+    def rhs(t, y):
+        p = y[3:]
+        gamma = math.sqrt(1.0 + float(p @ p) / (m_si * c_si) ** 2)
+        v = p / (gamma * m_si)
+        return np.concatenate([v, q_si * (E_si + np.cross(v, B_si))])
+
+    # the fields are uniform, so p evolves on its own: bring it to t = 0 first
+    p0 = solve_ivp(lambda t, p: rhs(t, np.concatenate([np.zeros(3), p]))[3:], (-dt / 2, 0.0), p_start,
+                   method="DOP853", rtol=1e-12, atol=1e-40).y[:, -1]
+    cell_si = np.array(st.cell_si)
+    x0 = (start[0] + pos0[0].astype(np.float64)) * cell_si
+    sol = solve_ivp(rhs, (0.0, nsteps * dt), np.concatenate([x0, p0]), method="DOP853", rtol=1e-12,
+                    atol=np.concatenate([np.full(3, 1e-20), np.full(3, 1e-40)]))
+    x_ref = sol.y[:3, -1] / cell_si
+    L = np.array(st.grid, np.float64)
+    diff = (x_gpu - x_ref + L / 2) % L - L / 2
+    travelled = float(np.sum(np.linalg.norm(np.diff(sol.y[:3], axis=1), axis=0))) / cell_si[0]
+    bound = nsteps * 2.0 ** -24 * travelled
+    print(f"  travelled {travelled:.1f} cells, |x - FP64| = {np.abs(diff).max():.2e} cells (FP32 bound {bound:.1e})",
+          flush=True)
+    assert np.abs(diff).max() <= bound
