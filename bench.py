@@ -284,8 +284,9 @@ def main():
         "data": "synthetic (device-seeded initial state)",
         "config": {"workload": st.name + ("+incident" if st.incident else ""), "grid": list(st.grid), "species": [s.name for s in st.species],
                    "ppc": [s.ppc for s in st.species], "shape": "TSC" if st.species[0].shape == 2 else
-                   ("CIC" if st.species[0].shape == 1 else "PQS"), "pusher": "Boris" if st.species[0].pusher == 1
-                   else "Vay", "solver": "Yee" if st.solver == 0 else "Lehe", "particles": n_particles,
+                   ("CIC" if st.species[0].shape == 1 else "PQS"),
+                   "pusher": {1: "Boris", 2: "Vay", 3: "HigueraCary"}.get(st.species[0].pusher, "none"),
+                   "solver": "Yee" if st.solver == 0 else "Lehe", "particles": n_particles,
                    "cells": ncells},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -18,7 +18,11 @@
  * Storage layout (the reference's supercell frame semantics,
  * include/pmacc/particles/memory/dataTypes/SuperCell.hpp:31-119): the box is
  * tiled by supercells of 8x8x4 cells; every supercell owns `frames_per_sc`
- * frames of 256 slots; its N particles occupy slots [0,N) so all frames are
+ * frames of 256 slots (grown for all supercells of a species when a shift or
+ * a load would overfill one, so no particle is ever dropped: the counterpart
+ * of the frames the reference takes on demand from its mallocMC heap,
+ * include/pmacc/particles/memory/boxes/ParticlesBox.hpp:96-122);
+ * its N particles occupy slots [0,N) so all frames are
  * full except the last, which holds (N-1)%256+1. localCellIdx is the
  * x-fastest linear cell index inside the supercell; multiMask is 0 (empty),
  * 1 (valid) or 2..27 (leaving towards exchange direction multiMask-1, with
@@ -43,7 +47,7 @@ extern "C" {
 #define PIC_OK 0
 #define PIC_E_INVALID -1     /* bad argument or configuration */
 #define PIC_E_HIP -2         /* HIP runtime error */
-#define PIC_E_CAPACITY -3    /* a supercell ran out of frames */
+#define PIC_E_CAPACITY -3    /* a supercell could not grow to hold its particles (device memory) */
 #define PIC_E_STATE -4       /* call not valid in the current state */
 
 /* particle shapes: include/picongpu/particles/shapes/{CIC,TSC,PQS}.hpp */
@@ -133,7 +137,9 @@ typedef struct pic_species_desc {
     double charge_ratio;    /* relative to BASE_CHARGE_SI (electron charge) */
     int32_t shape;          /* PIC_SHAPE_* */
     int32_t pusher;         /* PIC_PUSHER_* */
-    int32_t frames_per_sc;  /* frames of 256 slots preallocated per supercell */
+    int32_t frames_per_sc;  /* frames of 256 slots preallocated per supercell; every
+                               supercell of the species grows when one needs more
+                               (pic_slots_per_supercell reports the current size) */
     int32_t current_solver; /* PIC_CURRENT_SOLVER_*: the species' current<> flag */
     int32_t particle_boundary[3]; /* PIC_PARTICLE_* per axis, used on open axes */
 } pic_species_desc;
@@ -313,7 +319,8 @@ int pic_current_reset(pic_ctx* ctx);
  * on each particle's move and adds the current to J. */
 int pic_push(pic_ctx* ctx, int32_t species, int32_t fuse_deposit);
 /* shiftBetweenSupercells + periodic boundary (Particles.tpp:370-377,
- * pmacc/particles/ParticlesBase.kernel:61-630, boundary/Apply.hpp:74-116) */
+ * pmacc/particles/ParticlesBase.kernel:61-630, boundary/Apply.hpp:74-116);
+ * a supercell that overflows grows before the call returns (it synchronises) */
 int pic_shift(pic_ctx* ctx, int32_t species);
 /* CurrentDeposition from the current particle state (unfused form) */
 int pic_deposit(pic_ctx* ctx, int32_t species);
@@ -346,7 +353,15 @@ int pic_fdtd_step(pic_ctx* ctx);
  * at 0 and lives on the device, so captured steps advance it too. */
 int pic_set_step(pic_ctx* ctx, int64_t step);
 int pic_get_step(pic_ctx* ctx, int64_t* step);
-/* nsteps full PIC steps in the reference stage order, push and deposit fused */
+/* nsteps full PIC steps in the reference stage order, push and deposit fused.
+ * When a shift finds a supercell full, the records it cannot place wait in
+ * the senders' outboxes and the following steps are skipped on the device;
+ * before returning, the supercells of that species grow, the waiting records
+ * are placed where a larger capacity would have put them and the skipped
+ * steps run, so the result equals that of a larger frames_per_sc bit for bit
+ * (PIC_FLAG_CAPACITY_GREW is raised). Only a growth that cannot get device
+ * memory loses particles: then PIC_E_CAPACITY, and every later step call
+ * returns PIC_E_STATE. */
 int pic_step(pic_ctx* ctx, int32_t nsteps);
 /* the same steps replayed from one captured hipGraph of a step (captured on
  * first use, recaptured when the current's fixed-point scale changes);
@@ -354,14 +369,19 @@ int pic_step(pic_ctx* ctx, int32_t nsteps);
 int pic_step_graph(pic_ctx* ctx, int32_t nsteps);
 
 /* --- diagnostics ------------------------------------------------------- */
-/* sum over cells of E^2 and B^2 in FP64 (EnergyFields.x.cpp:183-247 without units) */
-int pic_field_energy(pic_ctx* ctx, double out[2]);
+/* EnergyFields::getEnergyFields (plugins/EnergyFields.x.cpp:183-247): per
+ * component the FP64 sum over cells of the FP32 square, times 0.5/mue0*V for
+ * B and eps0*V*0.5 for E (the reference's float/double mix), in joules.
+ * out[0] = total, out[1..3] = B x, y, z, out[4..6] = E x, y, z: the columns of
+ * the reference's output line after the step number. */
+int pic_field_energy(pic_ctx* ctx, double out[7]);
 /* Gauss-law residual eps0*div(E) - rho at the cell origins, FP64, nx*ny*nz
  * values x fastest; rho from every species' charge assignment shape
  * (ChargeConservation.tpp:181-260). Charge conservation keeps it constant
  * in time up to round-off. */
 int pic_gauss_residual(pic_ctx* ctx, double* out);
-/* bit flags of device-side errors since the last call (1 = frame capacity exceeded) */
+/* PIC_FLAG_* bits raised since the last call */
+#define PIC_FLAG_CAPACITY_GREW 1u /* a supercell overflowed and the species' supercells grew (nothing lost) */
 int pic_error_flags(pic_ctx* ctx, uint32_t* flags);
 /* per-kernel timing with HIP events on the context stream */
 int pic_timing_enable(pic_ctx* ctx, int32_t enable);
@@ -413,7 +433,8 @@ int pic_adjoint_backward(pic_ctx* ctx, const double* dE, const double* dB, doubl
  * diagnosed through the single-simulation calls on pic_ensemble_member();
  * stepping a member on its own is allowed between ensemble steps only in even
  * numbers (the fused field step alternates two E buffers). Every member must
- * use the same shape, pusher and solver, at most 32 frames per supercell. */
+ * use the same shape, pusher and solver, at most 64 frames per supercell (a
+ * member that overflows grows up to that and runs its skipped steps on its own). */
 typedef struct pic_ensemble pic_ensemble;
 
 /* order of the work items of a batched phase */

@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
             "oracle_current_reset": (None, [_P]),
             "oracle_push": (None, [_P, C.c_int32, C.c_int32]),
             "oracle_deposit": (None, [_P, C.c_int32]),
+            "oracle_gather": (None, [_P, C.c_int32, _I32, _F, _D]),
             "oracle_shift": (C.c_int32, [_P, C.c_int32]),
             "oracle_fdtd_before_current": (None, [_P]),
             "oracle_add_current": (None, [_P]),
@@ -286,8 +287,18 @@ class Oracle:
         if r != 0:
             raise RuntimeError("oracle: supercell capacity exceeded")
 
+    def gather(self, shape, cell, pos):
+        """E and B of the current fields at one particle (global cell, in-cell
+        position), interpolated with the shape: (Ex, Ey, Ez, Bx, By, Bz)."""
+        c = np.ascontiguousarray(cell, np.int32)
+        q = np.ascontiguousarray(pos, np.float32)
+        out = np.zeros(6)
+        lib().oracle_gather(self.h, int(shape), _p(c, C.c_int32), _p(q, C.c_float), _p(out, C.c_double))
+        return out
+
     def field_energy(self):
-        out = np.zeros(2)
+        """EnergyFields output in joules: total, B x, y, z, E x, y, z."""
+        out = np.zeros(7)
         lib().oracle_field_energy(self.h, _p(out, C.c_double))
         return out
 

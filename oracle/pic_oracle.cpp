@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -2134,6 +2135,24 @@ void oracle_push(void* p, int32_t species, int32_t fused) {
     dispatch<void>(static_cast<Handle*>(p), impl, impl);
 }
 
+// the field gather of pushOne for one particle: E and B of the state at the
+// particle in global cell `cell` with in-cell position `pos`, interpolated
+// with the shape's on-support assignment (out: Ex, Ey, Ez, Bx, By, Bz)
+void oracle_gather(void* p, int32_t shape, const int32_t* cell, const float* pos, double* out) {
+    auto impl = [&](auto* st) {
+        using T = typename std::remove_reference<decltype(st->k.dt)>::type;
+        T const q[3] = {static_cast<T>(pos[0]), static_cast<T>(pos[1]), static_cast<T>(pos[2])};
+        T const fE[3][3] = {{T(0.5), T(0), T(0)}, {T(0), T(0.5), T(0)}, {T(0), T(0), T(0.5)}};
+        T const fB[3][3] = {{T(0), T(0.5), T(0.5)}, {T(0.5), T(0), T(0.5)}, {T(0.5), T(0.5), T(0)}};
+        int const supp = supportOf(shape);
+        for (int c = 0; c < 3; ++c) {
+            out[c] = static_cast<double>(gather1(*st, st->E[c], supp, cell[0], cell[1], cell[2], q, fE[c]));
+            out[3 + c] = static_cast<double>(gather1(*st, st->B[c], supp, cell[0], cell[1], cell[2], q, fB[c]));
+        }
+    };
+    dispatch<void>(static_cast<Handle*>(p), impl, impl);
+}
+
 void oracle_deposit(void* p, int32_t species) {
     auto impl = [&](auto* st) { deposit(*st, species); };
     dispatch<void>(static_cast<Handle*>(p), impl, impl);
@@ -2170,17 +2189,37 @@ int32_t oracle_step(void* p, int32_t n) {
     return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
 }
 
+// EnergyFields::getEnergyFields (plugins/EnergyFields.x.cpp:183-247): per
+// component the FP64 sum of the squares taken in T (squareComponentWise, then
+// cast64Bit, :61-69, :244-247), times 0.5/mue0*V for B and eps0*V*0.5 for E
+// (:210-220; mue0, eps0 and the cell volume in T), in joules (unit.energy(),
+// unitless/simulation.unitless:455-458). out: total, B x, y, z, E x, y, z.
 void oracle_field_energy(void* p, double* out) {
     auto impl = [&](auto* st) {
-        double e = 0.0, b = 0.0;
+        using T = typename std::remove_reference<decltype(st->k.dt)>::type;
+        double sb[3] = {0.0, 0.0, 0.0}, se[3] = {0.0, 0.0, 0.0};
         for (int c = 0; c < 3; ++c)
             for (size_t i = 0; i < st->ncells; ++i) {
-                double const ev = st->E[c][i], bv = st->B[c][i];
-                e += ev * ev;
-                b += bv * bv;
+                T const bv = st->B[c][i], ev = st->E[c][i];
+                T const b2 = bv * bv, e2 = ev * ev;
+                sb[c] += static_cast<double>(b2);
+                se[c] += static_cast<double>(e2);
             }
-        out[0] = e;
-        out[1] = b;
+        T const volume = st->k.cell[0] * st->k.cell[1] * st->k.cell[2];
+        double const bFactor = 0.5 / static_cast<double>(st->k.mue0) * static_cast<double>(volume);
+        T const eV = st->k.eps0 * volume;
+        double const eFactor = static_cast<double>(eV) * 0.5;
+        double const unitEnergy =
+            st->k.unit_mass * st->k.unit_length * st->k.unit_length / (st->k.unit_time * st->k.unit_time);
+        double eB = 0.0, eE = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            double const b = sb[d] * bFactor, e = se[d] * eFactor;
+            eB += b;
+            eE += e;
+            out[1 + d] = b * unitEnergy;
+            out[4 + d] = e * unitEnergy;
+        }
+        out[0] = (eE + eB) * unitEnergy;
     };
     dispatch<void>(static_cast<Handle*>(p), impl, impl);
 }

@@ -32,6 +32,7 @@ FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
 TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2
 
 ERR = {0: "ok", -1: "invalid", -2: "hip", -3: "capacity", -4: "state"}
+FLAG_CAPACITY_GREW = 1  # PIC_FLAG_CAPACITY_GREW: a supercell overflowed and the supercells grew
 
 
 class SpeciesDesc(C.Structure):
@@ -525,7 +526,8 @@ class Simulation:
 
     # -- diagnostics ---------------------------------------------------------
     def field_energy(self):
-        out = np.zeros(2, np.float64)
+        """EnergyFields output in joules: total, B x, y, z, E x, y, z."""
+        out = np.zeros(7, np.float64)
         check(self.lib.pic_field_energy(self.h, ptr(out, C.c_double)), "pic_field_energy")
         return out
 

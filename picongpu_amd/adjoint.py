@@ -37,6 +37,9 @@ class DifferentiablePIC:
         self.every = int(checkpoint_every)
         self.ncells = setup.ncells
         self.device = torch.device("cuda", 0)
+        # the context keeps the checkpoints of its last forward only: every
+        # forward gets a new generation, and a backward of an older one is refused
+        self.generation = 0
 
     def close(self):
         self.sim.close()
@@ -52,6 +55,7 @@ class DifferentiablePIC:
         E = torch.empty_like(e0)
         B = torch.empty_like(b0)
         torch.cuda.synchronize(self.device)
+        self.generation += 1
         self.sim.adjoint_forward(e0.data_ptr(), b0.data_ptr(), self.nsteps, self.every, E.data_ptr(), B.data_ptr())
         return E, B
 
@@ -80,11 +84,16 @@ class _PICAdjoint(torch.autograd.Function):
         ctx.pic = pic
         ctx.dtypes = (E0.dtype, B0.dtype)
         E, B = pic.forward_raw(E0, B0)
+        ctx.generation = pic.generation
         return E.to(torch.float64), B.to(torch.float64)
 
     @staticmethod
     def backward(ctx, gE, gB):
         pic = ctx.pic
+        if ctx.generation != pic.generation:
+            raise RuntimeError("the differentiable context ran another forward after this one; its checkpoints "
+                               "belong to that forward, so this backward would return wrong gradients: "
+                               "call fields() again before backward()")
         if gE is None:
             gE = torch.zeros((3, pic.ncells), dtype=torch.float64, device=pic.device)
         if gB is None:
@@ -173,7 +182,7 @@ def a0_field_unit(setup: setups.Setup, wavelength_si: float = 0.8e-6) -> float:
     """PIC-unit field amplitude of a0 = 1: m_e c omega / e in V/m divided by the
     unit of the electric field (UNIT_MASS UNIT_LENGTH / (UNIT_TIME^2 UNIT_CHARGE))."""
     k = abi.constants_from_config(setup.config())
-    m_e, e = 9.1093837015e-31, 1.602176634e-19
+    m_e, e = setups.ELECTRON_MASS_SI, 1.602176634e-19
     omega = 2 * math.pi * setups.SPEED_OF_LIGHT_SI / wavelength_si
     e_si = m_e * setups.SPEED_OF_LIGHT_SI * omega / e
     unit_e = k.unit_mass * k.unit_length / (k.unit_time ** 2 * k.unit_charge)

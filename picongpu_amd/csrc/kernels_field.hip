@@ -80,6 +80,7 @@ __device__ __forceinline__ float derivE(const float* f, const Idx& I, int d, con
 
 template <int SOLVER>
 __global__ __launch_bounds__(256) void kBHalf(FieldArgs a) {
+    if (halted(a.g)) return;
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -100,6 +101,7 @@ __device__ __forceinline__ float bwd(const float* f, const Idx& I, int d, float 
 }
 
 __global__ __launch_bounds__(256) void kEFull(FieldArgs a) {
+    if (halted(a.g)) return;
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -114,6 +116,7 @@ __global__ __launch_bounds__(256) void kEFull(FieldArgs a) {
 
 // kEFull then kAddCurrent (None) per cell in one pass, the new E into Enew
 __global__ __launch_bounds__(256) void kEFullJ(FieldArgs a, float* En0, float* En1, float* En2) {
+    if (halted(a.g)) return;
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -694,6 +697,7 @@ __device__ __forceinline__ void fieldAfterTile(const FieldArgs& a, float* const*
 
 template <int SOLVER, bool OPEN>
 __global__ __launch_bounds__(256) void kFieldBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2) {
+    if (halted(a.g)) return;
     __shared__ FieldLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
     fieldBeforeAddTile<SOLVER, OPEN>(a, En, blockIdx.x, L);
@@ -701,6 +705,7 @@ __global__ __launch_bounds__(256) void kFieldBeforeAdd(FieldArgs a, float* Enew0
 
 template <bool OPEN>
 __global__ __launch_bounds__(256) void kYeeBeforeAdd(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
+    if (halted(a.g)) return;
     __shared__ YeeBeforeLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
     yeeBeforeAddTile<OPEN>(a, En, blockIdx.x, zc, L);
@@ -708,6 +713,7 @@ __global__ __launch_bounds__(256) void kYeeBeforeAdd(FieldArgs a, float* Enew0, 
 
 template <bool OPEN>
 __global__ __launch_bounds__(256) void kYeeAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2, int zc) {
+    if (halted(a.g)) return;
     __shared__ YeeAfterLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
     yeeAfterTile<OPEN, true>(a, En, blockIdx.x, zc, L);
@@ -716,12 +722,14 @@ __global__ __launch_bounds__(256) void kYeeAfter(FieldArgs a, float* Enew0, floa
 // one B half step (the stage call kBHalf) in the z-marching form
 template <bool OPEN>
 __global__ __launch_bounds__(256) void kLeheBHalfZ(FieldArgs a, int zc) {
+    if (halted(a.g)) return;
     __shared__ LehePlanes L;
     leheBHalfTile<OPEN>(a, blockIdx.x, zc, L);
 }
 
 template <bool OPEN>
 __global__ __launch_bounds__(256) void kYeeBHalfZ(FieldArgs a, int zc) {
+    if (halted(a.g)) return;
     __shared__ YeeAfterLds L;
     yeeAfterTile<OPEN, false>(a, nullptr, blockIdx.x, zc, L);
 }
@@ -731,6 +739,7 @@ __global__ __launch_bounds__(256) void kYeeBeforeAddBatch(const FieldBatchEntry*
     __shared__ YeeBeforeLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
+    if (halted(e.a.g)) return;
     yeeBeforeAddTile<false>(e.a, e.Enew, it.y, kZC, L);
 }
 
@@ -739,11 +748,13 @@ __global__ __launch_bounds__(256) void kYeeAfterBatch(const FieldBatchEntry* __r
     __shared__ YeeAfterLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
+    if (halted(e.a.g)) return;
     yeeAfterTile<false, true>(e.a, e.Enew, it.y, kZC, L);
 }
 
 template <int SOLVER, bool OPEN>
 __global__ __launch_bounds__(256) void kFieldAfter(FieldArgs a, float* Enew0, float* Enew1, float* Enew2) {
+    if (halted(a.g)) return;
     __shared__ AfterLds L;
     float* const En[3] = {Enew0, Enew1, Enew2};
     fieldAfterTile<SOLVER, OPEN>(a, En, blockIdx.x, L);
@@ -756,6 +767,7 @@ __global__ __launch_bounds__(256) void kFieldBeforeAddBatch(const FieldBatchEntr
     __shared__ FieldLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
+    if (halted(e.a.g)) return;
     fieldBeforeAddTile<SOLVER, false>(e.a, e.Enew, it.y, L);
 }
 
@@ -765,6 +777,7 @@ __global__ __launch_bounds__(256) void kFieldAfterBatch(const FieldBatchEntry* _
     __shared__ AfterLds L;
     int2 const it = tiles[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
+    if (halted(e.a.g)) return;
     fieldAfterTile<SOLVER, false>(e.a, e.Enew, it.y, L);
 }
 
@@ -772,6 +785,7 @@ __global__ __launch_bounds__(256) void kCurrentResetBatch(const FieldBatchEntry*
                                                           const int2* __restrict__ blocks) {
     int2 const it = blocks[blockIdx.x];
     const FieldBatchEntry& e = table[it.x];
+    if (halted(e.a.g)) return;
     size_t const n = (size_t)e.a.g.nx * e.a.g.ny * e.a.g.nz;
     size_t const i = (size_t)it.y * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -781,6 +795,7 @@ __global__ __launch_bounds__(256) void kCurrentResetBatch(const FieldBatchEntry*
 }
 
 __global__ __launch_bounds__(256) void kAddCurrent(FieldArgs a) {
+    if (halted(a.g)) return;
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -804,6 +819,7 @@ __device__ __forceinline__ float binomialJ(const float* J, const Idx& I) {
 }
 
 __global__ __launch_bounds__(256) void kAddCurrentBinomial(FieldArgs a) {
+    if (halted(a.g)) return;
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -815,6 +831,7 @@ __global__ __launch_bounds__(256) void kAddCurrentBinomial(FieldArgs a) {
 
 // the filtered current into out (the fused field step then adds it as J)
 __global__ __launch_bounds__(256) void kBinomialJ(FieldArgs a, float* o0, float* o1, float* o2) {
+    if (halted(a.g)) return;
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -829,32 +846,66 @@ __global__ __launch_bounds__(256) void kFill(float* p, size_t n, float v) {
     if (i < n) p[i] = v;
 }
 
-// per-block partial sums of E^2 and B^2 in FP64
+// CurrentReset (simulation/stage/CurrentReset.hpp:39-53) at the start of a step
+__global__ __launch_bounds__(256) void kCurrentReset(GridDev g, float* J0, float* J1, float* J2, size_t n) {
+    if (halted(g)) return;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    J0[i] = 0.0f;
+    J1[i] = 0.0f;
+    J2[i] = 0.0f;
+}
+
+// end of a batched step, per member: a halted member counts the step as
+// skipped; otherwise an overflow in this step halts the following ones
+__global__ __launch_bounds__(64) void kEndStepBatch(const FieldBatchEntry* __restrict__ table, int n) {
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    StepFlags* f = table[i].a.g.flags;
+    if (!f) return;
+    if (f->halt)
+        f->skipped += 1u;
+    else if (f->overflow)
+        f->halt = 1u;
+}
+
+// EnergyFields::reduceField (plugins/EnergyFields.x.cpp:240-247): per
+// component the square in FP32 (squareComponentWise), cast to FP64
+// (cast64Bit), summed in FP64. Per block the partial sums of the six
+// components (B x, y, z, then E x, y, z) at partials[6 * block + k].
 __global__ __launch_bounds__(256) void kEnergy(FieldArgs a, double* partials) {
-    __shared__ double sE[256], sB[256];
+    __shared__ double sAcc[6][256];
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
-    double e = 0.0, b = 0.0;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            double const ev = a.E[c][i], bv = a.B[c][i];
-            e += ev * ev;
-            b += bv * bv;
+            float const bv = a.B[c][i], ev = a.E[c][i];
+            float const b2 = bv * bv, e2 = ev * ev;
+            acc[c] += (double)b2;
+            acc[3 + c] += (double)e2;
         }
     }
-    sE[threadIdx.x] = e;
-    sB[threadIdx.x] = b;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sAcc[k][threadIdx.x] = acc[k];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            sE[threadIdx.x] += sE[threadIdx.x + s];
-            sB[threadIdx.x] += sB[threadIdx.x + s];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sAcc[k][threadIdx.x] += sAcc[k][threadIdx.x + s];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = sE[0];
-        partials[2 * blockIdx.x + 1] = sB[0];
-    }
+    if (threadIdx.x < 6) partials[6 * blockIdx.x + threadIdx.x] = sAcc[threadIdx.x][0];
+}
+
+// the six sums over the blocks' partials, block by block in ascending order
+__global__ __launch_bounds__(64) void kEnergyFinal(const double* __restrict__ partials, int nblocks, double* out) {
+    int const k = threadIdx.x;
+    if (k >= 6) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblocks; ++b) acc += partials[6 * b + k];
+    out[k] = acc;
 }
 
 __global__ void kCountSum(const uint32_t* c, int n, unsigned long long* out) {
@@ -917,6 +968,7 @@ hipError_t launchEFull(const FieldArgs& a, hipStream_t s) {
 // walk the slab x fastest, so rows along x stay coalesced.
 __global__ __launch_bounds__(256) void kAbsorb(float* f0, float* f1, float* f2, GridDev g, int axis, int minSide,
                                                int depth, const float* __restrict__ table) {
+    if (halted(g)) return;
     int const sx = axis == 0 ? depth : g.nx, sy = axis == 1 ? depth : g.ny, sz = axis == 2 ? depth : g.nz;
     size_t const t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)sx * sy * sz) return;
@@ -1043,14 +1095,28 @@ hipError_t launchBinomialJ(const FieldArgs& a, float* const out[3], hipStream_t 
     return hipGetLastError();
 }
 
+hipError_t launchCurrentReset(const GridDev& g, float* const J[3], size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kCurrentReset, dim3(blocksFor(n)), dim3(256), 0, s, g, J[0], J[1], J[2], n);
+    return hipGetLastError();
+}
+
+hipError_t launchEndStepBatch(const FieldBatchEntry* table, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kEndStepBatch, dim3((n + 63) / 64), dim3(64), 0, s, table, n);
+    return hipGetLastError();
+}
+
 hipError_t launchFill(float* p, size_t n, float v, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(kFill, dim3(blocksFor(n)), dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
 
-hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, hipStream_t s) {
+hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, double* out, hipStream_t s) {
+    if (nblocks <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kEnergy, dim3(nblocks), dim3(256), 0, s, a, partials);
+    hipLaunchKernelGGL(kEnergyFinal, dim3(1), dim3(64), 0, s, partials, nblocks, out);
     return hipGetLastError();
 }
 

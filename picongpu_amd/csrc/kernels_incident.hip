@@ -185,6 +185,7 @@ __device__ float incidentComponent(const IncidentUpdate& u, float tOrigin, const
 
 __global__ __launch_bounds__(256) void kIncident(IncidentUpdate u, GridDev g, float* F0, float* F1, float* F2,
                                                  const int64_t* step, float stepOffset) {
+    if (halted(g)) return;
     int const ex = u.end[0] - u.begin[0], ey = u.end[1] - u.begin[1], ez = u.end[2] - u.begin[2];
     long long const n = static_cast<long long>(ex) * ey * ez;
     long long const t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -256,8 +257,16 @@ __global__ __launch_bounds__(256) void kIncident(IncidentUpdate u, GridDev g, fl
     F2[at] += res[2];
 }
 
-__global__ void kAdvanceStep(int64_t* step) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1;
+// end of a step: the field step counter advances; a halted simulation counts
+// the step as skipped instead, and an overflow in this step halts the next ones
+__global__ void kAdvanceStep(int64_t* step, StepFlags* flags) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (flags && flags->halt) {
+        flags->skipped += 1u;
+        return;
+    }
+    step[0] += 1;
+    if (flags && flags->overflow) flags->halt = 1u;
 }
 
 }  // namespace
@@ -272,8 +281,8 @@ hipError_t launchIncident(const IncidentUpdate& u, const GridDev& g, float* cons
     return hipGetLastError();
 }
 
-hipError_t launchAdvanceStep(int64_t* step, hipStream_t s) {
-    hipLaunchKernelGGL(kAdvanceStep, dim3(1), dim3(64), 0, s, step);
+hipError_t launchAdvanceStep(int64_t* step, StepFlags* flags, hipStream_t s) {
+    hipLaunchKernelGGL(kAdvanceStep, dim3(1), dim3(64), 0, s, step, flags);
     return hipGetLastError();
 }
 

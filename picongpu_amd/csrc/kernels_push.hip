@@ -130,6 +130,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     uint32_t* const sWaveCnt = sPre + kPushMaxIters * kPushWaves + 1;
     GroupScratch& sGroup = *reinterpret_cast<GroupScratch*>(sWaveCnt + kPushWaves * kNumDirs);
 
+    if (halted(a.g)) return;  // uniform: a later step of an overflowed simulation
     int const sx = sc % a.g.nscx;
     int const sy = (sc / a.g.nscx) % a.g.nscy;
     int const sz = sc / (a.g.nscx * a.g.nscy);
@@ -279,7 +280,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                 }
             }
         }
-        if (MODE != 2 && S.lists) S.leaveBits[(size_t)sc * kPushThreads + tid] = (uint16_t)leaveBits;
+        if (MODE != 2 && S.lists) S.leaveBits[(size_t)sc * kPushThreads + tid] = leaveBits;
     }
     __syncthreads();
 

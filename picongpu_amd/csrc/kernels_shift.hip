@@ -17,6 +17,8 @@
 // In a full step the push epilogue stages (its supercell's lines are still in
 // L2), so the shift itself is the insert: coalesced reads of 32-byte records
 // and coalesced appends.
+#include <algorithm>
+
 #include "pic_shift_common.hpp"
 
 namespace pic {
@@ -26,10 +28,11 @@ constexpr int kWaves = kThreads / 64;
 
 // Leaver lists from the multiMask of every slot (used when the push did not
 // build them, e.g. after frames were written from the host).
-__global__ __launch_bounds__(kThreads) void kCollect(SpeciesDev S) {
+__global__ __launch_bounds__(kThreads) void kCollect(SpeciesDev S, GridDev grid) {
     __shared__ uint32_t sWave[kWaves];
     __shared__ uint32_t sWaveCnt[kWaves * kNumDirs];
     __shared__ GroupScratch g;
+    if (halted(grid)) return;
     int const sc = blockIdx.x;
     int const tid = threadIdx.x;
     int const n = (int)S.count[sc];
@@ -48,8 +51,9 @@ __global__ __launch_bounds__(kThreads) void kCollect(SpeciesDev S) {
 }
 
 // Staging of every supercell from the leaver lists (when the push did not stage).
-__global__ __launch_bounds__(kThreads) void kStage(SpeciesDev S) {
+__global__ __launch_bounds__(kThreads) void kStage(SpeciesDev S, GridDev grid) {
     __shared__ uint32_t sWave[kWaves];
+    if (halted(grid)) return;
     int const sc = blockIdx.x;
     stageSC<kWaves>(S, sc, (int)S.outOff[(size_t)sc * kOffStride + kNumDirs], sWave);
 }
@@ -68,8 +72,12 @@ __device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t v) {
     return v;
 }
 
-// insert for destination supercell d; called by one whole wave
-__device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, int d, uint32_t* errFlags) {
+// insert for destination supercell d; called by one whole wave. A record
+// whose slot lies beyond the capacity is left in the sender's outbox, the
+// count still grows by every incoming record and the overflow flag is set;
+// with redo the supercell's count is taken as its count before that insert
+// (count - incoming) and the insert runs again into grown supercells.
+__device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, int d, bool redo) {
     int const lane = threadIdx.x & 63;
     int const dx = d % g.nscx;
     int const dy = (d / g.nscx) % g.nscy;
@@ -96,7 +104,7 @@ __device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, 
     uint32_t const incl = waveInclusiveScan(cnt);
     uint32_t const start = incl - cnt;  // exclusive prefix: first incoming index of this code
     uint32_t const total = __shfl(incl, 63);
-    uint32_t const n = S.count[d];
+    uint32_t const n = redo ? S.count[d] - total : S.count[d];
     size_t const dbase = (size_t)d * (size_t)S.cap;
     // Wave-uniform loop: every lane runs the shuffles of every round (a
     // shuffle reads nothing valid from lanes that have left a divergent
@@ -123,7 +131,7 @@ __device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, 
         if (!valid) continue;
         uint32_t const slot = n + k;
         if (slot >= (uint32_t)S.cap) {
-            atomicOr(errFlags, 1u);
+            if (g.flags) atomicOr(&g.flags->overflow, 1u);
             continue;
         }
         size_t const rec = (size_t)sc * (size_t)S.cap + bg + (k - st);
@@ -140,16 +148,19 @@ __device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, 
         S.cell[dst] = (uint16_t)__float_as_int(r1.w);
         S.mask[dst] = 1;
     }
-    if (lane == 0) {
-        uint32_t const nt = n + total;
-        S.count[d] = nt < (uint32_t)S.cap ? nt : (uint32_t)S.cap;
-    }
+    if (lane == 0) S.count[d] = n + total;
 }
 
-__global__ __launch_bounds__(kThreads) void kInsert(SpeciesDev S, GridDev g, int nsc, uint32_t* errFlags) {
+__global__ __launch_bounds__(kThreads) void kInsert(SpeciesDev S, GridDev g, int nsc, int redo) {
+    if (!redo && halted(g)) return;
     int const d = blockIdx.x * kScPerBlock + (threadIdx.x >> 6);
     if (d >= nsc) return;  // whole wave
-    insertSC(S, g, d, errFlags);
+    insertSC(S, g, d, redo != 0);
+}
+
+__global__ __launch_bounds__(256) void kClampCounts(SpeciesDev S, int nsc) {
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nsc && S.count[i] > (uint32_t)S.cap) S.count[i] = (uint32_t)S.cap;
 }
 
 // Batched forms for ensembles: work items (table entry, supercell), one wave
@@ -160,7 +171,8 @@ __global__ __launch_bounds__(kThreads) void kInsertBatch(const ShiftEntry* __res
     if (w >= nwork) return;  // whole wave
     int2 const it = work[w];
     const ShiftEntry& e = table[it.x];
-    insertSC(e.S, e.g, it.y, e.errFlags);
+    if (halted(e.g)) return;
+    insertSC(e.S, e.g, it.y, false);
 }
 
 hipError_t launchShiftBatch(const ShiftEntry* table, const int2* work, int nwork, hipStream_t s) {
@@ -172,19 +184,54 @@ hipError_t launchShiftBatch(const ShiftEntry* table, const int2* work, int nwork
 
 hipError_t launchCollect(const GridDev& g, const SpeciesDev& sp, hipStream_t s) {
     int const nsc = g.nscx * g.nscy * g.nscz;
-    hipLaunchKernelGGL(kCollect, dim3(nsc), dim3(kThreads), 0, s, sp);
+    hipLaunchKernelGGL(kCollect, dim3(nsc), dim3(kThreads), 0, s, sp, g);
     return hipGetLastError();
 }
 
-hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, bool staged, hipStream_t s) {
+hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, bool staged, hipStream_t s) {
     int const nsc = g.nscx * g.nscy * g.nscz;
     if (!staged) {
-        hipLaunchKernelGGL(kStage, dim3(nsc), dim3(kThreads), 0, s, sp);
+        hipLaunchKernelGGL(kStage, dim3(nsc), dim3(kThreads), 0, s, sp, g);
         hipError_t const e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     int const blocks = (nsc + kScPerBlock - 1) / kScPerBlock;
-    hipLaunchKernelGGL(kInsert, dim3(blocks), dim3(kThreads), 0, s, sp, g, nsc, errFlags);
+    hipLaunchKernelGGL(kInsert, dim3(blocks), dim3(kThreads), 0, s, sp, g, nsc, 0);
+    return hipGetLastError();
+}
+
+hipError_t launchInsertRedo(const GridDev& g, const SpeciesDev& sp, hipStream_t s) {
+    int const nsc = g.nscx * g.nscy * g.nscz;
+    int const blocks = (nsc + kScPerBlock - 1) / kScPerBlock;
+    hipLaunchKernelGGL(kInsert, dim3(blocks), dim3(kThreads), 0, s, sp, g, nsc, 1);
+    return hipGetLastError();
+}
+
+// Rows of oldWords 16-byte words from src (row pitch oldWords) to dst (row
+// pitch newWords >= oldWords): the slots of every supercell keep their index
+// when the supercells grow.
+__global__ __launch_bounds__(256) void kRowCopy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t oldWords,
+                                                size_t newWords, size_t rows) {
+    size_t const total = oldWords * rows;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        size_t const r = t / oldWords, c = t - r * oldWords;
+        dst[r * newWords + c] = src[t];
+    }
+}
+
+hipError_t launchRowCopy(void* dst, const void* src, size_t oldRowBytes, size_t newRowBytes, size_t rows,
+                         hipStream_t s) {
+    if (oldRowBytes % 16 != 0 || newRowBytes % 16 != 0 || newRowBytes < oldRowBytes) return hipErrorInvalidValue;
+    size_t const total = oldRowBytes / 16 * rows;
+    if (total == 0) return hipSuccess;
+    size_t const blocks = std::min<size_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(kRowCopy, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), oldRowBytes / 16, newRowBytes / 16, rows);
+    return hipGetLastError();
+}
+
+hipError_t launchClampCounts(const SpeciesDev& sp, int nsc, hipStream_t s) {
+    hipLaunchKernelGGL(kClampCounts, dim3((nsc + 255) / 256), dim3(256), 0, s, sp, nsc);
     return hipGetLastError();
 }
 

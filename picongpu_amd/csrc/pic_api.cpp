@@ -639,7 +639,9 @@ struct pic_ctx {
     float* B[3] = {nullptr, nullptr, nullptr};
     float* J[3] = {nullptr, nullptr, nullptr};
     std::vector<SpeciesMem> sp;
-    uint32_t* errFlags = nullptr;
+    pic::StepFlags* flags = nullptr;  // device step state (capacity overflow, halt)
+    uint32_t hostFlags = 0;           // PIC_FLAG_* raised since the last pic_error_flags
+    bool broken = false;              // particles were lost: the context refuses further steps
     double* energyPartials = nullptr;
     int energyBlocks = 0;
     unsigned long long* countOut = nullptr;
@@ -816,7 +818,6 @@ pic::PushArgs makePushArgs(pic_ctx* c) {
     a.inv_vdt = c->k.inv_cell_volume_dt;
     a.jscale = std::ldexp(1.0, c->jexp);
     a.jinv = std::ldexp(1.0, -c->jexp);
-    a.errFlags = c->errFlags;
     return a;
 }
 
@@ -909,12 +910,169 @@ int runPush(pic_ctx* c, int32_t species, int mode, bool stageShift = false) {
     return PIC_OK;
 }
 
-int checkErrors(pic_ctx* c) {
-    uint32_t flags = 0;
-    PIC_HIP(hipMemcpyAsync(&flags, c->errFlags, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
+// EnergyFields::getEnergyFields (plugins/EnergyFields.x.cpp:183-237): the
+// per-component sums of squares times 0.5/mue0*V (B) and eps0*V*0.5 (E) in
+// the reference's float/double mix (mue0, eps0 and the cell volume are
+// float_X), added per field and converted to joules with unit.energy()
+// (unitless/simulation.unitless:455-458). sums: B x, y, z, E x, y, z.
+// out: total, B x, y, z, E x, y, z.
+void fieldEnergyFromSums(const pic_ctx* c, const double sums[6], double out[7]) {
+    float const volume = c->k.cell[0] * c->k.cell[1] * c->k.cell[2];  // productOfComponents
+    double const bFactor = 0.5 / static_cast<double>(c->k.mue0) * static_cast<double>(volume);
+    float const eV = c->k.eps0 * volume;
+    double const eFactor = static_cast<double>(eV) * 0.5;
+    double const unitEnergy = c->k.unit_mass * c->k.unit_length * c->k.unit_length / (c->k.unit_time * c->k.unit_time);
+    double eB = 0.0, eE = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        double const b = sums[d] * bFactor;
+        double const e = sums[3 + d] * eFactor;
+        eB += b;
+        eE += e;
+        out[1 + d] = b * unitEnergy;
+        out[4 + d] = e * unitEnergy;
+    }
+    out[0] = (eE + eB) * unitEnergy;
+}
+
+// the field step runs as the two fused kernels (no incident field between its stages)
+bool fusedFieldStep(const pic_ctx* c) { return c->incB.empty() && c->incE.empty(); }
+
+void dropGraphs(pic_ctx* c) {
+    for (int p = 0; p < 2; ++p) {
+        if (c->graphExec[p]) (void)hipGraphExecDestroy(c->graphExec[p]);
+        if (c->graph[p]) (void)hipGraphDestroy(c->graph[p]);
+        c->graphExec[p] = nullptr;
+        c->graph[p] = nullptr;
+    }
+}
+
+void devFree(pic_ctx* c, void* p) {
+    if (!p) return;
+    auto it = std::find(c->allocations.begin(), c->allocations.end(), p);
+    if (it != c->allocations.end()) c->allocations.erase(it);
+    (void)hipFree(p);
+}
+
+// Grows every supercell of species s to `frames` frames. Each attribute stays
+// one block of nsc*slots entries, so a supercell's slots stay contiguous and
+// keep their index; the new slots are empty (multiMask 0). The outbox moves
+// along, so records an overflowing insert left there can still be placed.
+// (The reference takes single frames from its mallocMC heap instead,
+// ParticlesBox.hpp:96-122; a supercell here grows by whole frames at once.)
+int growSpecies(pic_ctx* c, int s, int frames) {
+    SpeciesMem& m = c->sp[s];
+    int const oldCap = m.cap, newCap = frames * pic::kSCV;
+    if (newCap <= oldCap) return PIC_OK;
+    size_t const slots = static_cast<size_t>(c->nsc) * newCap;
+    pic::SpeciesDev const& o = m.dev;
+    pic::SpeciesDev n = o;
+    std::vector<void*> fresh;
+    auto alloc = [&](auto** p, size_t count) {
+        if (devAlloc(c, p, count) != PIC_OK) return false;
+        fresh.push_back(*p);
+        return true;
+    };
+    bool ok = true;
+    for (int a = 0; a < 3 && ok; ++a) ok = alloc(&n.pos[a], slots) && alloc(&n.mom[a], slots);
+    ok = ok && alloc(&n.w, slots) && alloc(&n.cell, slots) && alloc(&n.mask, slots) && alloc(&n.outbox, 2 * slots) &&
+         alloc(&n.outIdx, slots) && alloc(&n.scratch, slots);
+    if (!ok) {
+        for (void* p : fresh) devFree(c, p);
+        return fail(PIC_E_CAPACITY, "supercells of species " + std::to_string(s) + " cannot grow to " +
+                                        std::to_string(frames) + " frames: " + g_err);
+    }
+    size_t const rows = static_cast<size_t>(c->nsc);
+    auto move = [&](void* dst, const void* src, size_t bytesPerSlot) {
+        return pic::launchRowCopy(dst, src, oldCap * bytesPerSlot, newCap * bytesPerSlot, rows, c->stream);
+    };
+    for (int a = 0; a < 3; ++a) {
+        PIC_HIP(move(n.pos[a], o.pos[a], 4));
+        PIC_HIP(move(n.mom[a], o.mom[a], 4));
+    }
+    PIC_HIP(move(n.w, o.w, 4));
+    PIC_HIP(move(n.cell, o.cell, 2));
+    PIC_HIP(move(n.mask, o.mask, 1));
+    PIC_HIP(move(n.outbox, o.outbox, 32));
     PIC_HIP(hipStreamSynchronize(c->stream));
-    if (flags & 1u) return fail(PIC_E_CAPACITY, "a supercell exceeded its frame capacity during the shift");
+    for (int a = 0; a < 3; ++a) {
+        devFree(c, o.pos[a]);
+        devFree(c, o.mom[a]);
+    }
+    devFree(c, o.w);
+    devFree(c, o.cell);
+    devFree(c, o.mask);
+    devFree(c, o.outbox);
+    devFree(c, o.outIdx);
+    devFree(c, o.scratch);
+    n.cap = newCap;
+    n.lists = newCap <= pic::kListMaxSlots ? 1 : 0;
+    m.dev = n;
+    m.cap = newCap;
+    m.desc.frames_per_sc = frames;
+    c->cfg.species[s].frames_per_sc = frames;
+    dropGraphs(c);  // captured steps hold the old buffers
     return PIC_OK;
+}
+
+// frames for a supercell that must hold `count` particles: geometric growth
+// (half again) with at least two spare frames, at most maxFrames (0: no limit)
+int grownFrames(int frames, uint32_t count, int maxFrames) {
+    int const need = static_cast<int>((count + pic::kSCV - 1) / pic::kSCV);
+    int f = std::max(need + 2, frames + (frames + 1) / 2);
+    if (maxFrames > 0) f = std::min(f, maxFrames);
+    return f;
+}
+
+// After steps: when a shift found a supercell full, the insert kept the true
+// count and left the records it could not place in the senders' outboxes, and
+// the steps after it were skipped. Grow the supercells of every species that
+// needs it, insert again (the records land in the slots a large enough
+// capacity would have given them, so the result is the one of that capacity
+// bit for bit), clear the flags and return the number of skipped steps in
+// *skipped. Each skipped fused field step swapped E and Ealt on the host
+// without writing, so an odd count is swapped back. maxFrames bounds the
+// growth (0: none); beyond it the counts are clamped, particles are lost and
+// the context refuses further steps.
+int recoverCapacity(pic_ctx* c, int* skipped, int maxFrames) {
+    *skipped = 0;
+    pic::StepFlags f{};
+    PIC_HIP(hipMemcpyAsync(&f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    if (f.overflow == 0) return PIC_OK;
+    int r = PIC_OK;
+    std::vector<uint32_t> counts(c->nsc);
+    for (int s = 0; s < static_cast<int>(c->sp.size()); ++s) {
+        SpeciesMem& m = c->sp[s];
+        if (m.desc.pusher == PIC_PUSHER_NONE) continue;
+        PIC_HIP(hipMemcpyAsync(counts.data(), m.dev.count, c->nsc * 4, hipMemcpyDeviceToHost, c->stream));
+        PIC_HIP(hipStreamSynchronize(c->stream));
+        uint32_t const mx = *std::max_element(counts.begin(), counts.end());
+        if (mx <= static_cast<uint32_t>(m.cap)) continue;
+        int const frames = grownFrames(m.desc.frames_per_sc, mx, maxFrames);
+        if (static_cast<uint32_t>(frames) * pic::kSCV < mx)
+            r = fail(PIC_E_CAPACITY, "a supercell of species " + std::to_string(s) + " needs " + std::to_string(mx) +
+                                         " slots, more than the " + std::to_string(maxFrames) + " frames allowed here");
+        else
+            r = growSpecies(c, s, frames);
+        if (r != PIC_OK) {
+            // particles beyond the capacity are lost: keep the kernels in bounds
+            for (SpeciesMem& q : c->sp)
+                if (q.desc.pusher != PIC_PUSHER_NONE) PIC_HIP(pic::launchClampCounts(q.dev, c->nsc, c->stream));
+            c->broken = true;
+            break;
+        }
+        PIC_HIP(pic::launchInsertRedo(c->g, m.dev, c->stream));
+    }
+    PIC_HIP(hipMemsetAsync(c->flags, 0, sizeof(pic::StepFlags), c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    c->hostFlags |= PIC_FLAG_CAPACITY_GREW;
+    if ((f.skipped & 1u) && fusedFieldStep(c)) {
+        for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
+        c->eParity ^= 1;
+    }
+    dropGraphs(c);
+    *skipped = static_cast<int>(f.skipped);
+    return r;
 }
 
 }  // namespace
@@ -970,7 +1128,8 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
             devAlloc(c, &c->Ealt[d], c->ncells))
             return bail(PIC_E_HIP);
     }
-    if (devAlloc(c, &c->errFlags, 1)) return bail(PIC_E_HIP);
+    if (devAlloc(c, &c->flags, 1)) return bail(PIC_E_HIP);
+    c->g.flags = c->flags;
     if (cfg->current_interpolation == PIC_CURRENT_BINOMIAL)
         for (int d = 0; d < 3; ++d)
             if (devAlloc(c, &c->Jf[d], c->ncells)) return bail(PIC_E_HIP);
@@ -1004,7 +1163,7 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
     c->incB = incidentUpdates(*cfg, c->k, true);
     c->incE = incidentUpdates(*cfg, c->k, false);
     c->energyBlocks = 1024;
-    if (devAlloc(c, &c->energyPartials, 2 * c->energyBlocks)) return bail(PIC_E_HIP);
+    if (devAlloc(c, &c->energyPartials, 6 * c->energyBlocks + 6)) return bail(PIC_E_HIP);
     if (devAlloc(c, &c->countOut, 1)) return bail(PIC_E_HIP);
     std::vector<int32_t> all(c->nsc);
     for (int i = 0; i < c->nsc; ++i) all[i] = i;
@@ -1102,17 +1261,7 @@ int pic_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* cell, co
     if (r != PIC_OK) return r;
     if (n < 0 || (n > 0 && (!cell || !pos || !mom || !w))) return fail(PIC_E_INVALID, "null particle arrays");
     SpeciesMem& m = c->sp[s];
-    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
     std::vector<uint32_t> counts(c->nsc, 0);
-    std::vector<float> hp[3], hm[3];
-    for (int a = 0; a < 3; ++a) {
-        hp[a].assign(slots, 0.0f);
-        hm[a].assign(slots, 0.0f);
-    }
-    std::vector<float> hw(slots, 0.0f);
-    std::vector<uint16_t> hc(slots, 0);
-    std::vector<uint8_t> hk(slots, 0);
-    double wmax = 0.0;
     for (int64_t i = 0; i < n; ++i) {
         int const x = cell[3 * i], y = cell[3 * i + 1], z = cell[3 * i + 2];
         if (x < 0 || y < 0 || z < 0 || x >= c->g.nx || y >= c->g.ny || z >= c->g.nz)
@@ -1124,9 +1273,29 @@ int pic_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* cell, co
         }
         if (!std::isfinite(w[i])) return fail(PIC_E_INVALID, "non-finite weighting");
         int const sc = x / pic::kSCX + c->g.nscx * (y / pic::kSCY + c->g.nscy * (z / pic::kSCZ));
+        counts[sc] += 1;
+    }
+    // supercells that receive more particles than they hold grow first
+    uint32_t const mx = n > 0 ? *std::max_element(counts.begin(), counts.end()) : 0u;
+    if (mx > static_cast<uint32_t>(m.cap)) {
+        r = growSpecies(c, s, grownFrames(m.desc.frames_per_sc, mx, 0));
+        if (r != PIC_OK) return r;
+    }
+    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    std::vector<float> hp[3], hm[3];
+    for (int a = 0; a < 3; ++a) {
+        hp[a].assign(slots, 0.0f);
+        hm[a].assign(slots, 0.0f);
+    }
+    std::vector<float> hw(slots, 0.0f);
+    std::vector<uint16_t> hc(slots, 0);
+    std::vector<uint8_t> hk(slots, 0);
+    double wmax = 0.0;
+    std::fill(counts.begin(), counts.end(), 0u);
+    for (int64_t i = 0; i < n; ++i) {
+        int const x = cell[3 * i], y = cell[3 * i + 1], z = cell[3 * i + 2];
+        int const sc = x / pic::kSCX + c->g.nscx * (y / pic::kSCY + c->g.nscy * (z / pic::kSCZ));
         int const lc = x % pic::kSCX + pic::kSCX * (y % pic::kSCY + pic::kSCY * (z % pic::kSCZ));
-        if (counts[sc] >= static_cast<uint32_t>(m.cap))
-            return fail(PIC_E_CAPACITY, "supercell frame capacity exceeded while loading particles");
         size_t const g = static_cast<size_t>(sc) * m.cap + counts[sc]++;
         for (int a = 0; a < 3; ++a) {
             hp[a][g] = pos[3 * i + a];
@@ -1277,7 +1446,10 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
     if (r != PIC_OK) return r;
     if (!desc || desc->ppc <= 0) return fail(PIC_E_INVALID, "invalid init description");
     SpeciesMem& m = c->sp[s];
-    if (desc->ppc * pic::kSCV > m.cap) return fail(PIC_E_CAPACITY, "ppc*256 exceeds the supercell frame capacity");
+    if (desc->ppc * pic::kSCV > m.cap) {
+        r = growSpecies(c, s, grownFrames(m.desc.frames_per_sc, static_cast<uint32_t>(desc->ppc) * pic::kSCV, 0));
+        if (r != PIC_OK) return r;
+    }
     double const vol = c->cfg.cell_si[0] * c->cfg.cell_si[1] * c->cfg.cell_si[2];
     pic::InitArgs a{};
     a.g = c->g;
@@ -1377,10 +1549,31 @@ int pic_add_laser(pic_ctx* c, const pic_laser_desc* l) {
     return PIC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+int runShift(pic_ctx* c, int32_t species) {
+    for (int s = 0; s < static_cast<int>(c->sp.size()); ++s) {
+        if (species >= 0 && s != species) continue;
+        if (c->sp[s].desc.pusher == PIC_PUSHER_NONE) continue;
+        TimerScope ts(c, PIC_TIMER_SHIFT);
+        int const ready = c->sp[s].shiftReady;
+        if (ready == 0) PIC_HIP(pic::launchCollect(c->g, c->sp[s].dev, c->stream));
+        PIC_HIP(pic::launchShift(c->g, c->sp[s].dev, ready == 2, c->stream));
+        c->sp[s].shiftReady = 0;
+    }
+    return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pic_current_reset(pic_ctx* c) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     TimerScope ts(c, PIC_TIMER_FIELD);
-    for (int a = 0; a < 3; ++a) PIC_HIP(pic::launchFill(c->J[a], c->ncells, 0.0f, c->stream));
+    PIC_HIP(pic::launchCurrentReset(c->g, c->J, c->ncells, c->stream));
     return PIC_OK;
 }
 
@@ -1401,16 +1594,12 @@ int pic_deposit(pic_ctx* c, int32_t species) {
 int pic_shift(pic_ctx* c, int32_t species) {
     if (!c) return fail(PIC_E_INVALID, "null context");
     if (species >= static_cast<int>(c->sp.size())) return fail(PIC_E_INVALID, "species index out of range");
-    for (int s = 0; s < static_cast<int>(c->sp.size()); ++s) {
-        if (species >= 0 && s != species) continue;
-        if (c->sp[s].desc.pusher == PIC_PUSHER_NONE) continue;
-        TimerScope ts(c, PIC_TIMER_SHIFT);
-        int const ready = c->sp[s].shiftReady;
-        if (ready == 0) PIC_HIP(pic::launchCollect(c->g, c->sp[s].dev, c->stream));
-        PIC_HIP(pic::launchShift(c->g, c->sp[s].dev, c->errFlags, ready == 2, c->stream));
-        c->sp[s].shiftReady = 0;
-    }
-    return PIC_OK;
+    if (c->broken) return fail(PIC_E_STATE, "particles were lost to a failed capacity growth");
+    int r = runShift(c, species);
+    if (r != PIC_OK) return r;
+    // a supercell that overflowed grows now, so the next stage sees every particle
+    int skipped = 0;
+    return recoverCapacity(c, &skipped, 0);
 }
 
 int pic_fdtd_before_current(pic_ctx* c) {
@@ -1451,7 +1640,7 @@ int pic_fdtd_after_current(pic_ctx* c) {
     if (r != PIC_OK) return r;
     PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
     if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->B, c->absorb, c->stream));
-    PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->stream));
+    PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->flags, c->stream));
     return PIC_OK;
 }
 
@@ -1476,7 +1665,6 @@ namespace {
 
 // update_beforeCurrent + addCurrent + update_afterCurrent as the two fused
 // kernels; the new E lands in Ealt and the buffers swap
-bool fusedFieldStep(const pic_ctx* c) { return c->incB.empty() && c->incE.empty(); }
 
 int fieldStep(pic_ctx* c) {
     if (!fusedFieldStep(c)) {
@@ -1497,7 +1685,7 @@ int fieldStep(pic_ctx* c) {
         if ((r = applyIncident(c, c->incB, c->B, c->k.dt / c->k.dt)) != PIC_OK) return r;
         PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
         if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->B, c->absorb, c->stream));
-        PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->stream));
+        PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->flags, c->stream));
         return PIC_OK;
     }
     pic::FieldArgs a = makeFieldArgs(c);
@@ -1508,7 +1696,7 @@ int fieldStep(pic_ctx* c) {
             for (int d = 0; d < 3; ++d) a.J[d] = c->Jf[d];
         }
         PIC_HIP(pic::launchFieldStep(c->cfg.solver, a, c->Ealt, &c->absorb, c->stream));
-        PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->stream));
+        PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->flags, c->stream));
     }
     for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
     c->eParity ^= 1;
@@ -1521,33 +1709,11 @@ int oneStep(pic_ctx* c) {
     int r;
     if ((r = pic_current_reset(c)) != PIC_OK) return r;
     if ((r = runPush(c, -1, 1, true)) != PIC_OK) return r;
-    if ((r = pic_shift(c, -1)) != PIC_OK) return r;
+    if ((r = runShift(c, -1)) != PIC_OK) return r;
     return fieldStep(c);
 }
 
-}  // namespace
-
-extern "C" int pic_fdtd_step(pic_ctx* c) {
-    if (!c) return fail(PIC_E_INVALID, "null context");
-    return fieldStep(c);
-}
-
-extern "C" {
-
-int pic_step(pic_ctx* c, int32_t nsteps) {
-    if (!c) return fail(PIC_E_INVALID, "null context");
-    if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
-    for (int32_t i = 0; i < nsteps; ++i) {
-        int const r = oneStep(c);
-        if (r != PIC_OK) return r;
-    }
-    return checkErrors(c);
-}
-
-int pic_step_graph(pic_ctx* c, int32_t nsteps) {
-    if (!c) return fail(PIC_E_INVALID, "null context");
-    if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
-    if (c->timing) return fail(PIC_E_STATE, "pic_step_graph cannot run with kernel timing enabled");
+int stepGraphs(pic_ctx* c, int32_t nsteps) {
     for (int p = 0; p < 2; ++p) {
         if (c->graphExec[p] && c->graphJexp[p] != c->jexp) {
             (void)hipGraphExecDestroy(c->graphExec[p]);
@@ -1586,23 +1752,62 @@ int pic_step_graph(pic_ctx* c, int32_t nsteps) {
         PIC_HIP(hipGraphLaunch(c->graphExec[p], c->stream));
     }
     for (SpeciesMem& m : c->sp) m.shiftReady = 0;
-    return checkErrors(c);
+    return PIC_OK;
 }
 
-int pic_field_energy(pic_ctx* c, double out[2]) {
+
+}  // namespace
+
+extern "C" int pic_fdtd_step(pic_ctx* c) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    return fieldStep(c);
+}
+
+extern "C" {
+
+int pic_step(pic_ctx* c, int32_t nsteps) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
+    if (c->broken) return fail(PIC_E_STATE, "particles were lost to a failed capacity growth");
+    // steps after a capacity overflow are skipped on the device; they run
+    // again once the supercells have grown
+    int todo = nsteps;
+    do {
+        for (int32_t i = 0; i < todo; ++i) {
+            int const r = oneStep(c);
+            if (r != PIC_OK) return r;
+        }
+        int const r = recoverCapacity(c, &todo, 0);
+        if (r != PIC_OK) return r;
+    } while (todo > 0);
+    return PIC_OK;
+}
+
+int pic_step_graph(pic_ctx* c, int32_t nsteps) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (nsteps < 0) return fail(PIC_E_INVALID, "nsteps must be >= 0");
+    if (c->timing) return fail(PIC_E_STATE, "pic_step_graph cannot run with kernel timing enabled");
+    if (c->broken) return fail(PIC_E_STATE, "particles were lost to a failed capacity growth");
+    int todo = nsteps;
+    do {
+        int const r = stepGraphs(c, todo);
+        if (r != PIC_OK) return r;
+        int const rr = recoverCapacity(c, &todo, 0);  // a growth drops the graphs: recaptured on the next replay
+        if (rr != PIC_OK) return rr;
+    } while (todo > 0);
+    return PIC_OK;
+}
+
+int pic_field_energy(pic_ctx* c, double out[7]) {
     if (!c || !out) return fail(PIC_E_INVALID, "null argument");
     pic::FieldArgs a = makeFieldArgs(c);
-    PIC_HIP(pic::launchEnergy(a, c->energyPartials, c->energyBlocks, c->stream));
-    std::vector<double> h(2 * c->energyBlocks);
-    PIC_HIP(hipMemcpyAsync(h.data(), c->energyPartials, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(pic::launchEnergy(a, c->energyPartials, c->energyBlocks, c->energyPartials + 6 * c->energyBlocks,
+                              c->stream));
+    double sums[6];
+    PIC_HIP(hipMemcpyAsync(sums, c->energyPartials + 6 * c->energyBlocks, sizeof(sums), hipMemcpyDeviceToHost,
+                           c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
-    double e = 0.0, b = 0.0;
-    for (int i = 0; i < c->energyBlocks; ++i) {
-        e += h[2 * i];
-        b += h[2 * i + 1];
-    }
-    out[0] = e;
-    out[1] = b;
+    fieldEnergyFromSums(c, sums, out);
     return PIC_OK;
 }
 
@@ -1626,9 +1831,9 @@ int pic_gauss_residual(pic_ctx* c, double* out) {
 
 int pic_error_flags(pic_ctx* c, uint32_t* flags) {
     if (!c || !flags) return fail(PIC_E_INVALID, "null argument");
-    PIC_HIP(hipMemcpyAsync(flags, c->errFlags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    PIC_HIP(hipMemsetAsync(c->errFlags, 0, sizeof(uint32_t), c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
+    *flags = c->hostFlags;
+    c->hostFlags = 0;
     return PIC_OK;
 }
 
@@ -2146,7 +2351,7 @@ int ensUploadTables(pic_ensemble* e) {
     std::vector<pic::ShiftEntry> shift;
     for (pic_ctx* c : e->sims)
         for (const SpeciesMem& m : c->sp)
-            if (m.desc.pusher != PIC_PUSHER_NONE) shift.push_back(pic::ShiftEntry{m.dev, c->g, c->errFlags});
+            if (m.desc.pusher != PIC_PUSHER_NONE) shift.push_back(pic::ShiftEntry{m.dev, c->g});
     PIC_HIP(hipMemcpyAsync(e->dShift, shift.data(), shift.size() * sizeof(pic::ShiftEntry), hipMemcpyHostToDevice,
                            e->stream));
     PIC_HIP(hipStreamSynchronize(e->stream));
@@ -2230,6 +2435,7 @@ int ensOneStep(pic_ensemble* e) {
     PIC_HIP(pic::launchShiftBatch(e->dShift, e->dShiftWork, e->nShiftWork, e->stream));
     PIC_HIP(pic::launchFieldStepBatch(e->solver, e->dField[p], e->dTiles, e->nTiles, e->dBlocks, e->nBlocks,
                                       e->stream));
+    PIC_HIP(pic::launchEndStepBatch(e->dField[p], static_cast<int>(e->sims.size()), e->stream));
     for (pic_ctx* c : e->sims) {
         for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
         c->eParity ^= 1;
@@ -2238,13 +2444,30 @@ int ensOneStep(pic_ensemble* e) {
     return PIC_OK;
 }
 
+// After batched steps: a member whose shift overflowed skipped the steps
+// after it; it grows (within the frames the batched push can list), and runs
+// them on its own (its own steps equal its batched ones bit for bit). The
+// members' buffers may have moved, so the device tables are written again
+// (the captured ensemble graphs read them, so they stay valid).
 int ensFinish(pic_ensemble* e) {
     PIC_HIP(hipStreamSynchronize(e->stream));
+    bool moved = false;
     for (pic_ctx* c : e->sims) {
         for (SpeciesMem& m : c->sp) m.shiftReady = 0;
-        int const r = checkErrors(c);
+        int skipped = 0;
+        uint32_t const raised = c->hostFlags;
+        c->hostFlags = 0;
+        int r = recoverCapacity(c, &skipped, pic::kListMaxSlots / pic::kSCV);
+        if (c->hostFlags & PIC_FLAG_CAPACITY_GREW) moved = true;
+        c->hostFlags |= raised;
         if (r != PIC_OK) return r;
+        while (skipped > 0) {
+            for (int i = 0; i < skipped; ++i)
+                if ((r = oneStep(c)) != PIC_OK) return r;
+            if ((r = recoverCapacity(c, &skipped, pic::kListMaxSlots / pic::kSCV)) != PIC_OK) return r;
+        }
     }
+    if (moved) return ensUploadTables(e);
     return PIC_OK;
 }
 
@@ -2281,7 +2504,7 @@ int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out) {
             supp = sp;
             pusher = c.species[s].pusher;
             if (c.species[s].frames_per_sc * pic::kSCV > pic::kListMaxSlots)
-                return bail(fail(PIC_E_INVALID, "ensemble species need at most 32 frames per supercell"));
+                return bail(fail(PIC_E_INVALID, "ensemble species need at most 64 frames per supercell"));
         }
         if (i == 0) {
             e->supp = supp;

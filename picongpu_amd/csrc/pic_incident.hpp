@@ -68,7 +68,8 @@ struct IncidentUpdate {
 // (float(*step) + stepOffset) * dt (BaseFunctorE::currentTimeOrigin)
 hipError_t launchIncident(const IncidentUpdate& u, const GridDev& g, float* const F[3], const int64_t* step,
                           float stepOffset, hipStream_t s);
-// *step += 1 (the field step counter)
-hipError_t launchAdvanceStep(int64_t* step, hipStream_t s);
+// end of a step: *step += 1 (the field step counter) and the step flags
+// (a halted simulation counts a skipped step; an overflow halts the next steps)
+hipError_t launchAdvanceStep(int64_t* step, StepFlags* flags, hipStream_t s);
 
 }  // namespace pic

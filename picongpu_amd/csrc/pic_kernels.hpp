@@ -24,7 +24,7 @@ struct SpeciesDev {
     int32_t* outIdx;     // per supercell: leaver slots grouped by exchange code
     int32_t* scratch;    // per supercell: gap/source lists of the compaction
     uint32_t* outOff;    // per supercell: 28 offsets into outIdx (code 1..26, [27]=total)
-    uint16_t* leaveBits; // per supercell and push thread: leaver flags per loop iteration
+    uint32_t* leaveBits; // per supercell and push thread: leaver flags per loop iteration
     int32_t cap;         // slots per supercell
     int32_t lists;       // 1: the push kernel builds the leaver lists of this species
     int32_t reflect;     // bit d: particles crossing an open border of axis d are reflected
@@ -32,11 +32,28 @@ struct SpeciesDev {
     float charge;        // frame::getCharge()
 };
 
+// Step state of one simulation on the device. A shift that finds a supercell
+// full keeps the true count, leaves the records it could not place in the
+// sender's outbox and sets `overflow`; the end-of-step kernel then sets
+// `halt`, and every kernel of the following steps returns at once (counting
+// the skipped steps), so no particle is dropped and the host can grow the
+// supercells, place the waiting records and run the skipped steps.
+struct StepFlags {
+    uint32_t overflow;  // a supercell of some species overflowed its capacity in a shift
+    uint32_t halt;      // steps after the overflowing one are skipped
+    uint32_t skipped;   // steps skipped while halted
+    uint32_t pad;
+};
+
 struct GridDev {
     int nx, ny, nz;
     int nscx, nscy, nscz;
     int open;  // bit d: axis d has open borders (fields read zero beyond, particles leaving are absorbed)
+    StepFlags* flags;  // step state (null: never halted)
 };
+
+// every kernel of a step returns at once while the simulation is halted
+__device__ __forceinline__ bool halted(const GridDev& g) { return g.flags != nullptr && g.flags->halt != 0u; }
 
 // (x, y, z) lies beyond the border of an open axis
 __host__ __device__ __forceinline__ bool outsideOpen(const GridDev& g, int x, int y, int z) {
@@ -57,7 +74,6 @@ struct PushArgs {
     int nsp;
     SpeciesDev sp[PIC_MAX_SPECIES];
     const int32_t* scList;  // supercells of this launch, in launch order
-    uint32_t* errFlags;
     int stageShift;  // 1: the epilogue also moves the leavers to the outbox and compacts the supercell
 };
 
@@ -78,14 +94,25 @@ struct FieldArgs {
 hipError_t launchPush(int supp, int pusher, int mode, int currentSolver, const PushArgs& a, int nblocks,
                       hipStream_t s);
 // shift of one species: `staged` = the push epilogue already moved the leavers
-// to the outbox and compacted every supercell (then only the insert runs)
-hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, uint32_t* errFlags, bool staged, hipStream_t s);
+// to the outbox and compacted every supercell (then only the insert runs).
+// redo: the insert again after the supercells grew, from count - incoming
+// (the count an overflowing insert left) and regardless of the halt.
+hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, bool staged, hipStream_t s);
+hipError_t launchInsertRedo(const GridDev& g, const SpeciesDev& sp, hipStream_t s);
+// count[sc] = min(count[sc], cap): the last resort when a supercell cannot grow
+hipError_t launchClampCounts(const SpeciesDev& sp, int nsc, hipStream_t s);
+// rows of oldRowBytes from src (pitch oldRowBytes) to dst (pitch newRowBytes),
+// both multiples of 16
+hipError_t launchRowCopy(void* dst, const void* src, size_t oldRowBytes, size_t newRowBytes, size_t rows,
+                         hipStream_t s);
 hipError_t launchCollect(const GridDev& g, const SpeciesDev& sp, hipStream_t s);
 // the push kernel builds leaver lists for species with at most this many slots per supercell
 constexpr int kPushThreadsPerBlock = 512;
-constexpr int kPushMaxIters = 16;
+constexpr int kPushMaxIters = 32;  // bits of a leaveBits word
 constexpr int kListMaxSlots = kPushThreadsPerBlock * kPushMaxIters;
 hipError_t launchFill(float* p, size_t n, float v, hipStream_t s);
+// J = 0 at the start of a step (skipped while halted)
+hipError_t launchCurrentReset(const GridDev& g, float* const J[3], size_t n, hipStream_t s);
 hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s);
 // E full step and J -> E (None) per cell in one pass into Enew (may be a.E)
 hipError_t launchEFullJ(const FieldArgs& a, float* const Enew[3], hipStream_t s);
@@ -110,7 +137,9 @@ hipError_t launchAbsorb(const GridDev& g, float* const F[3], const AbsorbPlan& p
 hipError_t launchFieldStep(int solver, const FieldArgs& a, float* const Enew[3], const AbsorbPlan* ab, hipStream_t s);
 // tiles of the fused field step of one grid (work items of the batched form)
 int fieldStepTiles(int solver, const GridDev& g);
-hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, hipStream_t s);
+// per component (B x, y, z, E x, y, z) the FP64 sum of the FP32 squares:
+// nblocks partial sums (6 * nblocks doubles of scratch), then out[0..5]
+hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, double* out, hipStream_t s);
 hipError_t launchCountSum(const uint32_t* counts, int n, unsigned long long* out, hipStream_t s);
 hipError_t launchGaussResidual(const FieldArgs& f, const SpeciesDev* sp, const int* supp, int nsp, double qScale,
                                double eps0, double* buf, hipStream_t s);
@@ -154,7 +183,6 @@ hipError_t launchNoise(float* f, size_t n, uint64_t seed, int comp, float amp, h
 struct ShiftEntry {
     SpeciesDev S;
     GridDev g;
-    uint32_t* errFlags;
 };
 // items (simulation, supercell) into a table of PushArgs (fused push + deposit)
 hipError_t launchPushBatch(int supp, int pusher, const PushArgs* table, const int2* work, int nwork, int nblocks,
@@ -172,6 +200,8 @@ hipError_t launchFieldStepBatch(int solver, const FieldBatchEntry* table, const 
                                 const int2* blocks, int nblocks, hipStream_t s);
 // J = 0 for items (simulation, block of 256 cells)
 hipError_t launchCurrentResetBatch(const FieldBatchEntry* table, const int2* blocks, int nblocks, hipStream_t s);
+// end of a batched step: every member's step flags as the end-of-step kernel of pic_step sets them
+hipError_t launchEndStepBatch(const FieldBatchEntry* table, int n, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // differentiable form (one CIC/Boris species in identity order) and its adjoint

@@ -44,19 +44,28 @@ def make_pair(setup, e_amp=1e-3, b_amp=1e-3, seed=11, order=0, with_sim=True):
     return sim, ora, frames
 
 
+def occupied(frames) -> np.ndarray:
+    """Mask of the occupied slots [0, count) of every supercell of a frame pool
+    (its slots per supercell follow from the array sizes)."""
+    cap = frames["w"].shape[0] // frames["counts"].shape[0]
+    return (np.arange(cap)[None, :] < frames["counts"][:, None].astype(np.int64)).ravel()
+
+
 def assert_frames_equal(gpu, ora, what=""):
-    """Slot-exact equality of two frame pools (float compared bitwise)."""
+    """Slot-exact equality of two frame pools (float compared bitwise). The two
+    pools may have different slots per supercell (a grown pool against one
+    sized large enough from the start): the occupied slots of each supercell
+    are compared in slot order."""
     np.testing.assert_array_equal(gpu["counts"], ora["counts"], err_msg=f"{what}: supercell counts")
-    cap = gpu["w"].shape[0] // gpu["counts"].shape[0]
-    valid = (np.arange(cap)[None, :] < gpu["counts"][:, None].astype(np.int64)).ravel()
+    vg, vo = occupied(gpu), occupied(ora)
     for key in ("pos", "mom"):
-        g = bits(gpu[key][:, valid])
-        o = bits(np.asarray(ora[key][:, valid], np.float32))
+        g = bits(gpu[key][:, vg])
+        o = bits(np.asarray(ora[key][:, vo], np.float32))
         bad = np.nonzero(g != o)
         assert bad[0].size == 0, f"{what}: {key} differs at {bad[0].size} entries, first {bad[1][:5]}"
-    np.testing.assert_array_equal(bits(gpu["w"][valid]), bits(np.asarray(ora["w"][valid], np.float32)))
-    np.testing.assert_array_equal(gpu["cell"][valid], ora["cell"][valid], err_msg=f"{what}: localCellIdx")
-    np.testing.assert_array_equal(gpu["mask"][valid], ora["mask"][valid], err_msg=f"{what}: multiMask")
+    np.testing.assert_array_equal(bits(gpu["w"][vg]), bits(np.asarray(ora["w"][vo], np.float32)))
+    np.testing.assert_array_equal(gpu["cell"][vg], ora["cell"][vo], err_msg=f"{what}: localCellIdx")
+    np.testing.assert_array_equal(gpu["mask"][vg], ora["mask"][vo], err_msg=f"{what}: multiMask")
 
 
 def assert_field_equal(gpu, ora, what=""):

@@ -144,3 +144,24 @@ def test_checkpoint_spacing_does_not_change_gradient():
         pic.close()
     for a, b in out[1:]:
         assert rel_max(a, out[0][0]) < 1e-12 and rel_max(b, out[0][1]) < 1e-12
+
+
+def test_backward_of_an_older_forward_is_refused():
+    """The context keeps the checkpoints of its last forward only: a backward
+    through the autograd function after another forward must raise instead of
+    returning the gradients of the newer forward."""
+    nsteps = 4
+    print("case adjoint generation check", flush=True)
+    st, pic, g, X, p = make(nsteps, 2)
+    theta = torch.tensor([1.0, 0.0, 0.0], dtype=F64)
+    E0, B0 = adjoint.gaussian_pulse(st, theta.cuda(), amp=pulse_amp(st))
+    E0 = E0.clone().requires_grad_(True)
+    E, B = pic.fields(E0, B0)
+    obj = adjoint.probe_energy(st, E, B, LO, HI)
+    pic.forward_raw(E0.detach() * 0.5, B0)  # a second forward on the same context
+    with pytest.raises(RuntimeError, match="another forward"):
+        obj.backward()
+    E, B = pic.fields(E0, B0)  # forward again: the backward of the newest forward works
+    adjoint.probe_energy(st, E, B, LO, HI).backward()
+    assert E0.grad is not None and torch.isfinite(E0.grad).all()
+    pic.close()

@@ -1,9 +1,13 @@
 """Batched ensembles: every member advanced by the batched launches must equal
 the same simulation advanced on its own with pic_step, bit for bit (frames,
-E, B, J), whatever the schedule, eager or replayed from graphs."""
+E, B, J), whatever the schedule, eager or replayed from graphs, and the
+oracle run on the member's initial state."""
+import copy
+
 import numpy as np
 import pytest
 
+from oracle import oracle as O
 from picongpu_amd import abi, setups
 from tests import helpers as H
 
@@ -103,4 +107,89 @@ def test_member_stepped_alone_in_pairs():
     for i, (m, s) in enumerate(zip(ens.members, ref)):
         compare(m, s, f"member {i}")
         s.close()
+    ens.close()
+
+
+def oracle_of(member, st, frames=None):
+    """The oracle loaded with the member's current frames and fields (frames:
+    slots per supercell of the oracle's pool, default the member's)."""
+    big = copy.deepcopy(st)
+    if frames is not None:
+        big.species[0].frames_per_sc = frames
+    ora = O.Oracle(big, 32, 0, jexp=member.j_scale_exp())
+    f = member.read_frames(0)
+    if frames is not None:
+        v = H.occupied(f)
+        counts = f["counts"]
+        g = {"counts": counts}
+        cap = frames * 256
+        nsc = counts.shape[0]
+        slot = (np.repeat(np.arange(nsc, dtype=np.int64) * cap, counts.astype(np.int64)) +
+                np.concatenate([np.arange(c, dtype=np.int64) for c in counts]))
+        for key, shape, dt in (("pos", 3, np.float32), ("mom", 3, np.float32), ("w", 0, np.float32),
+                               ("cell", 0, np.uint16), ("mask", 0, np.uint8)):
+            if shape:
+                a = np.zeros((shape, nsc * cap), dt)
+                a[:, slot] = f[key][:, v]
+            else:
+                a = np.zeros(nsc * cap, dt)
+                a[slot] = f[key][v]
+            g[key] = a
+        f = g
+    ora.set_frames(0, f)
+    for fld in (abi.FIELD_E, abi.FIELD_B, abi.FIELD_J):
+        ora.set_field(fld, member.read_field(fld))
+    return ora
+
+
+def compare_oracle(member, ora, what):
+    H.assert_frames_equal(member.read_frames(0), ora.get_frames(0), f"{what} frames")
+    for f in (abi.FIELD_E, abi.FIELD_B, abi.FIELD_J):
+        H.assert_field_equal(member.read_field(f), ora.get_field(f), f"{what} field {f}")
+
+
+@pytest.mark.parametrize("mode", ["eager", "graph"])
+def test_ensemble_members_match_oracle(mode):
+    """Each member's batched steps against the oracle started from the
+    member's initial state (generated on the device), bit for bit."""
+    sts = members()
+    nsteps = 5
+    print(f"case ensemble vs oracle {mode}", flush=True)
+    ens = abi.Ensemble([st.config() for st in sts])
+    oras = []
+    for m, st in zip(ens.members, sts):
+        init(m, st)
+        oras.append(oracle_of(m, st))
+    if mode == "eager":
+        ens.step(nsteps)
+    else:
+        ens.step_graph(nsteps)
+    for i, (m, o) in enumerate(zip(ens.members, oras)):
+        o.step(nsteps)
+        compare_oracle(m, o, f"member {i}")
+        o.close()
+    ens.close()
+
+
+def test_ensemble_member_grows_without_loss():
+    """A member whose supercells start at exactly their particle count (one
+    frame per 256 particles) overflows in the first shift: it grows, runs the
+    skipped steps on its own, and still matches the oracle run at a capacity
+    that never overflows; the other members are untouched."""
+    sts = members()
+    sts[1].species[0].frames_per_sc = 3  # 3 ppc x 256 cells: exactly full
+    print("case ensemble member growth", flush=True)
+    ens = abi.Ensemble([st.config() for st in sts])
+    oras = []
+    for m, st in zip(ens.members, sts):
+        init(m, st)
+        oras.append(oracle_of(m, st, frames=24))
+    ens.step_graph(3)
+    ens.step(3)
+    assert ens.members[1].slots(0) > 3 * 256
+    assert ens.members[1].error_flags() & abi.FLAG_CAPACITY_GREW
+    for i, (m, o) in enumerate(zip(ens.members, oras)):
+        o.step(6)
+        compare_oracle(m, o, f"member {i}")
+        o.close()
     ens.close()

@@ -98,12 +98,12 @@ def test_absorber_removes_a_pulse_an_open_border_reflects_it():
         B[2] = (np.tile(np.exp(-((x + 0.5 - 128) / 10.0) ** 2), 32) / k.c).astype(np.float32)
         ora.set_field(abi.FIELD_E, E)
         ora.set_field(abi.FIELD_B, B)
-        e0 = sum(ora.field_energy())
+        e0 = ora.field_energy()[0]
         for _ in range(400):  # the pulse travels about 230 cells: into the border and back
             ora.update_before_current()
             ora.add_current()
             ora.update_after_current()
-        e1 = sum(ora.field_energy())
+        e1 = ora.field_energy()[0]
         ora.close()
         return e1 / e0
     assert run(abi.ABSORBER_NONE) > 0.99
