@@ -17,9 +17,10 @@ write x,p = 24 B) and 96 B per cell-step for the whole step; the fused
 push/deposit kernel alone moves 55 B per particle plus 36 B per cell
 (E, B read; J written).
 
-The CPU baseline is the FP32 oracle (C++/OpenMP, the same per-particle
-arithmetic) timed on this host on a bounded sample of the same physics (same
-species, shapes, ppc and cell/time step on a smaller periodic box).
+The CPU baseline is the FP32 oracle in the reference CPU's order (C++/OpenMP,
+the same per-particle arithmetic, the omp2b deposit order) timed on this host
+on a bounded sample of the same physics (same species, shapes, ppc and
+cell/time step on a smaller periodic box), 10 steps after 2 warm-ups.
 """
 from __future__ import annotations
 
@@ -53,15 +54,29 @@ BYTES_PER_CELL_STEP = 96.0
 BYTES_PER_CELL_PUSH = 36.0
 
 
-def cpu_baseline(setup_fn, grid, steps=4, warmup=1):
-    import numpy as np  # noqa: F401
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(setup_fn, grid, steps=10, warmup=2):
+    """The oracle in the reference CPU's order (alpaka omp2b semantics: push and
+    shift, then per supercell a float cache in reverse frame order flushed
+    under a stride-2 checkerboard), FP32, OpenMP over this host's cores, on a
+    bounded sample of the same physics (same species, shape, ppc, cell and
+    time step on a smaller periodic box)."""
     from oracle import oracle as O
     from picongpu_amd import setups
 
     st = setup_fn(grid=grid)
     threads = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    ora = O.Oracle(st, 32, 0, jexp=40)
+    ora = O.Oracle(st, 32, 1)
     for sp in range(len(st.species)):
         cells, pos, mom, w = setups.host_particles(st, sp)
         ora.set_frames(sp, setups.bin_frames(st, sp, cells, pos, mom, w))
@@ -71,8 +86,9 @@ def cpu_baseline(setup_fn, grid, steps=4, warmup=1):
     dt = time.perf_counter() - t0
     n = st.mobile_particles()
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{st.name} {grid[0]}x{grid[1]}x{grid[2]} cells, {n} particles, {steps} steps "
-                      f"after {warmup} warm-up, FP32 oracle with OpenMP, {dt:.2f} s"}
+            "sample": f"{st.name} {grid[0]}x{grid[1]}x{grid[2]} cells, {n} particles, {steps} steps after "
+                      f"{warmup} warm-up, FP32 oracle in the reference CPU order (omp2b), {threads} OpenMP threads "
+                      f"on {cpu_model()}, {dt:.2f} s"}
 
 
 def adjoint_timing(steps=200, every=20, grid=(64, 64, 64), ppc=8):
@@ -180,7 +196,7 @@ def main():
     ap.add_argument("--laser", default="initial", choices=["initial", "incident"],
                     help="laser_plasma only: the pulse as the initial field or injected through the Huygens surface")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-grid", default="128,128,64")
+    ap.add_argument("--cpu-grid", default="128,64,64")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble timing")
     ap.add_argument("--calibrate-copy", action="store_true",
