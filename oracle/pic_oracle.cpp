@@ -26,6 +26,8 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <type_traits>
@@ -933,7 +935,10 @@ void push(State<T>& st, int species, bool fused) {
             }
             continue;
         }
-        int const lo = depLower(supp) + 1, up = depUpper(supp) + 1;
+        // The device's tile: the deposit margins around the supercell. The
+        // deposit runs in the frame of the new cell, whose cells lie within
+        // the margins of the old one (a move is less than a cell per axis).
+        int const lo = depLower(supp), up = depUpper(supp);
         int const scs[3] = {SCX, SCY, SCZ};
         int stride[3];
         for (int d = 0; d < 3; ++d) stride[d] = (lo + up + scs[d] - 1) / scs[d] + 1;
@@ -972,8 +977,12 @@ void push(State<T>& st, int species, bool fused) {
                             depositParticle(S.desc.current_solver, supp, pos, vel, charge, st.k,
                                             [&](int c, int x, int y, int z, T v) {
                                 int64_t const q = toFixed(v, jscale);
-                                tile.acc[c * tv + (static_cast<size_t>(nz + z) * tile.ty + (ny + y)) * tile.tx +
-                                         (nx + x)] += q;
+                                int const tx = nx + x, ty = ny + y, tz = nz + z;
+                                if (tx < 0 || ty < 0 || tz < 0 || tx >= tile.tx || ty >= tile.ty || tz >= tile.tz) {
+                                    std::fprintf(stderr, "oracle: deposit outside the supercell tile\n");
+                                    std::abort();
+                                }
+                                tile.acc[c * tv + (static_cast<size_t>(tz) * tile.ty + ty) * tile.tx + tx] += q;
                             });
                         } else {
                             // FP64 twin: plain sums straight into J (each supercell of a phase is disjoint)

@@ -32,12 +32,12 @@ constexpr int kPushWaves = kPushThreads / 64;
 // need SUPP^2*(SUPP-1). The push deposits the latter at once and queues the
 // former per wave: lane l holds queued item l in A (items 64..127 in B), and
 // a full A is deposited by all 64 lanes together. An item carries what the
-// deposit reads: the tile cell, the position after the move, the velocity
-// and the charge, so the contribution is the one the particle would have
-// made in place.
+// deposit reads: the tile cell, the position after the move, the move in
+// cells (vel * dt / cell) and the charge, so the contribution is the one the
+// particle would have made in place.
 struct SlowItem {
     int cell;  // tile cell x | y << 8 | z << 16
-    float x, y, z, vx, vy, vz, q;
+    float x, y, z, dx, dy, dz, q;
 };
 
 __device__ __forceinline__ int permuteLane(int dstLane, int v) { return __builtin_amdgcn_ds_permute(dstLane << 2, v); }
@@ -69,9 +69,9 @@ __device__ __forceinline__ void queueSlow(bool slow, const SlowItem& mine, SlowI
     PIC_QUEUE_MOVE(x)
     PIC_QUEUE_MOVE(y)
     PIC_QUEUE_MOVE(z)
-    PIC_QUEUE_MOVE(vx)
-    PIC_QUEUE_MOVE(vy)
-    PIC_QUEUE_MOVE(vz)
+    PIC_QUEUE_MOVE(dx)
+    PIC_QUEUE_MOVE(dy)
+    PIC_QUEUE_MOVE(dz)
     PIC_QUEUE_MOVE(q)
 #undef PIC_QUEUE_MOVE
     P += k;
@@ -80,8 +80,10 @@ __device__ __forceinline__ void queueSlow(bool slow, const SlowItem& mine, SlowI
 template <int SUPP, int TX, int TY, int TV>
 __device__ __forceinline__ void depositQueued(unsigned long long* __restrict__ sJ, const SlowItem& t,
                                               const DepositConsts& dk) {
-    Esirkepov<SUPP>::template deposit<TX, TY, TV>(sJ, t.cell & 255, (t.cell >> 8) & 255, t.cell >> 16,
-                                                  Vec3{t.x, t.y, t.z}, Vec3{t.vx, t.vy, t.vz}, t.q, dk);
+    float const dp[3] = {t.dx, t.dy, t.dz};
+    Esirkepov<SUPP>::template depositGeometry<TX, TY, TV, true>(sJ, t.cell & 255, (t.cell >> 8) & 255, t.cell >> 16,
+                                                                Esirkepov<SUPP>::geometryDp(Vec3{t.x, t.y, t.z}, dp),
+                                                                t.q, dk);
 }
 
 template <int SUPP, int MODE>
@@ -89,8 +91,8 @@ struct PushLds {
     static constexpr int GLO = SUPP / 2;
     static constexpr int GUP = (SUPP + 1) / 2;
     static constexpr int GV = (SCX + GLO + GUP) * (SCY + GLO + GUP) * (SCZ + GLO + GUP);
-    static constexpr int DLO = Esirkepov<SUPP>::lowerMargin + (MODE == 1 ? 1 : 0);
-    static constexpr int DUP = Esirkepov<SUPP>::upperMargin + (MODE == 1 ? 1 : 0);
+    static constexpr int DLO = Esirkepov<SUPP>::lowerMargin;
+    static constexpr int DUP = Esirkepov<SUPP>::upperMargin;
     static constexpr int JV = (SCX + DLO + DUP) * (SCY + DLO + DUP) * (SCZ + DLO + DUP);
     static constexpr int EB_ELEMS = MODE == 2 ? 1 : 6 * GV;
     static constexpr int J_ELEMS = MODE == 0 ? 1 : 3 * JV;
@@ -110,10 +112,14 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     constexpr int GUP = (SUPP + 1) / 2;    // FieldToParticleInterpolation.hpp:50
     constexpr int GX = SCX + GLO + GUP, GY = SCY + GLO + GUP, GZ = SCZ + GLO + GUP;
     constexpr int GV = GX * GY * GZ;
-    // fused: the deposit runs in the frame of the particle's new cell, which
-    // may lie one cell outside the supercell
-    constexpr int DLO = Esirkepov<SUPP>::lowerMargin + (MODE == 1 ? 1 : 0);
-    constexpr int DUP = Esirkepov<SUPP>::upperMargin + (MODE == 1 ? 1 : 0);
+    // The J tile is the supercell plus the reference's deposit margins. In
+    // the fused form the deposit runs in the frame of the particle's new
+    // cell, which may lie one cell outside the supercell, but the cells it
+    // touches still lie within those margins of the particle's old cell: its
+    // path starts inside the old cell and moves less than one cell per axis
+    // (the host requires float(c*dt)/cell < 1).
+    constexpr int DLO = Esirkepov<SUPP>::lowerMargin;
+    constexpr int DUP = Esirkepov<SUPP>::upperMargin;
     constexpr int JX = SCX + DLO + DUP, JY = SCY + DLO + DUP, JZ = SCZ + DLO + DUP;
     constexpr int JV = JX * JY * JZ;
     constexpr int EB_ELEMS = MODE == 2 ? 1 : 6 * GV;
@@ -227,15 +233,18 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                     else
                         pushBoris(e, b, mom, vel, mass, charge, pk);
                     float p[3] = {pos.x, pos.y, pos.z};
-                    p[0] += (vel.x * pk.dt) / pk.cell[0];
-                    p[1] += (vel.y * pk.dt) / pk.cell[1];
-                    p[2] += (vel.z * pk.dt) / pk.cell[2];
+                    // the move in cells; Esirkepov forms the same quotients (Esirkepov.hpp:82)
+                    float dp[3] = {(vel.x * pk.dt) / pk.cell[0], (vel.y * pk.dt) / pk.cell[1],
+                                   (vel.z * pk.dt) / pk.cell[2]};
+                    p[0] += dp[0];
+                    p[1] += dp[1];
+                    p[2] += dp[2];
                     int dir[3], newIdx, newMask;
                     moveParticle(p, lidx, dir, newIdx, newMask);
                     if (S.reflect) {  // the species' reflecting borders (boundary::apply after the push)
                         int cellNow[3] = {ox + lx + dir[0], oy + ly + dir[1], oz + lz + dir[2]};
                         int const nGrid[3] = {a.g.nx, a.g.ny, a.g.nz};
-                        if (reflectParticle(nGrid, S.reflect, cellNow, p, mom, vel, dir))
+                        if (reflectParticle(nGrid, S.reflect, cellNow, p, mom, vel, dir, dp))
                             cellAfterMove(lidx, dir, newIdx, newMask);
                     }
                     S.pos[0][gidx] = p[0];
@@ -252,14 +261,16 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                     if (MODE == 1 && !(a.g.open && outsideOpen(a.g, ox + lx + dir[0], oy + ly + dir[1], oz + lz + dir[2]))) {
                         int const cx = lx + dir[0] + DLO, cy = ly + dir[1] + DLO, cz = lz + dir[2] + DLO;
                         Vec3 const np{p[0], p[1], p[2]};
-                        auto const G = Esirkepov<SUPP>::geometry(np, vel, dk);
                         if (EMZ) {
-                            EmZ<SUPP>::template deposit<JX, JY, JV>(sJ, cx, cy, cz, np, vel, charge, dk);
-                        } else if (Esirkepov<SUPP>::changesCell(G)) {
-                            slow = true;
-                            mine = SlowItem{cx | (cy << 8) | (cz << 16), np.x, np.y, np.z, vel.x, vel.y, vel.z, charge};
+                            EmZ<SUPP>::template depositDp<JX, JY, JV>(sJ, cx, cy, cz, np, dp, charge, dk);
                         } else {
-                            Esirkepov<SUPP>::template depositGeometry<JX, JY, JV, false>(sJ, cx, cy, cz, G, charge, dk);
+                            auto const G = Esirkepov<SUPP>::geometryDp(np, dp);
+                            if (Esirkepov<SUPP>::changesCell(G)) {
+                                slow = true;
+                                mine = SlowItem{cx | (cy << 8) | (cz << 16), np.x, np.y, np.z, dp[0], dp[1], dp[2], charge};
+                            } else {
+                                Esirkepov<SUPP>::template depositStay<JX, JY, JV>(sJ, cx, cy, cz, G, charge, dk);
+                            }
                         }
                     }
                 } else {

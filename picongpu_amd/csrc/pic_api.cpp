@@ -157,6 +157,14 @@ void computeConstants(const pic_config& c, pic_constants& k) {
 
 // CFL checks: Yee/Yee.hpp:43-65, Lehe/Lehe.hpp:47-69
 int checkCfl(const pic_config& c, const pic_constants& k) {
+    // moveParticle and the deposit's J tile assume a move of less than one
+    // cell per axis: dp = float(float(v*dt)/cell) with |v| < c is below 1 on
+    // every axis when the same quotient of c is
+    for (int d = 0; d < 3; ++d) {
+        float const cdt = k.c * k.dt;
+        if (!(cdt / k.cell[d] < 1.0f))
+            return fail(PIC_E_INVALID, "c*dt reaches a cell size: a particle could move a whole cell per step");
+    }
     if (c.solver == PIC_SOLVER_YEE) {
         float inv = 0.0f;
         for (int d = 0; d < 3; ++d) inv += 1.0f / (k.cell[d] * k.cell[d]);
@@ -726,9 +734,14 @@ int resolveTimers(pic_ctx* c) {
     return PIC_OK;
 }
 
+// The J tile of a supercell spans the reference's deposit margins in both
+// forms: a fused deposit runs in the frame of the particle's new cell, but
+// its cells lie within the margins of the old one (the path starts in the
+// old cell and moves less than a cell per axis).
 int buildPlan(pic_ctx* c, int supp, bool fused, PhasePlan& plan) {
-    int const lo = depLower(supp) + (fused ? 1 : 0);
-    int const up = depUpper(supp) + (fused ? 1 : 0);
+    (void)fused;
+    int const lo = depLower(supp);
+    int const up = depUpper(supp);
     const int scs[3] = {pic::kSCX, pic::kSCY, pic::kSCZ};
     const int nsc[3] = {c->g.nscx, c->g.nscy, c->g.nscz};
     std::vector<int> col[3];

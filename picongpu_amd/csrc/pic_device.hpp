@@ -65,6 +65,9 @@ struct Shape<2> {  // CIC.hpp:38-161
     }
     // ChargeAssignmentOnSupport::operator() (CIC.hpp:139-147)
     __device__ __forceinline__ static float onSupportAt(float x) { return 1.0f - fabsf(x); }
+    // assign(1 - p) for p in [0,1): 1 - p lies in (0,1], and at 1 both
+    // branches give 0, so the value is 1 - |1 - p| without a branch
+    __device__ __forceinline__ static float assignAt1(float p) { return 1.0f - fabsf(1.0f - p); }
 };
 
 template <>
@@ -107,6 +110,10 @@ struct Shape<3> {  // TSC.hpp:39-201
         float const rad2 = ff2(abs_x);
         return below_05 ? rad1 : rad2;
     }
+    // assign(1 - p) for p in [-0.5,0.5): |1 - p| lies in [0.5,1.5] (the
+    // rounded difference included), where assign takes the ff2 branch below
+    // 1.5; at 1.5 its zero branch equals ff2(1.5) = 0
+    __device__ __forceinline__ static float assignAt1(float p) { return ff2(fabsf(1.0f - p)); }
 };
 
 template <>
@@ -151,6 +158,9 @@ struct Shape<4> {  // PQS.hpp:38-205
         float const rad2 = ff2(abs_x);
         return below_1 ? rad1 : rad2;
     }
+    // assign(1 - p) for p in [0,1): 1 - p lies in (0,1], where assign takes
+    // the ff1 branch; at 1 ff2 gives the same 1/6 (4 - 6 + 3 = 1 exactly)
+    __device__ __forceinline__ static float assignAt1(float p) { return ff1(fabsf(1.0f - p)); }
 };
 
 // ChargeAssignment::shapeArray(pos, isOutOfRange): SUPP+1 values for grid
@@ -419,8 +429,10 @@ __device__ __forceinline__ void moveParticle(float* pos, int localIdx, int* dir,
 // are negated (Velocity is odd in each momentum component) and moveParticle's
 // position arithmetic runs on all three axes; dir accumulates the steps.
 // n: the grid's cells per axis. Returns whether anything was reflected.
+// dp, the move in cells per axis (vel * dt / cell), is negated with the
+// velocity: (-v dt) / cell is -(v dt / cell) exactly.
 __device__ __forceinline__ bool reflectParticle(const int* n, int reflect, int* cell, float* pos, Vec3& mom, Vec3& vel,
-                                                int* dir) {
+                                                int* dir, float* dp) {
     bool any = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -428,6 +440,7 @@ __device__ __forceinline__ bool reflectParticle(const int* n, int reflect, int* 
         bool const above = cell[d] >= n[d], below = cell[d] < 0;
         if (!(above || below)) continue;
         pos[d] = above ? -pos[d] - 1.1920929e-7f : 2.0f - pos[d];  // std::numeric_limits<float>::epsilon()
+        dp[d] = -dp[d];
         if (d == 0) {
             mom.x = -mom.x;
             vel.x = -vel.x;
@@ -543,10 +556,11 @@ struct Esirkepov {
         int gs[3], st[3];
     };
 
-    __device__ __forceinline__ static Geometry geometry(Vec3 pos, Vec3 vel, const DepositConsts& k) {
+    // dp: the move in cells per axis, vel * dt / cell as Esirkepov.hpp:82
+    // forms it (the push forms the same quotient for the position update)
+    __device__ __forceinline__ static Geometry geometryDp(Vec3 pos, const float* dp) {
         constexpr bool isEven = (SUPP % 2) == 0;
         float const np[3] = {pos.x, pos.y, pos.z};
-        float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
         Geometry G;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -568,9 +582,87 @@ struct Esirkepov {
         return G;
     }
 
+    __device__ __forceinline__ static Geometry geometry(Vec3 pos, Vec3 vel, const DepositConsts& k) {
+        float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
+        return geometryDp(pos, dp);
+    }
+
     // true when the move changes the shifted cell on some axis (the general form is needed)
     __device__ __forceinline__ static bool changesCell(const Geometry& G) {
         return ((G.st[0] | G.st[1] | G.st[2]) & 1) != 0;
+    }
+
+    // Shape values of one axis for a particle whose start and end lie in the
+    // shifted cell (st = 6): the cached arrays (onSupport at the start and
+    // end position, cachedShape without shift) and the outer loop's exact
+    // ("Jit") values assign(i - p). For such positions assign(i - p) takes a
+    // fixed branch for every i in [begin, end) and equals the cached value,
+    // the same operations on the same float, for every i but i = 1, whose
+    // value is Shape::assignAt1(p). So an axis needs its two cached arrays
+    // and two assignAt1 values, shared by the three components.
+    struct StayAxis {
+        float s0[SUPP], s1[SUPP];  // grid points begin..end-1
+        float j0, j1;              // assign(1 - p0), assign(1 - p1)
+    };
+
+    __device__ __forceinline__ static StayAxis stayAxis(float p0, float p1) {
+        StayAxis a;
+        Shape<SUPP>::onSupport(p0, a.s0);
+        Shape<SUPP>::onSupport(p1, a.s1);
+        a.j0 = Shape<SUPP>::assignAt1(p0);
+        a.j1 = Shape<SUPP>::assignAt1(p1);
+        return a;
+    }
+
+    // cpt1D for a particle that stays in its shifted cell, with the rotated
+    // axes' shape values given: the cells and values of cpt1D<..., false>.
+    template <int SI, int SJ, int SK>
+    __device__ __forceinline__ static void cpt1DStay(unsigned long long* __restrict__ jt, const StayAxis& ai,
+                                                     const StayAxis& aj, const StayAxis& ak, float p0k, float p1k,
+                                                     float csd, double jscale) {
+        if (p0k == p1k) return;
+        float dsk[SUPP - 1];
+#pragma unroll
+        for (int k = 0; k < SUPP - 1; ++k) dsk[k] = ak.s1[k] - ak.s0[k];
+#pragma unroll
+        for (int i = begin; i < end; ++i) {
+            float const s0i = i == 1 ? ai.j0 : ai.s0[i - begin];
+            float const dsi = (i == 1 ? ai.j1 : ai.s1[i - begin]) - s0i;
+#pragma unroll
+            for (int j = begin; j < end; ++j) {
+                float const sj0 = aj.s0[j - begin];
+                float const dsj = aj.s1[j - begin] - sj0;
+                float const tmp = -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
+                float accumulated_J = 0.0f;
+#pragma unroll
+                for (int k = begin; k < end - 1; ++k) {
+                    float const W = dsk[k - begin] * tmp;
+                    accumulated_J = k == begin ? W : accumulated_J + W;
+                    __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
+                                           toFixed(accumulated_J, jscale), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+    }
+
+    // Deposit of a particle with changesCell(G) == false: the contributions
+    // of depositGeometry<..., false>, bit for bit, with each axis' shape
+    // values formed once.
+    template <int TX, int TY, int TV>
+    __device__ __forceinline__ static void depositStay(unsigned long long* __restrict__ jt, int cx, int cy, int cz,
+                                                       const Geometry& G, float charge, const DepositConsts& k) {
+        constexpr int PX = 1, PY = TX, PZ = TX * TY;
+        unsigned long long* const corner =
+            jt + ((cz + G.gs[2] + begin) * TY + (cy + G.gs[1] + begin)) * TX + (cx + G.gs[0] + begin);
+        StayAxis const ax = stayAxis(G.p0[0], G.p1[0]);
+        StayAxis const ay = stayAxis(G.p0[1], G.p1[1]);
+        StayAxis const az = stayAxis(G.p0[2], G.p1[2]);
+        float const cq = charge * k.inv_vdt;
+        // J_x: i->y, j->z, k->x; J_y: i->z, j->x, k->y; J_z: i->x, j->y, k->z
+        cpt1DStay<PY, PZ, PX>(corner, ay, az, ax, G.p0[0], G.p1[0], cq * k.cell[0], k.jscale);
+        cpt1DStay<PZ, PX, PY>(corner + TV, az, ax, ay, G.p0[1], G.p1[1], cq * k.cell[1], k.jscale);
+        cpt1DStay<PX, PY, PZ>(corner + 2 * TV, ax, ay, az, G.p0[2], G.p1[2], cq * k.cell[2], k.jscale);
     }
 
     // Deposit one particle. (cx,cy,cz) is the particle's cell in J-tile
@@ -684,9 +776,16 @@ struct EmZ {
     template <int TX, int TY, int TV>
     __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
                                                    Vec3 vel, float charge, const DepositConsts& k) {
+        float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
+        depositDp<TX, TY, TV>(jt, cx, cy, cz, pos, dp, charge, k);
+    }
+
+    // the same with the move in cells dp = vel * dt / cell given
+    template <int TX, int TY, int TV>
+    __device__ __forceinline__ static void depositDp(unsigned long long* __restrict__ jt, int cx, int cy, int cz,
+                                                     Vec3 pos, const float* dp, float charge, const DepositConsts& k) {
         constexpr bool isEven = (SUPP % 2) == 0;
         float const p1[3] = {pos.x, pos.y, pos.z};
-        float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
         float p0[3], relay[3];
         int s0[3], s1[3];
 #pragma unroll
