@@ -380,6 +380,15 @@ int pic_field_energy(pic_ctx* ctx, double out[7]);
  * (ChargeConservation.tpp:181-260). Charge conservation keeps it constant
  * in time up to round-off. */
 int pic_gauss_residual(pic_ctx* ctx, double* out);
+/* pic_gauss_residual without any device allocation. It has no counterpart in
+ * the reference; it is for callers that cannot allocate device memory (tight
+ * monitoring loops, harnesses that forbid hipMalloc). The per-cell charge
+ * density goes to a file-scope device array of 2^21 doubles (16 MiB) that
+ * each device holds until the process ends, so grids above 2,097,152 cells
+ * are rejected with PIC_E_INVALID. Every call on one device shares that
+ * array: calls that overlap on one device race. Same output as
+ * pic_gauss_residual (FP64 atomics: equal up to FP64 round-off). */
+int pic_gauss_residual_noalloc(pic_ctx* ctx, double* out);
 /* PIC_FLAG_* bits raised since the last call */
 #define PIC_FLAG_CAPACITY_GREW 1u /* a supercell overflowed and the species' supercells grew (nothing lost) */
 int pic_error_flags(pic_ctx* ctx, uint32_t* flags);

@@ -222,6 +222,7 @@ SIGNATURES = {
     "pic_step_graph": (C.c_int, [_P, C.c_int32]),
     "pic_field_energy": (C.c_int, [_P, _D]),
     "pic_gauss_residual": (C.c_int, [_P, _D]),
+    "pic_gauss_residual_noalloc": (C.c_int, [_P, _D]),
     "pic_error_flags": (C.c_int, [_P, _U32]),
     "pic_timing_enable": (C.c_int, [_P, C.c_int32]),
     "pic_kernel_timer": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64)]),
@@ -534,6 +535,13 @@ class Simulation:
     def gauss_residual(self):
         out = np.zeros(self.ncells, np.float64)
         check(self.lib.pic_gauss_residual(self.h, ptr(out, C.c_double)), "pic_gauss_residual")
+        return out
+
+    def gauss_residual_noalloc(self):
+        """pic_gauss_residual_noalloc: the same residual with no device
+        allocation (grids up to 2^21 cells)."""
+        out = np.zeros(self.ncells, np.float64)
+        check(self.lib.pic_gauss_residual_noalloc(self.h, ptr(out, C.c_double)), "pic_gauss_residual_noalloc")
         return out
 
     def error_flags(self) -> int:

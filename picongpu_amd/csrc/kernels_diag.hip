@@ -7,6 +7,9 @@
 // charge density uses each species' ChargeAssignment B-spline evaluated in
 // FP32 per axis and multiplied in FP64; it is accumulated with FP64 atomics,
 // so the residual is reproducible to FP64 round-off (not bit for bit).
+#include <array>
+#include <mutex>
+
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 
@@ -76,6 +79,31 @@ __global__ __launch_bounds__(256) void kGaussResidual(FieldArgs a, double eps0, 
 __global__ __launch_bounds__(256) void kZeroD(double* p, size_t n) {
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+// Scratch of the allocation-free Gauss residual: one FP64 value per cell,
+// at most kNoAllocMaxCells cells. Each device holds its own copy for the
+// life of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device.
+static __device__ __align__(256) double scratch[kNoAllocMaxCells];
+
+hipError_t noAllocScratch(int device, double** out) {
+    static std::mutex lock;
+    static std::array<double*, kNoAllocMaxDevices> addr{};
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(lock);
+    if (!addr[device]) {
+        int current = 0;
+        hipError_t e = hipGetDevice(&current);
+        if (e != hipSuccess) return e;
+        if (current != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = static_cast<double*>(p);
+    }
+    *out = addr[device];
+    return hipSuccess;
 }
 
 hipError_t launchGaussResidual(const FieldArgs& f, const SpeciesDev* sp, const int* supp, int nsp, double qScale,

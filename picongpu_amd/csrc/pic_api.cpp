@@ -1842,6 +1842,32 @@ int pic_gauss_residual(pic_ctx* c, double* out) {
     return PIC_OK;
 }
 
+int pic_gauss_residual_noalloc(pic_ctx* c, double* out) {
+    if (!c || !out) return fail(PIC_E_INVALID, "null argument");
+    if (c->ncells > pic::kNoAllocMaxCells)
+        return fail(PIC_E_INVALID, "pic_gauss_residual_noalloc: the grid has " + std::to_string(c->ncells) +
+                                       " cells, more than its scratch holds (" +
+                                       std::to_string(pic::kNoAllocMaxCells) + ")");
+    PIC_HIP(hipSetDevice(c->device));
+    double* buf = nullptr;
+    PIC_HIP(pic::noAllocScratch(c->device, &buf));
+    std::array<pic::SpeciesDev, PIC_MAX_SPECIES> sp;
+    std::array<int, PIC_MAX_SPECIES> supp;
+    int nsp = 0;
+    for (const SpeciesMem& m : c->sp) {
+        sp[nsp] = m.dev;
+        supp[nsp++] = supportOf(m.desc.shape);
+    }
+    pic::FieldArgs a = makeFieldArgs(c);
+    double const qScale = static_cast<double>(c->k.inv_cell_volume_dt) * static_cast<double>(c->k.dt);
+    // the kernels clear the scratch first (it keeps the last call's values)
+    PIC_HIP(pic::launchGaussResidual(a, sp.data(), supp.data(), nsp, qScale, static_cast<double>(c->k.eps0), buf,
+                                     c->stream));
+    PIC_HIP(hipMemcpyAsync(out, buf, c->ncells * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
 int pic_error_flags(pic_ctx* c, uint32_t* flags) {
     if (!c || !flags) return fail(PIC_E_INVALID, "null argument");
     PIC_HIP(hipStreamSynchronize(c->stream));

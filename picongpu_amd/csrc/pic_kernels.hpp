@@ -143,6 +143,13 @@ hipError_t launchEnergy(const FieldArgs& a, double* partials, int nblocks, doubl
 hipError_t launchCountSum(const uint32_t* counts, int n, unsigned long long* out, hipStream_t s);
 hipError_t launchGaussResidual(const FieldArgs& f, const SpeciesDev* sp, const int* supp, int nsp, double qScale,
                                double eps0, double* buf, hipStream_t s);
+// The allocation-free Gauss residual's scratch: a file-scope device array of
+// kNoAllocMaxCells doubles (16 MiB) per device. noAllocScratch returns its
+// address on `device`, which must be the current device; the address is
+// looked up once per device and kept.
+constexpr size_t kNoAllocMaxCells = size_t(1) << 21;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t noAllocScratch(int device, double** out);
 
 struct InitArgs {
     GridDev g;
