@@ -18,10 +18,10 @@
  * Storage layout (the reference's supercell frame semantics,
  * include/pmacc/particles/memory/dataTypes/SuperCell.hpp:31-119): the box is
  * tiled by supercells of 8x8x4 cells; every supercell owns `frames_per_sc`
- * frames of 256 slots (grown for all supercells of a species when a shift or
- * a load would overfill one, so no particle is ever dropped: the counterpart
- * of the frames the reference takes on demand from its mallocMC heap,
- * include/pmacc/particles/memory/boxes/ParticlesBox.hpp:96-122);
+ * frames of 256 slots at first, and a supercell that a shift or a load would
+ * overfill grows by whole frames on its own, so no particle is ever dropped
+ * (the counterpart of the frames the reference takes on demand from its
+ * mallocMC heap, include/pmacc/particles/memory/boxes/ParticlesBox.hpp:96-122);
  * its N particles occupy slots [0,N) so all frames are
  * full except the last, which holds (N-1)%256+1. localCellIdx is the
  * x-fastest linear cell index inside the supercell; multiMask is 0 (empty),
@@ -137,9 +137,9 @@ typedef struct pic_species_desc {
     double charge_ratio;    /* relative to BASE_CHARGE_SI (electron charge) */
     int32_t shape;          /* PIC_SHAPE_* */
     int32_t pusher;         /* PIC_PUSHER_* */
-    int32_t frames_per_sc;  /* frames of 256 slots preallocated per supercell; every
-                               supercell of the species grows when one needs more
-                               (pic_slots_per_supercell reports the current size) */
+    int32_t frames_per_sc;  /* frames of 256 slots preallocated per supercell; a
+                               supercell that needs more grows on its own
+                               (pic_supercell_slots reports every capacity) */
     int32_t current_solver; /* PIC_CURRENT_SOLVER_*: the species' current<> flag */
     int32_t particle_boundary[3]; /* PIC_PARTICLE_* per axis, used on open axes */
 } pic_species_desc;
@@ -293,9 +293,16 @@ int pic_read_particles(pic_ctx* ctx, int32_t species, int64_t capacity, int64_t*
                        int32_t* cell, float* pos, float* mom, float* weighting);
 int64_t pic_particle_count(pic_ctx* ctx, int32_t species);
 int32_t pic_num_supercells(const pic_ctx* ctx);
+/* The largest capacity (slots) of a supercell of the species: the number of
+ * slots per supercell of the frame view pic_read_frames/pic_write_frames use.
+ * Supercells grow one by one (by whole frames) when they need more slots. */
 int32_t pic_slots_per_supercell(const pic_ctx* ctx, int32_t species);
-/* Raw frame pool: per-supercell counts plus the slot arrays (nsc*slots each).
- * pos/mom are 3 planes of nsc*slots floats. */
+/* The capacity (slots, a multiple of 256) of every supercell: out[nsc]. */
+int pic_supercell_slots(const pic_ctx* ctx, int32_t species, uint32_t* out);
+/* Frame view of the pool: per-supercell counts plus the slot arrays of
+ * nsc*slots entries each, slots = pic_slots_per_supercell at the call (a
+ * supercell with fewer slots reads zero beyond them). pos/mom are 3 planes of
+ * nsc*slots floats. */
 int pic_read_frames(pic_ctx* ctx, int32_t species, uint32_t* counts, float* pos, float* mom,
                     float* weighting, uint16_t* local_cell, uint8_t* multi_mask);
 int pic_write_frames(pic_ctx* ctx, int32_t species, const uint32_t* counts, const float* pos,

@@ -200,6 +200,7 @@ SIGNATURES = {
     "pic_particle_count": (C.c_int64, [_P, C.c_int32]),
     "pic_num_supercells": (C.c_int32, [_P]),
     "pic_slots_per_supercell": (C.c_int32, [_P, C.c_int32]),
+    "pic_supercell_slots": (C.c_int, [_P, C.c_int32, _U32]),
     "pic_read_frames": (C.c_int, [_P, C.c_int32, _U32, _F, _F, _F, _U16, _U8]),
     "pic_write_frames": (C.c_int, [_P, C.c_int32, _U32, _F, _F, _F, _U16, _U8]),
     "pic_init_species": (C.c_int, [_P, C.c_int32, C.POINTER(InitDesc)]),
@@ -380,6 +381,12 @@ class Simulation:
 
     def slots(self, sp: int) -> int:
         return self.lib.pic_slots_per_supercell(self.h, sp)
+
+    def supercell_slots(self, sp: int) -> np.ndarray:
+        """Capacity (slots) of every supercell of the species."""
+        out = np.zeros(self.nsc, np.uint32)
+        check(self.lib.pic_supercell_slots(self.h, sp, ptr(out, C.c_uint32)), "pic_supercell_slots")
+        return out
 
     def load_particles(self, sp, cell, pos, mom, w):
         cell = np.ascontiguousarray(cell, dtype=np.int32)

@@ -22,7 +22,7 @@ __global__ __launch_bounds__(256) void kChargeDensity(SpeciesDev S, GridDev g, d
     int const sy = (sc / g.nscx) % g.nscy;
     int const sz = sc / (g.nscx * g.nscy);
     int const n = (int)S.count[sc];
-    size_t const base = (size_t)sc * (size_t)S.cap;
+    size_t const base = slotBase(S, sc);
     constexpr int begin = Shape<SUPP>::begin;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         size_t const gi = base + i;

@@ -27,7 +27,8 @@ __global__ __launch_bounds__(256) void kInit(InitArgs a) {
     int const sx = sc % a.g.nscx;
     int const sy = (sc / a.g.nscx) % a.g.nscy;
     int const sz = sc / (a.g.nscx * a.g.nscy);
-    size_t const base = (size_t)sc * (size_t)a.sp.cap;
+    size_t const base = slotBase(a.sp, sc);
+    int const cap = (int)slotCap(a.sp, sc);
     // cells with a positive density, ranked in cell order (256 threads = 256 cells)
     int nActive;
     {
@@ -49,7 +50,7 @@ __global__ __launch_bounds__(256) void kInit(InitArgs a) {
     }
     // slot = k * nActive + rank (round-robin over the populated cells)
     int const n = a.ppc * nActive;
-    for (int i = threadIdx.x; i < a.sp.cap; i += blockDim.x) {
+    for (int i = threadIdx.x; i < cap; i += blockDim.x) {
         size_t const g = base + i;
         if (i >= n) {
             a.sp.mask[g] = 0;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(256) void kInit(InitArgs a) {
         a.sp.cell[g] = (uint16_t)lc;
         a.sp.mask[g] = 1;
     }
-    if (threadIdx.x == 0) a.sp.count[sc] = (uint32_t)(n < a.sp.cap ? n : a.sp.cap);
+    if (threadIdx.x == 0) a.sp.count[sc] = (uint32_t)(n < cap ? n : cap);
 }
 
 // Gaussian pulse added to E and B at the Yee positions (YeeCell.hpp:69-123)

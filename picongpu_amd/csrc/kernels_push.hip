@@ -77,13 +77,12 @@ __device__ __forceinline__ void queueSlow(bool slow, const SlowItem& mine, SlowI
     P += k;
 }
 
-template <int SUPP, int TX, int TY, int TV>
+template <int SUPP, class TG>
 __device__ __forceinline__ void depositQueued(unsigned long long* __restrict__ sJ, const SlowItem& t,
                                               const DepositConsts& dk) {
     float const dp[3] = {t.dx, t.dy, t.dz};
-    Esirkepov<SUPP>::template depositGeometry<TX, TY, TV, true>(sJ, t.cell & 255, (t.cell >> 8) & 255, t.cell >> 16,
-                                                                Esirkepov<SUPP>::geometryDp(Vec3{t.x, t.y, t.z}, dp),
-                                                                t.q, dk);
+    Esirkepov<SUPP>::template depositGeometry<TG, true>(sJ, t.cell & 255, (t.cell >> 8) & 255, t.cell >> 16,
+                                                        Esirkepov<SUPP>::geometryDp(Vec3{t.x, t.y, t.z}, dp), t.q, dk);
 }
 
 template <int SUPP, int MODE>
@@ -91,11 +90,9 @@ struct PushLds {
     static constexpr int GLO = SUPP / 2;
     static constexpr int GUP = (SUPP + 1) / 2;
     static constexpr int GV = (SCX + GLO + GUP) * (SCY + GLO + GUP) * (SCZ + GLO + GUP);
-    static constexpr int DLO = Esirkepov<SUPP>::lowerMargin;
-    static constexpr int DUP = Esirkepov<SUPP>::upperMargin;
-    static constexpr int JV = (SCX + DLO + DUP) * (SCY + DLO + DUP) * (SCZ + DLO + DUP);
+    using TG = JTileGeom<Esirkepov<SUPP>::lowerMargin, Esirkepov<SUPP>::upperMargin>;
     static constexpr int EB_ELEMS = MODE == 2 ? 1 : 6 * GV;
-    static constexpr int J_ELEMS = MODE == 0 ? 1 : 3 * JV;
+    static constexpr int J_ELEMS = MODE == 0 ? 1 : TG::total();
     static constexpr int EB_BYTES = ((EB_ELEMS * 4 + 15) / 16) * 16;
     static constexpr int EPI_BYTES =
         (kPushMaxIters * kPushWaves + 1) * 4 + kPushWaves * kNumDirs * 4 + (int)sizeof(GroupScratch);
@@ -119,11 +116,9 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     // path starts inside the old cell and moves less than one cell per axis
     // (the host requires float(c*dt)/cell < 1).
     constexpr int DLO = Esirkepov<SUPP>::lowerMargin;
-    constexpr int DUP = Esirkepov<SUPP>::upperMargin;
-    constexpr int JX = SCX + DLO + DUP, JY = SCY + DLO + DUP, JZ = SCZ + DLO + DUP;
-    constexpr int JV = JX * JY * JZ;
+    using TG = JTileGeom<DLO, Esirkepov<SUPP>::upperMargin>;
     constexpr int EB_ELEMS = MODE == 2 ? 1 : 6 * GV;
-    constexpr int J_ELEMS = MODE == 0 ? 1 : 3 * JV;
+    constexpr int J_ELEMS = MODE == 0 ? 1 : TG::total();
 
     // One LDS buffer: the E/B tile, then the J tile. After the particle loop
     // the E/B tile is dead and its space holds the epilogue's scratch.
@@ -179,7 +174,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             stageEB(std::false_type{});
     }
     if (MODE != 0) {
-        for (int t = tid; t < 3 * JV; t += blockDim.x) sJ[t] = 0ull;
+        for (int t = tid; t < J_ELEMS; t += blockDim.x) sJ[t] = 0ull;
     }
     __syncthreads();
 
@@ -204,7 +199,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     for (int s = 0; s < a.nsp; ++s) {
         const SpeciesDev& S = a.sp[s];
         int const n = (int)S.count[sc];
-        size_t const base = (size_t)sc * (size_t)S.cap;
+        size_t const base = slotBase(S, sc);
         uint32_t leaveBits = 0u;  // bit `it`: the particle of iteration `it` left the supercell
         int const iters = (n + (int)blockDim.x - 1) / (int)blockDim.x;  // uniform across the workgroup
         // the last species runs one iteration more, without particles, that drains the queue
@@ -262,30 +257,30 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                         int const cx = lx + dir[0] + DLO, cy = ly + dir[1] + DLO, cz = lz + dir[2] + DLO;
                         Vec3 const np{p[0], p[1], p[2]};
                         if (EMZ) {
-                            EmZ<SUPP>::template depositDp<JX, JY, JV>(sJ, cx, cy, cz, np, dp, charge, dk);
+                            EmZ<SUPP>::template depositDp<TG>(sJ, cx, cy, cz, np, dp, charge, dk);
                         } else {
                             auto const G = Esirkepov<SUPP>::geometryDp(np, dp);
                             if (Esirkepov<SUPP>::changesCell(G)) {
                                 slow = true;
                                 mine = SlowItem{cx | (cy << 8) | (cz << 16), np.x, np.y, np.z, dp[0], dp[1], dp[2], charge};
                             } else {
-                                Esirkepov<SUPP>::template depositStay<JX, JY, JV>(sJ, cx, cy, cz, G, charge, dk);
+                                Esirkepov<SUPP>::template depositStay<TG>(sJ, cx, cy, cz, G, charge, dk);
                             }
                         }
                     }
                 } else {
                     Vec3 const vel = velocityOf(mom, mass, a.rc2);
                     if (EMZ)
-                        EmZ<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge, dk);
+                        EmZ<SUPP>::template deposit<TG>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge, dk);
                     else
-                        Esirkepov<SUPP>::template deposit<JX, JY, JV>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge,
+                        Esirkepov<SUPP>::template deposit<TG>(sJ, lx + DLO, ly + DLO, lz + DLO, pos, vel, charge,
                                                                       dk);
                 }
             }
             if (MODE == 1 && !EMZ) {
                 queueSlow(slow, mine, qA, qB, queued);
                 if (queued >= 64 || (it == iters && queued > 0)) {
-                    if ((int)(tid & 63) < queued) depositQueued<SUPP, JX, JY, JV>(sJ, qA, dk);
+                    if ((int)(tid & 63) < queued) depositQueued<SUPP, TG>(sJ, qA, dk);
                     qA = qB;
                     queued = queued > 64 ? queued - 64 : 0;
                 }
@@ -296,14 +291,16 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     __syncthreads();
 
     auto flushJ = [&](auto openBorders) {
-        for (int t = tid; t < 3 * JV; t += blockDim.x) {
-            long long const acc = (long long)sJ[t];
+        for (int t = tid; t < J_ELEMS; t += blockDim.x) {
+            long long const acc = fixedTileValue(sJ[t]);
             if (acc == 0) continue;
-            int const c = t / JV;
-            int const r = t - c * JV;
-            int const tx = r % JX;
-            int const ty = (r / JX) % JY;
-            int const tz = r / (JX * JY);
+            int const c = t < TG::off(1) ? 0 : (t < TG::off(2) ? 1 : 2);
+            int const r = t - (c == 0 ? 0 : TG::off(c));
+            int const ex = c == 0 ? TG::ext(0, 0) : (c == 1 ? TG::ext(1, 0) : TG::ext(2, 0));
+            int const ey = c == 0 ? TG::ext(0, 1) : (c == 1 ? TG::ext(1, 1) : TG::ext(2, 1));
+            int const tx = r % ex;
+            int const ty = (r / ex) % ey;
+            int const tz = r / (ex * ey);
             if (decltype(openBorders)::value && outsideOpen(a.g, ox - DLO + tx, oy - DLO + ty, oz - DLO + tz))
                 continue;  // dropped beyond the border
             int const gx = wrapIdx(ox - DLO + tx, a.g.nx);
@@ -350,7 +347,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             }
             __syncthreads();
             int const L = (int)sPre[kPushMaxIters * kPushWaves];
-            size_t const base = (size_t)sc * (size_t)S.cap;
+            size_t const base = slotBase(S, sc);
             for (int it = 0; it < iters; ++it) {
                 bool const left = (bits >> it) & 1u;
                 unsigned long long const m = __ballot(left);
@@ -365,8 +362,11 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     }
 }
 
+// PQS: at most 128 VGPRs, so two workgroups share a CU (their LDS fits);
+// the compiler spills a few registers for it and the launch takes 0.63x
+// the time of one workgroup per CU
 template <int SUPP, int PUSHER, int MODE, bool EMZ = false>
-__global__ __launch_bounds__(kPushThreads) void kPushDeposit(PushArgs a) {
+__global__ __launch_bounds__(kPushThreads, SUPP == 4 ? 4 : 1) void kPushDeposit(PushArgs a) {
     __shared__ __align__(16) unsigned char smem[PushLds<SUPP, MODE>::BYTES];
     pushDepositSC<SUPP, PUSHER, MODE, EMZ>(a, a.scList[blockIdx.x], smem);
 }

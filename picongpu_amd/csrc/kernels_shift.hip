@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kThreads) void kCollect(SpeciesDev S, GridDev grid)
     int const sc = blockIdx.x;
     int const tid = threadIdx.x;
     int const n = (int)S.count[sc];
-    size_t const base = (size_t)sc * (size_t)S.cap;
+    size_t const base = slotBase(S, sc);
     int run = 0;
     for (int c0 = 0; c0 < n; c0 += kThreads) {
         int const i = c0 + tid;
@@ -105,7 +105,8 @@ __device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, 
     uint32_t const start = incl - cnt;  // exclusive prefix: first incoming index of this code
     uint32_t const total = __shfl(incl, 63);
     uint32_t const n = redo ? S.count[d] - total : S.count[d];
-    size_t const dbase = (size_t)d * (size_t)S.cap;
+    size_t const dbase = slotBase(S, d);
+    uint32_t const dcap = slotCap(S, d);
     // Wave-uniform loop: every lane runs the shuffles of every round (a
     // shuffle reads nothing valid from lanes that have left a divergent
     // loop); only the copy is predicated on k < total.
@@ -130,11 +131,11 @@ __device__ __forceinline__ void insertSC(const SpeciesDev& S, const GridDev& g, 
         int const sc = __shfl(src, code);
         if (!valid) continue;
         uint32_t const slot = n + k;
-        if (slot >= (uint32_t)S.cap) {
+        if (slot >= dcap) {
             if (g.flags) atomicOr(&g.flags->overflow, 1u);
             continue;
         }
-        size_t const rec = (size_t)sc * (size_t)S.cap + bg + (k - st);
+        size_t const rec = slotBase(S, sc) + bg + (k - st);
         float4 const r0 = S.outbox[2 * rec];
         float4 const r1 = S.outbox[2 * rec + 1];
         size_t const dst = dbase + slot;
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void kInsert(SpeciesDev S, GridDev g, int
 
 __global__ __launch_bounds__(256) void kClampCounts(SpeciesDev S, int nsc) {
     int const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nsc && S.count[i] > (uint32_t)S.cap) S.count[i] = (uint32_t)S.cap;
+    if (i < nsc && S.count[i] > slotCap(S, i)) S.count[i] = slotCap(S, i);
 }
 
 // Batched forms for ensembles: work items (table entry, supercell), one wave
@@ -210,23 +211,23 @@ hipError_t launchInsertRedo(const GridDev& g, const SpeciesDev& sp, hipStream_t 
 // Rows of oldWords 16-byte words from src (row pitch oldWords) to dst (row
 // pitch newWords >= oldWords): the slots of every supercell keep their index
 // when the supercells grow.
-__global__ __launch_bounds__(256) void kRowCopy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t oldWords,
-                                                size_t newWords, size_t rows) {
-    size_t const total = oldWords * rows;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        size_t const r = t / oldWords, c = t - r * oldWords;
-        dst[r * newWords + c] = src[t];
-    }
+// One workgroup per listed run: `slots` slots from slot `from` to slot `to`
+// (16-byte words: runs are whole frames of slots, so multiples of 16 bytes
+// for every attribute size).
+__global__ __launch_bounds__(256) void kMoveRuns(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                 const RunMove* __restrict__ moves, uint32_t bytesPerSlot) {
+    RunMove const m = moves[blockIdx.x];
+    size_t const words = (size_t)m.slots * bytesPerSlot / 16;
+    const uint4* from = src + (size_t)m.from * bytesPerSlot / 16;
+    uint4* to = dst + (size_t)m.to * bytesPerSlot / 16;
+    for (size_t t = threadIdx.x; t < words; t += blockDim.x) to[t] = from[t];
 }
 
-hipError_t launchRowCopy(void* dst, const void* src, size_t oldRowBytes, size_t newRowBytes, size_t rows,
-                         hipStream_t s) {
-    if (oldRowBytes % 16 != 0 || newRowBytes % 16 != 0 || newRowBytes < oldRowBytes) return hipErrorInvalidValue;
-    size_t const total = oldRowBytes / 16 * rows;
-    if (total == 0) return hipSuccess;
-    size_t const blocks = std::min<size_t>((total + 255) / 256, 65536);
-    hipLaunchKernelGGL(kRowCopy, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<const uint4*>(src),
-                       static_cast<uint4*>(dst), oldRowBytes / 16, newRowBytes / 16, rows);
+hipError_t launchMoveRuns(void* dst, const void* src, uint32_t bytesPerSlot, const RunMove* moves, int n,
+                          hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kMoveRuns, dim3((unsigned)n), dim3(256), 0, s, static_cast<const uint4*>(src),
+                       static_cast<uint4*>(dst), moves, bytesPerSlot);
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <array>
+#include <numeric>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -571,6 +572,17 @@ struct PhasePlan {
 
 struct SpeciesMem {
     pic_species_desc desc{};
+    // supercell sc owns the run of slots [base[sc], base[sc] + capSc[sc])
+    // (whole frames) of every per-slot array, a pool of `slots` slots whose
+    // runs end at `used`; a growing supercell gets a new run at the end. `cap`
+    // is the largest run: the pitch of the frame view pic_read_frames and
+    // pic_write_frames hand across the ABI.
+    std::vector<uint32_t> capSc;
+    std::vector<uint64_t> base;
+    uint64_t* dBase = nullptr;
+    uint32_t* dCap = nullptr;
+    size_t slots = 0;
+    size_t used = 0;
     int cap = 0;
     float mass = 0.0f;
     float charge = 0.0f;
@@ -676,13 +688,15 @@ struct pic_ctx {
 namespace {
 
 template <typename T>
-int devAlloc(pic_ctx* c, T** p, size_t n) {
+int devAlloc(pic_ctx* c, T** p, size_t n, bool zero = true) {
     void* q = nullptr;
     size_t const bytes = std::max<size_t>(n * sizeof(T), 16);
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return fail(PIC_E_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-    e = hipMemsetAsync(q, 0, bytes, c->stream);
-    if (e != hipSuccess) return fail(PIC_E_HIP, std::string("hipMemsetAsync failed: ") + hipGetErrorString(e));
+    if (zero) {
+        e = hipMemsetAsync(q, 0, bytes, c->stream);
+        if (e != hipSuccess) return fail(PIC_E_HIP, std::string("hipMemsetAsync failed: ") + hipGetErrorString(e));
+    }
     c->allocations.push_back(q);
     *p = static_cast<T*>(q);
     return PIC_OK;
@@ -808,10 +822,13 @@ void updateJexp(pic_ctx* c) {
         bound = std::max(bound, b);
     }
     if (!(bound > 0.0)) {
-        c->jexp = 40;
+        c->jexp = 30;
         return;
     }
-    int const e = 40 - static_cast<int>(std::ceil(std::log2(bound)));
+    // one contribution below 2^30: a tile element's sum over the particles of
+    // a supercell (at most 4 species x 255 frames, two values per particle)
+    // stays below 2^49, which the tile's decoding needs (below 2^50)
+    int const e = 30 - static_cast<int>(std::ceil(std::log2(bound)));
     c->jexp = std::max(-60, std::min(100, e));
 }
 
@@ -966,65 +983,176 @@ void devFree(pic_ctx* c, void* p) {
     (void)hipFree(p);
 }
 
-// Grows every supercell of species s to `frames` frames. Each attribute stays
-// one block of nsc*slots entries, so a supercell's slots stay contiguous and
-// keep their index; the new slots are empty (multiMask 0). The outbox moves
-// along, so records an overflowing insert left there can still be placed.
-// (The reference takes single frames from its mallocMC heap instead,
-// ParticlesBox.hpp:96-122; a supercell here grows by whole frames at once.)
-int growSpecies(pic_ctx* c, int s, int frames) {
+int grownFrames(int frames, uint32_t count, int maxFrames);
+
+// Contiguous runs of the given lengths in supercell order, in a pool of
+// `pool` slots: the host tables (base, capSc, used, cap; slots = pool).
+void contiguousRuns(SpeciesMem& m, const std::vector<uint32_t>& caps, size_t pool) {
+    m.capSc = caps;
+    m.base.assign(caps.size(), 0);
+    uint64_t at = 0;
+    for (size_t i = 0; i < caps.size(); ++i) {
+        m.base[i] = at;
+        at += caps[i];
+    }
+    m.used = static_cast<size_t>(at);
+    m.slots = pool;
+    m.cap = static_cast<int>(*std::max_element(caps.begin(), caps.end()));
+}
+
+// the run tables to the device
+int uploadRuns(pic_ctx* c, SpeciesMem& m) {
+    PIC_HIP(hipMemcpyAsync(m.dBase, m.base.data(), m.base.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                           c->stream));
+    PIC_HIP(hipMemcpyAsync(m.dCap, m.capSc.data(), m.capSc.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           c->stream));
+    return PIC_OK;
+}
+
+// Sets the slot layout of a new species: contiguous runs of `caps` slots.
+int setLayout(pic_ctx* c, SpeciesMem& m, const std::vector<uint32_t>& caps) {
+    uint64_t const total = std::accumulate(caps.begin(), caps.end(), uint64_t(0));
+    // an eighth more slots than the runs: room for the first supercells to grow
+    contiguousRuns(m, caps, static_cast<size_t>(total + total / 8));
+    if (devAlloc(c, &m.dBase, caps.size()) != PIC_OK || devAlloc(c, &m.dCap, caps.size()) != PIC_OK)
+        return PIC_E_HIP;
+    return uploadRuns(c, m);
+}
+
+// Grows the supercells of species s to the run lengths `caps` (slots, at
+// least the present ones), as the reference takes further frames from its
+// heap for a supercell that needs them (ParticlesBox.hpp:96-122). A growing
+// supercell's run moves, slot order kept, to a new run at the end of the
+// pool; its old run is left unused. The outbox moves along, so records an
+// overflowing insert left there can still be placed. The arrays stay where
+// they are, so captured steps stay valid (they read the run tables, which
+// change in place). Only when the pool is full are the arrays reallocated
+// with every run packed and a quarter more slots (the captured steps are
+// then dropped).
+int growSpecies(pic_ctx* c, int s, const std::vector<uint32_t>& caps) {
     SpeciesMem& m = c->sp[s];
-    int const oldCap = m.cap, newCap = frames * pic::kSCV;
-    if (newCap <= oldCap) return PIC_OK;
-    size_t const slots = static_cast<size_t>(c->nsc) * newCap;
-    pic::SpeciesDev const& o = m.dev;
-    pic::SpeciesDev n = o;
+    std::vector<pic::RunMove> moves;
+    uint64_t grow = 0;
+    for (int i = 0; i < c->nsc; ++i) {
+        if (caps[i] < m.capSc[i]) return fail(PIC_E_INVALID, "a supercell cannot shrink");
+        if (caps[i] > m.capSc[i]) grow += caps[i];
+    }
+    if (grow == 0) return PIC_OK;
+    pic::SpeciesDev& d = m.dev;
+    int r = PIC_OK;
+    pic::RunMove* dMoves = nullptr;
+    auto moveAll = [&](pic::SpeciesDev& to, const pic::SpeciesDev& from) -> int {
+        if (devAlloc(c, &dMoves, moves.size(), false) != PIC_OK) return PIC_E_HIP;
+        PIC_HIP(hipMemcpyAsync(dMoves, moves.data(), moves.size() * sizeof(pic::RunMove), hipMemcpyHostToDevice,
+                               c->stream));
+        int const n = static_cast<int>(moves.size());
+        for (int a = 0; a < 3; ++a) {
+            PIC_HIP(pic::launchMoveRuns(to.pos[a], from.pos[a], 4, dMoves, n, c->stream));
+            PIC_HIP(pic::launchMoveRuns(to.mom[a], from.mom[a], 4, dMoves, n, c->stream));
+        }
+        PIC_HIP(pic::launchMoveRuns(to.w, from.w, 4, dMoves, n, c->stream));
+        PIC_HIP(pic::launchMoveRuns(to.cell, from.cell, 2, dMoves, n, c->stream));
+        PIC_HIP(pic::launchMoveRuns(to.mask, from.mask, 1, dMoves, n, c->stream));
+        PIC_HIP(pic::launchMoveRuns(to.outbox, from.outbox, 32, dMoves, n, c->stream));
+        PIC_HIP(hipStreamSynchronize(c->stream));
+        devFree(c, dMoves);
+        return PIC_OK;
+    };
+    if (m.used + grow <= m.slots) {
+        // new runs at the end of the pool, in supercell order
+        for (int i = 0; i < c->nsc; ++i) {
+            if (caps[i] == m.capSc[i]) continue;
+            moves.push_back(pic::RunMove{m.base[i], m.used, m.capSc[i], 0});
+            m.base[i] = m.used;
+            m.used += caps[i];
+        }
+        if ((r = moveAll(d, d)) != PIC_OK) return r;
+        m.capSc = caps;
+        m.cap = static_cast<int>(*std::max_element(caps.begin(), caps.end()));
+        m.desc.frames_per_sc = m.cap / pic::kSCV;
+        c->cfg.species[s].frames_per_sc = m.desc.frames_per_sc;
+        d.cap = m.cap;
+        int const lists = m.cap <= pic::kListMaxSlots ? 1 : 0;
+        if (lists != d.lists) {  // captured steps hold the old flag
+            d.lists = lists;
+            dropGraphs(c);
+        }
+        return uploadRuns(c, m);
+    }
+    // repack into fresh arrays with a quarter more slots
+    uint64_t const total = std::accumulate(caps.begin(), caps.end(), uint64_t(0));
+    size_t const slots = static_cast<size_t>(total + total / 4);
+    pic::SpeciesDev n = d;
     std::vector<void*> fresh;
-    auto alloc = [&](auto** p, size_t count) {
-        if (devAlloc(c, p, count) != PIC_OK) return false;
+    // the kernels read a supercell's slots below its count and the lists
+    // they wrote themselves; only the multiMask of new slots is cleared
+    auto alloc = [&](auto** p, size_t count, bool zero) {
+        if (devAlloc(c, p, count, zero) != PIC_OK) return false;
         fresh.push_back(*p);
         return true;
     };
     bool ok = true;
-    for (int a = 0; a < 3 && ok; ++a) ok = alloc(&n.pos[a], slots) && alloc(&n.mom[a], slots);
-    ok = ok && alloc(&n.w, slots) && alloc(&n.cell, slots) && alloc(&n.mask, slots) && alloc(&n.outbox, 2 * slots) &&
-         alloc(&n.outIdx, slots) && alloc(&n.scratch, slots);
+    for (int a = 0; a < 3 && ok; ++a) ok = alloc(&n.pos[a], slots, false) && alloc(&n.mom[a], slots, false);
+    ok = ok && alloc(&n.w, slots, false) && alloc(&n.cell, slots, false) && alloc(&n.mask, slots, true) &&
+         alloc(&n.outbox, 2 * slots, false) && alloc(&n.outIdx, slots, false) && alloc(&n.scratch, slots, false);
     if (!ok) {
         for (void* p : fresh) devFree(c, p);
         return fail(PIC_E_CAPACITY, "supercells of species " + std::to_string(s) + " cannot grow to " +
-                                        std::to_string(frames) + " frames: " + g_err);
+                                        std::to_string(slots) + " slots: " + g_err);
     }
-    size_t const rows = static_cast<size_t>(c->nsc);
-    auto move = [&](void* dst, const void* src, size_t bytesPerSlot) {
-        return pic::launchRowCopy(dst, src, oldCap * bytesPerSlot, newCap * bytesPerSlot, rows, c->stream);
-    };
+    std::vector<uint64_t> const oldBase = m.base;
+    std::vector<uint32_t> const oldCap = m.capSc;
+    contiguousRuns(m, caps, slots);
+    for (int i = 0; i < c->nsc; ++i) moves.push_back(pic::RunMove{oldBase[i], m.base[i], oldCap[i], 0});
+    if ((r = moveAll(n, d)) != PIC_OK) return r;
     for (int a = 0; a < 3; ++a) {
-        PIC_HIP(move(n.pos[a], o.pos[a], 4));
-        PIC_HIP(move(n.mom[a], o.mom[a], 4));
+        devFree(c, d.pos[a]);
+        devFree(c, d.mom[a]);
     }
-    PIC_HIP(move(n.w, o.w, 4));
-    PIC_HIP(move(n.cell, o.cell, 2));
-    PIC_HIP(move(n.mask, o.mask, 1));
-    PIC_HIP(move(n.outbox, o.outbox, 32));
-    PIC_HIP(hipStreamSynchronize(c->stream));
-    for (int a = 0; a < 3; ++a) {
-        devFree(c, o.pos[a]);
-        devFree(c, o.mom[a]);
+    devFree(c, d.w);
+    devFree(c, d.cell);
+    devFree(c, d.mask);
+    devFree(c, d.outbox);
+    devFree(c, d.outIdx);
+    devFree(c, d.scratch);
+    n.cap = m.cap;
+    n.lists = m.cap <= pic::kListMaxSlots ? 1 : 0;
+    d = n;
+    m.desc.frames_per_sc = m.cap / pic::kSCV;
+    c->cfg.species[s].frames_per_sc = m.desc.frames_per_sc;
+    dropGraphs(c);  // captured steps hold the old arrays
+    return uploadRuns(c, m);
+}
+
+// capacities with every supercell that must hold more than its slots
+// (`need`) grown to grownFrames; false when nothing changes
+bool grownCaps(const SpeciesMem& m, const std::vector<uint32_t>& need, int maxFrames, std::vector<uint32_t>& caps) {
+    caps = m.capSc;
+    bool any = false;
+    for (size_t i = 0; i < caps.size(); ++i) {
+        if (need[i] <= caps[i]) continue;
+        caps[i] = static_cast<uint32_t>(grownFrames(static_cast<int>(caps[i] / pic::kSCV), need[i], maxFrames)) *
+                  pic::kSCV;
+        any = true;
     }
-    devFree(c, o.w);
-    devFree(c, o.cell);
-    devFree(c, o.mask);
-    devFree(c, o.outbox);
-    devFree(c, o.outIdx);
-    devFree(c, o.scratch);
-    n.cap = newCap;
-    n.lists = newCap <= pic::kListMaxSlots ? 1 : 0;
-    m.dev = n;
-    m.cap = newCap;
-    m.desc.frames_per_sc = frames;
-    c->cfg.species[s].frames_per_sc = frames;
-    dropGraphs(c);  // captured steps hold the old buffers
-    return PIC_OK;
+    return any;
+}
+
+// the most frames a supercell of a species may have (the configuration's
+// bound); the current tile's fixed-point range assumes it
+constexpr int kMaxFrames = 255;
+
+// grows the supercells of species s that must hold more than their slots
+// (`need` per supercell), at most to maxFrames frames
+int growFor(pic_ctx* c, int s, const std::vector<uint32_t>& need, int maxFrames) {
+    std::vector<uint32_t> caps;
+    if (!grownCaps(c->sp[s], need, maxFrames, caps)) return PIC_OK;
+    for (size_t i = 0; i < caps.size(); ++i)
+        if (need[i] > caps[i])
+            return fail(PIC_E_CAPACITY, "a supercell of species " + std::to_string(s) + " would need " +
+                                            std::to_string(need[i]) + " slots, more than " +
+                                            std::to_string(maxFrames) + " frames");
+    return growSpecies(c, s, caps);
 }
 
 // frames for a supercell that must hold `count` particles: geometric growth
@@ -1059,14 +1187,25 @@ int recoverCapacity(pic_ctx* c, int* skipped, int maxFrames) {
         if (m.desc.pusher == PIC_PUSHER_NONE) continue;
         PIC_HIP(hipMemcpyAsync(counts.data(), m.dev.count, c->nsc * 4, hipMemcpyDeviceToHost, c->stream));
         PIC_HIP(hipStreamSynchronize(c->stream));
-        uint32_t const mx = *std::max_element(counts.begin(), counts.end());
-        if (mx <= static_cast<uint32_t>(m.cap)) continue;
-        int const frames = grownFrames(m.desc.frames_per_sc, mx, maxFrames);
-        if (static_cast<uint32_t>(frames) * pic::kSCV < mx)
-            r = fail(PIC_E_CAPACITY, "a supercell of species " + std::to_string(s) + " needs " + std::to_string(mx) +
+        // supercells already filled to 7/8 grow along with the overflowing
+        // ones: the next overflows (a moving compression front) come later
+        std::vector<uint32_t> need(counts);
+        bool over = false;
+        for (int i = 0; i < c->nsc; ++i) {
+            over = over || counts[i] > m.capSc[i];
+            if (counts[i] <= m.capSc[i] && counts[i] * 8ull >= m.capSc[i] * 7ull) need[i] = m.capSc[i] + 1;
+        }
+        std::vector<uint32_t> caps;
+        if (!over || !grownCaps(m, need, maxFrames, caps)) continue;
+        for (int i = 0; i < c->nsc; ++i) caps[i] = std::max(caps[i], m.capSc[i]);
+        uint32_t bad = 0;
+        for (int i = 0; i < c->nsc; ++i)
+            if (counts[i] > caps[i]) bad = std::max(bad, counts[i]);
+        if (bad)
+            r = fail(PIC_E_CAPACITY, "a supercell of species " + std::to_string(s) + " needs " + std::to_string(bad) +
                                          " slots, more than the " + std::to_string(maxFrames) + " frames allowed here");
         else
-            r = growSpecies(c, s, frames);
+            r = growSpecies(c, s, caps);
         if (r != PIC_OK) {
             // particles beyond the capacity are lost: keep the kernels in bounds
             for (SpeciesMem& q : c->sp)
@@ -1083,7 +1222,6 @@ int recoverCapacity(pic_ctx* c, int* skipped, int maxFrames) {
         for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
         c->eParity ^= 1;
     }
-    dropGraphs(c);
     *skipped = static_cast<int>(f.skipped);
     return r;
 }
@@ -1186,10 +1324,12 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
     for (int s = 0; s < cfg->n_species; ++s) {
         SpeciesMem m;
         m.desc = cfg->species[s];
-        m.cap = cfg->species[s].frames_per_sc * pic::kSCV;
         m.mass = c->k.species_mass[s];
         m.charge = c->k.species_charge[s];
-        size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+        if (setLayout(c, m, std::vector<uint32_t>(c->nsc, static_cast<uint32_t>(cfg->species[s].frames_per_sc) *
+                                                             pic::kSCV)) != PIC_OK)
+            return bail(PIC_E_HIP);
+        size_t const slots = m.slots;
         pic::SpeciesDev& d = m.dev;
         for (int a = 0; a < 3; ++a) {
             if (devAlloc(c, &d.pos[a], slots) || devAlloc(c, &d.mom[a], slots)) return bail(PIC_E_HIP);
@@ -1199,6 +1339,8 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
             devAlloc(c, &d.scratch, slots) || devAlloc(c, &d.outOff, static_cast<size_t>(c->nsc) * pic::kOffStride) ||
             devAlloc(c, &d.leaveBits, static_cast<size_t>(c->nsc) * pic::kPushThreadsPerBlock))
             return bail(PIC_E_HIP);
+        d.base = m.dBase;
+        d.capOf = m.dCap;
         d.cap = m.cap;
         d.lists = m.cap <= pic::kListMaxSlots ? 1 : 0;
         d.reflect = 0;
@@ -1253,6 +1395,13 @@ int32_t pic_slots_per_supercell(const pic_ctx* c, int32_t s) {
     return c->sp[s].cap;
 }
 
+int pic_supercell_slots(const pic_ctx* c, int32_t s, uint32_t* out) {
+    if (!c || !out) return fail(PIC_E_INVALID, "null argument");
+    if (s < 0 || s >= static_cast<int>(c->sp.size())) return fail(PIC_E_INVALID, "species index out of range");
+    std::copy(c->sp[s].capSc.begin(), c->sp[s].capSc.end(), out);
+    return PIC_OK;
+}
+
 int32_t pic_j_scale_exp(const pic_ctx* c) { return c ? c->jexp : 0; }
 
 int32_t pic_num_push_phases(const pic_ctx* c, int32_t fused) {
@@ -1289,12 +1438,10 @@ int pic_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* cell, co
         counts[sc] += 1;
     }
     // supercells that receive more particles than they hold grow first
-    uint32_t const mx = n > 0 ? *std::max_element(counts.begin(), counts.end()) : 0u;
-    if (mx > static_cast<uint32_t>(m.cap)) {
-        r = growSpecies(c, s, grownFrames(m.desc.frames_per_sc, mx, 0));
-        if (r != PIC_OK) return r;
+    {
+        if ((r = growFor(c, s, counts, kMaxFrames)) != PIC_OK) return r;
     }
-    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    size_t const slots = m.slots;
     std::vector<float> hp[3], hm[3];
     for (int a = 0; a < 3; ++a) {
         hp[a].assign(slots, 0.0f);
@@ -1309,7 +1456,7 @@ int pic_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* cell, co
         int const x = cell[3 * i], y = cell[3 * i + 1], z = cell[3 * i + 2];
         int const sc = x / pic::kSCX + c->g.nscx * (y / pic::kSCY + c->g.nscy * (z / pic::kSCZ));
         int const lc = x % pic::kSCX + pic::kSCX * (y % pic::kSCY + pic::kSCY * (z % pic::kSCZ));
-        size_t const g = static_cast<size_t>(sc) * m.cap + counts[sc]++;
+        size_t const g = static_cast<size_t>(m.base[sc]) + counts[sc]++;
         for (int a = 0; a < 3; ++a) {
             hp[a][g] = pos[3 * i + a];
             hm[a][g] = mom[3 * i + a];
@@ -1335,16 +1482,73 @@ int pic_load_particles(pic_ctx* c, int32_t s, int64_t n, const int32_t* cell, co
     return PIC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// whether every supercell of the species has `pitch` slots (the device
+// layout is then the frame view itself)
+bool uniformLayout(const SpeciesMem& m, int pitch) {
+    for (size_t i = 0; i < m.capSc.size(); ++i)
+        if (m.capSc[i] != static_cast<uint32_t>(pitch) || m.base[i] != i * static_cast<uint64_t>(pitch)) return false;
+    return true;
+}
+
+// device array (the species' layout) -> frame view of `pitch` slots per
+// supercell; slots a supercell does not have read as zero
+template <typename T>
+int layoutToView(pic_ctx* c, const SpeciesMem& m, int pitch, const T* dev, T* view) {
+    size_t const n = static_cast<size_t>(c->nsc) * pitch;
+    if (uniformLayout(m, pitch)) {
+        PIC_HIP(hipMemcpyAsync(view, dev, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+        return PIC_OK;
+    }
+    std::vector<T> tmp(m.slots);
+    PIC_HIP(hipMemcpyAsync(tmp.data(), dev, m.slots * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    std::fill(view, view + n, T(0));
+    for (int sc = 0; sc < c->nsc; ++sc) {
+        size_t const k = std::min<size_t>(m.capSc[sc], static_cast<size_t>(pitch));
+        std::copy(tmp.begin() + m.base[sc], tmp.begin() + m.base[sc] + k, view + static_cast<size_t>(sc) * pitch);
+    }
+    return PIC_OK;
+}
+
+// frame view of `pitch` slots per supercell -> device array (the species'
+// layout); slots beyond the view's pitch are left empty
+template <typename T>
+int viewToLayout(pic_ctx* c, const SpeciesMem& m, int pitch, const T* view, T* dev) {
+    if (uniformLayout(m, pitch)) {
+        size_t const n = static_cast<size_t>(c->nsc) * pitch;
+        PIC_HIP(hipMemcpyAsync(dev, view, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(hipStreamSynchronize(c->stream));
+        return PIC_OK;
+    }
+    std::vector<T> tmp(m.slots, T(0));
+    for (int sc = 0; sc < c->nsc; ++sc) {
+        size_t const k = std::min<size_t>(m.capSc[sc], static_cast<size_t>(pitch));
+        const T* src = view + static_cast<size_t>(sc) * pitch;
+        std::copy(src, src + k, tmp.begin() + m.base[sc]);
+    }
+    PIC_HIP(hipMemcpyAsync(dev, tmp.data(), m.slots * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pic_write_frames(pic_ctx* c, int32_t s, const uint32_t* counts, const float* pos, const float* mom, const float* w,
                      const uint16_t* lc, const uint8_t* mask) {
     int r = checkSpecies(c, s);
     if (r != PIC_OK) return r;
     SpeciesMem& m = c->sp[s];
-    pic::SpeciesDev& d = m.dev;
-    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    int const pitch = m.cap;  // the frame view's slots per supercell
+    size_t const slots = static_cast<size_t>(c->nsc) * pitch;
     if (counts) {
         for (int i = 0; i < c->nsc; ++i)
-            if (counts[i] > static_cast<uint32_t>(m.cap)) return fail(PIC_E_CAPACITY, "count above frame capacity");
+            if (counts[i] > static_cast<uint32_t>(pitch)) return fail(PIC_E_CAPACITY, "count above frame capacity");
     }
     // the occupied slots of what is written must hold a particle the kernels
     // can take: local cell in the supercell, position in [0,1), finite
@@ -1354,12 +1558,12 @@ int pic_write_frames(pic_ctx* c, int32_t s, const uint32_t* counts, const float*
         if (counts) {
             std::copy(counts, counts + c->nsc, n.begin());
         } else {
-            PIC_HIP(hipMemcpyAsync(n.data(), d.count, c->nsc * 4, hipMemcpyDeviceToHost, c->stream));
+            PIC_HIP(hipMemcpyAsync(n.data(), m.dev.count, c->nsc * 4, hipMemcpyDeviceToHost, c->stream));
             PIC_HIP(hipStreamSynchronize(c->stream));
         }
         for (int sc = 0; sc < c->nsc; ++sc)
             for (uint32_t k = 0; k < n[sc]; ++k) {
-                size_t const i = static_cast<size_t>(sc) * m.cap + k;
+                size_t const i = static_cast<size_t>(sc) * pitch + k;
                 if (lc && lc[i] >= pic::kSCV) return fail(PIC_E_INVALID, "localCellIdx outside the supercell");
                 if (mask && (mask[i] < 1 || mask[i] > 27))
                     return fail(PIC_E_INVALID, "multiMask of an occupied slot outside 1..27");
@@ -1371,21 +1575,27 @@ int pic_write_frames(pic_ctx* c, int32_t s, const uint32_t* counts, const float*
                 }
             }
     }
+    // supercells whose new count exceeds their slots grow first
+    if (counts) {
+        std::vector<uint32_t> need(counts, counts + c->nsc);
+        if ((r = growFor(c, s, need, kMaxFrames)) != PIC_OK) return r;
+    }
+    pic::SpeciesDev& d = m.dev;
     m.shiftReady = 0;
     if (counts) PIC_HIP(hipMemcpyAsync(d.count, counts, c->nsc * 4, hipMemcpyHostToDevice, c->stream));
     for (int a = 0; a < 3; ++a) {
-        if (pos) PIC_HIP(hipMemcpyAsync(d.pos[a], pos + a * slots, slots * 4, hipMemcpyHostToDevice, c->stream));
-        if (mom) PIC_HIP(hipMemcpyAsync(d.mom[a], mom + a * slots, slots * 4, hipMemcpyHostToDevice, c->stream));
+        if (pos && (r = viewToLayout(c, m, pitch, pos + a * slots, d.pos[a])) != PIC_OK) return r;
+        if (mom && (r = viewToLayout(c, m, pitch, mom + a * slots, d.mom[a])) != PIC_OK) return r;
     }
     if (w) {
-        PIC_HIP(hipMemcpyAsync(d.w, w, slots * 4, hipMemcpyHostToDevice, c->stream));
+        if ((r = viewToLayout(c, m, pitch, w, d.w)) != PIC_OK) return r;
         double wmax = 0.0;
         for (size_t i = 0; i < slots; ++i) wmax = std::max(wmax, std::fabs(static_cast<double>(w[i])));
         m.wmax = wmax;
         updateJexp(c);
     }
-    if (lc) PIC_HIP(hipMemcpyAsync(d.cell, lc, slots * 2, hipMemcpyHostToDevice, c->stream));
-    if (mask) PIC_HIP(hipMemcpyAsync(d.mask, mask, slots, hipMemcpyHostToDevice, c->stream));
+    if (lc && (r = viewToLayout(c, m, pitch, lc, d.cell)) != PIC_OK) return r;
+    if (mask && (r = viewToLayout(c, m, pitch, mask, d.mask)) != PIC_OK) return r;
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
@@ -1396,15 +1606,16 @@ int pic_read_frames(pic_ctx* c, int32_t s, uint32_t* counts, float* pos, float* 
     if (r != PIC_OK) return r;
     SpeciesMem& m = c->sp[s];
     pic::SpeciesDev& d = m.dev;
-    size_t const slots = static_cast<size_t>(c->nsc) * m.cap;
+    int const pitch = m.cap;
+    size_t const slots = static_cast<size_t>(c->nsc) * pitch;
     if (counts) PIC_HIP(hipMemcpyAsync(counts, d.count, c->nsc * 4, hipMemcpyDeviceToHost, c->stream));
     for (int a = 0; a < 3; ++a) {
-        if (pos) PIC_HIP(hipMemcpyAsync(pos + a * slots, d.pos[a], slots * 4, hipMemcpyDeviceToHost, c->stream));
-        if (mom) PIC_HIP(hipMemcpyAsync(mom + a * slots, d.mom[a], slots * 4, hipMemcpyDeviceToHost, c->stream));
+        if (pos && (r = layoutToView(c, m, pitch, d.pos[a], pos + a * slots)) != PIC_OK) return r;
+        if (mom && (r = layoutToView(c, m, pitch, d.mom[a], mom + a * slots)) != PIC_OK) return r;
     }
-    if (w) PIC_HIP(hipMemcpyAsync(w, d.w, slots * 4, hipMemcpyDeviceToHost, c->stream));
-    if (lc) PIC_HIP(hipMemcpyAsync(lc, d.cell, slots * 2, hipMemcpyDeviceToHost, c->stream));
-    if (mask) PIC_HIP(hipMemcpyAsync(mask, d.mask, slots, hipMemcpyDeviceToHost, c->stream));
+    if (w && (r = layoutToView(c, m, pitch, d.w, w)) != PIC_OK) return r;
+    if (lc && (r = layoutToView(c, m, pitch, d.cell, lc)) != PIC_OK) return r;
+    if (mask && (r = layoutToView(c, m, pitch, d.mask, mask)) != PIC_OK) return r;
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
 }
@@ -1459,9 +1670,9 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
     if (r != PIC_OK) return r;
     if (!desc || desc->ppc <= 0) return fail(PIC_E_INVALID, "invalid init description");
     SpeciesMem& m = c->sp[s];
-    if (desc->ppc * pic::kSCV > m.cap) {
-        r = growSpecies(c, s, grownFrames(m.desc.frames_per_sc, static_cast<uint32_t>(desc->ppc) * pic::kSCV, 0));
-        if (r != PIC_OK) return r;
+    {  // every supercell must hold ppc particles in each of its cells
+        std::vector<uint32_t> need(c->nsc, static_cast<uint32_t>(desc->ppc) * pic::kSCV);
+        if ((r = growFor(c, s, need, kMaxFrames)) != PIC_OK) return r;
     }
     double const vol = c->cfg.cell_si[0] * c->cfg.cell_si[1] * c->cfg.cell_si[2];
     pic::InitArgs a{};
@@ -1612,7 +1823,7 @@ int pic_shift(pic_ctx* c, int32_t species) {
     if (r != PIC_OK) return r;
     // a supercell that overflowed grows now, so the next stage sees every particle
     int skipped = 0;
-    return recoverCapacity(c, &skipped, 0);
+    return recoverCapacity(c, &skipped, kMaxFrames);
 }
 
 int pic_fdtd_before_current(pic_ctx* c) {
@@ -1790,7 +2001,7 @@ int pic_step(pic_ctx* c, int32_t nsteps) {
             int const r = oneStep(c);
             if (r != PIC_OK) return r;
         }
-        int const r = recoverCapacity(c, &todo, 0);
+        int const r = recoverCapacity(c, &todo, kMaxFrames);
         if (r != PIC_OK) return r;
     } while (todo > 0);
     return PIC_OK;
@@ -1805,7 +2016,8 @@ int pic_step_graph(pic_ctx* c, int32_t nsteps) {
     do {
         int const r = stepGraphs(c, todo);
         if (r != PIC_OK) return r;
-        int const rr = recoverCapacity(c, &todo, 0);  // a growth drops the graphs: recaptured on the next replay
+        // a repacking growth drops the graphs: recaptured on the next replay
+        int const rr = recoverCapacity(c, &todo, kMaxFrames);
         if (rr != PIC_OK) return rr;
     } while (todo > 0);
     return PIC_OK;
@@ -2401,7 +2613,7 @@ int64_t ensParticles(pic_ensemble* e, int i) {
     pic_ctx* c = e->sims[i];
     int64_t n = 0;
     for (const SpeciesMem& m : c->sp)
-        if (m.desc.pusher != PIC_PUSHER_NONE) n += (int64_t)c->nsc * m.cap;
+        if (m.desc.pusher != PIC_PUSHER_NONE) n += (int64_t)m.slots;
     return n;
 }
 

@@ -490,6 +490,39 @@ __device__ __forceinline__ unsigned long long toFixed(float acc, double jscale) 
     return b - 0x4338000000000000ull;
 }
 
+// The bits of the same double, magic constant included: for |acc*2^e| < 2^51
+// they are 0x4338000000000000 + round(acc*2^e) as 64-bit integers. A tile
+// element adds N of them; the constant is a multiple of 2^51, so the sum's
+// low 51 bits are those of the sum S of the rounded values, and S follows
+// from them (fixedTileValue) while |S| < 2^50. The exponent e keeps one
+// contribution below 2^30; a supercell holds at most 255 frames (below 2^16
+// particles) of each of at most 4 species, and a particle adds at most two
+// values to one element (EmZ's two virtual particles), so |S| < 2^49 always;
+// one instruction per contribution less than subtracting the constant.
+__device__ __forceinline__ unsigned long long toFixedBits(float acc, double jscale) {
+    return (unsigned long long)__double_as_longlong(__fma_rn((double)acc, jscale, 6755399441055744.0));
+}
+
+// S from the sum of N toFixedBits values: the low 51 bits, sign-extended
+__device__ __forceinline__ long long fixedTileValue(unsigned long long bits) {
+    return (long long)(bits << 13) >> 13;
+}
+
+// Geometry of a current tile: the supercell plus LO cells below and UP
+// above on every axis, one box per component (planes [z][y][x], the
+// components one after another). A component reaches one cell less above
+// along its own axis: its innermost deposit loop (the prefix sum) stops one
+// grid point short of the others.
+template <int LO, int UP>
+struct JTileGeom {
+    static constexpr int ext(int c, int a) { return (a == 0 ? SCX : (a == 1 ? SCY : SCZ)) + LO + (a == c ? UP - 1 : UP); }
+    static constexpr int py(int c) { return ext(c, 0); }
+    static constexpr int pz(int c) { return ext(c, 0) * ext(c, 1); }
+    static constexpr int vol(int c) { return ext(c, 0) * ext(c, 1) * ext(c, 2); }
+    static constexpr int off(int c) { return c == 0 ? 0 : (c == 1 ? vol(0) : vol(0) + vol(1)); }
+    static constexpr int total() { return vol(0) + vol(1) + vol(2); }
+};
+
 template <int SUPP>
 struct Esirkepov {
     static constexpr int begin = Shape<SUPP>::begin;
@@ -538,7 +571,7 @@ struct Esirkepov {
                                 // fixed-point conversion does not see: start the sum at W
                                 accumulated_J = k == begin ? W : accumulated_J + W;
                                 __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
-                                                       toFixed(accumulated_J, jscale), __ATOMIC_RELAXED,
+                                                       toFixedBits(accumulated_J, jscale), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         }
@@ -639,7 +672,7 @@ struct Esirkepov {
                     float const W = dsk[k - begin] * tmp;
                     accumulated_J = k == begin ? W : accumulated_J + W;
                     __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
-                                           toFixed(accumulated_J, jscale), __ATOMIC_RELAXED,
+                                           toFixedBits(accumulated_J, jscale), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
@@ -649,40 +682,45 @@ struct Esirkepov {
     // Deposit of a particle with changesCell(G) == false: the contributions
     // of depositGeometry<..., false>, bit for bit, with each axis' shape
     // values formed once.
-    template <int TX, int TY, int TV>
+    // tile element of component c for the cell (x, y, z) in tile coordinates
+    template <class TG>
+    __device__ __forceinline__ static unsigned long long* cornerOf(unsigned long long* jt, int c, int x, int y, int z) {
+        return jt + TG::off(c) + (z * TG::ext(c, 1) + y) * TG::ext(c, 0) + x;
+    }
+
+    template <class TG>
     __device__ __forceinline__ static void depositStay(unsigned long long* __restrict__ jt, int cx, int cy, int cz,
                                                        const Geometry& G, float charge, const DepositConsts& k) {
-        constexpr int PX = 1, PY = TX, PZ = TX * TY;
-        unsigned long long* const corner =
-            jt + ((cz + G.gs[2] + begin) * TY + (cy + G.gs[1] + begin)) * TX + (cx + G.gs[0] + begin);
+        int const x = cx + G.gs[0] + begin, y = cy + G.gs[1] + begin, z = cz + G.gs[2] + begin;
         StayAxis const ax = stayAxis(G.p0[0], G.p1[0]);
         StayAxis const ay = stayAxis(G.p0[1], G.p1[1]);
         StayAxis const az = stayAxis(G.p0[2], G.p1[2]);
         float const cq = charge * k.inv_vdt;
         // J_x: i->y, j->z, k->x; J_y: i->z, j->x, k->y; J_z: i->x, j->y, k->z
-        cpt1DStay<PY, PZ, PX>(corner, ay, az, ax, G.p0[0], G.p1[0], cq * k.cell[0], k.jscale);
-        cpt1DStay<PZ, PX, PY>(corner + TV, az, ax, ay, G.p0[1], G.p1[1], cq * k.cell[1], k.jscale);
-        cpt1DStay<PX, PY, PZ>(corner + 2 * TV, ax, ay, az, G.p0[2], G.p1[2], cq * k.cell[2], k.jscale);
+        cpt1DStay<TG::py(0), TG::pz(0), 1>(cornerOf<TG>(jt, 0, x, y, z), ay, az, ax, G.p0[0], G.p1[0],
+                                           cq * k.cell[0], k.jscale);
+        cpt1DStay<TG::pz(1), 1, TG::py(1)>(cornerOf<TG>(jt, 1, x, y, z), az, ax, ay, G.p0[1], G.p1[1],
+                                           cq * k.cell[1], k.jscale);
+        cpt1DStay<1, TG::py(2), TG::pz(2)>(cornerOf<TG>(jt, 2, x, y, z), ax, ay, az, G.p0[2], G.p1[2],
+                                           cq * k.cell[2], k.jscale);
     }
 
     // Deposit one particle. (cx,cy,cz) is the particle's cell in J-tile
     // coordinates, G its move geometry. Tile layout: component-major planes
     // of TV elements, [z][y][x] with pitches TX, TX*TY. GENERAL = false only
     // for a geometry with changesCell(G) == false.
-    template <int TX, int TY, int TV, bool GENERAL>
+    template <class TG, bool GENERAL>
     __device__ __forceinline__ static void depositGeometry(unsigned long long* __restrict__ jt, int cx, int cy, int cz,
                                                            const Geometry& G, float charge, const DepositConsts& k) {
-        constexpr int PX = 1, PY = TX, PZ = TX * TY;
         // tile element of cell (gridShift + begin) in every axis
-        unsigned long long* const corner =
-            jt + ((cz + G.gs[2] + begin) * TY + (cy + G.gs[1] + begin)) * TX + (cx + G.gs[0] + begin);
+        int const x = cx + G.gs[0] + begin, y = cy + G.gs[1] + begin, z = cz + G.gs[2] + begin;
         // J_x: rotation (1,2,0): i->y, j->z, k->x
         {
             float const q0[3] = {G.p0[1], G.p0[2], G.p0[0]};
             float const q1[3] = {G.p1[1], G.p1[2], G.p1[0]};
             int const s3[3] = {G.st[1], G.st[2], G.st[0]};
             float const csd = charge * k.inv_vdt * k.cell[0];
-            cpt1D<PY, PZ, PX, GENERAL>(corner, q0, q1, s3, csd, k.jscale);
+            cpt1D<TG::py(0), TG::pz(0), 1, GENERAL>(cornerOf<TG>(jt, 0, x, y, z), q0, q1, s3, csd, k.jscale);
         }
         // J_y: rotation (2,0,1): i->z, j->x, k->y
         {
@@ -690,21 +728,21 @@ struct Esirkepov {
             float const q1[3] = {G.p1[2], G.p1[0], G.p1[1]};
             int const s3[3] = {G.st[2], G.st[0], G.st[1]};
             float const csd = charge * k.inv_vdt * k.cell[1];
-            cpt1D<PZ, PX, PY, GENERAL>(corner + TV, q0, q1, s3, csd, k.jscale);
+            cpt1D<TG::pz(1), 1, TG::py(1), GENERAL>(cornerOf<TG>(jt, 1, x, y, z), q0, q1, s3, csd, k.jscale);
         }
         // J_z: identity: i->x, j->y, k->z
         {
             float const csd = charge * k.inv_vdt * k.cell[2];
-            cpt1D<PX, PY, PZ, GENERAL>(corner + 2 * TV, G.p0, G.p1, G.st, csd, k.jscale);
+            cpt1D<1, TG::py(2), TG::pz(2), GENERAL>(cornerOf<TG>(jt, 2, x, y, z), G.p0, G.p1, G.st, csd, k.jscale);
         }
     }
 
     // Deposit one particle. (cx,cy,cz) is the particle's cell in J-tile
     // coordinates, pos its in-cell position after the move, vel its velocity.
-    template <int TX, int TY, int TV>
+    template <class TG>
     __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
                                                    Vec3 vel, float charge, const DepositConsts& k) {
-        depositGeometry<TX, TY, TV, true>(jt, cx, cy, cz, geometry(pos, vel, k), charge, k);
+        depositGeometry<TG, true>(jt, cx, cy, cz, geometry(pos, vel, k), charge, k);
     }
 };
 
@@ -745,43 +783,45 @@ struct EmZ {
                     float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
                     accumulated_J = k == begin ? W : accumulated_J + W;  // as in Esirkepov::cpt1D
                     __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
-                                           toFixed(accumulated_J, jscale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                           toFixedBits(accumulated_J, jscale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
             }
         }
     }
 
     // one virtual particle from a to b (positions relative to shifted cell s)
-    template <int TX, int TY, int TV>
+    template <class TG>
     __device__ __forceinline__ static void virtualParticle(unsigned long long* __restrict__ jt, int cx, int cy, int cz,
                                                            const int* s, const float* a, const float* b,
                                                            float chargeDensity, const DepositConsts& k) {
-        constexpr int PX = 1, PY = TX, PZ = TX * TY;
-        unsigned long long* const corner = jt + ((cz + s[2] + begin) * TY + (cy + s[1] + begin)) * TX + (cx + s[0] + begin);
+        int const x = cx + s[0] + begin, y = cy + s[1] + begin, z = cz + s[2] + begin;
         {  // J_x: rotation (1,2,0)
             float const q0[3] = {a[1], a[2], a[0]};
             float const q1[3] = {b[1], b[2], b[0]};
-            cpt1D<PY, PZ, PX>(corner, q0, q1, k.cell[0] * chargeDensity / k.dt, k.jscale);
+            cpt1D<TG::py(0), TG::pz(0), 1>(Esirkepov<SUPP>::template cornerOf<TG>(jt, 0, x, y, z), q0, q1,
+                                           k.cell[0] * chargeDensity / k.dt, k.jscale);
         }
         {  // J_y: rotation (2,0,1)
             float const q0[3] = {a[2], a[0], a[1]};
             float const q1[3] = {b[2], b[0], b[1]};
-            cpt1D<PZ, PX, PY>(corner + TV, q0, q1, k.cell[1] * chargeDensity / k.dt, k.jscale);
+            cpt1D<TG::pz(1), 1, TG::py(1)>(Esirkepov<SUPP>::template cornerOf<TG>(jt, 1, x, y, z), q0, q1,
+                                           k.cell[1] * chargeDensity / k.dt, k.jscale);
         }
-        cpt1D<PX, PY, PZ>(corner + 2 * TV, a, b, k.cell[2] * chargeDensity / k.dt, k.jscale);  // J_z
+        cpt1D<1, TG::py(2), TG::pz(2)>(Esirkepov<SUPP>::template cornerOf<TG>(jt, 2, x, y, z), a, b,
+                                       k.cell[2] * chargeDensity / k.dt, k.jscale);  // J_z
     }
 
     // Deposit one particle: (cx,cy,cz) its cell in J-tile coordinates, pos
     // its in-cell position after the move, vel its velocity.
-    template <int TX, int TY, int TV>
+    template <class TG>
     __device__ __forceinline__ static void deposit(unsigned long long* __restrict__ jt, int cx, int cy, int cz, Vec3 pos,
                                                    Vec3 vel, float charge, const DepositConsts& k) {
         float const dp[3] = {vel.x * k.dt / k.cell[0], vel.y * k.dt / k.cell[1], vel.z * k.dt / k.cell[2]};
-        depositDp<TX, TY, TV>(jt, cx, cy, cz, pos, dp, charge, k);
+        depositDp<TG>(jt, cx, cy, cz, pos, dp, charge, k);
     }
 
     // the same with the move in cells dp = vel * dt / cell given
-    template <int TX, int TY, int TV>
+    template <class TG>
     __device__ __forceinline__ static void depositDp(unsigned long long* __restrict__ jt, int cx, int cy, int cz,
                                                      Vec3 pos, const float* dp, float charge, const DepositConsts& k) {
         constexpr bool isEven = (SUPP % 2) == 0;
@@ -808,14 +848,14 @@ struct EmZ {
             a[d] = p0[d] - (float)s0[d];
             b[d] = relay[d] - (float)s0[d];
         }
-        virtualParticle<TX, TY, TV>(jt, cx, cy, cz, s0, a, b, chargeDensity, k);
+        virtualParticle<TG>(jt, cx, cy, cz, s0, a, b, chargeDensity, k);
         if (s0[0] != s1[0] || s0[1] != s1[1] || s0[2] != s1[2]) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 b[d] = p1[d] - (float)s1[d];
                 a[d] = relay[d] - (float)s1[d];
             }
-            virtualParticle<TX, TY, TV>(jt, cx, cy, cz, s1, a, b, chargeDensity, k);
+            virtualParticle<TG>(jt, cx, cy, cz, s1, a, b, chargeDensity, k);
         }
     }
 };

@@ -25,12 +25,20 @@ struct SpeciesDev {
     int32_t* scratch;    // per supercell: gap/source lists of the compaction
     uint32_t* outOff;    // per supercell: 28 offsets into outIdx (code 1..26, [27]=total)
     uint32_t* leaveBits; // per supercell and push thread: leaver flags per loop iteration
-    int32_t cap;         // slots per supercell
+    // supercell sc owns the run of slots [base[sc], base[sc] + capOf[sc]) of
+    // every per-slot array (whole frames): supercells grow one by one, as the
+    // reference's take frames from its heap (ParticlesBox.hpp:96-122)
+    const uint64_t* base;
+    const uint32_t* capOf;
+    int32_t cap;         // the largest capacity of a supercell (slots)
     int32_t lists;       // 1: the push kernel builds the leaver lists of this species
     int32_t reflect;     // bit d: particles crossing an open border of axis d are reflected
     float mass;          // frame::getMass()
     float charge;        // frame::getCharge()
 };
+
+__device__ __forceinline__ size_t slotBase(const SpeciesDev& S, int sc) { return (size_t)S.base[sc]; }
+__device__ __forceinline__ uint32_t slotCap(const SpeciesDev& S, int sc) { return S.capOf[sc]; }
 
 // Step state of one simulation on the device. A shift that finds a supercell
 // full keeps the true count, leaves the records it could not place in the
@@ -101,10 +109,15 @@ hipError_t launchShift(const GridDev& g, const SpeciesDev& sp, bool staged, hipS
 hipError_t launchInsertRedo(const GridDev& g, const SpeciesDev& sp, hipStream_t s);
 // count[sc] = min(count[sc], cap): the last resort when a supercell cannot grow
 hipError_t launchClampCounts(const SpeciesDev& sp, int nsc, hipStream_t s);
-// rows of oldRowBytes from src (pitch oldRowBytes) to dst (pitch newRowBytes),
-// both multiples of 16
-hipError_t launchRowCopy(void* dst, const void* src, size_t oldRowBytes, size_t newRowBytes, size_t rows,
-                         hipStream_t s);
+// the runs of the supercells listed in `moves` (one entry per supercell:
+// slots oldBase, count slots, newBase), copied from src to dst (may be the
+// same array when no run overlaps another)
+struct RunMove {
+    uint64_t from, to;
+    uint32_t slots, pad;
+};
+hipError_t launchMoveRuns(void* dst, const void* src, uint32_t bytesPerSlot, const RunMove* moves, int n,
+                          hipStream_t s);
 hipError_t launchCollect(const GridDev& g, const SpeciesDev& sp, hipStream_t s);
 // the push kernel builds leaver lists for species with at most this many slots per supercell
 constexpr int kPushThreadsPerBlock = 512;

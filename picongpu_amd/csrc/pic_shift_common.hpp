@@ -44,7 +44,7 @@ __device__ void groupLeavers(const SpeciesDev& S, int sc, int L, GroupScratch& g
     int const lane = tid & 63;
     int const wave = tid >> 6;
     constexpr int NT = NW * 64;
-    size_t const base = (size_t)sc * (size_t)S.cap;
+    size_t const base = slotBase(S, sc);
     uint32_t* off = S.outOff + (size_t)sc * kOffStride;
     if (tid < kNumDirs) g.cnt[tid] = 0u;
     __syncthreads();
@@ -125,7 +125,7 @@ __device__ void stageSC(const SpeciesDev& S, int sc, int L, uint32_t* sWave) {
     if (L == 0) return;  // uniform
     constexpr int NT = NW * 64;
     int const tid = threadIdx.x;
-    size_t const base = (size_t)sc * (size_t)S.cap;
+    size_t const base = slotBase(S, sc);
     uint32_t const n = S.count[sc];
     for (int k = tid; k < L; k += NT) {
         size_t const from = base + (size_t)S.outIdx[base + k];

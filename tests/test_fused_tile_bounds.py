@@ -71,6 +71,15 @@ def touched(supp, x0, dp, reflect=None):
     return lo, hi, d, np.abs(iS - iE)
 
 
+def touched_own_axis(supp, x0, dp, reflect=None):
+    """Upper end of the touched range along a current component's own axis
+    (the prefix-sum loop: Esirkepov rows up to gs+end-2+leave, EmZ up to
+    s+end-2), relative to the old cell."""
+    begin, end = SHAPES[supp]
+    lo, hi, d, _ = touched(supp, x0, dp, reflect)
+    return hi - 1
+
+
 def samples(rng, n):
     ulp = np.finfo(F).eps
     x_edge = np.array([0.0, 2.0**-24, 0.5 - ulp / 2, 0.5, 0.5 + ulp, 1.0 - ulp / 2, 0.25, 0.75], F)
@@ -97,6 +106,8 @@ def test_fused_deposit_stays_inside_margins(supp, reflect):
     assert lo.min() == -lower and hi.max() == upper
     # the start and end shifted cells differ by at most one
     assert jump.max() <= 1
+    # along a component's own axis the tile reaches one cell less above
+    assert touched_own_axis(supp, x, dp, reflect).max() == upper - 1
 
 
 def test_a_whole_cell_move_leaves_the_margins():

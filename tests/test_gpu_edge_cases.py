@@ -163,8 +163,42 @@ def test_full_supercells_grow_without_loss(graph):
         for fld in (abi.FIELD_E, abi.FIELD_B, abi.FIELD_J):
             H.assert_field_equal(sim.read_field(fld), ora.get_field(fld), f"field {fld} after {2 * k + 2} steps")
     assert sim.slots(0) > 256, "the supercells did not grow"
+    caps = sim.supercell_slots(0)
+    assert np.all(caps % 256 == 0) and caps.max() == sim.slots(0)
     assert sim.error_flags() & abi.FLAG_CAPACITY_GREW
     assert sim.error_flags() == 0  # read and cleared
+    assert sim.particle_count(0) == sum(counts)
+    sim.close()
+    ora.close()
+
+
+def test_only_the_supercells_that_fill_up_grow():
+    """The four full supercells of the middle z layer between sparse ones,
+    particles converging on the box centre (which lies in that layer): the
+    middle supercells overflow and grow by whole frames, the sparse ones,
+    which only lose particles, keep their one frame, and the result is
+    bit-exact with the oracle run at a capacity that never overflows."""
+    st = thermal(shape=abi.SHAPE_TSC, frames=1)
+    counts = [48] * 4 + [256] * 4 + [48] * 4
+    print("case per-supercell growth", flush=True)
+    cells, pos, mom, w = converging(st, counts, seed=17, u=1.5)
+    f = setups.bin_frames(st, 0, cells, pos, mom, w, frames_per_sc=1)
+    sim = abi.Simulation(st.config())
+    sim.write_frames(0, f)
+    big = copy.deepcopy(st)
+    big.species[0].frames_per_sc = 12
+    fb = setups.bin_frames(big, 0, cells, pos, mom, w, frames_per_sc=12)
+    ora = O.Oracle(big, 32, 0, jexp=sim.j_scale_exp())
+    ora.set_frames(0, fb)
+    sim.step_graph(6)
+    ora.step(6)
+    H.assert_frames_equal(sim.read_frames(0), ora.get_frames(0), "frames after 6 steps")
+    for fld in (abi.FIELD_E, abi.FIELD_B, abi.FIELD_J):
+        H.assert_field_equal(sim.read_field(fld), ora.get_field(fld), f"field {fld} after 6 steps")
+    caps = sim.supercell_slots(0)
+    print(f"capacities after 6 steps: {caps.tolist()}", flush=True)
+    assert np.all(caps[4:8] > 256) and np.all(caps[:4] == 256) and np.all(caps[8:] == 256)
+    assert np.all(caps % 256 == 0)
     assert sim.particle_count(0) == sum(counts)
     sim.close()
     ora.close()

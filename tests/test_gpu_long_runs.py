@@ -8,12 +8,15 @@ that do not depend on size (the oracle cannot follow them there):
 - Laser-plasma 192x1536x192, PQS, Vay, Lehe, open x/y borders, 2000 steps:
   protons untouched, electrons only leave through the open borders, no
   particle lost to capacity (the wake compresses the plasma, supercells grow),
-  the Gauss residual unchanged in the interior, fields finite.
+  the interior Gauss residual changes no more than the FP32 rounding of the
+  laser field itself changes it (the same pulse in vacuum), fields finite.
 - Batched ensemble (64 two-stream boxes, seed 7), 20 graph-replayed steps:
   every member conserves its particles and its Gauss residual.
 - Two members of that ensemble for their 300 steps against the FP64 oracle:
   the growth rate of the magnetic field energy.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -67,36 +70,70 @@ def test_kelvin_helmholtz_500_steps():
     sim.close()
 
 
-@pytest.mark.skip(reason="supercells grow all at once: the compressed wake needs more frames per supercell "
-                         "than device memory holds for every supercell (pic_step_graph: out of memory)")
+def interior_drift(sim, st, r0, m=16):
+    """Largest change of the Gauss residual, relative to one species' charge
+    density, away from the open x and y borders by more than the absorber
+    and the deposit's reach."""
+    r1 = sim.gauss_residual().reshape(st.grid[2], st.grid[1], st.grid[0])
+    return float(np.abs(r1 - r0)[:, m:-m, m:-m].max()) / rho_scale(sim)
+
+
 def test_laser_plasma_2000_steps():
+    """The Gauss residual's bound here is set by the laser: at a0 = 2,
+    eps0 |E| (1/dx + 1/dy + 1/dz) is about 24000 rho, so each FP32 rounding
+    of E moves eps0 div E by about 1.4e-3 rho, and 2000 steps of such rounding
+    leave a drift of order 1e-2 rho with no particle at all. The control run
+    (the same pulse, solver and borders in vacuum) measures that drift; the
+    plasma run may add its own field's rounding to it, but a systematic error
+    (a step's current missing or counted twice moves the residual by about
+    c dt / dx = 0.5 rho in the wake) or misplaced electrons (a PQS electron
+    at the box's highest density carries up to 0.04 rho to one cell) would
+    show. Single-particle faithfulness is the bit-exact parity tests' job."""
     st = setups.laser_plasma()
+    vac = setups.laser_plasma()
+    for s in vac.species:
+        s.ramp = (1, 1.0, 100e-6)  # the ramp starts 1 m away: no particles in the box
+    print(f"case laser-plasma {st.grid}, 2000 steps: vacuum control", flush=True)
+    sim = abi.Simulation(vac.config())
+    setups.init_device(sim, vac)
+    assert sim.particle_count(0) == 0 and sim.particle_count(1) == 0
+    r0 = sim.gauss_residual().reshape(st.grid[2], st.grid[1], st.grid[0])
+    for _ in range(4):
+        sim.step_graph(500)
+        sim.sync()
+    drift_vacuum = interior_drift(sim, st, r0)
+    assert np.all(np.isfinite(sim.field_energy()))
+    sim.close()
+    print(f"  vacuum: interior Gauss drift / rho {drift_vacuum:.3e}", flush=True)
+
     print(f"case laser-plasma {st.grid}, 2000 steps", flush=True)
     sim = abi.Simulation(st.config())
     setups.init_device(sim, st)
     n0 = [sim.particle_count(i) for i in range(2)]
     r0 = sim.gauss_residual().reshape(st.grid[2], st.grid[1], st.grid[0])
     e0 = sim.field_energy()
-    for _ in range(20):
+    t0 = time.perf_counter()
+    for k in range(20):
         sim.step_graph(100)
         sim.sync()
+        caps = sim.supercell_slots(0)
+        print(f"  step {100 * (k + 1)}: {time.perf_counter() - t0:.1f} s, electron slots per supercell up to "
+              f"{int(caps.max())}, {int(np.count_nonzero(caps > caps.min()))} supercells grown", flush=True)
     n1 = [sim.particle_count(i) for i in range(2)]
     e1 = sim.field_energy()
-    r1 = sim.gauss_residual().reshape(st.grid[2], st.grid[1], st.grid[0])
     flags = sim.error_flags()
-    slots = sim.slots(0)
-    # interior: away from the open x and y borders by more than the absorber
-    # and the deposit's reach
-    m = 16
-    d = np.abs(r1 - r0)[:, m:-m, m:-m]
-    drift = float(d.max()) / rho_scale(sim)
+    caps = sim.supercell_slots(0)
+    slots = f"max {int(caps.max())}, {int(np.count_nonzero(caps > caps.min()))} of {caps.size} supercells grown"
+    drift = interior_drift(sim, st, r0)
     print(f"laser-plasma 2000 steps: counts {n0} -> {n1}, electron slots per supercell {slots}, flags {flags:#x}, "
-          f"field energy {e0[0]:.4e} -> {e1[0]:.4e} J, interior Gauss drift / rho {drift:.3e}", flush=True)
+          f"field energy {e0[0]:.4e} -> {e1[0]:.4e} J, interior Gauss drift / rho {drift:.3e} "
+          f"(vacuum {drift_vacuum:.3e})", flush=True)
     assert n1[1] == n0[1]  # immobile protons
     assert 0 < n1[0] <= n0[0]  # electrons leave only through the open borders
     assert no_loss(flags)
     assert np.all(np.isfinite(e1))
-    assert drift < drift_bound(2000)
+    assert 1e-3 < drift_vacuum < 5e-2  # the laser's rounding is what sets the scale
+    assert drift < 2.0 * drift_vacuum
     sim.close()
 
 
