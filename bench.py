@@ -42,13 +42,33 @@ HBM_PEAK_GBS = 8000.0
 # workload, scaled by what the counters read for a 1 GiB device copy
 # (FETCH_SIZE reads 0.5 GiB for it: factor 2.0; WRITE_SIZE 1 GiB: factor 1.0).
 # The kernel's epilogue also stages the supercell shift (leaver records to
-# the outbox, compaction of the supercell), so these bytes include it.
+# the outbox, compaction of the supercell), and the TSC kernels' spilled
+# registers go to scratch, so these bytes include both.
 # Command: rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -- python3 bench.py
 # --steps 1 --warmup 1 --no-cpu-baseline --no-adjoint --no-ensemble --calibrate-copy
 MEASURED_TRAFFIC = {
-    ("kelvin_helmholtz", (256, 256, 256)): {"fetch_kib": 812761.5, "write_kib": 1082081.6, "fetch_scale": 2.0,
+    ("kelvin_helmholtz", (256, 256, 256)): {"fetch_kib": 896118.6, "write_kib": 1340060.5, "fetch_scale": 2.0,
                                             "write_scale": 1.0},
 }
+# Instruction counters of the fused push/deposit kernel on this workload,
+# means per dispatch: rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_LDS
+# SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_VALU SQ_BUSY_CYCLES SQ_WAIT_INST_LDS
+# SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE -- python3 bench.py --steps 1 --warmup 1
+# --no-cpu-baseline --no-adjoint --no-ensemble. SQ counters are summed over
+# the chip; GRBM_GUI_ACTIVE is summed over the 8 XCDs (cycles of one XCD x 8).
+MEASURED_INSTS = {
+    ("kelvin_helmholtz", (256, 256, 256)): {"valu": 826.64e6, "lds": 90.875e6, "lds_bank_conflict": 488.81e6,
+                                            "grbm_gui_active": 31.881e6},
+    ("laser_plasma", (192, 1536, 192)): {"valu": 514.99e6, "lds": 54.819e6, "lds_bank_conflict": 207.32e6,
+                                        "grbm_gui_active": 20.819e6},
+}
+# VALU issue peak: a wave64 VALU instruction occupies one of the 1,024 SIMDs
+# (256 CUs x 4) for 4 cycles at the 2.4 GHz peak engine clock
+VALU_SIMDS = 1024
+VALU_CYCLES_PER_INST = 4
+CLOCK_HZ = 2.4e9
+CUS = 256
+XCDS = 8
 BYTES_PER_PARTICLE = 55.0
 BYTES_PER_CELL_STEP = 96.0
 BYTES_PER_CELL_PUSH = 36.0
@@ -284,6 +304,21 @@ def main():
     t = MEASURED_TRAFFIC.get((st.name, tuple(st.grid)))
     if t is not None:
         traffic = (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
+    avg_launch_ms = push_ms / max(push_launches, 1)
+    compute = {"valu_frac": None}
+    mi = MEASURED_INSTS.get((st.name, tuple(st.grid)))
+    if mi is not None:
+        particles_per_launch = n_particles * args.steps / max(push_launches, 1)
+        compute = {
+            # measured VALU instructions per launch at the VALU issue peak,
+            # over the launch time measured now
+            "valu_frac": mi["valu"] * VALU_CYCLES_PER_INST / (VALU_SIMDS * CLOCK_HZ * avg_launch_ms * 1e-3),
+            "valu_per_64_particles": mi["valu"] / (particles_per_launch / 64.0),
+            "lds_per_64_particles": mi["lds"] / (particles_per_launch / 64.0),
+            # cycles with an LDS bank conflict per CU over the profiled launch's cycles
+            "lds_conflict_frac": mi["lds_bank_conflict"] / CUS / (mi["grbm_gui_active"] / XCDS),
+            "valu_basis": "SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz x launch time)",
+        }
 
     out = {
         "metric": "particle-steps/s (push+deposit+FDTD) on 1 GPU; adjoint/forward time ratio",
@@ -307,7 +342,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "kPushDeposit (fused gather+push+move+Esirkepov)",
-                     "avg_launch_ms": push_ms / max(push_launches, 1), "launches": push_launches},
+                     "avg_launch_ms": avg_launch_ms, "launches": push_launches, **compute},
         "step_roofline": {"achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS,
                           "push_ms": push_ms / args.steps, "shift_ms": shift_ms / args.steps,
                           "field_ms": field_ms / args.steps},

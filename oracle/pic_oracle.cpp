@@ -14,9 +14,12 @@
 //  order 0 ("device"): current of each particle added as int64 fixed point
 //          into a per-supercell tile during the push (deposit fused with the
 //          push in the frame of the particle's new cell), tiles flushed into
-//          J in the checkerboard phase order of the device. Bit-identical
+//          J in the checkerboard phase order of the device; the field
+//          gather's sums and Esirkepov's W and prefix sums are multiply-adds
+//          rounded once (std::fma), as the device computes them. Bit-identical
 //          to the HIP library by construction of the same sums.
-//  order 1 ("sequential"): the order of the reference's CPU (omp2b) path:
+//  order 1 ("sequential"): the order and arithmetic of the reference's CPU
+//          (omp2b) path, no multiply-add contracted:
 //          push and shift first, then per supercell a float cache filled
 //          particle by particle in reverse frame order and flushed with a
 //          stride-(skip+1) checkerboard (Deposit.hpp:63-84, ForEach.hpp:314-324).
@@ -134,7 +137,34 @@ struct Consts {
     T lehe_alpha, lehe_delta, lehe_beta1, lehe_beta2;
     T mass[4], charge[4];
     double unit_length, unit_time, unit_mass, unit_charge, wtyp;
+    // the device order's multiply-adds (order 0): the field gather's sums and
+    // Esirkepov's W and prefix sums rounded once; the reference's CPU order
+    // rounds the product and the sum separately
+    bool contract = false;
 };
+
+// a * b + c, rounded once when C (std::fma) and twice otherwise
+template <bool C, typename T>
+inline T madd(T a, T b, T c) {
+    if constexpr (C)
+        return std::fma(a, b, c);
+    else
+        return a * b + c;
+}
+
+// Esirkepov's weight S0i S0j + 1/2 (dSi S0j + S0i dSj) + 1/3 dSj dSi
+// (Esirkepov.hpp:218) in the reference's order; when C every addition is
+// fused with the product it adds
+template <bool C, typename T>
+inline T esirkepovWeight(T s0i, T dsi, T sj, T dsj) {
+    if constexpr (C) {
+        T const mixed = std::fma(dsi, sj, s0i * dsj);
+        T const half = std::fma(T(0.5), mixed, s0i * sj);
+        return std::fma((T(1.0) / T(3.0)) * dsj, dsi, half);
+    } else {
+        return s0i * sj + T(0.5) * (dsi * sj + s0i * dsj) + (T(1.0) / T(3.0)) * dsj * dsi;
+    }
+}
 
 template <typename T>
 Consts<T> makeConsts(const oracle_config& cfg) {
@@ -336,7 +366,7 @@ int supportOf(int shape) { return shape == 1 ? 2 : (shape == 2 ? 3 : 4); }
 
 // ---- field gather: FieldToParticleInterpolation.hpp:100-124,
 // ShiftCoordinateSystem.hpp:53-79, AssignedTrilinearInterpolation.hpp:53-86
-template <typename T, bool OPEN>
+template <typename T, bool OPEN, bool C>
 T gather1Impl(const State<T>& st, const std::vector<T>& F, int supp, int gx, int gy, int gz, const T* pos,
               const T* fpos) {
     ShapeFns<T> sh{supp};
@@ -365,15 +395,13 @@ T gather1Impl(const State<T>& st, const std::vector<T>& F, int supp, int gx, int
         for (int y = 0; y < supp; ++y) {
             T rx = T(0.0);
             for (int x = 0; x < supp; ++x) {
-                T const term =
-                    (OPEN ? st.at(F, gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)
-                          : F[st.idx(gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)]) *
-                    sx[x];
-                rx = x == 0 ? term : rx + term;
+                T const f = OPEN ? st.at(F, gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)
+                                 : F[st.idx(gx + is[0] + begin + x, gy + is[1] + begin + y, gz + is[2] + begin + z)];
+                rx = x == 0 ? f * sx[x] : madd<C>(f, sx[x], rx);
             }
-            ry = y == 0 ? rx * sy[y] : ry + rx * sy[y];
+            ry = y == 0 ? rx * sy[y] : madd<C>(rx, sy[y], ry);
         }
-        rz = z == 0 ? ry * sz[z] : rz + ry * sz[z];
+        rz = z == 0 ? ry * sz[z] : madd<C>(ry, sz[z], rz);
     }
     return rz;
 }
@@ -381,8 +409,11 @@ T gather1Impl(const State<T>& st, const std::vector<T>& F, int supp, int gx, int
 // the periodic form is kept free of border checks (they slow the loop down)
 template <typename T>
 T gather1(const State<T>& st, const std::vector<T>& F, int supp, int gx, int gy, int gz, const T* pos, const T* fpos) {
-    return st.anyOpen ? gather1Impl<T, true>(st, F, supp, gx, gy, gz, pos, fpos)
-                      : gather1Impl<T, false>(st, F, supp, gx, gy, gz, pos, fpos);
+    if (st.k.contract)
+        return st.anyOpen ? gather1Impl<T, true, true>(st, F, supp, gx, gy, gz, pos, fpos)
+                          : gather1Impl<T, false, true>(st, F, supp, gx, gy, gz, pos, fpos);
+    return st.anyOpen ? gather1Impl<T, true, false>(st, F, supp, gx, gy, gz, pos, fpos)
+                      : gather1Impl<T, false, false>(st, F, supp, gx, gy, gz, pos, fpos);
 }
 
 // ---- pushers ------------------------------------------------------------------
@@ -543,7 +574,7 @@ int moveParticle(T* newPos, uint16_t& localCellIdx, uint8_t& multiMask, int* cel
 // ---- Esirkepov: fields/currentDeposition/Esirkepov/Esirkepov.hpp:61-242 ----
 // `add(comp, x, y, z, value)` receives each accumulated_J at cell (x,y,z)
 // relative to the particle's cell.
-template <typename T, typename Add>
+template <bool C, typename T, typename Add>
 void cptCurrent1D(int supp, const int* st, const T* p0, const T* p1, T csd, int comp, const int* perm, Add add) {
     if (p0[2] == p1[2]) return;  // :157
     ShapeFns<T> sh{supp};
@@ -563,12 +594,14 @@ void cptCurrent1D(int supp, const int* st, const T* p0, const T* p1, T csd, int 
             if (!(j < end + leaveJ)) continue;
             T const sj = s0j[j - begin];
             T const dsj = s1j[j - begin] - sj;
-            T const tmp = -csd * (s0i * sj + T(0.5) * (dsi * sj + s0i * dsj) + (T(1.0) / T(3.0)) * dsj * dsi);
+            T const tmp = -csd * esirkepovWeight<C>(s0i, dsi, sj, dsj);
             T acc = T(0.0);
             for (int kk = begin; kk < end; ++kk) {
                 if (!(kk < end + leaveK - 1)) continue;
-                T const W = (s1k[kk - begin] - s0k[kk - begin]) * tmp;
-                acc += W;
+                T const dsk = s1k[kk - begin] - s0k[kk - begin];
+                // accumulated_J += W, W = dsk * tmp (Esirkepov.hpp:233-236); the
+                // contracted sum starts at the first W
+                acc = C ? (kk == begin ? dsk * tmp : madd<C>(dsk, tmp, acc)) : acc + dsk * tmp;
                 int const ijk[3] = {i, j, kk};
                 // permuted access (PermutatedFieldValueAccess.hpp:87-108)
                 int rel[3];
@@ -582,7 +615,7 @@ void cptCurrent1D(int supp, const int* st, const T* p0, const T* p1, T csd, int 
 // EmZ, one virtual particle (EmZ/DepositCurrent.hpp:46-112): on-support
 // shapes (Jit outer, Cached middle and inner), supp x supp x (supp-1) cells
 // of its own shifted cell, no leave rows
-template <typename T, typename Add>
+template <bool C, typename T, typename Add>
 void emzCpt1D(int supp, const T* p0, const T* p1, T csd, int comp, const int* perm, Add add) {
     if (p0[2] == p1[2]) return;
     ShapeFns<T> sh{supp};
@@ -599,11 +632,11 @@ void emzCpt1D(int supp, const T* p0, const T* p1, T csd, int comp, const int* pe
         for (int j = begin; j < end; ++j) {
             T const sj = s0j[j - begin];
             T const dsj = s1j[j - begin] - sj;
-            T const tmp = -csd * (s0i * sj + T(0.5) * (dsi * sj + s0i * dsj) + (T(1.0) / T(3.0)) * dsj * dsi);
+            T const tmp = -csd * esirkepovWeight<C>(s0i, dsi, sj, dsj);
             T acc = T(0.0);
             for (int kk = begin; kk < end - 1; ++kk) {
-                T const W = (s1k[kk - begin] - s0k[kk - begin]) * tmp;
-                acc += W;
+                T const dsk = s1k[kk - begin] - s0k[kk - begin];
+                acc = C ? (kk == begin ? dsk * tmp : madd<C>(dsk, tmp, acc)) : acc + dsk * tmp;
                 int const ijk[3] = {i, j, kk};
                 int rel[3];
                 for (int a = 0; a < 3; ++a) rel[perm[a]] = ijk[a];
@@ -611,6 +644,23 @@ void emzCpt1D(int supp, const T* p0, const T* p1, T csd, int comp, const int* pe
             }
         }
     }
+}
+
+template <typename T, typename Add>
+void emzCpt1DAs(bool contract, int supp, const T* p0, const T* p1, T csd, int comp, const int* perm, Add add) {
+    if (contract)
+        emzCpt1D<true>(supp, p0, p1, csd, comp, perm, add);
+    else
+        emzCpt1D<false>(supp, p0, p1, csd, comp, perm, add);
+}
+
+template <typename T, typename Add>
+void cptCurrent1DAs(bool contract, int supp, const int* st, const T* p0, const T* p1, T csd, int comp,
+                    const int* perm, Add add) {
+    if (contract)
+        cptCurrent1D<true>(supp, st, p0, p1, csd, comp, perm, add);
+    else
+        cptCurrent1D<false>(supp, st, p0, p1, csd, comp, perm, add);
 }
 
 // EmZ (currentDeposition/EmZ/EmZ.hpp:61-137): the move splits at the relay
@@ -639,16 +689,16 @@ void emz(int supp, const T* pos, const V3<T>& vel, T charge, const Consts<T>& k,
         {  // J_x: rotateOrigin<1,2,0>
             int const perm[3] = {1, 2, 0};
             T const q0[3] = {a[1], a[2], a[0]}, q1[3] = {b[1], b[2], b[0]};
-            emzCpt1D(supp, q0, q1, k.cell[0] * chargeDensity / k.dt, 0, perm, shifted);
+            emzCpt1DAs(k.contract, supp, q0, q1, k.cell[0] * chargeDensity / k.dt, 0, perm, shifted);
         }
         {  // J_y: rotateOrigin<2,0,1>
             int const perm[3] = {2, 0, 1};
             T const q0[3] = {a[2], a[0], a[1]}, q1[3] = {b[2], b[0], b[1]};
-            emzCpt1D(supp, q0, q1, k.cell[1] * chargeDensity / k.dt, 1, perm, shifted);
+            emzCpt1DAs(k.contract, supp, q0, q1, k.cell[1] * chargeDensity / k.dt, 1, perm, shifted);
         }
         {  // J_z
             int const perm[3] = {0, 1, 2};
-            emzCpt1D(supp, a, b, k.cell[2] * chargeDensity / k.dt, 2, perm, shifted);
+            emzCpt1DAs(k.contract, supp, a, b, k.cell[2] * chargeDensity / k.dt, 2, perm, shifted);
         }
     };
     T a[3], b[3];
@@ -701,19 +751,19 @@ void esirkepov(int supp, const T* pos, const V3<T>& vel, T charge, const Consts<
         T const q0[3] = {p0[1], p0[2], p0[0]}, q1[3] = {p1[1], p1[2], p1[0]};
         int const s3[3] = {st[1], st[2], st[0]};
         T const csd = charge * k.inv_vdt * k.cell[0];
-        cptCurrent1D(supp, s3, q0, q1, csd, 0, perm, shifted);
+        cptCurrent1DAs(k.contract, supp, s3, q0, q1, csd, 0, perm, shifted);
     }
     {  // J_y: rotateOrigin<2,0,1>
         int const perm[3] = {2, 0, 1};
         T const q0[3] = {p0[2], p0[0], p0[1]}, q1[3] = {p1[2], p1[0], p1[1]};
         int const s3[3] = {st[2], st[0], st[1]};
         T const csd = charge * k.inv_vdt * k.cell[1];
-        cptCurrent1D(supp, s3, q0, q1, csd, 1, perm, shifted);
+        cptCurrent1DAs(k.contract, supp, s3, q0, q1, csd, 1, perm, shifted);
     }
     {  // J_z
         int const perm[3] = {0, 1, 2};
         T const csd = charge * k.inv_vdt * k.cell[2];
-        cptCurrent1D(supp, st, p0, p1, csd, 2, perm, shifted);
+        cptCurrent1DAs(k.contract, supp, st, p0, p1, csd, 2, perm, shifted);
     }
 }
 
@@ -1914,6 +1964,7 @@ State<T>* create(const oracle_config& cfg, int order) {
     st->nsc = st->nscx * st->nscy * st->nscz;
     st->ncells = static_cast<size_t>(st->nx) * st->ny * st->nz;
     st->order = order;
+    st->k.contract = order == 0;
     st->anyOpen = cfg.boundary[0] || cfg.boundary[1] || cfg.boundary[2];
     for (int c = 0; c < 3; ++c) {
         st->E[c].assign(st->ncells, T(0));

@@ -144,18 +144,19 @@ __device__ __forceinline__ float gatherGlobalCIC(const float* __restrict__ F, co
     Shape<2>::onSupport(p[0], sx);
     Shape<2>::onSupport(p[1], sy);
     Shape<2>::onSupport(p[2], sz);
-    // partial sums start at their first term, as gather1
+    // partial sums start at their first term and add the others as
+    // multiply-adds, as gather1
     float result_z = 0.0f;
 #pragma unroll
     for (int z = 0; z < 2; ++z) {
         float result_y = 0.0f;
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
-            float const result_x = F[gidx(g, cx + is[0], cy + is[1] + y, cz + is[2] + z)] * sx[0] +
-                                   F[gidx(g, cx + is[0] + 1, cy + is[1] + y, cz + is[2] + z)] * sx[1];
-            result_y = y == 0 ? result_x * sy[0] : result_y + result_x * sy[y];
+            float const result_x = mad(F[gidx(g, cx + is[0] + 1, cy + is[1] + y, cz + is[2] + z)], sx[1],
+                                       F[gidx(g, cx + is[0], cy + is[1] + y, cz + is[2] + z)] * sx[0]);
+            result_y = y == 0 ? result_x * sy[0] : mad(result_x, sy[y], result_y);
         }
-        result_z = z == 0 ? result_y * sz[0] : result_z + result_y * sz[z];
+        result_z = z == 0 ? result_y * sz[0] : mad(result_y, sz[z], result_z);
     }
     return result_z;
 }
@@ -196,13 +197,13 @@ __device__ __forceinline__ void cptGlobal(unsigned long long* __restrict__ J, co
                 if (j < end + leaveJ) {
                     float const sj0 = s0j[j - begin];
                     float const dsj = s1j[j - begin] - sj0;
-                    float const tmp = -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
+                    float const tmp = -csd * esirkepovWeight(s0i, dsi, sj0, dsj);
                     float acc = 0.0f;
 #pragma unroll
                     for (int k = begin; k < end; ++k) {
                         if (k < end + leaveK - 1) {
-                            float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
-                            acc += W;
+                            float const dsk = s1k[k - begin] - s0k[k - begin];
+                            acc = k == begin ? dsk * tmp : mad(dsk, tmp, acc);
                             unsigned long long* const at = J + (off[AI][i] + off[AJ][j] + off[AK][k]);
                             if (LDS)
                                 __hip_atomic_fetch_add(at, toFixed(acc, jscale), __ATOMIC_RELAXED,

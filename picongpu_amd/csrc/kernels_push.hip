@@ -362,11 +362,13 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     }
 }
 
-// PQS: at most 128 VGPRs, so two workgroups share a CU (their LDS fits);
-// the compiler spills a few registers for it and the launch takes 0.63x
-// the time of one workgroup per CU
+// Waves per SIMD the register budget must allow: PQS at most 128 VGPRs, so
+// two workgroups share a CU (their LDS fits); TSC at most 80, so three do
+// (3 x 53.6 KB of LDS fit the CU's 160 KB): the compiler spills a few values
+// and the launch is faster, the LDS bank conflicts of the gather and the
+// deposit being hidden better by the extra waves
 template <int SUPP, int PUSHER, int MODE, bool EMZ = false>
-__global__ __launch_bounds__(kPushThreads, SUPP == 4 ? 4 : 1) void kPushDeposit(PushArgs a) {
+__global__ __launch_bounds__(kPushThreads, SUPP == 4 ? 4 : (SUPP == 3 ? 6 : 1)) void kPushDeposit(PushArgs a) {
     __shared__ __align__(16) unsigned char smem[PushLds<SUPP, MODE>::BYTES];
     pushDepositSC<SUPP, PUSHER, MODE, EMZ>(a, a.scList[blockIdx.x], smem);
 }

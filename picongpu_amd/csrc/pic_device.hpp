@@ -1,9 +1,12 @@
 // Device-side physics of the PIC cycle for gfx950.
 //
 // Every function here reproduces the floating-point operation order of the
-// reference functor it replaces (cited per function), so that results are
-// bit-identical to the reference's CPU path when compiled with
-// -ffp-contract=off and IEEE division/sqrt (the hipcc defaults we keep).
+// reference functor it replaces (cited per function); the code is compiled
+// with -ffp-contract=off and IEEE division/sqrt (the hipcc defaults we keep),
+// so the only multiply-adds are the ones written as `mad` below: the field
+// gather's sums and Esirkepov's W and prefix sums, the places where the
+// reference's GPU build (nvcc contracts by default) fuses them too. Elsewhere
+// the results are bit-identical to the reference's CPU path.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,6 +33,18 @@ __device__ __forceinline__ float l2norm2(Vec3 v) {
     t += v.y * v.y;
     t += v.z * v.z;
     return t;
+}
+
+// a * b + c rounded once: the device order's contracted multiply-add
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Esirkepov's weight S0i S0j + 1/2 (dSi S0j + S0i dSj) + 1/3 dSj dSi
+// (Esirkepov.hpp:218) in the reference's order, each addition fused with the
+// product it adds (the 1/2 product is exact, so that fusion rounds nothing)
+__device__ __forceinline__ float esirkepovWeight(float s0i, float dsi, float s0j, float dsj) {
+    float const mixed = mad(dsi, s0j, s0i * dsj);
+    float const half = mad(0.5f, mixed, s0i * s0j);
+    return mad((1.0f / 3.0f) * dsj, dsi, half);
 }
 
 // cross product (pmacc/math/vector/Vector.tpp:71-76)
@@ -211,7 +226,8 @@ __device__ __forceinline__ float gather1(const float* __restrict__ tile, int cx,
     int const bz = cz + ishift[2] + begin;
     // each partial sum starts at its first term; the reference adds that term
     // to 0.0, which differs only in the sign of an all-zero sum (a zero field
-    // value adds nothing to a momentum, whatever its sign)
+    // value adds nothing to a momentum, whatever its sign). Every further
+    // term is a multiply-add.
     float result_z = 0.0f;
 #pragma unroll
     for (int z = 0; z < SUPP; ++z) {
@@ -221,10 +237,10 @@ __device__ __forceinline__ float gather1(const float* __restrict__ tile, int cx,
             const float* row = tile + ((bz + z) * TY + (by + y)) * TX + bx;
             float result_x = row[0] * sx[0];
 #pragma unroll
-            for (int x = 1; x < SUPP; ++x) result_x += row[x] * sx[x];
-            result_y = y == 0 ? result_x * sy[0] : result_y + result_x * sy[y];
+            for (int x = 1; x < SUPP; ++x) result_x = mad(row[x], sx[x], result_x);
+            result_y = y == 0 ? result_x * sy[0] : mad(result_x, sy[y], result_y);
         }
-        result_z = z == 0 ? result_y * sz[0] : result_z + result_y * sz[z];
+        result_z = z == 0 ? result_y * sz[0] : mad(result_y, sz[z], result_z);
     }
     return result_z;
 }
@@ -560,16 +576,16 @@ struct Esirkepov {
                     if (j < end + leaveCellJ) {
                         float const sj0 = s0j[j - begin];
                         float const dsj = s1j[j - begin] - sj0;
-                        float const tmp =
-                            -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
+                        float const tmp = -csd * esirkepovWeight(s0i, dsi, sj0, dsj);
                         float accumulated_J = 0.0f;
 #pragma unroll
                         for (int k = begin; k < end; ++k) {
                             if (k < end + leaveCellK - 1) {
-                                float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
+                                float const dsk = s1k[k - begin] - s0k[k - begin];
                                 // 0 + W differs from W only in the sign of a zero, which the
                                 // fixed-point conversion does not see: start the sum at W
-                                accumulated_J = k == begin ? W : accumulated_J + W;
+                                // and add every further W = dsk * tmp as a multiply-add
+                                accumulated_J = k == begin ? dsk * tmp : mad(dsk, tmp, accumulated_J);
                                 __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
                                                        toFixedBits(accumulated_J, jscale), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -665,12 +681,11 @@ struct Esirkepov {
             for (int j = begin; j < end; ++j) {
                 float const sj0 = aj.s0[j - begin];
                 float const dsj = aj.s1[j - begin] - sj0;
-                float const tmp = -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
+                float const tmp = -csd * esirkepovWeight(s0i, dsi, sj0, dsj);
                 float accumulated_J = 0.0f;
 #pragma unroll
                 for (int k = begin; k < end - 1; ++k) {
-                    float const W = dsk[k - begin] * tmp;
-                    accumulated_J = k == begin ? W : accumulated_J + W;
+                    accumulated_J = k == begin ? dsk[k - begin] * tmp : mad(dsk[k - begin], tmp, accumulated_J);
                     __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
                                            toFixedBits(accumulated_J, jscale), __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -776,12 +791,12 @@ struct EmZ {
             for (int j = begin; j < end; ++j) {
                 float const sj0 = s0j[j - begin];
                 float const dsj = s1j[j - begin] - sj0;
-                float const tmp = -csd * (s0i * sj0 + 0.5f * (dsi * sj0 + s0i * dsj) + (1.0f / 3.0f) * dsj * dsi);
+                float const tmp = -csd * esirkepovWeight(s0i, dsi, sj0, dsj);
                 float accumulated_J = 0.0f;
 #pragma unroll
                 for (int k = begin; k < end - 1; ++k) {
-                    float const W = (s1k[k - begin] - s0k[k - begin]) * tmp;
-                    accumulated_J = k == begin ? W : accumulated_J + W;  // as in Esirkepov::cpt1D
+                    float const dsk = s1k[k - begin] - s0k[k - begin];
+                    accumulated_J = k == begin ? dsk * tmp : mad(dsk, tmp, accumulated_J);  // as in Esirkepov::cpt1D
                     __hip_atomic_fetch_add(jt + ((i - begin) * SI + (j - begin) * SJ + (k - begin) * SK),
                                            toFixedBits(accumulated_J, jscale), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }

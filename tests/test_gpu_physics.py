@@ -40,8 +40,11 @@ def global_state(st, frames, ids, cap, nsc):
 
 
 def test_against_sequential_order_one_step():
-    """One step: particles bit-exact (the push does not see this step's J),
-    J within 1e-5 (max-norm) of the reference CPU's sequential FP32 sums."""
+    """One step against the reference CPU's arithmetic and sequential FP32
+    sums: the push does not see this step's J, so the particles differ only
+    by the device's multiply-adds in the field gather (a few ulp: in-cell
+    positions within 4 * 2^-24, momenta within 1e-6 relative, the same cells
+    and slots); J within 1e-5 (max-norm)."""
     st = small_thermal(abi.SHAPE_TSC)
     print(f"case sequential-order grid={st.grid}", flush=True)
     sim, ora, frames = H.make_pair(st, order=1)
@@ -49,7 +52,14 @@ def test_against_sequential_order_one_step():
     ora.step(1)
     g = sim.read_frames(0)
     o = ora.get_frames(0)
-    H.assert_frames_equal(g, o, "after one step")
+    np.testing.assert_array_equal(g["counts"], o["counts"])
+    v = H.occupied(g)
+    np.testing.assert_array_equal(g["cell"][v], o["cell"][v])
+    np.testing.assert_array_equal(g["mask"][v], o["mask"][v])
+    dpos = float(np.abs(g["pos"][:, v].astype(np.float64) - o["pos"][:, v]).max())
+    dmom = rel_max(g["mom"][:, v], o["mom"][:, v])
+    print(f"particles vs sequential order: in-cell position {dpos:.3e}, momentum {dmom:.3e}")
+    assert dpos <= 4 * 2.0 ** -24 and dmom < 1e-6
     rJ = rel_max(sim.read_field(abi.FIELD_J), ora.get_field(abi.FIELD_J))
     rE = rel_max(sim.read_field(abi.FIELD_E), ora.get_field(abi.FIELD_E))
     print("J vs sequential order:", rJ, "E:", rE)
