@@ -1,4 +1,5 @@
-"""FP64 tape-based adjoint of the differentiable PIC scheme (CPU, PyTorch autograd).
+"""FP64 tape-based adjoint of the differentiable PIC scheme (PyTorch autograd,
+on the device of the tensors it is given).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference gradient of the
 HIP adjoint. Never imported by the picongpu_amd package.
@@ -64,7 +65,7 @@ def gather(g: Grid, F, X, field_pos):
             p = v - s + 0.5
             w.append((1.0 - p, p))
             idx_base.append(cell[:, d].long() + s.long())
-        acc = torch.zeros(X.shape[0], dtype=F64)
+        acc = torch.zeros(X.shape[0], dtype=F64, device=X.device)
         for a in range(2):
             for b in range(2):
                 for e in range(2):
@@ -91,17 +92,17 @@ def velocity(g: Grid, p):
 
 def deposit(g: Grid, X1, v):
     """Esirkepov CIC current of each particle moving from X1 - v dt/dx to X1."""
-    cellv = torch.tensor(g.cell, dtype=F64)
+    cellv = torch.tensor(g.cell, dtype=F64, device=X1.device)
     X0 = X1 - v * g.dt / cellv
     gs = torch.minimum(torch.floor(X0), torch.floor(X1)).detach()
     p0 = X0 - gs
     p1 = X1 - gs
     n = X1.shape[0]
-    nodes = torch.arange(3, dtype=F64)
+    nodes = torch.arange(3, dtype=F64, device=X1.device)
     S0 = [cic(nodes[None, :] - p0[:, d:d + 1]) for d in range(3)]  # (n,3) weights at nodes gs+0..2
     S1 = [cic(nodes[None, :] - p1[:, d:d + 1]) for d in range(3)]
     DS = [S1[d] - S0[d] for d in range(3)]
-    J = torch.zeros(3, g.nx * g.ny * g.nz, dtype=F64)
+    J = torch.zeros(3, g.nx * g.ny * g.nz, dtype=F64, device=X1.device)
     base = gs.long()
     for comp in range(3):
         a, b = [x for x in range(3) if x != comp]
@@ -155,7 +156,7 @@ def step(g: Grid, X, p, E, B):
     Bp = gather(g, B, X, POS_B)
     p1 = boris(g, p, Ep, Bp)
     v = velocity(g, p1)
-    cellv = torch.tensor(g.cell, dtype=F64)
+    cellv = torch.tensor(g.cell, dtype=F64, device=X.device)
     X1 = X + v * g.dt / cellv
     J = deposit(g, X1, v)
     B = B - 0.5 * g.dt * curl_fwd(g, E)

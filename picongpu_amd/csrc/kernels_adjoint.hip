@@ -18,7 +18,7 @@
 #include "pic_device.hpp"
 #include "pic_kernels.hpp"
 
-#include <hipcub/device/device_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 namespace pic {
 
@@ -843,9 +843,11 @@ __global__ __launch_bounds__(256) void kScatterAdjoint(const int32_t* __restrict
     for (int d = 0; d < 3; ++d) dst[d * n + i] = src[d * n + j];
 }
 
+// rocPRIM's stable LSD radix sort of (tile key, slot) pairs: equal keys keep
+// their slot order, so the permutation is deterministic
 hipError_t sortTempBytes(int64_t n, size_t* bytes) {
-    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              (const int32_t*)nullptr, (int32_t*)nullptr, (int)n, 0, 32, nullptr);
+    return rocprim::radix_sort_pairs(nullptr, *bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                     (const int32_t*)nullptr, (int32_t*)nullptr, (size_t)n, 0u, 32u, nullptr);
 }
 
 hipError_t launchTileSort(const int32_t* cell, int64_t n, const GridDev& g, uint32_t* keys[2], int32_t* vals[2],
@@ -858,7 +860,7 @@ hipError_t launchTileSort(const int32_t* cell, int64_t n, const GridDev& g, uint
     int bits = 1;
     while (bits < 32 && (1u << bits) < tiles) ++bits;
     size_t b = tempBytes;
-    e = hipcub::DeviceRadixSort::SortPairs(temp, b, keys[0], keys[1], vals[0], vals[1], (int)n, 0, bits, s);
+    e = rocprim::radix_sort_pairs(temp, b, keys[0], keys[1], vals[0], vals[1], (size_t)n, 0u, (unsigned)bits, s);
     *perm = vals[1];
     return e;
 }
