@@ -404,6 +404,22 @@ int pic_timing_enable(pic_ctx* ctx, int32_t enable);
 int pic_kernel_timer(pic_ctx* ctx, int32_t timer, double* total_ms, int64_t* launches);
 int32_t pic_j_scale_exp(const pic_ctx* ctx);
 int32_t pic_num_push_phases(const pic_ctx* ctx, int32_t fused);
+/* Launch schedule of the fused push/deposit of a step (no counterpart in the
+ * reference, whose deposit strategy launches per checkerboard colour
+ * (Strategy.def:196-200): an execution knob of this library). PHASES: one
+ * launch per checkerboard phase, each workgroup adding its supercell's tile
+ * to J. SINGLE: one launch for every supercell, the tiles left in a scratch
+ * of one tile per supercell and added to J by a gather launch, every J cell
+ * adding its covering tiles in phase order. Both give the same J, bit for
+ * bit; SINGLE fills the GPU when a phase holds fewer supercells than the
+ * device has workgroup slots. AUTO (the default) takes SINGLE when the
+ * largest phase has fewer supercells than twice the compute units.
+ * pic_push_schedule reports the schedule of a shape support (2, 3, 4). */
+#define PIC_PUSH_SCHEDULE_AUTO 0
+#define PIC_PUSH_SCHEDULE_PHASES 1
+#define PIC_PUSH_SCHEDULE_SINGLE 2
+int pic_set_push_schedule(pic_ctx* ctx, int32_t schedule);
+int32_t pic_push_schedule(const pic_ctx* ctx, int32_t support);
 
 /* --- differentiable form and its adjoint ------------------------------- */
 /* The reference has no adjoint; these entry points expose the same step

@@ -30,6 +30,7 @@ ORIGIN_ZERO, ORIGIN_CENTER = 0, 1
 INCIDENT_MAX_MODES = 16
 FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
 TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2
+PUSH_SCHEDULE_AUTO, PUSH_SCHEDULE_PHASES, PUSH_SCHEDULE_SINGLE = 0, 1, 2
 
 ERR = {0: "ok", -1: "invalid", -2: "hip", -3: "capacity", -4: "state"}
 FLAG_CAPACITY_GREW = 1  # PIC_FLAG_CAPACITY_GREW: a supercell overflowed and the supercells grew
@@ -229,6 +230,8 @@ SIGNATURES = {
     "pic_kernel_timer": (C.c_int, [_P, C.c_int32, _D, C.POINTER(C.c_int64)]),
     "pic_j_scale_exp": (C.c_int32, [_P]),
     "pic_num_push_phases": (C.c_int32, [_P, C.c_int32]),
+    "pic_set_push_schedule": (C.c_int, [_P, C.c_int32]),
+    "pic_push_schedule": (C.c_int32, [_P, C.c_int32]),
     "pic_adjoint_load_particles": (C.c_int, [_P, C.c_int32, C.c_int64, _I32, _F, _F, _F]),
     "pic_adjoint_read_particles": (C.c_int, [_P, _I32, _F, _F]),
     "pic_adjoint_forward": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P]),
@@ -570,6 +573,13 @@ class Simulation:
 
     def num_push_phases(self, fused=True) -> int:
         return self.lib.pic_num_push_phases(self.h, 1 if fused else 0)
+
+    def set_push_schedule(self, schedule: int) -> None:
+        """PUSH_SCHEDULE_AUTO, _PHASES or _SINGLE for the fused push."""
+        check(self.lib.pic_set_push_schedule(self.h, int(schedule)), "pic_set_push_schedule")
+
+    def push_schedule(self, support: int) -> int:
+        return self.lib.pic_push_schedule(self.h, int(support))
 
     # --- differentiable form (device buffers are passed as integer addresses) ---
     def adjoint_load_particles(self, sp, cell, pos, mom, w):

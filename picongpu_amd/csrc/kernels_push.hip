@@ -145,6 +145,8 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             for (int s = 0; s < a.nsp; ++s)
                 if (a.sp[s].lists) a.sp[s].outOff[(size_t)sc * kOffStride + tid] = 0u;
         }
+        if (MODE == 1 && a.tileOut)  // an empty tile for the gather flush
+            for (int t = tid; t < J_ELEMS; t += blockDim.x) a.tileOut[(size_t)sc * J_ELEMS + t] = 0ull;
         return;
     }
 
@@ -311,7 +313,10 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
             a.J[c][gi] += v;
         }
     };
-    if (MODE != 0) {
+    if (MODE == 1 && a.tileOut) {
+        // one launch for every supercell: the raw tile is left for the gather flush
+        for (int t = tid; t < J_ELEMS; t += blockDim.x) a.tileOut[(size_t)sc * J_ELEMS + t] = sJ[t];
+    } else if (MODE != 0) {
         if (a.g.open)
             flushJ(std::true_type{});
         else
@@ -429,6 +434,107 @@ static hipError_t launchS(int pusher, int mode, const PushArgs& a, int nblocks, 
     if (pusher == PIC_PUSHER_VAY) return launchT<SUPP, PIC_PUSHER_VAY, 1, EMZ>(a, nblocks, s);
     if (pusher == PIC_PUSHER_HIGUERA_CARY) return launchT<SUPP, PIC_PUSHER_HIGUERA_CARY, 1, EMZ>(a, nblocks, s);
     return launchT<SUPP, PIC_PUSHER_BORIS, 1, EMZ>(a, nblocks, s);
+}
+
+size_t pushTileElems(int supp) {
+    switch (supp) {
+        case 2: return PushLds<2, 1>::J_ELEMS;
+        case 3: return PushLds<3, 1>::J_ELEMS;
+        case 4: return PushLds<4, 1>::J_ELEMS;
+        default: return 0;
+    }
+}
+
+// The supercells whose J tile of component c covers global cell g along axis
+// a (tile origin at the supercell's origin minus DLO, extent ext): the
+// candidates' supercell index along the axis (wrapped on a periodic axis)
+// and the cell's tile coordinate, ordered by the axis' checkerboard colour
+// (colour[s]). Returns their number (at most 3).
+template <int SUPP>
+__device__ __forceinline__ int coveringAlong(int g, int scSize, int ext, int nsc, bool open,
+                                             const uint8_t* __restrict__ colour, int* scOut, int* tOut) {
+    constexpr int DLO = Esirkepov<SUPP>::lowerMargin;
+    int n = 0;
+    int col[3];
+    int const hi = (g + DLO) >= 0 ? (g + DLO) / scSize : -((-(g + DLO) + scSize - 1) / scSize);  // floor
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int const s = hi - k;
+        int const t = g + DLO - s * scSize;
+        if (t < 0 || t >= ext) continue;
+        int sw = s;
+        if (open) {
+            if (s < 0 || s >= nsc) continue;
+        } else {
+            sw = ((s % nsc) + nsc) % nsc;
+        }
+        int const cs = colour[sw];
+        int m = n++;
+        for (; m > 0 && col[m - 1] > cs; --m) {
+            col[m] = col[m - 1];
+            scOut[m] = scOut[m - 1];
+            tOut[m] = tOut[m - 1];
+        }
+        col[m] = cs;
+        scOut[m] = sw;
+        tOut[m] = t;
+    }
+    return n;
+}
+
+// One thread per J cell: for each component the covering supercells' tile
+// values, added as the flush adds them, in ascending checkerboard phase. A
+// phase's index is colour_x + ncol_x (colour_y + ncol_y colour_z), so
+// visiting z, then y, then x candidates each in ascending colour is
+// ascending phase (every supercell covering a cell lies in another phase).
+template <int SUPP>
+__global__ __launch_bounds__(256) void kGatherFlush(GridDev g, const unsigned long long* __restrict__ tiles,
+                                                     const uint8_t* __restrict__ colour, float* J0, float* J1,
+                                                     float* J2, double jinv) {
+    using TG = JTileGeom<Esirkepov<SUPP>::lowerMargin, Esirkepov<SUPP>::upperMargin>;
+    constexpr int J_ELEMS = PushLds<SUPP, 1>::J_ELEMS;
+    if (halted(g)) return;
+    size_t const n = (size_t)g.nx * g.ny * g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int const x = (int)(i % g.nx);
+    int const y = (int)((i / g.nx) % g.ny);
+    int const z = (int)(i / ((size_t)g.nx * g.ny));
+    float* const Jc[3] = {J0, J1, J2};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        int sx[3], tx[3], sy[3], ty[3], sz[3], tz[3];
+        int const nx = coveringAlong<SUPP>(x, SCX, TG::ext(c, 0), g.nscx, (g.open & 1) != 0, colour, sx, tx);
+        int const ny = coveringAlong<SUPP>(y, SCY, TG::ext(c, 1), g.nscy, (g.open & 2) != 0, colour + g.nscx, sy, ty);
+        int const nz = coveringAlong<SUPP>(z, SCZ, TG::ext(c, 2), g.nscz, (g.open & 4) != 0,
+                                           colour + g.nscx + g.nscy, sz, tz);
+        float v = Jc[c][i];
+        bool any = false;
+        for (int a = 0; a < nz; ++a)
+            for (int b = 0; b < ny; ++b)
+                for (int d = 0; d < nx; ++d) {
+                    int const sc = sx[d] + g.nscx * (sy[b] + g.nscy * sz[a]);
+                    long long const acc = fixedTileValue(
+                        tiles[(size_t)sc * J_ELEMS + TG::off(c) + (tz[a] * TG::ext(c, 1) + ty[b]) * TG::ext(c, 0) + tx[d]]);
+                    if (acc == 0) continue;
+                    v += (float)((double)acc * jinv);
+                    any = true;
+                }
+        if (any) Jc[c][i] = v;
+    }
+}
+
+hipError_t launchGatherFlush(int supp, const GridDev& g, const unsigned long long* tiles, const uint8_t* colour,
+                             float* const J[3], double jinv, hipStream_t s) {
+    size_t const n = (size_t)g.nx * g.ny * g.nz;
+    dim3 const grid((unsigned)((n + 255) / 256));
+    switch (supp) {
+        case 2: hipLaunchKernelGGL(kGatherFlush<2>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2], jinv); break;
+        case 3: hipLaunchKernelGGL(kGatherFlush<3>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2], jinv); break;
+        case 4: hipLaunchKernelGGL(kGatherFlush<4>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2], jinv); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launchPush(int supp, int pusher, int mode, int currentSolver, const PushArgs& a, int nblocks,

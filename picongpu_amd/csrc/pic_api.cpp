@@ -568,6 +568,12 @@ struct PhasePlan {
     std::vector<int> count;   // per phase
     std::vector<int> host;    // the list on the host, launch order per phase
     int32_t* dList = nullptr;
+    // fused plans: every supercell in one launch, the tiles added to J by the
+    // gather flush in phase order (same J, bit for bit), instead of one
+    // launch per phase; chosen when the phases are too small to fill the GPU
+    bool single = false;
+    unsigned long long* tiles = nullptr;  // nsc tiles
+    uint8_t* colour = nullptr;            // checkerboard colour per supercell index along x, y, z
 };
 
 struct SpeciesMem {
@@ -669,6 +675,8 @@ struct pic_ctx {
     int jexp = 40;
     // [supp-2][fused]
     PhasePlan plans[3][2];
+    int cus = 256;          // compute units of the device
+    int pushSchedule = 0;   // PIC_PUSH_SCHEDULE_*
     int32_t* dAll = nullptr;
     // timing
     bool timing = false;
@@ -793,6 +801,22 @@ int buildPlan(pic_ctx* c, int supp, bool fused, PhasePlan& plan) {
     plan.host = list;
     if (devAlloc(c, &plan.dList, list.size()) != PIC_OK) return PIC_E_HIP;
     PIC_HIP(hipMemcpyAsync(plan.dList, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    // one launch for every supercell when the largest phase would leave more
+    // than half of the CUs' workgroup slots idle (small boxes)
+    int const most = *std::max_element(plan.count.begin(), plan.count.end());
+    plan.single = fused && (c->pushSchedule == PIC_PUSH_SCHEDULE_SINGLE ||
+                            (c->pushSchedule == PIC_PUSH_SCHEDULE_AUTO && most < 2 * c->cus));
+    if (plan.single) {
+        std::vector<uint8_t> colour;
+        for (int d = 0; d < 3; ++d) {
+            if (plan.ncol[d] > 255) return fail(PIC_E_INVALID, "more than 255 checkerboard colours");
+            for (int v : col[d]) colour.push_back(static_cast<uint8_t>(v));
+        }
+        if (devAlloc(c, &plan.tiles, static_cast<size_t>(c->nsc) * pic::pushTileElems(supp), false) != PIC_OK ||
+            devAlloc(c, &plan.colour, colour.size(), false) != PIC_OK)
+            return PIC_E_HIP;
+        PIC_HIP(hipMemcpyAsync(plan.colour, colour.data(), colour.size(), hipMemcpyHostToDevice, c->stream));
+    }
     PIC_HIP(hipStreamSynchronize(c->stream));
     plan.valid = true;
     return PIC_OK;
@@ -931,6 +955,16 @@ int runPush(pic_ctx* c, int32_t species, int mode, bool stageShift = false) {
         PhasePlan* plan = nullptr;
         int const r = getPlan(c, supp, mode == 1, &plan);
         if (r != PIC_OK) return r;
+        if (plan->single) {
+            // every supercell in one launch (phase-ordered list), then the
+            // tiles gathered into J in phase order
+            a.scList = plan->dList;
+            a.tileOut = plan->tiles;
+            TimerScope ts(c, PIC_TIMER_PUSH);
+            PIC_HIP(pic::launchPush(supp, pusher, mode, scheme, a, c->nsc, c->stream));
+            PIC_HIP(pic::launchGatherFlush(supp, c->g, plan->tiles, plan->colour, c->J, a.jinv, c->stream));
+            continue;
+        }
         for (size_t ph = 0; ph < plan->count.size(); ++ph) {
             a.scList = plan->dList + plan->offset[ph];
             TimerScope ts(c, PIC_TIMER_PUSH);
@@ -1274,6 +1308,11 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
     if (e != hipSuccess) return bail(fail(PIC_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return bail(fail(PIC_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+    {
+        hipDeviceProp_t prop{};
+        if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->cus = prop.multiProcessorCount;
+    }
     for (int d = 0; d < 3; ++d) {
         if (devAlloc(c, &c->E[d], c->ncells) || devAlloc(c, &c->B[d], c->ncells) || devAlloc(c, &c->J[d], c->ncells) ||
             devAlloc(c, &c->Ealt[d], c->ncells))
@@ -1403,6 +1442,33 @@ int pic_supercell_slots(const pic_ctx* c, int32_t s, uint32_t* out) {
 }
 
 int32_t pic_j_scale_exp(const pic_ctx* c) { return c ? c->jexp : 0; }
+
+int pic_set_push_schedule(pic_ctx* c, int32_t schedule) {
+    if (!c) return fail(PIC_E_INVALID, "null context");
+    if (schedule < PIC_PUSH_SCHEDULE_AUTO || schedule > PIC_PUSH_SCHEDULE_SINGLE)
+        return fail(PIC_E_INVALID, "unknown push schedule");
+    if (schedule == c->pushSchedule) return PIC_OK;
+    PIC_HIP(hipSetDevice(c->device));
+    PIC_HIP(hipStreamSynchronize(c->stream));
+    c->pushSchedule = schedule;
+    dropGraphs(c);  // captured launches follow the old plans
+    for (int supp = 0; supp < 3; ++supp) {
+        PhasePlan& p = c->plans[supp][1];
+        if (!p.valid) continue;
+        devFree(c, p.dList);
+        devFree(c, p.tiles);
+        devFree(c, p.colour);
+        p = PhasePlan{};
+    }
+    return PIC_OK;
+}
+
+int32_t pic_push_schedule(const pic_ctx* c, int32_t support) {
+    if (!c || support < 2 || support > 4) return -1;
+    PhasePlan* p = nullptr;
+    if (getPlan(const_cast<pic_ctx*>(c), support, true, &p) != PIC_OK) return -1;
+    return p->single ? PIC_PUSH_SCHEDULE_SINGLE : PIC_PUSH_SCHEDULE_PHASES;
+}
 
 int32_t pic_num_push_phases(const pic_ctx* c, int32_t fused) {
     if (!c) return -1;

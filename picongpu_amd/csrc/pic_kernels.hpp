@@ -83,6 +83,10 @@ struct PushArgs {
     SpeciesDev sp[PIC_MAX_SPECIES];
     const int32_t* scList;  // supercells of this launch, in launch order
     int stageShift;  // 1: the epilogue also moves the leavers to the outbox and compacts the supercell
+    // non-null (fused push + deposit only): every supercell in one launch,
+    // each leaving its raw fixed-point J tile at tileOut[sc * tile elements]
+    // for launchGatherFlush instead of adding it to J
+    unsigned long long* tileOut;
 };
 
 struct FieldArgs {
@@ -101,6 +105,16 @@ struct FieldArgs {
 // currentSolver: PIC_CURRENT_SOLVER_* of the launch's species (one per launch)
 hipError_t launchPush(int supp, int pusher, int mode, int currentSolver, const PushArgs& a, int nblocks,
                       hipStream_t s);
+// elements of the fused kernel's J tile for a shape support
+size_t pushTileElems(int supp);
+// J += the tiles of a one-launch fused push (PushArgs::tileOut), each J cell
+// adding its covering supercells' tile values in the order of their
+// checkerboard phases: the float additions, in the same order, of the
+// phase-by-phase flush, so J is the same bit for bit. colour: the
+// checkerboard colour of every supercell index along x, then y, then z
+// (nscx + nscy + nscz entries).
+hipError_t launchGatherFlush(int supp, const GridDev& g, const unsigned long long* tiles, const uint8_t* colour,
+                             float* const J[3], double jinv, hipStream_t s);
 // shift of one species: `staged` = the push epilogue already moved the leavers
 // to the outbox and compacted every supercell (then only the insert runs).
 // redo: the insert again after the supercells grew, from count - incoming

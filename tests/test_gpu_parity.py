@@ -222,12 +222,19 @@ STEP_CASES = [
 ]
 
 
+@pytest.mark.parametrize("schedule", [abi.PUSH_SCHEDULE_PHASES, abi.PUSH_SCHEDULE_SINGLE], ids=["phases", "single"])
 @pytest.mark.parametrize("name,make", STEP_CASES, ids=[c[0] for c in STEP_CASES])
-def test_steps_bit_exact(name, make):
+def test_steps_bit_exact(name, make, schedule):
+    """Ten full steps against the oracle, the fused push launched per
+    checkerboard phase or in one launch with the gather flush (the oracle
+    has one order for both)."""
     st = make()
     nsteps = 10
-    print(f"case steps {name} grid={st.grid} steps={nsteps}", flush=True)
+    print(f"case steps {name} grid={st.grid} steps={nsteps} schedule={schedule}", flush=True)
     sim, ora, _ = H.make_pair(st, e_amp=1e-3, b_amp=1e-3)
+    sim.set_push_schedule(schedule)
+    supp = {abi.SHAPE_CIC: 2, abi.SHAPE_TSC: 3, abi.SHAPE_PQS: 4}[st.species[0].shape]
+    assert sim.push_schedule(supp) == schedule
     n0 = sum(int(sim.read_frames(sp)["counts"].sum()) for sp in range(len(st.species)))
     sim.step(nsteps)
     ora.step(nsteps)
