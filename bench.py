@@ -159,7 +159,8 @@ def ensemble_timing(steps=10, warmup=2):
     """Batched ensemble (64 two-stream boxes of 32..96 cells per axis, 2..16
     ppc, TSC, Boris, Yee, seed 7): particle-steps/s of the autotuned batched
     step replayed from graphs, against the same members advanced one at a time
-    with their own graph-replayed pic_step."""
+    with their own graph-replayed pic_step (automatic push schedule), and one
+    at a time with one push launch per checkerboard phase."""
     from picongpu_amd import abi, setups
 
     sts = setups.ensemble()
@@ -180,12 +181,31 @@ def ensemble_timing(steps=10, warmup=2):
     t_seq = time.perf_counter() - t0
     order, bpc = ens.schedule()
     ens.close()
+    # the same members one at a time with one push launch per checkerboard
+    # phase (the reference's per-colour launches; the automatic schedule
+    # above runs small members' pushes in one launch)
+    sims = []
+    for st in sts:
+        sim = abi.Simulation(st.config())
+        sim.set_push_schedule(abi.PUSH_SCHEDULE_PHASES)
+        setups.init_device(sim, st)
+        sims.append(sim)
+    for sim in sims:
+        sim.step_graph(warmup)
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.step_graph(steps)
+    t_phases = time.perf_counter() - t0
+    for sim in sims:
+        sim.close()
     names = {0: "sim-major", 1: "interleaved", 2: "heavy-first"}
     return {"workload": f"{len(sts)} two-stream boxes (32..96 cells per axis, 2..16 ppc, TSC, Boris, Yee, seed 7)",
             "particles": particles, "cells": int(sum(int(np.prod(st.grid)) for st in sts)), "steps": steps,
             "batched_ms_per_step": t_batch * 1e3 / steps, "one_at_a_time_ms_per_step": t_seq * 1e3 / steps,
             "batched_particle_steps_per_s": particles * steps / t_batch,
             "one_at_a_time_particle_steps_per_s": particles * steps / t_seq, "speedup": t_seq / t_batch,
+            "one_at_a_time_per_phase_ms_per_step": t_phases * 1e3 / steps,
+            "speedup_vs_per_phase": t_phases / t_batch,
             "schedule": {"order": names.get(order, order), "blocks_per_cu": bpc, "autotune_ms_per_step": tuned_ms,
                          "autotune_steps": taken}}
 

@@ -413,8 +413,10 @@ int32_t pic_num_push_phases(const pic_ctx* ctx, int32_t fused);
  * adding its covering tiles in phase order. Both give the same J, bit for
  * bit; SINGLE fills the GPU when a phase holds fewer supercells than the
  * device has workgroup slots. AUTO (the default) takes SINGLE when the
- * largest phase has fewer supercells than twice the compute units.
- * pic_push_schedule reports the schedule of a shape support (2, 3, 4). */
+ * largest phase has fewer supercells than twice the compute units. An
+ * ensemble member's schedule cannot change (PIC_E_INVALID): the ensemble's
+ * batched launches use its plans. pic_push_schedule reports the schedule of
+ * a shape support (2, 3, 4). */
 #define PIC_PUSH_SCHEDULE_AUTO 0
 #define PIC_PUSH_SCHEDULE_PHASES 1
 #define PIC_PUSH_SCHEDULE_SINGLE 2

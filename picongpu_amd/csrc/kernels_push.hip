@@ -488,15 +488,11 @@ __device__ __forceinline__ int coveringAlong(int g, int scSize, int ext, int nsc
 // visiting z, then y, then x candidates each in ascending colour is
 // ascending phase (every supercell covering a cell lies in another phase).
 template <int SUPP>
-__global__ __launch_bounds__(256) void kGatherFlush(GridDev g, const unsigned long long* __restrict__ tiles,
-                                                     const uint8_t* __restrict__ colour, float* J0, float* J1,
-                                                     float* J2, double jinv) {
+__device__ __forceinline__ void gatherFlushCell(const GridDev& g, const unsigned long long* __restrict__ tiles,
+                                                const uint8_t* __restrict__ colour, float* J0, float* J1, float* J2,
+                                                double jinv, size_t i) {
     using TG = JTileGeom<Esirkepov<SUPP>::lowerMargin, Esirkepov<SUPP>::upperMargin>;
     constexpr int J_ELEMS = PushLds<SUPP, 1>::J_ELEMS;
-    if (halted(g)) return;
-    size_t const n = (size_t)g.nx * g.ny * g.nz;
-    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     int const x = (int)(i % g.nx);
     int const y = (int)((i / g.nx) % g.ny);
     int const z = (int)(i / ((size_t)g.nx * g.ny));
@@ -522,6 +518,39 @@ __global__ __launch_bounds__(256) void kGatherFlush(GridDev g, const unsigned lo
                 }
         if (any) Jc[c][i] = v;
     }
+}
+
+template <int SUPP>
+__global__ __launch_bounds__(256) void kGatherFlush(GridDev g, const unsigned long long* __restrict__ tiles,
+                                                     const uint8_t* __restrict__ colour, float* J0, float* J1,
+                                                     float* J2, double jinv) {
+    if (halted(g)) return;
+    size_t const n = (size_t)g.nx * g.ny * g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) gatherFlushCell<SUPP>(g, tiles, colour, J0, J1, J2, jinv, i);
+}
+
+template <int SUPP>
+__global__ __launch_bounds__(256) void kGatherFlushBatch(const GatherEntry* __restrict__ table,
+                                                          const int2* __restrict__ blocks) {
+    int2 const it = blocks[blockIdx.x];
+    const GatherEntry& e = table[it.x];
+    if (halted(e.g)) return;
+    size_t const n = (size_t)e.g.nx * e.g.ny * e.g.nz;
+    size_t const i = (size_t)it.y * blockDim.x + threadIdx.x;
+    if (i < n) gatherFlushCell<SUPP>(e.g, e.tiles, e.colour, e.J[0], e.J[1], e.J[2], e.jinv, i);
+}
+
+hipError_t launchGatherFlushBatch(int supp, const GatherEntry* table, const int2* blocks, int nblocks,
+                                  hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    switch (supp) {
+        case 2: hipLaunchKernelGGL(kGatherFlushBatch<2>, dim3(nblocks), dim3(256), 0, s, table, blocks); break;
+        case 3: hipLaunchKernelGGL(kGatherFlushBatch<3>, dim3(nblocks), dim3(256), 0, s, table, blocks); break;
+        case 4: hipLaunchKernelGGL(kGatherFlushBatch<4>, dim3(nblocks), dim3(256), 0, s, table, blocks); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launchGatherFlush(int supp, const GridDev& g, const unsigned long long* tiles, const uint8_t* colour,

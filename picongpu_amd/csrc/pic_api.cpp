@@ -677,6 +677,7 @@ struct pic_ctx {
     PhasePlan plans[3][2];
     int cus = 256;          // compute units of the device
     int pushSchedule = 0;   // PIC_PUSH_SCHEDULE_*
+    bool member = false;    // owned by an ensemble, whose device tables hold its plans' scratch
     int32_t* dAll = nullptr;
     // timing
     bool timing = false;
@@ -1448,6 +1449,7 @@ int pic_set_push_schedule(pic_ctx* c, int32_t schedule) {
     if (schedule < PIC_PUSH_SCHEDULE_AUTO || schedule > PIC_PUSH_SCHEDULE_SINGLE)
         return fail(PIC_E_INVALID, "unknown push schedule");
     if (schedule == c->pushSchedule) return PIC_OK;
+    if (c->member) return fail(PIC_E_INVALID, "an ensemble member keeps the ensemble's push schedule");
     PIC_HIP(hipSetDevice(c->device));
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->pushSchedule = schedule;
@@ -2598,6 +2600,10 @@ struct pic_ensemble {
     int nTiles = 0;
     int2* dBlocks = nullptr;
     int nBlocks = 0;
+    // every member's fused push in one launch with its tiles gathered
+    // (all members' plans are one-launch plans), or one launch per phase
+    bool single = false;
+    pic::GatherEntry* dGather = nullptr;
     // schedule
     int order = 1;         // PIC_ENSEMBLE_ORDER_*
     int blocksPerCu = 0;   // workgroups per CU of the batched push (0: one per work item)
@@ -2653,6 +2659,7 @@ int ensUploadTables(pic_ensemble* e) {
                 if (m.desc.pusher != PIC_PUSHER_NONE) a.sp[k++] = m.dev;
             a.nsp = k;
             a.scList = nullptr;
+            if (e->single) a.tileOut = c->plans[e->supp - 2][1].tiles;
             push[i] = a;
             pic::FieldArgs f = makeFieldArgs(c);
             for (int d = 0; d < 3; ++d) {
@@ -2663,6 +2670,18 @@ int ensUploadTables(pic_ensemble* e) {
         }
         PIC_HIP(hipMemcpyAsync(e->dPush[par], push.data(), n * sizeof(pic::PushArgs), hipMemcpyHostToDevice, e->stream));
         PIC_HIP(hipMemcpyAsync(e->dField[par], field.data(), n * sizeof(pic::FieldBatchEntry), hipMemcpyHostToDevice,
+                               e->stream));
+    }
+    if (e->single) {
+        std::vector<pic::GatherEntry> gather(n);
+        for (int i = 0; i < n; ++i) {
+            pic_ctx* c = e->sims[i];
+            const PhasePlan& plan = c->plans[e->supp - 2][1];
+            pic::PushArgs const a = makePushArgs(c);
+            gather[i] = pic::GatherEntry{c->g, plan.tiles, plan.colour, {c->J[0], c->J[1], c->J[2]}, a.jinv};
+        }
+        if (!e->dGather && ensAlloc(e, &e->dGather, static_cast<size_t>(n)) != PIC_OK) return PIC_E_HIP;
+        PIC_HIP(hipMemcpyAsync(e->dGather, gather.data(), n * sizeof(pic::GatherEntry), hipMemcpyHostToDevice,
                                e->stream));
     }
     std::vector<pic::ShiftEntry> shift;
@@ -2698,11 +2717,17 @@ int ensBuildWork(pic_ensemble* e) {
     if (e->order == PIC_ENSEMBLE_ORDER_HEAVY_FIRST)
         std::stable_sort(simOrder.begin(), simOrder.end(),
                          [&](int a, int b) { return ensParticles(e, a) > ensParticles(e, b); });
+    // one launch for every member's supercells when every member runs its
+    // own push in one launch: the items of all phases in one list
+    e->single = true;
+    for (int i = 0; i < n; ++i) e->single = e->single && plans[i]->single;
     std::vector<int2> work;
-    e->phaseOffset.assign(nph, 0);
-    e->phaseCount.assign(nph, 0);
+    size_t const lists = e->single ? 1 : nph;
+    e->phaseOffset.assign(lists, 0);
+    e->phaseCount.assign(lists, 0);
+    std::vector<int2> items;
     for (size_t ph = 0; ph < nph; ++ph) {
-        std::vector<int2> items;
+        if (!e->single) items.clear();
         if (e->order == PIC_ENSEMBLE_ORDER_INTERLEAVED) {
             size_t longest = 0;
             for (int i = 0; i < n; ++i)
@@ -2717,14 +2742,16 @@ int ensBuildWork(pic_ensemble* e) {
                     for (int k = 0; k < plans[i]->count[ph]; ++k)
                         items.push_back(int2{i, plans[i]->host[plans[i]->offset[ph] + k]});
         }
+        if (e->single && ph + 1 < nph) continue;  // one list of every phase's items
         // workgroup b runs on XCD b % 8: contiguous runs of items per XCD
         int const m = static_cast<int>(items.size());
         std::vector<int2> ordered(m);
         int pos = 0;
         for (int x = 0; x < 8; ++x)
             for (int b = x; b < m; b += 8) ordered[b] = items[pos++];
-        e->phaseOffset[ph] = static_cast<int>(work.size());
-        e->phaseCount[ph] = m;
+        size_t const l = e->single ? 0 : ph;
+        e->phaseOffset[l] = static_cast<int>(work.size());
+        e->phaseCount[l] = m;
         work.insert(work.end(), ordered.begin(), ordered.end());
     }
     if (!e->dPhaseWork && ensAlloc(e, &e->dPhaseWork, work.size()) != PIC_OK) return PIC_E_HIP;
@@ -2749,6 +2776,7 @@ int ensOneStep(pic_ensemble* e) {
         PIC_HIP(pic::launchPushBatch(e->supp, e->pusher, e->dPush[p], e->dPhaseWork + e->phaseOffset[ph], m, grid,
                                      e->stream));
     }
+    if (e->single) PIC_HIP(pic::launchGatherFlushBatch(e->supp, e->dGather, e->dBlocks, e->nBlocks, e->stream));
     PIC_HIP(pic::launchShiftBatch(e->dShift, e->dShiftWork, e->nShiftWork, e->stream));
     PIC_HIP(pic::launchFieldStepBatch(e->solver, e->dField[p], e->dTiles, e->nTiles, e->dBlocks, e->nBlocks,
                                       e->stream));
@@ -2834,6 +2862,7 @@ int pic_ensemble_create(const pic_config* cfgs, int32_t n, pic_ensemble** out) {
         pic_ctx* ctx = nullptr;
         int const r = pic_create(&c, &ctx);
         if (r != PIC_OK) return bail(r);
+        ctx->member = true;
         e->sims.push_back(ctx);
     }
     hipError_t he = hipSetDevice(e->device);

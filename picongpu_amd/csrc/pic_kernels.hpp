@@ -234,6 +234,17 @@ hipError_t launchFieldStepBatch(int solver, const FieldBatchEntry* table, const 
                                 const int2* blocks, int nblocks, hipStream_t s);
 // J = 0 for items (simulation, block of 256 cells)
 hipError_t launchCurrentResetBatch(const FieldBatchEntry* table, const int2* blocks, int nblocks, hipStream_t s);
+// the gather flush (launchGatherFlush) of every member of a batch that ran
+// its fused push in one launch, for items (simulation, block of 256 cells)
+struct GatherEntry {
+    GridDev g;
+    const unsigned long long* tiles;
+    const uint8_t* colour;
+    float* J[3];
+    double jinv;
+};
+hipError_t launchGatherFlushBatch(int supp, const GatherEntry* table, const int2* blocks, int nblocks,
+                                  hipStream_t s);
 // end of a batched step: every member's step flags as the end-of-step kernel of pic_step sets them
 hipError_t launchEndStepBatch(const FieldBatchEntry* table, int n, hipStream_t s);
 
