@@ -146,7 +146,7 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                 if (a.sp[s].lists) a.sp[s].outOff[(size_t)sc * kOffStride + tid] = 0u;
         }
         if (MODE == 1 && a.tileOut)  // an empty tile for the gather flush
-            for (int t = tid; t < J_ELEMS; t += blockDim.x) a.tileOut[(size_t)sc * J_ELEMS + t] = 0ull;
+            for (int t = tid; t < J_ELEMS; t += blockDim.x) a.tileOut[(size_t)sc * J_ELEMS + t] = 0.0f;
         return;
     }
 
@@ -314,8 +314,14 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
         }
     };
     if (MODE == 1 && a.tileOut) {
-        // one launch for every supercell: the raw tile is left for the gather flush
-        for (int t = tid; t < J_ELEMS; t += blockDim.x) a.tileOut[(size_t)sc * J_ELEMS + t] = sJ[t];
+        // one launch for every supercell: the tile is left for the gather
+        // flush as the floats the flush would add, zero where it adds
+        // nothing (adding a zero would change nothing either: J never holds
+        // -0, its sums start at +0 and round to nearest)
+        for (int t = tid; t < J_ELEMS; t += blockDim.x) {
+            long long const acc = fixedTileValue(sJ[t]);
+            a.tileOut[(size_t)sc * J_ELEMS + t] = acc == 0 ? 0.0f : (float)((double)acc * a.jinv);
+        }
     } else if (MODE != 0) {
         if (a.g.open)
             flushJ(std::true_type{});
@@ -488,9 +494,9 @@ __device__ __forceinline__ int coveringAlong(int g, int scSize, int ext, int nsc
 // visiting z, then y, then x candidates each in ascending colour is
 // ascending phase (every supercell covering a cell lies in another phase).
 template <int SUPP>
-__device__ __forceinline__ void gatherFlushCell(const GridDev& g, const unsigned long long* __restrict__ tiles,
+__device__ __forceinline__ void gatherFlushCell(const GridDev& g, const float* __restrict__ tiles,
                                                 const uint8_t* __restrict__ colour, float* J0, float* J1, float* J2,
-                                                double jinv, size_t i) {
+                                                size_t i) {
     using TG = JTileGeom<Esirkepov<SUPP>::lowerMargin, Esirkepov<SUPP>::upperMargin>;
     constexpr int J_ELEMS = PushLds<SUPP, 1>::J_ELEMS;
     int const x = (int)(i % g.nx);
@@ -510,10 +516,10 @@ __device__ __forceinline__ void gatherFlushCell(const GridDev& g, const unsigned
             for (int b = 0; b < ny; ++b)
                 for (int d = 0; d < nx; ++d) {
                     int const sc = sx[d] + g.nscx * (sy[b] + g.nscy * sz[a]);
-                    long long const acc = fixedTileValue(
-                        tiles[(size_t)sc * J_ELEMS + TG::off(c) + (tz[a] * TG::ext(c, 1) + ty[b]) * TG::ext(c, 0) + tx[d]]);
-                    if (acc == 0) continue;
-                    v += (float)((double)acc * jinv);
+                    float const w =
+                        tiles[(size_t)sc * J_ELEMS + TG::off(c) + (tz[a] * TG::ext(c, 1) + ty[b]) * TG::ext(c, 0) + tx[d]];
+                    if (w == 0.0f) continue;
+                    v += w;
                     any = true;
                 }
         if (any) Jc[c][i] = v;
@@ -521,13 +527,13 @@ __device__ __forceinline__ void gatherFlushCell(const GridDev& g, const unsigned
 }
 
 template <int SUPP>
-__global__ __launch_bounds__(256) void kGatherFlush(GridDev g, const unsigned long long* __restrict__ tiles,
+__global__ __launch_bounds__(256) void kGatherFlush(GridDev g, const float* __restrict__ tiles,
                                                      const uint8_t* __restrict__ colour, float* J0, float* J1,
-                                                     float* J2, double jinv) {
+                                                     float* J2) {
     if (halted(g)) return;
     size_t const n = (size_t)g.nx * g.ny * g.nz;
     size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) gatherFlushCell<SUPP>(g, tiles, colour, J0, J1, J2, jinv, i);
+    if (i < n) gatherFlushCell<SUPP>(g, tiles, colour, J0, J1, J2, i);
 }
 
 template <int SUPP>
@@ -538,7 +544,7 @@ __global__ __launch_bounds__(256) void kGatherFlushBatch(const GatherEntry* __re
     if (halted(e.g)) return;
     size_t const n = (size_t)e.g.nx * e.g.ny * e.g.nz;
     size_t const i = (size_t)it.y * blockDim.x + threadIdx.x;
-    if (i < n) gatherFlushCell<SUPP>(e.g, e.tiles, e.colour, e.J[0], e.J[1], e.J[2], e.jinv, i);
+    if (i < n) gatherFlushCell<SUPP>(e.g, e.tiles, e.colour, e.J[0], e.J[1], e.J[2], i);
 }
 
 hipError_t launchGatherFlushBatch(int supp, const GatherEntry* table, const int2* blocks, int nblocks,
@@ -553,14 +559,14 @@ hipError_t launchGatherFlushBatch(int supp, const GatherEntry* table, const int2
     return hipGetLastError();
 }
 
-hipError_t launchGatherFlush(int supp, const GridDev& g, const unsigned long long* tiles, const uint8_t* colour,
-                             float* const J[3], double jinv, hipStream_t s) {
+hipError_t launchGatherFlush(int supp, const GridDev& g, const float* tiles, const uint8_t* colour,
+                             float* const J[3], hipStream_t s) {
     size_t const n = (size_t)g.nx * g.ny * g.nz;
     dim3 const grid((unsigned)((n + 255) / 256));
     switch (supp) {
-        case 2: hipLaunchKernelGGL(kGatherFlush<2>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2], jinv); break;
-        case 3: hipLaunchKernelGGL(kGatherFlush<3>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2], jinv); break;
-        case 4: hipLaunchKernelGGL(kGatherFlush<4>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2], jinv); break;
+        case 2: hipLaunchKernelGGL(kGatherFlush<2>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2]); break;
+        case 3: hipLaunchKernelGGL(kGatherFlush<3>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2]); break;
+        case 4: hipLaunchKernelGGL(kGatherFlush<4>, grid, dim3(256), 0, s, g, tiles, colour, J[0], J[1], J[2]); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

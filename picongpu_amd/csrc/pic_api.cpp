@@ -572,7 +572,7 @@ struct PhasePlan {
     // gather flush in phase order (same J, bit for bit), instead of one
     // launch per phase; chosen when the phases are too small to fill the GPU
     bool single = false;
-    unsigned long long* tiles = nullptr;  // nsc tiles
+    float* tiles = nullptr;               // nsc tiles
     uint8_t* colour = nullptr;            // checkerboard colour per supercell index along x, y, z
 };
 
@@ -963,7 +963,7 @@ int runPush(pic_ctx* c, int32_t species, int mode, bool stageShift = false) {
             a.tileOut = plan->tiles;
             TimerScope ts(c, PIC_TIMER_PUSH);
             PIC_HIP(pic::launchPush(supp, pusher, mode, scheme, a, c->nsc, c->stream));
-            PIC_HIP(pic::launchGatherFlush(supp, c->g, plan->tiles, plan->colour, c->J, a.jinv, c->stream));
+            PIC_HIP(pic::launchGatherFlush(supp, c->g, plan->tiles, plan->colour, c->J, c->stream));
             continue;
         }
         for (size_t ph = 0; ph < plan->count.size(); ++ph) {
@@ -2677,8 +2677,7 @@ int ensUploadTables(pic_ensemble* e) {
         for (int i = 0; i < n; ++i) {
             pic_ctx* c = e->sims[i];
             const PhasePlan& plan = c->plans[e->supp - 2][1];
-            pic::PushArgs const a = makePushArgs(c);
-            gather[i] = pic::GatherEntry{c->g, plan.tiles, plan.colour, {c->J[0], c->J[1], c->J[2]}, a.jinv};
+            gather[i] = pic::GatherEntry{c->g, plan.tiles, plan.colour, {c->J[0], c->J[1], c->J[2]}};
         }
         if (!e->dGather && ensAlloc(e, &e->dGather, static_cast<size_t>(n)) != PIC_OK) return PIC_E_HIP;
         PIC_HIP(hipMemcpyAsync(e->dGather, gather.data(), n * sizeof(pic::GatherEntry), hipMemcpyHostToDevice,

@@ -84,9 +84,10 @@ struct PushArgs {
     const int32_t* scList;  // supercells of this launch, in launch order
     int stageShift;  // 1: the epilogue also moves the leavers to the outbox and compacts the supercell
     // non-null (fused push + deposit only): every supercell in one launch,
-    // each leaving its raw fixed-point J tile at tileOut[sc * tile elements]
-    // for launchGatherFlush instead of adding it to J
-    unsigned long long* tileOut;
+    // each leaving its J tile at tileOut[sc * tile elements], as the floats
+    // the flush would add to J (zero where it adds none), for
+    // launchGatherFlush instead of adding it to J
+    float* tileOut;
 };
 
 struct FieldArgs {
@@ -113,8 +114,8 @@ size_t pushTileElems(int supp);
 // phase-by-phase flush, so J is the same bit for bit. colour: the
 // checkerboard colour of every supercell index along x, then y, then z
 // (nscx + nscy + nscz entries).
-hipError_t launchGatherFlush(int supp, const GridDev& g, const unsigned long long* tiles, const uint8_t* colour,
-                             float* const J[3], double jinv, hipStream_t s);
+hipError_t launchGatherFlush(int supp, const GridDev& g, const float* tiles, const uint8_t* colour,
+                             float* const J[3], hipStream_t s);
 // shift of one species: `staged` = the push epilogue already moved the leavers
 // to the outbox and compacted every supercell (then only the insert runs).
 // redo: the insert again after the supercells grew, from count - incoming
@@ -238,10 +239,9 @@ hipError_t launchCurrentResetBatch(const FieldBatchEntry* table, const int2* blo
 // its fused push in one launch, for items (simulation, block of 256 cells)
 struct GatherEntry {
     GridDev g;
-    const unsigned long long* tiles;
+    const float* tiles;
     const uint8_t* colour;
     float* J[3];
-    double jinv;
 };
 hipError_t launchGatherFlushBatch(int supp, const GatherEntry* table, const int2* blocks, int nblocks,
                                   hipStream_t s);
