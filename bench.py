@@ -235,6 +235,8 @@ def main():
     ap.add_argument("--grid", default="", help="override grid, e.g. 128,128,128")
     ap.add_argument("--laser", default="initial", choices=["initial", "incident"],
                     help="laser_plasma only: the pulse as the initial field or injected through the Huygens surface")
+    ap.add_argument("--absorber", default="pml", choices=["pml", "exponential"],
+                    help="laser_plasma only: the field absorber on the open x and y borders")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-grid", default="128,64,64")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
@@ -263,6 +265,7 @@ def main():
         kw["grid"] = tuple(int(v) for v in args.grid.split(","))
     if args.config == "laser_plasma":
         kw["laser"] = args.laser
+        kw["absorber"] = args.absorber
     st = setup_fn(**kw)
     n_particles = st.mobile_particles()
     ncells = st.ncells
@@ -358,7 +361,9 @@ def main():
                    ("CIC" if st.species[0].shape == 1 else "PQS"),
                    "pusher": {1: "Boris", 2: "Vay", 3: "HigueraCary"}.get(st.species[0].pusher, "none"),
                    "solver": "Yee" if st.solver == 0 else "Lehe", "particles": n_particles,
-                   "cells": ncells},
+                   "cells": ncells,
+                   "field_absorber": {0: "none", 1: "exponential", 2: "pml"}.get(st.absorber)
+                   if any(b == abi.BOUNDARY_OPEN for b in st.boundary) else "none"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "kPushDeposit (fused gather+push+move+Esirkepov)",

@@ -91,10 +91,13 @@ extern "C" {
 #define PIC_PARTICLE_ABSORBING 0
 #define PIC_PARTICLE_REFLECTING 1
 
-/* field absorber at open borders (fields/absorber/Absorber.hpp): NONE, or the
- * exponential damping of absorber/exponential/Exponential.kernel:63-112 */
+/* field absorber at open borders (fields/absorber/Absorber.hpp): NONE, the
+ * exponential damping of absorber/exponential/Exponential.kernel:63-112, or
+ * the convolutional PML of absorber/pml/Pml.kernel:190-590 (the reference's
+ * default kind, simulation/stage/FieldAbsorber.hpp:88) */
 #define PIC_ABSORBER_NONE 0
 #define PIC_ABSORBER_EXPONENTIAL 1
+#define PIC_ABSORBER_PML 2
 #define PIC_ABSORBER_MAX_CELLS 64
 
 /* current interpolation before J is added to E (fields/currentInterpolation):
@@ -122,6 +125,13 @@ extern "C" {
 #define PIC_FIELD_E 0
 #define PIC_FIELD_B 1
 #define PIC_FIELD_J 2
+/* the PML's convolution fields of E and B (absorber/pml/Field.hpp: "Convolutional
+ * PML E" / "Convolutional PML B", checkpointed by the reference): six planes
+ * of nx*ny*nz floats in NodeValues' component order xy, xz, yx, yz, zx, zy,
+ * zero outside the layers; pic_load_field / pic_read_field only, and only
+ * with absorber PIC_ABSORBER_PML */
+#define PIC_FIELD_PML_E 3
+#define PIC_FIELD_PML_B 4
 
 /* per-kernel timers (pic_kernel_timer) */
 #define PIC_TIMER_PUSH 0      /* fused gather+push+move+Esirkepov launches */
@@ -171,6 +181,20 @@ typedef struct pic_incident_profile {
     double ramp_init;                  /* RAMP_INIT */
 } pic_incident_profile;
 
+/* PML parameters (param/fieldAbsorber.param namespace pml). With custom == 0
+ * the reference's defaults apply: SIGMA_KAPPA_GRADING_ORDER 4, SIGMA_MAX_SI =
+ * SIGMA_OPT_SI = 0.8 (order + 1) / (Z0 cell size) per axis, KAPPA_MAX 1,
+ * ALPHA_GRADING_ORDER 1, ALPHA_MAX_SI 0.2 S/m. The layer thickness is
+ * pic_config.absorber_cells, on open axes only. */
+typedef struct pic_pml_params {
+    int32_t custom;                    /* 0: the defaults above; 1: the fields below */
+    double sigma_kappa_grading_order;  /* SIGMA_KAPPA_GRADING_ORDER (>= 0) */
+    double sigma_max_si[3];            /* SIGMA_MAX_SI [S/m] (>= 0) */
+    double kappa_max[3];               /* KAPPA_MAX (>= 1) */
+    double alpha_grading_order;        /* ALPHA_GRADING_ORDER (>= 0) */
+    double alpha_max_si[3];            /* ALPHA_MAX_SI [S/m] (>= 0) */
+} pic_pml_params;
+
 /* Simulation description: the .param surface of the hot path
  * (param/simulation.param DELTA_T_SI/CELL_*_SI, param/density.param
  * BASE_DENSITY_SI, param/particle.param TYPICAL_PARTICLES_PER_CELL,
@@ -199,6 +223,7 @@ typedef struct pic_config {
      * cell index from the negative border, and from the positive border when
      * the entry is <= 0 (a positive entry is an absolute index) */
     int32_t incident_position[3][2];
+    pic_pml_params pml;       /* PIC_ABSORBER_PML parameters */
 } pic_config;
 
 /* FP32 constants in PIC units derived from a pic_config, computed as

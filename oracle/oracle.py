@@ -60,6 +60,11 @@ def _fill_incident(dst, prof):
         dst.laguerre_phases[m] = float(phases[m])
 
 
+class OraclePml(C.Structure):
+    _fields_ = [("custom", C.c_int32), ("sigma_kappa_grading_order", C.c_double), ("sigma_max_si", C.c_double * 3),
+                ("kappa_max", C.c_double * 3), ("alpha_grading_order", C.c_double), ("alpha_max_si", C.c_double * 3)]
+
+
 class OracleConfig(C.Structure):
     _fields_ = [("grid", C.c_int32 * 3), ("cell_si", C.c_double * 3), ("dt_si", C.c_double),
                 ("base_density_si", C.c_double), ("typical_ppc", C.c_int32), ("solver", C.c_int32),
@@ -67,7 +72,8 @@ class OracleConfig(C.Structure):
                 ("j_scale_exp", C.c_int32), ("device", C.c_int32), ("boundary", C.c_int32 * 3),
                 ("absorber", C.c_int32), ("absorber_cells", (C.c_int32 * 2) * 3),
                 ("absorber_strength", (C.c_float * 2) * 3), ("current_interpolation", C.c_int32),
-                ("incident", (OracleIncidentProfile * 2) * 3), ("incident_position", (C.c_int32 * 2) * 3)]
+                ("incident", (OracleIncidentProfile * 2) * 3), ("incident_position", (C.c_int32 * 2) * 3),
+                ("pml", OraclePml)]
 
 
 class OracleConstants(C.Structure):
@@ -110,6 +116,7 @@ def lib() -> C.CDLL:
             "oracle_get_ids": (C.c_int32, [_P, C.c_int32, _U32]),
             "oracle_set_field": (C.c_int32, [_P, C.c_int32, _F]),
             "oracle_get_field": (C.c_int32, [_P, C.c_int32, _D]),
+            "oracle_get_pml": (C.c_int32, [_P, C.c_int32, _D]),
             "oracle_current_reset": (None, [_P]),
             "oracle_push": (None, [_P, C.c_int32, C.c_int32]),
             "oracle_deposit": (None, [_P, C.c_int32]),
@@ -178,6 +185,15 @@ def config_from(setup) -> OracleConfig:
             c.incident_position[d][side] = int(setup.incident_position[d][side])
     for (axis, side), prof in (setup.incident or {}).items():
         _fill_incident(c.incident[axis][side], prof)
+    pml = getattr(setup, "pml", None)
+    if pml is not None:
+        c.pml.custom = 1
+        c.pml.sigma_kappa_grading_order = float(pml["sigma_kappa_grading_order"])
+        c.pml.alpha_grading_order = float(pml["alpha_grading_order"])
+        for d in range(3):
+            c.pml.sigma_max_si[d] = float(pml["sigma_max_si"][d])
+            c.pml.kappa_max[d] = float(pml["kappa_max"][d])
+            c.pml.alpha_max_si[d] = float(pml["alpha_max_si"][d])
     return c
 
 
@@ -251,6 +267,13 @@ class Oracle:
     def get_field(self, which):
         out = np.zeros((3, self.ncells))
         lib().oracle_get_field(self.h, which, _p(out, C.c_double))
+        return out
+
+    def get_pml(self, which):
+        """psi of E (which 0) or B (1): (6, ncells) in NodeValues order xy, xz, yx, yz, zx, zy."""
+        out = np.zeros((6, self.ncells))
+        if lib().oracle_get_pml(self.h, which, _p(out, C.c_double)) != 0:
+            raise ValueError("the state has no PML")
         return out
 
     def current_reset(self):

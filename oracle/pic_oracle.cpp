@@ -86,6 +86,15 @@ typedef struct oracle_incident_profile {
     double ramp_init;
 } oracle_incident_profile;
 
+typedef struct oracle_pml_params {
+    int32_t custom;                    // 0: the fieldAbsorber.param defaults
+    double sigma_kappa_grading_order;  // SIGMA_KAPPA_GRADING_ORDER
+    double sigma_max_si[3];            // SIGMA_MAX_SI [S/m]
+    double kappa_max[3];               // KAPPA_MAX
+    double alpha_grading_order;        // ALPHA_GRADING_ORDER
+    double alpha_max_si[3];            // ALPHA_MAX_SI [S/m]
+} oracle_pml_params;
+
 typedef struct oracle_config {
     int32_t grid[3];
     double cell_si[3];
@@ -105,6 +114,7 @@ typedef struct oracle_config {
     int32_t current_interpolation;  // 0 None, 1 Binomial
     oracle_incident_profile incident[3][2];  // incidentField.param XMin..ZMax as [axis][side]
     int32_t incident_position[3][2];         // incidentField.param POSITION
+    oracle_pml_params pml;                   // fieldAbsorber.param namespace pml (absorber 2)
 } oracle_config;
 
 typedef struct oracle_constants {
@@ -360,7 +370,73 @@ struct State {
     }
     T at(const std::vector<T>& f, int x, int y, int z) const { return outside(x, y, z) ? T(0) : f[idx(x, y, z)]; }
     int64_t step = 0;  // currentStep of the field solver calls
+    // PML (absorber 2): convolution fields of E and B (yx, zx, xy, zy, xz,
+    // yz) and the coefficient tables of E and B positions (pmlTables)
+    bool pml = false;
+    std::vector<T> psiE[6], psiB[6];
+    std::vector<float> pmlE, pmlB;
 };
+
+// PML coefficients (absorber/pml/Pml.kernel:60-270: getRelativeDepth,
+// getAbsorptionParameters, getCoefficients) per axis and index, for E (cell
+// index) or B (index + 1/2), in float_X arithmetic; the layer is
+// absorber_cells thick on open axes, 0 on periodic ones; the maxima are
+// fieldAbsorber.param's (SIGMA_OPT_SI = 0.8 (m + 1) / (Z0 dx), KAPPA_MAX 1,
+// ALPHA_MAX_SI 0.2, orders 4 and 1) unless given, normalised as
+// fieldAbsorber.unitless:72-100 (/ eps0 * unit time). Layout: b of x, y, z,
+// then c, then kappa.
+void pmlTables(const oracle_config& cfg, const Consts<float>& k, bool half, std::vector<float>& out) {
+    int const n[3] = {cfg.grid[0], cfg.grid[1], cfg.grid[2]};
+    int const total = n[0] + n[1] + n[2];
+    out.assign(3 * static_cast<size_t>(total), 0.0f);
+    double const z0 = kMue0 * kC;
+    double const eps0 = 1.0 / kMue0 / kC / kC;
+    double mSK = 4.0, mA = 1.0;
+    double sigmaMax[3], kappaMax[3] = {1.0, 1.0, 1.0}, alphaMax[3] = {0.2, 0.2, 0.2};
+    for (int d = 0; d < 3; ++d) sigmaMax[d] = 0.8 * (mSK + 1.0) / (z0 * cfg.cell_si[d]);
+    if (cfg.pml.custom != 0) {
+        mSK = cfg.pml.sigma_kappa_grading_order;
+        mA = cfg.pml.alpha_grading_order;
+        for (int d = 0; d < 3; ++d) {
+            sigmaMax[d] = cfg.pml.sigma_max_si[d];
+            kappaMax[d] = cfg.pml.kappa_max[d];
+            alphaMax[d] = cfg.pml.alpha_max_si[d];
+        }
+    }
+    float const m = static_cast<float>(mSK), ma = static_cast<float>(mA);
+    int off = 0;
+    for (int d = 0; d < 3; ++d) {
+        bool const open = cfg.boundary[d] != 0;
+        float const negB = static_cast<float>(open ? cfg.absorber_cells[d][0] : 0);
+        float const posB = static_cast<float>(open ? cfg.absorber_cells[d][1] : 0);
+        float const nSigma = static_cast<float>(sigmaMax[d] / eps0 * k.unit_time);
+        float const nAlpha = static_cast<float>(alphaMax[d] / eps0 * k.unit_time);
+        float const kMax = static_cast<float>(kappaMax[d]);
+        for (int i = 0; i < n[d]; ++i) {
+            float const zero = static_cast<float>(i) + (half ? 0.5f : 0.0f);  // cellIdx - numGuardCells
+            float depth = 0.0f;
+            if (zero < negB) {
+                depth = (negB - zero) / negB;
+            } else {
+                float const rightStart = static_cast<float>(n[d]) - posB;
+                if (zero > rightStart) depth = (zero - rightStart) / posB;
+            }
+            float sigma = 0.0f, kappa = 1.0f, alpha = 0.0f;
+            if (depth != 0.0f) {
+                float const g = std::pow(depth, m);
+                sigma = nSigma * g;
+                kappa = 1.0f + (kMax - 1.0f) * g;
+                alpha = nAlpha * std::pow(1.0f - depth, ma);
+            }
+            float const b = std::exp(-(sigma / kappa + alpha) * k.dt);
+            float const den = kappa * (sigma + alpha * kappa);
+            out[off + i] = b;
+            out[total + off + i] = den != 0.0f ? sigma * (b - 1.0f) / den : 0.0f;
+            out[2 * total + off + i] = kappa;
+        }
+        off += n[d];
+    }
+}
 
 int supportOf(int shape) { return shape == 1 ? 2 : (shape == 2 ? 3 : 4); }
 
@@ -1297,8 +1373,24 @@ T bwd(const State<T>& st, const std::vector<T>& f, int d, int x, int y, int z) {
     return (st.at(f, x, y, z) - st.at(f, x - ux, y - uy, z - uz)) / st.k.cell[d];
 }
 
+// coefficients b, c, kappa of a cell from a PML table (pmlTables layout)
 template <typename T>
-void bHalf(State<T>& st) {
+void pmlCell(const State<T>& st, const std::vector<float>& tab, int x, int y, int z, T* b, T* c, T* kappa) {
+    int const total = st.nx + st.ny + st.nz;
+    int const at[3] = {x, st.nx + y, st.nx + st.ny + z};
+    for (int d = 0; d < 3; ++d) {
+        b[d] = static_cast<T>(tab[at[d]]);
+        c[d] = static_cast<T>(tab[total + at[d]]);
+        kappa[d] = static_cast<T>(tab[2 * total + at[d]]);
+    }
+}
+
+// UpdateBHalfFunctor (FDTDBase.kernel:93-124) or, with the PML, the one of
+// Pml.kernel:487-590 inside the layer (isInPML: b_x b_y b_z != 1): psi of B
+// advances in the first half step of update_afterCurrent (updatePsiB) and
+// stays in the second, the curl's derivatives are divided by kappa
+template <typename T>
+void bHalf(State<T>& st, bool updatePsiB) {
     std::vector<T> nb[3];
     for (int c = 0; c < 3; ++c) nb[c] = st.B[c];
 #pragma omp parallel for schedule(static)
@@ -1306,6 +1398,30 @@ void bHalf(State<T>& st) {
         for (int y = 0; y < st.ny; ++y)
             for (int x = 0; x < st.nx; ++x) {
                 size_t const i = st.idx(x, y, z);
+                if (st.pml) {
+                    T b[3], c[3], kappa[3];
+                    pmlCell(st, st.pmlB, x, y, z, b, c, kappa);
+                    if (b[0] * b[1] * b[2] != T(1.0)) {
+                        T const dEdxY = derivE(st, st.E[1], 0, x, y, z), dEdxZ = derivE(st, st.E[2], 0, x, y, z);
+                        T const dEdyX = derivE(st, st.E[0], 1, x, y, z), dEdyZ = derivE(st, st.E[2], 1, x, y, z);
+                        T const dEdzX = derivE(st, st.E[0], 2, x, y, z), dEdzY = derivE(st, st.E[1], 2, x, y, z);
+                        T* const psi[6] = {&st.psiB[0][i], &st.psiB[1][i], &st.psiB[2][i],
+                                           &st.psiB[3][i], &st.psiB[4][i], &st.psiB[5][i]};
+                        if (updatePsiB) {
+                            *psi[0] = b[0] * *psi[0] + c[0] * dEdxZ;  // yx
+                            *psi[1] = b[0] * *psi[1] + c[0] * dEdxY;  // zx
+                            *psi[2] = b[1] * *psi[2] + c[1] * dEdyZ;  // xy
+                            *psi[3] = b[1] * *psi[3] + c[1] * dEdyX;  // zy
+                            *psi[4] = b[2] * *psi[4] + c[2] * dEdzY;  // xz
+                            *psi[5] = b[2] * *psi[5] + c[2] * dEdzX;  // yz
+                        }
+                        T const halfDt = T(0.5) * st.k.dt;
+                        nb[0][i] = st.B[0][i] + halfDt * (dEdzY / kappa[2] - dEdyZ / kappa[1] + *psi[4] - *psi[2]);
+                        nb[1][i] = st.B[1][i] + halfDt * (dEdxZ / kappa[0] - dEdzX / kappa[2] + *psi[0] - *psi[5]);
+                        nb[2][i] = st.B[2][i] + halfDt * (dEdyX / kappa[1] - dEdxY / kappa[0] + *psi[3] - *psi[1]);
+                        continue;
+                    }
+                }
                 T const cx = derivE(st, st.E[2], 1, x, y, z) - derivE(st, st.E[1], 2, x, y, z);
                 T const cy = derivE(st, st.E[0], 2, x, y, z) - derivE(st, st.E[2], 0, x, y, z);
                 T const cz = derivE(st, st.E[1], 0, x, y, z) - derivE(st, st.E[0], 1, x, y, z);
@@ -1316,6 +1432,9 @@ void bHalf(State<T>& st) {
     for (int c = 0; c < 3; ++c) st.B[c].swap(nb[c]);
 }
 
+// UpdateEFunctor (FDTDBase.kernel:50-91) or, with the PML, the one of
+// Pml.kernel:385-485 inside the layer: psi of E advances, the curl's
+// derivatives are divided by kappa
 template <typename T>
 void eFull(State<T>& st) {
     T const c2 = st.k.c * st.k.c;
@@ -1324,6 +1443,32 @@ void eFull(State<T>& st) {
         for (int y = 0; y < st.ny; ++y)
             for (int x = 0; x < st.nx; ++x) {
                 size_t const i = st.idx(x, y, z);
+                if (st.pml) {
+                    T b[3], c[3], kappa[3];
+                    pmlCell(st, st.pmlE, x, y, z, b, c, kappa);
+                    if (b[0] * b[1] * b[2] != T(1.0)) {
+                        T const dBdxY = bwd(st, st.B[1], 0, x, y, z), dBdxZ = bwd(st, st.B[2], 0, x, y, z);
+                        T const dBdyX = bwd(st, st.B[0], 1, x, y, z), dBdyZ = bwd(st, st.B[2], 1, x, y, z);
+                        T const dBdzX = bwd(st, st.B[0], 2, x, y, z), dBdzY = bwd(st, st.B[1], 2, x, y, z);
+                        T& yx = st.psiE[0][i];
+                        T& zx = st.psiE[1][i];
+                        T& xy = st.psiE[2][i];
+                        T& zy = st.psiE[3][i];
+                        T& xz = st.psiE[4][i];
+                        T& yz = st.psiE[5][i];
+                        yx = b[0] * yx + c[0] * dBdxZ;
+                        zx = b[0] * zx + c[0] * dBdxY;
+                        xy = b[1] * xy + c[1] * dBdyZ;
+                        zy = b[1] * zy + c[1] * dBdyX;
+                        xz = b[2] * xz + c[2] * dBdzY;
+                        yz = b[2] * yz + c[2] * dBdzX;
+                        T const c2dt = c2 * st.k.dt;
+                        st.E[0][i] = st.E[0][i] + c2dt * (dBdyZ / kappa[1] - dBdzY / kappa[2] + xy - xz);
+                        st.E[1][i] = st.E[1][i] + c2dt * (dBdzX / kappa[2] - dBdxZ / kappa[0] + yz - yx);
+                        st.E[2][i] = st.E[2][i] + c2dt * (dBdxY / kappa[0] - dBdyX / kappa[1] + zx - zy);
+                        continue;
+                    }
+                }
                 T const cx = bwd(st, st.B[2], 1, x, y, z) - bwd(st, st.B[1], 2, x, y, z);
                 T const cy = bwd(st, st.B[0], 2, x, y, z) - bwd(st, st.B[2], 0, x, y, z);
                 T const cz = bwd(st, st.B[1], 0, x, y, z) - bwd(st, st.B[0], 1, x, y, z);
@@ -1884,7 +2029,7 @@ void incidentUpdate(State<T>& st, bool updateB, T currentStep) {
 // update_beforeCurrent (FDTDBase.hpp:97-121)
 template <typename T>
 void beforeCurrent(State<T>& st) {
-    bHalf(st);
+    bHalf(st, false);  // updateBSecondHalf
     T const cs = static_cast<T>(st.step);
     incidentUpdate(st, true, cs);
     incidentUpdate(st, false, cs + T(0.5) * st.k.dt / st.k.dt);
@@ -1897,7 +2042,7 @@ template <typename T>
 void afterCurrent(State<T>& st) {
     absorb(st, st.E);
     incidentUpdate(st, true, static_cast<T>(st.step) + st.k.dt / st.k.dt);
-    bHalf(st);
+    bHalf(st, true);  // updateBFirstHalf
     absorb(st, st.B);
     ++st.step;
 }
@@ -1965,6 +2110,17 @@ State<T>* create(const oracle_config& cfg, int order) {
     st->ncells = static_cast<size_t>(st->nx) * st->ny * st->nz;
     st->order = order;
     st->k.contract = order == 0;
+    if (cfg.absorber == 2) {
+        st->pml = true;
+        Consts<float> const kf = makeConsts<float>(cfg);
+        pmlTables(cfg, kf, false, st->pmlE);
+        pmlTables(cfg, kf, true, st->pmlB);
+        size_t const n = static_cast<size_t>(cfg.grid[0]) * cfg.grid[1] * cfg.grid[2];
+        for (int q = 0; q < 6; ++q) {
+            st->psiE[q].assign(n, T(0));
+            st->psiB[q].assign(n, T(0));
+        }
+    }
     st->anyOpen = cfg.boundary[0] || cfg.boundary[1] || cfg.boundary[2];
     for (int c = 0; c < 3; ++c) {
         st->E[c].assign(st->ncells, T(0));
@@ -2178,6 +2334,21 @@ int32_t oracle_get_field(void* p, int32_t which, double* data) {
         auto* f = which == 0 ? st->E : (which == 1 ? st->B : st->J);
         for (int c = 0; c < 3; ++c)
             for (size_t i = 0; i < st->ncells; ++i) data[c * st->ncells + i] = static_cast<double>(f[c][i]);
+        return 0;
+    };
+    return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
+}
+
+// the PML's psi of E (which 0) or B (1): six planes in the reference's
+// NodeValues order xy, xz, yx, yz, zx, zy (absorber/pml/Field.hpp); -1
+// without the PML
+int32_t oracle_get_pml(void* p, int32_t which, double* data) {
+    auto impl = [&](auto* st) -> int32_t {
+        if (!st->pml) return -1;
+        auto* psi = which == 0 ? st->psiE : st->psiB;
+        constexpr int kNodeOrder[6] = {2, 4, 0, 5, 1, 3};  // psi[] holds yx, zx, xy, zy, xz, yz
+        for (int q = 0; q < 6; ++q)
+            for (size_t i = 0; i < st->ncells; ++i) data[q * st->ncells + i] = static_cast<double>(psi[kNodeOrder[q]][i]);
         return 0;
     };
     return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);

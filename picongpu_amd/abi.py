@@ -20,7 +20,7 @@ SHAPE_CIC, SHAPE_TSC, SHAPE_PQS = 1, 2, 3
 PUSHER_NONE, PUSHER_BORIS, PUSHER_VAY, PUSHER_HIGUERA_CARY = 0, 1, 2, 3
 BOUNDARY_PERIODIC, BOUNDARY_OPEN = 0, 1
 PARTICLE_ABSORBING, PARTICLE_REFLECTING = 0, 1
-ABSORBER_NONE, ABSORBER_EXPONENTIAL = 0, 1
+ABSORBER_NONE, ABSORBER_EXPONENTIAL, ABSORBER_PML = 0, 1, 2
 CURRENT_NONE, CURRENT_BINOMIAL = 0, 1
 CURRENT_SOLVER_ESIRKEPOV, CURRENT_SOLVER_EMZ = 0, 1
 SOLVER_YEE, SOLVER_LEHE = 0, 1
@@ -29,6 +29,9 @@ POLARISATION_LINEAR, POLARISATION_CIRCULAR = 0, 1
 ORIGIN_ZERO, ORIGIN_CENTER = 0, 1
 INCIDENT_MAX_MODES = 16
 FIELD_E, FIELD_B, FIELD_J = 0, 1, 2
+# the PML's convolution fields: six planes in NodeValues order xy, xz, yx, yz, zx, zy
+FIELD_PML_E, FIELD_PML_B = 3, 4
+PML_COMPONENTS = ("xy", "xz", "yx", "yz", "zx", "zy")
 TIMER_PUSH, TIMER_SHIFT, TIMER_FIELD = 0, 1, 2
 PUSH_SCHEDULE_AUTO, PUSH_SCHEDULE_PHASES, PUSH_SCHEDULE_SINGLE = 0, 1, 2
 
@@ -106,6 +109,33 @@ def fill_incident_profile(dst, kind, wavelength_si, amplitude_si, pulse_duration
     return dst
 
 
+class PmlParams(C.Structure):
+    _fields_ = [
+        ("custom", C.c_int32),
+        ("sigma_kappa_grading_order", C.c_double),
+        ("sigma_max_si", C.c_double * 3),
+        ("kappa_max", C.c_double * 3),
+        ("alpha_grading_order", C.c_double),
+        ("alpha_max_si", C.c_double * 3),
+    ]
+
+
+def fill_pml(dst: PmlParams, pml) -> None:
+    """pml: None (the fieldAbsorber.param defaults) or a dict with
+    sigma_kappa_grading_order, sigma_max_si (3), kappa_max (3),
+    alpha_grading_order, alpha_max_si (3)."""
+    if pml is None:
+        dst.custom = 0
+        return
+    dst.custom = 1
+    dst.sigma_kappa_grading_order = float(pml["sigma_kappa_grading_order"])
+    dst.alpha_grading_order = float(pml["alpha_grading_order"])
+    for d in range(3):
+        dst.sigma_max_si[d] = float(pml["sigma_max_si"][d])
+        dst.kappa_max[d] = float(pml["kappa_max"][d])
+        dst.alpha_max_si[d] = float(pml["alpha_max_si"][d])
+
+
 class Config(C.Structure):
     _fields_ = [
         ("grid", C.c_int32 * 3),
@@ -126,6 +156,7 @@ class Config(C.Structure):
         ("current_interpolation", C.c_int32),
         ("incident", (IncidentProfile * 2) * 3),
         ("incident_position", (C.c_int32 * 2) * 3),
+        ("pml", PmlParams),
     ]
 
 
@@ -292,10 +323,11 @@ def ptr(a: np.ndarray, ctype):
 def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YEE, lehe_dir=1, j_scale_exp=0,
                 device=0, boundary=(0, 0, 0), absorber=0, absorber_cells=((0, 0), (0, 0), (0, 0)),
                 absorber_strength=((0.0, 0.0), (0.0, 0.0), (0.0, 0.0)), current_interpolation=0, incident=None,
-                incident_position=((16, -16), (16, -16), (16, -16))) -> Config:
+                incident_position=((16, -16), (16, -16), (16, -16)), pml=None) -> Config:
     """species: list of dicts with mass_ratio, charge_ratio, shape, pusher, frames_per_sc;
     boundary: BOUNDARY_* per axis; absorber: ABSORBER_*, with thickness and
-    strength per [axis][negative, positive] side as in fieldAbsorber.param;
+    strength per [axis][negative, positive] side as in fieldAbsorber.param,
+    pml the PML parameters (fill_pml; None: the reference's defaults);
     incident: {(axis, side): keyword arguments of fill_incident_profile} for
     the faces of incidentField.param that carry a profile, incident_position
     its POSITION table."""
@@ -332,6 +364,7 @@ def make_config(grid, cell_si, dt_si, density_si, ppc, species, solver=SOLVER_YE
             cfg.incident_position[d][side] = int(incident_position[d][side])
     for (axis, side), prof in (incident or {}).items():
         fill_incident_profile(cfg.incident[axis][side], **prof)
+    fill_pml(cfg.pml, pml)
     return cfg
 
 
@@ -474,13 +507,17 @@ class Simulation:
         d.polarisation_axis = int(polarisation_axis)
         check(self.lib.pic_add_laser(self.h, C.byref(d)), "pic_add_laser")
 
+    @staticmethod
+    def field_planes(which):
+        return 6 if which in (FIELD_PML_E, FIELD_PML_B) else 3
+
     def load_field(self, which, data):
-        data = np.ascontiguousarray(data, dtype=np.float32).reshape(3, -1)
+        data = np.ascontiguousarray(data, dtype=np.float32).reshape(self.field_planes(which), -1)
         assert data.shape[1] == self.ncells
         check(self.lib.pic_load_field(self.h, which, ptr(data, C.c_float)), "pic_load_field")
 
     def read_field(self, which):
-        out = np.zeros((3, self.ncells), np.float32)
+        out = np.zeros((self.field_planes(which), self.ncells), np.float32)
         check(self.lib.pic_read_field(self.h, which, ptr(out, C.c_float)), "pic_read_field")
         return out
 

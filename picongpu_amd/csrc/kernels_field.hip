@@ -78,6 +78,12 @@ __device__ __forceinline__ float derivE(const float* f, const Idx& I, int d, con
     return fwd(f, I, d, 0, 0, 0, a.cell[d]);
 }
 
+// a cell known to lie outside every PML layer (FieldArgs::inner)
+__device__ __forceinline__ bool innerCell(const FieldArgs& a, int x, int y, int z) {
+    return a.inner && x >= a.in0[0] && x < a.in1[0] && y >= a.in0[1] && y < a.in1[1] && z >= a.in0[2] &&
+           z < a.in1[2];
+}
+
 template <int SOLVER>
 __global__ __launch_bounds__(256) void kBHalf(FieldArgs a) {
     if (halted(a.g)) return;
@@ -114,6 +120,168 @@ __global__ __launch_bounds__(256) void kEFull(FieldArgs a) {
     a.E[2][i] = a.E[2][i] + cz * a.c2 * a.dt;
 }
 
+// ---------------------------------------------------------------------------
+// PML (absorber/pml/Pml.kernel:385-590). A cell lies in the layer when the
+// product of its b coefficients along x, y, z is not 1 (isInPML); there the
+// convolution fields psi advance (b psi + c dF) and the curl's derivatives
+// are divided by kappa, in UpdateEFunctor's and UpdateBHalfFunctor's order of
+// operations. Outside it the cell takes kEFull's / kBHalf's arithmetic.
+// ---------------------------------------------------------------------------
+struct PmlCoeffs {
+    float b[3], c[3], kappa[3];
+};
+
+// f / kappa: a correctly rounded division by 1 returns f, so the default
+// kappa of 1 skips the division with the same result
+__device__ __forceinline__ float divKappa(float f, float kappa) { return kappa == 1.0f ? f : f / kappa; }
+
+__device__ __forceinline__ PmlCoeffs pmlCoeffs(const float* __restrict__ tab, const GridDev& g, int x, int y, int z) {
+    int const total = g.nx + g.ny + g.nz;
+    int const at[3] = {x, g.nx + y, g.nx + g.ny + z};
+    PmlCoeffs k;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        k.b[d] = tab[at[d]];
+        k.c[d] = tab[total + at[d]];
+        k.kappa[d] = tab[2 * total + at[d]];
+    }
+    return k;
+}
+
+// B half step of cell i from the derivatives of E along each axis (dEdxY:
+// of E.y along x, ...): the PML's update inside the layer, kBHalf's
+// (B -= curl E * 0.5 * dt) outside it
+__device__ __forceinline__ void bHalfPmlApply(const FieldArgs& a, const PmlArgs& p, int updatePsi, int x, int y, int z,
+                                              size_t i, float dEdxY, float dEdxZ, float dEdyX, float dEdyZ,
+                                              float dEdzX, float dEdzY) {
+    PmlCoeffs const k = pmlCoeffs(p.tabB, a.g, x, y, z);
+    if (k.b[0] * k.b[1] * k.b[2] == 1.0f) {  // outside the layer: kBHalf
+        float const cx = dEdyZ - dEdzY;
+        float const cy = dEdzX - dEdxZ;
+        float const cz = dEdxY - dEdyX;
+        a.B[0][i] = a.B[0][i] - cx * 0.5f * a.dt;
+        a.B[1][i] = a.B[1][i] - cy * 0.5f * a.dt;
+        a.B[2][i] = a.B[2][i] - cz * 0.5f * a.dt;
+        return;
+    }
+    float yx = p.psiB[0][i], zx = p.psiB[1][i], xy = p.psiB[2][i];
+    float zy = p.psiB[3][i], xz = p.psiB[4][i], yz = p.psiB[5][i];
+    if (updatePsi) {
+        yx = k.b[0] * yx + k.c[0] * dEdxZ;
+        zx = k.b[0] * zx + k.c[0] * dEdxY;
+        xy = k.b[1] * xy + k.c[1] * dEdyZ;
+        zy = k.b[1] * zy + k.c[1] * dEdyX;
+        xz = k.b[2] * xz + k.c[2] * dEdzY;
+        yz = k.b[2] * yz + k.c[2] * dEdzX;
+        p.psiB[0][i] = yx;
+        p.psiB[1][i] = zx;
+        p.psiB[2][i] = xy;
+        p.psiB[3][i] = zy;
+        p.psiB[4][i] = xz;
+        p.psiB[5][i] = yz;
+    }
+    float const halfDt = 0.5f * a.dt;
+    a.B[0][i] = a.B[0][i] + halfDt * (divKappa(dEdzY, k.kappa[2]) - divKappa(dEdyZ, k.kappa[1]) + xz - xy);
+    a.B[1][i] = a.B[1][i] + halfDt * (divKappa(dEdxZ, k.kappa[0]) - divKappa(dEdzX, k.kappa[2]) + yx - yz);
+    a.B[2][i] = a.B[2][i] + halfDt * (divKappa(dEdyX, k.kappa[1]) - divKappa(dEdxY, k.kappa[0]) + zy - zx);
+}
+
+template <int SOLVER>
+__device__ __forceinline__ void bHalfPmlAt(const FieldArgs& a, const PmlArgs& p, int updatePsi, int x, int y, int z) {
+    Idx I{a.g, x, y, z};
+    size_t const i = (size_t)x + (size_t)a.g.nx * ((size_t)y + (size_t)a.g.ny * (size_t)z);
+    // dEdx = curl.xDerivative(E) etc.: the derivative of every component along one axis
+    bHalfPmlApply(a, p, updatePsi, x, y, z, i, derivE<SOLVER>(a.E[1], I, 0, a), derivE<SOLVER>(a.E[2], I, 0, a),
+                  derivE<SOLVER>(a.E[0], I, 1, a), derivE<SOLVER>(a.E[2], I, 1, a), derivE<SOLVER>(a.E[0], I, 2, a),
+                  derivE<SOLVER>(a.E[1], I, 2, a));
+}
+
+// E full step with PML of cell i into Eout (a.E itself for the stage call)
+// from the backward derivatives of B along each axis (dBdxY: of B.y along
+// x, ...): the PML's update inside the layer, kEFull's outside it; ADDJ:
+// then J -> E as kAddCurrent (None), in the same pass
+template <bool ADDJ>
+__device__ __forceinline__ void eFullPmlApply(const FieldArgs& a, const PmlArgs& p, float* const* Eout, int x, int y,
+                                              int z, size_t i, float dBdxY, float dBdxZ, float dBdyX, float dBdyZ,
+                                              float dBdzX, float dBdzY) {
+    PmlCoeffs const k = pmlCoeffs(p.tabE, a.g, x, y, z);
+    float e0, e1, e2;
+    if (k.b[0] * k.b[1] * k.b[2] == 1.0f) {  // outside the layer: kEFull
+        e0 = a.E[0][i] + (dBdyZ - dBdzY) * a.c2 * a.dt;
+        e1 = a.E[1][i] + (dBdzX - dBdxZ) * a.c2 * a.dt;
+        e2 = a.E[2][i] + (dBdxY - dBdyX) * a.c2 * a.dt;
+    } else {
+        float const yx = k.b[0] * p.psiE[0][i] + k.c[0] * dBdxZ;
+        float const zx = k.b[0] * p.psiE[1][i] + k.c[0] * dBdxY;
+        float const xy = k.b[1] * p.psiE[2][i] + k.c[1] * dBdyZ;
+        float const zy = k.b[1] * p.psiE[3][i] + k.c[1] * dBdyX;
+        float const xz = k.b[2] * p.psiE[4][i] + k.c[2] * dBdzY;
+        float const yz = k.b[2] * p.psiE[5][i] + k.c[2] * dBdzX;
+        p.psiE[0][i] = yx;
+        p.psiE[1][i] = zx;
+        p.psiE[2][i] = xy;
+        p.psiE[3][i] = zy;
+        p.psiE[4][i] = xz;
+        p.psiE[5][i] = yz;
+        float const c2dt = a.c2 * a.dt;
+        e0 = a.E[0][i] + c2dt * (divKappa(dBdyZ, k.kappa[1]) - divKappa(dBdzY, k.kappa[2]) + xy - xz);
+        e1 = a.E[1][i] + c2dt * (divKappa(dBdzX, k.kappa[2]) - divKappa(dBdxZ, k.kappa[0]) + yz - yx);
+        e2 = a.E[2][i] + c2dt * (divKappa(dBdxY, k.kappa[0]) - divKappa(dBdyX, k.kappa[1]) + zx - zy);
+    }
+    if (ADDJ) {
+        e0 = e0 + a.coeff * a.J[0][i];
+        e1 = e1 + a.coeff * a.J[1][i];
+        e2 = e2 + a.coeff * a.J[2][i];
+    }
+    Eout[0][i] = e0;
+    Eout[1][i] = e1;
+    Eout[2][i] = e2;
+}
+
+template <int SOLVER>
+__global__ __launch_bounds__(256) void kBHalfPml(FieldArgs a, PmlArgs p, int updatePsi) {
+    if (halted(a.g)) return;
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bHalfPmlAt<SOLVER>(a, p, updatePsi, (int)(i % a.g.nx), (int)((i / a.g.nx) % a.g.ny),
+                       (int)(i / ((size_t)a.g.nx * a.g.ny)));
+}
+
+// the E full step of every cell with the PML; ADDJ: J -> E in the same pass,
+// into Eout. Cells of the inner box skip the coefficient lookup.
+template <bool ADDJ>
+__device__ __forceinline__ void eFullPmlCell(const FieldArgs& a, const PmlArgs& p, float* const* Eout, size_t i) {
+    int const x = (int)(i % a.g.nx), y = (int)((i / a.g.nx) % a.g.ny), z = (int)(i / ((size_t)a.g.nx * a.g.ny));
+    Idx I{a.g, x, y, z};
+    float const dBdxY = bwd(a.B[1], I, 0, a.cell[0]), dBdxZ = bwd(a.B[2], I, 0, a.cell[0]);
+    float const dBdyX = bwd(a.B[0], I, 1, a.cell[1]), dBdyZ = bwd(a.B[2], I, 1, a.cell[1]);
+    float const dBdzX = bwd(a.B[0], I, 2, a.cell[2]), dBdzY = bwd(a.B[1], I, 2, a.cell[2]);
+    if (!innerCell(a, x, y, z)) {
+        eFullPmlApply<ADDJ>(a, p, Eout, x, y, z, i, dBdxY, dBdxZ, dBdyX, dBdyZ, dBdzX, dBdzY);
+        return;
+    }
+    float e0 = a.E[0][i] + (dBdyZ - dBdzY) * a.c2 * a.dt;
+    float e1 = a.E[1][i] + (dBdzX - dBdxZ) * a.c2 * a.dt;
+    float e2 = a.E[2][i] + (dBdxY - dBdyX) * a.c2 * a.dt;
+    if (ADDJ) {
+        e0 = e0 + a.coeff * a.J[0][i];
+        e1 = e1 + a.coeff * a.J[1][i];
+        e2 = e2 + a.coeff * a.J[2][i];
+    }
+    Eout[0][i] = e0;
+    Eout[1][i] = e1;
+    Eout[2][i] = e2;
+}
+
+__global__ __launch_bounds__(256) void kEFullPml(FieldArgs a, PmlArgs p) {
+    if (halted(a.g)) return;
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    eFullPmlCell<false>(a, p, a.E, i);
+}
+
 // kEFull then kAddCurrent (None) per cell in one pass, the new E into Enew
 __global__ __launch_bounds__(256) void kEFullJ(FieldArgs a, float* En0, float* En1, float* En2) {
     if (halted(a.g)) return;
@@ -130,6 +298,16 @@ __global__ __launch_bounds__(256) void kEFullJ(FieldArgs a, float* En0, float* E
     En0[i] = e0 + a.coeff * a.J[0][i];
     En1[i] = e1 + a.coeff * a.J[1][i];
     En2[i] = e2 + a.coeff * a.J[2][i];
+}
+
+// the PML's E full step (kEFullPml) and J -> E (None) per cell in one pass into Enew
+__global__ __launch_bounds__(256) void kEFullPmlJ(FieldArgs a, PmlArgs p, float* En0, float* En1, float* En2) {
+    if (halted(a.g)) return;
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    size_t const i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float* const En[3] = {En0, En1, En2};
+    eFullPmlCell<true>(a, p, En, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -442,8 +620,9 @@ struct YeeAfterLds {
 
 // update_afterCurrent for one z-marching tile (Yee): B'' = (B - curlE(E)*0.5*dt) - curlE(Enew)*0.5*dt;
 // with TWO = false one B half step alone, B' = B - curlE(E)*0.5*dt (kBHalf's arithmetic)
-template <bool OPEN, bool TWO>
-__device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* Enew, int b, int zc, YeeAfterLds& L) {
+template <bool OPEN, bool TWO, bool PML = false>
+__device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* Enew, int b, int zc, YeeAfterLds& L,
+                                             const PmlArgs* p = nullptr, int updatePsi = 0) {
     ZTile const T = zTile(a.g, b, zc);
     int const nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
     size_t const plane = (size_t)nx * ny;
@@ -497,7 +676,23 @@ __device__ __forceinline__ void yeeAfterTile(const FieldArgs& a, float* const* E
         store(k + 1, nextE, nextN);
         if (k + 2 <= T.z1) fetch(k + 2, nextE, nextN);  // in flight during this step
         __syncthreads();
-        if (inside) {
+        if (PML && inside) {
+            // B half step with the PML (never with TWO): the six forward
+            // differences of the curl for every cell, the update per cell
+            int const s = k & 1, s1 = (k + 1) & 1;
+            float const f0 = L.E[s][0][ly][lx], f1 = L.E[s][1][ly][lx], f2 = L.E[s][2][ly][lx];
+            float const dxY = (L.E[s][1][ly][lx + 1] - f1) / a.cell[0], dxZ = (L.E[s][2][ly][lx + 1] - f2) / a.cell[0];
+            float const dyX = (L.E[s][0][ly + 1][lx] - f0) / a.cell[1], dyZ = (L.E[s][2][ly + 1][lx] - f2) / a.cell[1];
+            float const dzX = (L.E[s1][0][ly][lx] - f0) / a.cell[2], dzY = (L.E[s1][1][ly][lx] - f1) / a.cell[2];
+            size_t const i = ixy + plane * k;
+            if (!innerCell(a, T.x0 + lx, T.y0 + ly, k)) {
+                bHalfPmlApply(a, *p, updatePsi, T.x0 + lx, T.y0 + ly, k, i, dxY, dxZ, dyX, dyZ, dzX, dzY);
+            } else {
+                a.B[0][i] = a.B[0][i] - (dyZ - dzY) * 0.5f * a.dt;
+                a.B[1][i] = a.B[1][i] - (dzX - dxZ) * 0.5f * a.dt;
+                a.B[2][i] = a.B[2][i] - (dxY - dyX) * 0.5f * a.dt;
+            }
+        } else if (inside) {
             int const s = k & 1, s1 = (k + 1) & 1;
             auto curl = [&](float (*F)[kZY + 1][kZX + 1], float (*G)[kZY + 1][kZX + 1], float& cx, float& cy,
                             float& cz) {
@@ -631,8 +826,9 @@ __device__ __forceinline__ void lehePlaneLoad(const GridDev& g, float* const* F,
 
 // one B half step for one z-marching tile (Lehe): B' = B - curlE(E)*0.5*dt,
 // the arithmetic of kBHalf
-template <bool OPEN>
-__device__ __forceinline__ void leheBHalfTile(const FieldArgs& a, int b, int zc, LehePlanes& LE) {
+template <bool OPEN, bool PML>
+__device__ __forceinline__ void leheBHalfTile(const FieldArgs& a, const PmlArgs& p, int updatePsi, int b, int zc,
+                                              LehePlanes& LE) {
     ZTile const T = zTile(a.g, b, zc);
     int const nx = a.g.nx, ny = a.g.ny;
     size_t const plane = (size_t)nx * ny;
@@ -650,7 +846,21 @@ __device__ __forceinline__ void leheBHalfTile(const FieldArgs& a, int b, int zc,
         lehePlaneStore(nextE, k + 2, LE);
         if (k + 3 <= T.z1 + 1) lehePlaneFetch<OPEN>(a.g, a.E, eoff, k + 3, nextE);  // in flight during this step
         __syncthreads();
-        if (inside) {
+        if (PML && inside) {
+            // the six derivatives of the curl for every cell, so that layer
+            // and inner cells of a wave share them; only the update branches
+            float const dxY = lDeriv(LE, 1, 0, lx, ly, k, a), dxZ = lDeriv(LE, 2, 0, lx, ly, k, a);
+            float const dyX = lDeriv(LE, 0, 1, lx, ly, k, a), dyZ = lDeriv(LE, 2, 1, lx, ly, k, a);
+            float const dzX = lDeriv(LE, 0, 2, lx, ly, k, a), dzY = lDeriv(LE, 1, 2, lx, ly, k, a);
+            size_t const i = ixy + plane * k;
+            if (!innerCell(a, T.x0 + lx, T.y0 + ly, k)) {
+                bHalfPmlApply(a, p, updatePsi, T.x0 + lx, T.y0 + ly, k, i, dxY, dxZ, dyX, dyZ, dzX, dzY);
+            } else {
+                a.B[0][i] = a.B[0][i] - (dyZ - dzY) * 0.5f * a.dt;
+                a.B[1][i] = a.B[1][i] - (dzX - dxZ) * 0.5f * a.dt;
+                a.B[2][i] = a.B[2][i] - (dxY - dyX) * 0.5f * a.dt;
+            }
+        } else if (inside) {
             size_t const i = ixy + plane * k;
             float cx, cy, cz;
             lCurl(LE, lx, ly, k, a, cx, cy, cz);
@@ -724,7 +934,15 @@ template <bool OPEN>
 __global__ __launch_bounds__(256) void kLeheBHalfZ(FieldArgs a, int zc) {
     if (halted(a.g)) return;
     __shared__ LehePlanes L;
-    leheBHalfTile<OPEN>(a, blockIdx.x, zc, L);
+    leheBHalfTile<OPEN, false>(a, PmlArgs{}, 0, blockIdx.x, zc, L);
+}
+
+// with the PML: the layer cells (outside the inner box) as kBHalfPml
+template <bool OPEN>
+__global__ __launch_bounds__(256) void kLeheBHalfPmlZ(FieldArgs a, PmlArgs p, int updatePsi, int zc) {
+    if (halted(a.g)) return;
+    __shared__ LehePlanes L;
+    leheBHalfTile<OPEN, true>(a, p, updatePsi, blockIdx.x, zc, L);
 }
 
 template <bool OPEN>
@@ -732,6 +950,13 @@ __global__ __launch_bounds__(256) void kYeeBHalfZ(FieldArgs a, int zc) {
     if (halted(a.g)) return;
     __shared__ YeeAfterLds L;
     yeeAfterTile<OPEN, false>(a, nullptr, blockIdx.x, zc, L);
+}
+
+template <bool OPEN>
+__global__ __launch_bounds__(256) void kYeeBHalfPmlZ(FieldArgs a, PmlArgs p, int updatePsi, int zc) {
+    if (halted(a.g)) return;
+    __shared__ YeeAfterLds L;
+    yeeAfterTile<OPEN, false, true>(a, nullptr, blockIdx.x, zc, L, &p, updatePsi);
 }
 
 __global__ __launch_bounds__(256) void kYeeBeforeAddBatch(const FieldBatchEntry* __restrict__ table,
@@ -947,6 +1172,44 @@ hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s) {
         hipLaunchKernelGGL(kBHalf<PIC_SOLVER_LEHE>, dim3(blocksFor(n)), dim3(256), 0, s, a);
     else
         hipLaunchKernelGGL(kBHalf<PIC_SOLVER_YEE>, dim3(blocksFor(n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchBHalfPml(int solver, const FieldArgs& a, const PmlArgs& p, bool updatePsi, hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    int const zc = yeeChunk(a.g);  // the z-marching form when the grid fills the chip
+    if (zc > 0) {
+        unsigned const zt = (unsigned)zTileCount(a.g, zc);
+        int const u = updatePsi ? 1 : 0;
+        if (solver == PIC_SOLVER_LEHE) {
+            if (a.g.open)
+                hipLaunchKernelGGL(kLeheBHalfPmlZ<true>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
+            else
+                hipLaunchKernelGGL(kLeheBHalfPmlZ<false>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
+        } else {
+            if (a.g.open)
+                hipLaunchKernelGGL(kYeeBHalfPmlZ<true>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
+            else
+                hipLaunchKernelGGL(kYeeBHalfPmlZ<false>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
+        }
+        return hipGetLastError();
+    }
+    if (solver == PIC_SOLVER_LEHE)
+        hipLaunchKernelGGL(kBHalfPml<PIC_SOLVER_LEHE>, dim3(blocksFor(n)), dim3(256), 0, s, a, p, updatePsi ? 1 : 0);
+    else
+        hipLaunchKernelGGL(kBHalfPml<PIC_SOLVER_YEE>, dim3(blocksFor(n)), dim3(256), 0, s, a, p, updatePsi ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launchEFullPml(const FieldArgs& a, const PmlArgs& p, hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    hipLaunchKernelGGL(kEFullPml, dim3(blocksFor(n)), dim3(256), 0, s, a, p);
+    return hipGetLastError();
+}
+
+hipError_t launchEFullPmlJ(const FieldArgs& a, const PmlArgs& p, float* const Enew[3], hipStream_t s) {
+    size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
+    hipLaunchKernelGGL(kEFullPmlJ, dim3(blocksFor(n)), dim3(256), 0, s, a, p, Enew[0], Enew[1], Enew[2]);
     return hipGetLastError();
 }
 

@@ -88,12 +88,21 @@ int validateConfig(const pic_config& c) {
             if (!(c.absorber_strength[d][side] >= 0.0f) || !std::isfinite(c.absorber_strength[d][side]))
                 return fail(PIC_E_INVALID, "absorber_strength must be finite and >= 0");
         }
-        if (c.boundary[d] == PIC_BOUNDARY_OPEN && c.absorber == PIC_ABSORBER_EXPONENTIAL &&
+        if (c.boundary[d] == PIC_BOUNDARY_OPEN && c.absorber != PIC_ABSORBER_NONE &&
             c.absorber_cells[d][0] + c.absorber_cells[d][1] > c.grid[d])
             return fail(PIC_E_INVALID, "absorber layers are thicker than the grid");
     }
-    if (c.absorber != PIC_ABSORBER_NONE && c.absorber != PIC_ABSORBER_EXPONENTIAL)
-        return fail(PIC_E_INVALID, "absorber must be PIC_ABSORBER_NONE or PIC_ABSORBER_EXPONENTIAL");
+    if (c.absorber != PIC_ABSORBER_NONE && c.absorber != PIC_ABSORBER_EXPONENTIAL && c.absorber != PIC_ABSORBER_PML)
+        return fail(PIC_E_INVALID, "absorber must be PIC_ABSORBER_NONE, PIC_ABSORBER_EXPONENTIAL or PIC_ABSORBER_PML");
+    if (c.absorber == PIC_ABSORBER_PML && c.pml.custom != 0) {
+        const pic_pml_params& q = c.pml;
+        bool ok = q.sigma_kappa_grading_order >= 0.0 && q.alpha_grading_order >= 0.0;
+        for (int d = 0; d < 3; ++d)
+            ok = ok && q.sigma_max_si[d] >= 0.0 && q.kappa_max[d] >= 1.0 && q.alpha_max_si[d] >= 0.0;
+        if (!ok)
+            return fail(PIC_E_INVALID, "PML: grading orders, SIGMA_MAX_SI and ALPHA_MAX_SI must be >= 0, "
+                                       "KAPPA_MAX >= 1 (fieldAbsorber.unitless)");
+    }
     if (c.current_interpolation != PIC_CURRENT_NONE && c.current_interpolation != PIC_CURRENT_BINOMIAL)
         return fail(PIC_E_INVALID, "current_interpolation must be PIC_CURRENT_NONE or PIC_CURRENT_BINOMIAL");
     if (c.current_interpolation == PIC_CURRENT_BINOMIAL &&
@@ -179,6 +188,93 @@ int checkCfl(const pic_config& c, const pic_constants& k) {
 }
 
 // ---------------------------------------------------------------------------
+// PML coefficients (absorber/pml/Pml.kernel:60-270): per axis and index the
+// b, c and kappa of getCoefficients, for E (at the cell index, half = false)
+// or B (at index + 1/2), in the reference's float_X arithmetic. The relative
+// depth is getRelativeDepth's, the layer thickness absorber_cells on open
+// axes and 0 on periodic ones (a periodic border has a neighbour). The
+// normalised conductivity and frequency shift are fieldAbsorber.unitless's
+// SIGMA_MAX_SI / eps0 * unit time. Layout: b of x, y, z (nx + ny + nz
+// values), then c, then kappa.
+void pmlTables(const pic_config& cfg, const pic_constants& k, bool half, std::vector<float>& out) {
+    int const n[3] = {cfg.grid[0], cfg.grid[1], cfg.grid[2]};
+    int const total = n[0] + n[1] + n[2];
+    out.assign(3 * static_cast<size_t>(total), 0.0f);
+    double const z0 = kMue0SI * kSpeedOfLightSI;                        // simulation.unitless:277-280
+    double const eps0 = 1.0 / kMue0SI / kSpeedOfLightSI / kSpeedOfLightSI;  // :264-267
+    double gradingSK = 4.0, gradingA = 1.0;                             // fieldAbsorber.param:97,143
+    double sigmaMax[3], kappaMax[3] = {1.0, 1.0, 1.0}, alphaMax[3] = {0.2, 0.2, 0.2};
+    for (int d = 0; d < 3; ++d) sigmaMax[d] = 0.8 * (gradingSK + 1.0) / (z0 * cfg.cell_si[d]);  // SIGMA_OPT_SI
+    if (cfg.pml.custom != 0) {
+        gradingSK = cfg.pml.sigma_kappa_grading_order;
+        gradingA = cfg.pml.alpha_grading_order;
+        for (int d = 0; d < 3; ++d) {
+            sigmaMax[d] = cfg.pml.sigma_max_si[d];
+            kappaMax[d] = cfg.pml.kappa_max[d];
+            alphaMax[d] = cfg.pml.alpha_max_si[d];
+        }
+    }
+    float const m = static_cast<float>(gradingSK);
+    float const ma = static_cast<float>(gradingA);
+    float const dt = k.dt;
+    int off = 0;
+    for (int d = 0; d < 3; ++d) {
+        bool const open = cfg.boundary[d] == PIC_BOUNDARY_OPEN;
+        float const negB = static_cast<float>(open ? cfg.absorber_cells[d][0] : 0);
+        float const posB = static_cast<float>(open ? cfg.absorber_cells[d][1] : 0);
+        float const nSigma = static_cast<float>(sigmaMax[d] / eps0 * k.unit_time);  // NORMALIZED_SIGMA_MAX
+        float const nAlpha = static_cast<float>(alphaMax[d] / eps0 * k.unit_time);  // NORMALIZED_ALPHA_MAX
+        float const kMax = static_cast<float>(kappaMax[d]);
+        for (int i = 0; i < n[d]; ++i) {
+            float const zero = static_cast<float>(i) + (half ? 0.5f : 0.0f);
+            float depth = 0.0f;
+            if (zero < negB) {
+                depth = (negB - zero) / negB;
+            } else {
+                float const rightStart = static_cast<float>(n[d]) - posB;
+                if (zero > rightStart) depth = (zero - rightStart) / posB;
+            }
+            float sigma = 0.0f, kappa = 1.0f, alpha = 0.0f;
+            if (depth != 0.0f) {
+                float const g = std::pow(depth, m);
+                sigma = nSigma * g;
+                kappa = 1.0f + (kMax - 1.0f) * g;
+                float const ga = std::pow(1.0f - depth, ma);
+                alpha = nAlpha * ga;
+            }
+            float const b = std::exp(-(sigma / kappa + alpha) * dt);
+            float const den = kappa * (sigma + alpha * kappa);
+            out[off + i] = b;
+            out[total + off + i] = den != 0.0f ? sigma * (b - 1.0f) / den : 0.0f;
+            out[2 * total + off + i] = kappa;
+        }
+        off += n[d];
+    }
+}
+
+// The box of cells whose E and B coefficients are neutral (b == 1 along all
+// three axes, where the PML kernels take the plain update): the indices
+// along each axis that are not neutral form a prefix and a suffix (the
+// layers at both borders). Should an axis have a neutral index between
+// layer indices, its range is left empty (every cell looks its
+// coefficients up).
+void pmlInnerBox(const pic_config& cfg, const std::vector<float>& tabE, const std::vector<float>& tabB, int in0[3],
+                 int in1[3]) {
+    int const n[3] = {cfg.grid[0], cfg.grid[1], cfg.grid[2]};
+    int off = 0;
+    for (int d = 0; d < 3; ++d) {
+        auto neutral = [&](int i) { return tabE[off + i] == 1.0f && tabB[off + i] == 1.0f; };
+        int lo = 0, hi = n[d];
+        while (lo < n[d] && !neutral(lo)) ++lo;
+        while (hi > lo && !neutral(hi - 1)) --hi;
+        for (int i = lo; i < hi; ++i)
+            if (!neutral(i)) lo = hi = 0;
+        in0[d] = lo;
+        in1[d] = hi;
+        off += n[d];
+    }
+}
+
 // Incident field set-up (fields/incidentField/Solver.hpp, Functors.hpp,
 // profiles/*, DerivativeCoefficients.hpp, calculatePhaseVelocity.hpp)
 // ---------------------------------------------------------------------------
@@ -368,7 +464,7 @@ int validateIncident(const pic_config& c) {
             // checkRequirements (Solver.hpp:60-100): the surface keeps the
             // absorber plus the update margin from the border, and fits the domain
             int const margin = std::max(incidentMargin(c, true, axis), incidentMargin(c, false, axis));
-            int const th = c.absorber == PIC_ABSORBER_EXPONENTIAL ? c.absorber_cells[axis][side] : 0;
+            int const th = c.absorber != PIC_ABSORBER_NONE ? c.absorber_cells[axis][side] : 0;
             int offset;
             if (side == 0)
                 offset = c.incident_position[axis][0];
@@ -659,6 +755,9 @@ struct pic_ctx {
     float* Ealt[3] = {nullptr, nullptr, nullptr};  // receives the new E of the fused field step
     int eParity = 0;                               // flips with every swap of E and Ealt
     pic::AbsorbPlan absorb{};                      // exponential absorber layers, exchange order
+    pic::PmlArgs pml{};                            // PML: convolution fields and coefficient tables
+    bool hasPml = false;
+    int pmlIn0[3] = {0, 0, 0}, pmlIn1[3] = {0, 0, 0};  // cells outside every PML layer
     float* Jf[3] = {nullptr, nullptr, nullptr};    // binomially filtered J for the fused field step
     std::vector<pic::IncidentUpdate> incB, incE;   // updateBHalf / updateE launches, face order
     int64_t* stepDev = nullptr;                    // field step counter (the solver's currentStep)
@@ -999,8 +1098,33 @@ void fieldEnergyFromSums(const pic_ctx* c, const double sums[6], double out[7]) 
     out[0] = (eE + eB) * unitEnergy;
 }
 
+// the box of cells outside every PML layer, for the PML kernels
+pic::FieldArgs withPmlInner(const pic_ctx* c, pic::FieldArgs a) {
+    a.inner = 1;
+    for (int d = 0; d < 3; ++d) a.in0[d] = c->pmlIn0[d], a.in1[d] = c->pmlIn1[d];
+    return a;
+}
+
+pic::FieldArgs pmlFieldArgs(pic_ctx* c) { return withPmlInner(c, makeFieldArgs(c)); }
+
 // the field step runs as the two fused kernels (no incident field between its stages)
-bool fusedFieldStep(const pic_ctx* c) { return c->incB.empty() && c->incE.empty(); }
+bool fusedFieldStep(const pic_ctx* c) { return c->incB.empty() && c->incE.empty() && !c->hasPml; }
+// the PML's field step without an incident field: B half step, E full step
+// with J into Ealt, B half step
+bool pmlFieldStep(const pic_ctx* c) { return c->incB.empty() && c->incE.empty() && c->hasPml; }
+// the field step leaves the new E in Ealt and swaps the buffers
+bool fieldStepSwapsE(const pic_ctx* c) { return fusedFieldStep(c) || pmlFieldStep(c); }
+
+// B half step of update_beforeCurrent (second half: PML's psi of B stays) or
+// of update_afterCurrent (first half: psi of B advances), FDTDBase.hpp:
+// updateBSecondHalf / updateBFirstHalf
+int bHalfStage(pic_ctx* c, const pic::FieldArgs& a, bool firstHalf) {
+    if (c->hasPml)
+        PIC_HIP(pic::launchBHalfPml(c->cfg.solver, withPmlInner(c, a), c->pml, firstHalf, c->stream));
+    else
+        PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
+    return PIC_OK;
+}
 
 void dropGraphs(pic_ctx* c) {
     for (int p = 0; p < 2; ++p) {
@@ -1253,7 +1377,7 @@ int recoverCapacity(pic_ctx* c, int* skipped, int maxFrames) {
     PIC_HIP(hipMemsetAsync(c->flags, 0, sizeof(pic::StepFlags), c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     c->hostFlags |= PIC_FLAG_CAPACITY_GREW;
-    if ((f.skipped & 1u) && fusedFieldStep(c)) {
+    if ((f.skipped & 1u) && fieldStepSwapsE(c)) {
         for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
         c->eParity ^= 1;
     }
@@ -1349,6 +1473,25 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
         }
         e = hipMemcpyAsync(dTable, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(e)));
+    }
+    if (cfg->absorber == PIC_ABSORBER_PML) {
+        std::vector<float> tabE, tabB;
+        pmlTables(*cfg, c->k, false, tabE);
+        pmlTables(*cfg, c->k, true, tabB);
+        float* dE = nullptr;
+        float* dB = nullptr;
+        if (devAlloc(c, &dE, tabE.size()) || devAlloc(c, &dB, tabB.size())) return bail(PIC_E_HIP);
+        for (int q = 0; q < 6; ++q)
+            if (devAlloc(c, &c->pml.psiE[q], c->ncells) || devAlloc(c, &c->pml.psiB[q], c->ncells))
+                return bail(PIC_E_HIP);
+        e = hipMemcpyAsync(dE, tabE.data(), tabE.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dB, tabB.data(), tabB.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(e)));
+        c->pml.tabE = dE;
+        c->pml.tabB = dB;
+        c->hasPml = true;
+        pmlInnerBox(*cfg, tabE, tabB, c->pmlIn0, c->pmlIn1);
     }
     if (devAlloc(c, &c->stepDev, 1)) return bail(PIC_E_HIP);
     c->incB = incidentUpdates(*cfg, c->k, true);
@@ -1781,11 +1924,42 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
     return PIC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// the device planes of a field for pic_load_field / pic_read_field: E, B, J
+// (3) or the PML's psi (6, in NodeValues order xy, xz, yx, yz, zx, zy; the
+// kernels keep them as yx, zx, xy, zy, xz, yz)
+int fieldPlanes(pic_ctx* c, int32_t field, float* planes[6], int* n) {
+    if (field == PIC_FIELD_E || field == PIC_FIELD_B || field == PIC_FIELD_J) {
+        float** f = field == PIC_FIELD_E ? c->E : (field == PIC_FIELD_B ? c->B : c->J);
+        for (int a = 0; a < 3; ++a) planes[a] = f[a];
+        *n = 3;
+        return PIC_OK;
+    }
+    if (field == PIC_FIELD_PML_E || field == PIC_FIELD_PML_B) {
+        if (!c->hasPml) return fail(PIC_E_INVALID, "the PML fields exist only with absorber PIC_ABSORBER_PML");
+        float** psi = field == PIC_FIELD_PML_E ? c->pml.psiE : c->pml.psiB;
+        constexpr int kNodeOrder[6] = {2, 4, 0, 5, 1, 3};
+        for (int q = 0; q < 6; ++q) planes[q] = psi[kNodeOrder[q]];
+        *n = 6;
+        return PIC_OK;
+    }
+    return fail(PIC_E_INVALID, "unknown field");
+}
+
+}  // namespace
+
+extern "C" {
+
 int pic_load_field(pic_ctx* c, int32_t field, const float* data) {
     if (!c || !data) return fail(PIC_E_INVALID, "null argument");
-    float** f = field == PIC_FIELD_E ? c->E : (field == PIC_FIELD_B ? c->B : (field == PIC_FIELD_J ? c->J : nullptr));
-    if (!f) return fail(PIC_E_INVALID, "unknown field");
-    for (int a = 0; a < 3; ++a)
+    float* f[6];
+    int n = 0;
+    int const r = fieldPlanes(c, field, f, &n);
+    if (r != PIC_OK) return r;
+    for (int a = 0; a < n; ++a)
         PIC_HIP(hipMemcpyAsync(f[a], data + a * c->ncells, c->ncells * 4, hipMemcpyHostToDevice, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
@@ -1793,9 +1967,11 @@ int pic_load_field(pic_ctx* c, int32_t field, const float* data) {
 
 int pic_read_field(pic_ctx* c, int32_t field, float* data) {
     if (!c || !data) return fail(PIC_E_INVALID, "null argument");
-    float** f = field == PIC_FIELD_E ? c->E : (field == PIC_FIELD_B ? c->B : (field == PIC_FIELD_J ? c->J : nullptr));
-    if (!f) return fail(PIC_E_INVALID, "unknown field");
-    for (int a = 0; a < 3; ++a)
+    float* f[6];
+    int n = 0;
+    int const r = fieldPlanes(c, field, f, &n);
+    if (r != PIC_OK) return r;
+    for (int a = 0; a < n; ++a)
         PIC_HIP(hipMemcpyAsync(data + a * c->ncells, f[a], c->ncells * 4, hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
     return PIC_OK;
@@ -1900,11 +2076,14 @@ int pic_fdtd_before_current(pic_ctx* c) {
     TimerScope ts(c, PIC_TIMER_FIELD);
     // FDTDBase.hpp:97-121: B half step, incident B at currentStep, incident E
     // at currentStep + 0.5 * timeStep / dt, E full step
-    PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
     int r;
+    if ((r = bHalfStage(c, a, false)) != PIC_OK) return r;
     if ((r = applyIncident(c, c->incB, c->B, 0.0f)) != PIC_OK) return r;
     if ((r = applyIncident(c, c->incE, c->E, 0.5f * c->k.dt / c->k.dt)) != PIC_OK) return r;
-    PIC_HIP(pic::launchEFull(a, c->stream));
+    if (c->hasPml)
+        PIC_HIP(pic::launchEFullPml(a, c->pml, c->stream));
+    else
+        PIC_HIP(pic::launchEFull(a, c->stream));
     return PIC_OK;
 }
 
@@ -1928,9 +2107,9 @@ int pic_fdtd_after_current(pic_ctx* c) {
     // FDTDBase.hpp:151-183: absorber on E, incident B at currentStep +
     // timeStep / dt, B half step, absorber on B
     if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->E, c->absorb, c->stream));
-    int const r = applyIncident(c, c->incB, c->B, c->k.dt / c->k.dt);
+    int r = applyIncident(c, c->incB, c->B, c->k.dt / c->k.dt);
     if (r != PIC_OK) return r;
-    PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
+    if ((r = bHalfStage(c, a, true)) != PIC_OK) return r;
     if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->B, c->absorb, c->stream));
     PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->flags, c->stream));
     return PIC_OK;
@@ -1959,23 +2138,47 @@ namespace {
 // kernels; the new E lands in Ealt and the buffers swap
 
 int fieldStep(pic_ctx* c) {
+    if (pmlFieldStep(c)) {
+        // update_beforeCurrent, AddCurrentDensity, update_afterCurrent with
+        // the PML: the E full step and J in one pass into Ealt
+        pic::FieldArgs a = pmlFieldArgs(c);
+        TimerScope ts(c, PIC_TIMER_FIELD);
+        if (c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL) {
+            PIC_HIP(pic::launchBinomialJ(a, c->Jf, c->stream));
+            for (int d = 0; d < 3; ++d) a.J[d] = c->Jf[d];
+        }
+        PIC_HIP(pic::launchBHalfPml(c->cfg.solver, a, c->pml, false, c->stream));
+        PIC_HIP(pic::launchEFullPmlJ(a, c->pml, c->Ealt, c->stream));
+        for (int d = 0; d < 3; ++d) a.E[d] = c->Ealt[d];
+        PIC_HIP(pic::launchBHalfPml(c->cfg.solver, a, c->pml, true, c->stream));
+        PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->flags, c->stream));
+        for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
+        c->eParity ^= 1;
+        return PIC_OK;
+    }
     if (!fusedFieldStep(c)) {
         // the incident field adds between the half steps the fused kernels
         // join: the stage kernels in place, E full step and J in one pass
         pic::FieldArgs a = makeFieldArgs(c);
         TimerScope ts(c, PIC_TIMER_FIELD);
-        PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
         int r;
+        if ((r = bHalfStage(c, a, false)) != PIC_OK) return r;
         if ((r = applyIncident(c, c->incB, c->B, 0.0f)) != PIC_OK) return r;
         if ((r = applyIncident(c, c->incE, c->E, 0.5f * c->k.dt / c->k.dt)) != PIC_OK) return r;
-        if (c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL) {
-            PIC_HIP(pic::launchBinomialJ(a, c->Jf, c->stream));
-            for (int d = 0; d < 3; ++d) a.J[d] = c->Jf[d];
+        if (c->hasPml) {
+            // the PML's E update, then J (AddCurrentDensity) on its own
+            PIC_HIP(pic::launchEFullPml(a, c->pml, c->stream));
+            PIC_HIP(pic::launchAddCurrent(a, c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL, c->stream));
+        } else {
+            if (c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL) {
+                PIC_HIP(pic::launchBinomialJ(a, c->Jf, c->stream));
+                for (int d = 0; d < 3; ++d) a.J[d] = c->Jf[d];
+            }
+            PIC_HIP(pic::launchEFullJ(a, c->E, c->stream));
         }
-        PIC_HIP(pic::launchEFullJ(a, c->E, c->stream));
         if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->E, c->absorb, c->stream));
         if ((r = applyIncident(c, c->incB, c->B, c->k.dt / c->k.dt)) != PIC_OK) return r;
-        PIC_HIP(pic::launchBHalf(c->cfg.solver, a, c->stream));
+        if ((r = bHalfStage(c, a, true)) != PIC_OK) return r;
         if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->B, c->absorb, c->stream));
         PIC_HIP(pic::launchAdvanceStep(c->stepDev, c->flags, c->stream));
         return PIC_OK;
@@ -2036,7 +2239,7 @@ int stepGraphs(pic_ctx* c, int32_t nsteps) {
             c->graph[p] = g;
             PIC_HIP(hipGraphInstantiate(&c->graphExec[p], c->graph[p], nullptr, nullptr, 0));
             c->graphJexp[p] = c->jexp;
-        } else if (fusedFieldStep(c)) {
+        } else if (fieldStepSwapsE(c)) {
             // the replayed step writes the new E into the other buffer
             for (int d = 0; d < 3; ++d) std::swap(c->E[d], c->Ealt[d]);
             c->eParity ^= 1;

@@ -99,6 +99,11 @@ struct FieldArgs {
     float dt, c2, coeff;
     float lehe_alpha, lehe_delta, lehe_beta1, lehe_beta2;
     int lehe_dir;
+    // inner != 0: the cells in [in0, in1) per axis lie outside every PML
+    // layer, and the PML kernels take the plain update there without looking
+    // up the coefficients
+    int inner;
+    int in0[3], in1[3];
 };
 
 // launchers (return hipError_t of the launch)
@@ -142,6 +147,24 @@ hipError_t launchFill(float* p, size_t n, float v, hipStream_t s);
 // J = 0 at the start of a step (skipped while halted)
 hipError_t launchCurrentReset(const GridDev& g, float* const J[3], size_t n, hipStream_t s);
 hipError_t launchBHalf(int solver, const FieldArgs& a, hipStream_t s);
+// PML (absorber/pml/Pml.kernel:190-590): the convolution fields psi of E and
+// B (components yx, zx, xy, zy, xz, yz, one value per cell) and the
+// coefficient tables of E and B positions (b, c, kappa per axis and index:
+// b of x, y, z, then c, then kappa)
+struct PmlArgs {
+    float* psiE[6];
+    float* psiB[6];
+    const float* tabE;
+    const float* tabB;
+};
+// UpdateBHalfFunctor with PML: cells inside the layer update psi of B when
+// updatePsi and add the stretched curl; the others B -= curl E * 0.5 * dt
+// (the z-marching form when the grid fills the chip, as launchBHalf)
+hipError_t launchBHalfPml(int solver, const FieldArgs& a, const PmlArgs& p, bool updatePsi, hipStream_t s);
+// UpdateEFunctor with PML (E only; J is added by launchAddCurrent)
+hipError_t launchEFullPml(const FieldArgs& a, const PmlArgs& p, hipStream_t s);
+// the PML's E full step and J -> E (None) per cell in one pass into Enew
+hipError_t launchEFullPmlJ(const FieldArgs& a, const PmlArgs& p, float* const Enew[3], hipStream_t s);
 // E full step and J -> E (None) per cell in one pass into Enew (may be a.E)
 hipError_t launchEFullJ(const FieldArgs& a, float* const Enew[3], hipStream_t s);
 hipError_t launchEFull(const FieldArgs& a, hipStream_t s);

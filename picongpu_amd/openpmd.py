@@ -18,6 +18,11 @@ Records, as the reference names and scales them:
   with axisLabels ("z", "y", "x"), the Yee "position" of each component in
   (z, y, x) order, gridSpacing and gridUnitSI, unitSI eField / bField /
   charge/(time length^2);
+- with the PML absorber, its convolution fields "Convolutional PML E" and
+  "Convolutional PML B" (absorber/pml/Field.hpp:320-410, checkpointed by the
+  reference): components xy, xz, yx, yz, zx, zy at E's and B's Yee positions
+  in E's and B's units. The reference stores only the layer cells; here the
+  whole grid (zero outside the layers);
 - particles <species>: "position" (in-cell, float32, unitSI = cell size),
   "positionOffset" (global cell index, int32, unitSI = cell size),
   "momentum" (macro-particle momentum, unitSI = unit mass * unit speed,
@@ -61,6 +66,9 @@ _YEE = {"E": [(0.5, 0.0, 0.0), (0.0, 0.5, 0.0), (0.0, 0.0, 0.5)],
         "B": [(0.0, 0.5, 0.5), (0.5, 0.0, 0.5), (0.5, 0.5, 0.0)],
         "J": [(0.5, 0.0, 0.0), (0.0, 0.5, 0.0), (0.0, 0.0, 0.5)]}
 _FIELD_ID = {"E": abi.FIELD_E, "B": abi.FIELD_B, "J": abi.FIELD_J}
+PML_E, PML_B = "Convolutional PML E", "Convolutional PML B"
+_PML_ID = {PML_E: abi.FIELD_PML_E, PML_B: abi.FIELD_PML_B}
+_PML_OF = {PML_E: "E", PML_B: "B"}  # the field whose units and positions each takes
 _SHAPE_NAME = {abi.SHAPE_CIC: "CIC", abi.SHAPE_TSC: "TSC", abi.SHAPE_PQS: "PQS"}
 _PUSHER_NAME = {abi.PUSHER_BORIS: "Boris", abi.PUSHER_VAY: "Vay", abi.PUSHER_HIGUERA_CARY: "HigueraCary"}
 
@@ -105,6 +113,9 @@ def snapshot(sim: "abi.Simulation", species_names, with_current=False) -> dict:
                       else "none"}
     for nm in ("E", "B") + (("J",) if with_current else ()):
         snap["fields"][nm] = sim.read_field(_FIELD_ID[nm])
+    if int(cfg.absorber) == abi.ABSORBER_PML and any(int(b) == abi.BOUNDARY_OPEN for b in cfg.boundary):
+        for nm, which in _PML_ID.items():
+            snap["fields"][nm] = sim.read_field(which)
     for sp, name in enumerate(species_names):
         if name is None:
             continue
@@ -147,6 +158,7 @@ def iteration_dict(snap: dict, config: dict) -> dict:
                              "chargeCorrection": _attr("STRING", "none")}}
     unit_of = {"E": u["efield"], "B": u["bfield"], "J": u["current"]}
     for nm, data in snap["fields"].items():
+        base = _PML_OF.get(nm, nm)
         mesh = {"attributes": {
             "axisLabels": _attr("VEC_STRING", ["z", "y", "x"]),
             "dataOrder": _attr("STRING", "C"),
@@ -155,12 +167,15 @@ def iteration_dict(snap: dict, config: dict) -> dict:
             "gridSpacing": _attr("VEC_FLOAT", [cell[2], cell[1], cell[0]]),
             "gridUnitSI": _attr("DOUBLE", u["length"]),
             "timeOffset": _attr("FLOAT", 0.0),
-            "unitDimension": _attr("ARR_DBL_7", _DIM[nm]),
+            "unitDimension": _attr("ARR_DBL_7", _DIM[base]),
             "fieldSmoothing": _attr("STRING", "none")}}
-        for c, comp in enumerate("xyz"):
-            px, py, pz = _YEE[nm][c]
+        comps = abi.PML_COMPONENTS if nm in _PML_ID else "xyz"
+        for c, comp in enumerate(comps):
+            # a psi component takes the Yee position of its field's component along
+            # its first letter (the component the convolution enters)
+            px, py, pz = _YEE[base]["xyz".index(comp[0])]
             mesh[comp] = {"attributes": {"position": _attr("VEC_FLOAT", [pz, py, px]),
-                                         "unitSI": _attr("DOUBLE", unit_of[nm])},
+                                         "unitSI": _attr("DOUBLE", unit_of[base])},
                           "datatype": "FLOAT", "extent": [nz, ny, nx],
                           "data": _f32_list(np.asarray(data)[c].reshape(nz, ny, nx))}
         meshes[nm] = mesh
@@ -245,9 +260,11 @@ def read_iteration(path: str) -> tuple[int, dict]:
 
 
 def mesh_array(it: dict, name: str) -> np.ndarray:
-    """(3, nx*ny*nz) float32 planes, x fastest, of mesh `name`."""
+    """(3, nx*ny*nz) float32 planes, x fastest, of mesh `name` (6 planes in
+    NodeValues order for a PML mesh)."""
     m = it["meshes"][name]
-    return np.stack([np.asarray(m[c]["data"], np.float32).reshape(-1) for c in "xyz"])
+    comps = abi.PML_COMPONENTS if name in _PML_ID else "xyz"
+    return np.stack([np.asarray(m[c]["data"], np.float32).reshape(-1) for c in comps])
 
 
 def species_arrays(it: dict, name: str):
@@ -268,14 +285,18 @@ def write(sim: "abi.Simulation", directory: str, species_names, with_current: bo
 
 
 def restart(sim: "abi.Simulation", path: str, species_names) -> int:
-    """Load E, B and every named species of an iteration file into a context
-    created from the same configuration (with j_scale_exp set from
-    restart_config), and set the field step counter; returns the iteration."""
+    """Load E, B (the PML fields too, when the file has them) and every named
+    species of an iteration file into a context created from the same
+    configuration (with j_scale_exp set from restart_config), and set the
+    field step counter; returns the iteration."""
     step, it = read_iteration(path)
     for nm in ("E", "B"):
         sim.load_field(_FIELD_ID[nm], mesh_array(it, nm))
     if "J" in it["meshes"]:
         sim.load_field(_FIELD_ID["J"], mesh_array(it, "J"))
+    for nm, which in _PML_ID.items():
+        if nm in it["meshes"]:
+            sim.load_field(which, mesh_array(it, nm))
     for sp, name in enumerate(species_names):
         if name is None:
             continue

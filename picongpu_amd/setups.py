@@ -67,6 +67,8 @@ class Setup:
     # incidentField.param: {(axis, side): keyword arguments of abi.fill_incident_profile}
     incident: dict = None
     incident_position: tuple = ((16, -16), (16, -16), (16, -16))
+    # fieldAbsorber.param namespace pml for ABSORBER_PML (None: its defaults)
+    pml: dict = None
 
     def config(self, device: int = 0, j_scale_exp: int = 0) -> abi.Config:
         sp = [dict(mass_ratio=s.mass_ratio, charge_ratio=s.charge_ratio, shape=s.shape, pusher=s.pusher,
@@ -77,7 +79,7 @@ class Setup:
                                boundary=self.boundary, absorber=self.absorber, absorber_cells=self.absorber_cells,
                                absorber_strength=self.absorber_strength,
                                current_interpolation=self.current_interpolation, incident=self.incident,
-                               incident_position=self.incident_position)
+                               incident_position=self.incident_position, pml=self.pml)
 
     @property
     def ncells(self) -> int:
@@ -153,7 +155,7 @@ def khi_growth_rate(grid=(192, 512, 12), steps=3000, seed=0) -> Setup:
     return Setup("khi_growth_rate", tuple(grid), cell, dt, 1e25, 25, [e, p], steps=steps)
 
 
-def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0, laser="initial") -> Setup:
+def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0, laser="initial", absorber="pml") -> Setup:
     """Laser-plasma geometry: PQS shape, Vay pusher, Lehe solver Cherenkov-free
     along y, dx = dz = 0.1 um, dy = 0.05 um, cold electrons and immobile
     protons on a linear density ramp along y rising over 100 um to 3e24 m^-3
@@ -161,9 +163,12 @@ def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0, laser="initia
     laser pulse as the initial field: 0.8 um, a0 = 2, waist 8 um, intensity
     FWHM 25 fs, polarised along x, travelling along +y, centred at y = 10 um
     on the transverse axis. Borders as the reference's LaserWakefield example
-    (etc/picongpu/1.cfg: --periodic 0 0 1): x and y open, z periodic, with the
-    exponential absorber of param/fieldAbsorber.param (12 cells, strength
-    1e-3 on every side).
+    (etc/picongpu/1.cfg: --periodic 0 0 1, no --fieldAbsorber): x and y
+    open, z periodic, with the field absorber the reference then runs, the
+    PML (simulation/stage/FieldAbsorber.hpp: kindName "pml") of
+    param/fieldAbsorber.param (12 cells, its default grading, sigma, kappa
+    and alpha). absorber="exponential" takes that file's exponential
+    absorber instead (12 cells, strength 1e-3 on every side).
 
     laser="incident" injects the same pulse as the reference's
     LaserWakefield example does instead (its incidentField.param): a
@@ -192,9 +197,12 @@ def laser_plasma(grid=(192, 1536, 192), ppc=4, steps=2000, seed=0, laser="initia
                                  w0_si=8e-6, pulse_init=15.0)}
     elif laser is not None:
         raise ValueError("laser must be 'initial', 'incident' or None")
+    kinds = {"pml": abi.ABSORBER_PML, "exponential": abi.ABSORBER_EXPONENTIAL}
+    if absorber not in kinds:
+        raise ValueError("absorber must be 'pml' or 'exponential'")
     return Setup("laser_plasma", tuple(grid), cell, dt, 3e24, ppc, [e, p], solver=abi.SOLVER_LEHE, lehe_dir=1,
                  steps=steps, laser=initial, boundary=(abi.BOUNDARY_OPEN, abi.BOUNDARY_OPEN, abi.BOUNDARY_PERIODIC),
-                 absorber=abi.ABSORBER_EXPONENTIAL, absorber_cells=((12, 12), (12, 12), (12, 12)),
+                 absorber=kinds[absorber], absorber_cells=((12, 12), (12, 12), (12, 12)),
                  absorber_strength=((1e-3, 1e-3), (1e-3, 1e-3), (1e-3, 1e-3)), incident=incident)
 
 

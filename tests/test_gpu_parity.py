@@ -39,6 +39,16 @@ def open_box(st, boundary=(1, 0, 1), cells=((4, 6), (3, 5), (2, 4)), strength=((
     return st
 
 
+def pml_box(st, boundary=(1, 0, 1), cells=((4, 6), (3, 5), (2, 4)), pml=None):
+    """Open borders on the given axes with the PML (fieldAbsorber.param's
+    defaults unless pml is given)."""
+    st.boundary = boundary
+    st.absorber = abi.ABSORBER_PML
+    st.absorber_cells = cells
+    st.pml = pml
+    return st
+
+
 def small_two_species(shape=abi.SHAPE_TSC, grid=(16, 16, 12), ppc=3):
     st = setups.kelvin_helmholtz(grid=grid, ppc=ppc, shape=shape)
     for s in st.species:
@@ -145,6 +155,54 @@ def test_fields_bit_exact(solver, open_, binomial, grid):
     sim.close()
 
 
+CUSTOM_PML = dict(sigma_kappa_grading_order=3.0, sigma_max_si=(2.0e6, 1.5e6, 3.0e6), kappa_max=(1.0, 4.0, 7.0),
+                  alpha_grading_order=1.5, alpha_max_si=(0.3, 0.2, 0.15))
+# 128x64x64 takes the z-marching B half step on the inner box
+PML_CASES = [(abi.SOLVER_YEE, (1, 1, 1), None, (16, 16, 12)), (abi.SOLVER_LEHE, (1, 1, 1), None, (16, 16, 12)),
+             (abi.SOLVER_YEE, (0, 1, 1), CUSTOM_PML, (16, 16, 12)),
+             (abi.SOLVER_YEE, (1, 0, 1), None, (128, 64, 64)), (abi.SOLVER_LEHE, (1, 1, 0), None, (128, 64, 64))]
+
+
+@pytest.mark.parametrize("solver,boundary,pml,grid", PML_CASES,
+                         ids=[f"{'yee' if s == abi.SOLVER_YEE else 'lehe'}-{''.join(map(str, b))}"
+                              f"{'-custom' if p else ''}-{g[0]}" for s, b, p, g in PML_CASES])
+def test_pml_fields_bit_exact(solver, boundary, pml, grid):
+    """Eight field steps through the stage calls with the PML on the open
+    axes (psi of B advances in the first half step, E's in the E update),
+    then eight through pic_step's field path (the plain kernels on the cells
+    outside the layers, the PML kernels on the layer boxes), against the
+    oracle."""
+    st = pml_box(small_thermal(grid=grid, ppc=1), boundary=boundary, pml=pml)
+    st.solver = solver
+    st.lehe_dir = 1
+    if solver == abi.SOLVER_LEHE:
+        st.dt_si = 0.99 * st.cell_si[1] / setups.SPEED_OF_LIGHT_SI / 1.8
+    print(f"case pml fields solver={solver} boundary={boundary} custom={pml is not None} grid={grid}", flush=True)
+    sim, ora, _ = H.make_pair(st, e_amp=1.0, b_amp=1.0)
+    J = H.random_fields(st, 5, 0.1, 0.1)[0]
+    sim.load_field(abi.FIELD_J, J)
+    ora.set_field(abi.FIELD_J, J)
+    for it in range(8):
+        sim.update_before_current()
+        ora.update_before_current()
+        sim.add_current()
+        ora.add_current()
+        sim.update_after_current()
+        ora.update_after_current()
+    for it in range(8):
+        sim.fdtd_step()
+        ora.update_before_current()
+        ora.add_current()
+        ora.update_after_current()
+    for f in (abi.FIELD_E, abi.FIELD_B):
+        H.assert_field_equal(sim.read_field(f), ora.get_field(f), f"field {f}")
+    for f, which in ((abi.FIELD_PML_E, 0), (abi.FIELD_PML_B, 1)):
+        got, want = sim.read_field(f), ora.get_pml(which)
+        assert np.any(want != 0.0)
+        np.testing.assert_array_equal(got.astype(np.float64), want, err_msg=f"psi {f}")
+    sim.close()
+
+
 # launch forms of the fused field step: 64^3 runs 8x8x4 tiles; the others
 # z-march 32x8 xy tiles over chunks of 4 planes with partial xy tiles
 # (200x104x96), 2 planes (128x128x68), 8 planes with a partial last chunk
@@ -219,6 +277,8 @@ STEP_CASES = [
     ("binomial-kh-tsc", lambda: replace(small_two_species(abi.SHAPE_TSC), current_interpolation=abi.CURRENT_BINOMIAL)),
     ("emz-kh-tsc-electrons", lambda: emz(small_two_species(abi.SHAPE_TSC), species=(0,))),
     ("emz-open-cic", lambda: emz(open_box(small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS, u=0.3)))),
+    ("pml-xz-cic-boris", lambda: pml_box(small_thermal(abi.SHAPE_CIC, abi.PUSHER_BORIS, u=0.3))),
+    ("pml-xyz-tsc-vay-lehe", lambda: pml_box(small_lehe(), boundary=(1, 1, 1))),
 ]
 
 

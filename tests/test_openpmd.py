@@ -19,6 +19,8 @@ def synthetic_snapshot(step=7):
     n = 300
     fields = {nm: (rng.standard_normal((3, st.ncells)) * 10.0 ** rng.integers(-8, 8, (3, st.ncells))).astype(np.float32)
               for nm in ("E", "B", "J")}
+    for nm in (openpmd.PML_E, openpmd.PML_B):
+        fields[nm] = (rng.standard_normal((6, st.ncells)) * 1e-3).astype(np.float32)
     sp = {"name": "e", "index": 0,
           "cell": np.stack([rng.integers(0, g, n) for g in st.grid], 1).astype(np.int32),
           "pos": rng.random((n, 3), dtype=np.float32),
@@ -38,7 +40,7 @@ def test_round_trip_is_exact(tmp_path):
     assert path.endswith("simData_7.json")
     step, it = openpmd.read_iteration(path)
     assert step == 7
-    for nm in ("E", "B", "J"):
+    for nm in ("E", "B", "J", openpmd.PML_E, openpmd.PML_B):
         got = openpmd.mesh_array(it, nm)
         np.testing.assert_array_equal(got.view(np.uint32), snap["fields"][nm].view(np.uint32))
     cell, pos, mom, w = openpmd.species_arrays(it, "e")
@@ -73,6 +75,14 @@ def test_layout_and_units(tmp_path):
     np.testing.assert_allclose(gs, st.cell_si[::-1], rtol=1e-6)
     assert E["x"]["attributes"]["position"]["value"] == [0.0, 0.0, 0.5]  # E_x at (1/2, 0, 0), listed z, y, x
     assert it["meshes"]["B"]["x"]["attributes"]["position"]["value"] == [0.5, 0.5, 0.0]
+    # the PML's psi: NodeValues components, each at the Yee position and in the
+    # unit of the field component it enters (its first letter)
+    P = it["meshes"][openpmd.PML_E]
+    assert [c for c in P if c != "attributes"] == ["xy", "xz", "yx", "yz", "zx", "zy"]
+    assert P["yz"]["attributes"]["position"]["value"] == E["y"]["attributes"]["position"]["value"]
+    assert P["zx"]["attributes"]["unitSI"]["value"] == E["z"]["attributes"]["unitSI"]["value"]
+    assert it["meshes"][openpmd.PML_B]["xz"]["attributes"]["position"]["value"] == [0.5, 0.5, 0.0]
+    assert P["attributes"]["unitDimension"]["value"] == E["attributes"]["unitDimension"]["value"]
     assert E["x"]["extent"] == [st.grid[2], st.grid[1], st.grid[0]]
     # E unit: V/m per PIC unit; the electron's frame charge times E times dt gives momentum in the momentum unit
     assert E["x"]["attributes"]["unitSI"]["value"] == pytest.approx(u["mass"] * u["length"] /
