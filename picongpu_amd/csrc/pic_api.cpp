@@ -2081,7 +2081,7 @@ int pic_fdtd_before_current(pic_ctx* c) {
     if ((r = applyIncident(c, c->incB, c->B, 0.0f)) != PIC_OK) return r;
     if ((r = applyIncident(c, c->incE, c->E, 0.5f * c->k.dt / c->k.dt)) != PIC_OK) return r;
     if (c->hasPml)
-        PIC_HIP(pic::launchEFullPml(a, c->pml, c->stream));
+        PIC_HIP(pic::launchEFullPml(withPmlInner(c, a), c->pml, c->stream));
     else
         PIC_HIP(pic::launchEFull(a, c->stream));
     return PIC_OK;
@@ -2165,17 +2165,17 @@ int fieldStep(pic_ctx* c) {
         if ((r = bHalfStage(c, a, false)) != PIC_OK) return r;
         if ((r = applyIncident(c, c->incB, c->B, 0.0f)) != PIC_OK) return r;
         if ((r = applyIncident(c, c->incE, c->E, 0.5f * c->k.dt / c->k.dt)) != PIC_OK) return r;
-        if (c->hasPml) {
-            // the PML's E update, then J (AddCurrentDensity) on its own
-            PIC_HIP(pic::launchEFullPml(a, c->pml, c->stream));
-            PIC_HIP(pic::launchAddCurrent(a, c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL, c->stream));
-        } else {
-            if (c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL) {
-                PIC_HIP(pic::launchBinomialJ(a, c->Jf, c->stream));
-                for (int d = 0; d < 3; ++d) a.J[d] = c->Jf[d];
-            }
-            PIC_HIP(pic::launchEFullJ(a, c->E, c->stream));
+        pic::FieldArgs aj = a;
+        if (c->cfg.current_interpolation == PIC_CURRENT_BINOMIAL) {
+            PIC_HIP(pic::launchBinomialJ(a, c->Jf, c->stream));
+            for (int d = 0; d < 3; ++d) aj.J[d] = c->Jf[d];
         }
+        // E full step and J (AddCurrentDensity) in one pass, in place: a cell
+        // reads only its own E and J besides B
+        if (c->hasPml)
+            PIC_HIP(pic::launchEFullPmlJ(withPmlInner(c, aj), c->pml, c->E, c->stream));
+        else
+            PIC_HIP(pic::launchEFullJ(aj, c->E, c->stream));
         if (c->absorb.n > 0) PIC_HIP(pic::launchAbsorb(c->g, c->E, c->absorb, c->stream));
         if ((r = applyIncident(c, c->incB, c->B, c->k.dt / c->k.dt)) != PIC_OK) return r;
         if ((r = bHalfStage(c, a, true)) != PIC_OK) return r;
