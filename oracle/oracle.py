@@ -117,6 +117,8 @@ def lib() -> C.CDLL:
             "oracle_set_field": (C.c_int32, [_P, C.c_int32, _F]),
             "oracle_get_field": (C.c_int32, [_P, C.c_int32, _D]),
             "oracle_get_pml": (C.c_int32, [_P, C.c_int32, _D]),
+            "oracle_exp_f64": (C.c_double, [C.c_double]),
+            "oracle_log_f64": (C.c_double, [C.c_double]),
             "oracle_current_reset": (None, [_P]),
             "oracle_push": (None, [_P, C.c_int32, C.c_int32]),
             "oracle_deposit": (None, [_P, C.c_int32]),

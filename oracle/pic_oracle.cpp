@@ -375,6 +375,12 @@ struct State {
     bool pml = false;
     std::vector<T> psiE[6], psiB[6];
     std::vector<float> pmlE, pmlB;
+    // the PML's weight damping of absorbing species (pmlDampSetup)
+    int dampAxes = 0;
+    int dampLo[3] = {0, 0, 0}, dampHi[3] = {0, 0, 0};
+    float dampNeg[3] = {0, 0, 0}, dampPos[3] = {0, 0, 0}, dampN[3] = {0, 0, 0}, dampSigma[3] = {0, 0, 0};
+    float dampGrading = 0.0f;
+    int dampGuard[3] = {SCX, SCY, SCZ};
 };
 
 // PML coefficients (absorber/pml/Pml.kernel:60-270: getRelativeDepth,
@@ -435,6 +441,31 @@ void pmlTables(const oracle_config& cfg, const Consts<float>& k, bool half, std:
             out[2 * total + off + i] = kappa;
         }
         off += n[d];
+    }
+}
+
+// the damping areas and layer parameters of dampWeightsInPml per open axis
+// with a layer: the supercells the layer reaches at each side, the
+// thicknesses, NORMALIZED_SIGMA_MAX and the grading order (as pmlTables)
+template <typename T>
+void pmlDampSetup(const oracle_config& cfg, const Consts<float>& k, State<T>& st) {
+    double const z0 = kMue0 * kC;
+    double const eps0 = 1.0 / kMue0 / kC / kC;
+    double mSK = cfg.pml.custom ? cfg.pml.sigma_kappa_grading_order : 4.0;
+    st.dampGrading = static_cast<float>(mSK);
+    int const scs[3] = {SCX, SCY, SCZ};
+    for (int d = 0; d < 3; ++d) {
+        double const sigmaMax = cfg.pml.custom ? cfg.pml.sigma_max_si[d] : 0.8 * (4.0 + 1.0) / (z0 * cfg.cell_si[d]);
+        bool const open = cfg.boundary[d] != 0;
+        int const n = cfg.grid[d];
+        int const neg = open ? cfg.absorber_cells[d][0] : 0, pos = open ? cfg.absorber_cells[d][1] : 0;
+        st.dampNeg[d] = static_cast<float>(neg);
+        st.dampPos[d] = static_cast<float>(pos);
+        st.dampN[d] = static_cast<float>(n);
+        st.dampSigma[d] = static_cast<float>(sigmaMax / eps0 * k.unit_time);
+        st.dampLo[d] = neg > 0 ? scs[d] * ((neg + scs[d] - 1) / scs[d]) : 0;
+        st.dampHi[d] = pos > 0 ? n - scs[d] * ((pos + scs[d] - 1) / scs[d]) : n;
+        if (open && (neg > 0 || pos > 0)) st.dampAxes |= 1 << d;
     }
 }
 
@@ -939,13 +970,155 @@ bool reflectParticle(const State<T>& st, const Species<T>& S, int* cell, T* pos,
     return any;
 }
 
+// ---- PML weight damping of an absorbing particle boundary ----
+// boundary/Absorbing.hpp:95-200 (dampWeightsInPml, DampWeightsInPml) with
+// fields/absorber/pml/Pml.kernel:116-136 (getRelativeDepth), :196-216
+// (getNormalizedSigma), :326-386 (getNormalizedSigmaIntegral) and
+// traits/attribute/DampedWeighting.hpp:67-90 (dampWeighting, MIN_WEIGHTING
+// 10 of particle.param). After boundary::apply removed the particles beyond
+// the border, each exchange +x, -x, +y, -y, +z, -z of an open axis on which
+// the species is absorbing damps the particles in the supercells its layer
+// reaches (particle boundary offset 0): the integral of sigma / eps0 over the
+// last step's straight path, back from the particle's position by its
+// velocity, scales the weighting (kept at MIN_WEIGHTING or above) and the
+// momentum. The reference's FP32 pow and exp are taken, as on the device,
+// as FP64 results rounded to FP32 from the same FP64 routines: e^x by Cody
+// and Waite's reduction x = k ln2 + r (ln2 in the two parts of fdlibm) and
+// the Taylor polynomial of e^r to degree 13; ln d from d = m 2^e, m in
+// [sqrt(1/2), sqrt(2)), as 2 atanh((m - 1) / (m + 1)) by its series to the
+// 21st power; pow(d, g) = e^(g ln d), or repeated products for an integer
+// g. They are pinned against the C library's exp and log in the damping's
+// tests.
+double expNegF64(double x) {
+    if (x < -745.0) return 0.0;
+    double const k = std::nearbyint(x * 1.4426950408889634);
+    double const r = (x - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+    static const double inv[14] = {1.0,
+                                   1.0,
+                                   0.5,
+                                   1.0 / 6.0,
+                                   1.0 / 24.0,
+                                   1.0 / 120.0,
+                                   1.0 / 720.0,
+                                   1.0 / 5040.0,
+                                   1.0 / 40320.0,
+                                   1.0 / 362880.0,
+                                   1.0 / 3628800.0,
+                                   1.0 / 39916800.0,
+                                   1.0 / 479001600.0,
+                                   1.0 / 6227020800.0};
+    double p = inv[13];
+    for (int i = 12; i >= 0; --i) p = p * r + inv[i];
+    return std::ldexp(p, static_cast<int>(k));
+}
+
+double logUnitF64(double d) {
+    int e;
+    double m = std::frexp(d, &e);
+    if (m < 0.70710678118654752440) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double const s = (m - 1.0) / (m + 1.0);
+    double const s2 = s * s;
+    double q = 1.0 / 21.0;
+    for (int n = 19; n >= 1; n -= 2) q = q * s2 + 1.0 / static_cast<double>(n);
+    double const lnm = 2.0 * s * q;
+    double const de = static_cast<double>(e);
+    return (de * 6.93147180369123816490e-01 + lnm) + de * 1.90821492927058770002e-10;
+}
+
+// x^y: repeated FP64 products for an integer order in [0, 32], else
+// e^(y ln x)
+template <typename T>
+T powOf(T x, T y) {
+    if (y >= T(0) && y <= T(32) && y == std::floor(y)) {
+        double r = 1.0;
+        for (int i = 0; i < static_cast<int>(y); ++i) r = r * static_cast<double>(x);
+        return static_cast<T>(r);
+    }
+    return static_cast<T>(expNegF64(static_cast<double>(y) * logUnitF64(static_cast<double>(x))));
+}
+
+template <typename T>
+T expOf(T x) {
+    return static_cast<T>(expNegF64(static_cast<double>(x)));
+}
+
+template <typename T>
+T dampSigma(const State<T>& st, int d, T pos) {
+    // getNormalizedSigma adds the guard cells, getRelativeDepth takes them off again
+    T const withGuard = pos + static_cast<T>(st.dampGuard[d]);
+    T const zero = withGuard - static_cast<T>(st.dampGuard[d]);
+    T const neg = static_cast<T>(st.dampNeg[d]), posB = static_cast<T>(st.dampPos[d]);
+    T depth = T(0);
+    if (zero < neg) {
+        depth = (neg - zero) / neg;
+    } else {
+        T const rightStart = static_cast<T>(st.dampN[d]) - posB;
+        if (zero > rightStart) depth = (zero - rightStart) / posB;
+    }
+    if (depth == T(0)) return T(0);
+    return static_cast<T>(st.dampSigma[d]) * powOf(depth, static_cast<T>(st.dampGrading));
+}
+
+template <typename T>
+T dampIntegral(const State<T>& st, int d, T start, T finish, T dt) {
+    T const vel = (finish - start) / dt;
+    if (static_cast<double>(std::abs(vel)) < 1e-5 * static_cast<double>(st.k.cell[d])) {
+        T const s0 = dampSigma(st, d, start), s1 = dampSigma(st, d, finish);
+        return dt * T(0.5) * (s0 + s1);
+    }
+    T as = start, af = finish;
+    T const negEnd = static_cast<T>(st.dampNeg[d]);
+    if (finish < negEnd && start >= negEnd) as = negEnd;
+    if (start < negEnd && finish >= negEnd) af = negEnd;
+    T const posStart = static_cast<T>(st.dampN[d]) - static_cast<T>(st.dampPos[d]);
+    if (finish > posStart && start <= posStart) as = posStart;
+    if (start > posStart && finish <= posStart) af = posStart;
+    T const adjT = (af - as) / vel;
+    T depthCoeff = T(0);
+    if (as <= negEnd) depthCoeff = as - negEnd;
+    if (as >= posStart) depthCoeff = as - posStart;
+    T const s0 = dampSigma(st, d, as), s1 = dampSigma(st, d, af);
+    return (s1 * adjT + (s1 - s0) * depthCoeff / vel) / (static_cast<T>(st.dampGrading) + T(1));
+}
+
+// returns whether any exchange damped the particle
+template <typename T>
+bool dampInPml(const State<T>& st, const Species<T>& S, const int* cell, const T* pos, V3<T>& mom, T& w) {
+    if (!st.dampAxes || st.outside(cell[0], cell[1], cell[2])) return false;
+    bool any = false;
+    for (int d = 0; d < 3; ++d) {
+        if (!((st.dampAxes >> d) & 1) || S.desc.particle_boundary[d] == 1) continue;
+        for (int side = 0; side < 2; ++side) {  // the max side's exchange comes first
+            bool const inArea = side == 0 ? cell[d] >= st.dampHi[d] : cell[d] < st.dampLo[d];
+            if (!inArea) continue;
+            V3<T> const vel = velocityOf(mom, S.mass * w, st.k.mue0_eps0);
+            T const current = static_cast<T>(cell[d]) + pos[d];
+            T const previous = current - vel[d] * st.k.dt / st.k.cell[d];
+            T const integral = dampIntegral(st, d, previous, current, st.k.dt);
+            T const multiplier = expOf(-integral);
+            T const attempted = w * multiplier;
+            T const damped = std::max(attempted, T(10));
+            T const ratio = damped / w;
+            mom.x = mom.x * ratio;
+            mom.y = mom.y * ratio;
+            mom.z = mom.z * ratio;
+            w = damped;
+            any = true;
+        }
+    }
+    return any;
+}
+
 template <typename T>
 V3<T> pushOne(State<T>& st, Species<T>& S, size_t g, int ox, int oy, int oz, int* step) {
     int const lc = S.cell[g];
     int const cx = ox + lc % SCX, cy = oy + (lc / SCX) % SCY, cz = oz + lc / (SCX * SCY);
     T pos[3] = {S.pos[0][g], S.pos[1][g], S.pos[2][g]};
     V3<T> mom{S.mom[0][g], S.mom[1][g], S.mom[2][g]};
-    T const w = S.w[g];
+    T w = S.w[g];
     T const mass = S.mass * w;
     T const charge = S.charge * w;
     // YeeCell.hpp:69-123 staggering
@@ -972,6 +1145,11 @@ V3<T> pushOne(State<T>& st, Species<T>& S, size_t g, int ox, int oy, int oz, int
         S.cell[g] = lc0;
         S.mask[g] = mask0;
         cellAfterSteps(step, S.cell[g], S.mask[g]);
+    }
+    if (dampInPml(st, S, cellNow, pos, mom, w)) {
+        // the deposit stage takes the velocity and charge of the damped particle
+        S.w[g] = w;
+        vel = velocityOf(mom, S.mass * w, st.k.mue0_eps0);
     }
     for (int d = 0; d < 3; ++d) S.pos[d][g] = pos[d];
     S.mom[0][g] = mom.x;
@@ -1509,12 +1687,14 @@ void addCurrent(State<T>& st) {
 }
 
 // Exponential absorber (absorber/exponential/Exponential.hpp:52-95,
-// Exponential.kernel:63-112): for the plane exchanges LEFT, RIGHT, BOTTOM,
-// TOP, BACK, FRONT in this order, on open axes, the cell at distance d from
-// that border (d = 0 outermost) is multiplied by exp(-strength * (T-1-d)) for
-// T-1-d > 0. Thickness and strength are read at [axis][exchange % 2], so the
-// negative side uses entry 1 and the positive side entry 0, as the reference
-// indexes them.
+// Exponential.kernel:63-112): for the plane exchanges 1, 2, 3, 6, 9, 18 in
+// this order (RIGHT +x, LEFT -x, BOTTOM +y, TOP -y, BACK +z, FRONT -z,
+// pmacc/type/Exchange.hpp:48-53; an odd code is the max side,
+// pmacc/boundary/Utility.hpp isMaxSide), on open axes, the cell at distance
+// d from that border (d = 0 outermost) is multiplied by
+// exp(-strength * (T-1-d)) for T-1-d > 0. Thickness and strength are read
+// at [axis][exchange % 2]: entry 0 on the negative side, entry 1 on the
+// positive one, fieldAbsorber.param's {negative, positive}.
 template <typename T>
 void absorb(State<T>& st, std::vector<T>* F) {
     if (st.cfg.absorber != 1) return;
@@ -1522,7 +1702,7 @@ void absorb(State<T>& st, std::vector<T>* F) {
     int const exchanges[6] = {1, 2, 3, 6, 9, 18};
     for (int e : exchanges) {
         int const axis = e >= 9 ? 2 : (e >= 3 ? 1 : 0);
-        bool const minSide = e == 1 || e == 3 || e == 9;
+        bool const minSide = e % 2 == 0;
         int const th = st.cfg.absorber_cells[axis][e % 2];
         T const strength = static_cast<T>(st.cfg.absorber_strength[axis][e % 2]);
         if (!st.cfg.boundary[axis] || th <= 1) continue;
@@ -2115,6 +2295,7 @@ State<T>* create(const oracle_config& cfg, int order) {
         Consts<float> const kf = makeConsts<float>(cfg);
         pmlTables(cfg, kf, false, st->pmlE);
         pmlTables(cfg, kf, true, st->pmlB);
+        pmlDampSetup(cfg, kf, *st);
         size_t const n = static_cast<size_t>(cfg.grid[0]) * cfg.grid[1] * cfg.grid[2];
         for (int q = 0; q < 6; ++q) {
             st->psiE[q].assign(n, T(0));
@@ -2338,6 +2519,10 @@ int32_t oracle_get_field(void* p, int32_t which, double* data) {
     };
     return dispatch<int32_t>(static_cast<Handle*>(p), impl, impl);
 }
+
+// the damping's FP64 e^x and ln x (test hooks)
+double oracle_exp_f64(double x) { return expNegF64(x); }
+double oracle_log_f64(double x) { return logUnitF64(x); }
 
 // the PML's psi of E (which 0) or B (1): six planes in the reference's
 // NodeValues order xy, xz, yx, yz, zx, zy (absorber/pml/Field.hpp); -1

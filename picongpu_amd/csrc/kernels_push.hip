@@ -103,7 +103,7 @@ struct PushLds {
 
 // The work of one supercell `sc` of the simulation described by `a`; every
 // thread of the workgroup calls it.
-template <int SUPP, int PUSHER, int MODE, bool EMZ = false>
+template <int SUPP, int PUSHER, int MODE, bool EMZ = false, bool DAMP = false>
 __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigned char* smem) {
     constexpr int GLO = SUPP / 2;          // FieldToParticleInterpolation.hpp:49
     constexpr int GUP = (SUPP + 1) / 2;    // FieldToParticleInterpolation.hpp:50
@@ -214,11 +214,11 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                 size_t const gidx = base + i;
                 Vec3 const pos{S.pos[0][gidx], S.pos[1][gidx], S.pos[2][gidx]};
                 Vec3 mom{S.mom[0][gidx], S.mom[1][gidx], S.mom[2][gidx]};
-                float const w = S.w[gidx];
+                float w = S.w[gidx];
                 int const lidx = S.cell[gidx];
                 int const lx = lidx % SCX, ly = (lidx / SCX) % SCY, lz = lidx / (SCX * SCY);
                 float const mass = S.mass * w;     // attribute::getMass (GetMass.hpp:40-44)
-                float const charge = S.charge * w; // attribute::getCharge (GetCharge.hpp:45-66)
+                float charge = S.charge * w;       // attribute::getCharge (GetCharge.hpp:45-66)
                 if (MODE != 2) {
                     Vec3 const e = gatherE<SUPP, GX, GY, GV>(sEB, lx + GLO, ly + GLO, lz + GLO, pos);
                     Vec3 const b = gatherB<SUPP, GX, GY, GV>(sEB + 3 * GV, lx + GLO, ly + GLO, lz + GLO, pos);
@@ -243,6 +243,34 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                         int const nGrid[3] = {a.g.nx, a.g.ny, a.g.nz};
                         if (reflectParticle(nGrid, S.reflect, cellNow, p, mom, vel, dir, dp))
                             cellAfterMove(lidx, dir, newIdx, newMask);
+                    }
+                    if (DAMP && a.damp.axes) {
+                        // the absorbing boundary's PML damping, exchange by exchange
+                        // (+x, -x, +y, -y, +z, -z); the deposit stage then takes the
+                        // velocity and charge of the damped particle
+                        int const cellNow[3] = {ox + lx + dir[0], oy + ly + dir[1], oz + lz + dir[2]};
+                        bool any = false;
+                        if (!(a.g.open && outsideOpen(a.g, cellNow[0], cellNow[1], cellNow[2]))) {
+                            for (int d = 0; d < 3; ++d) {
+                                if (!((a.damp.axes >> d) & 1) || ((S.reflect >> d) & 1)) continue;
+                                if (cellNow[d] >= a.damp.hi[d]) {
+                                    pmlDamp(a.damp, d, cellNow[d], p[d], S.mass, pk.rc2, pk.dt, pk.cell[d], mom, w);
+                                    any = true;
+                                }
+                                if (cellNow[d] < a.damp.lo[d]) {
+                                    pmlDamp(a.damp, d, cellNow[d], p[d], S.mass, pk.rc2, pk.dt, pk.cell[d], mom, w);
+                                    any = true;
+                                }
+                            }
+                        }
+                        if (any) {
+                            S.w[gidx] = w;
+                            charge = S.charge * w;
+                            vel = velocityOf(mom, S.mass * w, pk.rc2);
+                            dp[0] = (vel.x * pk.dt) / pk.cell[0];
+                            dp[1] = (vel.y * pk.dt) / pk.cell[1];
+                            dp[2] = (vel.z * pk.dt) / pk.cell[2];
+                        }
                     }
                     S.pos[0][gidx] = p[0];
                     S.pos[1][gidx] = p[1];
@@ -384,6 +412,15 @@ __global__ __launch_bounds__(kPushThreads, SUPP == 4 ? 4 : (SUPP == 3 ? 6 : 1)) 
     pushDepositSC<SUPP, PUSHER, MODE, EMZ>(a, a.scList[blockIdx.x], smem);
 }
 
+// the same with the PML's weight damping of absorbing species (a kernel of
+// its own: the FP64 pow and exp of the damping would raise the register
+// pressure of every launch)
+template <int SUPP, int PUSHER, int MODE, bool EMZ = false>
+__global__ __launch_bounds__(kPushThreads, SUPP == 4 ? 4 : (SUPP == 3 ? 6 : 1)) void kPushDepositPml(PushArgs a) {
+    __shared__ __align__(16) unsigned char smem[PushLds<SUPP, MODE>::BYTES];
+    pushDepositSC<SUPP, PUSHER, MODE, EMZ, true>(a, a.scList[blockIdx.x], smem);
+}
+
 // Batched form for ensembles: work items (simulation, supercell) of one
 // checkerboard phase of every simulation in one launch; a workgroup takes
 // items blockIdx.x, blockIdx.x + gridDim.x, ... so a grid smaller than the
@@ -425,7 +462,10 @@ hipError_t launchPushBatch(int supp, int pusher, const PushArgs* table, const in
 
 template <int SUPP, int PUSHER, int MODE, bool EMZ>
 static hipError_t launchT(const PushArgs& a, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((kPushDeposit<SUPP, PUSHER, MODE, EMZ>), dim3(nblocks), dim3(kPushThreads), 0, s, a);
+    if (MODE != 2 && a.damp.axes)
+        hipLaunchKernelGGL((kPushDepositPml<SUPP, PUSHER, MODE, EMZ>), dim3(nblocks), dim3(kPushThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((kPushDeposit<SUPP, PUSHER, MODE, EMZ>), dim3(nblocks), dim3(kPushThreads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -196,35 +196,51 @@ int checkCfl(const pic_config& c, const pic_constants& k) {
 // normalised conductivity and frequency shift are fieldAbsorber.unitless's
 // SIGMA_MAX_SI / eps0 * unit time. Layout: b of x, y, z (nx + ny + nz
 // values), then c, then kappa.
+// fieldAbsorber.param's namespace pml (its defaults unless custom) and the
+// normalised maxima of fieldAbsorber.unitless (SI value / eps0 * unit time)
+struct PmlParameters {
+    double gradingSK = 4.0, gradingA = 1.0;  // fieldAbsorber.param:97,143
+    double sigmaMax[3], kappaMax[3] = {1.0, 1.0, 1.0}, alphaMax[3] = {0.2, 0.2, 0.2};
+    float nSigma[3], nAlpha[3];
+};
+
+PmlParameters pmlParameters(const pic_config& cfg, const pic_constants& k) {
+    PmlParameters p;
+    double const z0 = kMue0SI * kSpeedOfLightSI;                        // simulation.unitless:277-280
+    double const eps0 = 1.0 / kMue0SI / kSpeedOfLightSI / kSpeedOfLightSI;  // :264-267
+    for (int d = 0; d < 3; ++d) p.sigmaMax[d] = 0.8 * (p.gradingSK + 1.0) / (z0 * cfg.cell_si[d]);  // SIGMA_OPT_SI
+    if (cfg.pml.custom != 0) {
+        p.gradingSK = cfg.pml.sigma_kappa_grading_order;
+        p.gradingA = cfg.pml.alpha_grading_order;
+        for (int d = 0; d < 3; ++d) {
+            p.sigmaMax[d] = cfg.pml.sigma_max_si[d];
+            p.kappaMax[d] = cfg.pml.kappa_max[d];
+            p.alphaMax[d] = cfg.pml.alpha_max_si[d];
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        p.nSigma[d] = static_cast<float>(p.sigmaMax[d] / eps0 * k.unit_time);  // NORMALIZED_SIGMA_MAX
+        p.nAlpha[d] = static_cast<float>(p.alphaMax[d] / eps0 * k.unit_time);  // NORMALIZED_ALPHA_MAX
+    }
+    return p;
+}
+
 void pmlTables(const pic_config& cfg, const pic_constants& k, bool half, std::vector<float>& out) {
     int const n[3] = {cfg.grid[0], cfg.grid[1], cfg.grid[2]};
     int const total = n[0] + n[1] + n[2];
     out.assign(3 * static_cast<size_t>(total), 0.0f);
-    double const z0 = kMue0SI * kSpeedOfLightSI;                        // simulation.unitless:277-280
-    double const eps0 = 1.0 / kMue0SI / kSpeedOfLightSI / kSpeedOfLightSI;  // :264-267
-    double gradingSK = 4.0, gradingA = 1.0;                             // fieldAbsorber.param:97,143
-    double sigmaMax[3], kappaMax[3] = {1.0, 1.0, 1.0}, alphaMax[3] = {0.2, 0.2, 0.2};
-    for (int d = 0; d < 3; ++d) sigmaMax[d] = 0.8 * (gradingSK + 1.0) / (z0 * cfg.cell_si[d]);  // SIGMA_OPT_SI
-    if (cfg.pml.custom != 0) {
-        gradingSK = cfg.pml.sigma_kappa_grading_order;
-        gradingA = cfg.pml.alpha_grading_order;
-        for (int d = 0; d < 3; ++d) {
-            sigmaMax[d] = cfg.pml.sigma_max_si[d];
-            kappaMax[d] = cfg.pml.kappa_max[d];
-            alphaMax[d] = cfg.pml.alpha_max_si[d];
-        }
-    }
-    float const m = static_cast<float>(gradingSK);
-    float const ma = static_cast<float>(gradingA);
+    PmlParameters const P = pmlParameters(cfg, k);
+    float const m = static_cast<float>(P.gradingSK);
+    float const ma = static_cast<float>(P.gradingA);
     float const dt = k.dt;
     int off = 0;
     for (int d = 0; d < 3; ++d) {
         bool const open = cfg.boundary[d] == PIC_BOUNDARY_OPEN;
         float const negB = static_cast<float>(open ? cfg.absorber_cells[d][0] : 0);
         float const posB = static_cast<float>(open ? cfg.absorber_cells[d][1] : 0);
-        float const nSigma = static_cast<float>(sigmaMax[d] / eps0 * k.unit_time);  // NORMALIZED_SIGMA_MAX
-        float const nAlpha = static_cast<float>(alphaMax[d] / eps0 * k.unit_time);  // NORMALIZED_ALPHA_MAX
-        float const kMax = static_cast<float>(kappaMax[d]);
+        float const nSigma = P.nSigma[d];
+        float const nAlpha = P.nAlpha[d];
+        float const kMax = static_cast<float>(P.kappaMax[d]);
         for (int i = 0; i < n[d]; ++i) {
             float const zero = static_cast<float>(i) + (half ? 0.5f : 0.0f);
             float depth = 0.0f;
@@ -250,6 +266,35 @@ void pmlTables(const pic_config& cfg, const pic_constants& k, bool half, std::ve
         }
         off += n[d];
     }
+}
+
+// The PML weight damping of absorbing species (boundary/Absorbing.hpp:95-130
+// dampWeightsInPml): on each open axis with a layer on a side, the area of
+// that side's exchange is the supercells the layer reaches (particle boundary
+// offset 0), the layer parameters those of getNormalizedSigma.
+pic::PmlDampArgs pmlDampArgs(const pic_config& cfg, const pic_constants& k) {
+    pic::PmlDampArgs a{};
+    PmlParameters const P = pmlParameters(cfg, k);
+    int const scs[3] = {pic::kSCX, pic::kSCY, pic::kSCZ};
+    a.grading = static_cast<float>(P.gradingSK);
+    a.gradingInt = (a.grading >= 0.0f && a.grading <= 32.0f && a.grading == std::floor(a.grading))
+                       ? static_cast<int>(a.grading)
+                       : -1;
+    for (int d = 0; d < 3; ++d) {
+        int const n = cfg.grid[d];
+        bool const open = cfg.boundary[d] == PIC_BOUNDARY_OPEN;
+        int const neg = open ? cfg.absorber_cells[d][0] : 0, pos = open ? cfg.absorber_cells[d][1] : 0;
+        a.negB[d] = static_cast<float>(neg);
+        a.posB[d] = static_cast<float>(pos);
+        a.nCells[d] = static_cast<float>(n);
+        a.guard[d] = scs[d];  // memory.param GuardSize: one supercell
+        a.sigmaMax[d] = P.nSigma[d];
+        a.slow[d] = 1e-5 * static_cast<double>(k.cell[d]);
+        a.lo[d] = neg > 0 ? scs[d] * ((neg + scs[d] - 1) / scs[d]) : 0;
+        a.hi[d] = pos > 0 ? n - scs[d] * ((pos + scs[d] - 1) / scs[d]) : n;
+        if (open && (neg > 0 || pos > 0)) a.axes |= 1 << d;
+    }
+    return a;
 }
 
 // The box of cells whose E and B coefficients are neutral (b == 1 along all
@@ -757,6 +802,7 @@ struct pic_ctx {
     pic::AbsorbPlan absorb{};                      // exponential absorber layers, exchange order
     pic::PmlArgs pml{};                            // PML: convolution fields and coefficient tables
     bool hasPml = false;
+    pic::PmlDampArgs pmlDamp{};                    // the PML's weight damping of absorbing species
     int pmlIn0[3] = {0, 0, 0}, pmlIn1[3] = {0, 0, 0};  // cells outside every PML layer
     float* Jf[3] = {nullptr, nullptr, nullptr};    // binomially filtered J for the fused field step
     std::vector<pic::IncidentUpdate> incB, incE;   // updateBHalf / updateE launches, face order
@@ -972,6 +1018,7 @@ pic::PushArgs makePushArgs(pic_ctx* c) {
     a.inv_vdt = c->k.inv_cell_volume_dt;
     a.jscale = std::ldexp(1.0, c->jexp);
     a.jinv = std::ldexp(1.0, -c->jexp);
+    a.damp = c->pmlDamp;
     return a;
 }
 
@@ -1449,9 +1496,12 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
         for (int d = 0; d < 3; ++d)
             if (devAlloc(c, &c->Jf[d], c->ncells)) return bail(PIC_E_HIP);
     if (cfg->absorber == PIC_ABSORBER_EXPONENTIAL) {
-        // ExponentialImpl::run (Exponential.hpp:52-95): plane exchanges LEFT,
-        // RIGHT, BOTTOM, TOP, BACK, FRONT in this order, on borders without a
-        // neighbour (open axes), thickness and strength read at [axis][exchange % 2];
+        // ExponentialImpl::run (Exponential.hpp:52-95): plane exchanges 1, 2,
+        // 3, 6, 9, 18 in this order (RIGHT +x, LEFT -x, BOTTOM +y, TOP -y,
+        // BACK +z, FRONT -z: pmacc/type/Exchange.hpp:48-53, an odd code is
+        // the max side, pmacc/boundary/Utility.hpp isMaxSide), on borders
+        // without a neighbour (open axes), thickness and strength read at
+        // [axis][exchange % 2] ({negative, positive} of fieldAbsorber.param);
         // the factors exp(-strength * f) of Exponential.kernel:99 in float
         constexpr int kExchanges[6] = {1, 2, 3, 6, 9, 18};
         constexpr int kStride = PIC_ABSORBER_MAX_CELLS + 1;
@@ -1467,7 +1517,7 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
             for (int f = 1; f < th; ++f) table[l * kStride + f] = std::exp(-strength * static_cast<float>(f));
             pic::AbsorbLayer& L = c->absorb.layer[c->absorb.n++];
             L.axis = axis;
-            L.minSide = (ex == 1 || ex == 3 || ex == 9) ? 1 : 0;
+            L.minSide = ex % 2 == 0 ? 1 : 0;
             L.depth = th - 1;
             L.table = dTable + l * kStride;
         }
@@ -1492,6 +1542,7 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
         c->pml.tabB = dB;
         c->hasPml = true;
         pmlInnerBox(*cfg, tabE, tabB, c->pmlIn0, c->pmlIn1);
+        c->pmlDamp = pmlDampArgs(*cfg, c->k);
     }
     if (devAlloc(c, &c->stepDev, 1)) return bail(PIC_E_HIP);
     c->incB = incidentUpdates(*cfg, c->k, true);

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pic_kernels.hpp"
+
 namespace pic {
 
 // Supercell shape (param/memory.param:51 SuperCellSize = 8,8,4) and frame size.
@@ -276,6 +278,141 @@ __device__ __forceinline__ Vec3 velocityOf(Vec3 mom, float mass, float rc2) {
     float const fMom2 = l2norm2(mom);
     float const t = 1.0f / sqrtf(m0_2 + fMom2 * rc2);
     return Vec3{t * mom.x, t * mom.y, t * mom.z};
+}
+
+// ---------------------------------------------------------------------------
+// PML weight damping (boundary/Absorbing.hpp:95-200 dampWeightsInPml and
+// DampWeightsInPml, fields/absorber/pml/Pml.kernel:116-136 getRelativeDepth,
+// :196-216 getNormalizedSigma, :326-386 getNormalizedSigmaIntegral,
+// traits/attribute/DampedWeighting.hpp:67-90 dampWeighting). The FP32 pow
+// and exp the reference calls are taken as FP64 results rounded to FP32,
+// from the FP64 routines below (the platform's FP64 pow and exp would spill
+// registers in the push kernel): IEEE add, multiply and divide in a fixed
+// order plus exact scalings by powers of two, so every platform gets the
+// same bits.
+// ---------------------------------------------------------------------------
+
+// e^x for x <= 0: x = k ln2 + r, |r| <= ln2 / 2, with ln2 in two parts (Cody
+// and Waite's reduction: k ln2_hi is exact for |k| < 2^20), e^r by its
+// Taylor polynomial of degree 13 (remainder below 1e-17), scaled by 2^k
+__device__ __forceinline__ double expNegF64(double x) {
+    if (x < -745.0) return 0.0;
+    double const k = rint(x * 1.4426950408889634);
+    double const r = (x - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+    double p = 1.0 / 6227020800.0;  // 1/13!
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp(p, static_cast<int>(k));
+}
+
+// ln(d) for 0 < d <= 1: d = m 2^e with m in [sqrt(1/2), sqrt(2)), ln m =
+// 2 atanh(s), s = (m - 1) / (m + 1), |s| < 0.172, by its series to s^21
+__device__ __forceinline__ double logUnitF64(double d) {
+    int e;
+    double m = frexp(d, &e);  // [0.5, 1)
+    if (m < 0.70710678118654752440) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double const s = (m - 1.0) / (m + 1.0);
+    double const s2 = s * s;
+    double q = 1.0 / 21.0;
+    q = q * s2 + 1.0 / 19.0;
+    q = q * s2 + 1.0 / 17.0;
+    q = q * s2 + 1.0 / 15.0;
+    q = q * s2 + 1.0 / 13.0;
+    q = q * s2 + 1.0 / 11.0;
+    q = q * s2 + 1.0 / 9.0;
+    q = q * s2 + 1.0 / 7.0;
+    q = q * s2 + 1.0 / 5.0;
+    q = q * s2 + 1.0 / 3.0;
+    q = q * s2 + 1.0;
+    double const lnm = 2.0 * s * q;
+    double const de = static_cast<double>(e);
+    return (de * 6.93147180369123816490e-01 + lnm) + de * 1.90821492927058770002e-10;
+}
+
+constexpr float kMinWeighting = 10.0f;  // particle.param MIN_WEIGHTING
+
+// normalised sigma at a position along axis d (cells, no guard)
+__device__ __forceinline__ float pmlSigmaAt(const PmlDampArgs& p, int d, float pos) {
+    // the reference adds the guard cells and subtracts them again
+    float const withGuard = pos + static_cast<float>(p.guard[d]);
+    float const zero = withGuard - static_cast<float>(p.guard[d]);
+    float depth = 0.0f;
+    if (zero < p.negB[d]) {
+        depth = (p.negB[d] - zero) / p.negB[d];
+    } else {
+        float const rightStart = p.nCells[d] - p.posB[d];
+        if (zero > rightStart) depth = (zero - rightStart) / p.posB[d];
+    }
+    if (depth == 0.0f) return 0.0f;
+    // depth^grading in FP64: repeated products for an integer order, else
+    // e^(grading ln depth) (depth in (0, 1]; a particle reconstructed beyond
+    // the border may give more, where both still hold)
+    double powered;
+    if (p.gradingInt >= 0) {
+        powered = 1.0;
+        for (int i = 0; i < p.gradingInt; ++i) powered = powered * static_cast<double>(depth);
+    } else {
+        powered = expNegF64(static_cast<double>(p.grading) * logUnitF64(static_cast<double>(depth)));
+    }
+    return p.sigmaMax[d] * static_cast<float>(powered);
+}
+
+// integral of the normalised sigma over the straight path start -> finish
+// taking dt (cells along axis d)
+__device__ __forceinline__ float pmlSigmaIntegral(const PmlDampArgs& p, int d, float start, float finish, float dt) {
+    float const vel = (finish - start) / dt;
+    if (static_cast<double>(fabsf(vel)) < p.slow[d]) {
+        float const s0 = pmlSigmaAt(p, d, start), s1 = pmlSigmaAt(p, d, finish);
+        return dt * 0.5f * (s0 + s1);
+    }
+    float as = start, af = finish;
+    float const negEnd = p.negB[d];
+    if (finish < negEnd && start >= negEnd) as = negEnd;
+    if (start < negEnd && finish >= negEnd) af = negEnd;
+    float const posStart = p.nCells[d] - p.posB[d];
+    if (finish > posStart && start <= posStart) as = posStart;
+    if (start > posStart && finish <= posStart) af = posStart;
+    float const adjT = (af - as) / vel;
+    float depthCoeff = 0.0f;
+    if (as <= negEnd) depthCoeff = as - negEnd;
+    if (as >= posStart) depthCoeff = as - posStart;
+    float const s0 = pmlSigmaAt(p, d, as), s1 = pmlSigmaAt(p, d, af);
+    return (s1 * adjT + (s1 - s0) * depthCoeff / vel) / (p.grading + 1.0f);
+}
+
+// one exchange's damping along axis d of a particle at in-cell position pos
+// of global cell `cell`: the path over the last step from its velocity,
+// weighting and momentum scaled by exp(-integral), the weighting kept at
+// MIN_WEIGHTING or above
+__device__ __forceinline__ void pmlDamp(const PmlDampArgs& p, int d, int cell, float pos, float massPerWeight,
+                                        float rc2, float dt, float cellSize, Vec3& mom, float& w) {
+    Vec3 const vel = velocityOf(mom, massPerWeight * w, rc2);
+    float const v = d == 0 ? vel.x : (d == 1 ? vel.y : vel.z);
+    float const current = static_cast<float>(cell) + pos;
+    float const previous = current - v * dt / cellSize;
+    float const integral = pmlSigmaIntegral(p, d, previous, current, dt);
+    float const multiplier = static_cast<float>(expNegF64(static_cast<double>(-integral)));
+    float const attempted = w * multiplier;
+    float const damped = fmaxf(attempted, kMinWeighting);
+    float const ratio = damped / w;
+    mom.x = mom.x * ratio;
+    mom.y = mom.y * ratio;
+    mom.z = mom.z * ratio;
+    w = damped;
 }
 
 struct PushConsts {

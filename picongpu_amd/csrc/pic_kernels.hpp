@@ -69,6 +69,22 @@ __host__ __device__ __forceinline__ bool outsideOpen(const GridDev& g, int x, in
            ((g.open & 4) && (z < 0 || z >= g.nz));
 }
 
+// Weight damping of particles in the PML layers by an absorbing particle
+// boundary (boundary/Absorbing.hpp dampWeightsInPml, Lehe et al. 2022):
+// per open axis with layers, the damping areas (the supercells the layers
+// reach) and the layer parameters of getNormalizedSigma (Pml.kernel:196-216)
+struct PmlDampArgs {
+    int axes;              // bit d: axis d is open and has a PML layer
+    int lo[3], hi[3];      // areas: cells below lo[d] (min side), at or above hi[d] (max side)
+    float negB[3], posB[3];  // layer thickness at the min and max side
+    float nCells[3];       // cells along each axis
+    int guard[3];          // the reference's guard cells (one supercell)
+    float sigmaMax[3];     // NORMALIZED_SIGMA_MAX
+    float grading;         // SIGMA_KAPPA_GRADING_ORDER
+    int gradingInt;        // the order when it is an integer in [0, 32], else -1
+    double slow[3];        // 1e-5 * cell size: the integral's bound for a slow particle
+};
+
 struct PushArgs {
     GridDev g;
     const float* E[3];
@@ -88,6 +104,7 @@ struct PushArgs {
     // the flush would add to J (zero where it adds none), for
     // launchGatherFlush instead of adding it to J
     float* tileOut;
+    PmlDampArgs damp;  // axes == 0: no PML damping
 };
 
 struct FieldArgs {

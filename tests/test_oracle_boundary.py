@@ -7,7 +7,9 @@ current deposited there is dropped, particles crossing it are removed.
 Exponential absorber (absorber/exponential/Exponential.hpp:52-95,
 Exponential.kernel:63-112): at distance d from an open border, fields are
 multiplied by exp(-strength * (T - 1 - d)) for d < T - 1, thickness T and
-strength read at [axis][exchange % 2] (the negative side uses entry 1)."""
+strength read at [axis][exchange % 2]: exchange 1 is RIGHT, the +x side, and
+2 is LEFT, -x (pmacc/type/Exchange.hpp:48-49), so the negative side uses
+entry 0 and the positive side entry 1, as fieldAbsorber.param orders them."""
 import math
 
 import numpy as np
@@ -38,15 +40,15 @@ def test_absorber_factors_closed_form():
     ora.update_after_current()  # absorber on E, B half step, absorber on B
     got = ora.get_field(abi.FIELD_E).reshape(3, 4, 8, nx)
     want = np.ones(nx, np.float32)
-    # LEFT is exchange 1: entry [x][1] = 10 cells, strength 0.02
-    for x in range(0, 9):
-        want[x] = np.exp(np.float32(-0.02) * np.float32(10 - 1 - x))
-    # RIGHT is exchange 2: entry [x][0] = 6 cells, strength 0.05
-    for d in range(0, 5):
-        want[nx - 1 - d] = np.exp(np.float32(-0.05) * np.float32(6 - 1 - d))
+    # LEFT (-x) is exchange 2: entry [x][0] = 6 cells, strength 0.05
+    for x in range(0, 5):
+        want[x] = np.exp(np.float32(-0.05) * np.float32(6 - 1 - x))
+    # RIGHT (+x) is exchange 1: entry [x][1] = 10 cells, strength 0.02
+    for d in range(0, 9):
+        want[nx - 1 - d] = np.exp(np.float32(-0.02) * np.float32(10 - 1 - d))
     for c in range(3):
         np.testing.assert_allclose(got[c], np.broadcast_to(want, (4, 8, nx)), rtol=2e-7, atol=0)
-    assert np.all(got[:, :, :, 9:nx - 5] == 1.0)
+    assert np.all(got[:, :, :, 5:nx - 9] == 1.0)
     ora.close()
 
 
