@@ -196,6 +196,17 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
     dk.inv_vdt = a.inv_vdt;
     dk.jscale = a.jscale;
 
+    // whether a particle of this supercell can end in a damping area (a move
+    // is less than a cell per axis): the others skip the exchange loop
+    bool dampSC = false;
+    if (DAMP && a.damp.axes) {
+        int const org[3] = {ox, oy, oz}, ext[3] = {SCX, SCY, SCZ};
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if ((a.damp.axes >> d) & 1)
+                dampSC = dampSC || org[d] - 1 < a.damp.lo[d] || org[d] + ext[d] >= a.damp.hi[d];
+    }
+
     SlowItem qA{}, qB{};
     int queued = 0;  // wave-uniform
     for (int s = 0; s < a.nsp; ++s) {
@@ -244,23 +255,26 @@ __device__ __forceinline__ void pushDepositSC(const PushArgs& a, int sc, unsigne
                         if (reflectParticle(nGrid, S.reflect, cellNow, p, mom, vel, dir, dp))
                             cellAfterMove(lidx, dir, newIdx, newMask);
                     }
-                    if (DAMP && a.damp.axes) {
+                    if (DAMP && dampSC) {
                         // the absorbing boundary's PML damping, exchange by exchange
                         // (+x, -x, +y, -y, +z, -z); the deposit stage then takes the
-                        // velocity and charge of the damped particle
-                        int const cellNow[3] = {ox + lx + dir[0], oy + ly + dir[1], oz + lz + dir[2]};
+                        // velocity and charge of the damped particle. One loop, not
+                        // unrolled, with the axis' values selected (no indexed local
+                        // arrays, which the compiler would move to LDS)
+                        int const c0 = ox + lx + dir[0], c1 = oy + ly + dir[1], c2 = oz + lz + dir[2];
                         bool any = false;
-                        if (!(a.g.open && outsideOpen(a.g, cellNow[0], cellNow[1], cellNow[2]))) {
-                            for (int d = 0; d < 3; ++d) {
+                        if (!(a.g.open && outsideOpen(a.g, c0, c1, c2))) {
+#pragma unroll 1
+                            for (int e = 0; e < 6; ++e) {
+                                int const d = e >> 1;
                                 if (!((a.damp.axes >> d) & 1) || ((S.reflect >> d) & 1)) continue;
-                                if (cellNow[d] >= a.damp.hi[d]) {
-                                    pmlDamp(a.damp, d, cellNow[d], p[d], S.mass, pk.rc2, pk.dt, pk.cell[d], mom, w);
-                                    any = true;
-                                }
-                                if (cellNow[d] < a.damp.lo[d]) {
-                                    pmlDamp(a.damp, d, cellNow[d], p[d], S.mass, pk.rc2, pk.dt, pk.cell[d], mom, w);
-                                    any = true;
-                                }
+                                int const cn = d == 0 ? c0 : (d == 1 ? c1 : c2);
+                                bool const inArea = (e & 1) == 0 ? cn >= a.damp.hi[d] : cn < a.damp.lo[d];
+                                if (!inArea) continue;
+                                float const pd = d == 0 ? p[0] : (d == 1 ? p[1] : p[2]);
+                                float const cs = d == 0 ? pk.cell[0] : (d == 1 ? pk.cell[1] : pk.cell[2]);
+                                pmlDamp(a.damp, d, cn, pd, S.mass, pk.rc2, pk.dt, cs, mom, w);
+                                any = true;
                             }
                         }
                         if (any) {
