@@ -64,8 +64,8 @@ MEASURED_INSTS = {
                                                     "grbm_gui_active": 31.881e6},
     ("laser_plasma", (192, 1536, 192), "exponential"): {"valu": 515.0e6, "lds": 54.82e6, "lds_bank_conflict": 207.3e6,
                                                         "grbm_gui_active": 20.93e6},
-    ("laser_plasma", (192, 1536, 192), "pml"): {"valu": 554.5e6, "lds": 55.27e6, "lds_bank_conflict": 207.2e6,
-                                                "grbm_gui_active": 24.37e6},
+    ("laser_plasma", (192, 1536, 192), "pml"): {"valu": 538.17e6, "lds": 54.796e6, "lds_bank_conflict": 207.25e6,
+                                                "grbm_gui_active": 23.572e6},
 }
 # VALU issue peak: a wave64 VALU instruction occupies one of the 1,024 SIMDs
 # (256 CUs x 4) for 4 cycles at the 2.4 GHz peak engine clock
