@@ -1,9 +1,13 @@
 // Initial particle state on the device, in the manner of
 // KernelFillGridWithParticles (particles/ParticlesInit.kernel:66-302): each
-// supercell gets ppc particles per cell, placed round-robin over its 256
-// cells (slot = k*256 + localCellIdx), at uniform random in-cell positions
-// (startPosition/RandomImpl.hpp:46-56) with weighting density*V/ppc
-// (WeightMacroParticles.hpp:30-42) and momentum mass*(sign*u_drift + u_th*N(0,1))
+// cell gets the macro-particle count of WeightMacroParticles
+// (startPosition/detail/WeightMacroParticles.hpp:30-42: floor(min(ppc,
+// N / MIN_WEIGHTING)) for N real particles in the cell, so that no weighting
+// falls below MIN_WEIGHTING = 10; full density gives ppc), placed
+// round-robin over the supercell's cells (round k: the cells with more than k
+// particles, in cell order), at uniform random in-cell positions
+// (startPosition/RandomImpl.hpp:46-56) with weighting N / count (density*V/ppc
+// where the count is ppc) and momentum mass*(sign*u_drift + u_th*N(0,1))
 // (manipulators/unary/Drift.hpp:53-76, Temperature.hpp:60-84); the drift sign
 // is +1, or set by the cell's y band (shear flow), or alternates with the
 // particle's index in its cell (two counter-streaming beams).
@@ -20,21 +24,42 @@ __device__ __forceinline__ double densityFactor(const InitArgs& a, int gx, int g
     return f <= 0.0 ? 0.0 : (f >= 1.0 ? 1.0 : f);
 }
 
+constexpr float kInitMinWeighting = 10.0f;  // particle.param MIN_WEIGHTING
+
+// WeightMacroParticles for a cell of density factor f: the macro-particle count
+// and (when below ppc) their weighting N / count, N = density*V*f in FP32
+__device__ __forceinline__ int macroCount(const InitArgs& a, double f, float* wLow) {
+    if (!(f > 0.0)) return 0;
+    float const real = (float)(a.weighting * (double)a.ppc * f);
+    float const maxPar = real / kInitMinWeighting;
+    float const m = fminf((float)a.ppc, maxPar);
+    int const n = (int)floorf(m);
+    *wLow = n > 0 ? real / (float)n : 0.0f;
+    return n;
+}
+
 __global__ __launch_bounds__(256) void kInit(InitArgs a) {
     __shared__ uint32_t sWave[4];
     __shared__ int16_t sCellOfRank[kSCV];
+    __shared__ uint8_t sCount[kSCV];        // macro particles per cell (<= ppc <= 255)
+    __shared__ uint32_t sRound[kMaxInitPpc + 1];  // first slot of each round
     int const sc = blockIdx.x;
     int const sx = sc % a.g.nscx;
     int const sy = (sc / a.g.nscx) % a.g.nscy;
     int const sz = sc / (a.g.nscx * a.g.nscy);
     size_t const base = slotBase(a.sp, sc);
     int const cap = (int)slotCap(a.sp, sc);
-    // cells with a positive density, ranked in cell order (256 threads = 256 cells)
+    // cells with macro particles, ranked in cell order (256 threads = 256 cells)
     int nActive;
     {
         int const lc = threadIdx.x;
-        bool const active = densityFactor(a, sx * kSCX + lc % kSCX, sy * kSCY + (lc / kSCX) % kSCY,
-                                          sz * kSCZ + lc / (kSCX * kSCY)) > 0.0;
+        float wLow;
+        int const cnt = macroCount(a, densityFactor(a, sx * kSCX + lc % kSCX, sy * kSCY + (lc / kSCX) % kSCY,
+                                                    sz * kSCZ + lc / (kSCX * kSCY)),
+                                   &wLow);
+        sCount[lc] = (uint8_t)cnt;
+        bool const active = cnt > 0;
+        for (int k = threadIdx.x; k <= a.ppc; k += blockDim.x) sRound[k] = 0u;
         int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         unsigned long long const b = __ballot(active);
         if (lane == 0) sWave[wave] = (uint32_t)__popcll(b);
@@ -46,18 +71,39 @@ __global__ __launch_bounds__(256) void kInit(InitArgs a) {
             nActive += (int)sWave[w];
         }
         if (active) sCellOfRank[before + __popcll(b & ((1ull << lane) - 1ull))] = (int16_t)lc;
+        // round k holds the cells with more than k particles
+        for (int k = 0; k < cnt; ++k) atomicAdd(&sRound[k + 1], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 1; k <= a.ppc; ++k) sRound[k] += sRound[k - 1];
         __syncthreads();
     }
-    // slot = k * nActive + rank (round-robin over the populated cells)
-    int const n = a.ppc * nActive;
+    // slot sRound[k] + j: the j-th cell, in rank order, of round k (with every
+    // populated cell at ppc particles: slot = k * nActive + rank)
+    int const n = (int)sRound[a.ppc];
     for (int i = threadIdx.x; i < cap; i += blockDim.x) {
         size_t const g = base + i;
         if (i >= n) {
             a.sp.mask[g] = 0;
             continue;
         }
-        int const k = i / nActive;
-        int const lc = sCellOfRank[i % nActive];
+        int k = 0;
+        while ((int)sRound[k + 1] <= i) ++k;
+        int j = i - (int)sRound[k];
+        int lc;
+        if ((int)(sRound[k + 1] - sRound[k]) == nActive) {
+            lc = sCellOfRank[j];
+        } else {
+            int r = 0;
+            for (;; ++r) {
+                int const c = sCellOfRank[r];
+                if (sCount[c] > k) {
+                    if (j == 0) break;
+                    --j;
+                }
+            }
+            lc = sCellOfRank[r];
+        }
         int const gx = sx * kSCX + lc % kSCX;
         int const gy = sy * kSCY + (lc / kSCX) % kSCY;
         int const gz = sz * kSCZ + lc / (kSCX * kSCY);
@@ -84,7 +130,10 @@ __global__ __launch_bounds__(256) void kInit(InitArgs a) {
         double sign = 1.0;
         if (a.drift_band == 1) sign = (gy >= a.g.ny / 4 && gy < 3 * a.g.ny / 4) ? 1.0 : -1.0;
         if (a.drift_band == 2) sign = (k & 1) ? -1.0 : 1.0;  // counter-streaming beams
-        float const w = (float)(a.weighting * densityFactor(a, gx, gy, gz));
+        float wLow;
+        double const f = densityFactor(a, gx, gy, gz);
+        int const cnt = macroCount(a, f, &wLow);
+        float const w = cnt == a.ppc ? (float)(a.weighting * f) : wLow;
         double const m = (double)(a.frameMass * w);
         a.sp.mom[0][g] = (float)(m * (sign * a.u_drift[0] + a.u_th[0] * n0));
         a.sp.mom[1][g] = (float)(m * (sign * a.u_drift[1] + a.u_th[1] * n1));

@@ -1931,6 +1931,7 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
     int r = checkSpecies(c, s);
     if (r != PIC_OK) return r;
     if (!desc || desc->ppc <= 0) return fail(PIC_E_INVALID, "invalid init description");
+    if (desc->ppc > pic::kMaxInitPpc) return fail(PIC_E_INVALID, "at most 255 particles per cell");
     SpeciesMem& m = c->sp[s];
     {  // every supercell must hold ppc particles in each of its cells
         std::vector<uint32_t> need(c->nsc, static_cast<uint32_t>(desc->ppc) * pic::kSCV);
@@ -1970,7 +1971,16 @@ int pic_init_species(pic_ctx* c, int32_t s, const pic_init_desc* desc) {
     m.shiftReady = 0;
     PIC_HIP(pic::launchInit(a, c->stream));
     PIC_HIP(hipStreamSynchronize(c->stream));
+    // the largest weighting: density*V/ppc, or, where WeightMacroParticles
+    // takes fewer particles than ppc, N / count (below 2 MIN_WEIGHTING on a
+    // ramp's thin cells)
     m.wmax = std::fabs(static_cast<double>(static_cast<float>(a.weighting)));
+    {
+        double const full = a.weighting * static_cast<double>(a.ppc);
+        double const nFull = std::floor(std::min(static_cast<double>(a.ppc), full / 10.0));
+        if (nFull >= 1.0 && nFull < a.ppc) m.wmax = std::max(m.wmax, full / nFull);
+        if (a.profile == 1) m.wmax = std::max(m.wmax, 20.0);
+    }
     updateJexp(c);
     return PIC_OK;
 }

@@ -237,6 +237,7 @@ struct InitArgs {
     int32_t quiet[3];    // particles per dimension of the quiet start, 0: random positions
 };
 hipError_t launchInit(const InitArgs& a, hipStream_t s);
+constexpr int kMaxInitPpc = 255;  // macro particles per cell of an initialisation
 struct LaserArgs {
     GridDev g;
     float* E[3];

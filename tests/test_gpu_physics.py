@@ -280,27 +280,48 @@ def test_full_size_kelvin_helmholtz_properties():
     sim.close()
 
 
-def test_density_ramp_init():
+def macro_rule(w0, ppc, f):
+    """WeightMacroParticles (startPosition/detail/WeightMacroParticles.hpp:30-42)
+    per cell of density factor f: floor(min(ppc, N / 10)) macro particles of
+    weighting N / count, N = w0 * ppc * f real particles in FP32; where the
+    count is ppc the weighting is w0 * f."""
+    real = (np.float64(w0) * ppc * f).astype(np.float32)
+    count = np.floor(np.minimum(np.float32(ppc), real / np.float32(10.0))).astype(np.int64)
+    count = np.where(f > 0.0, count, 0)
+    w_low = np.where(count > 0, real / np.maximum(count, 1).astype(np.float32), 0).astype(np.float32)
+    w = np.where(count == ppc, (np.float64(w0) * f).astype(np.float32), w_low)
+    return count, w
+
+
+@pytest.mark.parametrize("dense", [True, False])
+def test_density_ramp_init(dense):
     """Linear ramp along y: cells whose origin lies before the ramp start get
-    no particles, the others ppc particles of weighting w0*clamp(f,0,1)."""
+    no particles, the others WeightMacroParticles' count, never a weighting
+    below MIN_WEIGHTING; the dilute case (40 real particles per macro
+    particle at full density) gives the ramp's first cells 0, 1 and 2
+    particles instead of 3."""
     st = setups.two_stream(grid=(16, 32, 8), ppc=3)
     dy = st.cell_si[1]
+    vol = st.cell_si[0] * st.cell_si[1] * st.cell_si[2]
+    if not dense:
+        st.species[0].density_si = 40.0 * 3 / vol
     st.species[0].ramp = (1, 8 * dy, 16 * dy)
-    print("case density ramp", flush=True)
+    print(f"case density ramp dense={dense}", flush=True)
     sim = abi.Simulation(st.config())
     setups.init_device(sim, st)
     cells, pos, mom, w = sim.read_particles(0)
-    vol = st.cell_si[0] * st.cell_si[1] * st.cell_si[2]
     w0 = st.species[0].density_si * vol / st.species[0].ppc
     start, length = (8 * dy) / dy, (16 * dy) / dy  # the ramp in cells, as the library converts it
+    fy = np.clip((np.arange(32) - start) / length, 0.0, 1.0)
+    count_y, w_y = macro_rule(w0, 3, fy)
     gy = cells[:, 1].astype(np.int64)
-    f = np.clip((gy - start) / length, 0.0, 1.0)
-    assert np.all(f > 0.0)
-    np.testing.assert_array_equal(H.bits(w), H.bits((w0 * f).astype(np.float32)))
+    np.testing.assert_array_equal(H.bits(w), H.bits(w_y[gy]))
+    assert np.all(w >= 10.0)
     flat = cells[:, 0] + 16 * (cells[:, 1] + 32 * cells[:, 2])
     per_cell = np.bincount(flat, minlength=16 * 32 * 8).reshape(8, 32, 16)
-    expect = np.where((np.arange(32) - start) / length > 0.0, 3, 0)
-    assert np.all(per_cell == expect[None, :, None])
+    assert np.all(per_cell == count_y[None, :, None])
+    if not dense:
+        assert list(count_y[9:13]) == [0, 1, 2, 3]
     sim.close()
 
 
