@@ -937,12 +937,22 @@ __global__ __launch_bounds__(256) void kLeheBHalfZ(FieldArgs a, int zc) {
     leheBHalfTile<OPEN, false>(a, PmlArgs{}, 0, blockIdx.x, zc, L);
 }
 
-// with the PML: the layer cells (outside the inner box) as kBHalfPml
+// with the PML: the layer cells (outside the inner box) as kBHalfPml; the
+// tiles listed after the plain ones when there is a list
 template <bool OPEN>
 __global__ __launch_bounds__(256) void kLeheBHalfPmlZ(FieldArgs a, PmlArgs p, int updatePsi, int zc) {
     if (halted(a.g)) return;
     __shared__ LehePlanes L;
-    leheBHalfTile<OPEN, true>(a, p, updatePsi, blockIdx.x, zc, L);
+    int const b = p.tiles ? p.tiles[p.nPlain + blockIdx.x] : (int)blockIdx.x;
+    leheBHalfTile<OPEN, true>(a, p, updatePsi, b, zc, L);
+}
+
+// the plain B half step on the listed tiles (no cell of them in a layer)
+template <bool OPEN>
+__global__ __launch_bounds__(256) void kLeheBHalfListZ(FieldArgs a, const int* __restrict__ tiles, int zc) {
+    if (halted(a.g)) return;
+    __shared__ LehePlanes L;
+    leheBHalfTile<OPEN, false>(a, PmlArgs{}, 0, tiles[blockIdx.x], zc, L);
 }
 
 template <bool OPEN>
@@ -956,7 +966,15 @@ template <bool OPEN>
 __global__ __launch_bounds__(256) void kYeeBHalfPmlZ(FieldArgs a, PmlArgs p, int updatePsi, int zc) {
     if (halted(a.g)) return;
     __shared__ YeeAfterLds L;
-    yeeAfterTile<OPEN, false, true>(a, nullptr, blockIdx.x, zc, L, &p, updatePsi);
+    int const b = p.tiles ? p.tiles[p.nPlain + blockIdx.x] : (int)blockIdx.x;
+    yeeAfterTile<OPEN, false, true>(a, nullptr, b, zc, L, &p, updatePsi);
+}
+
+template <bool OPEN>
+__global__ __launch_bounds__(256) void kYeeBHalfListZ(FieldArgs a, const int* __restrict__ tiles, int zc) {
+    if (halted(a.g)) return;
+    __shared__ YeeAfterLds L;
+    yeeAfterTile<OPEN, false>(a, nullptr, tiles[blockIdx.x], zc, L);
 }
 
 __global__ __launch_bounds__(256) void kYeeBeforeAddBatch(const FieldBatchEntry* __restrict__ table,
@@ -1179,15 +1197,26 @@ hipError_t launchBHalfPml(int solver, const FieldArgs& a, const PmlArgs& p, bool
     size_t const n = (size_t)a.g.nx * a.g.ny * a.g.nz;
     int const zc = yeeChunk(a.g);  // the z-marching form when the grid fills the chip
     if (zc > 0) {
-        unsigned const zt = (unsigned)zTileCount(a.g, zc);
+        unsigned const zt = p.tiles ? (unsigned)p.nLayer : (unsigned)zTileCount(a.g, zc);
         int const u = updatePsi ? 1 : 0;
-        if (solver == PIC_SOLVER_LEHE) {
-            if (a.g.open)
-                hipLaunchKernelGGL(kLeheBHalfPmlZ<true>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
+        bool const lehe = solver == PIC_SOLVER_LEHE;
+        if (p.tiles && p.nPlain > 0) {  // the tiles without a layer cell: the plain kernel
+            dim3 const grid((unsigned)p.nPlain);
+            if (lehe && a.g.open)
+                hipLaunchKernelGGL(kLeheBHalfListZ<true>, grid, dim3(256), 0, s, a, p.tiles, zc);
+            else if (lehe)
+                hipLaunchKernelGGL(kLeheBHalfListZ<false>, grid, dim3(256), 0, s, a, p.tiles, zc);
+            else if (a.g.open)
+                hipLaunchKernelGGL(kYeeBHalfListZ<true>, grid, dim3(256), 0, s, a, p.tiles, zc);
             else
+                hipLaunchKernelGGL(kYeeBHalfListZ<false>, grid, dim3(256), 0, s, a, p.tiles, zc);
+        }
+        if (zt > 0) {
+            if (lehe && a.g.open)
+                hipLaunchKernelGGL(kLeheBHalfPmlZ<true>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
+            else if (lehe)
                 hipLaunchKernelGGL(kLeheBHalfPmlZ<false>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
-        } else {
-            if (a.g.open)
+            else if (a.g.open)
                 hipLaunchKernelGGL(kYeeBHalfPmlZ<true>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
             else
                 hipLaunchKernelGGL(kYeeBHalfPmlZ<false>, dim3(zt), dim3(256), 0, s, a, p, u, zc);
@@ -1256,6 +1285,34 @@ hipError_t launchAbsorb(const GridDev& g, float* const F[3], const AbsorbPlan& p
                            L.minSide, L.depth, L.table);
     }
     return hipGetLastError();
+}
+
+int pmlTileCount(const GridDev& g) {
+    int const zc = yeeChunk(g);
+    return zc > 0 ? zTileCount(g, zc) : 0;
+}
+
+void pmlTileSplit(const GridDev& g, const int in0[3], const int in1[3], int* out, int* nPlain) {
+    int const zc = yeeChunk(g);
+    int const n = zTileCount(g, zc);
+    int const ntx = (g.nx + kZX - 1) / kZX, nty = (g.ny + kZY - 1) / kZY;
+    int plain = 0;
+    for (int b = 0; b < n; ++b) {
+        int const x0 = (b % ntx) * kZX, y0 = ((b / ntx) % nty) * kZY, z0 = (b / (ntx * nty)) * zc;
+        int const x1 = min(x0 + kZX, g.nx), y1 = min(y0 + kZY, g.ny), z1 = min(z0 + zc, g.nz);
+        bool const inside = x0 >= in0[0] && x1 <= in1[0] && y0 >= in0[1] && y1 <= in1[1] && z0 >= in0[2] &&
+                            z1 <= in1[2];
+        if (inside) out[plain++] = b;
+    }
+    *nPlain = plain;
+    int k = plain;
+    for (int b = 0; b < n; ++b) {
+        int const x0 = (b % ntx) * kZX, y0 = ((b / ntx) % nty) * kZY, z0 = (b / (ntx * nty)) * zc;
+        int const x1 = min(x0 + kZX, g.nx), y1 = min(y0 + kZY, g.ny), z1 = min(z0 + zc, g.nz);
+        bool const inside = x0 >= in0[0] && x1 <= in1[0] && y0 >= in0[1] && y1 <= in1[1] && z0 >= in0[2] &&
+                            z1 <= in1[2];
+        if (!inside) out[k++] = b;
+    }
 }
 
 int yeeChunk(const GridDev& g) {

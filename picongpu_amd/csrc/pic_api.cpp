@@ -1543,6 +1543,19 @@ int pic_create(const pic_config* cfg, pic_ctx** out) {
         c->hasPml = true;
         pmlInnerBox(*cfg, tabE, tabB, c->pmlIn0, c->pmlIn1);
         c->pmlDamp = pmlDampArgs(*cfg, c->k);
+        int const tiles = pic::pmlTileCount(c->g);
+        if (tiles > 0) {
+            std::vector<int> list(static_cast<size_t>(tiles));
+            int nPlain = 0;
+            pic::pmlTileSplit(c->g, c->pmlIn0, c->pmlIn1, list.data(), &nPlain);
+            int* dList = nullptr;
+            if (devAlloc(c, &dList, list.size())) return bail(PIC_E_HIP);
+            e = hipMemcpyAsync(dList, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+            if (e != hipSuccess) return bail(fail(PIC_E_HIP, hipGetErrorString(e)));
+            c->pml.tiles = dList;
+            c->pml.nPlain = nPlain;
+            c->pml.nLayer = tiles - nPlain;
+        }
     }
     if (devAlloc(c, &c->stepDev, 1)) return bail(PIC_E_HIP);
     c->incB = incidentUpdates(*cfg, c->k, true);

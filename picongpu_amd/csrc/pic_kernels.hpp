@@ -173,7 +173,17 @@ struct PmlArgs {
     float* psiB[6];
     const float* tabE;
     const float* tabB;
+    // the z-marching B half step's tiles: the nPlain tiles whose cells all
+    // lie outside the layers, then the nLayer others (null: every tile takes
+    // the PML kernel)
+    const int* tiles;
+    int nPlain, nLayer;
 };
+// tiles of the z-marching B half step on this grid (0: the per-cell form)
+int pmlTileCount(const GridDev& g);
+// their split: out[0, *nPlain) the tiles inside [in0, in1) on every axis, the
+// rest after them
+void pmlTileSplit(const GridDev& g, const int in0[3], const int in1[3], int* out, int* nPlain);
 // UpdateBHalfFunctor with PML: cells inside the layer update psi of B when
 // updatePsi and add the stretched curl; the others B -= curl E * 0.5 * dt
 // (the z-marching form when the grid fills the chip, as launchBHalf)
