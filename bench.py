@@ -62,10 +62,10 @@ MEASURED_TRAFFIC = {
 MEASURED_INSTS = {
     ("kelvin_helmholtz", (256, 256, 256), "none"): {"valu": 826.64e6, "lds": 90.875e6, "lds_bank_conflict": 488.81e6,
                                                     "grbm_gui_active": 31.881e6},
-    ("laser_plasma", (192, 1536, 192), "exponential"): {"valu": 515.0e6, "lds": 54.82e6, "lds_bank_conflict": 207.3e6,
-                                                        "grbm_gui_active": 20.93e6},
-    ("laser_plasma", (192, 1536, 192), "pml"): {"valu": 538.17e6, "lds": 54.796e6, "lds_bank_conflict": 207.25e6,
-                                                "grbm_gui_active": 23.572e6},
+    ("laser_plasma", (192, 1536, 192), "exponential"): {"valu": 495.28e6, "lds": 52.231e6,
+                                                        "lds_bank_conflict": 197.88e6, "grbm_gui_active": 20.089e6},
+    ("laser_plasma", (192, 1536, 192), "pml"): {"valu": 517.90e6, "lds": 52.208e6, "lds_bank_conflict": 197.81e6,
+                                                "grbm_gui_active": 22.487e6},
 }
 # VALU issue peak: a wave64 VALU instruction occupies one of the 1,024 SIMDs
 # (256 CUs x 4) for 4 cycles at the 2.4 GHz peak engine clock
