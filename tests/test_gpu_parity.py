@@ -353,6 +353,16 @@ def reflecting(st, species_axes, boundary=(1, 1, 1)):
     return st
 
 
+def pml_reflecting(st, species_axes, boundary=(1, 1, 1)):
+    """Open borders with the PML (4 to 6 cells); species s reflects on the axes
+    species_axes[s] and is absorbing, so damped in the layers, on the others."""
+    pml_box(st, boundary=boundary)
+    for s, axes in species_axes.items():
+        st.species[s].particle_boundary = tuple(abi.PARTICLE_REFLECTING if d in axes else abi.PARTICLE_ABSORBING
+                                                for d in range(3))
+    return st
+
+
 REFLECT_CASES = [
     ("reflect-xyz-tsc-boris", lambda: reflecting(small_thermal(abi.SHAPE_TSC, abi.PUSHER_BORIS, u=0.3),
                                                  {0: (0, 1, 2)})),
@@ -361,6 +371,13 @@ REFLECT_CASES = [
     ("reflect-y-pqs-vay-lehe", lambda: reflecting(small_lehe(), {0: (1,)}, boundary=(0, 1, 0))),
     ("reflect-xz-cic-hc-emz", lambda: emz(reflecting(small_thermal(abi.SHAPE_CIC, abi.PUSHER_HIGUERA_CARY, u=0.3),
                                                      {0: (0, 2)}, boundary=(1, 0, 1)))),
+    # with the PML: the species reflecting on x is damped only in the y layers
+    ("pml-reflect-x-tsc-boris", lambda: pml_reflecting(small_thermal(abi.SHAPE_TSC, abi.PUSHER_BORIS, u=0.3),
+                                                       {0: (0,)}, boundary=(1, 1, 0))),
+    ("pml-reflect-none-cic-hc-emz", lambda: emz(pml_reflecting(small_thermal(abi.SHAPE_CIC, abi.PUSHER_HIGUERA_CARY,
+                                                                             u=0.3), {0: ()}, boundary=(1, 1, 1)))),
+    ("pml-reflect-y-kh-electrons", lambda: pml_reflecting(small_two_species(abi.SHAPE_TSC), {0: (1,)},
+                                                          boundary=(1, 1, 0))),
 ]
 
 
