@@ -115,10 +115,10 @@ def test_full_thermal_against_reference_cpu_order(c1):
         fp64["ref_" + nm] = floor[nm]
     print(f"GPU vs reference CPU order after {STEPS} steps: {err}", flush=True)
     print(f"against the FP64 twin (reference order): {fp64}", flush=True)
-    # north_star (i): cell indices, supercell membership and per-frame counts bit-exact
+    # cell indices, supercell membership and per-frame counts: bit-exact
     assert err["cell_flips"] == 0 and err["sc_flips"] == 0
     np.testing.assert_array_equal(c1["gpu_frames"]["counts"], f1["counts"])
-    # north_star (ii): 1e-5 relative (FP32)
+    # positions, momenta and fields: within 1e-5 relative (FP32)
     assert err["pos_cells"] < 1e-5 and err["mom"] < 1e-5
     assert err["E"] < 1e-5 and err["J"] < 1e-5
     # B, the smallest field, is round-off noise amplified by 100 steps: the
@@ -126,7 +126,7 @@ def test_full_thermal_against_reference_cpu_order(c1):
     # (exact fixed-point sums) must lie no farther from the reference order
     # than the reference order lies from FP64.
     assert err["B"] < floor["B"] and err["B"] < 2e-5
-    # north_star (iv): the FP32 results against the FP64 oracle
+    # the FP32 results against the FP64 oracle
     assert fp64["gpu_pos_cells"] < 1e-5 and fp64["gpu_mom"] < 1e-5
     for nm in ("E", "B", "J"):
         assert fp64["gpu_" + nm] < 5e-5
